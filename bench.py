@@ -1,0 +1,174 @@
+"""Benchmark of the participating-media hot path on one MI355X.
+
+Default workload (one step = one full frame):
+  NEE volume path tracer with delta tracking over a 256^3 FP32 grid of 4-octave
+  fBm value noise (lacunarity 2, gain 0.5) clamped to [0,1]*40 inside a sphere,
+  albedo 0.9, Henyey-Greenstein g=0.6, one directional light + constant
+  environment 0.1, 16^3 majorant grid, 512x512 pixels, 64 spp, max 16 bounces,
+  RGB FP32 output.
+
+Prints ONE JSON line with the metric (camera samples per second), the roofline
+of the dominant kernel (gather model: 32 B per trilinear lookup + 12 B per
+output pixel, lookups counted by the kernel's own per-thread counter) and a CPU
+baseline (the oracle on a bounded pixel sample of the same frame).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_scene(res: int, width: int, height: int, device):
+    import torch
+    import rendervous_amd as rv
+
+    sigma = rv.functions.fbm_value_noise(res, seed=7, octaves=4, lacunarity=2.0, gain=0.5, base_frequency=4.0,
+                                         scale=40.0, sphere_radius=0.5, device=device)
+    ds = rv.objects.DependencySet()
+    ds.add_parameters(sigma_tensor=sigma, scattering_albedo_tensor=torch.tensor([0.9, 0.9, 0.9]),
+                      environment_tensor=torch.tensor([0.1, 0.1, 0.1]), phase_g_tensor=torch.tensor([0.6]))
+    ds.requires(rv.objects.medium_majorant_coarse_grid, resolution=16)
+    ds.requires(rv.objects.medium_environment_sampler_sun, direction=(0.4, 0.8, 0.45), radiance=(3.0, 3.0, 3.0),
+                probability=0.5)
+    ds.requires(rv.objects.medium_phase_HG, mixture=False)
+    ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
+    ds.requires(rv.objects.medium_transmittance_DT)
+    ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DT, max_bounces=16)
+    poses = rv.functions.look_at_pose((0.0, 0.3, 2.5))[None].to(device)
+    camera = rv.PerspectiveCameraSensor(width, height, poses, jittered=True)
+    return sigma, ds.radiance, camera
+
+
+def cpu_baseline(sigma_cpu, radiance, width, height, spp, budget_s: float = 15.0):
+    """Oracle (C, OpenMP) on every k-th row of the same frame; returns samples/s."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import Scene, oracle_capture_fw, pose, L
+
+    m = radiance._lower_map()[1].medium
+    sc = Scene(sigma=sigma_cpu.numpy(), albedo_c=tuple(m.albedo_c), env_c=tuple(m.env_c), sun_dir=tuple(m.sun_dir),
+               sun_radiance=tuple(m.sun_radiance), sun_prob=m.sun_prob, phase_kind=m.phase_kind, phase_g=m.phase_g,
+               majorant="grid", majorant_res=(16, 16, 16), transmittance_kind=m.transmittance_kind,
+               max_bounces=m.max_bounces)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # bounded sample: pixels on every `step`-th row, grown until the budget is used
+    rows = 1
+    done_samples, elapsed = 0, 0.0
+    p = pose((0.0, 0.3, 2.5))[None]
+    while elapsed < budget_s and rows <= height:
+        ys = np.linspace(0, height - 1, rows).astype(np.int64)
+        sensors = np.stack(np.meshgrid(np.zeros(1, np.int64), ys, np.arange(width, dtype=np.int64), indexing="ij"),
+                           -1).reshape(-1, 3)
+        t0 = time.perf_counter()
+        oracle_capture_fw(sc, L.VR_MAP_NEE_DT, dict(poses=p, width=width, height=height, jittered=True,
+                                                     sensors=sensors), (11, 22, 33, 44), spp)
+        dt = time.perf_counter() - t0
+        done_samples, elapsed = sensors.shape[0] * spp, dt
+        if dt > budget_s / 4:
+            break
+        rows *= 2
+    return {"value": done_samples / elapsed, "unit": "camera samples/s", "cores": threads, "kind": "port",
+            "sample": f"{done_samples // spp} pixels ({rows} rows x {width}) of the same frame at {spp} spp, "
+                      f"{elapsed:.1f} s, C oracle with OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--width", type=int, default=512)
+    ap.add_argument("--height", type=int, default=512)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import rendervous_amd as rv
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log(f"building {args.res}^3 fBm scene on {torch.cuda.get_device_name(dev)}")
+    sigma, radiance, camera = build_scene(args.res, args.width, args.height, dev)
+    torch.cuda.synchronize()
+
+    samples = args.width * args.height * args.spp
+    pixels = args.width * args.height
+
+    # lookups per launch from the kernel's per-thread counter (untimed run)
+    dbg = torch.zeros(pixels, dtype=torch.int32, device=dev)
+    rv.seed(0)
+    with torch.no_grad():
+        camera.capture(radiance, fw_samples=args.spp, debug_out=dbg)
+    torch.cuda.synchronize()
+    lookups = float(dbg.to(torch.float64).sum().item())
+    log(f"lookups per frame: {lookups:.4e} ({lookups / samples:.1f} per sample)")
+
+    with torch.no_grad():
+        for _ in range(args.warmup):
+            img = camera.capture(radiance, fw_samples=args.spp)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream(dev)
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            starts[k].record(stream)
+            img = camera.capture(radiance, fw_samples=args.spp)
+            ends[k].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+    ms_per_step = wall * 1000.0 / args.steps
+    assert torch.isfinite(img).all()
+    log(f"frame mean {img.mean().item():.5f}; {kernel_ms:.2f} ms per frame (events), {ms_per_step:.2f} ms wall")
+
+    bytes_per_launch = 32.0 * lookups + 12.0 * pixels
+    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    result = {
+        "metric": "rays/s (camera samples/s, NEE delta-tracking volume path tracer)",
+        "value": samples / (ms_per_step * 1e-3),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm value-noise density)",
+        "config": {"workload": f"NEE+DT volume path tracing, {args.res}^3 fBm x40 in sphere, albedo 0.9, HG g=0.6, "
+                               f"sun + env 0.1, 16^3 majorant grid, {args.width}x{args.height}, {args.spp} spp, "
+                               f"max 16 bounces", "samples_per_step": samples, "lookups_per_step": lookups},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(sigma.cpu(), radiance, args.width, args.height, args.spp, args.cpu_budget)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,220 @@
+/*
+ * vr_api.h -- C ABI of the MI355X (gfx950) participating-media library
+ * (librendervous_amd.so) and of its CPU oracle (oracle/liboracle.so, whose
+ * functions carry the prefix "oracle_" instead of "vr_" and take host pointers).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * an int status (VR_OK == 0).  The caller owns every buffer; the library never
+ * allocates or frees caller memory.  All device work is ordered on the caller's
+ * HIP stream (`stream`, a hipStream_t; NULL = the default stream) and no entry
+ * point synchronises the device.
+ *
+ * Reference interfaces replaced (rendervous, files relative to its root):
+ *   - the vulky-dispatched generated GLSL kernels of DispatcherEngine
+ *       eval_capture_forward   _maps.py:749-821   -> vr_capture_fw
+ *       eval_capture_backward  _maps.py:824-882   -> vr_capture_bw
+ *       eval_map_forward       _maps.py:885-925   -> vr_map_fw, vr_grid_fw
+ *       eval_map_backward      _maps.py:928-976   -> vr_map_bw, vr_grid_bw
+ *   - the Parameter tensor view + param_buffer   include/common.h:88-94,110-179 -> vr_tensor
+ *   - the 6->2 majorant map contract             include/vr_majorant.h:1-7    -> vr_majorant_build + VR_MAJ_GRID
+ * Entry points without a reference counterpart are marked as such below.
+ */
+#ifndef VR_API_H
+#define VR_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VR_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum vr_status {
+    VR_OK = 0,
+    VR_ERR_ARG = 1,          /* invalid argument (shape, stride, null pointer, range) */
+    VR_ERR_UNSUPPORTED = 2,  /* map kind / option combination not built */
+    VR_ERR_DEVICE = 3        /* HIP runtime error (message in vr_last_error) */
+};
+
+/* ---- enums --------------------------------------------------------------- */
+enum vr_field_kind {         /* scattering albedo source */
+    VR_FIELD_CONST = 0,      /* const[r,g,b]                       (_maps.py:1957-1985) */
+    VR_FIELD_GRID = 1        /* Grid3D with C == 3 (or C == 1)     (_maps.py:2319-2365) */
+};
+
+enum vr_phase_kind {
+    VR_PHASE_HG_MIX = 0,     /* HGPhase eval 0.8 HG(g)+0.2 HG(g/5) and VHGPhaseSampler
+                                (g/5 with p=0.8), _maps.py:4436-4484 */
+    VR_PHASE_HG = 1,         /* single Henyey-Greenstein for eval and sampling
+                                (_scenes.py:1969-2004) */
+    VR_PHASE_ISO = 2         /* isotropic 1/(4 pi) */
+};
+
+enum vr_majorant_kind {
+    VR_MAJ_CONST = 0,        /* const[value, distance] (_objects.py:92-114) */
+    VR_MAJ_GRID = 1          /* coarse max grid built by vr_majorant_build, traversed by DDA */
+};
+
+enum vr_transmittance_kind { /* estimator behind transmittance() in NEE / single scatter */
+    VR_T_DT = 0,             /* include/vr_transmittance_DT.h */
+    VR_T_RT = 1,             /* include/vr_transmittance_RT.h */
+    VR_T_DDA = 2             /* include/vr_transmittance_DDA.h */
+};
+
+enum vr_map_kind {           /* input -> output dims */
+    VR_MAP_RAYBOX = 1,       /* 6 -> 2   RayBoxIntersection          _maps.py:4128-4154 */
+    VR_MAP_T_DDA = 2,        /* 6 -> 1   GridDDATransmittance        _maps.py:2640-2667 */
+    VR_MAP_T_DT = 3,         /* 6 -> 1   DeltatrackingTransmittance  _maps.py:2685-2697 */
+    VR_MAP_T_RT = 4,         /* 6 -> 1   RatiotrackingTransmittance  _maps.py:2670-2682 */
+    VR_MAP_COLLISION_DT = 5, /* 6 -> 3   DeltatrackingCollisionSampler (C, t, T) _maps.py:2714-2727 */
+    VR_MAP_SS_RADIANCE = 6,  /* 6 -> 3   SingleScatteredRadiance     _maps.py:2761-2784 */
+    VR_MAP_MC_COLLISION = 7, /* 6 -> 3   MCCollisionIntegrator(collision DT, exitance =
+                                         SingleScatteredRadiance, environment) _maps.py:2788-2808 */
+    VR_MAP_NEE_DT = 8,       /* 6 -> 3   DeltatrackingNEEPathIntegrator _maps.py:2924-2978 */
+    VR_MAP_MEDIUM_DT = 9     /* 7 -> 12  DTMediumPathIntegrator      _scenes.py:1614-1632 */
+};
+
+enum vr_seed_mode {          /* per-thread RNG state derivation from the 4 dispatch seeds */
+    VR_SEED_MAP = 0,         /* map eval kernels            _maps.py:390-393,465-469 */
+    VR_SEED_CAPTURE_FW = 1,  /* capture forward kernel      _maps.py:549-560 */
+    VR_SEED_CAPTURE_BW = 2   /* capture backward kernel     _maps.py:676-685 */
+};
+
+enum vr_bw_mode {
+    VR_BW_REFERENCE = 0,     /* the reference's backward estimators, quirks included */
+    VR_BW_CONSISTENT = 1     /* same estimators with the primal/replay majorants matched and the
+                                single-scatter transmittance gradient taken along the sampled ray */
+};
+
+/* ---- data structures ------------------------------------------------------ */
+
+/* Strided FP32 tensor view (mirrors `Parameter`, include/common.h:88-94).
+ * Index order is the torch order [D,H,W,C]; strides are in elements.
+ * grad == NULL means "no gradient requested" (include/maps/grid3d.h:41). */
+typedef struct vr_tensor {
+    float*  data;
+    float*  grad;
+    int64_t shape[4];
+    int64_t stride[4];
+    int32_t ndim;
+    int32_t reserved;
+} vr_tensor;
+
+/* Grid3D map: vertex-centred trilinear field over the box [bmin, bmax]
+ * (include/maps/grid3d.h:6-75). */
+typedef struct vr_grid {
+    vr_tensor t;             /* [D,H,W,C] */
+    float bmin[3];           /* x, y, z */
+    float bmax[3];
+} vr_grid;
+
+/* A heterogeneous medium: the lowered form of the reference's map graph for
+ * sigma / boundary / majorant / albedo / emission / environment / phase. */
+typedef struct vr_medium {
+    vr_grid sigma;                 /* C == 1 */
+    float   box_min[3];            /* boundary = RayBoxIntersection(box_min, box_max) */
+    float   box_max[3];
+    int32_t albedo_kind;           /* vr_field_kind */
+    float   albedo_c[3];
+    vr_grid albedo;                /* used when albedo_kind == VR_FIELD_GRID */
+    float   emission_c[3];         /* constant emission (ZERO in the recipes) */
+    float   env_c[3];              /* constant environment radiance */
+    float   sun_dir[3];            /* directional light: unit vector pointing TO the light */
+    float   sun_radiance[3];
+    float   sun_prob;              /* 0 = no sun; else mixture selection probability */
+    int32_t phase_kind;            /* vr_phase_kind */
+    float   phase_g;
+    int32_t majorant_kind;         /* vr_majorant_kind */
+    float   majorant_value;        /* VR_MAJ_CONST: [value, distance] */
+    float   majorant_distance;
+    const float* majorant_grid;    /* VR_MAJ_GRID: [rz, ry, rx] row-major (x fastest) */
+    int32_t majorant_res[3];       /* rz, ry, rx */
+    int32_t transmittance_kind;    /* vr_transmittance_kind */
+    float   ds_epsilon;            /* defensive-sampling epsilon of the backward estimators */
+    int32_t max_bounces;           /* < 0: unbounded (reference); else scatter events per path */
+    int32_t medium_filter;         /* DTMediumPathIntegrator MEDIUM_FILTER: bit0 scatter, bit1 gather */
+    float   gathering_c[3];        /* constant gathering map of the 7->12 integrator */
+    float   medium_majorant;       /* scalar majorant of the 7->12 integrator */
+} vr_medium;
+
+/* PerspectiveCameraSensor (_maps.py:1765-1786, include/maps/sensor_perspective_camera.h). */
+typedef struct vr_camera {
+    const float*   poses;          /* [num_poses, 9] = origin, direction, up */
+    int32_t        num_poses;
+    int32_t        width;
+    int32_t        height;
+    int32_t        jittered;       /* generation_mode: 0 pixel centre, 1 two randoms */
+    float          fov;
+    float          znear;
+    const int64_t* sensors;        /* optional [num_sensors, 3] (pose, row, col); NULL = all */
+    int64_t        num_sensors;
+} vr_camera;
+
+/* Per-dispatch settings (the uniform block of the generated kernels, _maps.py:530-542). */
+typedef struct vr_launch {
+    uint32_t seeds[4];             /* dispatch seeds, each in [129, 2^30) (_internal.py:26-34) */
+    int32_t  samples;              /* spp (capture); ignored by maps */
+    int32_t  bw_mode;              /* vr_bw_mode */
+    int32_t  seed_mode;            /* vr_seed_mode used by vr_capture_bw (fw uses CAPTURE_FW) */
+    int32_t  reserved;
+    int64_t  start_index;          /* first thread index of this batch (batch_size chunking) */
+    int64_t  count;                /* threads in this batch; < 0 = all remaining */
+    int32_t* debug_out;            /* optional per-thread counter of trilinear lookups */
+} vr_launch;
+
+/* ---- entry points ---------------------------------------------------------- */
+
+/* ABI version of the loaded library (VR_ABI_VERSION). No reference counterpart. */
+int vr_abi_version(void);
+
+/* Thread-local message for the last non-OK status. No reference counterpart
+ * (the reference reports errors with Python asserts, _maps.py:760,902-903). */
+const char* vr_last_error(void);
+
+/* Grid3D forward on points x[n,3] -> out[n,C]  (include/maps/grid3d.h:6-37;
+ * dispatched by eval_map_forward, _maps.py:885-925). */
+int vr_grid_fw(const vr_grid* grid, const float* x, int64_t n, float* out, void* stream);
+
+/* Grid3D backward: accumulates out_grad[n,C] trilinearly into grid->t.grad
+ * (include/maps/grid3d.h:39-75; eval_map_backward, _maps.py:928-976). */
+int vr_grid_bw(const vr_grid* grid, const float* x, const float* out_grad, int64_t n, void* stream);
+
+/* Coarse majorant grid: out[rz,ry,rx] = max of sigma over the vertices whose
+ * trilinear footprint touches each coarse cell.  Implements the 6->2 majorant
+ * contract (include/vr_majorant.h:1-7) that the reference fills with
+ * const[max(sigma), 1e6] (_objects.py:96-101). */
+int vr_majorant_build(const vr_grid* sigma, const int32_t res[3], float* out, void* stream);
+
+/* Map forward on rays: in[n,IN] -> out[n,OUT] for the map `kind`
+ * (eval_map_forward, _maps.py:885-925; per-thread seeds _maps.py:390-393). */
+int vr_map_fw(const vr_medium* medium, int32_t kind, const float* in, int64_t n, float* out,
+              const vr_launch* launch, void* stream);
+
+/* Map backward: accumulates parameter gradients (sigma.grad, albedo.grad) for
+ * out_grad[n,OUT] (eval_map_backward, _maps.py:928-976). */
+int vr_map_bw(const vr_medium* medium, int32_t kind, const float* in, const float* out_grad, int64_t n,
+              const vr_launch* launch, void* stream);
+
+/* Capture forward: out[cams,H,W,OUT] (or [num_sensors,OUT]) = spp-mean of the
+ * field `kind` over camera rays (eval_capture_forward, _maps.py:749-821). */
+int vr_capture_fw(const vr_medium* medium, int32_t kind, const vr_camera* camera,
+                  const vr_launch* launch, float* out, void* stream);
+
+/* Capture backward: accumulates into sigma.grad / albedo.grad the gradient of
+ * sum(out_grad * out) (eval_capture_backward, _maps.py:824-882). */
+int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera,
+                  const vr_launch* launch, const float* out_grad, void* stream);
+
+/* Fused Adam step over n parameters (torch.optim.Adam semantics, bias-corrected,
+ * `step` >= 1); zero_grad != 0 also clears g.  No reference counterpart: the
+ * reference leaves optimisation to torch. */
+int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
+                 float eps, int32_t step, int32_t zero_grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VR_API_H */

@@ -1,0 +1,1534 @@
+/*
+ * vr_oracle.c -- CPU oracle for the participating-media hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
+ * reference's GLSL algorithms (rendervous, the include/ headers), used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker the
+ * HIP library is compared against.  Nothing in the product links, loads or
+ * calls it.
+ *
+ * Precision policy: every real-valued quantity is IEEE double (the "FP64
+ * oracle"); the RNG is the reference's integer HybridTaus generator, restated
+ * bit-exactly, and its uniforms are the same float values the GPU draws.  With
+ * the same seeds the oracle therefore makes the same stochastic decisions as the
+ * FP32 kernels except where a comparison falls within rounding distance.
+ *
+ * Exported symbols mirror include/vr_api.h with the prefix "oracle_" and host
+ * pointers (no stream argument).
+ *
+ * Parity pins (what checks each function; see tests/):
+ *   rng_*                golden LCG sequence, Tausworthe step structure, uniformity
+ *   ray_box / boundary   slab-test special cases, axis-parallel rays
+ *   grid_lookup / _bw    closed form (affine fields are reproduced exactly),
+ *                        torch grid_sample(align_corners=True) in the interior,
+ *                        adjoint (dot-product) identity
+ *   dda_fw / dda_bw      constant grid => exp(-sigma*chord); field linear along the
+ *                        ray => exact; central finite differences of dda_fw
+ *   t_dt / t_rt          homogeneous closed form exp(-sigma*d); exact line integral
+ *                        (adaptive quadrature) of a heterogeneous field
+ *   collision_dt         homogeneous P(C=1) = 1-exp(-sigma*d), KS test of t
+ *   nee_fw               white furnace (albedo 1, consistent phase) R == env;
+ *                        albedo 0 => R == T*env
+ *   *_bw                 central finite differences with common random numbers
+ *   majorant_build       brute-force bound of the trilinear field
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/vr_api.h"
+
+#define PI_D 3.1415926535897932384626433832795
+#define TWO_PI_D 6.283185307179586476925286766559
+/* Remaining distance below which a majorant-cell crossing ends the tracking.
+ * Where the last coarse cell's exit coincides with the medium's exit the
+ * crossing test is a rounding tie; ending there keeps the random stream of the
+ * FP32 and FP64 implementations aligned. */
+#define CROSS_EPS 1e-6
+
+static __thread char g_err[512];
+
+static int fail(int code, const char* msg) {
+    snprintf(g_err, sizeof(g_err), "%s", msg);
+    return code;
+}
+
+const char* oracle_last_error(void) { return g_err; }
+int oracle_abi_version(void) { return VR_ABI_VERSION; }
+
+/* ------------------------------------------------------------------------- */
+/* RNG: include/randoms.h:6-59                                                 */
+/* ------------------------------------------------------------------------- */
+typedef struct { uint32_t x, y, z, w; } rng_t;
+
+/* randoms.h:6 */
+static uint32_t taus_step(uint32_t z, int s1, int s2, int s3, uint32_t m) {
+    uint32_t b = ((z << s1) ^ z) >> s2;
+    return ((z & m) << s3) ^ b;
+}
+
+/* randoms.h:41-47 */
+static void rng_advance(rng_t* s) {
+    s->x = taus_step(s->x, 13, 19, 12, 4294967294u);
+    s->y = taus_step(s->y, 2, 25, 4, 4294967288u);
+    s->z = taus_step(s->z, 3, 11, 17, 4294967280u);
+    s->w = 1664525u * s->w + 1013904223u;
+}
+
+static float bits_to_float(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static uint32_t float_to_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+/* randoms.h:53-59: 23 random mantissa bits in [1,2), minus 1. */
+static float rng_random(rng_t* s) {
+    rng_advance(s);
+    uint32_t v = s->x ^ s->y ^ s->z ^ s->w;
+    return bits_to_float((v & 0x007FFFFFu) | 0x3F800000u) - 1.0f;
+}
+
+/* Capture forward per-thread init, _maps.py:549-560. */
+static rng_t seed_capture_fw(const uint32_t sd[4], uint32_t i) {
+    rng_t s;
+    s.x = sd[0] + (0x23F1u + i);
+    s.y = sd[1] + (0x3137u + (i + 14u) * i * 17923113u);
+    s.z = sd[2] + 129u * i;
+    s.w = sd[3] + (i + 129u);
+    rng_advance(&s);
+    rng_advance(&s);
+    float a = rng_random(&s), b = rng_random(&s), c = rng_random(&s), d = rng_random(&s);
+    s.x = float_to_bits(a); s.y = float_to_bits(b); s.z = float_to_bits(c); s.w = float_to_bits(d);
+    rng_advance(&s); rng_advance(&s); rng_advance(&s); rng_advance(&s);
+    return s;
+}
+
+/* Capture backward per-thread init, _maps.py:676-685. */
+static rng_t seed_capture_bw(const uint32_t sd[4], uint32_t i) {
+    rng_t s;
+    s.x = sd[0] + 0x23F1u;
+    s.y = sd[1] + 0x3137u;
+    s.z = sd[2] + 129u;
+    s.w = sd[3] + (i + 129u);
+    rng_advance(&s); rng_advance(&s); rng_advance(&s); rng_advance(&s);
+    float a = rng_random(&s), b = rng_random(&s), c = rng_random(&s), d = rng_random(&s);
+    s.x = float_to_bits(a); s.y = float_to_bits(b); s.z = float_to_bits(c); s.w = float_to_bits(d);
+    return s;
+}
+
+/* Map eval per-thread init, _maps.py:390-393 (fw) and :465-469 (bw). */
+static rng_t seed_map(const uint32_t sd[4], uint32_t i) {
+    rng_t s;
+    s.x = sd[0] ^ (i ^ 17u);
+    s.y = sd[1] ^ (i * 123111171u);
+    s.z = sd[2] ^ (i + 11u);
+    s.w = sd[3] ^ (i ^ (sd[0] + 13u * sd[1]));
+    rng_random(&s); rng_random(&s); rng_random(&s);
+    return s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* small vector helpers                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct { double x, y, z; } v3;
+static v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 vmul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static v3 vs(v3 a, double s) { return V(a.x * s, a.y * s, a.z * s); }
+static double vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 vcross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static v3 vnorm(v3 a) { double l = sqrt(vdot(a, a)); return vs(a, 1.0 / l); }
+static double vget(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+static v3 vf(const float* p) { return V(p[0], p[1], p[2]); }
+static double dmin(double a, double b) { return a < b ? a : b; }
+static double dmax(double a, double b) { return a > b ? a : b; }
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+static int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* ------------------------------------------------------------------------- */
+/* Ray-box: include/maths.h:11-38 and include/vr_boundary.h:1-11             */
+/* ------------------------------------------------------------------------- */
+static void ray_box_intersection(v3 x, v3 w, v3 bmin, v3 bmax, double* tmin, double* tmax) {
+    /* maths.h:30-38: |w_i| <= 1e-6 is replaced by 1e-6 ("un-parallelize"). */
+    double wx = fabs(w.x) <= 1e-6 ? 1e-6 : w.x;
+    double wy = fabs(w.y) <= 1e-6 ? 1e-6 : w.y;
+    double wz = fabs(w.z) <= 1e-6 ? 1e-6 : w.z;
+    double c0x = (bmin.x - x.x) / wx, c1x = (bmax.x - x.x) / wx;
+    double c0y = (bmin.y - x.y) / wy, c1y = (bmax.y - x.y) / wy;
+    double c0z = (bmin.z - x.z) / wz, c1z = (bmax.z - x.z) / wz;
+    *tmin = dmax(dmax(dmin(c0x, c1x), dmin(c0y, c1y)), dmin(c0z, c1z));
+    *tmax = dmin(dmin(dmax(c0x, c1x), dmax(c0y, c1y)), dmax(c0z, c1z));
+}
+
+/* maths.h:11-26 */
+static int intersect_ray_box(v3 x, v3 w, v3 bmin, v3 bmax, double* tmin, double* tmax) {
+    ray_box_intersection(x, w, bmin, bmax, tmin, tmax);
+    *tmin = dmax(0.0, *tmin);
+    if (*tmax <= *tmin || *tmax <= 0) return 0;
+    return 1;
+}
+
+/* vr_boundary.h:1-11 with boundary = RayBoxIntersection(box_min, box_max). */
+static int boundary(const vr_medium* M, v3 x, v3 w, double* tmin, double* tmax) {
+    ray_box_intersection(x, w, vf(M->box_min), vf(M->box_max), tmin, tmax);
+    *tmin = dmax(0.0, *tmin);
+    if (*tmax < *tmin) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Grid3D: include/maps/grid3d.h:6-75, Parameter addressing common.h:110-179  */
+/* ------------------------------------------------------------------------- */
+static int64_t goff(const vr_tensor* t, int z, int y, int x) {
+    return t->stride[0] * z + t->stride[1] * y + t->stride[2] * x;
+}
+
+/* Grid3D FORWARD: zero outside [0,1)^3 of the normalised box; vertex-centred
+ * trilinear with dim = shape-1 (grid3d.h:13-36).  Writes C channels. */
+static void grid_lookup(const vr_grid* g, v3 c, double* out) {
+    const vr_tensor* t = &g->t;
+    int C = (int)t->shape[3];
+    for (int k = 0; k < C; k++) out[k] = 0.0;
+    double nx = (c.x - g->bmin[0]) / ((double)g->bmax[0] - g->bmin[0]);
+    double ny = (c.y - g->bmin[1]) / ((double)g->bmax[1] - g->bmin[1]);
+    double nz = (c.z - g->bmin[2]) / ((double)g->bmax[2] - g->bmin[2]);
+    if (nx < 0 || ny < 0 || nz < 0 || nx >= 1 || ny >= 1 || nz >= 1) return;
+    int dx = (int)t->shape[2] - 1, dy = (int)t->shape[1] - 1, dz = (int)t->shape[0] - 1;
+    double gx = nx * dx, gy = ny * dy, gz = nz * dz;
+    double ax = gx - floor(gx), ay = gy - floor(gy), az = gz - floor(gz);
+    int px = iclamp((int)gx, 0, dx - 1), py = iclamp((int)gy, 0, dy - 1), pz = iclamp((int)gz, 0, dz - 1);
+    double wgt[8] = {
+        (1 - ax) * (1 - ay) * (1 - az), ax * (1 - ay) * (1 - az), (1 - ax) * ay * (1 - az), ax * ay * (1 - az),
+        (1 - ax) * (1 - ay) * az, ax * (1 - ay) * az, (1 - ax) * ay * az, ax * ay * az};
+    for (int k = 0; k < 8; k++) {
+        int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const float* p = t->data + goff(t, pz + oz, py + oy, px + ox);
+        for (int ch = 0; ch < C; ch++) out[ch] += (double)p[t->stride[3] * ch] * wgt[k];
+    }
+}
+
+/* Gradient accumulation target: either the grid's float grad (serial), or a
+ * private double buffer (parallel capture bw, reduced afterwards). */
+typedef struct {
+    double* sig;     /* private accumulation for sigma grad, or NULL */
+    double* alb;     /* private accumulation for albedo grad, or NULL */
+} gacc_t;
+
+static void acc_add(const vr_tensor* t, double* priv, int64_t off, double v) {
+    if (priv) priv[off] += v;
+    else t->grad[off] += (float)v;
+}
+
+/* Grid3D BACKWARD (grid3d.h:39-75): 8 trilinear-weighted accumulations. */
+static void grid_scatter(const vr_grid* g, double* priv, v3 c, const double* og) {
+    const vr_tensor* t = &g->t;
+    if (t->grad == NULL) return;
+    int C = (int)t->shape[3];
+    double nx = (c.x - g->bmin[0]) / ((double)g->bmax[0] - g->bmin[0]);
+    double ny = (c.y - g->bmin[1]) / ((double)g->bmax[1] - g->bmin[1]);
+    double nz = (c.z - g->bmin[2]) / ((double)g->bmax[2] - g->bmin[2]);
+    if (nx < 0 || ny < 0 || nz < 0 || nx >= 1 || ny >= 1 || nz >= 1) return;
+    int dx = (int)t->shape[2] - 1, dy = (int)t->shape[1] - 1, dz = (int)t->shape[0] - 1;
+    double gx = nx * dx, gy = ny * dy, gz = nz * dz;
+    double ax = gx - floor(gx), ay = gy - floor(gy), az = gz - floor(gz);
+    int px = iclamp((int)gx, 0, dx - 1), py = iclamp((int)gy, 0, dy - 1), pz = iclamp((int)gz, 0, dz - 1);
+    double wgt[8] = {
+        (1 - ax) * (1 - ay) * (1 - az), ax * (1 - ay) * (1 - az), (1 - ax) * ay * (1 - az), ax * ay * (1 - az),
+        (1 - ax) * (1 - ay) * az, ax * (1 - ay) * az, (1 - ax) * ay * az, ax * ay * az};
+    for (int k = 0; k < 8; k++) {
+        int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        int64_t base = goff(t, pz + oz, py + oy, px + ox);
+        for (int ch = 0; ch < C; ch++) acc_add(t, priv, base + t->stride[3] * ch, og[ch] * wgt[k]);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Medium fields: sigma (vr_sigma.h), albedo (vr_scattering_albedo.h),        */
+/* emission (vr_emission.h), environment (vr_environment.h)                   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    const vr_medium* M;
+    rng_t rng;
+    gacc_t acc;
+    int64_t lookups;  /* trilinear sigma lookups (debug counter) */
+    int bw_mode;
+} ctx_t;
+
+static double sigma_at(ctx_t* c, v3 x) {
+    double v;
+    c->lookups++;
+    grid_lookup(&c->M->sigma, x, &v);
+    return v;
+}
+
+static void sigma_bw(ctx_t* c, v3 x, double g) {
+    grid_scatter(&c->M->sigma, c->acc.sig, x, &g);
+}
+
+static v3 albedo_at(ctx_t* c, v3 x) {
+    const vr_medium* M = c->M;
+    if (M->albedo_kind == VR_FIELD_CONST) return vf(M->albedo_c);
+    double v[4] = {0, 0, 0, 0};
+    grid_lookup(&M->albedo, x, v);
+    if (M->albedo.t.shape[3] == 1) return V(v[0], v[0], v[0]);  /* promote 1 -> 3 */
+    return V(v[0], v[1], v[2]);
+}
+
+static void albedo_bw(ctx_t* c, v3 x, v3 g) {
+    const vr_medium* M = c->M;
+    if (M->albedo_kind != VR_FIELD_GRID || M->albedo.t.grad == NULL) return;
+    double og[3];
+    if (M->albedo.t.shape[3] == 1) { og[0] = g.x + g.y + g.z; }
+    else { og[0] = g.x; og[1] = g.y; og[2] = g.z; }
+    grid_scatter(&M->albedo, c->acc.alb, x, og);
+}
+
+static double rnd(ctx_t* c) { return (double)rng_random(&c->rng); }
+
+/* ------------------------------------------------------------------------- */
+/* Directions: randoms.h:144-165, scattering_AD.h:4-52                       */
+/* ------------------------------------------------------------------------- */
+/* randoms.h:144-151 */
+static void ortho_basis(v3 n, v3* t, v3* b) {
+    double sign = n.z >= 0 ? 1.0 : -1.0;
+    double a = -1.0 / (sign + n.z);
+    double bb = n.x * n.y * a;
+    *t = V(1.0 + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+    *b = V(bb, sign + n.y * n.y * a, -n.y);
+}
+
+/* randoms.h:153-165 randomDirection(D) */
+static v3 random_direction(ctx_t* c, v3 D) {
+    double r1 = rnd(c);
+    double r2 = rnd(c) * 2 - 1;
+    double s = sqrt(dmax(0.0, 1.0 - r2 * r2));
+    double xx = cos(TWO_PI_D * r1) * s, yy = sin(TWO_PI_D * r1) * s, zz = r2;
+    v3 t0, t1;
+    ortho_basis(D, &t0, &t1);
+    return vadd(vadd(vs(t0, xx), vs(t1, yy)), vs(D, zz));
+}
+
+/* scattering_AD.h:4-19 */
+static v3 hg_sample(ctx_t* c, v3 win, double g) {
+    double phi = rnd(c) * 2 * PI_D;
+    double xi = rnd(c);
+    double cos_t;
+    if (fabs(g) < 0.001) cos_t = 2 * xi - 1;
+    else {
+        double g2 = g * g;
+        double t = (1.0 - g2) / (1.0 - g + 2.0 * g * xi);
+        cos_t = (0.5 / g) * (1.0 + g2 - t * t);
+    }
+    double sin_t = sqrt(dmax(0.0, 1.0 - cos_t * cos_t));
+    v3 t0, t1;
+    ortho_basis(win, &t0, &t1);
+    return vadd(vadd(vs(t0, sin_t * sin(phi)), vs(t1, sin_t * cos(phi))), vs(win, cos_t));
+}
+
+/* scattering_AD.h:39-47 */
+static double hg_eval(double cos_t, double g) {
+    if (fabs(g) < 0.001) return 0.25 / PI_D;
+    double g2 = g * g;
+    return 0.25 / PI_D * (1.0 - g2) / pow(1.0 + g2 - 2 * g * cos_t, 1.5);
+}
+
+/* phase(x, w, wo): HGPhase (_maps.py:4446-4454) / single HG / isotropic. */
+static double phase_eval(const vr_medium* M, v3 w, v3 wo) {
+    double ct = vdot(w, wo);
+    switch (M->phase_kind) {
+    case VR_PHASE_HG_MIX: {
+        double a = hg_eval(ct, M->phase_g), b = hg_eval(ct, M->phase_g / 5);
+        return a * (1 - 0.2) + b * 0.2;
+    }
+    case VR_PHASE_HG: return hg_eval(ct, M->phase_g);
+    default: return 0.25 / PI_D;
+    }
+}
+
+/* phase_sampler(x, w) -> (rho, wo): VHGPhaseSampler (_maps.py:4463-4481) /
+ * HGPhaseSampler (_scenes.py:1984-2004) / uniform. */
+static double phase_sample(ctx_t* c, v3 w, v3* wo) {
+    const vr_medium* M = c->M;
+    switch (M->phase_kind) {
+    case VR_PHASE_HG_MIX: {
+        double g = M->phase_g;
+        if (rnd(c) < 0.8) g /= 5;
+        *wo = hg_sample(c, w, g);
+        return 1.0;
+    }
+    case VR_PHASE_HG: *wo = hg_sample(c, w, M->phase_g); return 1.0;
+    default: *wo = random_direction(c, V(0, 1, 0)); return 1.0;
+    }
+}
+
+/* environment_sampler(x, w) -> (L/pdf, wo):
+ * _functionsampler(UniformDirectionSampler, const env) (_maps.py:2552-2585,
+ * maps/uniform_direction_sampler.h), optionally mixed with a directional light
+ * through MixtureSampler semantics (_maps.py:1541-1571; sub-samplers pre-scaled
+ * by their selection probability). */
+static v3 env_sample(ctx_t* c, v3* wo) {
+    const vr_medium* M = c->M;
+    if (M->sun_prob > 0) {
+        if (rnd(c) < M->sun_prob) {
+            *wo = vf(M->sun_dir);
+            return vs(vf(M->sun_radiance), 1.0 / M->sun_prob);
+        }
+        *wo = random_direction(c, V(0, 1, 0));
+        return vs(vf(M->env_c), 4 * PI_D / (1.0 - M->sun_prob));
+    }
+    *wo = random_direction(c, V(0, 1, 0));
+    return vs(vf(M->env_c), 4 * PI_D);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Majorant: include/vr_majorant.h:1-7 (value, distance).                     */
+/* VR_MAJ_CONST returns the constant pair.  VR_MAJ_GRID walks the coarse max  */
+/* grid with a 3-D DDA: value = current coarse cell's bound, distance = to the */
+/* cell's exit along the ray.                                                 */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int grid;
+    double value, dist;          /* const */
+    int cell[3], step[3], res[3];
+    double tnext[3], tdelta[3], t;
+    int inside;
+    const float* data;
+} maj_t;
+
+static void maj_begin(const vr_medium* M, maj_t* m, v3 x, v3 w) {
+    m->grid = M->majorant_kind == VR_MAJ_GRID;
+    if (!m->grid) { m->value = M->majorant_value; m->dist = M->majorant_distance; return; }
+    m->data = M->majorant_grid;
+    m->res[0] = M->majorant_res[2]; m->res[1] = M->majorant_res[1]; m->res[2] = M->majorant_res[0];
+    m->t = 0;
+    m->inside = 1;
+    for (int a = 0; a < 3; a++) {
+        double bmin = M->sigma.bmin[a], bmax = M->sigma.bmax[a];
+        double cs = (bmax - bmin) / m->res[a];
+        double xa = vget(x, a), wa = vget(w, a);
+        int cl = (int)floor((xa - bmin) / cs);
+        cl = iclamp(cl, 0, m->res[a] - 1);
+        m->cell[a] = cl;
+        if (wa > 0) { m->step[a] = 1; m->tnext[a] = (bmin + (cl + 1) * cs - xa) / wa; m->tdelta[a] = cs / wa; }
+        else if (wa < 0) { m->step[a] = -1; m->tnext[a] = (bmin + cl * cs - xa) / wa; m->tdelta[a] = -cs / wa; }
+        else { m->step[a] = 0; m->tnext[a] = INFINITY; m->tdelta[a] = INFINITY; }
+    }
+}
+
+static double maj_query(const maj_t* m, double* md) {
+    if (!m->grid) { *md = m->dist; return m->value; }
+    if (!m->inside) { *md = INFINITY; return 0.0; }
+    double tn = dmin(m->tnext[0], dmin(m->tnext[1], m->tnext[2]));
+    *md = dmax(0.0, tn - m->t);
+    return m->data[((int64_t)m->cell[2] * m->res[1] + m->cell[1]) * m->res[0] + m->cell[0]];
+}
+
+/* The tracked point moved by dt inside the current cell. */
+static void maj_move(maj_t* m, double dt) { if (m->grid) m->t += dt; }
+
+/* The tracked point moved to the current cell's exit: step the DDA. */
+static void maj_cross(maj_t* m) {
+    if (!m->grid || !m->inside) return;
+    int a = (m->tnext[0] <= m->tnext[1] && m->tnext[0] <= m->tnext[2]) ? 0 : (m->tnext[1] <= m->tnext[2] ? 1 : 2);
+    m->t = m->tnext[a];
+    m->cell[a] += m->step[a];
+    m->tnext[a] += m->tdelta[a];
+    if (m->cell[a] < 0 || m->cell[a] >= m->res[a]) m->inside = 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Transmittance estimators                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* vr_transmittance_DT.h:1-28 */
+static double t_dt(ctx_t* c, v3 x, v3 w) {
+    double tmin, tmax;
+    if (!boundary(c->M, x, w, &tmin, &tmax)) return 1.0;
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    maj_t mj;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(1e-7, maj_query(&mj, &md));
+        double dt = -log(1.0 - rnd(c)) / maj;
+        double t = dmin(md, dt);
+        x = vadd(x, vs(w, t));
+        d -= t;
+        if (dt > md) { maj_cross(&mj); if (d <= CROSS_EPS) return 1.0; continue; }
+        maj_move(&mj, t);
+        if (d <= 1e-7) return 1.0;
+        if (rnd(c) < sigma_at(c, x) / maj) return 0.0;
+    }
+}
+
+/* vr_transmittance_RT.h:1-49 (with majorant-cell crossings for VR_MAJ_GRID). */
+static double t_rt(ctx_t* c, v3 x, v3 w) {
+    double T = 1.0, tmin, tmax;
+    if (!boundary(c->M, x, w, &tmin, &tmax)) return T;
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    maj_t mj;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(1e-7, maj_query(&mj, &md));
+        double s = -log(1.0 - rnd(c)) / maj;
+        if (mj.grid && md < d && s > md) {
+            x = vadd(x, vs(w, md)); d -= md; maj_cross(&mj);
+            if (d <= CROSS_EPS) return T;
+            continue;
+        }
+        double dt = dmin(s, dmin(d, md));
+        x = vadd(x, vs(w, dt));
+        d -= dt;
+        maj_move(&mj, dt);
+        if (d <= 1e-7) return T;
+        T *= (1 - sigma_at(c, x) / maj);
+        if (T < 0.01) {
+            if (rnd(c) < 1 - T) return 0.0;
+            T = 1.0;
+        }
+    }
+}
+
+/* vr_sigma_grid.h:1-28 (cell corners clamped to [0, dim]) */
+static double dda_sigma_at(const vr_grid* g, int x, int y, int z) {
+    const vr_tensor* t = &g->t;
+    int dx = (int)t->shape[2] - 1, dy = (int)t->shape[1] - 1, dz = (int)t->shape[0] - 1;
+    x = iclamp(x, 0, dx); y = iclamp(y, 0, dy); z = iclamp(z, 0, dz);
+    return t->data[goff(t, z, y, x)];
+}
+
+static double lerp(double a, double b, double t) { return a * (1 - t) + b * t; }
+
+typedef struct {
+    v3 x;                 /* entry point */
+    double d;             /* length inside both boxes */
+    int cell[3], side[3];
+    double ainc[3], alpha0[3];
+    double vn[3], vm[3];
+    int dim[3];
+} dda_t;
+
+/* vr_transmittance_DDA.h:3-33: returns 0 when the ray misses. */
+static int dda_begin(const vr_medium* M, v3 x, v3 w, dda_t* s) {
+    double tmin, tmax, gmin, gmax;
+    if (!boundary(M, x, w, &tmin, &tmax)) return 0;
+    const vr_grid* g = &M->sigma;
+    v3 bmin = vf(g->bmin), bmax = vf(g->bmax);
+    if (!intersect_ray_box(x, w, bmin, bmax, &gmin, &gmax)) return 0;
+    tmin = dmax(tmin, gmin);
+    tmax = dmin(tmax, gmax);
+    if (tmax <= tmin) return 0;
+    x = vadd(x, vs(w, tmin));
+    s->x = x;
+    s->d = tmax - tmin;
+    v3 box = vsub(bmax, bmin);
+    s->dim[0] = (int)g->t.shape[2] - 1; s->dim[1] = (int)g->t.shape[1] - 1; s->dim[2] = (int)g->t.shape[0] - 1;
+    for (int a = 0; a < 3; a++) {
+        double xa = vget(x, a), wa = vget(w, a), bm = vget(bmin, a), bs = vget(box, a);
+        double cs = bs / s->dim[a];
+        int cl = (int)((xa - bm) * s->dim[a] / bs);
+        cl = iclamp(cl, 0, s->dim[a] - 1);
+        s->cell[a] = cl;
+        double aw = dmax(1e-6, fabs(wa));
+        s->ainc[a] = cs / aw;
+        s->side[a] = wa > 0 ? 1 : (wa < 0 ? -1 : 0);
+        double corner = (cl + s->side[a] * 0.5 + 0.5) * cs + bm;
+        s->alpha0[a] = fabs(corner - xa) / aw;
+        s->vn[a] = (xa - bm) * s->dim[a] / bs;
+        s->vm[a] = wa * s->dim[a] / bs;
+    }
+    return 1;
+}
+
+/* The DDA cell step with the axis tie-break of vr_transmittance_DDA.h:48-55. */
+static void dda_step(double alpha[3], int cell[3], const dda_t* s) {
+    int ix = alpha[0] <= alpha[1] && alpha[0] <= alpha[2];
+    int iy = alpha[0] > alpha[1] && alpha[1] <= alpha[2];
+    int iz = alpha[0] > alpha[2] && alpha[1] > alpha[2];
+    alpha[0] += ix * s->ainc[0]; alpha[1] += iy * s->ainc[1]; alpha[2] += iz * s->ainc[2];
+    cell[0] += ix * s->side[0]; cell[1] += iy * s->side[1]; cell[2] += iz * s->side[2];
+}
+
+/* vr_transmittance_DDA.h:1-59: midpoint rule per cell segment. */
+static double dda_tau(ctx_t* c, const dda_t* s) {
+    const vr_grid* g = &c->M->sigma;
+    double alpha[3] = {s->alpha0[0], s->alpha0[1], s->alpha0[2]};
+    int cell[3] = {s->cell[0], s->cell[1], s->cell[2]};
+    double cur = 0, tau = 0;
+    while (cur < s->d - 0.0001) {
+        double next = dmin(s->d, dmin(alpha[0], dmin(alpha[1], alpha[2])));
+        double sg[8];
+        for (int k = 0; k < 8; k++) sg[k] = dda_sigma_at(g, cell[0] + (k & 1), cell[1] + ((k >> 1) & 1), cell[2] + ((k >> 2) & 1));
+        c->lookups++;
+        double tm = lerp(cur, next, 0.5);
+        double ax = s->vm[0] * tm + s->vn[0] - cell[0];
+        double ay = s->vm[1] * tm + s->vn[1] - cell[1];
+        double az = s->vm[2] * tm + s->vn[2] - cell[2];
+        double v = lerp(lerp(lerp(sg[0], sg[1], ax), lerp(sg[2], sg[3], ax), ay),
+                        lerp(lerp(sg[4], sg[5], ax), lerp(sg[6], sg[7], ax), ay), az);
+        tau += v * (next - cur);
+        dda_step(alpha, cell, s);
+        cur = next;
+    }
+    return tau;
+}
+
+static double t_dda(ctx_t* c, v3 x, v3 w) {
+    dda_t s;
+    if (!dda_begin(c->M, x, w, &s)) return 1.0;
+    return exp(-dda_tau(c, &s));
+}
+
+/* vr_transmittance_DDA.h:62-156 */
+static void t_dda_bw(ctx_t* c, v3 x, v3 w, double dL_dT) {
+    if (dL_dT == 0) return;
+    const vr_grid* g = &c->M->sigma;
+    if (g->t.grad == NULL) return;
+    dda_t s;
+    if (!dda_begin(c->M, x, w, &s)) return;
+    double T = exp(-dda_tau(c, &s));
+    double dL_dtau = -dL_dT * T;
+    double alpha[3] = {s.alpha0[0], s.alpha0[1], s.alpha0[2]};
+    int cell[3] = {s.cell[0], s.cell[1], s.cell[2]};
+    double cur = 0;
+    const vr_tensor* t = &g->t;
+    while (cur < s.d - 0.0001) {
+        double next = dmin(s.d, dmin(alpha[0], dmin(alpha[1], alpha[2])));
+        double tm = lerp(cur, next, 0.5);
+        double ax = s.vm[0] * tm + s.vn[0] - cell[0];
+        double ay = s.vm[1] * tm + s.vn[1] - cell[1];
+        double az = s.vm[2] * tm + s.vn[2] - cell[2];
+        double gs = dL_dtau * (next - cur);
+        double wg[8] = {(1 - ax) * (1 - ay) * (1 - az), ax * (1 - ay) * (1 - az), (1 - ax) * ay * (1 - az), ax * ay * (1 - az),
+                        (1 - ax) * (1 - ay) * az, ax * (1 - ay) * az, (1 - ax) * ay * az, ax * ay * az};
+        for (int k = 0; k < 8; k++) {
+            int xx = iclamp(cell[0] + (k & 1), 0, s.dim[0]);
+            int yy = iclamp(cell[1] + ((k >> 1) & 1), 0, s.dim[1]);
+            int zz = iclamp(cell[2] + ((k >> 2) & 1), 0, s.dim[2]);
+            acc_add(t, c->acc.sig, goff(t, zz, yy, xx), gs * wg[k]);
+        }
+        dda_step(alpha, cell, &s);
+        cur = next;
+    }
+}
+
+/* vr_transmittance_DT.h:30-90 */
+static void t_dt_bw(ctx_t* c, v3 x, v3 w, double dL_dT) {
+    if (dL_dT == 0) return;
+    double tmin, tmax;
+    if (!boundary(c->M, x, w, &tmin, &tmax)) return;
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    v3 x0 = x;
+    double d0 = d;
+    rng_t saved = c->rng;
+    maj_t mj;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(1e-7, maj_query(&mj, &md) + 0.1);
+        double dt = -log(1.0 - rnd(c)) / maj;
+        double t = dmin(md, dt);
+        x = vadd(x, vs(w, t));
+        d -= t;
+        if (dt > md) { maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        maj_move(&mj, t);
+        if (d <= 1e-7) break;
+        if (rnd(c) < sigma_at(c, x) / maj) return;
+    }
+    x = x0; d = d0;
+    c->rng = saved;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(1e-7, maj_query(&mj, &md) + 0.1);
+        double dt = -log(1.0 - rnd(c)) / maj;
+        double t = dmin(md, dt);
+        x = vadd(x, vs(w, t));
+        d -= t;
+        if (dt > md) { maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        maj_move(&mj, t);
+        if (d <= 1e-7) break;
+        rnd(c);
+        double Pc = sigma_at(c, x) / maj;
+        sigma_bw(c, x, -dL_dT / maj / (1 - Pc));
+    }
+}
+
+/* vr_transmittance_RT.h:51-141 */
+static void t_rt_bw(ctx_t* c, v3 x, v3 w, double dL_dT) {
+    if (dL_dT == 0) return;
+    double tmin, tmax;
+    if (!boundary(c->M, x, w, &tmin, &tmax)) return;
+    const double EPS = 1e-7;
+    double T = 1.0;
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    v3 x0 = x;
+    double d0 = d;
+    rng_t saved = c->rng;
+    maj_t mj;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(EPS, maj_query(&mj, &md) + 0.1);
+        double s = -log(1.0 - rnd(c)) / maj;
+        if (mj.grid && md < d && s > md) { x = vadd(x, vs(w, md)); d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        double dt = dmin(s, dmin(d, md));
+        x = vadd(x, vs(w, dt));
+        d -= dt;
+        maj_move(&mj, dt);
+        if (d <= EPS) break;
+        T *= (1 - sigma_at(c, x) / maj);
+        if (T < 0.1) {
+            if (rnd(c) < 1 - T) return;
+            T = 1.0;
+        }
+    }
+    double out_T = T;
+    T = 1.0;
+    c->rng = saved;
+    x = x0; d = d0;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(EPS, maj_query(&mj, &md) + 0.1);
+        double s = -log(1.0 - rnd(c)) / maj;
+        if (mj.grid && md < d && s > md) { x = vadd(x, vs(w, md)); d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        double dt = dmin(s, dmin(d, md));
+        x = vadd(x, vs(w, dt));
+        d -= dt;
+        maj_move(&mj, dt);
+        if (d <= EPS) break;
+        double r = 1 - sigma_at(c, x) / maj;
+        T *= r;
+        sigma_bw(c, x, -out_T * dL_dT / maj / r);
+        if (T < 0.1) {
+            rng_advance(&c->rng);  /* vr_transmittance_RT.h:125 skips the RR draw */
+            T = 1.0;
+        }
+    }
+}
+
+static double transmittance(ctx_t* c, v3 x, v3 w) {
+    switch (c->M->transmittance_kind) {
+    case VR_T_RT: return t_rt(c, x, w);
+    case VR_T_DDA: return t_dda(c, x, w);
+    default: return t_dt(c, x, w);
+    }
+}
+
+static void transmittance_bw(ctx_t* c, v3 x, v3 w, double dL_dT) {
+    switch (c->M->transmittance_kind) {
+    case VR_T_RT: t_rt_bw(c, x, w, dL_dT); break;
+    case VR_T_DDA: t_dda_bw(c, x, w, dL_dT); break;
+    default: t_dt_bw(c, x, w, dL_dT); break;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Collision sampler: include/vr_collision_sampler_DT.h:5-134                 */
+/* ------------------------------------------------------------------------- */
+static double collision_dt(ctx_t* c, v3 x, v3 w, double* out_t, double* out_T) {
+    *out_T = 1.0; *out_t = 0.0;
+    double C = 0.0, tmin, tmax;
+    if (!boundary(c->M, x, w, &tmin, &tmax)) return C;
+    double d = tmax - tmin;
+    x = vadd(x, vs(w, tmin));
+    double t = 0;
+    maj_t mj;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = maj_query(&mj, &md);
+        if (mj.grid) maj = dmax(1e-7, maj);
+        double dt = dmin(-log(1.0 - rnd(c)) / maj, dmin(d, md));
+        x = vadd(x, vs(w, dt));
+        t += dt;
+        if (dt >= md) { maj_cross(&mj); if (t >= d - CROSS_EPS) break; continue; }
+        maj_move(&mj, dt);
+        if (t >= d - 1e-7) break;
+        if (rnd(c) < sigma_at(c, x) / maj) { *out_T = 0.0; C = 1.0; break; }
+    }
+    *out_t = tmin + t;
+    return C;
+}
+
+/* Defensive primal pass of vr_collision_sampler_DT.h:52-90: returns C, the
+ * in-volume distance t and the path weight Wc (or the escape weight out_T in
+ * *Wc when C == 0). */
+static void collision_ds_primal(ctx_t* c, v3 x, v3 w, double eps, double tmin, double tmax,
+                                double* C, double* t_out, double* Wc) {
+    double d = tmax - tmin;
+    x = vadd(x, vs(w, tmin));
+    double t = 0, W = 1.0;
+    *C = 0.0;
+    maj_t mj;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(1e-10, maj_query(&mj, &md) + 2 * eps);
+        double dt = dmin(-log(1.0 - rnd(c)) / maj, dmin(d, md));
+        x = vadd(x, vs(w, dt));
+        t += dt;
+        if (dt >= md) { maj_cross(&mj); if (t >= d - CROSS_EPS) break; continue; }
+        maj_move(&mj, dt);
+        if (t >= d - 1e-7) break;
+        double s = sigma_at(c, x);
+        double Pc = (s + eps) / maj;
+        if (rnd(c) < Pc) { *C = 1.0; W *= (s / maj) / Pc; break; }
+        W *= (1 - s / maj) / (1 - Pc);
+    }
+    *t_out = t;
+    *Wc = W;
+}
+
+/* vr_collision_sampler_DT.h:46-134.  VR_BW_REFERENCE scales the collision term
+ * by the collision ratio only (s_t = s/maj/Pc, :86); VR_BW_CONSISTENT uses the
+ * whole path weight (null ratios before the collision included), which makes
+ * the estimator unbiased for eps > 0. */
+static void collision_dt_bw(ctx_t* c, v3 x, v3 w, double eps, double dL_dC, double dL_dT) {
+    double tmin, tmax;
+    if (!boundary(c->M, x, w, &tmin, &tmax)) return;
+    rng_t saved = c->rng;
+    double out_T = 1.0, C = 0.0, s_t = 0.0, Wpre = 1.0;
+    double d = tmax - tmin;
+    x = vadd(x, vs(w, tmin));
+    v3 x0 = x;
+    double t = 0;
+    maj_t mj;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(1e-10, maj_query(&mj, &md) + 2 * eps);
+        double dt = dmin(-log(1.0 - rnd(c)) / maj, dmin(d, md));
+        x = vadd(x, vs(w, dt));
+        t += dt;
+        if (dt >= md) { maj_cross(&mj); if (t >= d - CROSS_EPS) break; continue; }
+        maj_move(&mj, dt);
+        if (t >= d - 1e-7) break;
+        double s = sigma_at(c, x);
+        double Pc = (s + eps) / maj;
+        if (rnd(c) < Pc) { out_T = 0.0; C = 1.0; s_t = s / maj / Pc; break; }
+        double r = (1 - s / maj) / (1 - Pc);
+        out_T *= r;
+        Wpre *= r;
+    }
+    int consistent = c->bw_mode == VR_BW_CONSISTENT;
+    if (consistent) s_t *= Wpre;
+    c->rng = saved;
+    x = x0; t = 0;
+    maj_begin(c->M, &mj, x, w);
+    for (;;) {
+        double md;
+        double maj = dmax(1e-10, maj_query(&mj, &md) + 2 * eps);
+        double dt = dmin(-log(1.0 - rnd(c)) / maj, dmin(d, md));
+        x = vadd(x, vs(w, dt));
+        t += dt;
+        if (dt >= md) { maj_cross(&mj); if (t >= d - CROSS_EPS) break; continue; }
+        maj_move(&mj, dt);
+        if (t >= d - 1e-7) break;
+        double s = sigma_at(c, x);
+        double Pc = (s + eps) / maj;
+        if (rnd(c) < Pc) {
+            double tr = C / Pc;
+            if (consistent) tr *= Wpre;
+            sigma_bw(c, x, dL_dC * tr * (1.0 / maj));
+            break;
+        } else {
+            double q = (dL_dT * out_T + dL_dC * C * s_t) / ((1 - s / maj) / (1 - Pc)) / (1 - Pc);
+            sigma_bw(c, x, q * (-1.0 / maj));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Single scattered radiance: include/maps/single_scattered_radiance_mc.h     */
+/* ------------------------------------------------------------------------- */
+static v3 ss_radiance(ctx_t* c, v3 x, v3 w) {
+    v3 wo;
+    v3 env = env_sample(c, &wo);
+    double ph = phase_eval(c->M, vs(w, -1.0), wo);
+    v3 sa = albedo_at(c, x);
+    double T = transmittance(c, x, wo);
+    return vs(vmul(env, sa), ph * T);
+}
+
+static void ss_radiance_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
+    rng_t saved = c->rng;
+    v3 wo;
+    v3 env = env_sample(c, &wo);
+    double ph = phase_eval(c->M, vs(w, -1.0), wo);
+    v3 sa = albedo_at(c, x);
+    rng_t before_T = c->rng;
+    double T = transmittance(c, x, wo);
+    v3 dL_dsa = vs(vmul(dL_dR, env), ph * T);
+    double dL_dT = vdot(dL_dR, vs(vmul(env, sa), ph));
+    albedo_bw(c, x, dL_dsa);
+    if (c->bw_mode == VR_BW_CONSISTENT) {
+        /* transmittance gradient along the sampled ray wo, replaying its randoms */
+        rng_t after = c->rng;
+        c->rng = before_T;
+        transmittance_bw(c, x, wo, dL_dT);
+        c->rng = after;
+    } else {
+        /* single_scattered_radiance_mc.h:37 differentiates T(x, w) */
+        transmittance_bw(c, x, w, dL_dT);
+    }
+    (void)saved;
+}
+
+/* include/maps/collision_integrator_mc.h:10-44 with exitance = SS radiance.
+ * R = C * SS(x_t, -w) + T * env.  The exitance term is skipped when C == 0: it is
+ * multiplied by 0 and nothing after it in the sample uses the random stream. */
+static v3 mc_collision(ctx_t* c, v3 x, v3 w) {
+    double T, t;
+    double C = collision_dt(c, x, w, &t, &T);
+    v3 xt = vadd(vs(w, t), x);
+    v3 R = vs(vf(c->M->env_c), T);
+    if (C != 0.0) R = vadd(R, vs(ss_radiance(c, xt, vs(w, -1.0)), C));
+    return R;
+}
+
+static void mc_collision_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
+    v3 env = vf(c->M->env_c);
+    if (c->bw_mode == VR_BW_REFERENCE) {
+        /* collision_integrator_mc.h:30-44: forward collision + exitance, then the
+         * collision sampler's own defensive backward with dL_dC = <dL_dR, eR>. */
+        double T, t;
+        double C = collision_dt(c, x, w, &t, &T);
+        v3 xt = vadd(vs(w, t), x);
+        v3 eR = ss_radiance(c, xt, vs(w, -1.0));
+        double dL_dC = vdot(dL_dR, eR);
+        double dL_dT = vdot(dL_dR, env);
+        collision_dt_bw(c, x, w, c->M->ds_epsilon, dL_dC, dL_dT);
+        ss_radiance_bw(c, xt, vs(w, -1.0), vs(dL_dR, C));
+        return;
+    }
+    /* Consistent: the exitance is estimated at the collision the backward's own
+     * defensive primal pass samples, and its pathwise gradient carries that
+     * collision's path weight. */
+    double tmin, tmax;
+    if (!boundary(c->M, x, w, &tmin, &tmax)) return;
+    rng_t saved = c->rng;
+    double C, t, Wc;
+    collision_ds_primal(c, x, w, c->M->ds_epsilon, tmin, tmax, &C, &t, &Wc);
+    v3 xt = vadd(vs(w, tmin + t), x);
+    v3 eR = V(0, 0, 0);
+    if (C != 0.0) eR = ss_radiance(c, xt, vs(w, -1.0));
+    rng_t after = c->rng;
+    double dL_dC = vdot(dL_dR, eR);
+    double dL_dT = vdot(dL_dR, env);
+    c->rng = saved;
+    collision_dt_bw(c, x, w, c->M->ds_epsilon, dL_dC, dL_dT);
+    c->rng = after;
+    if (C != 0.0) ss_radiance_bw(c, xt, vs(w, -1.0), vs(dL_dR, Wc));
+}
+
+/* ------------------------------------------------------------------------- */
+/* NEE path integrator: include/vr_path_integrator_NEE.h:1-66                 */
+/* ------------------------------------------------------------------------- */
+static v3 nee_fw(ctx_t* c, v3 x, v3 w) {
+    const vr_medium* M = c->M;
+    double tmin, tmax;
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    v3 R = V(0, 0, 0), W = V(1, 1, 1);
+    R = vadd(R, vs(vf(M->env_c), transmittance(c, x, w)));
+    const double EPS = 0.00001;
+    int bounces = 0;
+    maj_t mj;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double m = dmax(EPS, maj_query(&mj, &md));
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        if (rnd(c) >= s / m) { d -= t; continue; }
+        v3 sa = albedo_at(c, x);
+        v3 em = vf(M->emission_c);
+        R = vadd(R, vmul(vmul(W, V(1 - sa.x, 1 - sa.y, 1 - sa.z)), em));
+        W = vmul(W, sa);
+        v3 wnee;
+        v3 env = env_sample(c, &wnee);
+        double Tn = transmittance(c, x, wnee);
+        R = vadd(R, vs(vmul(W, env), Tn * phase_eval(M, w, wnee)));
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) return R;
+        v3 wo;
+        double rho = phase_sample(c, w, &wo);
+        W = vs(W, rho);
+        if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+    }
+    return R;
+}
+
+/* include/vr_path_integrator_NEE_DS.h:6-206 (VR_BW_REFERENCE: replay majorant
+ * 1.2 m + 2 eps as in :120; VR_BW_CONSISTENT: the primal majorant in both). */
+static void nee_ds_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
+    const vr_medium* M = c->M;
+    rng_t before_path = c->rng;
+    v3 x0 = x, w0 = w;
+    double tmin, tmax;
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    v3 R = V(0, 0, 0), R_after_sp = V(0, 0, 0), W = V(1, 1, 1);
+    double null_W = 1.0;
+    const double EPS = 0.00000001;
+    int scattered = 0, bounces = 0;
+    maj_t mj;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double m = dmax(EPS, maj_query(&mj, &md) + 2 * eps + 0.1);
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        double Pc = (s + eps) / m;
+        if (rnd(c) >= Pc) { d -= t; W = vs(W, (1 - s / m) / (1 - Pc)); continue; }
+        scattered = 1;
+        if (s < EPS) {
+            if (null_W < EPS) { W = V(0, 0, 0); break; }
+            null_W = 0;
+        } else W = vs(W, (s / m) / Pc);
+        v3 em = vf(M->emission_c);
+        v3 sa = albedo_at(c, x);
+        v3 added = vmul(vmul(W, V(1 - sa.x, 1 - sa.y, 1 - sa.z)), em);
+        W = vmul(W, sa);
+        v3 wnee;
+        v3 env = env_sample(c, &wnee);
+        double Tn = transmittance(c, x, wnee);
+        double phn = phase_eval(M, w, wnee);
+        added = vadd(added, vs(vmul(W, env), Tn * phn));
+        if (null_W < EPS) R_after_sp = vadd(R_after_sp, added);
+        else R = vadd(R, added);
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) { W = V(0, 0, 0); break; }
+        v3 wo;
+        double rho = phase_sample(c, w, &wo);
+        W = vs(W, rho);
+        if (W.x < EPS && W.y < EPS && W.z < EPS) { W = V(0, 0, 0); break; }
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+    }
+    if (!scattered) R = vadd(R, vmul(vf(M->env_c), W));
+
+    /* replay */
+    c->rng = before_path;
+    x = x0; w = w0;
+    W = V(1, 1, 1);
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    d = tmax - tmin;
+    bounces = 0;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double mq = maj_query(&mj, &md);
+        double m = c->bw_mode == VR_BW_REFERENCE ? dmax(EPS, mq * 1.2 + 2 * eps) : dmax(EPS, mq + 2 * eps + 0.1);
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        double Pc = (s + eps) / m;
+        if (rnd(c) >= Pc) {
+            sigma_bw(c, x, vdot(dL_dR, vs(R, -1.0 / m / (1 - s / m))));
+            W = vs(W, (1 - s / m) / (1 - Pc));
+            d -= t;
+            continue;
+        }
+        double gs = s < EPS ? vdot(dL_dR, vs(R_after_sp, 1.0 / eps)) : vdot(dL_dR, vs(R, 1.0 / s));
+        sigma_bw(c, x, gs);
+        W = vs(W, s / m / Pc);
+        if (s < EPS) return;
+        v3 em = vf(M->emission_c);
+        v3 sa = albedo_at(c, x);
+        v3 dL_dsa = vmul(vmul(dL_dR, W), vs(em, -1.0));
+        R = vsub(R, vmul(vmul(W, V(1 - sa.x, 1 - sa.y, 1 - sa.z)), em));
+        dL_dsa = vadd(dL_dsa, vmul(dL_dR, V(R.x / dmax(EPS, sa.x), R.y / dmax(EPS, sa.y), R.z / dmax(EPS, sa.z))));
+        rng_t before_env = c->rng;
+        v3 wnee;
+        v3 env = env_sample(c, &wnee);
+        rng_t before_T = c->rng;
+        double Tn = transmittance(c, x, wnee);
+        double phn = phase_eval(M, w, wnee);
+        rng_t before_ps = c->rng;
+        v3 wo;
+        double rho = phase_sample(c, w, &wo);
+        rng_t before_next = c->rng;
+        W = vmul(W, sa);
+        double dL_dTn = vdot(dL_dR, vs(vmul(W, env), phn));
+        R = vsub(R, vs(vmul(W, env), Tn * phn));
+        albedo_bw(c, x, dL_dsa);
+        c->rng = before_T;
+        transmittance_bw(c, x, wnee, dL_dTn);
+        (void)before_env; (void)before_ps;
+        c->rng = before_next;
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) return;
+        W = vs(W, rho);
+        if (W.x < EPS && W.y < EPS && W.z < EPS) return;
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Medium path integrator (7 -> 12): include/sc_medium_path_integrator_DT_*.h  */
+/* ------------------------------------------------------------------------- */
+static void medium_dt_fw(ctx_t* c, v3 x, v3 w, double d, double* out) {
+    const vr_medium* M = c->M;
+    v3 W = V(1, 1, 1), A = V(0, 0, 0);
+    v3 xo = vadd(vs(w, d), x), wo = w;
+    double maj = dmax(0.00001, M->medium_majorant);
+    for (;;) {
+        double t = -log(1.0 - rnd(c)) / maj;
+        if (t > d - 0.00001) break;
+        x = vadd(x, vs(w, t));
+        d -= t;
+        double s = sigma_at(c, x);
+        if (rnd(c) < s / maj) {
+            if (M->medium_filter & 1) {
+                xo = x;
+                (void)phase_sample(c, w, &wo);
+                W = albedo_at(c, x);
+            } else W = V(0, 0, 0);
+            if (M->medium_filter & 2) A = vf(M->gathering_c);
+            break;
+        }
+    }
+    out[0] = xo.x; out[1] = xo.y; out[2] = xo.z;
+    out[3] = wo.x; out[4] = wo.y; out[5] = wo.z;
+    out[6] = W.x; out[7] = W.y; out[8] = W.z;
+    out[9] = A.x; out[10] = A.y; out[11] = A.z;
+}
+
+static void medium_dt_bw(ctx_t* c, v3 x, v3 w, double d, v3 dL_dW, v3 dL_dA) {
+    const vr_medium* M = c->M;
+    v3 W = V(1, 1, 1), A = V(0, 0, 0);
+    v3 xi = x, wi = w;
+    double di = d;
+    rng_t before = c->rng;
+    double maj = dmax(0.00001, M->medium_majorant);
+    int f = M->medium_filter;
+    for (;;) {
+        double t = -log(1.0 - rnd(c)) / maj;
+        if (t > d - 0.00001) break;
+        x = vadd(x, vs(w, t));
+        d -= t;
+        double s = sigma_at(c, x);
+        if (rnd(c) < s / maj) {
+            if (f & 1) { v3 wo; double wt = phase_sample(c, w, &wo); W = vs(albedo_at(c, x), wt); }
+            else W = V(0, 0, 0);
+            if (f & 2) A = vf(M->gathering_c);
+            break;
+        }
+    }
+    c->rng = before;
+    x = xi; w = wi; d = di;
+    /* `#if MEDIUM_FILTER & 1 == 1` and `& 2 == 2` both parse as `& 1`
+     * (sc_medium_path_integrator_DT_bw.h:85,95,109); VR_BW_CONSISTENT uses the
+     * intended bit tests. */
+    int scat_blk = f & 1;
+    int acc_blk = c->bw_mode == VR_BW_REFERENCE ? (f & 1) : (f & 2);
+    for (;;) {
+        double t = -log(1.0 - rnd(c)) / maj;
+        if (t > d - 0.00001) break;
+        x = vadd(x, vs(w, t));
+        d -= t;
+        double s = sigma_at(c, x);
+        double Pc = s / maj;
+        if (rnd(c) < Pc) {
+            double g = 0;
+            if (scat_blk) {
+                v3 wo;
+                (void)phase_sample(c, w, &wo);
+                g += vdot(dL_dW, vs(W, 1.0 / s));
+                albedo_bw(c, x, dL_dW);
+            }
+            if (acc_blk) g += vdot(dL_dA, vs(A, 1.0 / s));
+            if (g != 0) sigma_bw(c, x, g);
+            break;
+        } else {
+            double g = -vdot(dL_dW, W) / maj / (1 - Pc);
+            if (acc_blk) g += -vdot(dL_dA, A) / maj / (1 - Pc);
+            if (g != 0) sigma_bw(c, x, g);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Map dispatch                                                               */
+/* ------------------------------------------------------------------------- */
+static int map_dims(int kind, int* in_dim, int* out_dim) {
+    switch (kind) {
+    case VR_MAP_RAYBOX: *in_dim = 6; *out_dim = 2; return 1;
+    case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: *in_dim = 6; *out_dim = 1; return 1;
+    case VR_MAP_COLLISION_DT: case VR_MAP_SS_RADIANCE: case VR_MAP_MC_COLLISION: case VR_MAP_NEE_DT:
+        *in_dim = 6; *out_dim = 3; return 1;
+    case VR_MAP_MEDIUM_DT: *in_dim = 7; *out_dim = 12; return 1;
+    default: return 0;
+    }
+}
+
+static void field_fw(ctx_t* c, int kind, const double* in, double* out) {
+    v3 x = V(in[0], in[1], in[2]), w = V(in[3], in[4], in[5]);
+    switch (kind) {
+    case VR_MAP_RAYBOX: ray_box_intersection(x, w, vf(c->M->box_min), vf(c->M->box_max), &out[0], &out[1]); break;
+    case VR_MAP_T_DDA: out[0] = t_dda(c, x, w); break;
+    case VR_MAP_T_DT: out[0] = t_dt(c, x, w); break;
+    case VR_MAP_T_RT: out[0] = t_rt(c, x, w); break;
+    case VR_MAP_COLLISION_DT: { double t, T; out[0] = collision_dt(c, x, w, &t, &T); out[1] = t; out[2] = T; break; }
+    case VR_MAP_SS_RADIANCE: { v3 r = ss_radiance(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
+    case VR_MAP_MC_COLLISION: { v3 r = mc_collision(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
+    case VR_MAP_NEE_DT: { v3 r = nee_fw(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
+    case VR_MAP_MEDIUM_DT: medium_dt_fw(c, x, w, in[6], out); break;
+    default: break;
+    }
+}
+
+static void field_bw(ctx_t* c, int kind, const double* in, const double* og) {
+    v3 x = V(in[0], in[1], in[2]), w = V(in[3], in[4], in[5]);
+    switch (kind) {
+    case VR_MAP_T_DDA: t_dda_bw(c, x, w, og[0]); break;
+    case VR_MAP_T_DT: t_dt_bw(c, x, w, og[0]); break;
+    case VR_MAP_T_RT: t_rt_bw(c, x, w, og[0]); break;
+    case VR_MAP_COLLISION_DT: collision_dt_bw(c, x, w, c->M->ds_epsilon, og[0], og[2]); break;
+    case VR_MAP_SS_RADIANCE: ss_radiance_bw(c, x, w, V(og[0], og[1], og[2])); break;
+    case VR_MAP_MC_COLLISION: mc_collision_bw(c, x, w, V(og[0], og[1], og[2])); break;
+    case VR_MAP_NEE_DT: nee_ds_bw(c, x, w, c->M->ds_epsilon, V(og[0], og[1], og[2])); break;
+    case VR_MAP_MEDIUM_DT: medium_dt_bw(c, x, w, in[6], V(og[6], og[7], og[8]), V(og[9], og[10], og[11])); break;
+    default: break;
+    }
+}
+
+static int check_medium(const vr_medium* M, int kind) {
+    if (!M) return fail(VR_ERR_ARG, "medium is NULL");
+    int id = 0, od = 0;
+    if (!map_dims(kind, &id, &od)) return fail(VR_ERR_UNSUPPORTED, "unknown map kind");
+    if (kind != VR_MAP_RAYBOX) {
+        if (!M->sigma.t.data || M->sigma.t.ndim != 4 || M->sigma.t.shape[3] != 1)
+            return fail(VR_ERR_ARG, "sigma must be a [D,H,W,1] grid");
+        for (int a = 0; a < 3; a++) if (M->sigma.t.shape[a] < 2) return fail(VR_ERR_ARG, "sigma grid needs >= 2 vertices per axis");
+    }
+    if (M->majorant_kind == VR_MAJ_GRID && !M->majorant_grid) return fail(VR_ERR_ARG, "majorant grid is NULL");
+    if (M->albedo_kind == VR_FIELD_GRID && (!M->albedo.t.data || M->albedo.t.ndim != 4))
+        return fail(VR_ERR_ARG, "albedo grid must be [D,H,W,C]");
+    return VR_OK;
+}
+
+/* Private double gradient buffers per thread, reduced into the float grads. */
+typedef struct { int64_t sig_n, alb_n; } gsize_t;
+
+static int64_t grid_span(const vr_grid* g) {
+    const vr_tensor* t = &g->t;
+    int64_t span = 1;
+    for (int i = 0; i < 4; i++) span += (t->shape[i] - 1) * t->stride[i];
+    return span;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Exported entry points                                                      */
+/* ------------------------------------------------------------------------- */
+int oracle_grid_fw(const vr_grid* g, const float* x, int64_t n, float* out) {
+    if (!g || !x || !out || g->t.ndim != 4) return fail(VR_ERR_ARG, "bad grid arguments");
+    int C = (int)g->t.shape[3];
+    for (int64_t i = 0; i < n; i++) {
+        double v[64];
+        grid_lookup(g, vf(x + 3 * i), v);
+        for (int k = 0; k < C; k++) out[i * C + k] = (float)v[k];
+    }
+    return VR_OK;
+}
+
+int oracle_grid_fw_f64(const vr_grid* g, const float* x, int64_t n, double* out) {
+    if (!g || !x || !out || g->t.ndim != 4) return fail(VR_ERR_ARG, "bad grid arguments");
+    int C = (int)g->t.shape[3];
+    for (int64_t i = 0; i < n; i++) {
+        double v[64];
+        grid_lookup(g, vf(x + 3 * i), v);
+        for (int k = 0; k < C; k++) out[i * C + k] = v[k];
+    }
+    return VR_OK;
+}
+
+int oracle_grid_bw(const vr_grid* g, const float* x, const float* og, int64_t n) {
+    if (!g || !x || !og || g->t.ndim != 4) return fail(VR_ERR_ARG, "bad grid arguments");
+    if (!g->t.grad) return VR_OK;
+    int C = (int)g->t.shape[3];
+    for (int64_t i = 0; i < n; i++) {
+        double o[64];
+        for (int k = 0; k < C; k++) o[k] = og[i * C + k];
+        grid_scatter(g, NULL, vf(x + 3 * i), o);
+    }
+    return VR_OK;
+}
+
+int oracle_majorant_build(const vr_grid* g, const int32_t res[3], float* out) {
+    if (!g || !res || !out) return fail(VR_ERR_ARG, "bad majorant arguments");
+    const vr_tensor* t = &g->t;
+    int rz = res[0], ry = res[1], rx = res[2];
+    int n[3] = {(int)t->shape[2] - 1, (int)t->shape[1] - 1, (int)t->shape[0] - 1};  /* cells x,y,z */
+    int r[3] = {rx, ry, rz};
+    for (int cz = 0; cz < rz; cz++)
+        for (int cy = 0; cy < ry; cy++)
+            for (int cx = 0; cx < rx; cx++) {
+                int c[3] = {cx, cy, cz}, lo[3], hi[3];
+                for (int a = 0; a < 3; a++) {
+                    /* vertices of the fine cells overlapping [c, c+1) * n/r, widened by one
+                     * vertex on each side to absorb rounding of the coarse boundaries */
+                    lo[a] = imax(0, (int)floor((double)c[a] * n[a] / r[a]) - 1);
+                    hi[a] = imin(n[a], (int)ceil((double)(c[a] + 1) * n[a] / r[a]) + 1);
+                }
+                float m = 0.0f;
+                for (int z = lo[2]; z <= hi[2]; z++)
+                    for (int y = lo[1]; y <= hi[1]; y++)
+                        for (int x = lo[0]; x <= hi[0]; x++) {
+                            float v = t->data[goff(t, z, y, x)];
+                            if (v > m) m = v;
+                        }
+                out[((int64_t)cz * ry + cy) * rx + cx] = m;
+            }
+    return VR_OK;
+}
+
+static void reduce_priv(const vr_grid* g, double* priv) {
+    if (!priv || !g->t.grad) return;
+    int64_t span = grid_span(g);
+    for (int64_t i = 0; i < span; i++) if (priv[i] != 0) g->t.grad[i] += (float)priv[i];
+}
+
+int oracle_map_fw(const vr_medium* M, int32_t kind, const float* in, int64_t n, float* out, const vr_launch* L) {
+    int st = check_medium(M, kind);
+    if (st) return st;
+    if (!in || !out || !L) return fail(VR_ERR_ARG, "null buffer");
+    int id = 0, od = 0;
+    map_dims(kind, &id, &od);
+    int64_t start = L->start_index, cnt = L->count < 0 ? n - start : L->count;
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = start; i < start + cnt; i++) {
+        ctx_t c;
+        memset(&c, 0, sizeof(c));
+        c.M = M; c.bw_mode = L->bw_mode;
+        c.rng = seed_map(L->seeds, (uint32_t)i);
+        double a[16], o[16];
+        for (int k = 0; k < id; k++) a[k] = in[i * id + k];
+        field_fw(&c, kind, a, o);
+        for (int k = 0; k < od; k++) out[i * od + k] = (float)o[k];
+        if (L->debug_out) L->debug_out[i] = (int32_t)c.lookups;
+    }
+    return VR_OK;
+}
+
+int oracle_map_bw(const vr_medium* M, int32_t kind, const float* in, const float* og, int64_t n, const vr_launch* L) {
+    int st = check_medium(M, kind);
+    if (st) return st;
+    if (!in || !og || !L) return fail(VR_ERR_ARG, "null buffer");
+    int id = 0, od = 0;
+    map_dims(kind, &id, &od);
+    int64_t start = L->start_index, cnt = L->count < 0 ? n - start : L->count;
+    int64_t ssig = grid_span(&M->sigma);
+    int64_t salb = M->albedo_kind == VR_FIELD_GRID ? grid_span(&M->albedo) : 0;
+    #pragma omp parallel
+    {
+        gacc_t acc;
+        acc.sig = M->sigma.t.grad ? (double*)calloc((size_t)ssig, sizeof(double)) : NULL;
+        acc.alb = (salb && M->albedo.t.grad) ? (double*)calloc((size_t)salb, sizeof(double)) : NULL;
+        #pragma omp for schedule(dynamic, 16)
+        for (int64_t i = start; i < start + cnt; i++) {
+            ctx_t c;
+            memset(&c, 0, sizeof(c));
+            c.M = M; c.bw_mode = L->bw_mode; c.acc = acc;
+            c.rng = seed_map(L->seeds, (uint32_t)i);
+            double a[16], o[16];
+            for (int k = 0; k < id; k++) a[k] = in[i * id + k];
+            for (int k = 0; k < od; k++) o[k] = og[i * od + k];
+            field_bw(&c, kind, a, o);
+        }
+        #pragma omp critical
+        {
+            reduce_priv(&M->sigma, acc.sig);
+            if (acc.alb) reduce_priv(&M->albedo, acc.alb);
+        }
+        free(acc.sig);
+        free(acc.alb);
+    }
+    return VR_OK;
+}
+
+/* include/maps/sensor_perspective_camera.h:1-27 */
+static void camera_ray(const vr_camera* cam, int pose, int row, int col, ctx_t* c, v3* xo, v3* wo) {
+    const float* P = cam->poses + 9 * pose;
+    v3 o = vf(P), d = vf(P + 3), n = vf(P + 6);
+    double u = 0.5, v = 0.5;
+    if (cam->jittered) { u = rnd(c); v = rnd(c); }
+    double sx = ((col + u) * 2 - cam->width) * cam->znear / cam->height;
+    double sy = ((row + v) * 2 - cam->height) * cam->znear / cam->height;
+    double sz = cam->znear / tan(cam->fov * 0.5);
+    v3 za = vnorm(d);
+    v3 xa = vnorm(vcross(n, za));
+    v3 ya = vcross(za, xa);
+    v3 w = vadd(vadd(vs(xa, sx), vs(ya, sy)), vs(za, sz));
+    *xo = vadd(o, w);
+    *wo = vnorm(w);
+}
+
+static void sensor_index(const vr_camera* cam, int64_t i, int* pose, int* row, int* col) {
+    if (cam->sensors) {
+        *pose = (int)cam->sensors[3 * i]; *row = (int)cam->sensors[3 * i + 1]; *col = (int)cam->sensors[3 * i + 2];
+    } else {
+        *col = (int)(i % cam->width);
+        *row = (int)((i / cam->width) % cam->height);
+        *pose = (int)(i / ((int64_t)cam->width * cam->height));
+    }
+}
+
+static int check_camera(const vr_camera* cam) {
+    if (!cam || !cam->poses || cam->width <= 0 || cam->height <= 0 || cam->num_poses <= 0)
+        return fail(VR_ERR_ARG, "bad camera");
+    return VR_OK;
+}
+
+static int64_t camera_threads(const vr_camera* cam) {
+    return cam->sensors ? cam->num_sensors : (int64_t)cam->num_poses * cam->width * cam->height;
+}
+
+int oracle_capture_fw(const vr_medium* M, int32_t kind, const vr_camera* cam, const vr_launch* L, float* out) {
+    int st = check_medium(M, kind);
+    if (st) return st;
+    if ((st = check_camera(cam))) return st;
+    if (!L || !out) return fail(VR_ERR_ARG, "null buffer");
+    int id = 0, od = 0;
+    map_dims(kind, &id, &od);
+    if (id != 6) return fail(VR_ERR_UNSUPPORTED, "capture needs a 6-input map");
+    int64_t total = camera_threads(cam);
+    int64_t start = L->start_index, cnt = L->count < 0 ? total - start : L->count;
+    int spp = L->samples < 1 ? 1 : L->samples;
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = start; i < start + cnt; i++) {
+        int pose, row, col;
+        sensor_index(cam, i, &pose, &row, &col);
+        double acc[16] = {0};
+        int64_t lk = 0;
+        for (int s = 0; s < spp; s++) {
+            ctx_t c;
+            memset(&c, 0, sizeof(c));
+            c.M = M; c.bw_mode = L->bw_mode;
+            c.rng = seed_capture_fw(L->seeds, (uint32_t)((int64_t)s * total + i));
+            v3 x, w;
+            camera_ray(cam, pose, row, col, &c, &x, &w);
+            double a[6] = {x.x, x.y, x.z, w.x, w.y, w.z}, o[16];
+            field_fw(&c, kind, a, o);
+            for (int k = 0; k < od; k++) acc[k] += o[k];
+            lk += c.lookups;
+        }
+        for (int k = 0; k < od; k++) out[i * od + k] = (float)(acc[k] / spp);
+        if (L->debug_out) L->debug_out[i] = (int32_t)lk;
+    }
+    return VR_OK;
+}
+
+int oracle_capture_bw(const vr_medium* M, int32_t kind, const vr_camera* cam, const vr_launch* L, const float* og) {
+    int st = check_medium(M, kind);
+    if (st) return st;
+    if ((st = check_camera(cam))) return st;
+    if (!L || !og) return fail(VR_ERR_ARG, "null buffer");
+    int id = 0, od = 0;
+    map_dims(kind, &id, &od);
+    if (id != 6) return fail(VR_ERR_UNSUPPORTED, "capture needs a 6-input map");
+    int64_t total = camera_threads(cam);
+    int64_t start = L->start_index, cnt = L->count < 0 ? total - start : L->count;
+    int spp = L->samples < 1 ? 1 : L->samples;
+    int64_t ssig = grid_span(&M->sigma);
+    int64_t salb = M->albedo_kind == VR_FIELD_GRID ? grid_span(&M->albedo) : 0;
+    #pragma omp parallel
+    {
+        gacc_t acc;
+        acc.sig = M->sigma.t.grad ? (double*)calloc((size_t)ssig, sizeof(double)) : NULL;
+        acc.alb = (salb && M->albedo.t.grad) ? (double*)calloc((size_t)salb, sizeof(double)) : NULL;
+        #pragma omp for schedule(dynamic, 4)
+        for (int64_t i = start; i < start + cnt; i++) {
+            int pose, row, col;
+            sensor_index(cam, i, &pose, &row, &col);
+            double o[16];
+            for (int k = 0; k < od; k++) o[k] = og[i * od + k] / spp;
+            ctx_t c;
+            memset(&c, 0, sizeof(c));
+            c.M = M; c.bw_mode = L->bw_mode; c.acc = acc;
+            if (L->seed_mode == VR_SEED_CAPTURE_BW) c.rng = seed_capture_bw(L->seeds, (uint32_t)i);
+            for (int s = 0; s < spp; s++) {
+                if (L->seed_mode != VR_SEED_CAPTURE_BW)
+                    c.rng = seed_capture_fw(L->seeds, (uint32_t)((int64_t)s * total + i));
+                v3 x, w;
+                camera_ray(cam, pose, row, col, &c, &x, &w);
+                double a[6] = {x.x, x.y, x.z, w.x, w.y, w.z};
+                field_bw(&c, kind, a, o);
+            }
+        }
+        #pragma omp critical
+        {
+            reduce_priv(&M->sigma, acc.sig);
+            if (acc.alb) reduce_priv(&M->albedo, acc.alb);
+        }
+        free(acc.sig);
+        free(acc.alb);
+    }
+    return VR_OK;
+}
+
+int oracle_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                     float eps, int32_t step, int32_t zero_grad) {
+    if (step < 1) return fail(VR_ERR_ARG, "step must be >= 1");
+    double bc1 = 1.0 - pow((double)b1, step), bc2 = 1.0 - pow((double)b2, step);
+    for (int64_t i = 0; i < n; i++) {
+        double gi = g[i];
+        double mi = b1 * (double)m[i] + (1.0 - b1) * gi;
+        double vi = b2 * (double)v[i] + (1.0 - b2) * gi * gi;
+        m[i] = (float)mi; v[i] = (float)vi;
+        double denom = sqrt(vi / bc2) + eps;
+        p[i] = (float)((double)p[i] - lr / bc1 * mi / denom);
+        if (zero_grad) g[i] = 0.0f;
+    }
+    return VR_OK;
+}
+
+/* Raw RNG access for known-answer tests: n uniforms from a state. */
+int oracle_rng_uniforms(const uint32_t state[4], int64_t n, float* out, uint32_t* final_state) {
+    rng_t s = {state[0], state[1], state[2], state[3]};
+    for (int64_t i = 0; i < n; i++) out[i] = rng_random(&s);
+    if (final_state) { final_state[0] = s.x; final_state[1] = s.y; final_state[2] = s.z; final_state[3] = s.w; }
+    return VR_OK;
+}
+
+/* Per-thread initial states for known-answer tests. */
+int oracle_rng_seed_states(int32_t mode, const uint32_t seeds[4], int64_t first, int64_t n, uint32_t* out) {
+    for (int64_t k = 0; k < n; k++) {
+        uint32_t i = (uint32_t)(first + k);
+        rng_t s = mode == VR_SEED_MAP ? seed_map(seeds, i) : (mode == VR_SEED_CAPTURE_BW ? seed_capture_bw(seeds, i) : seed_capture_fw(seeds, i));
+        out[4 * k] = s.x; out[4 * k + 1] = s.y; out[4 * k + 2] = s.z; out[4 * k + 3] = s.w;
+    }
+    return VR_OK;
+}

@@ -1,0 +1,72 @@
+"""Seeds, device and run-time settings of the map engine.
+
+Mirrors rendervous _internal.py:21-34: ``seed(n)`` arms torch's generator and
+``get_seeds()`` draws the 4 dispatch seeds in [129, 2^30) used by every kernel.
+"""
+from __future__ import annotations
+
+import contextlib
+import dataclasses
+import typing
+
+import torch
+
+__MANUAL_SEED__: typing.Optional[int] = None
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("rendervous_amd needs a HIP device (torch.cuda on ROCm); no CPU fallback exists")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def seed(manual_seed: typing.Optional[int] = None) -> None:
+    """Arm the seed generator (_internal.py:21-23)."""
+    global __MANUAL_SEED__
+    __MANUAL_SEED__ = manual_seed
+
+
+def get_seeds() -> typing.Tuple[int, int, int, int]:
+    """Four dispatch seeds in [129, 2^30) drawn from torch's CPU generator (_internal.py:26-34)."""
+    global __MANUAL_SEED__
+    if __MANUAL_SEED__ is not None:
+        torch.manual_seed(__MANUAL_SEED__)
+        __MANUAL_SEED__ = None
+    s = torch.randint(low=129, high=1 << 30, size=(4,), dtype=torch.int32)
+    return tuple(int(v) for v in s.tolist())
+
+
+@dataclasses.dataclass
+class BackwardSettings:
+    """How capture/map backward passes draw randoms and which estimators they use.
+
+    seeds:     'replay' reuses the forward dispatch seeds (each sample's stream is
+               re-derived from its flat index, so the backward replays the forward's
+               randoms); 'fresh' draws new seeds with the capture-backward
+               per-thread init (_maps.py:864, :676-685), as the reference does.
+    estimator: 'consistent' fixes the reference backward defects (matched
+               primal/replay majorants, single-scatter transmittance gradient
+               along the sampled direction, full path weights in the collision
+               sampler); 'reference' reproduces the reference formulas.
+    """
+    seeds: str = "replay"
+    estimator: str = "consistent"
+
+
+backward_settings = BackwardSettings()
+
+
+@contextlib.contextmanager
+def backward_mode(seeds: typing.Optional[str] = None, estimator: typing.Optional[str] = None):
+    old = dataclasses.replace(backward_settings)
+    if seeds is not None:
+        assert seeds in ("replay", "fresh")
+        backward_settings.seeds = seeds
+    if estimator is not None:
+        assert estimator in ("consistent", "reference")
+        backward_settings.estimator = estimator
+    try:
+        yield
+    finally:
+        backward_settings.seeds = old.seeds
+        backward_settings.estimator = old.estimator

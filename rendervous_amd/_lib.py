@@ -1,0 +1,232 @@
+"""ctypes binding of the C ABI declared in include/vr_api.h.
+
+The structures below mirror the header field by field.  The shared library is
+loaded from the package directory (built in-tree by ``rendervous_amd.build``);
+there is no fallback: every entry point raises if the library is missing or a
+call returns a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librendervous_amd.so")
+
+# enums (include/vr_api.h)
+VR_OK, VR_ERR_ARG, VR_ERR_UNSUPPORTED, VR_ERR_DEVICE = 0, 1, 2, 3
+VR_FIELD_CONST, VR_FIELD_GRID = 0, 1
+VR_PHASE_HG_MIX, VR_PHASE_HG, VR_PHASE_ISO = 0, 1, 2
+VR_MAJ_CONST, VR_MAJ_GRID = 0, 1
+VR_T_DT, VR_T_RT, VR_T_DDA = 0, 1, 2
+VR_MAP_RAYBOX = 1
+VR_MAP_T_DDA = 2
+VR_MAP_T_DT = 3
+VR_MAP_T_RT = 4
+VR_MAP_COLLISION_DT = 5
+VR_MAP_SS_RADIANCE = 6
+VR_MAP_MC_COLLISION = 7
+VR_MAP_NEE_DT = 8
+VR_MAP_MEDIUM_DT = 9
+VR_SEED_MAP, VR_SEED_CAPTURE_FW, VR_SEED_CAPTURE_BW = 0, 1, 2
+VR_BW_REFERENCE, VR_BW_CONSISTENT = 0, 1
+
+MAP_DIMS = {
+    VR_MAP_RAYBOX: (6, 2),
+    VR_MAP_T_DDA: (6, 1),
+    VR_MAP_T_DT: (6, 1),
+    VR_MAP_T_RT: (6, 1),
+    VR_MAP_COLLISION_DT: (6, 3),
+    VR_MAP_SS_RADIANCE: (6, 3),
+    VR_MAP_MC_COLLISION: (6, 3),
+    VR_MAP_NEE_DT: (6, 3),
+    VR_MAP_MEDIUM_DT: (7, 12),
+}
+
+F3 = ctypes.c_float * 3
+
+
+class VrTensor(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("grad", ctypes.c_void_p),
+        ("shape", ctypes.c_int64 * 4),
+        ("stride", ctypes.c_int64 * 4),
+        ("ndim", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class VrGrid(ctypes.Structure):
+    _fields_ = [("t", VrTensor), ("bmin", F3), ("bmax", F3)]
+
+
+class VrMedium(ctypes.Structure):
+    _fields_ = [
+        ("sigma", VrGrid),
+        ("box_min", F3),
+        ("box_max", F3),
+        ("albedo_kind", ctypes.c_int32),
+        ("albedo_c", F3),
+        ("albedo", VrGrid),
+        ("emission_c", F3),
+        ("env_c", F3),
+        ("sun_dir", F3),
+        ("sun_radiance", F3),
+        ("sun_prob", ctypes.c_float),
+        ("phase_kind", ctypes.c_int32),
+        ("phase_g", ctypes.c_float),
+        ("majorant_kind", ctypes.c_int32),
+        ("majorant_value", ctypes.c_float),
+        ("majorant_distance", ctypes.c_float),
+        ("majorant_grid", ctypes.c_void_p),
+        ("majorant_res", ctypes.c_int32 * 3),
+        ("transmittance_kind", ctypes.c_int32),
+        ("ds_epsilon", ctypes.c_float),
+        ("max_bounces", ctypes.c_int32),
+        ("medium_filter", ctypes.c_int32),
+        ("gathering_c", F3),
+        ("medium_majorant", ctypes.c_float),
+    ]
+
+
+class VrCamera(ctypes.Structure):
+    _fields_ = [
+        ("poses", ctypes.c_void_p),
+        ("num_poses", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("jittered", ctypes.c_int32),
+        ("fov", ctypes.c_float),
+        ("znear", ctypes.c_float),
+        ("sensors", ctypes.c_void_p),
+        ("num_sensors", ctypes.c_int64),
+    ]
+
+
+class VrLaunch(ctypes.Structure):
+    _fields_ = [
+        ("seeds", ctypes.c_uint32 * 4),
+        ("samples", ctypes.c_int32),
+        ("bw_mode", ctypes.c_int32),
+        ("seed_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("start_index", ctypes.c_int64),
+        ("count", ctypes.c_int64),
+        ("debug_out", ctypes.c_void_p),
+    ]
+
+
+P = ctypes.POINTER
+_SIGNATURES = {
+    "vr_abi_version": (ctypes.c_int, []),
+    "vr_last_error": (ctypes.c_char_p, []),
+    "vr_grid_fw": (ctypes.c_int, [P(VrGrid), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_grid_bw": (ctypes.c_int, [P(VrGrid), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "vr_majorant_build": (ctypes.c_int, [P(VrGrid), P(ctypes.c_int32), ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_map_fw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, P(VrLaunch), ctypes.c_void_p]),
+    "vr_map_bw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, P(VrLaunch), ctypes.c_void_p]),
+    "vr_capture_fw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, P(VrCamera), P(VrLaunch), ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_capture_bw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, P(VrCamera), P(VrLaunch), ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_void_p]),
+}
+
+# Symbols declared in include/vr_api.h (checked by the CPU test suite).
+EXPORTED = tuple(_SIGNATURES)
+
+_lock = threading.Lock()
+_lib = None
+
+
+class VrError(RuntimeError):
+    """A non-zero status returned by the native library."""
+
+
+def bind(lib: ctypes.CDLL, prefix_from: str = "vr_", prefix_to: str = "vr_") -> ctypes.CDLL:
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, prefix_to + name[len(prefix_from):])
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (built by rendervous_amd.build); raises if absent."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with `python -m rendervous_amd.build` "
+                    "(there is no CPU fallback for the HIP path)")
+            lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+            bind(lib)
+            if lib.vr_abi_version() != 1:
+                raise ImportError("librendervous_amd.so ABI version mismatch")
+            _lib = lib
+        return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != VR_OK:
+        msg = load().vr_last_error()
+        raise VrError(f"{what} failed with status {status}: {msg.decode() if msg else ''}")
+
+
+def stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def f3(v) -> "ctypes.Array":
+    a = [float(x) for x in v]
+    assert len(a) == 3
+    return F3(*a)
+
+
+def tensor_view(t: torch.Tensor, grad: torch.Tensor | None = None) -> VrTensor:
+    """A vr_tensor over a float32 tensor of up to 4 dims (Parameter, common.h:88-94)."""
+    if t.dtype != torch.float32:
+        raise TypeError("vr tensors must be float32")
+    if t.dim() > 4:
+        raise ValueError("vr tensors have at most 4 dims")
+    v = VrTensor()
+    v.data = t.data_ptr()
+    v.grad = grad.data_ptr() if grad is not None else None
+    v.ndim = t.dim()
+    for i in range(4):
+        v.shape[i] = t.shape[i] if i < t.dim() else 1
+        v.stride[i] = t.stride(i) if i < t.dim() else 1
+    if grad is not None and (grad.shape != t.shape or grad.stride() != t.stride()):
+        raise ValueError("gradient buffer must share the parameter's shape and strides")
+    return v
+
+
+def grid_view(t: torch.Tensor, bmin, bmax, grad: torch.Tensor | None = None) -> VrGrid:
+    g = VrGrid()
+    g.t = tensor_view(t, grad)
+    g.bmin = f3(bmin)
+    g.bmax = f3(bmax)
+    return g
+
+
+def launch(seeds, samples: int = 1, bw_mode: int = VR_BW_REFERENCE, seed_mode: int = VR_SEED_CAPTURE_FW,
+           start_index: int = 0, count: int = -1, debug_out: torch.Tensor | None = None) -> VrLaunch:
+    L = VrLaunch()
+    for i in range(4):
+        L.seeds[i] = int(seeds[i]) & 0xFFFFFFFF
+    L.samples = int(samples)
+    L.bw_mode = int(bw_mode)
+    L.seed_mode = int(seed_mode)
+    L.start_index = int(start_index)
+    L.count = int(count)
+    L.debug_out = None if debug_out is None else debug_out.data_ptr()
+    return L

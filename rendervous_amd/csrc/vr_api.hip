@@ -1,0 +1,582 @@
+// vr_api.hip -- kernels and C ABI (include/vr_api.h) of the MI355X
+// participating-media library.
+//
+// Launch model: one work-item per sensor (pixel) or per input row, wave64,
+// 256-thread workgroups (the reference dispatches 32-wide workgroups,
+// _maps.py:523, which leaves half of each CDNA wave empty).  Each capture
+// work-item loops over its samples and re-seeds the HybridTaus state of sample
+// s from the flat index s * threads + i, so sample 0 draws exactly the
+// reference's per-thread stream (_maps.py:549-560) and every sample can be
+// replayed on its own by the backward kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "vr_device.hpp"
+
+using namespace vr;
+
+namespace {
+
+thread_local char g_err[512];
+
+int set_err(int code, const char* fmt, const char* a = "") {
+    std::snprintf(g_err, sizeof(g_err), fmt, a);
+    return code;
+}
+
+int check_hip(hipError_t e, const char* where) {
+    if (e == hipSuccess) return VR_OK;
+    std::snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+    return VR_ERR_DEVICE;
+}
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------ host views --
+int make_grid_view(const vr_grid* g, GridView& v, int want_c, const char* what) {
+    if (!g) return set_err(VR_ERR_ARG, "%s: grid is NULL", what);
+    const vr_tensor& t = g->t;
+    if (!t.data) return set_err(VR_ERR_ARG, "%s: grid data is NULL", what);
+    if (t.ndim != 4) return set_err(VR_ERR_ARG, "%s: grid must be 4-D [D,H,W,C]", what);
+    if (t.shape[0] < 2 || t.shape[1] < 2 || t.shape[2] < 2)
+        return set_err(VR_ERR_ARG, "%s: grid needs at least 2 vertices per axis", what);
+    if (t.shape[3] < 1 || t.shape[3] > 4) return set_err(VR_ERR_ARG, "%s: grid channels must be 1..4", what);
+    if (want_c > 0 && t.shape[3] != want_c) return set_err(VR_ERR_ARG, "%s: wrong channel count", what);
+    int64_t span = 1;
+    for (int i = 0; i < 4; i++) {
+        if (t.stride[i] < 0) return set_err(VR_ERR_ARG, "%s: negative strides are not supported", what);
+        span += (t.shape[i] - 1) * t.stride[i];
+    }
+    if (span >= (int64_t(1) << 31)) return set_err(VR_ERR_ARG, "%s: grid spans more than 2^31 elements", what);
+    v.data = t.data;
+    v.grad = t.grad;
+    v.sz = (int)t.stride[0]; v.sy = (int)t.stride[1]; v.sx = (int)t.stride[2]; v.sc = (int)t.stride[3];
+    v.nz = (int)t.shape[0]; v.ny = (int)t.shape[1]; v.nx = (int)t.shape[2]; v.C = (int)t.shape[3];
+    double n[3] = {(double)v.nx - 1, (double)v.ny - 1, (double)v.nz - 1};
+    float sc[3], of[3];
+    for (int a = 0; a < 3; a++) {
+        double ext = (double)g->bmax[a] - (double)g->bmin[a];
+        if (!(ext > 0)) return set_err(VR_ERR_ARG, "%s: bmax must exceed bmin", what);
+        double s = n[a] / ext;
+        sc[a] = (float)s;
+        of[a] = (float)(-(double)g->bmin[a] * s);
+    }
+    v.scale = F3(sc[0], sc[1], sc[2]);
+    v.offset = F3(of[0], of[1], of[2]);
+    v.dimf = F3((float)n[0], (float)n[1], (float)n[2]);
+    v.bmin = F3(g->bmin[0], g->bmin[1], g->bmin[2]);
+    v.bmax = F3(g->bmax[0], g->bmax[1], g->bmax[2]);
+    return VR_OK;
+}
+
+int map_dims(int kind, int& in_dim, int& out_dim) {
+    switch (kind) {
+    case VR_MAP_RAYBOX: in_dim = 6; out_dim = 2; return 1;
+    case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: in_dim = 6; out_dim = 1; return 1;
+    case VR_MAP_COLLISION_DT: case VR_MAP_SS_RADIANCE: case VR_MAP_MC_COLLISION: case VR_MAP_NEE_DT:
+        in_dim = 6; out_dim = 3; return 1;
+    case VR_MAP_MEDIUM_DT: in_dim = 7; out_dim = 12; return 1;
+    default: return 0;
+    }
+}
+
+int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
+    if (!m) return set_err(VR_ERR_ARG, "medium is NULL");
+    std::memset(&v, 0, sizeof(v));
+    int st;
+    if (kind != VR_MAP_RAYBOX) {
+        if ((st = make_grid_view(&m->sigma, v.sigma, 1, "sigma"))) return st;
+    }
+    v.albedo_grid = m->albedo_kind == VR_FIELD_GRID;
+    if (v.albedo_grid) {
+        if ((st = make_grid_view(&m->albedo, v.albedo, 0, "albedo"))) return st;
+        if (v.albedo.C != 1 && v.albedo.C != 3) return set_err(VR_ERR_ARG, "albedo grid needs 1 or 3 channels");
+    } else if (m->albedo_kind != VR_FIELD_CONST) return set_err(VR_ERR_ARG, "unknown albedo kind");
+    v.albedo_c = load3(m->albedo_c);
+    v.box_min = load3(m->box_min);
+    v.box_max = load3(m->box_max);
+    v.emission_c = load3(m->emission_c);
+    v.env_c = load3(m->env_c);
+    v.sun_dir = load3(m->sun_dir);
+    v.sun_radiance = load3(m->sun_radiance);
+    v.sun_prob = m->sun_prob;
+    if (m->sun_prob < 0 || m->sun_prob >= 1) return set_err(VR_ERR_ARG, "sun_prob must be in [0,1)");
+    if (m->phase_kind < VR_PHASE_HG_MIX || m->phase_kind > VR_PHASE_ISO) return set_err(VR_ERR_ARG, "unknown phase kind");
+    v.phase_kind = m->phase_kind;
+    v.g = m->phase_g;
+    v.maj_value = m->majorant_value;
+    v.maj_distance = m->majorant_distance;
+    if (m->majorant_kind == VR_MAJ_GRID) {
+        if (!m->majorant_grid) return set_err(VR_ERR_ARG, "majorant grid is NULL");
+        if (m->majorant_res[0] < 1 || m->majorant_res[1] < 1 || m->majorant_res[2] < 1)
+            return set_err(VR_ERR_ARG, "majorant resolution must be >= 1");
+        for (int a = 0; a < 3; a++)
+            if (m->box_min[a] < m->sigma.bmin[a] - 1e-6f || m->box_max[a] > m->sigma.bmax[a] + 1e-6f)
+                return set_err(VR_ERR_UNSUPPORTED, "majorant grid requires the boundary box inside the sigma grid box");
+        v.maj_grid = m->majorant_grid;
+        v.mrz = m->majorant_res[0]; v.mry = m->majorant_res[1]; v.mrx = m->majorant_res[2];
+        v.maj_bmin = load3(m->sigma.bmin);
+        v.maj_csize = F3((m->sigma.bmax[0] - m->sigma.bmin[0]) / v.mrx, (m->sigma.bmax[1] - m->sigma.bmin[1]) / v.mry,
+                         (m->sigma.bmax[2] - m->sigma.bmin[2]) / v.mrz);
+    } else if (m->majorant_kind != VR_MAJ_CONST) return set_err(VR_ERR_ARG, "unknown majorant kind");
+    if (m->transmittance_kind < VR_T_DT || m->transmittance_kind > VR_T_DDA) return set_err(VR_ERR_ARG, "unknown transmittance kind");
+    v.t_kind = m->transmittance_kind;
+    v.ds_eps = m->ds_epsilon;
+    v.max_bounces = m->max_bounces;
+    v.medium_filter = m->medium_filter;
+    v.gathering_c = load3(m->gathering_c);
+    v.medium_majorant = m->medium_majorant;
+    if (bw_mode != VR_BW_REFERENCE && bw_mode != VR_BW_CONSISTENT) return set_err(VR_ERR_ARG, "unknown bw mode");
+    v.bw_mode = bw_mode;
+    return VR_OK;
+}
+
+int make_camera_view(const vr_camera* c, CameraView& v) {
+    if (!c || !c->poses) return set_err(VR_ERR_ARG, "camera or poses is NULL");
+    if (c->width <= 0 || c->height <= 0 || c->num_poses <= 0) return set_err(VR_ERR_ARG, "camera sizes must be positive");
+    if (c->sensors && c->num_sensors < 0) return set_err(VR_ERR_ARG, "negative sensor count");
+    v.poses = c->poses;
+    v.sensors = c->sensors;
+    v.num_poses = c->num_poses;
+    v.width = c->width;
+    v.height = c->height;
+    v.jittered = c->jittered;
+    v.fov = c->fov;
+    v.znear = c->znear;
+    return VR_OK;
+}
+
+Seeds make_seeds(const vr_launch* L) {
+    Seeds s;
+    for (int i = 0; i < 4; i++) s.s[i] = L->seeds[i];
+    return s;
+}
+
+// --------------------------------------------------------- field dispatch --
+template <int KIND, int MAJ>
+__device__ __forceinline__ void field_fw(Tracer<MAJ>& tr, const float* in, float* out) {
+    f3 x = F3(in[0], in[1], in[2]), w = F3(in[3], in[4], in[5]);
+    if constexpr (KIND == VR_MAP_RAYBOX) {
+        ray_box_intersection(x, w, tr.M.box_min, tr.M.box_max, out[0], out[1]);
+    } else if constexpr (KIND == VR_MAP_T_DDA) {
+        out[0] = tr.t_dda(x, w);
+    } else if constexpr (KIND == VR_MAP_T_DT) {
+        out[0] = tr.t_dt(x, w);
+    } else if constexpr (KIND == VR_MAP_T_RT) {
+        out[0] = tr.t_rt(x, w);
+    } else if constexpr (KIND == VR_MAP_COLLISION_DT) {
+        float t, T;
+        out[0] = tr.collision_dt(x, w, t, T);
+        out[1] = t;
+        out[2] = T;
+    } else if constexpr (KIND == VR_MAP_SS_RADIANCE) {
+        f3 r = tr.ss_radiance(x, w);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    } else if constexpr (KIND == VR_MAP_MC_COLLISION) {
+        f3 r = tr.mc_collision(x, w);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    } else if constexpr (KIND == VR_MAP_NEE_DT) {
+        f3 r = tr.nee(x, w);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    } else if constexpr (KIND == VR_MAP_MEDIUM_DT) {
+        tr.medium_dt(x, w, in[6], out);
+    }
+}
+
+template <int KIND, int MAJ>
+__device__ __forceinline__ void field_bw(Tracer<MAJ>& tr, const float* in, const float* og) {
+    f3 x = F3(in[0], in[1], in[2]), w = F3(in[3], in[4], in[5]);
+    if constexpr (KIND == VR_MAP_T_DDA) {
+        tr.t_dda_bw(x, w, og[0]);
+    } else if constexpr (KIND == VR_MAP_T_DT) {
+        tr.t_dt_bw(x, w, og[0]);
+    } else if constexpr (KIND == VR_MAP_T_RT) {
+        tr.t_rt_bw(x, w, og[0]);
+    } else if constexpr (KIND == VR_MAP_COLLISION_DT) {
+        tr.collision_dt_bw(x, w, tr.M.ds_eps, og[0], og[2]);
+    } else if constexpr (KIND == VR_MAP_SS_RADIANCE) {
+        tr.ss_radiance_bw(x, w, F3(og[0], og[1], og[2]));
+    } else if constexpr (KIND == VR_MAP_MC_COLLISION) {
+        tr.mc_collision_bw(x, w, F3(og[0], og[1], og[2]));
+    } else if constexpr (KIND == VR_MAP_NEE_DT) {
+        tr.nee_ds_bw(x, w, tr.M.ds_eps, F3(og[0], og[1], og[2]));
+    } else if constexpr (KIND == VR_MAP_MEDIUM_DT) {
+        tr.medium_dt_bw(x, w, in[6], F3(og[6], og[7], og[8]), F3(og[9], og[10], og[11]));
+    }
+}
+
+template <int KIND>
+struct Dims;
+template <> struct Dims<VR_MAP_RAYBOX> { static constexpr int in = 6, out = 2; };
+template <> struct Dims<VR_MAP_T_DDA> { static constexpr int in = 6, out = 1; };
+template <> struct Dims<VR_MAP_T_DT> { static constexpr int in = 6, out = 1; };
+template <> struct Dims<VR_MAP_T_RT> { static constexpr int in = 6, out = 1; };
+template <> struct Dims<VR_MAP_COLLISION_DT> { static constexpr int in = 6, out = 3; };
+template <> struct Dims<VR_MAP_SS_RADIANCE> { static constexpr int in = 6, out = 3; };
+template <> struct Dims<VR_MAP_MC_COLLISION> { static constexpr int in = 6, out = 3; };
+template <> struct Dims<VR_MAP_NEE_DT> { static constexpr int in = 6, out = 3; };
+template <> struct Dims<VR_MAP_MEDIUM_DT> { static constexpr int in = 7, out = 12; };
+
+// ---------------------------------------------------------------- kernels --
+__global__ void __launch_bounds__(kBlock) k_grid_fw(GridView g, const float* __restrict__ x, int64_t n, float* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 c = load3(x + 3 * i);
+    if (g.C == 1) {
+        out[i] = grid_lookup1(g, c);
+    } else {
+        float v[4];
+        grid_lookupC(g, c, v);
+        for (int k = 0; k < g.C; k++) out[i * g.C + k] = v[k];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_grid_bw(GridView g, const float* __restrict__ x, const float* __restrict__ og, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    grid_scatterC(g, load3(x + 3 * i), og + i * g.C);
+}
+
+// One workgroup per coarse cell: max over the vertices whose trilinear
+// footprint can reach the cell (widened by one vertex per side).
+__global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry, int rx, float* __restrict__ out) {
+    int cell = blockIdx.x;
+    int cx = cell % rx, cy = (cell / rx) % ry, cz = cell / (rx * ry);
+    int n[3] = {g.nx - 1, g.ny - 1, g.nz - 1}, r[3] = {rx, ry, rz}, c[3] = {cx, cy, cz}, lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = max(0, (int)floor((double)c[a] * n[a] / r[a]) - 1);
+        hi[a] = min(n[a], (int)ceil((double)(c[a] + 1) * n[a] / r[a]) + 1);
+    }
+    int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    int total = ex * ey * ez;
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        int xx = lo[0] + k % ex, yy = lo[1] + (k / ex) % ey, zz = lo[2] + k / (ex * ey);
+        m = fmaxf(m, g.data[zz * g.sz + yy * g.sy + xx * g.sx]);
+    }
+    __shared__ float red[kBlock];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[cell] = red[0];
+}
+
+template <int KIND, int MAJ>
+__global__ void __launch_bounds__(kBlock) k_map_fw(MediumView M, const float* __restrict__ in, int64_t start, int64_t cnt,
+                                                   float* __restrict__ out, Seeds seeds, int* __restrict__ dbg) {
+    int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= start + cnt) return;
+    Tracer<MAJ> tr(M);
+    tr.rng = seed_map(seeds, (uint32_t)i);
+    float a[Dims<KIND>::in], o[Dims<KIND>::out];
+#pragma unroll
+    for (int k = 0; k < Dims<KIND>::in; k++) a[k] = in[i * Dims<KIND>::in + k];
+    field_fw<KIND, MAJ>(tr, a, o);
+#pragma unroll
+    for (int k = 0; k < Dims<KIND>::out; k++) out[i * Dims<KIND>::out + k] = o[k];
+    if (dbg) dbg[i] = tr.lookups;
+}
+
+template <int KIND, int MAJ>
+__global__ void __launch_bounds__(kBlock) k_map_bw(MediumView M, const float* __restrict__ in, const float* __restrict__ og,
+                                                   int64_t start, int64_t cnt, Seeds seeds) {
+    int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= start + cnt) return;
+    Tracer<MAJ> tr(M);
+    tr.rng = seed_map(seeds, (uint32_t)i);
+    float a[Dims<KIND>::in], o[Dims<KIND>::out];
+#pragma unroll
+    for (int k = 0; k < Dims<KIND>::in; k++) a[k] = in[i * Dims<KIND>::in + k];
+#pragma unroll
+    for (int k = 0; k < Dims<KIND>::out; k++) o[k] = og[i * Dims<KIND>::out + k];
+    field_bw<KIND, MAJ>(tr, a, o);
+}
+
+template <int KIND, int MAJ>
+__global__ void __launch_bounds__(kBlock) k_capture_fw(MediumView M, CameraView cam, Seeds seeds, int spp, int64_t total,
+                                                       int64_t start, int64_t cnt, float* __restrict__ out, int* __restrict__ dbg) {
+    int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= start + cnt) return;
+    constexpr int OD = Dims<KIND>::out;
+    int pose, row, col;
+    sensor_index(cam, i, pose, row, col);
+    if (pose < 0 || pose >= cam.num_poses || row < 0 || row >= cam.height || col < 0 || col >= cam.width) {
+#pragma unroll
+        for (int k = 0; k < OD; k++) out[i * OD + k] = 0.0f;
+        return;
+    }
+    float acc[OD];
+#pragma unroll
+    for (int k = 0; k < OD; k++) acc[k] = 0.0f;
+    Tracer<MAJ> tr(M);
+    for (int s = 0; s < spp; s++) {
+        tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
+        f3 x, w;
+        camera_ray(cam, pose, row, col, tr.rng, x, w);
+        float a[6] = {x.x, x.y, x.z, w.x, w.y, w.z}, o[OD];
+        field_fw<KIND, MAJ>(tr, a, o);
+#pragma unroll
+        for (int k = 0; k < OD; k++) acc[k] += o[k];
+    }
+    float inv = 1.0f / (float)spp;
+#pragma unroll
+    for (int k = 0; k < OD; k++) out[i * OD + k] = spp == 1 ? acc[k] : acc[k] * inv;
+    if (dbg) dbg[i] = tr.lookups;
+}
+
+template <int KIND, int MAJ>
+__global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView cam, Seeds seeds, int spp, int seed_mode,
+                                                       int64_t total, int64_t start, int64_t cnt, const float* __restrict__ og) {
+    int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= start + cnt) return;
+    constexpr int OD = Dims<KIND>::out;
+    int pose, row, col;
+    sensor_index(cam, i, pose, row, col);
+    if (pose < 0 || pose >= cam.num_poses || row < 0 || row >= cam.height || col < 0 || col >= cam.width) return;
+    float o[OD];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < OD; k++) { o[k] = og[i * OD + k] / (float)spp; any |= o[k] != 0.0f; }
+    if (!any) return;
+    Tracer<MAJ> tr(M);
+    if (seed_mode == VR_SEED_CAPTURE_BW) tr.rng = seed_capture_bw(seeds, (uint32_t)i);
+    for (int s = 0; s < spp; s++) {
+        if (seed_mode != VR_SEED_CAPTURE_BW) tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
+        f3 x, w;
+        camera_ray(cam, pose, row, col, tr.rng, x, w);
+        float a[6] = {x.x, x.y, x.z, w.x, w.y, w.z};
+        field_bw<KIND, MAJ>(tr, a, o);
+    }
+}
+
+// torch.optim.Adam (no weight decay, no amsgrad), optionally clearing the gradient.
+__global__ void __launch_bounds__(kBlock) k_adam(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m,
+                                                 float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                                                 float bc1, float bc2_sqrt, int zero_grad) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        float gi = g[i];
+        float mi = b1 * m[i] + (1.0f - b1) * gi;
+        float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        float denom = sqrtf(vi) / bc2_sqrt + eps;
+        p[i] = p[i] - (lr / bc1) * mi / denom;
+        if (zero_grad) g[i] = 0.0f;
+    }
+}
+
+// ------------------------------------------------------------- dispatch --
+template <int MAJ>
+int launch_map_fw_maj(int kind, const MediumView& M, const float* in, int64_t start, int64_t cnt, float* out, Seeds sd, int* dbg,
+                      hipStream_t st) {
+    dim3 g(grid_for(cnt)), b(kBlock);
+    switch (kind) {
+#define VR_CASE(K) case K: hipLaunchKernelGGL((k_map_fw<K, MAJ>), g, b, 0, st, M, in, start, cnt, out, sd, dbg); break;
+        VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
+        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_MEDIUM_DT)
+#undef VR_CASE
+    default: return set_err(VR_ERR_UNSUPPORTED, "unknown map kind");
+    }
+    return check_hip(hipGetLastError(), "vr_map_fw launch");
+}
+
+template <int MAJ>
+int launch_map_bw_maj(int kind, const MediumView& M, const float* in, const float* og, int64_t start, int64_t cnt, Seeds sd,
+                      hipStream_t st) {
+    dim3 g(grid_for(cnt)), b(kBlock);
+    switch (kind) {
+#define VR_CASE(K) case K: hipLaunchKernelGGL((k_map_bw<K, MAJ>), g, b, 0, st, M, in, og, start, cnt, sd); break;
+        VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
+        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_MEDIUM_DT)
+#undef VR_CASE
+    case VR_MAP_RAYBOX: return VR_OK;  // no parameters
+    default: return set_err(VR_ERR_UNSUPPORTED, "unknown map kind");
+    }
+    return check_hip(hipGetLastError(), "vr_map_bw launch");
+}
+
+template <int MAJ>
+int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
+                          int64_t cnt, float* out, int* dbg, hipStream_t st) {
+    dim3 g(grid_for(cnt)), b(kBlock);
+    switch (kind) {
+#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_fw<K, MAJ>), g, b, 0, st, M, C, sd, spp, total, start, cnt, out, dbg); break;
+        VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
+        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT)
+#undef VR_CASE
+    default: return set_err(VR_ERR_UNSUPPORTED, "map kind cannot be captured (needs 6 inputs)");
+    }
+    return check_hip(hipGetLastError(), "vr_capture_fw launch");
+}
+
+template <int MAJ>
+int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int seed_mode, int64_t total,
+                          int64_t start, int64_t cnt, const float* og, hipStream_t st) {
+    dim3 g(grid_for(cnt)), b(kBlock);
+    switch (kind) {
+#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_bw<K, MAJ>), g, b, 0, st, M, C, sd, spp, seed_mode, total, start, cnt, og); break;
+        VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
+        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT)
+#undef VR_CASE
+    case VR_MAP_RAYBOX: return VR_OK;
+    default: return set_err(VR_ERR_UNSUPPORTED, "map kind cannot be captured (needs 6 inputs)");
+    }
+    return check_hip(hipGetLastError(), "vr_capture_bw launch");
+}
+
+int resolve_range(const vr_launch* L, int64_t total, int64_t& start, int64_t& cnt) {
+    start = L->start_index;
+    if (start < 0 || start > total) return set_err(VR_ERR_ARG, "start_index out of range");
+    cnt = L->count < 0 ? total - start : L->count;
+    if (start + cnt > total) return set_err(VR_ERR_ARG, "start_index + count exceeds the thread count");
+    return VR_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI ==
+extern "C" {
+
+int vr_abi_version(void) { return VR_ABI_VERSION; }
+
+const char* vr_last_error(void) { return g_err; }
+
+int vr_grid_fw(const vr_grid* grid, const float* x, int64_t n, float* out, void* stream) {
+    GridView g;
+    int st = make_grid_view(grid, g, 0, "vr_grid_fw");
+    if (st) return st;
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_grid_fw: negative n");
+    if (n == 0) return VR_OK;
+    if (!x || !out) return set_err(VR_ERR_ARG, "vr_grid_fw: null buffer");
+    hipLaunchKernelGGL(k_grid_fw, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, g, x, n, out);
+    return check_hip(hipGetLastError(), "vr_grid_fw launch");
+}
+
+int vr_grid_bw(const vr_grid* grid, const float* x, const float* out_grad, int64_t n, void* stream) {
+    GridView g;
+    int st = make_grid_view(grid, g, 0, "vr_grid_bw");
+    if (st) return st;
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_grid_bw: negative n");
+    if (n == 0 || g.grad == nullptr) return VR_OK;
+    if (!x || !out_grad) return set_err(VR_ERR_ARG, "vr_grid_bw: null buffer");
+    hipLaunchKernelGGL(k_grid_bw, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, g, x, out_grad, n);
+    return check_hip(hipGetLastError(), "vr_grid_bw launch");
+}
+
+int vr_majorant_build(const vr_grid* sigma, const int32_t res[3], float* out, void* stream) {
+    GridView g;
+    int st = make_grid_view(sigma, g, 1, "vr_majorant_build");
+    if (st) return st;
+    if (!res || !out) return set_err(VR_ERR_ARG, "vr_majorant_build: null argument");
+    if (res[0] < 1 || res[1] < 1 || res[2] < 1 || (int64_t)res[0] * res[1] * res[2] > (1 << 24))
+        return set_err(VR_ERR_ARG, "vr_majorant_build: resolution out of range");
+    hipLaunchKernelGGL(k_majorant, dim3(res[0] * res[1] * res[2]), dim3(kBlock), 0, (hipStream_t)stream, g, res[0], res[1], res[2], out);
+    return check_hip(hipGetLastError(), "vr_majorant_build launch");
+}
+
+int vr_map_fw(const vr_medium* medium, int32_t kind, const float* in, int64_t n, float* out, const vr_launch* L, void* stream) {
+    int id, od;
+    if (!map_dims(kind, id, od)) return set_err(VR_ERR_UNSUPPORTED, "vr_map_fw: unknown map kind");
+    if (!L) return set_err(VR_ERR_ARG, "vr_map_fw: launch is NULL");
+    MediumView M;
+    int st = make_medium_view(medium, kind, L->bw_mode, M);
+    if (st) return st;
+    int64_t start, cnt;
+    if ((st = resolve_range(L, n, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!in || !out) return set_err(VR_ERR_ARG, "vr_map_fw: null buffer");
+    Seeds sd = make_seeds(L);
+    hipStream_t s = (hipStream_t)stream;
+    if (medium->majorant_kind == VR_MAJ_GRID) return launch_map_fw_maj<VR_MAJ_GRID>(kind, M, in, start, cnt, out, sd, L->debug_out, s);
+    return launch_map_fw_maj<VR_MAJ_CONST>(kind, M, in, start, cnt, out, sd, L->debug_out, s);
+}
+
+int vr_map_bw(const vr_medium* medium, int32_t kind, const float* in, const float* og, int64_t n, const vr_launch* L, void* stream) {
+    int id, od;
+    if (!map_dims(kind, id, od)) return set_err(VR_ERR_UNSUPPORTED, "vr_map_bw: unknown map kind");
+    if (!L) return set_err(VR_ERR_ARG, "vr_map_bw: launch is NULL");
+    MediumView M;
+    int st = make_medium_view(medium, kind, L->bw_mode, M);
+    if (st) return st;
+    int64_t start, cnt;
+    if ((st = resolve_range(L, n, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!in || !og) return set_err(VR_ERR_ARG, "vr_map_bw: null buffer");
+    Seeds sd = make_seeds(L);
+    hipStream_t s = (hipStream_t)stream;
+    if (medium->majorant_kind == VR_MAJ_GRID) return launch_map_bw_maj<VR_MAJ_GRID>(kind, M, in, og, start, cnt, sd, s);
+    return launch_map_bw_maj<VR_MAJ_CONST>(kind, M, in, og, start, cnt, sd, s);
+}
+
+int vr_capture_fw(const vr_medium* medium, int32_t kind, const vr_camera* camera, const vr_launch* L, float* out, void* stream) {
+    int id, od;
+    if (!map_dims(kind, id, od)) return set_err(VR_ERR_UNSUPPORTED, "vr_capture_fw: unknown map kind");
+    if (id != 6) return set_err(VR_ERR_UNSUPPORTED, "vr_capture_fw: capture needs a 6-input map");
+    if (!L) return set_err(VR_ERR_ARG, "vr_capture_fw: launch is NULL");
+    MediumView M;
+    int st = make_medium_view(medium, kind, L->bw_mode, M);
+    if (st) return st;
+    CameraView C;
+    if ((st = make_camera_view(camera, C))) return st;
+    int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    int64_t start, cnt;
+    if ((st = resolve_range(L, total, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!out) return set_err(VR_ERR_ARG, "vr_capture_fw: out is NULL");
+    int spp = L->samples < 1 ? 1 : L->samples;
+    Seeds sd = make_seeds(L);
+    hipStream_t s = (hipStream_t)stream;
+    if (medium->majorant_kind == VR_MAJ_GRID)
+        return launch_capture_fw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, s);
+    return launch_capture_fw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, s);
+}
+
+int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera, const vr_launch* L, const float* og, void* stream) {
+    int id, od;
+    if (!map_dims(kind, id, od)) return set_err(VR_ERR_UNSUPPORTED, "vr_capture_bw: unknown map kind");
+    if (id != 6) return set_err(VR_ERR_UNSUPPORTED, "vr_capture_bw: capture needs a 6-input map");
+    if (!L) return set_err(VR_ERR_ARG, "vr_capture_bw: launch is NULL");
+    MediumView M;
+    int st = make_medium_view(medium, kind, L->bw_mode, M);
+    if (st) return st;
+    CameraView C;
+    if ((st = make_camera_view(camera, C))) return st;
+    int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    int64_t start, cnt;
+    if ((st = resolve_range(L, total, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!og) return set_err(VR_ERR_ARG, "vr_capture_bw: out_grad is NULL");
+    int spp = L->samples < 1 ? 1 : L->samples;
+    Seeds sd = make_seeds(L);
+    hipStream_t s = (hipStream_t)stream;
+    if (medium->majorant_kind == VR_MAJ_GRID)
+        return launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, s);
+    return launch_capture_bw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, s);
+}
+
+int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps, int32_t step,
+                 int32_t zero_grad, void* stream) {
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_adam_step: negative n");
+    if (step < 1) return set_err(VR_ERR_ARG, "vr_adam_step: step must be >= 1");
+    if (n == 0) return VR_OK;
+    if (!p || !g || !m || !v) return set_err(VR_ERR_ARG, "vr_adam_step: null buffer");
+    float bc1 = (float)(1.0 - std::pow((double)beta1, step));
+    float bc2s = (float)std::sqrt(1.0 - std::pow((double)beta2, step));
+    int64_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(kBlock), 0, (hipStream_t)stream, p, g, m, v, n, lr, beta1, beta2, eps,
+                       bc1, bc2s, zero_grad);
+    return check_hip(hipGetLastError(), "vr_adam_step launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,1162 @@
+// vr_device.hpp -- gfx950 device functions of the participating-media path.
+//
+// FP32 restatements of the reference's GLSL routines (rendervous include/*.h;
+// each function cites the lines it follows).  The estimators are templated on
+// the majorant kind so the constant-majorant form (the reference default,
+// _objects.py:96-101) and the coarse-grid DDA form compile to separate code
+// without run-time branching in the inner loops.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vr_api.h"
+
+#define VR_DEV __device__ __forceinline__
+#define VR_HD __host__ __device__ __forceinline__
+
+namespace vr {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr float kInf = __builtin_inff();
+// Remaining distance below which a majorant-cell crossing ends the tracking:
+// where the last coarse cell's exit coincides with the medium's exit the
+// crossing test is a rounding tie, and ending there keeps the random stream
+// independent of which side of the tie the arithmetic lands on.
+constexpr float kCrossEps = 1e-6f;
+
+// ---------------------------------------------------------------- vectors --
+struct f3 {
+    float x, y, z;
+};
+VR_HD f3 F3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+VR_HD f3 operator+(f3 a, f3 b) { return F3(a.x + b.x, a.y + b.y, a.z + b.z); }
+VR_HD f3 operator-(f3 a, f3 b) { return F3(a.x - b.x, a.y - b.y, a.z - b.z); }
+VR_HD f3 operator*(f3 a, f3 b) { return F3(a.x * b.x, a.y * b.y, a.z * b.z); }
+VR_HD f3 operator*(f3 a, float s) { return F3(a.x * s, a.y * s, a.z * s); }
+VR_HD f3 operator*(float s, f3 a) { return F3(a.x * s, a.y * s, a.z * s); }
+VR_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+VR_HD f3 cross(f3 a, f3 b) { return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+VR_DEV f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+VR_HD f3 load3(const float* p) { return F3(p[0], p[1], p[2]); }
+
+// -------------------------------------------------------------------- RNG --
+// HybridTaus: three Tausworthe steps and an LCG (include/randoms.h:6-59).
+struct Rng {
+    uint32_t x, y, z, w;
+};
+
+VR_DEV uint32_t taus_step(uint32_t z, int s1, int s2, int s3, uint32_t m) {
+    uint32_t b = ((z << s1) ^ z) >> s2;
+    return ((z & m) << s3) ^ b;
+}
+
+VR_DEV void rng_advance(Rng& s) {
+    s.x = taus_step(s.x, 13, 19, 12, 4294967294u);
+    s.y = taus_step(s.y, 2, 25, 4, 4294967288u);
+    s.z = taus_step(s.z, 3, 11, 17, 4294967280u);
+    s.w = 1664525u * s.w + 1013904223u;
+}
+
+// randoms.h:53-59: uniform in [0,1) with 23 random mantissa bits.
+VR_DEV float rng_uniform(Rng& s) {
+    rng_advance(s);
+    uint32_t v = s.x ^ s.y ^ s.z ^ s.w;
+    return __uint_as_float((v & 0x007FFFFFu) | 0x3F800000u) - 1.0f;
+}
+
+struct Seeds {
+    uint32_t s[4];
+};
+
+// Capture forward per-thread init (_maps.py:549-560).
+VR_DEV Rng seed_capture_fw(const Seeds& sd, uint32_t i) {
+    Rng s;
+    s.x = sd.s[0] + (0x23F1u + i);
+    s.y = sd.s[1] + (0x3137u + (i + 14u) * i * 17923113u);
+    s.z = sd.s[2] + 129u * i;
+    s.w = sd.s[3] + (i + 129u);
+    rng_advance(s);
+    rng_advance(s);
+    float a = rng_uniform(s), b = rng_uniform(s), c = rng_uniform(s), d = rng_uniform(s);
+    s.x = __float_as_uint(a); s.y = __float_as_uint(b); s.z = __float_as_uint(c); s.w = __float_as_uint(d);
+    rng_advance(s); rng_advance(s); rng_advance(s); rng_advance(s);
+    return s;
+}
+
+// Capture backward per-thread init (_maps.py:676-685).
+VR_DEV Rng seed_capture_bw(const Seeds& sd, uint32_t i) {
+    Rng s;
+    s.x = sd.s[0] + 0x23F1u;
+    s.y = sd.s[1] + 0x3137u;
+    s.z = sd.s[2] + 129u;
+    s.w = sd.s[3] + (i + 129u);
+    rng_advance(s); rng_advance(s); rng_advance(s); rng_advance(s);
+    float a = rng_uniform(s), b = rng_uniform(s), c = rng_uniform(s), d = rng_uniform(s);
+    s.x = __float_as_uint(a); s.y = __float_as_uint(b); s.z = __float_as_uint(c); s.w = __float_as_uint(d);
+    return s;
+}
+
+// Map-eval per-thread init (_maps.py:390-393, :465-469).
+VR_DEV Rng seed_map(const Seeds& sd, uint32_t i) {
+    Rng s;
+    s.x = sd.s[0] ^ (i ^ 17u);
+    s.y = sd.s[1] ^ (i * 123111171u);
+    s.z = sd.s[2] ^ (i + 11u);
+    s.w = sd.s[3] ^ (i ^ (sd.s[0] + 13u * sd.s[1]));
+    rng_uniform(s); rng_uniform(s); rng_uniform(s);
+    return s;
+}
+
+// ------------------------------------------------------------------ views --
+// Device view of a Grid3D parameter (include/common.h:88-94 + grid3d.h).
+struct GridView {
+    const float* data;
+    float* grad;
+    int sx, sy, sz, sc;  // element strides of W, H, D, C
+    int nx, ny, nz, C;   // vertices per axis, channels
+    f3 scale;            // grid coordinate = c * scale + offset, scale = (n-1)/(bmax-bmin)
+    f3 offset;
+    f3 dimf;             // n - 1 as floats
+    f3 bmin, bmax;
+};
+
+struct MediumView {
+    GridView sigma;
+    GridView albedo;
+    int albedo_grid;
+    f3 albedo_c;
+    f3 box_min, box_max;
+    f3 emission_c, env_c, sun_dir, sun_radiance;
+    float sun_prob;
+    int phase_kind;
+    float g;
+    float maj_value, maj_distance;
+    const float* maj_grid;
+    int mrx, mry, mrz;
+    f3 maj_bmin, maj_csize;
+    int t_kind;
+    float ds_eps;
+    int max_bounces;
+    int medium_filter;
+    f3 gathering_c;
+    float medium_majorant;
+    int bw_mode;
+};
+
+struct CameraView {
+    const float* poses;
+    const int64_t* sensors;
+    int num_poses, width, height, jittered;
+    float fov, znear;
+};
+
+// ------------------------------------------------------------- ray / box --
+// include/maths.h:28-38 (axis components with |w| <= 1e-6 replaced by 1e-6).
+VR_DEV void ray_box_intersection(f3 x, f3 w, f3 bmin, f3 bmax, float& tmin, float& tmax) {
+    float wx = fabsf(w.x) <= 0.000001f ? 0.000001f : w.x;
+    float wy = fabsf(w.y) <= 0.000001f ? 0.000001f : w.y;
+    float wz = fabsf(w.z) <= 0.000001f ? 0.000001f : w.z;
+    float c0x = (bmin.x - x.x) / wx, c1x = (bmax.x - x.x) / wx;
+    float c0y = (bmin.y - x.y) / wy, c1y = (bmax.y - x.y) / wy;
+    float c0z = (bmin.z - x.z) / wz, c1z = (bmax.z - x.z) / wz;
+    tmin = fmaxf(fmaxf(fminf(c0x, c1x), fminf(c0y, c1y)), fminf(c0z, c1z));
+    tmax = fminf(fminf(fmaxf(c0x, c1x), fmaxf(c0y, c1y)), fmaxf(c0z, c1z));
+}
+
+// include/vr_boundary.h:1-11
+VR_DEV bool boundary(const MediumView& M, f3 x, f3 w, float& tmin, float& tmax) {
+    ray_box_intersection(x, w, M.box_min, M.box_max, tmin, tmax);
+    tmin = fmaxf(0.0f, tmin);
+    return !(tmax < tmin);
+}
+
+// ----------------------------------------------------------------- Grid3D --
+// include/maps/grid3d.h:6-37 for a single channel.
+VR_DEV float grid_lookup1(const GridView& g, f3 c) {
+    float gx = fmaf(c.x, g.scale.x, g.offset.x);
+    float gy = fmaf(c.y, g.scale.y, g.offset.y);
+    float gz = fmaf(c.z, g.scale.z, g.offset.z);
+    if (!(gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx < g.dimf.x && gy < g.dimf.y && gz < g.dimf.z)) return 0.0f;
+    float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float ax = gx - fx, ay = gy - fy, az = gz - fz;
+    int px = min((int)fx, g.nx - 2), py = min((int)fy, g.ny - 2), pz = min((int)fz, g.nz - 2);
+    const float* p = g.data + (pz * g.sz + py * g.sy + px * g.sx);
+    float v000 = p[0], v100 = p[g.sx], v010 = p[g.sy], v110 = p[g.sx + g.sy];
+    float v001 = p[g.sz], v101 = p[g.sz + g.sx], v011 = p[g.sz + g.sy], v111 = p[g.sz + g.sy + g.sx];
+    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    float r = v000 * (bx * by * bz);
+    r += v100 * (ax * by * bz);
+    r += v010 * (bx * ay * bz);
+    r += v110 * (ax * ay * bz);
+    r += v001 * (bx * by * az);
+    r += v101 * (ax * by * az);
+    r += v011 * (bx * ay * az);
+    r += v111 * (ax * ay * az);
+    return r;
+}
+
+// Multi-channel lookup (C <= 4), grid3d.h:6-37.
+VR_DEV void grid_lookupC(const GridView& g, f3 c, float* out) {
+    for (int k = 0; k < g.C; k++) out[k] = 0.0f;
+    float gx = fmaf(c.x, g.scale.x, g.offset.x);
+    float gy = fmaf(c.y, g.scale.y, g.offset.y);
+    float gz = fmaf(c.z, g.scale.z, g.offset.z);
+    if (!(gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx < g.dimf.x && gy < g.dimf.y && gz < g.dimf.z)) return;
+    float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float ax = gx - fx, ay = gy - fy, az = gz - fz;
+    int px = min((int)fx, g.nx - 2), py = min((int)fy, g.ny - 2), pz = min((int)fz, g.nz - 2);
+    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    float wgt[8] = {bx * by * bz, ax * by * bz, bx * ay * bz, ax * ay * bz,
+                    bx * by * az, ax * by * az, bx * ay * az, ax * ay * az};
+    const float* p = g.data + (pz * g.sz + py * g.sy + px * g.sx);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float* q = p + ((k & 1) ? g.sx : 0) + ((k & 2) ? g.sy : 0) + ((k & 4) ? g.sz : 0);
+        for (int ch = 0; ch < g.C; ch++) out[ch] += q[ch * g.sc] * wgt[k];
+    }
+}
+
+VR_DEV void atomic_addf(float* p, float v) {
+    if (v != 0.0f) unsafeAtomicAdd(p, v);
+}
+
+// include/maps/grid3d.h:39-75 (out_grad per channel, og[C]).
+VR_DEV void grid_scatterC(const GridView& g, f3 c, const float* og) {
+    if (g.grad == nullptr) return;
+    float gx = fmaf(c.x, g.scale.x, g.offset.x);
+    float gy = fmaf(c.y, g.scale.y, g.offset.y);
+    float gz = fmaf(c.z, g.scale.z, g.offset.z);
+    if (!(gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx < g.dimf.x && gy < g.dimf.y && gz < g.dimf.z)) return;
+    float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float ax = gx - fx, ay = gy - fy, az = gz - fz;
+    int px = min((int)fx, g.nx - 2), py = min((int)fy, g.ny - 2), pz = min((int)fz, g.nz - 2);
+    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    float wgt[8] = {bx * by * bz, ax * by * bz, bx * ay * bz, ax * ay * bz,
+                    bx * by * az, ax * by * az, bx * ay * az, ax * ay * az};
+    float* p = g.grad + (pz * g.sz + py * g.sy + px * g.sx);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float* q = p + ((k & 1) ? g.sx : 0) + ((k & 2) ? g.sy : 0) + ((k & 4) ? g.sz : 0);
+        for (int ch = 0; ch < g.C; ch++) atomic_addf(q + ch * g.sc, og[ch] * wgt[k]);
+    }
+}
+
+VR_DEV void grid_scatter1(const GridView& g, f3 c, float og) { grid_scatterC(g, c, &og); }
+
+// ------------------------------------------------------------- directions --
+// include/randoms.h:144-151
+VR_DEV void ortho_basis(f3 n, f3& t, f3& b) {
+    float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sign + n.z);
+    float bb = n.x * n.y * a;
+    t = F3(1.0f + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+    b = F3(bb, sign + n.y * n.y * a, -n.y);
+}
+
+// include/randoms.h:153-165 randomDirection(D)
+VR_DEV f3 random_direction(Rng& rng, f3 D) {
+    float r1 = rng_uniform(rng);
+    float r2 = rng_uniform(rng) * 2.0f - 1.0f;
+    float s = sqrtf(fmaxf(0.0f, 1.0f - r2 * r2));
+    float sn, cs;
+    sincosf(kTwoPi * r1, &sn, &cs);
+    f3 t0, t1;
+    ortho_basis(D, t0, t1);
+    return t0 * (cs * s) + t1 * (sn * s) + D * r2;
+}
+
+// include/scattering_AD.h:4-19
+VR_DEV f3 hg_sample(Rng& rng, f3 win, float g) {
+    float phi = rng_uniform(rng) * 2.0f * kPi;
+    float xi = rng_uniform(rng);
+    float cos_t;
+    if (fabsf(g) < 0.001f) cos_t = 2.0f * xi - 1.0f;
+    else {
+        float g2 = g * g;
+        float t = (1.0f - g2) / (1.0f - g + 2.0f * g * xi);
+        cos_t = (0.5f / g) * (1.0f + g2 - t * t);
+    }
+    float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    f3 t0, t1;
+    ortho_basis(win, t0, t1);
+    return t0 * (sin_t * sp) + t1 * (sin_t * cp) + win * cos_t;
+}
+
+// include/scattering_AD.h:39-47
+VR_DEV float hg_eval(float cos_t, float g) {
+    if (fabsf(g) < 0.001f) return 0.25f / kPi;
+    float g2 = g * g;
+    float den = 1.0f + g2 - 2.0f * g * cos_t;
+    return 0.25f / kPi * (1.0f - g2) / (den * sqrtf(den));
+}
+
+// HGPhase (_maps.py:4446-4454) / single HG / isotropic.
+VR_DEV float phase_eval(const MediumView& M, f3 w, f3 wo) {
+    float ct = dot(w, wo);
+    if (M.phase_kind == VR_PHASE_HG_MIX) {
+        float a = hg_eval(ct, M.g), b = hg_eval(ct, M.g / 5.0f);
+        return a * (1.0f - 0.2f) + b * 0.2f;
+    }
+    if (M.phase_kind == VR_PHASE_HG) return hg_eval(ct, M.g);
+    return 0.25f / kPi;
+}
+
+// VHGPhaseSampler (_maps.py:4463-4481) / HGPhaseSampler (_scenes.py:1984-2004) / uniform.
+VR_DEV float phase_sample(const MediumView& M, Rng& rng, f3 w, f3& wo) {
+    if (M.phase_kind == VR_PHASE_HG_MIX) {
+        float g = M.g;
+        if (rng_uniform(rng) < 0.8f) g /= 5.0f;
+        wo = hg_sample(rng, w, g);
+        return 1.0f;
+    }
+    if (M.phase_kind == VR_PHASE_HG) { wo = hg_sample(rng, w, M.g); return 1.0f; }
+    wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
+    return 1.0f;
+}
+
+// _functionsampler(UniformDirectionSampler, const env) (_maps.py:2552-2585),
+// mixed with a directional light with MixtureSampler semantics (_maps.py:1541-1571).
+VR_DEV f3 env_sample(const MediumView& M, Rng& rng, f3& wo) {
+    if (M.sun_prob > 0.0f) {
+        if (rng_uniform(rng) < M.sun_prob) {
+            wo = M.sun_dir;
+            return M.sun_radiance * (1.0f / M.sun_prob);
+        }
+        wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
+        return M.env_c * (4.0f * kPi / (1.0f - M.sun_prob));
+    }
+    wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
+    return M.env_c * (4.0f * kPi);
+}
+
+// ---------------------------------------------------------------- majorant --
+// include/vr_majorant.h:1-7 contract (value, distance).
+template <int MAJ>
+struct Walk;
+
+template <>
+struct Walk<VR_MAJ_CONST> {
+    static constexpr bool grid = false;
+    VR_DEV void begin(const MediumView&, f3, f3) {}
+    VR_DEV float query(const MediumView& M, float& md) const { md = M.maj_distance; return M.maj_value; }
+    VR_DEV void move(float) {}
+    VR_DEV void cross() {}
+};
+
+// Coarse max-grid DDA: value = bound of the current coarse cell, distance = to
+// that cell's exit along the ray.
+template <>
+struct Walk<VR_MAJ_GRID> {
+    static constexpr bool grid = true;
+    int cx, cy, cz, stx, sty, stz;
+    float tnx, tny, tnz, tdx, tdy, tdz, t;
+    int idx;  // linear cell index, -1 when outside
+
+    VR_DEV void axis_begin(float xa, float wa, float bmin, float cs, int res, int& cell, int& st, float& tn, float& td) {
+        int c = (int)floorf((xa - bmin) / cs);
+        c = max(0, min(res - 1, c));
+        cell = c;
+        if (wa > 0.0f) { st = 1; tn = (bmin + (float)(c + 1) * cs - xa) / wa; td = cs / wa; }
+        else if (wa < 0.0f) { st = -1; tn = (bmin + (float)c * cs - xa) / wa; td = -cs / wa; }
+        else { st = 0; tn = kInf; td = kInf; }
+    }
+    VR_DEV void begin(const MediumView& M, f3 x, f3 w) {
+        axis_begin(x.x, w.x, M.maj_bmin.x, M.maj_csize.x, M.mrx, cx, stx, tnx, tdx);
+        axis_begin(x.y, w.y, M.maj_bmin.y, M.maj_csize.y, M.mry, cy, sty, tny, tdy);
+        axis_begin(x.z, w.z, M.maj_bmin.z, M.maj_csize.z, M.mrz, cz, stz, tnz, tdz);
+        t = 0.0f;
+        idx = (cz * M.mry + cy) * M.mrx + cx;
+        mrx = M.mrx; mry = M.mry; mrz = M.mrz;
+        data = M.maj_grid;
+    }
+    int mrx, mry, mrz;
+    const float* data;
+    VR_DEV float query(const MediumView&, float& md) const {
+        if (idx < 0) { md = kInf; return 0.0f; }
+        md = fmaxf(0.0f, fminf(tnx, fminf(tny, tnz)) - t);
+        return data[idx];
+    }
+    VR_DEV void move(float dt) { t += dt; }
+    VR_DEV void cross() {
+        if (idx < 0) return;
+        if (tnx <= tny && tnx <= tnz) {
+            t = tnx; cx += stx; tnx += tdx;
+            if (cx < 0 || cx >= mrx) { idx = -1; return; }
+        } else if (tny <= tnz) {
+            t = tny; cy += sty; tny += tdy;
+            if (cy < 0 || cy >= mry) { idx = -1; return; }
+        } else {
+            t = tnz; cz += stz; tnz += tdz;
+            if (cz < 0 || cz >= mrz) { idx = -1; return; }
+        }
+        idx = (cz * mry + cy) * mrx + cx;
+    }
+};
+
+// -------------------------------------------------------------- estimators --
+template <int MAJ>
+struct Tracer {
+    const MediumView& M;
+    Rng rng;
+    int lookups;
+
+    VR_DEV explicit Tracer(const MediumView& m) : M(m), lookups(0) {}
+
+    VR_DEV float rnd() { return rng_uniform(rng); }
+    VR_DEV float flight(float maj) { return -logf(1.0f - rnd()) / maj; }
+
+    VR_DEV float sigma(f3 x) { lookups++; return grid_lookup1(M.sigma, x); }
+    VR_DEV void sigma_bw(f3 x, float g) { grid_scatter1(M.sigma, x, g); }
+
+    VR_DEV f3 albedo(f3 x) {
+        if (!M.albedo_grid) return M.albedo_c;
+        float v[4];
+        grid_lookupC(M.albedo, x, v);
+        if (M.albedo.C == 1) return F3(v[0], v[0], v[0]);
+        return F3(v[0], v[1], v[2]);
+    }
+    VR_DEV void albedo_bw(f3 x, f3 g) {
+        if (!M.albedo_grid || M.albedo.grad == nullptr) return;
+        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) return;
+        float og[3];
+        if (M.albedo.C == 1) og[0] = g.x + g.y + g.z;
+        else { og[0] = g.x; og[1] = g.y; og[2] = g.z; }
+        grid_scatterC(M.albedo, x, og);
+    }
+
+    // include/vr_transmittance_DT.h:1-28
+    VR_DEV float t_dt(f3 x, f3 w) {
+        float tmin, tmax;
+        if (!boundary(M, x, w, tmin, tmax)) return 1.0f;
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(0.0000001f, mj.query(M, md));
+            float dt = flight(maj);
+            float t = fminf(md, dt);
+            x = x + w * t;
+            d -= t;
+            if (dt > md) { mj.cross(); if (d <= kCrossEps) return 1.0f; continue; }
+            mj.move(t);
+            if (d <= 0.0000001f) return 1.0f;
+            if (rnd() < sigma(x) / maj) return 0.0f;
+        }
+    }
+
+    // include/vr_transmittance_RT.h:1-49
+    VR_DEV float t_rt(f3 x, f3 w) {
+        float T = 1.0f, tmin, tmax;
+        if (!boundary(M, x, w, tmin, tmax)) return T;
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(0.0000001f, mj.query(M, md));
+            float s = flight(maj);
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(); if (d <= kCrossEps) return T; continue; }
+            float dt = fminf(s, fminf(d, md));
+            x = x + w * dt;
+            d -= dt;
+            mj.move(dt);
+            if (d <= 0.0000001f) return T;
+            T *= (1.0f - sigma(x) / maj);
+            if (T < 0.01f) {
+                if (rnd() < 1.0f - T) return 0.0f;
+                T = 1.0f;
+            }
+        }
+    }
+
+    // ---- DDA (include/vr_transmittance_DDA.h), traversal in FP64 -------------
+    struct Dda {
+        double x[3], d;
+        int cell[3], side[3], dim[3];
+        double ainc[3], alpha0[3], vn[3], vm[3];
+    };
+
+    VR_DEV bool dda_begin(f3 xf, f3 wf, Dda& s) {
+        float tminf, tmaxf;
+        if (!boundary(M, xf, wf, tminf, tmaxf)) return false;
+        // intersect_ray_box (maths.h:11-26) with the grid box, in double
+        double x[3] = {xf.x, xf.y, xf.z}, w[3] = {wf.x, wf.y, wf.z};
+        double bmin[3] = {M.sigma.bmin.x, M.sigma.bmin.y, M.sigma.bmin.z};
+        double bmax[3] = {M.sigma.bmax.x, M.sigma.bmax.y, M.sigma.bmax.z};
+        double gmin = -1e300, gmax = 1e300;
+        for (int a = 0; a < 3; a++) {
+            double wa = fabs(w[a]) <= 0.000001 ? 0.000001 : w[a];
+            double c0 = (bmin[a] - x[a]) / wa, c1 = (bmax[a] - x[a]) / wa;
+            gmin = fmax(gmin, fmin(c0, c1));
+            gmax = fmin(gmax, fmax(c0, c1));
+        }
+        gmin = fmax(0.0, gmin);
+        if (gmax <= gmin || gmax <= 0.0) return false;
+        // the boundary map's interval, recomputed in double (maths.h:28-38)
+        double bmn[3] = {M.box_min.x, M.box_min.y, M.box_min.z}, bmx[3] = {M.box_max.x, M.box_max.y, M.box_max.z};
+        double tmin = -1e300, tmax = 1e300;
+        for (int a = 0; a < 3; a++) {
+            double wa = fabs(w[a]) <= 0.000001 ? 0.000001 : w[a];
+            double c0 = (bmn[a] - x[a]) / wa, c1 = (bmx[a] - x[a]) / wa;
+            tmin = fmax(tmin, fmin(c0, c1));
+            tmax = fmin(tmax, fmax(c0, c1));
+        }
+        tmin = fmax(0.0, tmin);
+        if (tmax < tmin) return false;
+        tmin = fmax(tmin, gmin);
+        tmax = fmin(tmax, gmax);
+        if (tmax <= tmin) return false;
+        s.d = tmax - tmin;
+        s.dim[0] = M.sigma.nx - 1; s.dim[1] = M.sigma.ny - 1; s.dim[2] = M.sigma.nz - 1;
+        for (int a = 0; a < 3; a++) {
+            double xa = x[a] + w[a] * tmin;
+            s.x[a] = xa;
+            double bs = bmax[a] - bmin[a];
+            double cs = bs / s.dim[a];
+            int cl = (int)((xa - bmin[a]) * s.dim[a] / bs);
+            cl = max(0, min(s.dim[a] - 1, cl));
+            s.cell[a] = cl;
+            double aw = fmax(0.000001, fabs(w[a]));
+            s.ainc[a] = cs / aw;
+            s.side[a] = w[a] > 0 ? 1 : (w[a] < 0 ? -1 : 0);
+            double corner = (cl + s.side[a] * 0.5 + 0.5) * cs + bmin[a];
+            s.alpha0[a] = fabs(corner - xa) / aw;
+            s.vn[a] = (xa - bmin[a]) * s.dim[a] / bs;
+            s.vm[a] = w[a] * s.dim[a] / bs;
+        }
+        return true;
+    }
+
+    VR_DEV float dda_sigma_at(int x, int y, int z) {
+        const GridView& g = M.sigma;
+        x = max(0, min(g.nx - 1, x)); y = max(0, min(g.ny - 1, y)); z = max(0, min(g.nz - 1, z));
+        return g.data[z * g.sz + y * g.sy + x * g.sx];
+    }
+
+    static VR_DEV double lerpd(double a, double b, double t) { return a * (1 - t) + b * t; }
+
+    static VR_DEV void dda_step(double* alpha, int* cell, const Dda& s) {
+        int ix = alpha[0] <= alpha[1] && alpha[0] <= alpha[2];
+        int iy = alpha[0] > alpha[1] && alpha[1] <= alpha[2];
+        int iz = alpha[0] > alpha[2] && alpha[1] > alpha[2];
+        alpha[0] += ix * s.ainc[0]; alpha[1] += iy * s.ainc[1]; alpha[2] += iz * s.ainc[2];
+        cell[0] += ix * s.side[0]; cell[1] += iy * s.side[1]; cell[2] += iz * s.side[2];
+    }
+
+    VR_DEV double dda_tau(const Dda& s) {
+        double alpha[3] = {s.alpha0[0], s.alpha0[1], s.alpha0[2]};
+        int cell[3] = {s.cell[0], s.cell[1], s.cell[2]};
+        double cur = 0.0, tau = 0.0;
+        while (cur < s.d - 0.0001) {
+            double nxt = fmin(s.d, fmin(alpha[0], fmin(alpha[1], alpha[2])));
+            double v0 = dda_sigma_at(cell[0], cell[1], cell[2]), v1 = dda_sigma_at(cell[0] + 1, cell[1], cell[2]);
+            double v2 = dda_sigma_at(cell[0], cell[1] + 1, cell[2]), v3 = dda_sigma_at(cell[0] + 1, cell[1] + 1, cell[2]);
+            double v4 = dda_sigma_at(cell[0], cell[1], cell[2] + 1), v5 = dda_sigma_at(cell[0] + 1, cell[1], cell[2] + 1);
+            double v6 = dda_sigma_at(cell[0], cell[1] + 1, cell[2] + 1), v7 = dda_sigma_at(cell[0] + 1, cell[1] + 1, cell[2] + 1);
+            lookups++;
+            double tm = lerpd(cur, nxt, 0.5);
+            double ax = s.vm[0] * tm + s.vn[0] - cell[0];
+            double ay = s.vm[1] * tm + s.vn[1] - cell[1];
+            double az = s.vm[2] * tm + s.vn[2] - cell[2];
+            double v = lerpd(lerpd(lerpd(v0, v1, ax), lerpd(v2, v3, ax), ay), lerpd(lerpd(v4, v5, ax), lerpd(v6, v7, ax), ay), az);
+            tau += v * (nxt - cur);
+            dda_step(alpha, cell, s);
+            cur = nxt;
+        }
+        return tau;
+    }
+
+    VR_DEV float t_dda(f3 x, f3 w) {
+        Dda s;
+        if (!dda_begin(x, w, s)) return 1.0f;
+        return (float)exp(-dda_tau(s));
+    }
+
+    // include/vr_transmittance_DDA.h:62-156
+    VR_DEV void t_dda_bw(f3 x, f3 w, float dL_dT) {
+        if (dL_dT == 0.0f || M.sigma.grad == nullptr) return;
+        Dda s;
+        if (!dda_begin(x, w, s)) return;
+        double T = exp(-dda_tau(s));
+        double dL_dtau = -(double)dL_dT * T;
+        double alpha[3] = {s.alpha0[0], s.alpha0[1], s.alpha0[2]};
+        int cell[3] = {s.cell[0], s.cell[1], s.cell[2]};
+        double cur = 0.0;
+        const GridView& g = M.sigma;
+        while (cur < s.d - 0.0001) {
+            double nxt = fmin(s.d, fmin(alpha[0], fmin(alpha[1], alpha[2])));
+            double tm = lerpd(cur, nxt, 0.5);
+            double ax = s.vm[0] * tm + s.vn[0] - cell[0];
+            double ay = s.vm[1] * tm + s.vn[1] - cell[1];
+            double az = s.vm[2] * tm + s.vn[2] - cell[2];
+            double gs = dL_dtau * (nxt - cur);
+            double bx = 1 - ax, by = 1 - ay, bz = 1 - az;
+            double wg[8] = {bx * by * bz, ax * by * bz, bx * ay * bz, ax * ay * bz, bx * by * az, ax * by * az, bx * ay * az, ax * ay * az};
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int xx = max(0, min(g.nx - 1, cell[0] + (k & 1)));
+                int yy = max(0, min(g.ny - 1, cell[1] + ((k >> 1) & 1)));
+                int zz = max(0, min(g.nz - 1, cell[2] + ((k >> 2) & 1)));
+                atomic_addf(g.grad + (zz * g.sz + yy * g.sy + xx * g.sx), (float)(gs * wg[k]));
+            }
+            dda_step(alpha, cell, s);
+            cur = nxt;
+        }
+    }
+
+    // include/vr_transmittance_DT.h:30-90
+    VR_DEV void t_dt_bw(f3 x, f3 w, float dL_dT) {
+        if (dL_dT == 0.0f) return;
+        float tmin, tmax;
+        if (!boundary(M, x, w, tmin, tmax)) return;
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        f3 x0 = x;
+        float d0 = d;
+        Rng saved = rng;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(0.0000001f, mj.query(M, md) + 0.1f);
+            float dt = flight(maj);
+            float t = fminf(md, dt);
+            x = x + w * t;
+            d -= t;
+            if (dt > md) { mj.cross(); if (d <= kCrossEps) break; continue; }
+            mj.move(t);
+            if (d <= 0.0000001f) break;
+            if (rnd() < sigma(x) / maj) return;
+        }
+        x = x0; d = d0;
+        rng = saved;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(0.0000001f, mj.query(M, md) + 0.1f);
+            float dt = flight(maj);
+            float t = fminf(md, dt);
+            x = x + w * t;
+            d -= t;
+            if (dt > md) { mj.cross(); if (d <= kCrossEps) break; continue; }
+            mj.move(t);
+            if (d <= 0.0000001f) break;
+            rnd();
+            float Pc = sigma(x) / maj;
+            sigma_bw(x, -dL_dT / maj / (1.0f - Pc));
+        }
+    }
+
+    // include/vr_transmittance_RT.h:51-141
+    VR_DEV void t_rt_bw(f3 x, f3 w, float dL_dT) {
+        if (dL_dT == 0.0f) return;
+        float tmin, tmax;
+        if (!boundary(M, x, w, tmin, tmax)) return;
+        const float EPS = 0.0000001f;
+        float T = 1.0f;
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        f3 x0 = x;
+        float d0 = d;
+        Rng saved = rng;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(EPS, mj.query(M, md) + 0.1f);
+            float s = flight(maj);
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            float dt = fminf(s, fminf(d, md));
+            x = x + w * dt;
+            d -= dt;
+            mj.move(dt);
+            if (d <= EPS) break;
+            T *= (1.0f - sigma(x) / maj);
+            if (T < 0.1f) {
+                if (rnd() < 1.0f - T) return;
+                T = 1.0f;
+            }
+        }
+        float out_T = T;
+        T = 1.0f;
+        rng = saved;
+        x = x0; d = d0;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(EPS, mj.query(M, md) + 0.1f);
+            float s = flight(maj);
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            float dt = fminf(s, fminf(d, md));
+            x = x + w * dt;
+            d -= dt;
+            mj.move(dt);
+            if (d <= EPS) break;
+            float r = 1.0f - sigma(x) / maj;
+            T *= r;
+            sigma_bw(x, -out_T * dL_dT / maj / r);
+            if (T < 0.1f) {
+                rng_advance(rng);
+                T = 1.0f;
+            }
+        }
+    }
+
+    VR_DEV float transmittance(f3 x, f3 w) {
+        if (M.t_kind == VR_T_RT) return t_rt(x, w);
+        if (M.t_kind == VR_T_DDA) return t_dda(x, w);
+        return t_dt(x, w);
+    }
+
+    VR_DEV void transmittance_bw(f3 x, f3 w, float dL_dT) {
+        if (M.t_kind == VR_T_RT) t_rt_bw(x, w, dL_dT);
+        else if (M.t_kind == VR_T_DDA) t_dda_bw(x, w, dL_dT);
+        else t_dt_bw(x, w, dL_dT);
+    }
+
+    // include/vr_collision_sampler_DT.h:5-44
+    VR_DEV float collision_dt(f3 x, f3 w, float& out_t, float& out_T) {
+        out_T = 1.0f; out_t = 0.0f;
+        float C = 0.0f, tmin, tmax;
+        if (!boundary(M, x, w, tmin, tmax)) return C;
+        float d = tmax - tmin;
+        x = x + w * tmin;
+        float t = 0.0f;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = mj.query(M, md);
+            if (Walk<MAJ>::grid) maj = fmaxf(0.0000001f, maj);
+            float dt = fminf(flight(maj), fminf(d, md));
+            x = x + w * dt;
+            t += dt;
+            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            mj.move(dt);
+            if (t >= d - 0.0000001f) break;
+            if (rnd() < sigma(x) / maj) { out_T = 0.0f; C = 1.0f; break; }
+        }
+        out_t = tmin + t;
+        return C;
+    }
+
+    // Defensive primal pass of vr_collision_sampler_DT.h:52-90.
+    VR_DEV void collision_ds_primal(f3 x, f3 w, float eps, float tmin, float tmax, float& C, float& t_out, float& Wc) {
+        float d = tmax - tmin;
+        x = x + w * tmin;
+        float t = 0.0f, W = 1.0f;
+        C = 0.0f;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(0.0000000001f, mj.query(M, md) + 2.0f * eps);
+            float dt = fminf(flight(maj), fminf(d, md));
+            x = x + w * dt;
+            t += dt;
+            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            mj.move(dt);
+            if (t >= d - 0.0000001f) break;
+            float s = sigma(x);
+            float Pc = (s + eps) / maj;
+            if (rnd() < Pc) { C = 1.0f; W *= (s / maj) / Pc; break; }
+            W *= (1.0f - s / maj) / (1.0f - Pc);
+        }
+        t_out = t;
+        Wc = W;
+    }
+
+    // include/vr_collision_sampler_DT.h:46-134 (VR_BW_CONSISTENT: full path weight)
+    VR_DEV void collision_dt_bw(f3 x, f3 w, float eps, float dL_dC, float dL_dT) {
+        float tmin, tmax;
+        if (!boundary(M, x, w, tmin, tmax)) return;
+        Rng saved = rng;
+        float out_T = 1.0f, C = 0.0f, s_t = 0.0f, Wpre = 1.0f;
+        float d = tmax - tmin;
+        x = x + w * tmin;
+        f3 x0 = x;
+        float t = 0.0f;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(0.0000000001f, mj.query(M, md) + 2.0f * eps);
+            float dt = fminf(flight(maj), fminf(d, md));
+            x = x + w * dt;
+            t += dt;
+            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            mj.move(dt);
+            if (t >= d - 0.0000001f) break;
+            float s = sigma(x);
+            float Pc = (s + eps) / maj;
+            if (rnd() < Pc) { out_T = 0.0f; C = 1.0f; s_t = s / maj / Pc; break; }
+            float r = (1.0f - s / maj) / (1.0f - Pc);
+            out_T *= r;
+            Wpre *= r;
+        }
+        bool consistent = M.bw_mode == VR_BW_CONSISTENT;
+        if (consistent) s_t *= Wpre;
+        rng = saved;
+        x = x0; t = 0.0f;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float maj = fmaxf(0.0000000001f, mj.query(M, md) + 2.0f * eps);
+            float dt = fminf(flight(maj), fminf(d, md));
+            x = x + w * dt;
+            t += dt;
+            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            mj.move(dt);
+            if (t >= d - 0.0000001f) break;
+            float s = sigma(x);
+            float Pc = (s + eps) / maj;
+            if (rnd() < Pc) {
+                float tr = C / Pc;
+                if (consistent) tr *= Wpre;
+                sigma_bw(x, dL_dC * tr * (1.0f / maj));
+                break;
+            } else {
+                float q = (dL_dT * out_T + dL_dC * C * s_t) / ((1.0f - s / maj) / (1.0f - Pc)) / (1.0f - Pc);
+                sigma_bw(x, q * (-1.0f / maj));
+            }
+        }
+    }
+
+    // include/maps/single_scattered_radiance_mc.h:6-16
+    VR_DEV f3 ss_radiance(f3 x, f3 w) {
+        f3 wo;
+        f3 env = env_sample(M, rng, wo);
+        float ph = phase_eval(M, w * -1.0f, wo);
+        f3 sa = albedo(x);
+        float T = transmittance(x, wo);
+        return (env * sa) * (ph * T);
+    }
+
+    // include/maps/single_scattered_radiance_mc.h:19-38
+    VR_DEV void ss_radiance_bw(f3 x, f3 w, f3 dL_dR) {
+        f3 wo;
+        f3 env = env_sample(M, rng, wo);
+        float ph = phase_eval(M, w * -1.0f, wo);
+        f3 sa = albedo(x);
+        Rng before_T = rng;
+        float T = transmittance(x, wo);
+        f3 dL_dsa = (dL_dR * env) * (ph * T);
+        float dL_dT = dot(dL_dR, (env * sa) * ph);
+        albedo_bw(x, dL_dsa);
+        if (M.bw_mode == VR_BW_CONSISTENT) {
+            Rng after = rng;
+            rng = before_T;
+            transmittance_bw(x, wo, dL_dT);
+            rng = after;
+        } else {
+            transmittance_bw(x, w, dL_dT);
+        }
+    }
+
+    // include/maps/collision_integrator_mc.h:10-28 (exitance = SS radiance)
+    VR_DEV f3 mc_collision(f3 x, f3 w) {
+        float T, t;
+        float C = collision_dt(x, w, t, T);
+        f3 xt = w * t + x;
+        f3 R = M.env_c * T;
+        if (C != 0.0f) R = R + ss_radiance(xt, w * -1.0f) * C;
+        return R;
+    }
+
+    // include/maps/collision_integrator_mc.h:30-44
+    VR_DEV void mc_collision_bw(f3 x, f3 w, f3 dL_dR) {
+        if (M.bw_mode == VR_BW_REFERENCE) {
+            float T, t;
+            float C = collision_dt(x, w, t, T);
+            f3 xt = w * t + x;
+            f3 eR = ss_radiance(xt, w * -1.0f);
+            collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, M.env_c));
+            ss_radiance_bw(xt, w * -1.0f, dL_dR * C);
+            return;
+        }
+        float tmin, tmax;
+        if (!boundary(M, x, w, tmin, tmax)) return;
+        Rng saved = rng;
+        float C, t, Wc;
+        collision_ds_primal(x, w, M.ds_eps, tmin, tmax, C, t, Wc);
+        f3 xt = x + w * (tmin + t);
+        f3 eR = F3(0.0f, 0.0f, 0.0f);
+        if (C != 0.0f) eR = ss_radiance(xt, w * -1.0f);
+        Rng after = rng;
+        rng = saved;
+        collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, M.env_c));
+        rng = after;
+        if (C != 0.0f) ss_radiance_bw(xt, w * -1.0f, dL_dR * Wc);
+    }
+
+    // include/vr_path_integrator_NEE.h:1-66 (max_bounces >= 0 caps scatter events)
+    VR_DEV f3 nee(f3 x, f3 w) {
+        float tmin, tmax;
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        f3 R = M.env_c * transmittance(x, w);
+        f3 W = F3(1.0f, 1.0f, 1.0f);
+        const float EPS = 0.00001f;
+        int bounces = 0;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        while (d > EPS) {
+            float md;
+            float m = fmaxf(EPS, mj.query(M, md));
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            if (rnd() >= s / m) { d -= t; continue; }
+            f3 sa = albedo(x);
+            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            W = W * sa;
+            f3 wnee;
+            f3 env = env_sample(M, rng, wnee);
+            float Tn = transmittance(x, wnee);
+            R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
+            f3 wo;
+            float rho = phase_sample(M, rng, w, wo);
+            W = W * rho;
+            if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+        }
+        return R;
+    }
+
+    // include/vr_path_integrator_NEE_DS.h:6-206
+    VR_DEV void nee_ds_bw(f3 x, f3 w, float eps, f3 dL_dR) {
+        Rng before_path = rng;
+        f3 x0 = x, w0 = w;
+        float tmin, tmax;
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        f3 R = F3(0.0f, 0.0f, 0.0f), R_sp = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
+        float null_W = 1.0f;
+        const float EPS = 0.00000001f;
+        bool scattered = false;
+        int bounces = 0;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        while (d > EPS) {
+            float md;
+            float m = fmaxf(EPS, mj.query(M, md) + 2.0f * eps + 0.1f);
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            float Pc = (s + eps) / m;
+            if (rnd() >= Pc) { d -= t; W = W * ((1.0f - s / m) / (1.0f - Pc)); continue; }
+            scattered = true;
+            if (s < EPS) {
+                if (null_W < EPS) { W = F3(0.0f, 0.0f, 0.0f); break; }
+                null_W = 0.0f;
+            } else W = W * ((s / m) / Pc);
+            f3 sa = albedo(x);
+            f3 added = (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            W = W * sa;
+            f3 wnee;
+            f3 env = env_sample(M, rng, wnee);
+            float Tn = transmittance(x, wnee);
+            float phn = phase_eval(M, w, wnee);
+            added = added + (W * env) * (Tn * phn);
+            if (null_W < EPS) R_sp = R_sp + added;
+            else R = R + added;
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
+            f3 wo;
+            float rho = phase_sample(M, rng, w, wo);
+            W = W * rho;
+            if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); break; }
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+        }
+        if (!scattered) R = R + M.env_c * W;
+
+        rng = before_path;
+        x = x0; w = w0;
+        W = F3(1.0f, 1.0f, 1.0f);
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        d = tmax - tmin;
+        bounces = 0;
+        mj.begin(M, x, w);
+        const bool ref = M.bw_mode == VR_BW_REFERENCE;
+        while (d > EPS) {
+            float md;
+            float mq = mj.query(M, md);
+            float m = ref ? fmaxf(EPS, mq * 1.2f + 2.0f * eps) : fmaxf(EPS, mq + 2.0f * eps + 0.1f);
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            float Pc = (s + eps) / m;
+            if (rnd() >= Pc) {
+                sigma_bw(x, dot(dL_dR, R * (-1.0f / m / (1.0f - s / m))));
+                W = W * ((1.0f - s / m) / (1.0f - Pc));
+                d -= t;
+                continue;
+            }
+            float gs = s < EPS ? dot(dL_dR, R_sp * (1.0f / eps)) : dot(dL_dR, R * (1.0f / s));
+            sigma_bw(x, gs);
+            W = W * (s / m / Pc);
+            if (s < EPS) return;
+            f3 sa = albedo(x);
+            f3 dL_dsa = (dL_dR * W) * (M.emission_c * -1.0f);
+            R = R - (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            dL_dsa = dL_dsa + dL_dR * F3(R.x / fmaxf(EPS, sa.x), R.y / fmaxf(EPS, sa.y), R.z / fmaxf(EPS, sa.z));
+            f3 wnee;
+            f3 env = env_sample(M, rng, wnee);
+            Rng before_T = rng;
+            float Tn = transmittance(x, wnee);
+            float phn = phase_eval(M, w, wnee);
+            f3 wo;
+            float rho = phase_sample(M, rng, w, wo);
+            Rng before_next = rng;
+            W = W * sa;
+            float dL_dTn = dot(dL_dR, (W * env) * phn);
+            R = R - (W * env) * (Tn * phn);
+            albedo_bw(x, dL_dsa);
+            rng = before_T;
+            transmittance_bw(x, wnee, dL_dTn);
+            rng = before_next;
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return;
+            W = W * rho;
+            if (W.x < EPS && W.y < EPS && W.z < EPS) return;
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+        }
+    }
+
+    // include/sc_medium_path_integrator_DT_fw.h:1-32
+    VR_DEV void medium_dt(f3 x, f3 w, float d, float* out) {
+        f3 W = F3(1.0f, 1.0f, 1.0f), A = F3(0.0f, 0.0f, 0.0f);
+        f3 xo = w * d + x, wo = w;
+        float maj = fmaxf(0.00001f, M.medium_majorant);
+        for (;;) {
+            float t = flight(maj);
+            if (t > d - 0.00001f) break;
+            x = x + w * t;
+            d -= t;
+            float s = sigma(x);
+            if (rnd() < s / maj) {
+                if (M.medium_filter & 1) {
+                    xo = x;
+                    phase_sample(M, rng, w, wo);
+                    W = albedo(x);
+                } else W = F3(0.0f, 0.0f, 0.0f);
+                if (M.medium_filter & 2) A = M.gathering_c;
+                break;
+            }
+        }
+        out[0] = xo.x; out[1] = xo.y; out[2] = xo.z;
+        out[3] = wo.x; out[4] = wo.y; out[5] = wo.z;
+        out[6] = W.x; out[7] = W.y; out[8] = W.z;
+        out[9] = A.x; out[10] = A.y; out[11] = A.z;
+    }
+
+    // include/sc_medium_path_integrator_DT_bw.h:1-86
+    VR_DEV void medium_dt_bw(f3 x, f3 w, float d, f3 dL_dW, f3 dL_dA) {
+        f3 W = F3(1.0f, 1.0f, 1.0f), A = F3(0.0f, 0.0f, 0.0f);
+        f3 xi = x, wi = w;
+        float di = d;
+        Rng before = rng;
+        float maj = fmaxf(0.00001f, M.medium_majorant);
+        int f = M.medium_filter;
+        for (;;) {
+            float t = flight(maj);
+            if (t > d - 0.00001f) break;
+            x = x + w * t;
+            d -= t;
+            float s = sigma(x);
+            if (rnd() < s / maj) {
+                if (f & 1) { f3 wo; float wt = phase_sample(M, rng, w, wo); W = albedo(x) * wt; }
+                else W = F3(0.0f, 0.0f, 0.0f);
+                if (f & 2) A = M.gathering_c;
+                break;
+            }
+        }
+        rng = before;
+        x = xi; w = wi; d = di;
+        const bool scat_blk = (f & 1) != 0;
+        const bool acc_blk = M.bw_mode == VR_BW_REFERENCE ? (f & 1) != 0 : (f & 2) != 0;
+        for (;;) {
+            float t = flight(maj);
+            if (t > d - 0.00001f) break;
+            x = x + w * t;
+            d -= t;
+            float s = sigma(x);
+            float Pc = s / maj;
+            if (rnd() < Pc) {
+                float g = 0.0f;
+                if (scat_blk) {
+                    f3 wo;
+                    phase_sample(M, rng, w, wo);
+                    g += dot(dL_dW, W * (1.0f / s));
+                    albedo_bw(x, dL_dW);
+                }
+                if (acc_blk) g += dot(dL_dA, A * (1.0f / s));
+                if (g != 0.0f) sigma_bw(x, g);
+                break;
+            } else {
+                float g = -dot(dL_dW, W) / maj / (1.0f - Pc);
+                if (acc_blk) g += -dot(dL_dA, A) / maj / (1.0f - Pc);
+                if (g != 0.0f) sigma_bw(x, g);
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------------ camera --
+// include/maps/sensor_perspective_camera.h:1-27
+VR_DEV void camera_ray(const CameraView& cam, int pose, int row, int col, Rng& rng, f3& xo, f3& wo) {
+    const float* P = cam.poses + 9 * pose;
+    f3 o = load3(P), d = load3(P + 3), n = load3(P + 6);
+    float u = 0.5f, v = 0.5f;
+    if (cam.jittered) { u = rng_uniform(rng); v = rng_uniform(rng); }
+    float sx = (((float)col + u) * 2.0f - (float)cam.width) * cam.znear / (float)cam.height;
+    float sy = (((float)row + v) * 2.0f - (float)cam.height) * cam.znear / (float)cam.height;
+    float sz = cam.znear / tanf(cam.fov * 0.5f);
+    f3 za = normalize(d);
+    f3 xa = normalize(cross(n, za));
+    f3 ya = cross(za, xa);
+    f3 w = xa * sx + ya * sy + za * sz;
+    xo = o + w;
+    wo = normalize(w);
+}
+
+VR_DEV void sensor_index(const CameraView& cam, int64_t i, int& pose, int& row, int& col) {
+    if (cam.sensors) {
+        pose = (int)cam.sensors[3 * i];
+        row = (int)cam.sensors[3 * i + 1];
+        col = (int)cam.sensors[3 * i + 2];
+    } else {
+        col = (int)(i % cam.width);
+        row = (int)((i / cam.width) % cam.height);
+        pose = (int)(i / ((int64_t)cam.width * cam.height));
+    }
+}
+
+}  // namespace vr

@@ -1,0 +1,847 @@
+"""Map API of the participating-media hot path.
+
+Same class names, constructor signatures and call semantics as the reference's
+map library (rendervous _maps.py) for the maps on the volume path.  A map graph
+is not compiled to a shader: it is *lowered* to a ``vr_medium`` descriptor plus
+a map kind and evaluated by the hand-written HIP kernels behind the C ABI
+(include/vr_api.h).  Graphs outside the supported set raise NotImplementedError;
+there is no fallback evaluator.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import typing
+
+import numpy as np
+import torch
+
+from . import _internal
+from . import _lib as L
+
+__all__ = [
+    "vec3", "MapBase", "ConstantMap", "const", "ZERO", "ONE", "Grid3D", "RayBoxIntersection", "ray_box_intersection",
+    "MajorantGrid", "GridDDATransmittance", "DeltatrackingTransmittance", "RatiotrackingTransmittance",
+    "DeltatrackingCollisionSampler", "HGPhase", "VHGPhaseSampler", "UniformDirectionSampler", "_functionsampler",
+    "DirectionalLightEnvironmentSampler", "SingleScatteredRadiance", "MCCollisionIntegrator",
+    "DeltatrackingNEEPathIntegrator", "SensorsBase", "PerspectiveCameraSensor", "normalized_box",
+    "AutogradMapFunction", "AutogradCaptureFunction",
+]
+
+_ENGINE_LOCK = threading.Lock()  # serialises capture fw/bw and map bw (_maps.py:57,752,827,929)
+
+
+def vec3(x: float, y: float, z: float) -> torch.Tensor:
+    """A float32 3-vector (the reference's vulky vec3)."""
+    return torch.tensor([x, y, z], dtype=torch.float32)
+
+
+def _host3(v) -> typing.List[float]:
+    if isinstance(v, torch.Tensor):
+        return [float(a) for a in v.detach().cpu().tolist()]
+    return [float(a) for a in v]
+
+
+# ----------------------------------------------------------------- base --
+class MapBase(torch.nn.Module):
+    """A map R^INPUT_DIM -> R^OUTPUT_DIM (include/common.h:23-25)."""
+
+    input_dim: typing.Optional[int] = None
+    output_dim: typing.Optional[int] = None
+
+    def __init__(self, input_dim: typing.Optional[int] = None, output_dim: typing.Optional[int] = None):
+        super().__init__()
+        object.__setattr__(self, "input_dim", input_dim)
+        object.__setattr__(self, "output_dim", output_dim)
+
+    @property
+    def is_generic_input(self) -> bool:
+        return self.input_dim is None
+
+    def cast(self, input_dim: typing.Optional[int] = None, output_dim: typing.Optional[int] = None) -> "MapBase":
+        """Type a generic map (_maps.py:1260-1271)."""
+        assert input_dim is None or self.input_dim is None or self.input_dim == input_dim, \
+            f"Required input {input_dim} was not satisfied {self.input_dim}"
+        assert output_dim is None or self.output_dim == output_dim or self.output_dim == 1, \
+            f"Required output {output_dim} was not satisfied {self.output_dim}"
+        return self
+
+    def get_maximum(self) -> float:
+        raise NotImplementedError()
+
+    def support_maximum_query(self) -> bool:
+        return False
+
+    # Lowering to the C ABI: returns (kind, medium-builder) for ray maps.
+    def _lower_map(self) -> typing.Tuple[int, "_Lowered"]:
+        raise NotImplementedError(f"{type(self).__name__} cannot be evaluated by the HIP path")
+
+    def forward(self, *args):
+        x, = args
+        assert self.output_dim is not None, "Can not eval a map with generic output"
+        assert self.input_dim is None or self.input_dim == x.shape[-1], "Map input dimension and tensor last dim missmatch"
+        params = [p for p in self.parameters() if p.requires_grad]
+        return AutogradMapFunction.apply(x, self, *params)
+
+
+class ConstantMap(MapBase):
+    """const[v...] (_maps.py:1957-1985, include/maps/constant.h)."""
+
+    def __init__(self, value: torch.Tensor, input_dim: typing.Optional[int] = None):
+        assert len(value.shape) <= 1, f"Error {value.shape}"
+        assert value.dtype == torch.float
+        super().__init__(input_dim, value.numel())
+        self.value = value if isinstance(value, torch.nn.Parameter) else torch.nn.Parameter(value, requires_grad=False)
+        self._host = None
+        self._host_version = -1
+
+    def host_values(self) -> typing.List[float]:
+        if self._host is None or self._host_version != self.value._version:
+            self._host = [float(a) for a in self.value.detach().reshape(-1).cpu().tolist()]
+            self._host_version = self.value._version
+        return self._host
+
+    def cast(self, input_dim=None, output_dim=None):
+        assert output_dim is None or output_dim == self.output_dim or self.output_dim == 1, \
+            f"Constant map cast to an invalid output dim {output_dim} from {self.output_dim}"
+        c = self
+        if input_dim is not None and self.input_dim is None:
+            c = ConstantMap(self.value, input_dim)
+        if output_dim is not None and self.output_dim == 1 and output_dim > 1:
+            c = _PromotedConstant(c, output_dim)
+        return c
+
+    def vec(self, n: int) -> typing.List[float]:
+        v = self.host_values()
+        if len(v) == 1:
+            return v * n
+        assert len(v) == n, f"constant has {len(v)} values, {n} required"
+        return v
+
+    def get_maximum(self) -> float:
+        return max(self.host_values())
+
+
+class _PromotedConstant(ConstantMap):
+    """A scalar constant promoted to `dim` outputs (ComposePromoteMap of a ConstantMap)."""
+
+    def __init__(self, base: ConstantMap, dim: int):
+        super().__init__(base.value, base.input_dim)
+        object.__setattr__(self, "output_dim", dim)
+
+    def host_values(self):
+        v = super().host_values()
+        return v * self.output_dim if len(v) == 1 else v
+
+
+class _ConstFactory:
+    """const[...] parametric constructor (_maps.py:4791-4806)."""
+
+    def __getitem__(self, args):
+        if not isinstance(args, tuple):
+            args = (args,)
+        input_dim = None
+        if len(args) > 0 and isinstance(args[-1], dict):
+            input_dim = args[-1].get("input_dim")
+            args = args[:-1]
+        if len(args) == 1 and isinstance(args[0], torch.Tensor):
+            return ConstantMap(args[0], input_dim=input_dim)
+        return ConstantMap(torch.tensor([float(a) for a in args], dtype=torch.float32), input_dim=input_dim)
+
+
+const = _ConstFactory()
+ZERO = const[0.0]
+ONE = const[1.0]
+
+
+# ---------------------------------------------------------------- Grid3D --
+class Grid3D(MapBase):
+    """Vertex-centred trilinear grid [D,H,W,C] over [bmin,bmax] (_maps.py:2319-2365)."""
+
+    def __init__(self, grid: torch.Tensor, bmin: torch.Tensor = None, bmax: torch.Tensor = None):
+        assert len(grid.shape) == 4
+        super().__init__(3, grid.shape[-1])
+        bmin = vec3(-1.0, -1.0, -1.0) if bmin is None else bmin
+        bmax = vec3(1.0, 1.0, 1.0) if bmax is None else bmax
+        self.grid = grid if isinstance(grid, torch.nn.Parameter) else torch.nn.Parameter(grid, requires_grad=False)
+        self.bmin = _host3(bmin)
+        self.bmax = _host3(bmax)
+        self.width = grid.shape[2]
+        self.height = grid.shape[1]
+        self.depth = grid.shape[0]
+
+    def view(self, grad: typing.Optional[torch.Tensor] = None) -> L.VrGrid:
+        return L.grid_view(self.grid.data, self.bmin, self.bmax, grad)
+
+    def get_maximum(self) -> float:
+        return self.grid.max().item()
+
+    def support_maximum_query(self) -> bool:
+        return True
+
+    def forward(self, *args):
+        x, = args
+        assert x.shape[-1] == 3
+        params = [self.grid] if self.grid.requires_grad else []
+        return _Grid3DFunction.apply(x, self, *params)
+
+
+class _Grid3DFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, grid_map: Grid3D, *params):
+        lib = L.load()
+        xc = x.detach().contiguous()
+        n = xc.numel() // 3
+        out = torch.empty(*x.shape[:-1], grid_map.output_dim, device=x.device, dtype=torch.float32)
+        g = grid_map.view()
+        L.check(lib.vr_grid_fw(g, L.ptr(xc), n, L.ptr(out), L.stream_handle(x.device)), "vr_grid_fw")
+        ctx.save_for_backward(xc)
+        ctx.grid_map = grid_map
+        return out
+
+    @staticmethod
+    def backward(ctx, og):
+        xc, = ctx.saved_tensors
+        grid_map: Grid3D = ctx.grid_map
+        if not grid_map.grid.requires_grad:
+            return (None, None)
+        lib = L.load()
+        grad = torch.zeros_like(grid_map.grid)
+        og = og.contiguous()
+        g = grid_map.view(grad)
+        L.check(lib.vr_grid_bw(g, L.ptr(xc), L.ptr(og), xc.numel() // 3, L.stream_handle(xc.device)), "vr_grid_bw")
+        return (None, None, grad)
+
+
+# -------------------------------------------------------------- boundary --
+class RayBoxIntersection(MapBase):
+    """(x,w) -> (tMin, tMax) of the slab test (_maps.py:4128-4154, maths.h:28-38)."""
+
+    def __init__(self, box_min, box_max, **kwargs):
+        super().__init__(6, 2)
+        self.box_min = _host3(box_min)
+        self.box_max = _host3(box_max)
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.medium.box_min = L.f3(self.box_min)
+        lw.medium.box_max = L.f3(self.box_max)
+        return L.VR_MAP_RAYBOX, lw
+
+
+class _RayBoxFactory:
+    def __getitem__(self, args):
+        return RayBoxIntersection(*args)
+
+
+ray_box_intersection = _RayBoxFactory()
+
+
+def normalized_box(s) -> typing.Tuple[torch.Tensor, torch.Tensor]:
+    """Box of half-size (shape-1)/(2*max_dim) (_maps.py:4780-4788)."""
+    shape = s.shape if isinstance(s, torch.Tensor) else s
+    max_dim = max(shape[0], shape[1], shape[2]) - 1
+    b_max = vec3(shape[2] - 1, shape[1] - 1, shape[0] - 1) * 0.5 / max_dim
+    return -b_max, b_max
+
+
+# -------------------------------------------------------------- majorant --
+class MajorantGrid(MapBase):
+    """(x,w) -> (majorant, distance) from a coarse max grid over a Grid3D.
+
+    Implements the 6->2 majorant contract (include/vr_majorant.h:1-7) that the
+    reference fills with const[max(sigma), 1e6] (_objects.py:96-101).  Each
+    coarse cell stores the maximum of the vertices that can influence the
+    trilinear field inside it; the trackers walk the cells with a DDA.  The
+    coarse grid is rebuilt when the density tensor changes (TensorCheck,
+    _maps.py:1061-1073).
+    """
+
+    def __init__(self, sigma: Grid3D, resolution: typing.Union[int, typing.Sequence[int]] = 16):
+        super().__init__(6, 2)
+        assert isinstance(sigma, Grid3D) and sigma.output_dim == 1
+        self.sigma_grid = sigma
+        r = [resolution] * 3 if isinstance(resolution, int) else list(resolution)
+        self.resolution = [int(a) for a in r]  # rz, ry, rx
+        self._cache = None
+        self._cache_key = None
+
+    def majorant_tensor(self) -> torch.Tensor:
+        g = self.sigma_grid.grid
+        key = (g.data_ptr(), g._version, tuple(g.shape), tuple(g.stride()))
+        if self._cache is None or self._cache_key != key:
+            out = torch.empty(*self.resolution, device=g.device, dtype=torch.float32)
+            res = (L.ctypes.c_int32 * 3)(*self.resolution)
+            L.check(L.load().vr_majorant_build(self.sigma_grid.view(), res, L.ptr(out), L.stream_handle(g.device)),
+                    "vr_majorant_build")
+            self._cache = out
+            self._cache_key = key
+        return self._cache
+
+
+# ---------------------------------------------------------- directions --
+class UniformDirectionSampler(MapBase):
+    """Uniform sphere direction with weight 4*pi (include/maps/uniform_direction_sampler.h)."""
+
+    def __init__(self, input_dim):
+        super().__init__(input_dim, 4)
+
+
+class _functionsampler(MapBase):
+    """f(w)*weight, w from a point sampler (_maps.py:2552-2585)."""
+
+    def __init__(self, point_sampler: MapBase, function_map: MapBase):
+        assert point_sampler.output_dim - 1 == function_map.input_dim or function_map.input_dim is None
+        super().__init__(point_sampler.input_dim, point_sampler.output_dim - 1 + function_map.output_dim)
+        self.point_sampler = point_sampler
+        self.function_map = function_map
+
+
+class DirectionalLightEnvironmentSampler(MapBase):
+    """Environment sampler for a constant environment plus a directional light.
+
+    The reference has no directional light; this sampler expresses one with the
+    reference's MixtureSampler semantics (_maps.py:1541-1571): with probability
+    `probability` it returns the light direction with weight radiance/probability,
+    otherwise a uniform direction with weight env*4*pi/(1-probability).  The
+    light's direction points from the medium towards the light.
+    """
+
+    def __init__(self, environment: MapBase, direction, radiance, probability: float = 0.5):
+        super().__init__(6, 6)
+        assert 0.0 < probability < 1.0
+        self.environment = environment
+        d = np.asarray(_host3(direction), dtype=np.float64)
+        d = d / np.linalg.norm(d)
+        self.direction = [float(a) for a in d]
+        self.radiance = _host3(radiance)
+        self.probability = float(probability)
+
+
+class HGPhase(MapBase):
+    """Henyey-Greenstein phase (x, w, wo) -> p (_maps.py:4436-4456).
+
+    mixture=True evaluates the reference's 0.8*HG(g) + 0.2*HG(g/5) (:4453);
+    mixture=False a single HG(g).
+    """
+
+    def __init__(self, phase_g: MapBase, mixture: bool = True):
+        super().__init__(9, 1)
+        self.phase_g = phase_g.cast(3, 1)
+        self.mixture = mixture
+
+
+class VHGPhaseSampler(MapBase):
+    """HG direction sampler (x, w) -> (1, wo) (_maps.py:4463-4484).
+
+    mixture=True samples HG(g/5) with probability 0.8 as the reference does (:4475);
+    mixture=False samples HG(g).
+    """
+
+    def __init__(self, phase_g: MapBase, mixture: bool = True):
+        super().__init__(6, 4)
+        self.phase_g = phase_g.cast(3, 1)
+        self.mixture = mixture
+
+
+# ---------------------------------------------------------- lowering --
+class _Lowered:
+    """A vr_medium under construction plus the tensors it points to."""
+
+    def __init__(self):
+        self.medium = L.VrMedium()
+        self.medium.albedo_kind = L.VR_FIELD_CONST
+        self.medium.albedo_c = L.f3([0.0, 0.0, 0.0])
+        self.medium.phase_kind = L.VR_PHASE_ISO
+        self.medium.majorant_kind = L.VR_MAJ_CONST
+        self.medium.majorant_value = 0.0
+        self.medium.majorant_distance = 1e6
+        self.medium.transmittance_kind = L.VR_T_DT
+        self.medium.max_bounces = -1
+        self.medium.medium_filter = 0
+        self.keep: typing.List[torch.Tensor] = []
+        self.sigma: typing.Optional[Grid3D] = None
+        self.albedo: typing.Optional[Grid3D] = None
+        self.majorant_set = False
+        self.device: typing.Optional[torch.device] = None
+
+    def set_sigma(self, sigma: MapBase):
+        if not isinstance(sigma, Grid3D) or sigma.output_dim != 1:
+            raise NotImplementedError("sigma must be a single-channel Grid3D")
+        if self.sigma is not None and self.sigma is not sigma:
+            raise NotImplementedError("all sub-maps must share one sigma Grid3D")
+        self.sigma = sigma
+        self.device = sigma.grid.device
+        self.medium.sigma = sigma.view()
+
+    def set_boundary(self, boundary: MapBase):
+        if not isinstance(boundary, RayBoxIntersection):
+            raise NotImplementedError("boundary must be a RayBoxIntersection")
+        self.medium.box_min = L.f3(boundary.box_min)
+        self.medium.box_max = L.f3(boundary.box_max)
+
+    def set_majorant(self, majorant: MapBase):
+        before = (self.medium.majorant_kind, self.medium.majorant_value, self.medium.majorant_distance,
+                  self.medium.majorant_grid)
+        self._set_majorant(majorant)
+        after = (self.medium.majorant_kind, self.medium.majorant_value, self.medium.majorant_distance,
+                 self.medium.majorant_grid)
+        if self.majorant_set and before != after:
+            raise NotImplementedError("all sub-maps must share one majorant")
+        self.majorant_set = True
+
+    def _set_majorant(self, majorant: MapBase):
+        if isinstance(majorant, MajorantGrid):
+            if majorant.sigma_grid is not self.sigma:
+                raise NotImplementedError("MajorantGrid must bound the medium's sigma")
+            t = majorant.majorant_tensor()
+            self.keep.append(t)
+            self.medium.majorant_kind = L.VR_MAJ_GRID
+            self.medium.majorant_grid = t.data_ptr()
+            for i in range(3):
+                self.medium.majorant_res[i] = majorant.resolution[i]
+        elif isinstance(majorant, ConstantMap):
+            v = majorant.vec(2)
+            self.medium.majorant_kind = L.VR_MAJ_CONST
+            self.medium.majorant_value = v[0]
+            self.medium.majorant_distance = v[1]
+        else:
+            raise NotImplementedError("majorant must be const[value, distance] or a MajorantGrid")
+
+    def set_albedo(self, albedo: MapBase):
+        if isinstance(albedo, Grid3D):
+            if albedo.output_dim not in (1, 3):
+                raise NotImplementedError("albedo grid must have 1 or 3 channels")
+            self.albedo = albedo
+            self.medium.albedo_kind = L.VR_FIELD_GRID
+            self.medium.albedo = albedo.view()
+        elif isinstance(albedo, ConstantMap):
+            self.medium.albedo_kind = L.VR_FIELD_CONST
+            self.medium.albedo_c = L.f3(albedo.vec(3))
+        else:
+            raise NotImplementedError("scattering albedo must be a constant or a Grid3D")
+
+    def set_emission(self, emission: MapBase):
+        if not isinstance(emission, ConstantMap):
+            raise NotImplementedError("emission must be a constant")
+        self.medium.emission_c = L.f3(emission.vec(3))
+
+    def set_environment(self, environment: MapBase):
+        if not isinstance(environment, ConstantMap):
+            raise NotImplementedError("environment must be a constant")
+        self.medium.env_c = L.f3(environment.vec(3))
+
+    def set_environment_sampler(self, sampler: MapBase):
+        if isinstance(sampler, _functionsampler) and isinstance(sampler.point_sampler, UniformDirectionSampler) \
+                and isinstance(sampler.function_map, ConstantMap):
+            env = sampler.function_map.vec(3)
+            self.medium.sun_prob = 0.0
+        elif isinstance(sampler, DirectionalLightEnvironmentSampler) and isinstance(sampler.environment, ConstantMap):
+            env = sampler.environment.vec(3)
+            self.medium.sun_dir = L.f3(sampler.direction)
+            self.medium.sun_radiance = L.f3(sampler.radiance)
+            self.medium.sun_prob = sampler.probability
+        else:
+            raise NotImplementedError("environment sampler must sample a constant environment")
+        if list(self.medium.env_c) != [0.0, 0.0, 0.0] and [float(a) for a in self.medium.env_c] != [float(a) for a in L.f3(env)]:
+            raise NotImplementedError("environment sampler and environment must share the constant")
+        self.medium.env_c = L.f3(env)
+
+    def set_phase(self, phase: MapBase, sampler: MapBase):
+        if isinstance(phase, HGPhase) and isinstance(sampler, VHGPhaseSampler):
+            if not (isinstance(phase.phase_g, ConstantMap) and isinstance(sampler.phase_g, ConstantMap)):
+                raise NotImplementedError("phase g must be constant")
+            g1, g2 = phase.phase_g.vec(1)[0], sampler.phase_g.vec(1)[0]
+            if g1 != g2 or phase.mixture != sampler.mixture:
+                raise NotImplementedError("phase and phase sampler must use the same HG parameters")
+            self.medium.phase_kind = L.VR_PHASE_HG_MIX if phase.mixture else L.VR_PHASE_HG
+            self.medium.phase_g = g1
+        elif isinstance(phase, ConstantMap) and abs(phase.vec(1)[0] - 1.0 / (4 * math.pi)) < 1e-7 \
+                and _is_uniform_phase_sampler(sampler):
+            self.medium.phase_kind = L.VR_PHASE_ISO
+        else:
+            raise NotImplementedError("phase must be HGPhase + VHGPhaseSampler or isotropic")
+
+    def set_transmittance(self, t: MapBase):
+        if isinstance(t, GridDDATransmittance):
+            self.set_sigma(t.grid_model)
+            self.medium.transmittance_kind = L.VR_T_DDA
+            return
+        if isinstance(t, (DeltatrackingTransmittance, RatiotrackingTransmittance)):
+            self.set_sigma(t.sigma)
+            self.set_boundary(t.boundary)
+            self.set_majorant(t.majorant)
+            self.medium.transmittance_kind = L.VR_T_DT if isinstance(t, DeltatrackingTransmittance) else L.VR_T_RT
+            return
+        raise NotImplementedError("transmittance must be a DT, RT or DDA transmittance map")
+
+
+def _is_uniform_phase_sampler(s: MapBase) -> bool:
+    return isinstance(s, UniformDirectionSampler) or getattr(s, "_uniform_phase", False)
+
+
+class _RayMap(MapBase):
+    """Base of maps evaluated by vr_map_fw / vr_map_bw."""
+
+    def _lower_map(self):
+        raise NotImplementedError
+
+    def forward(self, *args):
+        x, = args
+        assert x.shape[-1] == self.input_dim, "Map input dimension and tensor last dim missmatch"
+        params = [p for p in self.parameters() if p.requires_grad]
+        return AutogradMapFunction.apply(x, self, *params)
+
+
+class GridDDATransmittance(_RayMap):
+    """Deterministic DDA transmittance over a Grid3D (_maps.py:2640-2667)."""
+
+    def __init__(self, grid: Grid3D, boundary: MapBase):
+        super().__init__(6, 1)
+        self.boundary = boundary
+        self.grid_model = grid
+
+    def update_grid(self, grid: typing.Optional[Grid3D] = None):
+        if grid is not None:
+            self.grid_model = grid
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.set_sigma(self.grid_model)
+        lw.set_boundary(self.boundary)
+        lw.medium.transmittance_kind = L.VR_T_DDA
+        return L.VR_MAP_T_DDA, lw
+
+
+class DeltatrackingTransmittance(_RayMap):
+    """Delta-tracking transmittance (_maps.py:2685-2697, vr_transmittance_DT.h)."""
+
+    def __init__(self, sigma: MapBase, boundary: MapBase, majorant: MapBase):
+        super().__init__(6, 1)
+        self.sigma = sigma
+        self.boundary = boundary
+        self.majorant = majorant
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.set_transmittance(self)
+        return L.VR_MAP_T_DT, lw
+
+
+class RatiotrackingTransmittance(_RayMap):
+    """Ratio-tracking transmittance (_maps.py:2670-2682, vr_transmittance_RT.h)."""
+
+    def __init__(self, sigma: MapBase, boundary: MapBase, majorant: MapBase):
+        super().__init__(6, 1)
+        self.sigma = sigma.cast(3, 1)
+        self.boundary = boundary.cast(6, 2)
+        self.majorant = majorant.cast(6, 2)
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.set_transmittance(self)
+        return L.VR_MAP_T_RT, lw
+
+
+class DeltatrackingCollisionSampler(_RayMap):
+    """(x,w) -> (C, t, T) collision sampling (_maps.py:2714-2727, vr_collision_sampler_DT.h)."""
+
+    def __init__(self, sigma: MapBase, boundary: MapBase, majorant: MapBase, ds_epsilon: float):
+        super().__init__(6, 3)
+        self.sigma = sigma
+        self.boundary = boundary
+        self.majorant = majorant
+        self.ds_epsilon = ds_epsilon
+
+    def _lower_into(self, lw: _Lowered):
+        lw.set_sigma(self.sigma)
+        lw.set_boundary(self.boundary)
+        lw.set_majorant(self.majorant)
+        lw.medium.ds_epsilon = self.ds_epsilon
+
+    def _lower_map(self):
+        lw = _Lowered()
+        self._lower_into(lw)
+        return L.VR_MAP_COLLISION_DT, lw
+
+
+class SingleScatteredRadiance(_RayMap):
+    """env * albedo * phase * T(x, wo) (_maps.py:2761-2784, single_scattered_radiance_mc.h)."""
+
+    def __init__(self, scattering_albedo: MapBase, phase: MapBase, environment_sampler: MapBase, transmittance: MapBase):
+        super().__init__(6, 3)
+        self.scattering_albedo = scattering_albedo.cast(3, 3)
+        self.phase = phase.cast(9, 1)
+        self.environment_sampler = environment_sampler.cast(6, 6)
+        self.transmittance = transmittance.cast(6, 1)
+
+    def _lower_into(self, lw: _Lowered):
+        lw.set_transmittance(self.transmittance)
+        lw.set_albedo(self.scattering_albedo)
+        lw.set_environment_sampler(self.environment_sampler)
+        if isinstance(self.phase, HGPhase):
+            g = self.phase.phase_g.vec(1)[0]
+            lw.medium.phase_kind = L.VR_PHASE_HG_MIX if self.phase.mixture else L.VR_PHASE_HG
+            lw.medium.phase_g = g
+        elif isinstance(self.phase, ConstantMap):
+            lw.medium.phase_kind = L.VR_PHASE_ISO
+        else:
+            raise NotImplementedError("phase must be HGPhase or isotropic")
+
+    def _lower_map(self):
+        lw = _Lowered()
+        self._lower_into(lw)
+        return L.VR_MAP_SS_RADIANCE, lw
+
+
+class MCCollisionIntegrator(_RayMap):
+    """C * exitance(x_t, -w) + T * env(w) (_maps.py:2788-2808, collision_integrator_mc.h).
+
+    Lowered when the exitance is a SingleScatteredRadiance and the environment a constant.
+    """
+
+    def __init__(self, collision_sampler: MapBase, exitance_radiance: MapBase, environment: MapBase):
+        super().__init__(6, 3)
+        self.collision_sampler = collision_sampler
+        self.exitance_radiance = exitance_radiance
+        self.environment = environment
+
+    def _lower_map(self):
+        if not isinstance(self.collision_sampler, DeltatrackingCollisionSampler):
+            raise NotImplementedError("collision sampler must be DeltatrackingCollisionSampler")
+        if not isinstance(self.exitance_radiance, SingleScatteredRadiance):
+            raise NotImplementedError("exitance radiance must be SingleScatteredRadiance")
+        lw = _Lowered()
+        self.collision_sampler._lower_into(lw)
+        self.exitance_radiance._lower_into(lw)
+        lw.set_environment(self.environment)
+        return L.VR_MAP_MC_COLLISION, lw
+
+
+class DeltatrackingNEEPathIntegrator(_RayMap):
+    """NEE volume path tracer with delta tracking (_maps.py:2924-2978,
+    vr_path_integrator_NEE.h; backward vr_path_integrator_NEE_DS.h).
+
+    max_bounces < 0 keeps the reference's unbounded paths; otherwise the path
+    ends after that many scattering events (their NEE included).
+    """
+
+    def __init__(self, sigma: MapBase, scattering_albedo: MapBase, emission: MapBase, environment: MapBase,
+                 environment_sampler: MapBase, phase: MapBase, phase_sampler: MapBase, boundary: MapBase,
+                 majorant: MapBase, transmittance: MapBase, ds_epsilon: float, max_bounces: int = -1):
+        super().__init__(6, 3)
+        self.sigma = sigma.cast(3, 1)
+        self.scattering_albedo = scattering_albedo.cast(3, 3)
+        self.emission = emission.cast(6, 3)
+        self.environment = environment.cast(3, 3)
+        self.environment_sampler = environment_sampler.cast(6, 6)
+        self.phase = phase.cast(9, 1)
+        self.phase_sampler = phase_sampler.cast(6, 4)
+        self.boundary = boundary.cast(6, 2)
+        self.majorant = majorant.cast(6, 2)
+        self.transmittance = transmittance.cast(6, 1)
+        self.ds_epsilon = ds_epsilon
+        self.max_bounces = max_bounces
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.set_sigma(self.sigma)
+        lw.set_boundary(self.boundary)
+        lw.set_majorant(self.majorant)
+        lw.set_albedo(self.scattering_albedo)
+        lw.set_emission(self.emission)
+        lw.set_environment(self.environment)
+        lw.set_environment_sampler(self.environment_sampler)
+        lw.set_phase(self.phase, self.phase_sampler)
+        lw.set_transmittance(self.transmittance)
+        lw.medium.ds_epsilon = self.ds_epsilon
+        lw.medium.max_bounces = int(self.max_bounces)
+        return L.VR_MAP_NEE_DT, lw
+
+
+# ------------------------------------------------------------- autograd --
+def _grad_targets(lw: _Lowered, params: typing.Sequence[torch.Tensor]):
+    """Fresh zero gradient buffers for the parameters that require grad."""
+    grads = {}
+    if lw.sigma is not None and lw.sigma.grid.requires_grad:
+        grads["sigma"] = torch.zeros_like(lw.sigma.grid)
+        lw.medium.sigma = lw.sigma.view(grads["sigma"])
+    if lw.albedo is not None and lw.albedo.grid.requires_grad:
+        grads["albedo"] = torch.zeros_like(lw.albedo.grid)
+        lw.medium.albedo = lw.albedo.view(grads["albedo"])
+    out = []
+    for p in params:
+        if lw.sigma is not None and p is lw.sigma.grid:
+            out.append(grads.get("sigma"))
+        elif lw.albedo is not None and p is lw.albedo.grid:
+            out.append(grads.get("albedo"))
+        else:
+            out.append(None)
+    return out
+
+
+def _bw_mode() -> int:
+    return L.VR_BW_CONSISTENT if _internal.backward_settings.estimator == "consistent" else L.VR_BW_REFERENCE
+
+
+class AutogradMapFunction(torch.autograd.Function):
+    """Map evaluation on explicit inputs (_maps.py:1598-1613)."""
+
+    @staticmethod
+    def forward(ctx, x, map_object, *params):
+        kind, lw = map_object._lower_map()
+        in_dim, out_dim = L.MAP_DIMS[kind]
+        xc = x.detach().to(torch.float32).contiguous()
+        n = xc.numel() // in_dim
+        out = torch.empty(*x.shape[:-1], out_dim, device=x.device, dtype=torch.float32)
+        seeds = _internal.get_seeds()
+        launch = L.launch(seeds, bw_mode=_bw_mode(), seed_mode=L.VR_SEED_MAP)
+        L.check(L.load().vr_map_fw(lw.medium, kind, L.ptr(xc), n, L.ptr(out), launch, L.stream_handle(x.device)), "vr_map_fw")
+        ctx.save_for_backward(xc)
+        ctx.map_object = map_object
+        ctx.seeds = seeds
+        return out
+
+    @staticmethod
+    def backward(ctx, og):
+        xc, = ctx.saved_tensors
+        map_object = ctx.map_object
+        params = [p for p in map_object.parameters() if p.requires_grad]
+        with _ENGINE_LOCK:
+            kind, lw = map_object._lower_map()
+            grads = _grad_targets(lw, params)
+            in_dim, _ = L.MAP_DIMS[kind]
+            seeds = ctx.seeds if _internal.backward_settings.seeds == "replay" else _internal.get_seeds()
+            launch = L.launch(seeds, bw_mode=_bw_mode(), seed_mode=L.VR_SEED_MAP)
+            og = og.to(torch.float32).contiguous()
+            L.check(L.load().vr_map_bw(lw.medium, kind, L.ptr(xc), L.ptr(og), xc.numel() // in_dim, launch,
+                                       L.stream_handle(xc.device)), "vr_map_bw")
+        return (None, None, *grads)
+
+
+class AutogradCaptureFunction(torch.autograd.Function):
+    """Camera capture of a field (_maps.py:1616-1642).
+
+    The backward returns fresh gradients for the trigger parameters; torch adds
+    them into .grad once (the reference also accumulates them in place,
+    Appendix A-7 of the survey of its backward).
+    """
+
+    @staticmethod
+    def forward(ctx, sensors, batch_size, fw_samples, bw_samples, field, capture_object, debug_out, *parameters):
+        with _ENGINE_LOCK:
+            kind, lw = field._lower_map()
+            in_dim, out_dim = L.MAP_DIMS[kind]
+            if in_dim != 6:
+                raise NotImplementedError("only 6-input fields can be captured")
+            cam, keep = capture_object._camera(sensors)
+            total = cam.num_sensors if sensors is not None else capture_object.num_sensors()
+            shape = (*sensors.shape[:-1],) if sensors is not None else tuple(capture_object.index_shape)
+            out = torch.empty(*shape, out_dim, device=capture_object.poses.device, dtype=torch.float32)
+            seeds = _internal.get_seeds()
+            bs = total if batch_size is None else int(batch_size)
+            lib = L.load()
+            for start in range(0, total, max(1, bs)):
+                launch = L.launch(seeds, samples=fw_samples, bw_mode=_bw_mode(), start_index=start,
+                                  count=min(bs, total - start), debug_out=debug_out)
+                L.check(lib.vr_capture_fw(lw.medium, kind, cam, launch, L.ptr(out), L.stream_handle(out.device)),
+                        "vr_capture_fw")
+        ctx.field = field
+        ctx.capture_object = capture_object
+        ctx.sensors = sensors
+        ctx.batch_size = batch_size
+        ctx.bw_samples = bw_samples
+        ctx.fw_samples = fw_samples
+        ctx.seeds = seeds
+        ctx.n_params = len(parameters)
+        ctx.save_for_backward(*parameters)
+        return out
+
+    @staticmethod
+    def backward(ctx, og):
+        params = list(ctx.saved_tensors)
+        with _ENGINE_LOCK:
+            kind, lw = ctx.field._lower_map()
+            grads = _grad_targets(lw, params)
+            cam, keep = ctx.capture_object._camera(ctx.sensors)
+            total = cam.num_sensors if ctx.sensors is not None else ctx.capture_object.num_sensors()
+            replay = _internal.backward_settings.seeds == "replay"
+            if replay:
+                seeds, samples, seed_mode = ctx.seeds, ctx.fw_samples, L.VR_SEED_CAPTURE_FW
+            else:
+                seeds, samples, seed_mode = _internal.get_seeds(), ctx.bw_samples, L.VR_SEED_CAPTURE_BW
+            og = og.to(torch.float32).contiguous()
+            bs = total if ctx.batch_size is None else int(ctx.batch_size)
+            lib = L.load()
+            for start in range(0, total, max(1, bs)):
+                launch = L.launch(seeds, samples=samples, bw_mode=_bw_mode(), seed_mode=seed_mode,
+                                  start_index=start, count=min(bs, total - start))
+                L.check(lib.vr_capture_bw(lw.medium, kind, cam, launch, L.ptr(og), L.stream_handle(og.device)),
+                        "vr_capture_bw")
+        return (None, None, None, None, None, None, None, *grads)
+
+
+# ---------------------------------------------------------------- sensors --
+class SensorsBase(MapBase):
+    """Sensor collections with capture() (_maps.py:1690-1762)."""
+
+    def __init__(self, index_shape: typing.List[int], output_dim: int):
+        super().__init__(len(index_shape), output_dim)
+        self.index_shape = list(index_shape)
+
+    def num_sensors(self) -> int:
+        return int(np.prod(self.index_shape))
+
+    def capture(self, field: MapBase, sensors_batch: typing.Optional[torch.Tensor] = None,
+                batch_size: typing.Optional[int] = None, fw_samples: int = 1, bw_samples: int = 1,
+                debug_out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+        field = field.cast(input_dim=self.output_dim)
+        triggers = [p for p in field.parameters() if p.requires_grad]
+        return AutogradCaptureFunction.apply(sensors_batch, batch_size, fw_samples, bw_samples, field, self, debug_out,
+                                             *triggers)
+
+    def random_sensors(self, batch_size: int, out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Uniform random sensor indices [batch_size, dims] (_maps.py:1751-1762)."""
+        dims = torch.tensor(self.index_shape, dtype=torch.float64)
+        r = (torch.rand(batch_size, len(self.index_shape), dtype=torch.float64) * dims).long()
+        r = torch.minimum(r, (dims - 1).long())
+        r = r.to(self.poses.device)
+        if out is not None:
+            out.copy_(r)
+            return out
+        return r
+
+
+class PerspectiveCameraSensor(SensorsBase):
+    """Pinhole cameras from [N,9] poses (origin, direction, up) (_maps.py:1765-1786)."""
+
+    def __init__(self, width: int, height: int, poses: torch.Tensor, *, fov: float = np.pi / 4, jittered: bool = False):
+        super().__init__([len(poses), height, width], 6)
+        assert poses.shape[-1] == 9
+        self.poses = poses.to(torch.float32).contiguous()
+        self.fov = fov
+        self.znear = 0.001
+        self.width = width
+        self.height = height
+        self.generation_mode = 0 if not jittered else 1
+
+    def _camera(self, sensors: typing.Optional[torch.Tensor]):
+        c = L.VrCamera()
+        c.poses = self.poses.data_ptr()
+        c.num_poses = self.poses.shape[0]
+        c.width = self.width
+        c.height = self.height
+        c.jittered = self.generation_mode
+        c.fov = self.fov
+        c.znear = self.znear
+        keep = [self.poses]
+        if sensors is not None:
+            s = sensors.to(torch.int64).contiguous()
+            assert s.shape[-1] == 3, "sensor indices are (pose, row, col)"
+            c.sensors = s.data_ptr()
+            c.num_sensors = s.numel() // 3
+            keep.append(s)
+        else:
+            c.sensors = None
+            c.num_sensors = 0
+        return c, keep

@@ -1,0 +1,338 @@
+"""Shared fixtures: build the same medium / camera / launch for the oracle
+(host numpy buffers) and for the HIP library (device torch tensors)."""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+import sys
+import typing
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from rendervous_amd import _lib as L  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+@dataclasses.dataclass
+class Scene:
+    sigma: np.ndarray                      # [D,H,W,1] float32
+    bmin: tuple = (-0.5, -0.5, -0.5)
+    bmax: tuple = (0.5, 0.5, 0.5)
+    box_min: typing.Optional[tuple] = None  # boundary box (defaults to the grid box)
+    box_max: typing.Optional[tuple] = None
+    albedo_c: tuple = (0.9, 0.9, 0.9)
+    albedo: typing.Optional[np.ndarray] = None  # [D,H,W,3] or [D,H,W,1]
+    emission_c: tuple = (0.0, 0.0, 0.0)
+    env_c: tuple = (1.0, 1.0, 1.0)
+    sun_dir: tuple = (0.0, 1.0, 0.0)
+    sun_radiance: tuple = (0.0, 0.0, 0.0)
+    sun_prob: float = 0.0
+    phase_kind: int = L.VR_PHASE_HG
+    phase_g: float = 0.0
+    majorant: str = "const"                # "const" or "grid"
+    majorant_value: typing.Optional[float] = None
+    majorant_distance: float = 1e6
+    majorant_res: tuple = (4, 4, 4)
+    transmittance_kind: int = L.VR_T_DT
+    ds_epsilon: float = 0.1
+    max_bounces: int = -1
+    medium_filter: int = 3
+    gathering_c: tuple = (0.0, 0.0, 0.0)
+    medium_majorant: float = 1.0
+
+
+class Built:
+    """A vr_medium plus the buffers it points to (and gradient buffers)."""
+
+    def __init__(self, scene: Scene, device: str, grad_sigma: bool = False, grad_albedo: bool = False):
+        self.scene = scene
+        self.device = device
+        self.keep = []
+        m = L.VrMedium()
+        sig = self._buf(scene.sigma)
+        self.sigma = sig
+        self.sigma_grad = self._zeros_like(sig) if grad_sigma else None
+        m.sigma = L.VrGrid()
+        m.sigma.t = self._view(sig, self.sigma_grad)
+        m.sigma.bmin = L.f3(scene.bmin)
+        m.sigma.bmax = L.f3(scene.bmax)
+        m.box_min = L.f3(scene.box_min or scene.bmin)
+        m.box_max = L.f3(scene.box_max or scene.bmax)
+        self.albedo_grad = None
+        if scene.albedo is not None:
+            a = self._buf(scene.albedo)
+            self.albedo = a
+            self.albedo_grad = self._zeros_like(a) if grad_albedo else None
+            m.albedo_kind = L.VR_FIELD_GRID
+            m.albedo = L.VrGrid()
+            m.albedo.t = self._view(a, self.albedo_grad)
+            m.albedo.bmin = L.f3(scene.bmin)
+            m.albedo.bmax = L.f3(scene.bmax)
+        else:
+            m.albedo_kind = L.VR_FIELD_CONST
+        m.albedo_c = L.f3(scene.albedo_c)
+        m.emission_c = L.f3(scene.emission_c)
+        m.env_c = L.f3(scene.env_c)
+        m.sun_dir = L.f3(scene.sun_dir)
+        m.sun_radiance = L.f3(scene.sun_radiance)
+        m.sun_prob = scene.sun_prob
+        m.phase_kind = scene.phase_kind
+        m.phase_g = scene.phase_g
+        if scene.majorant == "grid":
+            res = np.zeros(scene.majorant_res, dtype=np.float32)
+            g = L.VrGrid()
+            g.t = L.VrTensor()
+            g.t = _np_view(scene.sigma)
+            g.bmin = L.f3(scene.bmin)
+            g.bmax = L.f3(scene.bmax)
+            r = (L.ctypes.c_int32 * 3)(*scene.majorant_res)
+            O.check(O.load().oracle_majorant_build(g, r, O.nptr(res)), "majorant_build")
+            self.majorant_grid_np = res
+            mg = self._buf(res)
+            m.majorant_kind = L.VR_MAJ_GRID
+            m.majorant_grid = self._addr(mg)
+            for i in range(3):
+                m.majorant_res[i] = scene.majorant_res[i]
+        else:
+            m.majorant_kind = L.VR_MAJ_CONST
+            m.majorant_value = float(scene.sigma.max()) if scene.majorant_value is None else scene.majorant_value
+            m.majorant_distance = scene.majorant_distance
+        m.transmittance_kind = scene.transmittance_kind
+        m.ds_epsilon = scene.ds_epsilon
+        m.max_bounces = scene.max_bounces
+        m.medium_filter = scene.medium_filter
+        m.gathering_c = L.f3(scene.gathering_c)
+        m.medium_majorant = scene.medium_majorant
+        self.medium = m
+
+    def _buf(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if self.device == "cpu":
+            self.keep.append(a)
+            return a
+        t = torch.from_numpy(a).to(self.device)
+        self.keep.append(t)
+        return t
+
+    def _zeros_like(self, b):
+        z = np.zeros_like(b) if self.device == "cpu" else torch.zeros_like(b)
+        self.keep.append(z)
+        return z
+
+    def _addr(self, b) -> int:
+        return O.nptr(b) if self.device == "cpu" else b.data_ptr()
+
+    def _view(self, b, g) -> L.VrTensor:
+        v = L.VrTensor()
+        v.data = self._addr(b)
+        v.grad = None if g is None else self._addr(g)
+        v.ndim = 4
+        for i in range(4):
+            v.shape[i] = b.shape[i]
+            v.stride[i] = int(np.prod(b.shape[i + 1:]))
+        return v
+
+    def grads(self):
+        def host(x):
+            if x is None:
+                return None
+            return x.copy() if self.device == "cpu" else x.detach().cpu().numpy()
+        return host(self.sigma_grad), host(self.albedo_grad)
+
+
+def _np_view(a: np.ndarray) -> L.VrTensor:
+    v = L.VrTensor()
+    v.data = O.nptr(a)
+    v.grad = None
+    v.ndim = 4
+    for i in range(4):
+        v.shape[i] = a.shape[i]
+        v.stride[i] = int(np.prod(a.shape[i + 1:]))
+    return v
+
+
+def np_grid(a: np.ndarray, bmin=(-0.5, -0.5, -0.5), bmax=(0.5, 0.5, 0.5), grad: typing.Optional[np.ndarray] = None) -> L.VrGrid:
+    g = L.VrGrid()
+    g.t = _np_view(a)
+    if grad is not None:
+        g.t.grad = O.nptr(grad)
+    g.bmin = L.f3(bmin)
+    g.bmax = L.f3(bmax)
+    return g
+
+
+class Camera:
+    def __init__(self, poses: np.ndarray, width: int, height: int, jittered: bool = False, fov: float = math.pi / 4,
+                 sensors: typing.Optional[np.ndarray] = None, device: str = "cpu"):
+        self.keep = []
+        c = L.VrCamera()
+        p = np.ascontiguousarray(poses, dtype=np.float32)
+        if device == "cpu":
+            self.keep.append(p)
+            c.poses = O.nptr(p)
+        else:
+            t = torch.from_numpy(p).to(device)
+            self.keep.append(t)
+            c.poses = t.data_ptr()
+        c.num_poses = p.shape[0]
+        c.width = width
+        c.height = height
+        c.jittered = int(jittered)
+        c.fov = fov
+        c.znear = 0.001
+        if sensors is not None:
+            s = np.ascontiguousarray(sensors, dtype=np.int64)
+            if device == "cpu":
+                self.keep.append(s)
+                c.sensors = O.nptr(s)
+            else:
+                t = torch.from_numpy(s).to(device)
+                self.keep.append(t)
+                c.sensors = t.data_ptr()
+            c.num_sensors = s.shape[0]
+        self.c = c
+        self.threads = sensors.shape[0] if sensors is not None else p.shape[0] * width * height
+
+
+def pose(origin, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    o = np.asarray(origin, dtype=np.float64)
+    d = np.asarray(target, dtype=np.float64) - o
+    d /= np.linalg.norm(d)
+    return np.array([*o, *d, *up], dtype=np.float32)
+
+
+def blob_grid(n: int, seed: int = 0, max_value: float = 20.0) -> np.ndarray:
+    from rendervous_amd.functions import gaussian_blobs
+    return gaussian_blobs(n, seed=seed, max_value=max_value).numpy()
+
+
+def fbm_grid(n: int, seed: int = 0, scale: float = 40.0) -> np.ndarray:
+    from rendervous_amd.functions import fbm_value_noise
+    return fbm_value_noise(n, seed=seed, scale=scale).numpy()
+
+
+def random_rays(n: int, seed: int = 0, radius: float = 1.2, inside: bool = False) -> np.ndarray:
+    """Rays [n,6]: origins on a sphere (or inside the box) aimed near the centre."""
+    rng = np.random.default_rng(seed)
+    if inside:
+        o = rng.uniform(-0.45, 0.45, size=(n, 3))
+        d = rng.normal(size=(n, 3))
+    else:
+        u = rng.normal(size=(n, 3))
+        o = u / np.linalg.norm(u, axis=1, keepdims=True) * radius
+        t = rng.uniform(-0.3, 0.3, size=(n, 3))
+        d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], axis=1).astype(np.float32)
+
+
+def oracle_map_fw(scene: Scene, kind: int, rays: np.ndarray, seeds, bw_mode=L.VR_BW_REFERENCE) -> np.ndarray:
+    b = Built(scene, "cpu")
+    ind, outd = L.MAP_DIMS[kind]
+    x = np.ascontiguousarray(rays, dtype=np.float32)
+    out = np.zeros((x.shape[0], outd), dtype=np.float32)
+    lau = L.launch(seeds, bw_mode=bw_mode, seed_mode=L.VR_SEED_MAP)
+    O.check(O.load().oracle_map_fw(b.medium, kind, O.nptr(x), x.shape[0], O.nptr(out), lau), "oracle_map_fw")
+    return out
+
+
+def oracle_capture_fw(scene: Scene, kind: int, cam_args: dict, seeds, spp: int, bw_mode=L.VR_BW_REFERENCE,
+                      debug: bool = False):
+    b = Built(scene, "cpu")
+    cam = Camera(device="cpu", **cam_args)
+    _, outd = L.MAP_DIMS[kind]
+    out = np.zeros((cam.threads, outd), dtype=np.float32)
+    dbg = np.zeros(cam.threads, dtype=np.int32) if debug else None
+    lau = L.launch(seeds, samples=spp, bw_mode=bw_mode)
+    if dbg is not None:
+        lau.debug_out = O.nptr(dbg)
+    O.check(O.load().oracle_capture_fw(b.medium, kind, cam.c, lau, O.nptr(out)), "oracle_capture_fw")
+    return (out, dbg) if debug else out
+
+
+def oracle_capture_bw(scene: Scene, kind: int, cam_args: dict, seeds, spp: int, out_grad: np.ndarray,
+                      bw_mode=L.VR_BW_CONSISTENT, seed_mode=L.VR_SEED_CAPTURE_FW, grad_albedo=False):
+    b = Built(scene, "cpu", grad_sigma=True, grad_albedo=grad_albedo)
+    cam = Camera(device="cpu", **cam_args)
+    og = np.ascontiguousarray(out_grad, dtype=np.float32)
+    lau = L.launch(seeds, samples=spp, bw_mode=bw_mode, seed_mode=seed_mode)
+    O.check(O.load().oracle_capture_bw(b.medium, kind, cam.c, lau, O.nptr(og)), "oracle_capture_bw")
+    return b.grads()
+
+
+# ---------------------------------------------------------------- GPU side --
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def gpu_map_fw(scene: Scene, kind: int, rays: np.ndarray, seeds, bw_mode=L.VR_BW_REFERENCE) -> np.ndarray:
+    b = Built(scene, "cuda")
+    _, outd = L.MAP_DIMS[kind]
+    x = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).cuda()
+    out = torch.zeros(x.shape[0], outd, device="cuda")
+    lau = L.launch(seeds, bw_mode=bw_mode, seed_mode=L.VR_SEED_MAP)
+    L.check(L.load().vr_map_fw(b.medium, kind, x.data_ptr(), x.shape[0], out.data_ptr(), lau, _stream()), "vr_map_fw")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def gpu_map_bw(scene: Scene, kind: int, rays: np.ndarray, out_grad: np.ndarray, seeds, bw_mode=L.VR_BW_REFERENCE,
+               grad_albedo=False):
+    b = Built(scene, "cuda", grad_sigma=True, grad_albedo=grad_albedo)
+    x = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).cuda()
+    og = torch.from_numpy(np.ascontiguousarray(out_grad, dtype=np.float32)).cuda()
+    lau = L.launch(seeds, bw_mode=bw_mode, seed_mode=L.VR_SEED_MAP)
+    L.check(L.load().vr_map_bw(b.medium, kind, x.data_ptr(), og.data_ptr(), x.shape[0], lau, _stream()), "vr_map_bw")
+    torch.cuda.synchronize()
+    return b.grads()
+
+
+def oracle_map_bw(scene: Scene, kind: int, rays: np.ndarray, out_grad: np.ndarray, seeds, bw_mode=L.VR_BW_REFERENCE,
+                  grad_albedo=False):
+    b = Built(scene, "cpu", grad_sigma=True, grad_albedo=grad_albedo)
+    x = np.ascontiguousarray(rays, dtype=np.float32)
+    og = np.ascontiguousarray(out_grad, dtype=np.float32)
+    lau = L.launch(seeds, bw_mode=bw_mode, seed_mode=L.VR_SEED_MAP)
+    O.check(O.load().oracle_map_bw(b.medium, kind, O.nptr(x), O.nptr(og), x.shape[0], lau), "oracle_map_bw")
+    return b.grads()
+
+
+def gpu_capture_fw(scene: Scene, kind: int, cam_args: dict, seeds, spp: int, bw_mode=L.VR_BW_REFERENCE, debug=False):
+    b = Built(scene, "cuda")
+    cam = Camera(device="cuda", **cam_args)
+    _, outd = L.MAP_DIMS[kind]
+    out = torch.zeros(cam.threads, outd, device="cuda")
+    dbg = torch.zeros(cam.threads, dtype=torch.int32, device="cuda") if debug else None
+    lau = L.launch(seeds, samples=spp, bw_mode=bw_mode, debug_out=dbg)
+    L.check(L.load().vr_capture_fw(b.medium, kind, cam.c, lau, out.data_ptr(), _stream()), "vr_capture_fw")
+    torch.cuda.synchronize()
+    if debug:
+        return out.cpu().numpy(), dbg.cpu().numpy()
+    return out.cpu().numpy()
+
+
+def gpu_capture_bw(scene: Scene, kind: int, cam_args: dict, seeds, spp: int, out_grad: np.ndarray,
+                   bw_mode=L.VR_BW_CONSISTENT, seed_mode=L.VR_SEED_CAPTURE_FW, grad_albedo=False):
+    b = Built(scene, "cuda", grad_sigma=True, grad_albedo=grad_albedo)
+    cam = Camera(device="cuda", **cam_args)
+    og = torch.from_numpy(np.ascontiguousarray(out_grad, dtype=np.float32)).cuda()
+    lau = L.launch(seeds, samples=spp, bw_mode=bw_mode, seed_mode=seed_mode)
+    L.check(L.load().vr_capture_bw(b.medium, kind, cam.c, lau, og.data_ptr(), _stream()), "vr_capture_bw")
+    torch.cuda.synchronize()
+    return b.grads()
+
+
+def match_fraction(a: np.ndarray, b: np.ndarray, rtol=1e-3, atol=1e-5) -> float:
+    """Fraction of rows whose entries all agree within tolerance."""
+    ok = np.all(np.abs(a - b) <= atol + rtol * np.abs(b), axis=-1)
+    return float(ok.mean())
+
+
+def rel_l1(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-30))

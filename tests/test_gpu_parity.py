@@ -1,0 +1,235 @@
+"""Parity of the HIP kernels (through the C ABI) with the CPU oracle.
+
+Deterministic paths (Grid3D, ray-box, DDA, majorant build, Adam) are compared
+value by value.  Stochastic estimators use the same dispatch seeds on both
+sides, so the GPU draws the same HybridTaus streams as the oracle: outputs are
+compared sample by sample (a path diverges only when a float comparison falls
+within rounding distance, so a small fraction of mismatches is allowed), and
+the gradients of seed-matched backward passes are compared as grids.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import (L, O, Scene, blob_grid, fbm_grid, gpu_capture_bw, gpu_capture_fw, gpu_map_bw, gpu_map_fw,
+                     match_fraction, np_grid, oracle_capture_bw, oracle_capture_fw, oracle_map_bw, oracle_map_fw,
+                     pose, random_rays, rel_l1)
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = (12345, 67890, 13579, 24680)
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+# ------------------------------------------------------------- Grid3D --
+@pytest.mark.parametrize("channels", [1, 3])
+def test_grid_lookup_matches_fp64_oracle(gpu, channels):
+    _print_case("grid_lookup", channels=channels)
+    rng = np.random.default_rng(1)
+    g = rng.uniform(0, 20, size=(9, 11, 13, channels)).astype(np.float32)
+    x = rng.uniform(-0.6, 0.6, size=(4096, 3)).astype(np.float32)  # includes points outside the box
+    ref = np.zeros((4096, channels), np.float64)
+    O.check(O.load().oracle_grid_fw_f64(np_grid(g), O.nptr(x), 4096, O.nptr(ref)), "grid_fw_f64")
+    gt = torch.from_numpy(g).cuda()
+    xt = torch.from_numpy(x).cuda()
+    out = torch.zeros(4096, channels, device="cuda")
+    vg = L.grid_view(gt, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    L.check(L.load().vr_grid_fw(vg, xt.data_ptr(), 4096, out.data_ptr(), None), "vr_grid_fw")
+    torch.cuda.synchronize()
+    out = out.cpu().numpy().astype(np.float64)
+    # FP32 coordinates: tolerance 1e-5 of the value plus 1e-5 of the grid's range
+    err = np.abs(out - ref)
+    assert np.all(err <= 1e-5 * np.abs(ref) + 1e-5 * 20.0), float(err.max())
+
+
+def test_grid_scatter_matches_oracle(gpu):
+    _print_case("grid_scatter")
+    rng = np.random.default_rng(2)
+    g = rng.uniform(0, 1, size=(6, 7, 8, 1)).astype(np.float32)
+    x = rng.uniform(-0.5, 0.5, size=(2048, 3)).astype(np.float32)
+    og = rng.normal(size=(2048, 1)).astype(np.float32)
+    ref = np.zeros_like(g)
+    O.check(O.load().oracle_grid_bw(np_grid(g, grad=ref), O.nptr(x), O.nptr(og), 2048), "grid_bw")
+    gt = torch.from_numpy(g).cuda()
+    grad = torch.zeros_like(gt)
+    xt, ogt = torch.from_numpy(x).cuda(), torch.from_numpy(og).cuda()  # keep both alive across the launch
+    vg = L.grid_view(gt, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), grad)
+    L.check(L.load().vr_grid_bw(vg, xt.data_ptr(), ogt.data_ptr(), 2048, None), "vr_grid_bw")
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(grad.cpu().numpy(), ref, rtol=1e-4, atol=1e-4)
+
+
+def test_majorant_build_matches_oracle(gpu):
+    _print_case("majorant_build")
+    g = fbm_grid(33, seed=3)
+    res = (4, 5, 6)
+    ref = np.zeros(res, np.float32)
+    r = (L.ctypes.c_int32 * 3)(*res)
+    O.check(O.load().oracle_majorant_build(np_grid(g), r, O.nptr(ref)), "majorant")
+    gt = torch.from_numpy(g).cuda()
+    out = torch.zeros(res, device="cuda")
+    L.check(L.load().vr_majorant_build(L.grid_view(gt, (-0.5,) * 3, (0.5,) * 3), r, out.data_ptr(), None), "vr_majorant_build")
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+# --------------------------------------------------------------- DDA --
+def test_dda_transmittance_matches_fp64_oracle(gpu):
+    _print_case("dda_fw")
+    sc = Scene(sigma=blob_grid(64, seed=0, max_value=20.0))
+    rays = random_rays(4096, seed=4)
+    ref = oracle_map_fw(sc, L.VR_MAP_T_DDA, rays, SEEDS)
+    out = gpu_map_fw(sc, L.VR_MAP_T_DDA, rays, SEEDS)
+    rel = np.abs(out - ref) / np.maximum(ref, 1e-30)
+    assert np.all((rel <= 1e-5) | (np.abs(out - ref) <= 1e-9)), float(rel.max())
+
+
+def test_dda_capture_config1(gpu):
+    """64^3 blobs, 128x128 pinhole camera at (0,0,2.5), 1 ray/pixel."""
+    _print_case("dda_capture")
+    sc = Scene(sigma=blob_grid(64, seed=0, max_value=20.0))
+    cam = dict(poses=pose((0.0, 0.0, 2.5))[None], width=128, height=128)
+    ref = oracle_capture_fw(sc, L.VR_MAP_T_DDA, cam, SEEDS, 1)
+    out = gpu_capture_fw(sc, L.VR_MAP_T_DDA, cam, SEEDS, 1)
+    rel = np.abs(out - ref) / np.maximum(ref, 1e-30)
+    assert np.all((rel <= 1e-5) | (np.abs(out - ref) <= 1e-9)), float(rel.max())
+
+
+def test_dda_backward_matches_oracle(gpu):
+    _print_case("dda_bw")
+    sc = Scene(sigma=blob_grid(16, seed=1, max_value=5.0))
+    rays = random_rays(1024, seed=5)
+    og = np.random.default_rng(6).normal(size=(1024, 1)).astype(np.float32)
+    ref, _ = oracle_map_bw(sc, L.VR_MAP_T_DDA, rays, og, SEEDS)
+    out, _ = gpu_map_bw(sc, L.VR_MAP_T_DDA, rays, og, SEEDS)
+    np.testing.assert_allclose(out, ref, rtol=1e-4, atol=1e-5 * np.abs(ref).max())
+
+
+# ------------------------------------------------------- stochastic fw --
+@pytest.mark.parametrize("majorant", ["const", "grid"])
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_COLLISION_DT])
+def test_trackers_seed_matched(gpu, kind, majorant):
+    _print_case("trackers", kind=kind, majorant=majorant)
+    sc = Scene(sigma=fbm_grid(24, seed=7, scale=30.0), majorant=majorant, majorant_res=(4, 4, 4))
+    rays = random_rays(8192, seed=8)
+    ref = oracle_map_fw(sc, kind, rays, SEEDS)
+    out = gpu_map_fw(sc, kind, rays, SEEDS)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.995
+
+
+def test_collision_sampler_pins_rng_stream(gpu):
+    """Homogeneous medium at the majorant: the first tentative collision is real, so
+    t = tMin - log(1-u)/maj exposes the first uniform of every thread's stream."""
+    _print_case("rng_stream")
+    sc = Scene(sigma=np.full((4, 4, 4, 1), 3.0, np.float32))
+    rays = np.tile(np.array([[0, 0, -2, 0, 0, 1]], np.float32), (4096, 1))
+    ref = oracle_map_fw(sc, L.VR_MAP_COLLISION_DT, rays, SEEDS)
+    out = gpu_map_fw(sc, L.VR_MAP_COLLISION_DT, rays, SEEDS)
+    np.testing.assert_array_equal(out[:, 0], ref[:, 0])
+    np.testing.assert_allclose(out[:, 1], ref[:, 1], rtol=2e-6, atol=2e-6)
+
+
+@pytest.mark.parametrize("majorant", ["const", "grid"])
+@pytest.mark.parametrize("tkind", [L.VR_T_DT, L.VR_T_RT])
+def test_nee_capture_seed_matched(gpu, majorant, tkind):
+    _print_case("nee_capture", majorant=majorant, tkind=tkind)
+    sc = Scene(sigma=fbm_grid(24, seed=9, scale=20.0), albedo_c=(0.9, 0.8, 0.7), phase_kind=L.VR_PHASE_HG_MIX,
+               phase_g=0.6, env_c=(0.1, 0.1, 0.1), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(3.0, 3.0, 3.0), sun_prob=0.5,
+               majorant=majorant, transmittance_kind=tkind, max_bounces=16)
+    cam = dict(poses=pose((0.0, 0.3, 2.5))[None], width=48, height=48, jittered=True)
+    ref = oracle_capture_fw(sc, L.VR_MAP_NEE_DT, cam, SEEDS, 1)
+    out = gpu_capture_fw(sc, L.VR_MAP_NEE_DT, cam, SEEDS, 1)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.98
+
+
+@pytest.mark.parametrize("kind", [L.VR_MAP_SS_RADIANCE, L.VR_MAP_MC_COLLISION])
+def test_single_scatter_seed_matched(gpu, kind):
+    _print_case("single_scatter", kind=kind)
+    sc = Scene(sigma=fbm_grid(24, seed=11, scale=20.0), albedo_c=(0.9, 0.9, 0.9), phase_kind=L.VR_PHASE_HG_MIX,
+               phase_g=0.5, env_c=(1.0, 0.5, 0.25), transmittance_kind=L.VR_T_RT)
+    rays = random_rays(4096, seed=12, inside=(kind == L.VR_MAP_SS_RADIANCE))
+    ref = oracle_map_fw(sc, kind, rays, SEEDS)
+    out = gpu_map_fw(sc, kind, rays, SEEDS)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.99
+
+
+def test_medium_dt_seed_matched(gpu):
+    _print_case("medium_dt")
+    sc = Scene(sigma=fbm_grid(16, seed=13, scale=10.0), albedo_c=(0.7, 0.8, 0.9), phase_kind=L.VR_PHASE_HG,
+               phase_g=0.3, medium_filter=3, gathering_c=(0.2, 0.3, 0.4), medium_majorant=10.0)
+    r = random_rays(4096, seed=14, inside=True)
+    d = np.random.default_rng(15).uniform(0.0, 0.5, size=(4096, 1)).astype(np.float32)
+    seg = np.concatenate([r, d], axis=1)
+    ref = oracle_map_fw(sc, L.VR_MAP_MEDIUM_DT, seg, SEEDS)
+    out = gpu_map_fw(sc, L.VR_MAP_MEDIUM_DT, seg, SEEDS)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.99
+
+
+# ------------------------------------------------------- backward --
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_COLLISION_DT, L.VR_MAP_MC_COLLISION,
+                                  L.VR_MAP_SS_RADIANCE, L.VR_MAP_NEE_DT])
+@pytest.mark.parametrize("bw_mode", [L.VR_BW_REFERENCE, L.VR_BW_CONSISTENT])
+def test_map_backward_seed_matched(gpu, kind, bw_mode):
+    _print_case("map_bw", kind=kind, bw_mode=bw_mode)
+    sc = Scene(sigma=fbm_grid(12, seed=16, scale=8.0), albedo_c=(0.9, 0.9, 0.9), phase_kind=L.VR_PHASE_HG_MIX,
+               phase_g=0.4, env_c=(1.0, 1.0, 1.0), transmittance_kind=L.VR_T_RT, ds_epsilon=0.1, max_bounces=8)
+    rays = random_rays(2048, seed=17, inside=(kind == L.VR_MAP_SS_RADIANCE))
+    _, od = L.MAP_DIMS[kind]
+    og = np.random.default_rng(18).uniform(0.5, 1.5, size=(2048, od)).astype(np.float32)
+    ref, _ = oracle_map_bw(sc, kind, rays, og, SEEDS, bw_mode=bw_mode)
+    out, _ = gpu_map_bw(sc, kind, rays, og, SEEDS, bw_mode=bw_mode)
+    assert rel_l1(out, ref) <= 0.03
+
+
+def test_medium_dt_backward_seed_matched(gpu):
+    _print_case("medium_dt_bw")
+    sc = Scene(sigma=fbm_grid(12, seed=19, scale=6.0), albedo=np.full((12, 12, 12, 3), 0.8, np.float32),
+               phase_kind=L.VR_PHASE_HG, phase_g=0.3, medium_filter=3, gathering_c=(0.2, 0.3, 0.4), medium_majorant=6.0)
+    r = random_rays(2048, seed=20, inside=True)
+    d = np.random.default_rng(21).uniform(0.0, 0.5, size=(2048, 1)).astype(np.float32)
+    seg = np.concatenate([r, d], axis=1)
+    og = np.random.default_rng(22).uniform(0.5, 1.5, size=(2048, 12)).astype(np.float32)
+    ref_s, ref_a = oracle_map_bw(sc, L.VR_MAP_MEDIUM_DT, seg, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT, grad_albedo=True)
+    out_s, out_a = gpu_map_bw(sc, L.VR_MAP_MEDIUM_DT, seg, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT, grad_albedo=True)
+    assert rel_l1(out_s, ref_s) <= 0.03
+    assert rel_l1(out_a, ref_a) <= 0.03
+
+
+@pytest.mark.parametrize("kind", [L.VR_MAP_MC_COLLISION, L.VR_MAP_NEE_DT, L.VR_MAP_T_DDA])
+def test_capture_backward_seed_matched(gpu, kind):
+    _print_case("capture_bw", kind=kind)
+    sc = Scene(sigma=fbm_grid(16, seed=23, scale=10.0), albedo=np.full((16, 16, 16, 1), 0.85, np.float32),
+               phase_kind=L.VR_PHASE_HG, phase_g=0.5, env_c=(1.0, 1.0, 1.0), transmittance_kind=L.VR_T_RT,
+               max_bounces=4)
+    cam = dict(poses=pose((0.2, 0.3, 2.5))[None], width=24, height=24, jittered=True)
+    _, od = L.MAP_DIMS[kind]
+    og = np.random.default_rng(24).uniform(0.5, 1.5, size=(24 * 24, od)).astype(np.float32)
+    ref_s, ref_a = oracle_capture_bw(sc, kind, cam, SEEDS, 2, og, grad_albedo=True)
+    out_s, out_a = gpu_capture_bw(sc, kind, cam, SEEDS, 2, og, grad_albedo=True)
+    assert rel_l1(out_s, ref_s) <= 0.03
+    if kind != L.VR_MAP_T_DDA:
+        assert rel_l1(out_a, ref_a) <= 0.03
+
+
+# ---------------------------------------------------------------- Adam --
+def test_adam_matches_oracle(gpu):
+    _print_case("adam")
+    rng = np.random.default_rng(25)
+    n = 10007
+    p, g, m, v = (rng.normal(size=n).astype(np.float32) for _ in range(4))
+    v = np.abs(v)
+    rp, rg, rm, rv = p.copy(), g.copy(), m.copy(), v.copy()
+    O.check(O.load().oracle_adam_step(O.nptr(rp), O.nptr(rg), O.nptr(rm), O.nptr(rv), n, 1e-2, 0.9, 0.999, 1e-8, 3, 1), "adam")
+    tp, tg, tm, tv = (torch.from_numpy(a.copy()).cuda() for a in (p, g, m, v))
+    L.check(L.load().vr_adam_step(tp.data_ptr(), tg.data_ptr(), tm.data_ptr(), tv.data_ptr(), n, 1e-2, 0.9, 0.999, 1e-8, 3, 1,
+                                  None), "vr_adam_step")
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(tp.cpu().numpy(), rp, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(tm.cpu().numpy(), rm, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(tv.cpu().numpy(), rv, rtol=1e-5, atol=1e-7)
+    assert float(tg.abs().max()) == 0.0
