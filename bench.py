@@ -64,22 +64,23 @@ def cpu_baseline(sigma_cpu, radiance, width, height, spp, budget_s: float = 15.0
                majorant="grid", majorant_res=(16, 16, 16), transmittance_kind=m.transmittance_kind,
                max_bounces=m.max_bounces)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # bounded sample: pixels on every `step`-th row, grown until the budget is used
+    # bounded sample: full rows spread evenly over the frame, doubled until one
+    # run takes at least a quarter of the budget (or the whole frame is used)
     rows = 1
     done_samples, elapsed = 0, 0.0
     p = pose((0.0, 0.3, 2.5))[None]
-    while elapsed < budget_s and rows <= height:
-        ys = np.linspace(0, height - 1, rows).astype(np.int64)
+    while True:
+        ys = np.unique(np.linspace(0, height - 1, rows).astype(np.int64))
         sensors = np.stack(np.meshgrid(np.zeros(1, np.int64), ys, np.arange(width, dtype=np.int64), indexing="ij"),
                            -1).reshape(-1, 3)
         t0 = time.perf_counter()
         oracle_capture_fw(sc, L.VR_MAP_NEE_DT, dict(poses=p, width=width, height=height, jittered=True,
                                                      sensors=sensors), (11, 22, 33, 44), spp)
         dt = time.perf_counter() - t0
-        done_samples, elapsed = sensors.shape[0] * spp, dt
-        if dt > budget_s / 4:
+        done_samples, elapsed, rows = sensors.shape[0] * spp, dt, len(ys)
+        if dt > budget_s / 4 or rows >= height:
             break
-        rows *= 2
+        rows = min(height, rows * 2)
     return {"value": done_samples / elapsed, "unit": "camera samples/s", "cores": threads, "kind": "port",
             "sample": f"{done_samples // spp} pixels ({rows} rows x {width}) of the same frame at {spp} spp, "
                       f"{elapsed:.1f} s, C oracle with OpenMP"}
@@ -96,6 +97,8 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -108,6 +111,13 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.calibrate_copy:
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)  # reads and writes exactly 256 MiB
+        torch.cuda.synchronize()
+        del a, b
     log(f"building {args.res}^3 fBm scene on {torch.cuda.get_device_name(dev)}")
     sigma, radiance, camera = build_scene(args.res, args.width, args.height, dev)
     torch.cuda.synchronize()

@@ -13,8 +13,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <mutex>
 
 #include "vr_device.hpp"
+#include "vr_nee_wavefront.hpp"
 
 using namespace vr;
 
@@ -375,6 +378,61 @@ __global__ void __launch_bounds__(kBlock) k_adam(float* __restrict__ p, float* _
     }
 }
 
+// --------------------------------------------------- persistent NEE queue --
+// One 64-bit work counter per device, allocated on first use and kept for the
+// life of the process.  Launches on one device are ordered by their streams;
+// two captures running concurrently on different streams of one device would
+// share the counter, so the map layer serialises captures.
+constexpr int kMaxDevices = 64;
+std::mutex g_counter_mu;
+unsigned long long* g_counters[kMaxDevices] = {};
+
+int work_counter(unsigned long long** out) {
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st) return st;
+    if (dev < 0 || dev >= kMaxDevices) return set_err(VR_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_counter_mu);
+    if (!g_counters[dev]) {
+        void* p = nullptr;
+        if ((st = check_hip(hipMalloc(&p, 256), "hipMalloc(work counter)"))) return st;
+        g_counters[dev] = (unsigned long long*)p;
+    }
+    *out = g_counters[dev];
+    return VR_OK;
+}
+
+template <typename K>
+int persistent_blocks(K kernel, int* blocks) {
+    int dev = 0, cus = 0, per_cu = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (!st) st = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    if (!st) st = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0), "occupancy");
+    if (st) return st;
+    *blocks = cus * (per_cu > 0 ? per_cu : 1);
+    return VR_OK;
+}
+
+template <int MAJ, int TKIND>
+int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
+                          int64_t cnt, float* out, int* dbg, hipStream_t st) {
+    unsigned long long* counter = nullptr;
+    int rc = work_counter(&counter);
+    if (rc) return rc;
+    int blocks = 0;
+    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND>, &blocks))) return rc;
+    // samples per work item: enough items for ~16 per lane in flight, at most spp
+    const int64_t lanes = (int64_t)blocks * kBlock;
+    int64_t chunk = (cnt * (int64_t)spp) / (16 * lanes);
+    chunk = chunk < 1 ? 1 : (chunk > spp ? spp : chunk);
+    unsigned prep = (unsigned)std::min<int64_t>((cnt + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, start, cnt, dbg, counter);
+    if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
+    hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, spp, (int)chunk, total,
+                       start, cnt, counter, out, dbg);
+    return check_hip(hipGetLastError(), "vr_capture_fw launch");
+}
+
 // ------------------------------------------------------------- dispatch --
 template <int MAJ>
 int launch_map_fw_maj(int kind, const MediumView& M, const float* in, int64_t start, int64_t cnt, float* out, Seeds sd, int* dbg,
@@ -409,6 +467,8 @@ template <int MAJ>
 int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
                           int64_t cnt, float* out, int* dbg, hipStream_t st) {
     dim3 g(grid_for(cnt)), b(kBlock);
+    if (kind == VR_MAP_NEE_DT && M.t_kind == VR_T_DT) return launch_nee_persistent<MAJ, VR_T_DT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+    if (kind == VR_MAP_NEE_DT && M.t_kind == VR_T_RT) return launch_nee_persistent<MAJ, VR_T_RT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
     switch (kind) {
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_fw<K, MAJ>), g, b, 0, st, M, C, sd, spp, total, start, cnt, out, dbg); break;
         VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
