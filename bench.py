@@ -86,6 +86,91 @@ def cpu_baseline(sigma_cpu, radiance, width, height, spp, budget_s: float = 15.0
                       f"{elapsed:.1f} s, C oracle with OpenMP"}
 
 
+def _events(n):
+    import torch
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+def gradient_leg(dev, res=256, views=8, hw=256, spp=16, steps=3):
+    """Inverse-render step of the 256^3 single-scatter configuration: forward
+    capture (DT collision + single scatter with RT shadow rays), L2 loss,
+    adjoint backward replaying the forward's seeds, fused Adam.  Reports grad
+    voxel-updates/s = 8 x trilinear scatters / backward time."""
+    import torch
+    import rendervous_amd as rv
+    from rendervous_amd import inverse
+
+    target_sigma = rv.functions.fbm_value_noise(res, seed=11, scale=40.0, device=dev)
+    poses = rv.functions.orbit_poses(views, radius=2.5).to(dev)
+    camera = rv.PerspectiveCameraSensor(hw, hw, poses, jittered=True)
+    rv.seed(3)
+    with torch.no_grad():
+        target = camera.capture(inverse.single_scatter_scene(target_sigma).radiance, fw_samples=spp)
+    sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
+    field = inverse.single_scatter_scene(sigma).radiance
+    opt = inverse.FusedAdam(sigma, lr=1e-2)
+    dbg = torch.zeros(views * hw * hw, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    inverse.optimisation_step(camera, field, sigma, target, spp, opt)  # warm-up
+    torch.cuda.synchronize()
+    fw_ms = bw_ms = adam_ms = 0.0
+    scatters = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e = _events(4)
+        e[0].record(stream)
+        img = camera.capture(field, fw_samples=spp, bw_samples=spp, debug_out=dbg)
+        loss = ((img - target) ** 2).mean()
+        e[1].record(stream)
+        loss.backward()
+        e[2].record(stream)
+        opt.step(sigma.grad, zero_grad=True)
+        with torch.no_grad():
+            sigma.clamp_(min=0.0)
+        e[3].record(stream)
+        torch.cuda.synchronize()
+        fw_ms += e[0].elapsed_time(e[1])
+        bw_ms += e[1].elapsed_time(e[2])
+        adam_ms += e[2].elapsed_time(e[3])
+        scatters += float(dbg.to(torch.float64).sum().item())
+    wall = (time.perf_counter() - t0) * 1000.0 / steps
+    fw_ms, bw_ms, adam_ms, scatters = fw_ms / steps, bw_ms / steps, adam_ms / steps, scatters / steps
+    samples = views * hw * hw * spp
+    return {
+        "workload": f"inverse render step: {res}^3 density, {views} views {hw}x{hw}, {spp} spp, DT collision + "
+                    f"single scatter (RT shadow rays), L2 loss, adjoint backward, fused Adam",
+        "grad_voxel_updates_per_s": 8.0 * scatters / (bw_ms * 1e-3),
+        "trilinear_scatters_per_step": scatters,
+        "fw_ms": fw_ms, "bw_ms": bw_ms, "adam_ms": adam_ms, "step_ms_wall": wall,
+        "fw_bw_samples_per_s": 2.0 * samples / ((fw_ms + bw_ms) * 1e-3),
+        "loss": float(loss.item()),
+    }
+
+
+def dda_leg(dev, reps=20):
+    """Deterministic DDA transmittance of the 64^3 Gaussian-blob grid, 128x128, 1 ray/pixel."""
+    import torch
+    import rendervous_amd as rv
+
+    sigma = rv.functions.gaussian_blobs(64, seed=0, max_value=20.0, device=dev)
+    bmin, bmax = rv.normalized_box(sigma)
+    field = rv.GridDDATransmittance(rv.Grid3D(sigma, bmin, bmax), rv.RayBoxIntersection(bmin, bmax))
+    camera = rv.PerspectiveCameraSensor(128, 128, rv.functions.look_at_pose((0.0, 0.0, 2.5))[None].to(dev))
+    stream = torch.cuda.current_stream(dev)
+    with torch.no_grad():
+        camera.capture(field)
+        torch.cuda.synchronize()
+        e = _events(2)
+        e[0].record(stream)
+        for _ in range(reps):
+            img = camera.capture(field)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+    ms = e[0].elapsed_time(e[1]) / reps
+    return {"workload": "DDA transmittance, 64^3 blobs, 128x128, 1 ray/pixel", "rays_per_s": 128 * 128 / (ms * 1e-3),
+            "ms_per_frame": ms, "mean_T": float(img.mean().item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +181,7 @@ def main():
     ap.add_argument("--height", type=int, default=512)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the gradient and DDA legs")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM byte counters)")
@@ -175,8 +261,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
+    if not args.no_secondary:
+        del img, radiance, camera
+        torch.cuda.empty_cache()
+        result["secondary"] = {"gradient": gradient_leg(dev), "dda": dda_leg(dev)}
+        log(f"secondary: {json.dumps(result['secondary'])}")
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(sigma.cpu(), radiance, args.width, args.height, args.spp, args.cpu_budget)
+        m = build_scene(args.res, args.width, args.height, dev)[1]
+        result["cpu_baseline"] = cpu_baseline(sigma.cpu(), m, args.width, args.height, args.spp, args.cpu_budget)
     print(json.dumps(result), flush=True)
 
 

@@ -161,7 +161,8 @@ typedef struct vr_launch {
     int32_t  reserved;
     int64_t  start_index;          /* first thread index of this batch (batch_size chunking) */
     int64_t  count;                /* threads in this batch; < 0 = all remaining */
-    int32_t* debug_out;            /* optional per-thread counter of trilinear lookups */
+    int32_t* debug_out;            /* optional per-thread counter: trilinear lookups (forward
+                                      entry points) or trilinear gradient scatters (backward) */
 } vr_launch;
 
 /* ---- entry points ---------------------------------------------------------- */

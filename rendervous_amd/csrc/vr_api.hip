@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <mutex>
@@ -116,8 +117,9 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
     v.maj_distance = m->majorant_distance;
     if (m->majorant_kind == VR_MAJ_GRID) {
         if (!m->majorant_grid) return set_err(VR_ERR_ARG, "majorant grid is NULL");
-        if (m->majorant_res[0] < 1 || m->majorant_res[1] < 1 || m->majorant_res[2] < 1)
-            return set_err(VR_ERR_ARG, "majorant resolution must be >= 1");
+        for (int a = 0; a < 3; a++)
+            if (m->majorant_res[a] < 1 || m->majorant_res[a] > 1023)
+                return set_err(VR_ERR_ARG, "majorant resolution must be in [1, 1023]");
         for (int a = 0; a < 3; a++)
             if (m->box_min[a] < m->sigma.bmin[a] - 1e-6f || m->box_max[a] > m->sigma.bmax[a] + 1e-6f)
                 return set_err(VR_ERR_UNSUPPORTED, "majorant grid requires the boundary box inside the sigma grid box");
@@ -126,6 +128,7 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
         v.maj_bmin = load3(m->sigma.bmin);
         v.maj_csize = F3((m->sigma.bmax[0] - m->sigma.bmin[0]) / v.mrx, (m->sigma.bmax[1] - m->sigma.bmin[1]) / v.mry,
                          (m->sigma.bmax[2] - m->sigma.bmin[2]) / v.mrz);
+        v.maj_icsize = F3(1.0f / v.maj_csize.x, 1.0f / v.maj_csize.y, 1.0f / v.maj_csize.z);
     } else if (m->majorant_kind != VR_MAJ_CONST) return set_err(VR_ERR_ARG, "unknown majorant kind");
     if (m->transmittance_kind < VR_T_DT || m->transmittance_kind > VR_T_DDA) return set_err(VR_ERR_ARG, "unknown transmittance kind");
     v.t_kind = m->transmittance_kind;
@@ -151,6 +154,8 @@ int make_camera_view(const vr_camera* c, CameraView& v) {
     v.jittered = c->jittered;
     v.fov = c->fov;
     v.znear = c->znear;
+    v.znear_over_h = c->znear / (float)c->height;
+    v.sz = c->znear / std::tan(c->fov * 0.5f);
     return VR_OK;
 }
 
@@ -337,7 +342,8 @@ __global__ void __launch_bounds__(kBlock) k_capture_fw(MediumView M, CameraView 
 
 template <int KIND, int MAJ>
 __global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView cam, Seeds seeds, int spp, int seed_mode,
-                                                       int64_t total, int64_t start, int64_t cnt, const float* __restrict__ og) {
+                                                       int64_t total, int64_t start, int64_t cnt, const float* __restrict__ og,
+                                                       int* __restrict__ dbg) {
     int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= start + cnt) return;
     constexpr int OD = Dims<KIND>::out;
@@ -348,7 +354,10 @@ __global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView 
     bool any = false;
 #pragma unroll
     for (int k = 0; k < OD; k++) { o[k] = og[i * OD + k] / (float)spp; any |= o[k] != 0.0f; }
-    if (!any) return;
+    if (!any) {
+        if (dbg) dbg[i] = 0;
+        return;
+    }
     Tracer<MAJ> tr(M);
     if (seed_mode == VR_SEED_CAPTURE_BW) tr.rng = seed_capture_bw(seeds, (uint32_t)i);
     for (int s = 0; s < spp; s++) {
@@ -358,6 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView 
         float a[6] = {x.x, x.y, x.z, w.x, w.y, w.z};
         field_bw<KIND, MAJ>(tr, a, o);
     }
+    if (dbg) dbg[i] = tr.scatters;
 }
 
 // torch.optim.Adam (no weight decay, no amsgrad), optionally clearing the gradient.
@@ -402,6 +412,12 @@ int work_counter(unsigned long long** out) {
     return VR_OK;
 }
 
+// Tuning knob read from the environment (unset: the built-in default).
+int env_int(const char* name, int fallback) {
+    const char* v = std::getenv(name);
+    return v && *v ? std::atoi(v) : fallback;
+}
+
 template <typename K>
 int persistent_blocks(K kernel, int* blocks) {
     int dev = 0, cus = 0, per_cu = 0;
@@ -409,7 +425,8 @@ int persistent_blocks(K kernel, int* blocks) {
     if (!st) st = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
     if (!st) st = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0), "occupancy");
     if (st) return st;
-    *blocks = cus * (per_cu > 0 ? per_cu : 1);
+    // resident blocks per CU from the occupancy query (VR_NEE_BLOCKS_PER_CU overrides)
+    *blocks = cus * std::max(1, env_int("VR_NEE_BLOCKS_PER_CU", per_cu));
     return VR_OK;
 }
 
@@ -421,9 +438,9 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
     if (rc) return rc;
     int blocks = 0;
     if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND>, &blocks))) return rc;
-    // samples per work item: enough items for ~16 per lane in flight, at most spp
+    // samples per work item: enough items for ~8 per lane in flight, at most spp
     const int64_t lanes = (int64_t)blocks * kBlock;
-    int64_t chunk = (cnt * (int64_t)spp) / (16 * lanes);
+    int64_t chunk = (cnt * (int64_t)spp) / (8 * lanes);
     chunk = chunk < 1 ? 1 : (chunk > spp ? spp : chunk);
     unsigned prep = (unsigned)std::min<int64_t>((cnt + kBlock - 1) / kBlock, 4096);
     hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, start, cnt, dbg, counter);
@@ -481,10 +498,10 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
 
 template <int MAJ>
 int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int seed_mode, int64_t total,
-                          int64_t start, int64_t cnt, const float* og, hipStream_t st) {
+                          int64_t start, int64_t cnt, const float* og, int* dbg, hipStream_t st) {
     dim3 g(grid_for(cnt)), b(kBlock);
     switch (kind) {
-#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_bw<K, MAJ>), g, b, 0, st, M, C, sd, spp, seed_mode, total, start, cnt, og); break;
+#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_bw<K, MAJ>), g, b, 0, st, M, C, sd, spp, seed_mode, total, start, cnt, og, dbg); break;
         VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT)
 #undef VR_CASE
@@ -620,8 +637,8 @@ int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera
     Seeds sd = make_seeds(L);
     hipStream_t s = (hipStream_t)stream;
     if (medium->majorant_kind == VR_MAJ_GRID)
-        return launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, s);
-    return launch_capture_bw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, s);
+        return launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, L->debug_out, s);
+    return launch_capture_bw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, L->debug_out, s);
 }
 
 int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps, int32_t step,

@@ -26,6 +26,20 @@ constexpr float kInf = __builtin_inff();
 // independent of which side of the tie the arithmetic lands on.
 constexpr float kCrossEps = 1e-6f;
 
+// ------------------------------------------------------- fast primitives --
+// Hardware reciprocal, reciprocal square root, log2 and sine/cosine
+// (v_rcp_f32, v_rsq_f32, v_log_f32, v_sin_f32/v_cos_f32; about 1 ulp) in place
+// of the correctly rounded division/libm sequences on the tracking paths.
+VR_DEV float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+VR_DEV float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+VR_DEV float rsqrt_fast(float x) { return __builtin_amdgcn_rsqf(x); }
+VR_DEV float neg_log1m(float u) { return -__logf(1.0f - u); }  // -log(1-u) of a free-flight draw
+// sine and cosine of 2*pi*turns
+VR_DEV void sincos_turns(float turns, float& s, float& c) {
+    s = __builtin_amdgcn_sinf(turns);
+    c = __builtin_amdgcn_cosf(turns);
+}
+
 // ---------------------------------------------------------------- vectors --
 struct f3 {
     float x, y, z;
@@ -38,7 +52,7 @@ VR_HD f3 operator*(f3 a, float s) { return F3(a.x * s, a.y * s, a.z * s); }
 VR_HD f3 operator*(float s, f3 a) { return F3(a.x * s, a.y * s, a.z * s); }
 VR_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 VR_HD f3 cross(f3 a, f3 b) { return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-VR_DEV f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+VR_DEV f3 normalize(f3 a) { return a * rsqrt_fast(dot(a, a)); }
 VR_HD f3 load3(const float* p) { return F3(p[0], p[1], p[2]); }
 
 // -------------------------------------------------------------------- RNG --
@@ -135,7 +149,7 @@ struct MediumView {
     float maj_value, maj_distance;
     const float* maj_grid;
     int mrx, mry, mrz;
-    f3 maj_bmin, maj_csize;
+    f3 maj_bmin, maj_csize, maj_icsize;
     int t_kind;
     float ds_eps;
     int max_bounces;
@@ -150,17 +164,19 @@ struct CameraView {
     const int64_t* sensors;
     int num_poses, width, height, jittered;
     float fov, znear;
+    float znear_over_h;  // znear / height
+    float sz;            // znear / tan(fov / 2)
 };
 
 // ------------------------------------------------------------- ray / box --
 // include/maths.h:28-38 (axis components with |w| <= 1e-6 replaced by 1e-6).
 VR_DEV void ray_box_intersection(f3 x, f3 w, f3 bmin, f3 bmax, float& tmin, float& tmax) {
-    float wx = fabsf(w.x) <= 0.000001f ? 0.000001f : w.x;
-    float wy = fabsf(w.y) <= 0.000001f ? 0.000001f : w.y;
-    float wz = fabsf(w.z) <= 0.000001f ? 0.000001f : w.z;
-    float c0x = (bmin.x - x.x) / wx, c1x = (bmax.x - x.x) / wx;
-    float c0y = (bmin.y - x.y) / wy, c1y = (bmax.y - x.y) / wy;
-    float c0z = (bmin.z - x.z) / wz, c1z = (bmax.z - x.z) / wz;
+    float ix = rcp(fabsf(w.x) <= 0.000001f ? 0.000001f : w.x);
+    float iy = rcp(fabsf(w.y) <= 0.000001f ? 0.000001f : w.y);
+    float iz = rcp(fabsf(w.z) <= 0.000001f ? 0.000001f : w.z);
+    float c0x = (bmin.x - x.x) * ix, c1x = (bmax.x - x.x) * ix;
+    float c0y = (bmin.y - x.y) * iy, c1y = (bmax.y - x.y) * iy;
+    float c0z = (bmin.z - x.z) * iz, c1z = (bmax.z - x.z) * iz;
     tmin = fmaxf(fmaxf(fminf(c0x, c1x), fminf(c0y, c1y)), fminf(c0z, c1z));
     tmax = fminf(fminf(fmaxf(c0x, c1x), fmaxf(c0y, c1y)), fmaxf(c0z, c1z));
 }
@@ -261,7 +277,7 @@ VR_DEV f3 random_direction(Rng& rng, f3 D) {
     float r2 = rng_uniform(rng) * 2.0f - 1.0f;
     float s = sqrtf(fmaxf(0.0f, 1.0f - r2 * r2));
     float sn, cs;
-    sincosf(kTwoPi * r1, &sn, &cs);
+    sincos_turns(r1, sn, cs);
     f3 t0, t1;
     ortho_basis(D, t0, t1);
     return t0 * (cs * s) + t1 * (sn * s) + D * r2;
@@ -269,18 +285,18 @@ VR_DEV f3 random_direction(Rng& rng, f3 D) {
 
 // include/scattering_AD.h:4-19
 VR_DEV f3 hg_sample(Rng& rng, f3 win, float g) {
-    float phi = rng_uniform(rng) * 2.0f * kPi;
+    float phi_turns = rng_uniform(rng);  // phi = 2 pi u
     float xi = rng_uniform(rng);
     float cos_t;
     if (fabsf(g) < 0.001f) cos_t = 2.0f * xi - 1.0f;
     else {
         float g2 = g * g;
-        float t = (1.0f - g2) / (1.0f - g + 2.0f * g * xi);
-        cos_t = (0.5f / g) * (1.0f + g2 - t * t);
+        float t = fdiv(1.0f - g2, 1.0f - g + 2.0f * g * xi);
+        cos_t = fdiv(0.5f, g) * (1.0f + g2 - t * t);
     }
     float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincos_turns(phi_turns, sp, cp);
     f3 t0, t1;
     ortho_basis(win, t0, t1);
     return t0 * (sin_t * sp) + t1 * (sin_t * cp) + win * cos_t;
@@ -291,7 +307,8 @@ VR_DEV float hg_eval(float cos_t, float g) {
     if (fabsf(g) < 0.001f) return 0.25f / kPi;
     float g2 = g * g;
     float den = 1.0f + g2 - 2.0f * g * cos_t;
-    return 0.25f / kPi * (1.0f - g2) / (den * sqrtf(den));
+    float r = rsqrt_fast(den);
+    return (0.25f / kPi) * (1.0f - g2) * (r * r * r);
 }
 
 // HGPhase (_maps.py:4446-4454) / single HG / isotropic.
@@ -324,10 +341,10 @@ VR_DEV f3 env_sample(const MediumView& M, Rng& rng, f3& wo) {
     if (M.sun_prob > 0.0f) {
         if (rng_uniform(rng) < M.sun_prob) {
             wo = M.sun_dir;
-            return M.sun_radiance * (1.0f / M.sun_prob);
+            return M.sun_radiance * rcp(M.sun_prob);
         }
         wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
-        return M.env_c * (4.0f * kPi / (1.0f - M.sun_prob));
+        return M.env_c * (4.0f * kPi * rcp(1.0f - M.sun_prob));
     }
     wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
     return M.env_c * (4.0f * kPi);
@@ -344,56 +361,65 @@ struct Walk<VR_MAJ_CONST> {
     VR_DEV void begin(const MediumView&, f3, f3) {}
     VR_DEV float query(const MediumView& M, float& md) const { md = M.maj_distance; return M.maj_value; }
     VR_DEV void move(float) {}
-    VR_DEV void cross() {}
+    VR_DEV void cross(const MediumView&, f3) {}
 };
 
 // Coarse max-grid DDA: value = bound of the current coarse cell, distance = to
-// that cell's exit along the ray.
+// that cell's exit along the ray.  Per-lane state is the packed cell index and
+// the ray parameters of the next x/y/z faces and of the tracked point; the
+// per-axis increments are recomputed from the ray direction at each crossing.
 template <>
 struct Walk<VR_MAJ_GRID> {
     static constexpr bool grid = true;
-    int cx, cy, cz, stx, sty, stz;
-    float tnx, tny, tnz, tdx, tdy, tdz, t;
-    int idx;  // linear cell index, -1 when outside
+    int cell;                // cx | cy << 10 | cz << 20; -1 once the ray has left the grid
+    float tnx, tny, tnz, t;  // ray parameter of the next x/y/z face and of the tracked point
 
-    VR_DEV void axis_begin(float xa, float wa, float bmin, float cs, int res, int& cell, int& st, float& tn, float& td) {
-        int c = (int)floorf((xa - bmin) / cs);
-        c = max(0, min(res - 1, c));
-        cell = c;
-        if (wa > 0.0f) { st = 1; tn = (bmin + (float)(c + 1) * cs - xa) / wa; td = cs / wa; }
-        else if (wa < 0.0f) { st = -1; tn = (bmin + (float)c * cs - xa) / wa; td = -cs / wa; }
-        else { st = 0; tn = kInf; td = kInf; }
+    static VR_DEV int axis_cell(float xa, float bmin, float ics, int res) {
+        int c = (int)floorf((xa - bmin) * ics);
+        return max(0, min(res - 1, c));
+    }
+    static VR_DEV float axis_face(float xa, float wa, float bmin, float cs, int c) {
+        if (wa > 0.0f) return (bmin + (float)(c + 1) * cs - xa) * rcp(wa);
+        if (wa < 0.0f) return (bmin + (float)c * cs - xa) * rcp(wa);
+        return kInf;
     }
     VR_DEV void begin(const MediumView& M, f3 x, f3 w) {
-        axis_begin(x.x, w.x, M.maj_bmin.x, M.maj_csize.x, M.mrx, cx, stx, tnx, tdx);
-        axis_begin(x.y, w.y, M.maj_bmin.y, M.maj_csize.y, M.mry, cy, sty, tny, tdy);
-        axis_begin(x.z, w.z, M.maj_bmin.z, M.maj_csize.z, M.mrz, cz, stz, tnz, tdz);
+        int cx = axis_cell(x.x, M.maj_bmin.x, M.maj_icsize.x, M.mrx);
+        int cy = axis_cell(x.y, M.maj_bmin.y, M.maj_icsize.y, M.mry);
+        int cz = axis_cell(x.z, M.maj_bmin.z, M.maj_icsize.z, M.mrz);
+        tnx = axis_face(x.x, w.x, M.maj_bmin.x, M.maj_csize.x, cx);
+        tny = axis_face(x.y, w.y, M.maj_bmin.y, M.maj_csize.y, cy);
+        tnz = axis_face(x.z, w.z, M.maj_bmin.z, M.maj_csize.z, cz);
         t = 0.0f;
-        idx = (cz * M.mry + cy) * M.mrx + cx;
-        mrx = M.mrx; mry = M.mry; mrz = M.mrz;
-        data = M.maj_grid;
+        cell = cx | (cy << 10) | (cz << 20);
     }
-    int mrx, mry, mrz;
-    const float* data;
-    VR_DEV float query(const MediumView&, float& md) const {
-        if (idx < 0) { md = kInf; return 0.0f; }
+    VR_DEV float query(const MediumView& M, float& md) const {
+        if (cell < 0) { md = kInf; return 0.0f; }
         md = fmaxf(0.0f, fminf(tnx, fminf(tny, tnz)) - t);
-        return data[idx];
+        int cx = cell & 1023, cy = (cell >> 10) & 1023, cz = cell >> 20;
+        return M.maj_grid[(cz * M.mry + cy) * M.mrx + cx];
     }
     VR_DEV void move(float dt) { t += dt; }
-    VR_DEV void cross() {
-        if (idx < 0) return;
+    VR_DEV void cross(const MediumView& M, f3 w) {
+        if (cell < 0) return;
+        int cx = cell & 1023, cy = (cell >> 10) & 1023, cz = cell >> 20;
         if (tnx <= tny && tnx <= tnz) {
-            t = tnx; cx += stx; tnx += tdx;
-            if (cx < 0 || cx >= mrx) { idx = -1; return; }
+            t = tnx;
+            cx += w.x > 0.0f ? 1 : -1;
+            tnx += M.maj_csize.x * rcp(fabsf(w.x));
+            if (cx < 0 || cx >= M.mrx) { cell = -1; return; }
         } else if (tny <= tnz) {
-            t = tny; cy += sty; tny += tdy;
-            if (cy < 0 || cy >= mry) { idx = -1; return; }
+            t = tny;
+            cy += w.y > 0.0f ? 1 : -1;
+            tny += M.maj_csize.y * rcp(fabsf(w.y));
+            if (cy < 0 || cy >= M.mry) { cell = -1; return; }
         } else {
-            t = tnz; cz += stz; tnz += tdz;
-            if (cz < 0 || cz >= mrz) { idx = -1; return; }
+            t = tnz;
+            cz += w.z > 0.0f ? 1 : -1;
+            tnz += M.maj_csize.z * rcp(fabsf(w.z));
+            if (cz < 0 || cz >= M.mrz) { cell = -1; return; }
         }
-        idx = (cz * mry + cy) * mrx + cx;
+        cell = cx | (cy << 10) | (cz << 20);
     }
 };
 
@@ -402,15 +428,19 @@ template <int MAJ>
 struct Tracer {
     const MediumView& M;
     Rng rng;
-    int lookups;
+    int lookups;   // trilinear sigma lookups
+    int scatters;  // trilinear sigma gradient scatters
 
-    VR_DEV explicit Tracer(const MediumView& m) : M(m), lookups(0) {}
+    VR_DEV explicit Tracer(const MediumView& m) : M(m), lookups(0), scatters(0) {}
 
     VR_DEV float rnd() { return rng_uniform(rng); }
-    VR_DEV float flight(float maj) { return -logf(1.0f - rnd()) / maj; }
+    VR_DEV float flight(float maj) { return neg_log1m(rnd()) * rcp(maj); }
 
     VR_DEV float sigma(f3 x) { lookups++; return grid_lookup1(M.sigma, x); }
-    VR_DEV void sigma_bw(f3 x, float g) { grid_scatter1(M.sigma, x, g); }
+    VR_DEV void sigma_bw(f3 x, float g) {
+        if (g != 0.0f && M.sigma.grad != nullptr) scatters++;
+        grid_scatter1(M.sigma, x, g);
+    }
 
     VR_DEV f3 albedo(f3 x) {
         if (!M.albedo_grid) return M.albedo_c;
@@ -443,10 +473,10 @@ struct Tracer {
             float t = fminf(md, dt);
             x = x + w * t;
             d -= t;
-            if (dt > md) { mj.cross(); if (d <= kCrossEps) return 1.0f; continue; }
+            if (dt > md) { mj.cross(M, w); if (d <= kCrossEps) return 1.0f; continue; }
             mj.move(t);
             if (d <= 0.0000001f) return 1.0f;
-            if (rnd() < sigma(x) / maj) return 0.0f;
+            if (rnd() * maj < sigma(x)) return 0.0f;
         }
     }
 
@@ -462,13 +492,13 @@ struct Tracer {
             float md;
             float maj = fmaxf(0.0000001f, mj.query(M, md));
             float s = flight(maj);
-            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(); if (d <= kCrossEps) return T; continue; }
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(M, w); if (d <= kCrossEps) return T; continue; }
             float dt = fminf(s, fminf(d, md));
             x = x + w * dt;
             d -= dt;
             mj.move(dt);
             if (d <= 0.0000001f) return T;
-            T *= (1.0f - sigma(x) / maj);
+            T *= (1.0f - sigma(x) * rcp(maj));
             if (T < 0.01f) {
                 if (rnd() < 1.0f - T) return 0.0f;
                 T = 1.0f;
@@ -630,7 +660,7 @@ struct Tracer {
             float t = fminf(md, dt);
             x = x + w * t;
             d -= t;
-            if (dt > md) { mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (dt > md) { mj.cross(M, w); if (d <= kCrossEps) break; continue; }
             mj.move(t);
             if (d <= 0.0000001f) break;
             if (rnd() < sigma(x) / maj) return;
@@ -645,7 +675,7 @@ struct Tracer {
             float t = fminf(md, dt);
             x = x + w * t;
             d -= t;
-            if (dt > md) { mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (dt > md) { mj.cross(M, w); if (d <= kCrossEps) break; continue; }
             mj.move(t);
             if (d <= 0.0000001f) break;
             rnd();
@@ -672,7 +702,7 @@ struct Tracer {
             float md;
             float maj = fmaxf(EPS, mj.query(M, md) + 0.1f);
             float s = flight(maj);
-            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
             float dt = fminf(s, fminf(d, md));
             x = x + w * dt;
             d -= dt;
@@ -693,7 +723,7 @@ struct Tracer {
             float md;
             float maj = fmaxf(EPS, mj.query(M, md) + 0.1f);
             float s = flight(maj);
-            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
             float dt = fminf(s, fminf(d, md));
             x = x + w * dt;
             d -= dt;
@@ -738,10 +768,10 @@ struct Tracer {
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
-            if (rnd() < sigma(x) / maj) { out_T = 0.0f; C = 1.0f; break; }
+            if (rnd() * maj < sigma(x)) { out_T = 0.0f; C = 1.0f; break; }
         }
         out_t = tmin + t;
         return C;
@@ -761,7 +791,7 @@ struct Tracer {
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
             float s = sigma(x);
@@ -791,7 +821,7 @@ struct Tracer {
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
             float s = sigma(x);
@@ -812,7 +842,7 @@ struct Tracer {
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
             float s = sigma(x);
@@ -913,11 +943,11 @@ struct Tracer {
             float m = fmaxf(EPS, mj.query(M, md));
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
-            if (rnd() >= s / m) { d -= t; continue; }
+            if (rnd() * m >= s) { d -= t; continue; }
             f3 sa = albedo(x);
             R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
             W = W * sa;
@@ -958,7 +988,7 @@ struct Tracer {
             float m = fmaxf(EPS, mj.query(M, md) + 2.0f * eps + 0.1f);
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
@@ -1006,7 +1036,7 @@ struct Tracer {
             float m = ref ? fmaxf(EPS, mq * 1.2f + 2.0f * eps) : fmaxf(EPS, mq + 2.0f * eps + 0.1f);
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
@@ -1136,9 +1166,9 @@ VR_DEV void camera_ray(const CameraView& cam, int pose, int row, int col, Rng& r
     f3 o = load3(P), d = load3(P + 3), n = load3(P + 6);
     float u = 0.5f, v = 0.5f;
     if (cam.jittered) { u = rng_uniform(rng); v = rng_uniform(rng); }
-    float sx = (((float)col + u) * 2.0f - (float)cam.width) * cam.znear / (float)cam.height;
-    float sy = (((float)row + v) * 2.0f - (float)cam.height) * cam.znear / (float)cam.height;
-    float sz = cam.znear / tanf(cam.fov * 0.5f);
+    float sx = (((float)col + u) * 2.0f - (float)cam.width) * cam.znear_over_h;
+    float sy = (((float)row + v) * 2.0f - (float)cam.height) * cam.znear_over_h;
+    float sz = cam.sz;
     f3 za = normalize(d);
     f3 xa = normalize(cross(n, za));
     f3 ya = cross(za, xa);

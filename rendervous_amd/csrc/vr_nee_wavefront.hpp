@@ -24,6 +24,9 @@ namespace vr {
 
 enum : int { kModePath = 0, kModeTrack = 1 };
 
+// Lane events: what a lane needs before it can take its next tracking step.
+enum : int { kEvNone = 0, kEvBegin = 1, kEvCollision = 2, kEvTrackEnd = 3, kEvPathEnd = 4 };
+
 template <int MAJ, int TKIND>
 struct NeeLane {
     static constexpr float EPS = 0.00001f;  // vr_path_integrator_NEE.h:16
@@ -36,7 +39,8 @@ struct NeeLane {
     f3 px, pw;        // path vertex and incoming direction while a shadow ray runs
     float Tt;         // ratio-tracking estimate of the tracked ray
     float pd;         // path distance inside the medium while the primary term runs
-    int mode, primary, bounces, lookups;
+    float track_T;    // transmittance of a finished tracked ray (kEvTrackEnd)
+    int mode, primary, bounces, lookups, event;
 
     // Starts the transmittance estimate of ray (o, dir) (vr_transmittance_DT.h:3-8,
     // vr_transmittance_RT.h:95-100); false when the boundary is missed (T = 1).
@@ -60,7 +64,7 @@ struct NeeLane {
         mode = kModePath;
     }
 
-    // Camera ray and primary transmittance term of a new sample.
+    // Camera ray and primary transmittance term of a new sample (kEvBegin).
     VR_DEV void begin_sample(const MediumView& M, const CameraView& cam, const Seeds& seeds, uint32_t seed_index, int pose,
                              int row, int col) {
         rng = seed_capture_fw(seeds, seed_index);
@@ -76,6 +80,7 @@ struct NeeLane {
         bounces = 0;
         contrib = M.env_c;
         primary = 1;
+        event = kEvNone;
         if (!start_track(M, px, pw)) {
             R = R + contrib;  // T = 1
             primary = 0;
@@ -83,25 +88,26 @@ struct NeeLane {
         }
     }
 
-    // One tracking step.  Returns true when the sample has ended (radiance in R).
-    VR_DEV bool step(const MediumView& M) {
+    // One tracking step of the current ray; sets `event` when the lane needs a
+    // handler (collision, end of a transmittance ray, end of the path).
+    VR_DEV void step(const MediumView& M) {
         float md;
         float mq = walk.query(M, md);
-        bool need_sigma = false, finished_track = false, end_path = false;
-        float track_T = 1.0f, m = 1.0f, t = 0.0f;
+        bool need_sigma = false;
+        float m = 1.0f, t = 0.0f;
         if (mode == kModePath) {
             if (!(rd > EPS)) {
-                end_path = true;
+                event = kEvPathEnd;
             } else {
                 m = fmaxf(EPS, mq);
-                t = -logf(1.0f - rng_uniform(rng)) / m;
+                t = neg_log1m(rng_uniform(rng)) * rcp(m);
                 rx = rx + rw * fminf(t, md);
                 if (t > md) {
                     rd -= md;
-                    walk.cross();
-                    if (rd <= kCrossEps) end_path = true;
+                    walk.cross(M, rw);
+                    if (rd <= kCrossEps) event = kEvPathEnd;
                 } else if (t > rd) {
-                    end_path = true;
+                    event = kEvPathEnd;
                 } else {
                     walk.move(t);
                     need_sigma = true;
@@ -110,103 +116,104 @@ struct NeeLane {
         } else {
             m = fmaxf(0.0000001f, mq);
             if (TKIND == VR_T_DT) {  // vr_transmittance_DT.h:10-27
-                float dt = -logf(1.0f - rng_uniform(rng)) / m;
+                float dt = neg_log1m(rng_uniform(rng)) * rcp(m);
                 float tt = fminf(md, dt);
                 rx = rx + rw * tt;
                 rd -= tt;
                 if (dt > md) {
-                    walk.cross();
-                    if (rd <= kCrossEps) finished_track = true;
+                    walk.cross(M, rw);
+                    if (rd <= kCrossEps) { event = kEvTrackEnd; track_T = 1.0f; }
                 } else {
                     walk.move(tt);
-                    if (rd <= 0.0000001f) finished_track = true;
+                    if (rd <= 0.0000001f) { event = kEvTrackEnd; track_T = 1.0f; }
                     else need_sigma = true;
                 }
             } else {  // vr_transmittance_RT.h:102-119
-                float sd = -logf(1.0f - rng_uniform(rng)) / m;
+                float sd = neg_log1m(rng_uniform(rng)) * rcp(m);
                 if (Walk<MAJ>::grid && md < rd && sd > md) {
                     rx = rx + rw * md;
                     rd -= md;
-                    walk.cross();
-                    if (rd <= kCrossEps) { finished_track = true; track_T = Tt; }
+                    walk.cross(M, rw);
+                    if (rd <= kCrossEps) { event = kEvTrackEnd; track_T = Tt; }
                 } else {
                     float dt = fminf(sd, fminf(rd, md));
                     rx = rx + rw * dt;
                     rd -= dt;
                     walk.move(dt);
-                    if (rd <= 0.0000001f) { finished_track = true; track_T = Tt; }
+                    if (rd <= 0.0000001f) { event = kEvTrackEnd; track_T = Tt; }
                     else need_sigma = true;
                 }
             }
         }
 
-        // the shared trilinear gather
-        float sg = 0.0f;
         if (need_sigma) {
-            sg = grid_lookup1(M.sigma, rx);
+            // the shared trilinear gather
+            float sg = grid_lookup1(M.sigma, rx);
             lookups++;
-        }
-
-        if (mode == kModePath) {
-            if (need_sigma) {
-                if (rng_uniform(rng) >= sg / m) {
-                    rd -= t;  // null collision
-                } else {
-                    // real collision (vr_path_integrator_NEE.h:39-52)
-                    f3 sa;
-                    if (!M.albedo_grid) sa = M.albedo_c;
-                    else {
-                        float v[4];
-                        grid_lookupC(M.albedo, rx, v);
-                        sa = M.albedo.C == 1 ? F3(v[0], v[0], v[0]) : F3(v[0], v[1], v[2]);
-                    }
-                    R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
-                    W = W * sa;
-                    f3 wnee;
-                    f3 env = env_sample(M, rng, wnee);
-                    px = rx;
-                    pw = rw;
-                    contrib = (W * env) * phase_eval(M, rw, wnee);
-                    if (!start_track(M, px, wnee)) finished_track = true;
-                }
-            }
-        } else if (need_sigma) {
-            if (TKIND == VR_T_DT) {
-                if (rng_uniform(rng) < sg / m) { finished_track = true; track_T = 0.0f; }
+            if (mode == kModePath) {
+                if (rng_uniform(rng) * m >= sg) rd -= t;  // null collision
+                else event = kEvCollision;
+            } else if (TKIND == VR_T_DT) {
+                if (rng_uniform(rng) * m < sg) { event = kEvTrackEnd; track_T = 0.0f; }
             } else {
-                Tt *= (1.0f - sg / m);
+                // ratio tracking draws its roulette uniform only when T < 0.01
+                Tt *= (1.0f - sg * rcp(m));
                 if (Tt < 0.01f) {
-                    if (rng_uniform(rng) < 1.0f - Tt) { finished_track = true; track_T = 0.0f; }
+                    if (rng_uniform(rng) < 1.0f - Tt) { event = kEvTrackEnd; track_T = 0.0f; }
                     else Tt = 1.0f;
                 }
             }
         }
+    }
 
-        if (finished_track) {
-            R = R + contrib * track_T;
-            if (primary) {
-                primary = 0;
-                start_path_segment(M, px, pw, pd);
-            } else {
-                // continue the path after its NEE estimate (vr_path_integrator_NEE.h:53-62)
-                bounces++;
-                if (M.max_bounces >= 0 && bounces >= M.max_bounces) {
-                    end_path = true;
-                } else {
-                    f3 wo;
-                    float rho = phase_sample(M, rng, pw, wo);
-                    W = W * rho;
-                    if (W.x < EPS && W.y < EPS && W.z < EPS) {
-                        end_path = true;
-                    } else {
-                        float tmin, d;
-                        boundary(M, px, wo, tmin, d);
-                        start_path_segment(M, px, wo, d);
-                    }
-                }
-            }
+    // Real collision of the path (vr_path_integrator_NEE.h:39-52): emission, albedo,
+    // light sample and the start of its shadow ray.
+    VR_DEV void on_collision(const MediumView& M) {
+        f3 sa;
+        if (!M.albedo_grid) sa = M.albedo_c;
+        else {
+            float v[4];
+            grid_lookupC(M.albedo, rx, v);
+            sa = M.albedo.C == 1 ? F3(v[0], v[0], v[0]) : F3(v[0], v[1], v[2]);
         }
-        return end_path;
+        R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+        W = W * sa;
+        f3 wnee;
+        f3 env = env_sample(M, rng, wnee);
+        px = rx;
+        pw = rw;
+        contrib = (W * env) * phase_eval(M, rw, wnee);
+        event = kEvNone;
+        if (!start_track(M, px, wnee)) { event = kEvTrackEnd; track_T = 1.0f; }
+    }
+
+    // A transmittance ray ended: add its term, then start the path (after the
+    // primary term) or continue it after the NEE estimate (vr_path_integrator_NEE.h:53-62).
+    VR_DEV void on_track_end(const MediumView& M) {
+        R = R + contrib * track_T;
+        event = kEvNone;
+        if (primary) {
+            primary = 0;
+            start_path_segment(M, px, pw, pd);
+            return;
+        }
+        bounces++;
+        if (M.max_bounces >= 0 && bounces >= M.max_bounces) { event = kEvPathEnd; return; }
+        f3 wo;
+        float rho = phase_sample(M, rng, pw, wo);
+        W = W * rho;
+        if (W.x < EPS && W.y < EPS && W.z < EPS) { event = kEvPathEnd; return; }
+        float tmin, d;
+        boundary(M, px, wo, tmin, d);
+        start_path_segment(M, px, wo, d);
+    }
+
+    // Runs the handlers until the lane can step again or its sample has ended.
+    VR_DEV void settle(const MediumView& M) {
+        while (event == kEvCollision || event == kEvTrackEnd) {
+            if (event == kEvCollision) on_collision(M);
+            else on_track_end(M);
+        }
     }
 };
 
@@ -226,6 +233,23 @@ VR_DEV NeeItem nee_item(int64_t j, int64_t start, int64_t pixels, int chunk, int
     return it;
 }
 
+// Bijection of [0, width*height) that visits an image in bands of 8 rows and,
+// within a band, in blocks of 8 columns (row-major inside a block), so that 64
+// consecutive indices cover an 8x8 tile.  Returns the row-major pixel index.
+VR_DEV int64_t tiled_pixel(int64_t k, int width, int height) {
+    const int64_t band = k / (8 * (int64_t)width);
+    const int64_t m = k - band * 8 * (int64_t)width;
+    const int bh = min(8, height - (int)band * 8);
+    const int64_t blk = m / (8 * bh);
+    const int q = (int)(m - blk * 8 * bh);
+    const int bw = min(8, width - (int)blk * 8);
+    const int r = q / bw, c = q - (q / bw) * bw;
+    return (band * 8 + r) * (int64_t)width + blk * 8 + c;
+}
+
+// Item indices a wave takes from the global counter per atomic.
+constexpr int kPool = 64;
+
 VR_DEV unsigned long long lane_mask_lt() {
     unsigned lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -239,50 +263,26 @@ __global__ void __launch_bounds__(256) k_nee_persistent(MediumView M, CameraView
                                                         unsigned long long* __restrict__ counter, float* __restrict__ out,
                                                         int* __restrict__ dbg) {
     const int64_t chunks = (spp + chunk - 1) / chunk;
+    const bool tiled = cam.sensors == nullptr && start == 0 && cnt == total;
     const unsigned long long items = (unsigned long long)(cnt * chunks);
     const float inv = spp == 1 ? 1.0f : 1.0f / (float)spp;
     NeeLane<MAJ, TKIND> lane;
     lane.lookups = 0;
+    lane.event = kEvNone;
     bool have = false, done = false;
+    bool drained = false;                              // wave-uniform: the global queue is empty
+    unsigned long long pool_next = 0ull, pool_end = 0ull;  // wave-uniform pool of item indices
     NeeItem it;
     it.sensor = 0; it.s0 = 0; it.s1 = 0;
     int s = 0, pose = 0, row = 0, col = 0;
     f3 acc = F3(0.0f, 0.0f, 0.0f);
     for (;;) {
-        // wave-aggregated fetch of new items for the lanes that need one
-        const bool need = !have && !done;
-        const unsigned long long mask = __ballot(need);
-        if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
-            base = __shfl(base, leader);
-            if (need) {
-                const unsigned long long j = base + (unsigned long long)__popcll(mask & lane_mask_lt());
-                if (j >= items) {
-                    done = true;
-                } else {
-                    it = nee_item((int64_t)j, start, cnt, chunk, spp);
-                    sensor_index(cam, it.sensor, pose, row, col);
-                    if (pose >= 0 && pose < cam.num_poses && row >= 0 && row < cam.height && col >= 0 && col < cam.width) {
-                        have = true;
-                        s = it.s0;
-                        acc = F3(0.0f, 0.0f, 0.0f);
-                        lane.lookups = 0;
-                        lane.begin_sample(M, cam, seeds, (uint32_t)((int64_t)s * total + it.sensor), pose, row, col);
-                    }
-                }
-            }
-        }
-        if (__ballot(have) == 0ull) {
-            if (__ballot(!done) == 0ull) break;  // every lane of the wave has drained the queue
-            continue;
-        }
-        if (have && lane.step(M)) {
+        // end of a sample: accumulate, then the next sample of the item or a new item
+        if (have && lane.event == kEvPathEnd) {
             acc = acc + lane.R;
             s++;
             if (s < it.s1) {
-                lane.begin_sample(M, cam, seeds, (uint32_t)((int64_t)s * total + it.sensor), pose, row, col);
+                lane.event = kEvBegin;
             } else {
                 float* o = out + 3 * it.sensor;
                 atomic_addf(o, acc.x * inv);
@@ -290,8 +290,56 @@ __global__ void __launch_bounds__(256) k_nee_persistent(MediumView M, CameraView
                 atomic_addf(o + 2, acc.z * inv);
                 if (dbg) atomicAdd(dbg + it.sensor, lane.lookups);
                 have = false;
+                lane.event = kEvNone;
             }
         }
+        // Lanes that need an item take one from the wave's pool of consecutive
+        // item indices; an empty pool is refilled with kPool items by one atomic.
+        const bool need = !have && !done;
+        const unsigned long long mask = __ballot(need);
+        if (mask) {
+            if (pool_next >= pool_end && !drained) {
+                unsigned long long base = 0;
+                if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)kPool);
+                base = __shfl(base, 0);
+                pool_next = base;
+                pool_end = base + kPool < items ? base + kPool : items;
+                if (base >= items) drained = true;
+            }
+            const unsigned long long avail = pool_end > pool_next ? pool_end - pool_next : 0ull;
+            const unsigned long long rank = (unsigned long long)__popcll(mask & lane_mask_lt());
+            if (need) {
+                if (rank < avail) {
+                    it = nee_item((int64_t)(pool_next + rank), start, cnt, chunk, spp);
+                    if (tiled) {  // whole images: visit each pose's pixels in 8x8 tiles
+                        const int64_t hw = (int64_t)cam.width * cam.height;
+                        const int64_t pose_base = (it.sensor / hw) * hw;
+                        it.sensor = pose_base + tiled_pixel(it.sensor - pose_base, cam.width, cam.height);
+                    }
+                    sensor_index(cam, it.sensor, pose, row, col);
+                    if (pose >= 0 && pose < cam.num_poses && row >= 0 && row < cam.height && col >= 0 && col < cam.width) {
+                        have = true;
+                        s = it.s0;
+                        acc = F3(0.0f, 0.0f, 0.0f);
+                        lane.lookups = 0;
+                        lane.event = kEvBegin;
+                    }
+                } else if (drained) {
+                    done = true;
+                }
+            }
+            const unsigned long long taken = (unsigned long long)__popcll(mask);
+            pool_next += taken < avail ? taken : avail;
+        }
+        if (__ballot(have) == 0ull) {
+            if (__ballot(!done) == 0ull) break;  // every lane of the wave has drained the queue
+            continue;
+        }
+        if (have && lane.event == kEvBegin)
+            lane.begin_sample(M, cam, seeds, (uint32_t)((int64_t)s * total + it.sensor), pose, row, col);
+        if (have && lane.event == kEvNone) lane.step(M);
+        if (have && lane.event == kEvCollision) lane.on_collision(M);
+        if (have && lane.event == kEvTrackEnd) lane.on_track_end(M);
     }
 }
 

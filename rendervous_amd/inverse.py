@@ -15,6 +15,7 @@ import dataclasses
 import typing
 
 import torch
+from torch.autograd.graph import increment_version
 
 from . import _internal
 from . import _lib as L
@@ -57,6 +58,11 @@ class FusedAdam:
         L.check(L.load().vr_adam_step(self.p.data_ptr(), grad.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
                                       self.p.numel(), self.lr, self.b1, self.b2, self.eps, self.t, int(zero_grad),
                                       L.stream_handle(self.p.device)), "vr_adam_step")
+        # the kernel writes p and grad behind torch's back: bump their version
+        # counters so caches keyed on them (the majorant grid) are rebuilt
+        increment_version(self.p)
+        if zero_grad:
+            increment_version(grad)
 
 
 @dataclasses.dataclass
@@ -70,8 +76,8 @@ def render(camera: M.PerspectiveCameraSensor, radiance: M.MapBase, spp: int) -> 
 
 
 def optimisation_step(camera, radiance, sigma_param: torch.nn.Parameter, target: torch.Tensor, spp: int,
-                      opt: FusedAdam, clamp_min: float = 0.0) -> float:
-    """One Adam step of the L2 image loss; returns the loss (host sync)."""
+                      opt: FusedAdam, clamp_min: float = 0.0) -> torch.Tensor:
+    """One Adam step of the L2 image loss; returns the loss as a 0-d tensor (no host sync)."""
     img = render(camera, radiance, spp)
     loss = ((img - target) ** 2).mean()
     loss.backward()
@@ -79,4 +85,4 @@ def optimisation_step(camera, radiance, sigma_param: torch.nn.Parameter, target:
     if clamp_min is not None:
         with torch.no_grad():
             sigma_param.clamp_(min=clamp_min)
-    return loss
+    return loss.detach()

@@ -722,9 +722,11 @@ class AutogradMapFunction(torch.autograd.Function):
 class AutogradCaptureFunction(torch.autograd.Function):
     """Camera capture of a field (_maps.py:1616-1642).
 
-    The backward returns fresh gradients for the trigger parameters; torch adds
-    them into .grad once (the reference also accumulates them in place,
-    Appendix A-7 of the survey of its backward).
+    The backward returns fresh gradients for the trigger parameters and torch
+    adds them into .grad once (the reference's backward both accumulates into
+    .grad in place and returns it, _maps.py:1049-1058,1642).  A `debug_out`
+    tensor receives per-sensor trilinear lookups in the forward and trilinear
+    gradient scatters in the backward.
     """
 
     @staticmethod
@@ -753,6 +755,7 @@ class AutogradCaptureFunction(torch.autograd.Function):
         ctx.bw_samples = bw_samples
         ctx.fw_samples = fw_samples
         ctx.seeds = seeds
+        ctx.debug_out = debug_out
         ctx.n_params = len(parameters)
         ctx.save_for_backward(*parameters)
         return out
@@ -775,7 +778,7 @@ class AutogradCaptureFunction(torch.autograd.Function):
             lib = L.load()
             for start in range(0, total, max(1, bs)):
                 launch = L.launch(seeds, samples=samples, bw_mode=_bw_mode(), seed_mode=seed_mode,
-                                  start_index=start, count=min(bs, total - start))
+                                  start_index=start, count=min(bs, total - start), debug_out=ctx.debug_out)
                 L.check(lib.vr_capture_bw(lw.medium, kind, cam, launch, L.ptr(og), L.stream_handle(og.device)),
                         "vr_capture_bw")
         return (None, None, None, None, None, None, None, *grads)

@@ -2,9 +2,11 @@
 
 Each check compares the capture backward (through the map API and torch
 autograd, i.e. AutogradCaptureFunction -> vr_capture_bw) with a central
-difference of the expected loss, both estimated with enough samples that the
-Monte Carlo error is a few percent of the derivative.  The loss is a fixed
-linear functional of the image, L = sum(w * I), so dL/dI = w exactly.
+difference of the expected loss.  Both sides of a difference are rendered with
+the same dispatch seeds (common random numbers): a perturbation of one voxel
+changes only the paths that reach it, so the difference keeps little of the
+Monte Carlo noise of either render.  The loss is a fixed linear functional of
+the image, L = sum(w * I), so dL/dI = w exactly.
 """
 import math
 
@@ -27,13 +29,13 @@ def _loss(camera, field, weights, spp, seed):
     rv.seed(seed)
     with torch.no_grad():
         img = camera.capture(field, fw_samples=spp)
-    return float((img * weights).sum())
+    return float((img.double() * weights.double()).sum())
 
 
-def _fd_check(make_field, sigma0, voxels, spp_grad, spp_fd, h, rel_tol, cam_res=16):
+def _fd_check(make_field, sigma0, voxels, spp_grad, spp_fd, h, rel_tol, cam_res=16, jittered=True):
     dev = torch.device("cuda", 0)
     poses = torch.stack([rv.functions.look_at_pose((0.3, 0.4, 2.0)), rv.functions.look_at_pose((-1.8, 0.2, 0.9))]).to(dev)
-    camera = rv.PerspectiveCameraSensor(cam_res, cam_res, poses, jittered=True)
+    camera = rv.PerspectiveCameraSensor(cam_res, cam_res, poses, jittered=jittered)
     weights = torch.rand(2, cam_res, cam_res, 3, generator=torch.Generator().manual_seed(5)).to(dev) + 0.5
     sigma = torch.nn.Parameter(sigma0.clone().to(dev), requires_grad=True)
     field = make_field(sigma)
@@ -47,7 +49,7 @@ def _fd_check(make_field, sigma0, voxels, spp_grad, spp_fd, h, rel_tol, cam_res=
             s2 = sigma0.clone()
             s2[v] += sgn * h
             f2 = make_field(torch.nn.Parameter(s2.to(dev), requires_grad=False))
-            results.append(_loss(camera, f2, weights, spp_fd, 1000 + 2 * k + (sgn > 0)))
+            results.append(_loss(camera, f2, weights, spp_fd, 1000 + k))
         fd = (results[0] - results[1]) / (2 * h)
         g = float(grad[v])
         print(f"[fd] voxel {v}: adjoint {g:.5f} fd {fd:.5f}", flush=True)
@@ -106,11 +108,13 @@ def test_dda_gradient_matches_fd_exactly(gpu):
         b = rv.RayBoxIntersection(*rv.normalized_box(sig))
         return rv.GridDDATransmittance(rv.Grid3D(sig, *rv.normalized_box(sig)), b)
 
-    _fd_check(make, sigma0, [(3, 3, 4, 0), (4, 2, 5, 0), (2, 5, 3, 0)], spp_grad=1, spp_fd=1, h=1e-2, rel_tol=1e-2)
+    _fd_check(make, sigma0, [(3, 3, 4, 0), (4, 2, 5, 0), (2, 5, 3, 0)], spp_grad=1, spp_fd=1, h=1e-2, rel_tol=1e-2,
+              jittered=False)
 
 
 def test_inverse_render_reduces_loss(gpu):
-    """A few Adam steps on a 32^3 grid from constant 5 towards an fBm target lower the loss."""
+    """Adam on a 32^3 grid from constant 5 towards an fBm target: the image
+    error, measured at 256 spp before and after 30 steps at 8 spp, drops."""
     dev = torch.device("cuda", 0)
     target_sigma = rv.functions.fbm_value_noise(32, seed=3, scale=20.0, device=dev)
     poses = rv.functions.orbit_poses(4, radius=2.5).to(dev)
@@ -118,12 +122,19 @@ def test_inverse_render_reduces_loss(gpu):
     target_field = inverse.single_scatter_scene(target_sigma).radiance
     rv.seed(1)
     with torch.no_grad():
-        target = camera.capture(target_field, fw_samples=64)
+        target = camera.capture(target_field, fw_samples=1024)
     sigma = torch.nn.Parameter(torch.full((32, 32, 32, 1), 5.0, device=dev), requires_grad=True)
     field = inverse.single_scatter_scene(sigma).radiance
-    opt = inverse.FusedAdam(sigma, lr=0.2)
-    losses = []
-    for it in range(12):
-        losses.append(float(inverse.optimisation_step(camera, field, sigma, target, 8, opt)))
-    print("[inverse] losses", [f"{l:.5f}" for l in losses], flush=True)
-    assert np.mean(losses[-3:]) < 0.7 * np.mean(losses[:3])
+
+    def image_error():
+        rv.seed(2)
+        with torch.no_grad():
+            return float(((camera.capture(field, fw_samples=256) - target) ** 2).mean())
+
+    before = image_error()
+    opt = inverse.FusedAdam(sigma, lr=0.3)
+    losses = [float(inverse.optimisation_step(camera, field, sigma, target, 8, opt)) for _ in range(30)]
+    after = image_error()
+    print(f"[inverse] image error {before:.5f} -> {after:.5f}; step losses {losses[0]:.5f} .. {losses[-1]:.5f}",
+          flush=True)
+    assert after < 0.7 * before

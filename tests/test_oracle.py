@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import (L, O, Scene, blob_grid, fbm_grid, np_grid, oracle_capture_fw, oracle_map_bw, oracle_map_fw, pose,
+from helpers import (L, O, Scene, blob_grid, fbm_grid, np_grid, oracle_capture_bw, oracle_capture_fw, oracle_map_bw, oracle_map_fw, pose,
                      random_rays)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -282,3 +282,38 @@ def test_rt_backward_matches_finite_differences():
         gm[idx] -= h
         fd = (math.log(_exact_T(gp, ray)) - math.log(_exact_T(gm, ray))) / (2 * h) * _exact_T(g, ray)
         assert abs(grad[idx] - fd) <= 0.05 * abs(fd) + 2e-4, (idx, grad[idx], fd)
+
+
+def _capture_fd_check(kind, transmittance_kind, voxels, spp_bw, spp_fd, h=0.4, rel_tol=0.06):
+    """Mean of the oracle's capture backward (consistent estimator, replayed
+    forward seeds) against a central difference of the oracle's forward, both
+    sides of the difference rendered with the same seeds."""
+    rng = np.random.default_rng(1)
+    g = rng.uniform(1.0, 4.0, size=(6, 6, 6, 1)).astype(np.float32)
+    cam = dict(poses=np.stack([pose((0.3, 0.4, 2.0)), pose((-1.8, 0.2, 0.9))]), width=8, height=8, jittered=True)
+    w = (rng.uniform(size=(128, 3)) + 0.5).astype(np.float32)
+
+    def scene(s):
+        return Scene(sigma=s, albedo_c=(0.8, 0.8, 0.8), env_c=(1.0, 0.8, 0.6), phase_g=0.3,
+                     transmittance_kind=transmittance_kind, max_bounces=8)
+
+    grad = oracle_capture_bw(scene(g), kind, cam, (1, 2, 3, 4), spp_bw, w, bw_mode=L.VR_BW_CONSISTENT)[0]
+    for v in voxels:
+        r = []
+        for sgn in (1, -1):
+            s2 = g.copy()
+            s2[v] += sgn * h
+            r.append(float((oracle_capture_fw(scene(s2), kind, cam, (7, 8, 9, 10), spp_fd).astype(np.float64) * w).sum()))
+        fd = (r[0] - r[1]) / (2 * h)
+        assert abs(grad[v] - fd) <= rel_tol * abs(fd), (v, grad[v], fd)
+
+
+def test_collision_single_scatter_backward_matches_finite_differences():
+    """MC collision integrator over the DT collision sampler with single-scatter
+    exitance and RT shadow rays (the inverse-rendering graph)."""
+    _capture_fd_check(L.VR_MAP_MC_COLLISION, L.VR_T_RT, [(2, 3, 3, 0), (3, 2, 2, 0)], 16384, 65536)
+
+
+def test_nee_path_backward_matches_finite_differences():
+    """NEE delta-tracking path integrator, up to 8 bounces, DT shadow rays."""
+    _capture_fd_check(L.VR_MAP_NEE_DT, L.VR_T_DT, [(2, 3, 3, 0), (3, 2, 2, 0)], 32768, 262144, rel_tol=0.08)
