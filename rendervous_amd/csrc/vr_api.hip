@@ -433,21 +433,39 @@ int persistent_blocks(K kernel, int* blocks) {
 template <int MAJ, int TKIND>
 int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
                           int64_t cnt, float* out, int* dbg, hipStream_t st) {
+    if (total > (int64_t)UINT32_MAX) return set_err(VR_ERR_UNSUPPORTED, "capture has more than 2^32 sensors");
     unsigned long long* counter = nullptr;
     int rc = work_counter(&counter);
     if (rc) return rc;
     int blocks = 0;
     if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND>, &blocks))) return rc;
-    // samples per work item: enough items for ~8 per lane in flight, at most spp
     const int64_t lanes = (int64_t)blocks * kBlock;
-    int64_t chunk = (cnt * (int64_t)spp) / (8 * lanes);
-    chunk = chunk < 1 ? 1 : (chunk > spp ? spp : chunk);
-    unsigned prep = (unsigned)std::min<int64_t>((cnt + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, start, cnt, dbg, counter);
-    if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
-    hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, spp, (int)chunk, total,
-                       start, cnt, counter, out, dbg);
-    return check_hip(hipGetLastError(), "vr_capture_fw launch");
+    const int64_t hw = (int64_t)C.width * C.height;
+    // launches of at most 2^30 sensors (whole images when possible) keep item indices in 32 bits
+    int64_t batch = int64_t(1) << 30;
+    if (!C.sensors && hw <= batch) batch = batch / hw * hw;
+    for (int64_t b0 = start; b0 < start + cnt; b0 += batch) {
+        const int64_t n = std::min(batch, start + cnt - b0);
+        // samples per work item: enough items for ~8 per lane in flight, at most spp
+        int64_t chunk = (n * (int64_t)spp) / (8 * lanes);
+        chunk = chunk < 1 ? 1 : (chunk > spp ? spp : chunk);
+        while (n * ((spp + chunk - 1) / chunk) > (int64_t)UINT32_MAX - kPool) chunk *= 2;
+        NeeSchedule S;
+        S.total = (uint32_t)total;
+        S.start = (uint32_t)b0;
+        S.cnt = (uint32_t)n;
+        S.items = (uint32_t)(n * ((spp + chunk - 1) / chunk));
+        S.spp = spp;
+        S.chunk = (int)std::min<int64_t>(chunk, spp);
+        S.tiled = !C.sensors && b0 % hw == 0 && n % hw == 0;
+        S.ksteps = std::max(1, env_int("VR_NEE_STEPS", 3));
+        unsigned prep = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
+        hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, b0, n, dbg, counter);
+        if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
+        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, S, counter, out, dbg);
+        if ((rc = check_hip(hipGetLastError(), "vr_capture_fw launch"))) return rc;
+    }
+    return VR_OK;
 }
 
 // ------------------------------------------------------------- dispatch --

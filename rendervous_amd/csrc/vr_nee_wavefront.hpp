@@ -10,12 +10,16 @@
 // Here every lane runs a small state machine (NeeLane) whose step performs
 // exactly one tracking step of whichever ray it is on -- the primary
 // transmittance ray, the path ray or a shadow ray -- with one majorant query,
-// one free-flight draw and one trilinear gather shared by all lanes.  Lanes
-// pull work items (a pixel and a run of consecutive samples) from a global
-// counter with one atomic per wave, so every lane stays busy until the queue
-// drains.  Each sample still draws the stream seeded from its flat index
-// s * pixels + p, in the same order as the nested form (Tracer::nee), so
-// results are identical sample by sample.
+// one free-flight draw and one trilinear gather shared by all lanes.  The
+// events between steps (new sample, real collision, end of a transmittance
+// ray) only prepare the next ray; the box intersection and majorant-walk set-up
+// of every new ray run in one shared block, so a wave pays for that code once
+// per iteration however many handlers its lanes went through.  Lanes pull work
+// items (a pixel and a run of consecutive samples) from a global counter with
+// one atomic per wave, so every lane stays busy until the queue drains.  Each
+// sample still draws the stream seeded from its flat index s * pixels + p, in
+// the same order as the nested form (Tracer::nee), so results are identical
+// sample by sample.
 #pragma once
 
 #include "vr_device.hpp"
@@ -25,7 +29,7 @@ namespace vr {
 enum : int { kModePath = 0, kModeTrack = 1 };
 
 // Lane events: what a lane needs before it can take its next tracking step.
-enum : int { kEvNone = 0, kEvBegin = 1, kEvCollision = 2, kEvTrackEnd = 3, kEvPathEnd = 4 };
+enum : int { kEvNone = 0, kEvBegin = 1, kEvStart = 2, kEvCollision = 3, kEvTrackEnd = 4, kEvPathEnd = 5 };
 
 template <int MAJ, int TKIND>
 struct NeeLane {
@@ -34,37 +38,17 @@ struct NeeLane {
     Rng rng;
     Walk<MAJ> walk;
     f3 R, W, contrib;
-    f3 rx, rw;        // tracked ray: current point and direction
-    float rd;         // remaining distance on the tracked ray
-    f3 px, pw;        // path vertex and incoming direction while a shadow ray runs
+    f3 rx, rw;        // current ray: point and direction
+    float rd;         // remaining distance on the current ray
+    f3 px, pw;        // path vertex and incoming direction while a transmittance ray runs
     float Tt;         // ratio-tracking estimate of the tracked ray
     float pd;         // path distance inside the medium while the primary term runs
     float track_T;    // transmittance of a finished tracked ray (kEvTrackEnd)
-    int mode, primary, bounces, lookups, event;
+    int mode, bnd, primary, bounces, lookups, event;
 
-    // Starts the transmittance estimate of ray (o, dir) (vr_transmittance_DT.h:3-8,
-    // vr_transmittance_RT.h:95-100); false when the boundary is missed (T = 1).
-    VR_DEV bool start_track(const MediumView& M, f3 o, f3 dir) {
-        float tmin, tmax;
-        if (!boundary(M, o, dir, tmin, tmax)) return false;
-        rx = o + dir * tmin;
-        rw = dir;
-        rd = tmax - tmin;
-        walk.begin(M, rx, rw);
-        Tt = 1.0f;
-        mode = kModeTrack;
-        return true;
-    }
-
-    VR_DEV void start_path_segment(const MediumView& M, f3 x, f3 w, float d) {
-        rx = x;
-        rw = w;
-        rd = d;
-        walk.begin(M, rx, rw);
-        mode = kModePath;
-    }
-
-    // Camera ray and primary transmittance term of a new sample (kEvBegin).
+    // Camera ray of a new sample (kEvBegin): the path starts at the box entry
+    // (vr_path_integrator_NEE.h:3-6) and its first ray is the primary
+    // transmittance term T(x, w) * env(w) (:13-14).
     VR_DEV void begin_sample(const MediumView& M, const CameraView& cam, const Seeds& seeds, uint32_t seed_index, int pose,
                              int row, int col) {
         rng = seed_capture_fw(seeds, seed_index);
@@ -72,7 +56,7 @@ struct NeeLane {
         camera_ray(cam, pose, row, col, rng, x, w);
         float tmin, tmax;
         boundary(M, x, w, tmin, tmax);
-        px = x + w * tmin;  // the path starts at the box entry (vr_path_integrator_NEE.h:3-6)
+        px = x + w * tmin;
         pw = w;
         pd = tmax - tmin;
         R = F3(0.0f, 0.0f, 0.0f);
@@ -80,12 +64,37 @@ struct NeeLane {
         bounces = 0;
         contrib = M.env_c;
         primary = 1;
+        rx = px;
+        rw = pw;
+        mode = kModeTrack;
+        bnd = 1;
+        event = kEvStart;
+    }
+
+    // Set-up of the ray (rx, rw) prepared by a handler (kEvStart).  A
+    // transmittance ray is clipped to the medium box (vr_transmittance_DT.h:3-8,
+    // vr_transmittance_RT.h:95-100) and one that misses it ends at once with
+    // T = 1; a path ray keeps its origin and runs to the box exit (or over the
+    // distance already in rd when bnd == 0).
+    VR_DEV void start_ray(const MediumView& M) {
         event = kEvNone;
-        if (!start_track(M, px, pw)) {
-            R = R + contrib;  // T = 1
-            primary = 0;
-            start_path_segment(M, px, pw, pd);
+        if (bnd) {
+            float tmin, tmax;
+            bool hit = boundary(M, rx, rw, tmin, tmax);
+            if (mode == kModeTrack) {
+                if (!hit) {
+                    event = kEvTrackEnd;
+                    track_T = 1.0f;
+                    return;
+                }
+                rx = rx + rw * tmin;
+                rd = tmax - tmin;
+            } else {
+                rd = tmax;
+            }
         }
+        Tt = 1.0f;
+        walk.begin(M, rx, rw);
     }
 
     // One tracking step of the current ray; sets `event` when the lane needs a
@@ -166,8 +175,8 @@ struct NeeLane {
         }
     }
 
-    // Real collision of the path (vr_path_integrator_NEE.h:39-52): emission, albedo,
-    // light sample and the start of its shadow ray.
+    // Real collision of the path (vr_path_integrator_NEE.h:39-52): emission,
+    // albedo, light sample; the shadow ray starts at the collision.
     VR_DEV void on_collision(const MediumView& M) {
         f3 sa;
         if (!M.albedo_grid) sa = M.albedo_c;
@@ -183,18 +192,24 @@ struct NeeLane {
         px = rx;
         pw = rw;
         contrib = (W * env) * phase_eval(M, rw, wnee);
-        event = kEvNone;
-        if (!start_track(M, px, wnee)) { event = kEvTrackEnd; track_T = 1.0f; }
+        rw = wnee;
+        mode = kModeTrack;
+        bnd = 1;
+        event = kEvStart;
     }
 
     // A transmittance ray ended: add its term, then start the path (after the
     // primary term) or continue it after the NEE estimate (vr_path_integrator_NEE.h:53-62).
     VR_DEV void on_track_end(const MediumView& M) {
         R = R + contrib * track_T;
-        event = kEvNone;
+        rx = px;
+        mode = kModePath;
+        event = kEvStart;
         if (primary) {
             primary = 0;
-            start_path_segment(M, px, pw, pd);
+            rw = pw;
+            rd = pd;
+            bnd = 0;
             return;
         }
         bounces++;
@@ -203,49 +218,83 @@ struct NeeLane {
         float rho = phase_sample(M, rng, pw, wo);
         W = W * rho;
         if (W.x < EPS && W.y < EPS && W.z < EPS) { event = kEvPathEnd; return; }
-        float tmin, d;
-        boundary(M, px, wo, tmin, d);
-        start_path_segment(M, px, wo, d);
+        rw = wo;
+        bnd = 1;
     }
 
-    // Runs the handlers until the lane can step again or its sample has ended.
-    VR_DEV void settle(const MediumView& M) {
-        while (event == kEvCollision || event == kEvTrackEnd) {
-            if (event == kEvCollision) on_collision(M);
-            else on_track_end(M);
+    // Runs one event handler or one tracking step (host emulation and tests).
+    VR_DEV void advance(const MediumView& M) {
+        switch (event) {
+        case kEvStart: start_ray(M); break;
+        case kEvNone: step(M); break;
+        case kEvCollision: on_collision(M); break;
+        case kEvTrackEnd: on_track_end(M); break;
+        default: break;
         }
     }
 };
 
-// Work item j of a capture over `pixels` sensors in chunks of `chunk` samples:
-// sensor start + j % pixels, samples [c*chunk, min(spp, (c+1)*chunk)) with c = j / pixels.
-struct NeeItem {
-    int64_t sensor;
-    int s0, s1;
+// Launch-wide schedule of a persistent NEE capture.  Sensors [start, start+cnt)
+// of a capture over `total` sensors are split into items of `chunk`
+// consecutive samples: item j covers sensor start + j % cnt and samples
+// [c*chunk, min(spp, (c+1)*chunk)) with c = j / cnt.  All counts fit 32 bits
+// (the host splits larger captures).
+struct NeeSchedule {
+    uint32_t total, start, cnt, items;
+    int spp, chunk;
+    int tiled;   // whole images without a sensor list: visit each pose in 8x8 tiles
+    int ksteps;  // tracking steps per iteration of the event loop
 };
 
-VR_DEV NeeItem nee_item(int64_t j, int64_t start, int64_t pixels, int chunk, int spp) {
-    NeeItem it;
-    int64_t c = j / pixels;
-    it.sensor = start + (j - c * pixels);
-    it.s0 = (int)c * chunk;
-    it.s1 = min(spp, it.s0 + chunk);
-    return it;
+// Pixel of the k-th position of a width x height image visited in bands of 8
+// rows and, within a band, in blocks of 8 columns (row-major inside a block),
+// so that 64 consecutive positions cover an 8x8 tile.
+VR_DEV void tiled_pixel(uint32_t k, uint32_t width, uint32_t height, int& row, int& col) {
+    const uint32_t band = k / (8u * width);
+    const uint32_t m = k - band * 8u * width;
+    const uint32_t bh = min(8u, height - band * 8u);
+    const uint32_t blk = m / (8u * bh);
+    const uint32_t q = m - blk * 8u * bh;
+    const uint32_t bw = min(8u, width - blk * 8u);
+    const uint32_t r = q / bw;
+    row = (int)(band * 8u + r);
+    col = (int)(blk * 8u + (q - r * bw));
 }
 
-// Bijection of [0, width*height) that visits an image in bands of 8 rows and,
-// within a band, in blocks of 8 columns (row-major inside a block), so that 64
-// consecutive indices cover an 8x8 tile.  Returns the row-major pixel index.
-VR_DEV int64_t tiled_pixel(int64_t k, int width, int height) {
-    const int64_t band = k / (8 * (int64_t)width);
-    const int64_t m = k - band * 8 * (int64_t)width;
-    const int bh = min(8, height - (int)band * 8);
-    const int64_t blk = m / (8 * bh);
-    const int q = (int)(m - blk * 8 * bh);
-    const int bw = min(8, width - (int)blk * 8);
-    const int r = q / bw, c = q - (q / bw) * bw;
-    return (band * 8 + r) * (int64_t)width + blk * 8 + c;
+// Decodes work item j; false if its sensor lies outside the camera.
+VR_DEV bool nee_item(const NeeSchedule& S, const CameraView& cam, uint32_t j, uint32_t& sensor, int& s0, int& s1,
+                     int& pose, int& row, int& col) {
+    const uint32_t c = j / S.cnt;
+    const uint32_t idx = S.start + (j - c * S.cnt);
+    s0 = (int)c * S.chunk;
+    s1 = min(S.spp, s0 + S.chunk);
+    sensor = idx;
+    if (cam.sensors) {
+        pose = (int)cam.sensors[3 * (size_t)idx];
+        row = (int)cam.sensors[3 * (size_t)idx + 1];
+        col = (int)cam.sensors[3 * (size_t)idx + 2];
+        return pose >= 0 && pose < cam.num_poses && row >= 0 && row < cam.height && col >= 0 && col < cam.width;
+    }
+    const uint32_t w = (uint32_t)cam.width, hw = w * (uint32_t)cam.height;
+    const uint32_t p = idx / hw;
+    const uint32_t k = idx - p * hw;
+    pose = (int)p;
+    if (S.tiled) {
+        tiled_pixel(k, w, (uint32_t)cam.height, row, col);
+        sensor = p * hw + (uint32_t)row * w + (uint32_t)col;
+    } else {
+        const uint32_t r = k / w;
+        row = (int)r;
+        col = (int)(k - r * w);
+    }
+    return true;
 }
+
+// Resident waves per SIMD the persistent kernel is compiled for (caps its
+// registers: 3 waves leave 168 VGPRs per lane).
+#ifndef VR_NEE_WAVES_PER_EU
+#define VR_NEE_WAVES_PER_EU 3
+#endif
 
 // Item indices a wave takes from the global counter per atomic.
 constexpr int kPool = 64;
@@ -256,39 +305,34 @@ VR_DEV unsigned long long lane_mask_lt() {
 }
 
 // Persistent kernel: every lane pulls work items until the queue drains.  The
-// output must be zero on entry (k_fill_zero); item sums are added atomically.
+// output must be zero on entry (k_nee_prepare); item sums are added atomically.
 template <int MAJ, int TKIND>
-__global__ void __launch_bounds__(256) k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, int spp, int chunk,
-                                                        int64_t total, int64_t start, int64_t cnt,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU))) k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S,
                                                         unsigned long long* __restrict__ counter, float* __restrict__ out,
                                                         int* __restrict__ dbg) {
-    const int64_t chunks = (spp + chunk - 1) / chunk;
-    const bool tiled = cam.sensors == nullptr && start == 0 && cnt == total;
-    const unsigned long long items = (unsigned long long)(cnt * chunks);
-    const float inv = spp == 1 ? 1.0f : 1.0f / (float)spp;
+    const float inv = S.spp == 1 ? 1.0f : 1.0f / (float)S.spp;
     NeeLane<MAJ, TKIND> lane;
     lane.lookups = 0;
     lane.event = kEvNone;
     bool have = false, done = false;
-    bool drained = false;                              // wave-uniform: the global queue is empty
-    unsigned long long pool_next = 0ull, pool_end = 0ull;  // wave-uniform pool of item indices
-    NeeItem it;
-    it.sensor = 0; it.s0 = 0; it.s1 = 0;
-    int s = 0, pose = 0, row = 0, col = 0;
+    bool drained = false;                // wave-uniform: the global queue is empty
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform pool of item indices
+    uint32_t sensor = 0;
+    int s = 0, s1 = 0, pose = 0, row = 0, col = 0;
     f3 acc = F3(0.0f, 0.0f, 0.0f);
     for (;;) {
         // end of a sample: accumulate, then the next sample of the item or a new item
         if (have && lane.event == kEvPathEnd) {
             acc = acc + lane.R;
             s++;
-            if (s < it.s1) {
+            if (s < s1) {
                 lane.event = kEvBegin;
             } else {
-                float* o = out + 3 * it.sensor;
+                float* o = out + 3 * (size_t)sensor;
                 atomic_addf(o, acc.x * inv);
                 atomic_addf(o + 1, acc.y * inv);
                 atomic_addf(o + 2, acc.z * inv);
-                if (dbg) atomicAdd(dbg + it.sensor, lane.lookups);
+                if (dbg) atomicAdd(dbg + sensor, lane.lookups);
                 have = false;
                 lane.event = kEvNone;
             }
@@ -302,24 +346,22 @@ __global__ void __launch_bounds__(256) k_nee_persistent(MediumView M, CameraView
                 unsigned long long base = 0;
                 if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)kPool);
                 base = __shfl(base, 0);
-                pool_next = base;
-                pool_end = base + kPool < items ? base + kPool : items;
-                if (base >= items) drained = true;
+                if (base >= S.items) {
+                    drained = true;
+                    pool_next = pool_end = S.items;
+                } else {
+                    pool_next = (uint32_t)base;
+                    pool_end = S.items - pool_next > (uint32_t)kPool ? pool_next + (uint32_t)kPool : S.items;
+                }
             }
-            const unsigned long long avail = pool_end > pool_next ? pool_end - pool_next : 0ull;
-            const unsigned long long rank = (unsigned long long)__popcll(mask & lane_mask_lt());
+            const uint32_t avail = pool_end - pool_next;
+            const uint32_t rank = (uint32_t)__popcll(mask & lane_mask_lt());
             if (need) {
                 if (rank < avail) {
-                    it = nee_item((int64_t)(pool_next + rank), start, cnt, chunk, spp);
-                    if (tiled) {  // whole images: visit each pose's pixels in 8x8 tiles
-                        const int64_t hw = (int64_t)cam.width * cam.height;
-                        const int64_t pose_base = (it.sensor / hw) * hw;
-                        it.sensor = pose_base + tiled_pixel(it.sensor - pose_base, cam.width, cam.height);
-                    }
-                    sensor_index(cam, it.sensor, pose, row, col);
-                    if (pose >= 0 && pose < cam.num_poses && row >= 0 && row < cam.height && col >= 0 && col < cam.width) {
+                    int s0;
+                    if (nee_item(S, cam, pool_next + rank, sensor, s0, s1, pose, row, col)) {
                         have = true;
-                        s = it.s0;
+                        s = s0;
                         acc = F3(0.0f, 0.0f, 0.0f);
                         lane.lookups = 0;
                         lane.event = kEvBegin;
@@ -328,7 +370,7 @@ __global__ void __launch_bounds__(256) k_nee_persistent(MediumView M, CameraView
                     done = true;
                 }
             }
-            const unsigned long long taken = (unsigned long long)__popcll(mask);
+            const uint32_t taken = (uint32_t)__popcll(mask);
             pool_next += taken < avail ? taken : avail;
         }
         if (__ballot(have) == 0ull) {
@@ -336,8 +378,12 @@ __global__ void __launch_bounds__(256) k_nee_persistent(MediumView M, CameraView
             continue;
         }
         if (have && lane.event == kEvBegin)
-            lane.begin_sample(M, cam, seeds, (uint32_t)((int64_t)s * total + it.sensor), pose, row, col);
-        if (have && lane.event == kEvNone) lane.step(M);
+            lane.begin_sample(M, cam, seeds, (uint32_t)s * S.total + sensor, pose, row, col);
+        if (have && lane.event == kEvStart) lane.start_ray(M);
+        for (int k = 0;;) {
+            if (have && lane.event == kEvNone) lane.step(M);
+            if (++k >= S.ksteps || __ballot(have && lane.event == kEvNone) == 0ull) break;
+        }
         if (have && lane.event == kEvCollision) lane.on_collision(M);
         if (have && lane.event == kEvTrackEnd) lane.on_track_end(M);
     }
