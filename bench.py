@@ -27,6 +27,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM traffic of one launch of the persistent NEE kernel at the default workload
+# (256^3, 512x512, 64 spp), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in KiB
+# per dispatch, each scaled by what the counter reads for a 256 MiB device copy
+# (FETCH_SIZE reads 128 MiB for it, WRITE_SIZE 256 MiB: factors 2 and 1).  The
+# raw figures and the commands are in profiles/nee_persistent_pmc_256cube_512x512x64spp.txt.
+NEE_PROFILED = {"workload": (256, 512, 512, 64), "fetch_kib": 1.8694e7, "write_kib": 1.10636e5,
+                "fetch_scale": 2.0, "write_scale": 1.0}
+
+
+def profiled_traffic(res, width, height, spp):
+    p = NEE_PROFILED
+    if p["workload"] != (res, width, height, spp) or p["fetch_kib"] is None:
+        return None
+    return 1024.0 * (p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"])
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -259,7 +275,8 @@ def main():
                                f"sun + env 0.1, 16^3 majorant grid, {args.width}x{args.height}, {args.spp} spp, "
                                f"max 16 bounces", "samples_per_step": samples, "lookups_per_step": lookups},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args.res, args.width, args.height, args.spp)},
     }
     if not args.no_secondary:
         del img, radiance, camera

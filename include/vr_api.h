@@ -73,7 +73,11 @@ enum vr_map_kind {           /* input -> output dims */
     VR_MAP_MC_COLLISION = 7, /* 6 -> 3   MCCollisionIntegrator(collision DT, exitance =
                                          SingleScatteredRadiance, environment) _maps.py:2788-2808 */
     VR_MAP_NEE_DT = 8,       /* 6 -> 3   DeltatrackingNEEPathIntegrator _maps.py:2924-2978 */
-    VR_MAP_MEDIUM_DT = 9     /* 7 -> 12  DTMediumPathIntegrator      _scenes.py:1614-1632 */
+    VR_MAP_MEDIUM_DT = 9,    /* 7 -> 12  DTMediumPathIntegrator      _scenes.py:1614-1632 */
+    VR_MAP_PATH_DT = 10,     /* 6 -> 3   DeltatrackingPathIntegrator (no NEE; defensive replay bw)
+                                         _maps.py:2886-2921, maps/radiance_DT.h */
+    VR_MAP_NEE_DRT = 11      /* 6 -> 3   DRTPathIntegrator: NEE forward, reservoir backward
+                                         _maps.py:2981-3022, maps/radiance_NEE_DRT.h */
 };
 
 enum vr_seed_mode {          /* per-thread RNG state derivation from the 4 dispatch seeds */

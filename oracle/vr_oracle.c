@@ -1097,6 +1097,349 @@ static void nee_ds_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* Delta-tracking path integrator: include/vr_path_integrator_DT.h            */
+/* ------------------------------------------------------------------------- */
+/* vr_path_integrator_DT.h:7-56 (max_bounces >= 0 ends a path after that many
+ * scatter events, with no escape term). */
+static v3 path_dt_fw(ctx_t* c, v3 x, v3 w) {
+    const vr_medium* M = c->M;
+    double tmin, tmax;
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    v3 R = V(0, 0, 0), W = V(1, 1, 1);
+    int bounces = 0;
+    maj_t mj;
+    maj_begin(M, &mj, x, w);
+    while (d > 0.00000001) {
+        double md;
+        double m = dmax(0.00000001, maj_query(&mj, &md));
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        if (rnd(c) >= s / m) { d -= t; continue; }
+        v3 sa = albedo_at(c, x);
+        v3 em = vf(M->emission_c);
+        R = vadd(R, vmul(vmul(W, V(1 - sa.x, 1 - sa.y, 1 - sa.z)), em));
+        W = vmul(W, sa);
+        v3 wo;
+        double rho = phase_sample(c, w, &wo);
+        W = vs(W, rho);
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) return R;
+        if (W.x < 0.0000001 && W.y < 0.0000001 && W.z < 0.0000001) return R;
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+    }
+    return vadd(vmul(W, vf(M->env_c)), R);
+}
+
+/* vr_path_integrator_DT.h:59-235: a defensive-sampling pass (majorant + 2 eps,
+ * collision probability (s + eps)/m) that computes the path throughput, then a
+ * replay from the same stream that scatters d<W env>/d sigma at every null and
+ * real collision.  The gradient of the emission term is not propagated (the
+ * reference's emission/albedo backward lines are commented out). */
+static void path_dt_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
+    const vr_medium* M = c->M;
+    rng_t before_path = c->rng;
+    v3 x0 = x, w0 = w;
+    double tmin, tmax;
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    const double EPS = 0.00000001;
+    v3 W = V(1, 1, 1);
+    double null_W = 1.0;
+    int bounces = 0;
+    maj_t mj;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double m = dmax(EPS, maj_query(&mj, &md) + 2 * eps);
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        double Pc = (s + eps) / m;
+        if (rnd(c) >= Pc) { d -= t; W = vs(W, (1 - s / m) / (1 - Pc)); continue; }
+        if (s == 0.0) {
+            if (null_W == 0.0) { W = V(0, 0, 0); break; }
+            null_W = 0.0;
+        } else W = vs(W, (s / m) / Pc);
+        v3 sa = albedo_at(c, x);
+        v3 wo;
+        double rho = phase_sample(c, w, &wo);
+        W = vs(vmul(W, sa), rho);
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) { W = V(0, 0, 0); break; }
+        if (W.x < EPS && W.y < EPS && W.z < EPS) { W = V(0, 0, 0); break; }
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+    }
+    v3 dL_dW = vmul(dL_dR, vf(M->env_c));
+    v3 W_out = W;
+
+    c->rng = before_path;
+    x = x0; w = w0;
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    d = tmax - tmin;
+    bounces = 0;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double m = dmax(EPS, maj_query(&mj, &md) + 2 * eps);
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        double Pc = (s + eps) / m;
+        if (rnd(c) >= Pc) {
+            sigma_bw(c, x, vdot(dL_dW, vs(W_out, -null_W / m / (1 - s / m))));
+            d -= t;
+            continue;
+        }
+        sigma_bw(c, x, vdot(dL_dW, s == 0.0 ? W_out : vs(W_out, null_W / s)));
+        if (s == 0.0) break;
+        v3 wo;
+        phase_sample(c, w, &wo);
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) return;
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* NEE with reservoir backward: include/vr_path_integrator_NEE_DRT.h           */
+/* ------------------------------------------------------------------------- */
+/* randoms.h:131-139: a seed built from 4 uniforms of the current stream. */
+static rng_t create_branch_seed(ctx_t* c) {
+    rng_t s;
+    s.x = float_to_bits(rng_random(&c->rng)) + 129u;
+    s.y = float_to_bits(rng_random(&c->rng)) + 129u;
+    s.z = float_to_bits(rng_random(&c->rng)) + 129u;
+    s.w = float_to_bits(rng_random(&c->rng)) + 129u;
+    for (int k = 0; k < 4; k++) rng_advance(&s);
+    return s;
+}
+
+/* vr_path_integrator_NEE.h:1-66 with only_scatters = true (no primary
+ * transmittance term): the continuation path of the reservoir vertex. */
+static v3 nee_scatters(ctx_t* c, v3 x, v3 w) {
+    const vr_medium* M = c->M;
+    double tmin, tmax;
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    v3 R = V(0, 0, 0), W = V(1, 1, 1);
+    const double EPS = 0.00001;
+    int bounces = 0;
+    maj_t mj;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double m = dmax(EPS, maj_query(&mj, &md));
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        if (rnd(c) >= s / m) { d -= t; continue; }
+        v3 sa = albedo_at(c, x);
+        v3 em = vf(M->emission_c);
+        R = vadd(R, vmul(vmul(W, V(1 - sa.x, 1 - sa.y, 1 - sa.z)), em));
+        W = vmul(W, sa);
+        v3 wnee;
+        v3 env = env_sample(c, &wnee);
+        double Tn = transmittance(c, x, wnee);
+        R = vadd(R, vs(vmul(W, env), Tn * phase_eval(M, w, wnee)));
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) return R;
+        v3 wo;
+        double rho = phase_sample(c, w, &wo);
+        W = vs(W, rho);
+        if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+    }
+    return R;
+}
+
+/* vr_path_integrator_NEE_DRT.h:8-44: ratio-tracking walk over [0, d) that picks
+ * one free-flight segment with probability proportional to T * dt and a
+ * uniform position inside it; weight = sum of T * dt (<T>/pdf).  The majorant
+ * is queried along the walk (the reference queries it at the start point,
+ * which is the same for a constant majorant), and the end of the segment is
+ * tested before a cell crossing. */
+static double sample_transmittance_position(ctx_t* c, v3 x, v3 w, double d, double* sample_t, double* weight_t) {
+    const vr_medium* M = c->M;
+    double T = 1.0, t = 0.0, res_t = 0.0, res_dt = 0.0, C = 0.0;
+    const double EPS = 0.0000001;
+    maj_t mj;
+    maj_begin(M, &mj, x, w);
+    for (;;) {
+        double md;
+        double m = dmax(EPS, maj_query(&mj, &md));
+        double dt = dmin(-log(1.0 - rnd(c)) / m, dmin(md, d - t));
+        double w_step = T * dt;
+        C += w_step;
+        if (rnd(c) < w_step / C) { res_t = t; res_dt = dt; }
+        t += dt;
+        /* end of the segment first: with a majorant grid the last cell's exit
+         * can coincide with it (the reference's constant majorant never crosses) */
+        if (t >= d - EPS) break;
+        if (dt >= md - EPS) { maj_cross(&mj); continue; }
+        maj_move(&mj, dt);
+        T *= 1 - sigma_at(c, vadd(x, vs(w, t))) / m;
+    }
+    *weight_t = C;
+    *sample_t = res_t + rnd(c) * res_dt;
+    return T;
+}
+
+/* vr_path_integrator_NEE_DRT.h:46-269.  The secondary stream (a branch seed
+ * drawn from the main stream) drives every transmittance backward; the main
+ * stream drives the primal pass, its replay and the reservoir-vertex term. */
+static void nee_drt_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
+    const vr_medium* M = c->M;
+    double tmin, tmax;
+    boundary(M, x, w, &tmin, &tmax);
+    x = vadd(x, vs(w, tmin));
+    double d = tmax - tmin;
+    if (d <= 0) return;
+    rng_t secondary = create_branch_seed(c), main_s;
+
+    double direct_T = transmittance(c, x, w);
+    (void)direct_T;  /* dL/denv = dL_dR * direct_T: a constant environment has no parameters */
+    double dL_ddT = vdot(dL_dR, vf(M->env_c));
+    main_s = c->rng; c->rng = secondary;
+    transmittance_bw(c, x, w, dL_ddT);
+    secondary = c->rng; c->rng = main_s;
+
+    v3 R = V(0, 0, 0), W = V(1, 1, 1);
+    v3 res_x = x, res_w = w, res_W = W;
+    double res_P = 3, total_weight = 3;
+    v3 ini_x = x, ini_w = w;
+    double ini_d = d;
+    const double EPS = 0.00000001;
+    rng_t before_path = c->rng;
+    int bounces = 0;
+    maj_t mj;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double m = maj_query(&mj, &md) * 1.05 + 0.1;
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        if (rnd(c) >= s / m) { d -= t; continue; }
+        v3 sa = albedo_at(c, x);
+        W = vmul(W, sa);
+        v3 wnee;
+        v3 env = env_sample(c, &wnee);
+        double Tn = transmittance(c, x, wnee);
+        double phn = phase_eval(M, w, wnee);
+        R = vadd(R, vs(vmul(W, env), Tn * phn));
+        bounces++;
+        if (M->max_bounces >= 0 && bounces >= M->max_bounces) break;
+        v3 wo;
+        double rho = phase_sample(c, w, &wo);
+        W = vs(W, rho);
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+        double cur_P = W.x + W.y + W.z;
+        total_weight += cur_P;
+        if (rnd(c) < cur_P / total_weight) { res_P = cur_P; res_W = W; res_x = x; res_w = w; }
+        if (W.x < EPS && W.y < EPS && W.z < EPS) break;
+    }
+
+    /* replay: null collisions and the MIS share s^2/(1+s^2) of real collisions */
+    W = V(1, 1, 1);
+    x = ini_x; w = ini_w; d = ini_d;
+    c->rng = before_path;
+    bounces = 0;
+    maj_begin(M, &mj, x, w);
+    while (d > EPS) {
+        double md;
+        double m = maj_query(&mj, &md) * 1.05 + 0.1;
+        double t = -log(1.0 - rnd(c)) / m;
+        x = vadd(x, vs(w, dmin(t, md)));
+        if (t > md) { d -= md; maj_cross(&mj); if (d <= CROSS_EPS) break; continue; }
+        if (t > d) break;
+        maj_move(&mj, t);
+        double s = sigma_at(c, x);
+        double Pc = s / m;
+        if (rnd(c) >= Pc) {
+            double g = vdot(dL_dR, vs(R, -1.0 / m / (1 - s / m)));
+            main_s = c->rng; c->rng = secondary;
+            sigma_bw(c, x, g);
+            secondary = c->rng; c->rng = main_s;
+            d -= t;
+            continue;
+        }
+        double g = vdot(dL_dR, vs(R, (s * s / (1 + s * s)) / s));
+        v3 sa = albedo_at(c, x);
+        W = vmul(W, sa);
+        v3 wnee;
+        v3 env = env_sample(c, &wnee);
+        double Tn = transmittance(c, x, wnee);
+        double phn = phase_eval(M, w, wnee);
+        bounces++;
+        const int last = M->max_bounces >= 0 && bounces >= M->max_bounces;
+        v3 wo = w;
+        double rho = last ? 1.0 : phase_sample(c, w, &wo);
+        double dL_dTn = vdot(dL_dR, vs(vmul(W, env), phn));
+        R = vsub(R, vs(vmul(W, env), Tn * phn));
+        main_s = c->rng; c->rng = secondary;
+        sigma_bw(c, x, g);
+        transmittance_bw(c, x, wnee, dL_dTn);
+        secondary = c->rng; c->rng = main_s;
+        if (last) break;
+        W = vs(W, rho);
+        w = wo;
+        boundary(M, x, w, &tmin, &d);
+        maj_begin(M, &mj, x, w);
+        rnd(c);  /* the reservoir draw of the primal pass */
+        if (W.x < EPS && W.y < EPS && W.z < EPS) break;
+    }
+
+    /* reservoir vertex: the remaining 1/(1+s^2) share of the collision term */
+    boundary(M, res_x, res_w, &tmin, &d);
+    double s_t, weight_t;
+    sample_transmittance_position(c, res_x, res_w, d, &s_t, &weight_t);
+    v3 xs = vadd(res_x, vs(res_w, s_t));
+    v3 lw;
+    v3 Lenv = env_sample(c, &lw);
+    v3 Ls = vs(Lenv, transmittance(c, xs, lw) * phase_eval(M, res_w, lw));
+    v3 ws;
+    double rho_s = phase_sample(c, res_w, &ws);
+    Ls = vadd(Ls, vs(nee_scatters(c, xs, ws), rho_s));
+    Ls = vmul(vs(vmul(Ls, res_W), weight_t), albedo_at(c, xs));
+    double sxs = sigma_at(c, xs);
+    double g = vdot(dL_dR, Ls) * (1 / (1 + sxs * sxs)) * total_weight / res_P;
+    main_s = c->rng; c->rng = secondary;
+    sigma_bw(c, xs, g);
+    secondary = c->rng; c->rng = main_s;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Medium path integrator (7 -> 12): include/sc_medium_path_integrator_DT_*.h  */
 /* ------------------------------------------------------------------------- */
 static void medium_dt_fw(ctx_t* c, v3 x, v3 w, double d, double* out) {
@@ -1188,6 +1531,7 @@ static int map_dims(int kind, int* in_dim, int* out_dim) {
     case VR_MAP_RAYBOX: *in_dim = 6; *out_dim = 2; return 1;
     case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: *in_dim = 6; *out_dim = 1; return 1;
     case VR_MAP_COLLISION_DT: case VR_MAP_SS_RADIANCE: case VR_MAP_MC_COLLISION: case VR_MAP_NEE_DT:
+    case VR_MAP_PATH_DT: case VR_MAP_NEE_DRT:
         *in_dim = 6; *out_dim = 3; return 1;
     case VR_MAP_MEDIUM_DT: *in_dim = 7; *out_dim = 12; return 1;
     default: return 0;
@@ -1204,7 +1548,8 @@ static void field_fw(ctx_t* c, int kind, const double* in, double* out) {
     case VR_MAP_COLLISION_DT: { double t, T; out[0] = collision_dt(c, x, w, &t, &T); out[1] = t; out[2] = T; break; }
     case VR_MAP_SS_RADIANCE: { v3 r = ss_radiance(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
     case VR_MAP_MC_COLLISION: { v3 r = mc_collision(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
-    case VR_MAP_NEE_DT: { v3 r = nee_fw(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
+    case VR_MAP_NEE_DT: case VR_MAP_NEE_DRT: { v3 r = nee_fw(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
+    case VR_MAP_PATH_DT: { v3 r = path_dt_fw(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
     case VR_MAP_MEDIUM_DT: medium_dt_fw(c, x, w, in[6], out); break;
     default: break;
     }
@@ -1220,6 +1565,8 @@ static void field_bw(ctx_t* c, int kind, const double* in, const double* og) {
     case VR_MAP_SS_RADIANCE: ss_radiance_bw(c, x, w, V(og[0], og[1], og[2])); break;
     case VR_MAP_MC_COLLISION: mc_collision_bw(c, x, w, V(og[0], og[1], og[2])); break;
     case VR_MAP_NEE_DT: nee_ds_bw(c, x, w, c->M->ds_epsilon, V(og[0], og[1], og[2])); break;
+    case VR_MAP_PATH_DT: path_dt_bw(c, x, w, c->M->ds_epsilon, V(og[0], og[1], og[2])); break;
+    case VR_MAP_NEE_DRT: nee_drt_bw(c, x, w, V(og[0], og[1], og[2])); break;
     case VR_MAP_MEDIUM_DT: medium_dt_bw(c, x, w, in[6], V(og[6], og[7], og[8]), V(og[9], og[10], og[11])); break;
     default: break;
     }

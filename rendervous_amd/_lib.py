@@ -31,6 +31,8 @@ VR_MAP_SS_RADIANCE = 6
 VR_MAP_MC_COLLISION = 7
 VR_MAP_NEE_DT = 8
 VR_MAP_MEDIUM_DT = 9
+VR_MAP_PATH_DT = 10
+VR_MAP_NEE_DRT = 11
 VR_SEED_MAP, VR_SEED_CAPTURE_FW, VR_SEED_CAPTURE_BW = 0, 1, 2
 VR_BW_REFERENCE, VR_BW_CONSISTENT = 0, 1
 
@@ -44,6 +46,8 @@ MAP_DIMS = {
     VR_MAP_MC_COLLISION: (6, 3),
     VR_MAP_NEE_DT: (6, 3),
     VR_MAP_MEDIUM_DT: (7, 12),
+    VR_MAP_PATH_DT: (6, 3),
+    VR_MAP_NEE_DRT: (6, 3),
 }
 
 F3 = ctypes.c_float * 3

@@ -83,6 +83,7 @@ int map_dims(int kind, int& in_dim, int& out_dim) {
     case VR_MAP_RAYBOX: in_dim = 6; out_dim = 2; return 1;
     case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: in_dim = 6; out_dim = 1; return 1;
     case VR_MAP_COLLISION_DT: case VR_MAP_SS_RADIANCE: case VR_MAP_MC_COLLISION: case VR_MAP_NEE_DT:
+    case VR_MAP_PATH_DT: case VR_MAP_NEE_DRT:
         in_dim = 6; out_dim = 3; return 1;
     case VR_MAP_MEDIUM_DT: in_dim = 7; out_dim = 12; return 1;
     default: return 0;
@@ -188,8 +189,11 @@ __device__ __forceinline__ void field_fw(Tracer<MAJ>& tr, const float* in, float
     } else if constexpr (KIND == VR_MAP_MC_COLLISION) {
         f3 r = tr.mc_collision(x, w);
         out[0] = r.x; out[1] = r.y; out[2] = r.z;
-    } else if constexpr (KIND == VR_MAP_NEE_DT) {
+    } else if constexpr (KIND == VR_MAP_NEE_DT || KIND == VR_MAP_NEE_DRT) {
         f3 r = tr.nee(x, w);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    } else if constexpr (KIND == VR_MAP_PATH_DT) {
+        f3 r = tr.path_dt(x, w);
         out[0] = r.x; out[1] = r.y; out[2] = r.z;
     } else if constexpr (KIND == VR_MAP_MEDIUM_DT) {
         tr.medium_dt(x, w, in[6], out);
@@ -213,6 +217,10 @@ __device__ __forceinline__ void field_bw(Tracer<MAJ>& tr, const float* in, const
         tr.mc_collision_bw(x, w, F3(og[0], og[1], og[2]));
     } else if constexpr (KIND == VR_MAP_NEE_DT) {
         tr.nee_ds_bw(x, w, tr.M.ds_eps, F3(og[0], og[1], og[2]));
+    } else if constexpr (KIND == VR_MAP_PATH_DT) {
+        tr.path_dt_bw(x, w, tr.M.ds_eps, F3(og[0], og[1], og[2]));
+    } else if constexpr (KIND == VR_MAP_NEE_DRT) {
+        tr.nee_drt_bw(x, w, F3(og[0], og[1], og[2]));
     } else if constexpr (KIND == VR_MAP_MEDIUM_DT) {
         tr.medium_dt_bw(x, w, in[6], F3(og[6], og[7], og[8]), F3(og[9], og[10], og[11]));
     }
@@ -229,6 +237,8 @@ template <> struct Dims<VR_MAP_SS_RADIANCE> { static constexpr int in = 6, out =
 template <> struct Dims<VR_MAP_MC_COLLISION> { static constexpr int in = 6, out = 3; };
 template <> struct Dims<VR_MAP_NEE_DT> { static constexpr int in = 6, out = 3; };
 template <> struct Dims<VR_MAP_MEDIUM_DT> { static constexpr int in = 7, out = 12; };
+template <> struct Dims<VR_MAP_PATH_DT> { static constexpr int in = 6, out = 3; };
+template <> struct Dims<VR_MAP_NEE_DRT> { static constexpr int in = 6, out = 3; };
 
 // ---------------------------------------------------------------- kernels --
 __global__ void __launch_bounds__(kBlock) k_grid_fw(GridView g, const float* __restrict__ x, int64_t n, float* __restrict__ out) {
@@ -389,10 +399,11 @@ __global__ void __launch_bounds__(kBlock) k_adam(float* __restrict__ p, float* _
 }
 
 // --------------------------------------------------- persistent NEE queue --
-// One 64-bit work counter per device, allocated on first use and kept for the
+// The persistent NEE kernel's queue counters (kMaxQueues 64-bit words, one per
+// 128-byte line), one set per device, allocated on first use and kept for the
 // life of the process.  Launches on one device are ordered by their streams;
 // two captures running concurrently on different streams of one device would
-// share the counter, so the map layer serialises captures.
+// share the counters, so the map layer serialises captures.
 constexpr int kMaxDevices = 64;
 std::mutex g_counter_mu;
 unsigned long long* g_counters[kMaxDevices] = {};
@@ -405,7 +416,8 @@ int work_counter(unsigned long long** out) {
     std::lock_guard<std::mutex> lock(g_counter_mu);
     if (!g_counters[dev]) {
         void* p = nullptr;
-        if ((st = check_hip(hipMalloc(&p, 256), "hipMalloc(work counter)"))) return st;
+        if ((st = check_hip(hipMalloc(&p, kMaxQueues * kQueueStride * sizeof(unsigned long long)), "hipMalloc(work counters)")))
+            return st;
         g_counters[dev] = (unsigned long long*)p;
     }
     *out = g_counters[dev];
@@ -454,9 +466,11 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         S.total = (uint32_t)total;
         S.start = (uint32_t)b0;
         S.cnt = (uint32_t)n;
-        S.items = (uint32_t)(n * ((spp + chunk - 1) / chunk));
+        S.nq = std::max(1, std::min(kMaxQueues, env_int("VR_NEE_QUEUES", 1)));
+        S.qpix = (uint32_t)((n + S.nq - 1) / S.nq);
         S.spp = spp;
         S.chunk = (int)std::min<int64_t>(chunk, spp);
+        S.chunks = (spp + S.chunk - 1) / S.chunk;
         S.tiled = !C.sensors && b0 % hw == 0 && n % hw == 0;
         S.ksteps = std::max(1, env_int("VR_NEE_STEPS", 3));
         unsigned prep = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
@@ -477,6 +491,7 @@ int launch_map_fw_maj(int kind, const MediumView& M, const float* in, int64_t st
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_map_fw<K, MAJ>), g, b, 0, st, M, in, start, cnt, out, sd, dbg); break;
         VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_MEDIUM_DT)
+        VR_CASE(VR_MAP_PATH_DT) VR_CASE(VR_MAP_NEE_DRT)
 #undef VR_CASE
     default: return set_err(VR_ERR_UNSUPPORTED, "unknown map kind");
     }
@@ -491,6 +506,7 @@ int launch_map_bw_maj(int kind, const MediumView& M, const float* in, const floa
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_map_bw<K, MAJ>), g, b, 0, st, M, in, og, start, cnt, sd); break;
         VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_MEDIUM_DT)
+        VR_CASE(VR_MAP_PATH_DT) VR_CASE(VR_MAP_NEE_DRT)
 #undef VR_CASE
     case VR_MAP_RAYBOX: return VR_OK;  // no parameters
     default: return set_err(VR_ERR_UNSUPPORTED, "unknown map kind");
@@ -502,12 +518,15 @@ template <int MAJ>
 int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
                           int64_t cnt, float* out, int* dbg, hipStream_t st) {
     dim3 g(grid_for(cnt)), b(kBlock);
-    if (kind == VR_MAP_NEE_DT && M.t_kind == VR_T_DT) return launch_nee_persistent<MAJ, VR_T_DT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
-    if (kind == VR_MAP_NEE_DT && M.t_kind == VR_T_RT) return launch_nee_persistent<MAJ, VR_T_RT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+    // the NEE forward (also DRTPathIntegrator's) with a DT or RT shadow estimator runs persistent
+    const bool nee = kind == VR_MAP_NEE_DT || kind == VR_MAP_NEE_DRT;
+    if (nee && M.t_kind == VR_T_DT) return launch_nee_persistent<MAJ, VR_T_DT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+    if (nee && M.t_kind == VR_T_RT) return launch_nee_persistent<MAJ, VR_T_RT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
     switch (kind) {
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_fw<K, MAJ>), g, b, 0, st, M, C, sd, spp, total, start, cnt, out, dbg); break;
         VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
-        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT)
+        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_PATH_DT)
+        VR_CASE(VR_MAP_NEE_DRT)
 #undef VR_CASE
     default: return set_err(VR_ERR_UNSUPPORTED, "map kind cannot be captured (needs 6 inputs)");
     }
@@ -521,7 +540,8 @@ int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Se
     switch (kind) {
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_bw<K, MAJ>), g, b, 0, st, M, C, sd, spp, seed_mode, total, start, cnt, og, dbg); break;
         VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
-        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT)
+        VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_PATH_DT)
+        VR_CASE(VR_MAP_NEE_DRT)
 #undef VR_CASE
     case VR_MAP_RAYBOX: return VR_OK;
     default: return set_err(VR_ERR_UNSUPPORTED, "map kind cannot be captured (needs 6 inputs)");

@@ -1080,6 +1080,323 @@ struct Tracer {
         }
     }
 
+    // include/vr_path_integrator_DT.h:7-56 (max_bounces >= 0 ends the path after
+    // that many scatter events, with no escape term)
+    VR_DEV f3 path_dt(f3 x, f3 w) {
+        float tmin, tmax;
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        f3 R = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
+        int bounces = 0;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        while (d > 0.00000001f) {
+            float md;
+            float m = fmaxf(0.00000001f, mj.query(M, md));
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            if (rnd() * m >= s) { d -= t; continue; }
+            f3 sa = albedo(x);
+            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            W = W * sa;
+            f3 wo;
+            float rho = phase_sample(M, rng, w, wo);
+            W = W * rho;
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
+            if (W.x < 0.0000001f && W.y < 0.0000001f && W.z < 0.0000001f) return R;
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+        }
+        return W * M.env_c + R;
+    }
+
+    // include/vr_path_integrator_DT.h:59-235: defensive pass (majorant + 2 eps,
+    // collision probability (s + eps)/m) for the path throughput, then a replay
+    // of the same stream scattering d<W env>/d sigma at every collision.
+    VR_DEV void path_dt_bw(f3 x, f3 w, float eps, f3 dL_dR) {
+        Rng before_path = rng;
+        f3 x0 = x, w0 = w;
+        float tmin, tmax;
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        const float EPS = 0.00000001f;
+        f3 W = F3(1.0f, 1.0f, 1.0f);
+        float null_W = 1.0f;
+        int bounces = 0;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        while (d > EPS) {
+            float md;
+            float m = fmaxf(EPS, mj.query(M, md) + 2.0f * eps);
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            float Pc = (s + eps) / m;
+            if (rnd() >= Pc) { d -= t; W = W * ((1.0f - s / m) / (1.0f - Pc)); continue; }
+            if (s == 0.0f) {
+                if (null_W == 0.0f) { W = F3(0.0f, 0.0f, 0.0f); break; }
+                null_W = 0.0f;
+            } else W = W * ((s / m) / Pc);
+            f3 sa = albedo(x);
+            f3 wo;
+            float rho = phase_sample(M, rng, w, wo);
+            W = (W * sa) * rho;
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
+            if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); break; }
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+        }
+        const f3 dL_dW = dL_dR * M.env_c;
+        const f3 W_out = W;
+
+        rng = before_path;
+        x = x0; w = w0;
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        d = tmax - tmin;
+        bounces = 0;
+        mj.begin(M, x, w);
+        while (d > EPS) {
+            float md;
+            float m = fmaxf(EPS, mj.query(M, md) + 2.0f * eps);
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            float Pc = (s + eps) / m;
+            if (rnd() >= Pc) {
+                sigma_bw(x, dot(dL_dW, W_out * (-null_W / m / (1.0f - s / m))));
+                d -= t;
+                continue;
+            }
+            sigma_bw(x, dot(dL_dW, s == 0.0f ? W_out : W_out * (null_W / s)));
+            if (s == 0.0f) break;
+            f3 wo;
+            phase_sample(M, rng, w, wo);
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return;
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+        }
+    }
+
+    // randoms.h:131-139
+    VR_DEV Rng create_branch_seed() {
+        Rng s;
+        s.x = __float_as_uint(rnd()) + 129u;
+        s.y = __float_as_uint(rnd()) + 129u;
+        s.z = __float_as_uint(rnd()) + 129u;
+        s.w = __float_as_uint(rnd()) + 129u;
+        rng_advance(s); rng_advance(s); rng_advance(s); rng_advance(s);
+        return s;
+    }
+
+    // include/vr_path_integrator_NEE.h:1-66 with only_scatters = true
+    VR_DEV f3 nee_scatters(f3 x, f3 w) {
+        float tmin, tmax;
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        f3 R = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
+        const float EPS = 0.00001f;
+        int bounces = 0;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        while (d > EPS) {
+            float md;
+            float m = fmaxf(EPS, mj.query(M, md));
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            if (rnd() * m >= s) { d -= t; continue; }
+            f3 sa = albedo(x);
+            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            W = W * sa;
+            f3 wnee;
+            f3 env = env_sample(M, rng, wnee);
+            float Tn = transmittance(x, wnee);
+            R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
+            f3 wo;
+            float rho = phase_sample(M, rng, w, wo);
+            W = W * rho;
+            if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+        }
+        return R;
+    }
+
+    // include/vr_path_integrator_NEE_DRT.h:8-44 (the majorant is queried along
+    // the walk -- the reference queries it at the start point, which is the same
+    // for a constant majorant -- and the end of the segment is tested first)
+    VR_DEV float sample_transmittance_position(f3 x, f3 w, float d, float& sample_t, float& weight_t) {
+        float T = 1.0f, t = 0.0f, res_t = 0.0f, res_dt = 0.0f, C = 0.0f;
+        const float EPS = 0.0000001f;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        for (;;) {
+            float md;
+            float m = fmaxf(EPS, mj.query(M, md));
+            float dt = fminf(flight(m), fminf(md, d - t));
+            float w_step = T * dt;
+            C += w_step;
+            if (rnd() < w_step / C) { res_t = t; res_dt = dt; }
+            t += dt;
+            if (t >= d - EPS) break;  // before a crossing: the last cell's exit can coincide with the end
+            if (dt >= md - EPS) { mj.cross(M, w); continue; }
+            mj.move(dt);
+            T *= 1.0f - sigma(x + w * t) / m;
+        }
+        weight_t = C;
+        sample_t = res_t + rnd() * res_dt;
+        return T;
+    }
+
+    // include/vr_path_integrator_NEE_DRT.h:46-269: primal pass with a reservoir
+    // over the scattered segments, replay with the MIS share s^2/(1+s^2) of each
+    // real collision, then the remaining share at a transmittance-sampled point
+    // of the reservoir segment.  A branch seed drawn from the main stream drives
+    // every transmittance backward.
+    VR_DEV void nee_drt_bw(f3 x, f3 w, f3 dL_dR) {
+        float tmin, tmax;
+        boundary(M, x, w, tmin, tmax);
+        x = x + w * tmin;
+        float d = tmax - tmin;
+        if (d <= 0.0f) return;
+        Rng secondary = create_branch_seed(), main_s;
+        transmittance(x, w);  // the primary term's T: draws from the main stream
+        main_s = rng; rng = secondary;
+        transmittance_bw(x, w, dot(dL_dR, M.env_c));
+        secondary = rng; rng = main_s;
+
+        f3 R = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
+        f3 res_x = x, res_w = w, res_W = W;
+        float res_P = 3.0f, total_weight = 3.0f;
+        const f3 ini_x = x, ini_w = w;
+        const float ini_d = d;
+        const float EPS = 0.00000001f;
+        const Rng before_path = rng;
+        int bounces = 0;
+        Walk<MAJ> mj;
+        mj.begin(M, x, w);
+        while (d > EPS) {
+            float md;
+            float m = mj.query(M, md) * 1.05f + 0.1f;
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            if (rnd() >= s / m) { d -= t; continue; }
+            W = W * albedo(x);
+            f3 wnee;
+            f3 env = env_sample(M, rng, wnee);
+            float Tn = transmittance(x, wnee);
+            float phn = phase_eval(M, w, wnee);
+            R = R + (W * env) * (Tn * phn);
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) break;
+            f3 wo;
+            float rho = phase_sample(M, rng, w, wo);
+            W = W * rho;
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+            float cur_P = W.x + W.y + W.z;
+            total_weight += cur_P;
+            if (rnd() < cur_P / total_weight) { res_P = cur_P; res_W = W; res_x = x; res_w = w; }
+            if (W.x < EPS && W.y < EPS && W.z < EPS) break;
+        }
+
+        W = F3(1.0f, 1.0f, 1.0f);
+        x = ini_x; w = ini_w; d = ini_d;
+        rng = before_path;
+        bounces = 0;
+        mj.begin(M, x, w);
+        while (d > EPS) {
+            float md;
+            float m = mj.query(M, md) * 1.05f + 0.1f;
+            float t = flight(m);
+            x = x + w * fminf(t, md);
+            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > d) break;
+            mj.move(t);
+            float s = sigma(x);
+            float Pc = s / m;
+            if (rnd() >= Pc) {
+                float g = dot(dL_dR, R * (-1.0f / m / (1.0f - s / m)));
+                main_s = rng; rng = secondary;
+                sigma_bw(x, g);
+                secondary = rng; rng = main_s;
+                d -= t;
+                continue;
+            }
+            float g = dot(dL_dR, R * ((s * s / (1.0f + s * s)) / s));
+            W = W * albedo(x);
+            f3 wnee;
+            f3 env = env_sample(M, rng, wnee);
+            float Tn = transmittance(x, wnee);
+            float phn = phase_eval(M, w, wnee);
+            bounces++;
+            const bool last = M.max_bounces >= 0 && bounces >= M.max_bounces;
+            f3 wo = w;
+            float rho = last ? 1.0f : phase_sample(M, rng, w, wo);
+            float dL_dTn = dot(dL_dR, (W * env) * phn);
+            R = R - (W * env) * (Tn * phn);
+            main_s = rng; rng = secondary;
+            sigma_bw(x, g);
+            transmittance_bw(x, wnee, dL_dTn);
+            secondary = rng; rng = main_s;
+            if (last) break;
+            W = W * rho;
+            w = wo;
+            boundary(M, x, w, tmin, d);
+            mj.begin(M, x, w);
+            rnd();  // the reservoir draw of the primal pass
+            if (W.x < EPS && W.y < EPS && W.z < EPS) break;
+        }
+
+        boundary(M, res_x, res_w, tmin, d);
+        float s_t, weight_t;
+        sample_transmittance_position(res_x, res_w, d, s_t, weight_t);
+        f3 xs = res_x + res_w * s_t;
+        f3 lw;
+        f3 Lenv = env_sample(M, rng, lw);
+        f3 Ls = Lenv * (transmittance(xs, lw) * phase_eval(M, res_w, lw));
+        f3 ws;
+        float rho_s = phase_sample(M, rng, res_w, ws);
+        Ls = Ls + nee_scatters(xs, ws) * rho_s;
+        Ls = ((Ls * res_W) * weight_t) * albedo(xs);
+        float sxs = sigma(xs);
+        float g = dot(dL_dR, Ls) * (1.0f / (1.0f + sxs * sxs)) * total_weight / res_P;
+        main_s = rng; rng = secondary;
+        sigma_bw(xs, g);
+        secondary = rng; rng = main_s;
+    }
+
     // include/sc_medium_path_integrator_DT_fw.h:1-32
     VR_DEV void medium_dt(f3 x, f3 w, float d, float* out) {
         f3 W = F3(1.0f, 1.0f, 1.0f), A = F3(0.0f, 0.0f, 0.0f);

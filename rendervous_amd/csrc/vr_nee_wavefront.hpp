@@ -235,16 +235,26 @@ struct NeeLane {
 };
 
 // Launch-wide schedule of a persistent NEE capture.  Sensors [start, start+cnt)
-// of a capture over `total` sensors are split into items of `chunk`
-// consecutive samples: item j covers sensor start + j % cnt and samples
-// [c*chunk, min(spp, (c+1)*chunk)) with c = j / cnt.  All counts fit 32 bits
-// (the host splits larger captures).
+// of a capture over `total` sensors are cut into `nq` contiguous ranges of `qpix`
+// sensors (the last one shorter), one work queue each.  Queue q hands out items
+// of `chunk` consecutive samples: its item l covers sensor q*qpix + l % n_q and
+// samples [c*chunk, min(spp, (c+1)*chunk)) with c = l / n_q, n_q the queue's
+// sensor count.  All counts fit 32 bits (the host splits larger captures).
 struct NeeSchedule {
-    uint32_t total, start, cnt, items;
+    uint32_t total, start, cnt;
+    uint32_t qpix;  // sensors per queue
+    int nq;         // work queues (one per XCD); waves start on queue blockIdx.x % nq
     int spp, chunk;
-    int tiled;   // whole images without a sensor list: visit each pose in 8x8 tiles
-    int ksteps;  // tracking steps per iteration of the event loop
+    int chunks;     // ceil(spp / chunk)
+    int tiled;      // whole images without a sensor list: visit each pose in 8x8 tiles
+    int ksteps;     // tracking steps per iteration of the event loop
 };
+
+// Sensors and items of queue q.
+VR_DEV uint32_t nee_queue_sensors(const NeeSchedule& S, int q) {
+    const uint32_t b = (uint32_t)q * S.qpix;
+    return b >= S.cnt ? 0u : min(S.qpix, S.cnt - b);
+}
 
 // Pixel of the k-th position of a width x height image visited in bands of 8
 // rows and, within a band, in blocks of 8 columns (row-major inside a block),
@@ -261,11 +271,12 @@ VR_DEV void tiled_pixel(uint32_t k, uint32_t width, uint32_t height, int& row, i
     col = (int)(blk * 8u + (q - r * bw));
 }
 
-// Decodes work item j; false if its sensor lies outside the camera.
-VR_DEV bool nee_item(const NeeSchedule& S, const CameraView& cam, uint32_t j, uint32_t& sensor, int& s0, int& s1,
+// Decodes item l of queue q; false if its sensor lies outside the camera.
+VR_DEV bool nee_item(const NeeSchedule& S, const CameraView& cam, int q, uint32_t l, uint32_t& sensor, int& s0, int& s1,
                      int& pose, int& row, int& col) {
-    const uint32_t c = j / S.cnt;
-    const uint32_t idx = S.start + (j - c * S.cnt);
+    const uint32_t nqs = nee_queue_sensors(S, q);
+    const uint32_t c = l / nqs;
+    const uint32_t idx = S.start + (uint32_t)q * S.qpix + (l - c * nqs);
     s0 = (int)c * S.chunk;
     s1 = min(S.spp, s0 + S.chunk);
     sensor = idx;
@@ -296,27 +307,37 @@ VR_DEV bool nee_item(const NeeSchedule& S, const CameraView& cam, uint32_t j, ui
 #define VR_NEE_WAVES_PER_EU 3
 #endif
 
-// Item indices a wave takes from the global counter per atomic.
+// Item indices a wave takes from a queue counter per atomic.
 constexpr int kPool = 64;
+// Work queues of a launch (one per XCD) and the spacing of their counters
+// (one 128-byte cache line each).
+constexpr int kMaxQueues = 8;
+constexpr int kQueueStride = 16;
 
 VR_DEV unsigned long long lane_mask_lt() {
     unsigned lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Persistent kernel: every lane pulls work items until the queue drains.  The
-// output must be zero on entry (k_nee_prepare); item sums are added atomically.
+// Persistent kernel: every lane pulls work items until the queues drain.  A
+// wave starts on the queue of its XCD slot (blockIdx.x % nq: workgroups are
+// dispatched to the XCDs round-robin), so each XCD's L2 serves one contiguous
+// part of the image, and moves on to the next queue when that one is empty.
+// The output must be zero on entry (k_nee_prepare); item sums are added
+// atomically.
 template <int MAJ, int TKIND>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU))) k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S,
-                                                        unsigned long long* __restrict__ counter, float* __restrict__ out,
-                                                        int* __restrict__ dbg) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))
+k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsigned long long* __restrict__ counters,
+                 float* __restrict__ out, int* __restrict__ dbg) {
     const float inv = S.spp == 1 ? 1.0f : 1.0f / (float)S.spp;
     NeeLane<MAJ, TKIND> lane;
     lane.lookups = 0;
     lane.event = kEvNone;
     bool have = false, done = false;
-    bool drained = false;                // wave-uniform: the global queue is empty
-    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform pool of item indices
+    // wave-uniform queue state: current queue, queues tried, pool of item indices
+    int q = (int)(blockIdx.x % (unsigned)S.nq), tried = 1;
+    bool drained = false;
+    uint32_t pool_next = 0, pool_end = 0;
     uint32_t sensor = 0;
     int s = 0, s1 = 0, pose = 0, row = 0, col = 0;
     f3 acc = F3(0.0f, 0.0f, 0.0f);
@@ -342,16 +363,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE
         const bool need = !have && !done;
         const unsigned long long mask = __ballot(need);
         if (mask) {
-            if (pool_next >= pool_end && !drained) {
+            while (pool_next >= pool_end && !drained) {
+                const uint32_t items = nee_queue_sensors(S, q) * (uint32_t)S.chunks;
                 unsigned long long base = 0;
-                if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)kPool);
+                if (__lane_id() == 0) base = atomicAdd(counters + q * kQueueStride, (unsigned long long)kPool);
                 base = __shfl(base, 0);
-                if (base >= S.items) {
-                    drained = true;
-                    pool_next = pool_end = S.items;
-                } else {
+                if (base < items) {
                     pool_next = (uint32_t)base;
-                    pool_end = S.items - pool_next > (uint32_t)kPool ? pool_next + (uint32_t)kPool : S.items;
+                    pool_end = items - pool_next > (uint32_t)kPool ? pool_next + (uint32_t)kPool : items;
+                } else if (tried < S.nq) {
+                    q = q + 1 == S.nq ? 0 : q + 1;
+                    tried++;
+                } else {
+                    drained = true;
                 }
             }
             const uint32_t avail = pool_end - pool_next;
@@ -359,7 +383,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE
             if (need) {
                 if (rank < avail) {
                     int s0;
-                    if (nee_item(S, cam, pool_next + rank, sensor, s0, s1, pose, row, col)) {
+                    if (nee_item(S, cam, q, pool_next + rank, sensor, s0, s1, pose, row, col)) {
                         have = true;
                         s = s0;
                         acc = F3(0.0f, 0.0f, 0.0f);
@@ -374,7 +398,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE
             pool_next += taken < avail ? taken : avail;
         }
         if (__ballot(have) == 0ull) {
-            if (__ballot(!done) == 0ull) break;  // every lane of the wave has drained the queue
+            if (__ballot(!done) == 0ull) break;  // every lane of the wave has drained the queues
             continue;
         }
         if (have && lane.event == kEvBegin)
@@ -389,12 +413,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE
     }
 }
 
-// Zeroes the capture output rows [start, start+cnt) (3 floats each) and the work counter.
+// Zeroes the capture output rows [start, start+cnt) (3 floats each) and the queue counters.
 __global__ void k_nee_prepare(float* __restrict__ out, int64_t start, int64_t cnt, int* __restrict__ dbg,
-                              unsigned long long* __restrict__ counter) {
+                              unsigned long long* __restrict__ counters) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    if (i == 0) *counter = 0ull;
+    if (i < kMaxQueues) counters[i * kQueueStride] = 0ull;
     for (; i < cnt; i += stride) {
         out[3 * (start + i)] = 0.0f;
         out[3 * (start + i) + 1] = 0.0f;

@@ -24,7 +24,7 @@ __all__ = [
     "MajorantGrid", "GridDDATransmittance", "DeltatrackingTransmittance", "RatiotrackingTransmittance",
     "DeltatrackingCollisionSampler", "HGPhase", "VHGPhaseSampler", "UniformDirectionSampler", "_functionsampler",
     "DirectionalLightEnvironmentSampler", "SingleScatteredRadiance", "MCCollisionIntegrator",
-    "DeltatrackingNEEPathIntegrator", "SensorsBase", "PerspectiveCameraSensor", "normalized_box",
+    "DeltatrackingNEEPathIntegrator", "DeltatrackingPathIntegrator", "DRTPathIntegrator", "SensorsBase", "PerspectiveCameraSensor", "normalized_box",
     "AutogradMapFunction", "AutogradCaptureFunction",
 ]
 
@@ -462,6 +462,18 @@ class _Lowered:
         else:
             raise NotImplementedError("phase must be HGPhase + VHGPhaseSampler or isotropic")
 
+    def set_phase_sampler(self, sampler: MapBase):
+        """A phase sampler alone (integrators that never evaluate the phase)."""
+        if isinstance(sampler, VHGPhaseSampler):
+            if not isinstance(sampler.phase_g, ConstantMap):
+                raise NotImplementedError("phase g must be constant")
+            self.medium.phase_kind = L.VR_PHASE_HG_MIX if sampler.mixture else L.VR_PHASE_HG
+            self.medium.phase_g = sampler.phase_g.vec(1)[0]
+        elif _is_uniform_phase_sampler(sampler):
+            self.medium.phase_kind = L.VR_PHASE_ISO
+        else:
+            raise NotImplementedError("phase sampler must be a VHGPhaseSampler or uniform")
+
     def set_transmittance(self, t: MapBase):
         if isinstance(t, GridDDATransmittance):
             self.set_sigma(t.grid_model)
@@ -657,6 +669,80 @@ class DeltatrackingNEEPathIntegrator(_RayMap):
         lw.medium.ds_epsilon = self.ds_epsilon
         lw.medium.max_bounces = int(self.max_bounces)
         return L.VR_MAP_NEE_DT, lw
+
+
+class DeltatrackingPathIntegrator(_RayMap):
+    """Delta-tracking path tracer without NEE: radiance = W * environment at the
+    path's escape plus emission (_maps.py:2886-2921, maps/radiance_DT.h,
+    vr_path_integrator_DT.h:7-56); backward by defensive sampling and replay
+    (vr_path_integrator_DT.h:59-235).  max_bounces as in
+    DeltatrackingNEEPathIntegrator (a capped path has no escape term).
+    """
+
+    def __init__(self, sigma: MapBase, scattering_albedo: MapBase, emission: MapBase, environment: MapBase,
+                 phase_sampler: MapBase, boundary: MapBase, majorant: MapBase, ds_epsilon: float,
+                 max_bounces: int = -1):
+        super().__init__(6, 3)
+        self.sigma = sigma.cast(3, 1)
+        self.scattering_albedo = scattering_albedo.cast(3, 3)
+        self.emission = emission.cast(6, 3)
+        self.environment = environment.cast(3, 3)
+        self.phase_sampler = phase_sampler.cast(6, 4)
+        self.boundary = boundary.cast(6, 2)
+        self.majorant = majorant.cast(6, 2)
+        self.ds_epsilon = ds_epsilon
+        self.max_bounces = max_bounces
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.set_sigma(self.sigma)
+        lw.set_boundary(self.boundary)
+        lw.set_majorant(self.majorant)
+        lw.set_albedo(self.scattering_albedo)
+        lw.set_emission(self.emission)
+        lw.set_environment(self.environment)
+        lw.set_phase_sampler(self.phase_sampler)
+        lw.medium.ds_epsilon = self.ds_epsilon
+        lw.medium.max_bounces = int(self.max_bounces)
+        return L.VR_MAP_PATH_DT, lw
+
+
+class DRTPathIntegrator(_RayMap):
+    """NEE path tracer whose backward is the reservoir estimator of
+    vr_path_integrator_NEE_DRT.h:46-269 (_maps.py:2981-3022,
+    maps/radiance_NEE_DRT.h).  The forward is the NEE integrator
+    (vr_path_integrator_NEE.h).  max_bounces as in DeltatrackingNEEPathIntegrator.
+    """
+
+    def __init__(self, sigma: MapBase, scattering_albedo: MapBase, emission: MapBase, environment: MapBase,
+                 environment_sampler: MapBase, phase: MapBase, phase_sampler: MapBase, boundary: MapBase,
+                 majorant: MapBase, transmittance: MapBase, max_bounces: int = -1):
+        super().__init__(6, 3)
+        self.sigma = sigma.cast(3, 1)
+        self.scattering_albedo = scattering_albedo.cast(3, 3)
+        self.emission = emission.cast(6, 3)
+        self.environment = environment.cast(3, 3)
+        self.environment_sampler = environment_sampler.cast(6, 6)
+        self.phase = phase.cast(9, 1)
+        self.phase_sampler = phase_sampler.cast(6, 4)
+        self.boundary = boundary.cast(6, 2)
+        self.majorant = majorant.cast(6, 2)
+        self.transmittance = transmittance.cast(6, 1)
+        self.max_bounces = max_bounces
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.set_sigma(self.sigma)
+        lw.set_boundary(self.boundary)
+        lw.set_majorant(self.majorant)
+        lw.set_albedo(self.scattering_albedo)
+        lw.set_emission(self.emission)
+        lw.set_environment(self.environment)
+        lw.set_environment_sampler(self.environment_sampler)
+        lw.set_phase(self.phase, self.phase_sampler)
+        lw.set_transmittance(self.transmittance)
+        lw.medium.max_bounces = int(self.max_bounces)
+        return L.VR_MAP_NEE_DRT, lw
 
 
 # ------------------------------------------------------------- autograd --

@@ -356,6 +356,60 @@ def medium_radiance_path_integrator_NEE_DT(ds: DependencySet, *, ds_epsilon: flo
         ds.phase_sampler, ds.boundary, ds.majorant, ds.transmittance, ds_epsilon, max_bounces=max_bounces))
 
 
+def medium_radiance_path_integrator_DT(ds: DependencySet, *, ds_epsilon: float = 0.1, max_bounces: int = -1):
+    """_objects.py:309-331 (+ optional bounce cap)."""
+    ds.requires(medium_sigma)
+    if not ds.ensures("scattering_albedo", _maps.MapBase):
+        try:
+            ds.requires(medium_scattering_albedo)
+        except AssertionError:
+            ds.add_parameters(scattering_albedo=_maps.ZERO)
+    if not ds.ensures("emission", _maps.MapBase):
+        try:
+            ds.requires(medium_emission)
+        except AssertionError:
+            ds.add_parameters(emission=_maps.ZERO)
+    ds.requires(medium_boundary)
+    ds.requires(medium_majorant)
+    ds.requires(medium_environment)
+    ds.requires(medium_phase_sampler)
+    ds.add_parameters(radiance=_maps.DeltatrackingPathIntegrator(
+        ds.sigma, ds.scattering_albedo, ds.emission, ds.environment, ds.phase_sampler, ds.boundary, ds.majorant,
+        ds_epsilon, max_bounces=max_bounces))
+
+
+def medium_radiance_path_integrator_NEE_DRT(ds: DependencySet, *, max_bounces: int = -1):
+    """_objects.py:398-426 (+ optional bounce cap)."""
+    ds.requires(medium_sigma)
+    if not ds.ensures("scattering_albedo", _maps.MapBase):
+        try:
+            ds.requires(medium_scattering_albedo)
+        except AssertionError:
+            ds.add_parameters(scattering_albedo=_maps.ZERO)
+    if not ds.ensures("emission", _maps.MapBase):
+        try:
+            ds.requires(medium_emission)
+        except AssertionError:
+            ds.add_parameters(emission=_maps.ZERO)
+    ds.requires(medium_boundary)
+    ds.requires(medium_majorant)
+    ds.requires(medium_environment)
+    ds.requires(medium_environment_sampler)
+    ds.requires(medium_phase)
+    ds.requires(medium_phase_sampler)
+    ds.requires(medium_transmittance)
+    ds.add_parameters(radiance=_maps.DRTPathIntegrator(
+        ds.sigma, ds.scattering_albedo, ds.emission, ds.environment, ds.environment_sampler, ds.phase,
+        ds.phase_sampler, ds.boundary, ds.majorant, ds.transmittance, max_bounces=max_bounces))
+
+
+def medium_radiance(ds: DependencySet):
+    """_objects.py:492-498: the delta-tracking path integrator unless a radiance is given."""
+    if ds.ensures("radiance", _maps.MapBase):
+        return
+    ds.requires(medium_radiance_path_integrator_DT)
+
+
 def camera_sensors(ds: DependencySet, *, width: int, height: int, jittered: bool = False, fov: float = math.pi / 4):
     ds.assert_ensures("camera_poses", torch.Tensor)
     ds.add_parameters(camera=_maps.PerspectiveCameraSensor(width, height, ds.camera_poses, fov=fov, jittered=jittered))

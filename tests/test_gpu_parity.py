@@ -147,6 +147,20 @@ def test_nee_capture_seed_matched(gpu, majorant, tkind):
     assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.98
 
 
+@pytest.mark.parametrize("majorant", ["const", "grid"])
+@pytest.mark.parametrize("kind", [L.VR_MAP_PATH_DT, L.VR_MAP_NEE_DRT])
+def test_path_integrators_capture_seed_matched(gpu, kind, majorant):
+    """DeltatrackingPathIntegrator and the NEE forward of DRTPathIntegrator."""
+    _print_case("path_capture", kind=kind, majorant=majorant)
+    sc = Scene(sigma=fbm_grid(24, seed=9, scale=20.0), albedo_c=(0.9, 0.8, 0.7), phase_kind=L.VR_PHASE_HG_MIX,
+               phase_g=0.6, env_c=(0.5, 0.6, 0.7), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(3.0, 3.0, 3.0), sun_prob=0.5,
+               emission_c=(0.1, 0.0, 0.05), majorant=majorant, transmittance_kind=L.VR_T_DT, max_bounces=16)
+    cam = dict(poses=pose((0.0, 0.3, 2.5))[None], width=48, height=48, jittered=True)
+    ref = oracle_capture_fw(sc, kind, cam, SEEDS, 1)
+    out = gpu_capture_fw(sc, kind, cam, SEEDS, 1)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.98
+
+
 @pytest.mark.parametrize("kind", [L.VR_MAP_SS_RADIANCE, L.VR_MAP_MC_COLLISION])
 def test_single_scatter_seed_matched(gpu, kind):
     _print_case("single_scatter", kind=kind)
@@ -172,7 +186,7 @@ def test_medium_dt_seed_matched(gpu):
 
 # ------------------------------------------------------- backward --
 @pytest.mark.parametrize("kind", [L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_COLLISION_DT, L.VR_MAP_MC_COLLISION,
-                                  L.VR_MAP_SS_RADIANCE, L.VR_MAP_NEE_DT])
+                                  L.VR_MAP_SS_RADIANCE, L.VR_MAP_NEE_DT, L.VR_MAP_PATH_DT, L.VR_MAP_NEE_DRT])
 @pytest.mark.parametrize("bw_mode", [L.VR_BW_REFERENCE, L.VR_BW_CONSISTENT])
 def test_map_backward_seed_matched(gpu, kind, bw_mode):
     _print_case("map_bw", kind=kind, bw_mode=bw_mode)
@@ -183,6 +197,20 @@ def test_map_backward_seed_matched(gpu, kind, bw_mode):
     og = np.random.default_rng(18).uniform(0.5, 1.5, size=(2048, od)).astype(np.float32)
     ref, _ = oracle_map_bw(sc, kind, rays, og, SEEDS, bw_mode=bw_mode)
     out, _ = gpu_map_bw(sc, kind, rays, og, SEEDS, bw_mode=bw_mode)
+    assert rel_l1(out, ref) <= 0.03
+
+
+@pytest.mark.parametrize("tkind", [L.VR_T_DT, L.VR_T_RT])
+@pytest.mark.parametrize("kind", [L.VR_MAP_PATH_DT, L.VR_MAP_NEE_DRT])
+def test_path_integrators_backward_grid_majorant(gpu, kind, tkind):
+    _print_case("path_bw_grid", kind=kind, tkind=tkind)
+    sc = Scene(sigma=fbm_grid(12, seed=16, scale=8.0), albedo_c=(0.9, 0.9, 0.9), phase_kind=L.VR_PHASE_HG,
+               phase_g=0.4, env_c=(1.0, 1.0, 1.0), transmittance_kind=tkind, ds_epsilon=0.1, max_bounces=8,
+               majorant="grid", majorant_res=(3, 3, 3))
+    rays = random_rays(2048, seed=27)
+    og = np.random.default_rng(28).uniform(0.5, 1.5, size=(2048, 3)).astype(np.float32)
+    ref, _ = oracle_map_bw(sc, kind, rays, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT)
+    out, _ = gpu_map_bw(sc, kind, rays, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT)
     assert rel_l1(out, ref) <= 0.03
 
 
@@ -200,7 +228,8 @@ def test_medium_dt_backward_seed_matched(gpu):
     assert rel_l1(out_a, ref_a) <= 0.03
 
 
-@pytest.mark.parametrize("kind", [L.VR_MAP_MC_COLLISION, L.VR_MAP_NEE_DT, L.VR_MAP_T_DDA])
+@pytest.mark.parametrize("kind", [L.VR_MAP_MC_COLLISION, L.VR_MAP_NEE_DT, L.VR_MAP_T_DDA, L.VR_MAP_PATH_DT,
+                                  L.VR_MAP_NEE_DRT])
 def test_capture_backward_seed_matched(gpu, kind):
     _print_case("capture_bw", kind=kind)
     sc = Scene(sigma=fbm_grid(16, seed=23, scale=10.0), albedo=np.full((16, 16, 16, 1), 0.85, np.float32),
@@ -212,7 +241,7 @@ def test_capture_backward_seed_matched(gpu, kind):
     ref_s, ref_a = oracle_capture_bw(sc, kind, cam, SEEDS, 2, og, grad_albedo=True)
     out_s, out_a = gpu_capture_bw(sc, kind, cam, SEEDS, 2, og, grad_albedo=True)
     assert rel_l1(out_s, ref_s) <= 0.03
-    if kind != L.VR_MAP_T_DDA:
+    if kind not in (L.VR_MAP_T_DDA, L.VR_MAP_PATH_DT, L.VR_MAP_NEE_DRT):  # no albedo gradient in those estimators
         assert rel_l1(out_a, ref_a) <= 0.03
 
 

@@ -284,10 +284,11 @@ def test_rt_backward_matches_finite_differences():
         assert abs(grad[idx] - fd) <= 0.05 * abs(fd) + 2e-4, (idx, grad[idx], fd)
 
 
-def _capture_fd_check(kind, transmittance_kind, voxels, spp_bw, spp_fd, h=0.4, rel_tol=0.06):
+def _capture_fd_check(kind, transmittance_kind, voxels, spp_bw, spp_fd, h=0.4, rel_tol=0.06, bw_kinds=None):
     """Mean of the oracle's capture backward (consistent estimator, replayed
     forward seeds) against a central difference of the oracle's forward, both
-    sides of the difference rendered with the same seeds."""
+    sides of the difference rendered with the same seeds.  bw_kinds: several
+    backward estimators {kind: (spp, rel_tol)} of the same forward."""
     rng = np.random.default_rng(1)
     g = rng.uniform(1.0, 4.0, size=(6, 6, 6, 1)).astype(np.float32)
     cam = dict(poses=np.stack([pose((0.3, 0.4, 2.0)), pose((-1.8, 0.2, 0.9))]), width=8, height=8, jittered=True)
@@ -297,7 +298,9 @@ def _capture_fd_check(kind, transmittance_kind, voxels, spp_bw, spp_fd, h=0.4, r
         return Scene(sigma=s, albedo_c=(0.8, 0.8, 0.8), env_c=(1.0, 0.8, 0.6), phase_g=0.3,
                      transmittance_kind=transmittance_kind, max_bounces=8)
 
-    grad = oracle_capture_bw(scene(g), kind, cam, (1, 2, 3, 4), spp_bw, w, bw_mode=L.VR_BW_CONSISTENT)[0]
+    bw_kinds = bw_kinds or {kind: (spp_bw, rel_tol)}
+    grads = {k: oracle_capture_bw(scene(g), k, cam, (1, 2, 3, 4), n, w, bw_mode=L.VR_BW_CONSISTENT)[0]
+             for k, (n, _) in bw_kinds.items()}
     for v in voxels:
         r = []
         for sgn in (1, -1):
@@ -305,7 +308,8 @@ def _capture_fd_check(kind, transmittance_kind, voxels, spp_bw, spp_fd, h=0.4, r
             s2[v] += sgn * h
             r.append(float((oracle_capture_fw(scene(s2), kind, cam, (7, 8, 9, 10), spp_fd).astype(np.float64) * w).sum()))
         fd = (r[0] - r[1]) / (2 * h)
-        assert abs(grad[v] - fd) <= rel_tol * abs(fd), (v, grad[v], fd)
+        for k, (_, tol) in bw_kinds.items():
+            assert abs(grads[k][v] - fd) <= tol * abs(fd), (k, v, grads[k][v], fd)
 
 
 def test_collision_single_scatter_backward_matches_finite_differences():
@@ -315,5 +319,44 @@ def test_collision_single_scatter_backward_matches_finite_differences():
 
 
 def test_nee_path_backward_matches_finite_differences():
-    """NEE delta-tracking path integrator, up to 8 bounces, DT shadow rays."""
-    _capture_fd_check(L.VR_MAP_NEE_DT, L.VR_T_DT, [(2, 3, 3, 0), (3, 2, 2, 0)], 32768, 262144, rel_tol=0.08)
+    """NEE delta-tracking path integrator, up to 8 bounces, DT shadow rays: the
+    defensive replay estimator (vr_path_integrator_NEE_DS.h) and the reservoir
+    estimator (vr_path_integrator_NEE_DRT.h) against one central difference."""
+    _capture_fd_check(L.VR_MAP_NEE_DT, L.VR_T_DT, [(2, 3, 3, 0), (3, 2, 2, 0)], None, 262144,
+                      bw_kinds={L.VR_MAP_NEE_DT: (32768, 0.08), L.VR_MAP_NEE_DRT: (65536, 0.08)})
+
+
+def test_dt_path_backward_matches_finite_differences():
+    """Delta-tracking path integrator (no NEE), defensive replay backward."""
+    _capture_fd_check(L.VR_MAP_PATH_DT, L.VR_T_DT, [(2, 3, 3, 0), (3, 2, 2, 0)], 16384, 131072)
+
+
+def test_dt_path_white_furnace_is_exact():
+    """Albedo 1, HG sampling weight 1, constant environment L: every sample is L."""
+    sc = Scene(sigma=blob_grid(16, seed=3, max_value=15.0), albedo_c=(1, 1, 1), phase_kind=L.VR_PHASE_HG, phase_g=0.6,
+               env_c=(0.7, 0.7, 0.7))
+    R = oracle_map_fw(sc, L.VR_MAP_PATH_DT, random_rays(4000, seed=6), (3, 4, 5, 6))
+    np.testing.assert_allclose(R, 0.7, rtol=1e-6)
+
+
+def test_dt_path_absorbing_medium_is_transmittance_times_env():
+    g = blob_grid(12, seed=4, max_value=8.0)
+    sc = Scene(sigma=g, albedo_c=(0, 0, 0), env_c=(2.0, 1.0, 0.5))
+    ray = np.array([0.0, 0.05, -1.5, 0.0, 0.0, 1.0], np.float32)
+    n = 100000
+    R = oracle_map_fw(sc, L.VR_MAP_PATH_DT, np.tile(ray, (n, 1)), (7, 8, 9, 10))
+    se = R[:, 0].std() / math.sqrt(n)
+    assert abs(R[:, 0].mean() - 2.0 * _exact_T(g, ray)) <= 4 * se + 1e-4
+
+
+def test_dt_path_and_nee_estimate_the_same_radiance():
+    """Unbounded paths, constant environment, no sun: the path tracer without
+    NEE and the NEE path tracer are unbiased for the same radiance."""
+    sc = Scene(sigma=blob_grid(10, seed=8, max_value=10.0), albedo_c=(0.8, 0.7, 0.6), phase_kind=L.VR_PHASE_HG,
+               phase_g=0.4, env_c=(1.0, 0.9, 0.8))
+    ray = np.array([0.02, -0.03, -1.5, 0.0, 0.0, 1.0], np.float32)
+    n = 200000
+    a = oracle_map_fw(sc, L.VR_MAP_PATH_DT, np.tile(ray, (n, 1)), (1, 2, 3, 4)).astype(np.float64)
+    b = oracle_map_fw(sc, L.VR_MAP_NEE_DT, np.tile(ray, (n, 1)), (5, 6, 7, 8)).astype(np.float64)
+    se = np.sqrt(a.var(0) / n + b.var(0) / n)
+    assert np.all(np.abs(a.mean(0) - b.mean(0)) <= 4 * se), (a.mean(0), b.mean(0), se)
