@@ -163,6 +163,120 @@ def gradient_leg(dev, res=256, views=8, hw=256, spp=16, steps=3):
     }
 
 
+def _nee_scene(sigma, albedo, max_bounces):
+    import torch
+    import rendervous_amd as rv
+
+    ds = rv.objects.DependencySet()
+    ds.add_parameters(sigma_tensor=sigma, scattering_albedo_tensor=albedo,
+                      environment_tensor=torch.tensor([0.1, 0.1, 0.1]), phase_g_tensor=torch.tensor([0.6]))
+    ds.requires(rv.objects.medium_majorant_coarse_grid, resolution=16)
+    ds.requires(rv.objects.medium_environment_sampler_sun, direction=(0.4, 0.8, 0.45), radiance=(3.0, 3.0, 3.0),
+                probability=0.5)
+    ds.requires(rv.objects.medium_phase_HG, mixture=False)
+    ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
+    ds.requires(rv.objects.medium_transmittance_DT)
+    ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DT, max_bounces=max_bounces)
+    return ds.radiance
+
+
+def nee_gradient_leg(dev, res=512, ares=128, views=4, hw=512, spp=32, steps=2):
+    """One GPU's share (4 of 32 views) of the multiple-scattering inverse-render
+    step: 512^3 density + 128^3 RGB albedo, 512x512, 32 spp, NEE + DT, max 8
+    bounces; L2 loss, backward by the defensive replay estimator replaying the
+    forward's seeds, fused Adam on both grids."""
+    import torch
+    import rendervous_amd as rv
+    from rendervous_amd import inverse
+
+    t_sigma = rv.functions.fbm_value_noise(res, seed=21, scale=60.0, sphere_radius=0.5, device=dev)
+    t_albedo = (0.55 + 0.4 * rv.functions.fbm_value_noise(ares, seed=22, scale=1.0, device=dev)).expand(
+        ares, ares, ares, 3).contiguous()
+    poses = rv.functions.orbit_poses(32, radius=2.5)[:views].to(dev)
+    camera = rv.PerspectiveCameraSensor(hw, hw, poses, jittered=True)
+    rv.seed(5)
+    with torch.no_grad():
+        target = camera.capture(_nee_scene(t_sigma, t_albedo, 8), fw_samples=spp)
+    del t_sigma, t_albedo
+    sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
+    albedo = torch.nn.Parameter(torch.full((ares, ares, ares, 3), 0.7, device=dev), requires_grad=True)
+    field = _nee_scene(sigma, albedo, 8)
+    opt_s = inverse.FusedAdam(sigma, lr=1e-2)
+    opt_a = inverse.FusedAdam(albedo, lr=1e-2)
+    dbg = torch.zeros(views * hw * hw, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    fw_ms = bw_ms = adam_ms = 0.0
+    scatters = 0.0
+    for k in range(steps + 1):  # the first step is a warm-up
+        e = _events(4)
+        e[0].record(stream)
+        img = camera.capture(field, fw_samples=spp, bw_samples=spp, debug_out=dbg)
+        loss = ((img - target) ** 2).mean()
+        e[1].record(stream)
+        loss.backward()
+        e[2].record(stream)
+        opt_s.step(sigma.grad, zero_grad=True)
+        opt_a.step(albedo.grad, zero_grad=True)
+        with torch.no_grad():
+            sigma.clamp_(min=0.0)
+            albedo.clamp_(0.0, 1.0)
+        e[3].record(stream)
+        torch.cuda.synchronize()
+        if k == 0:
+            continue
+        fw_ms += e[0].elapsed_time(e[1])
+        bw_ms += e[1].elapsed_time(e[2])
+        adam_ms += e[2].elapsed_time(e[3])
+        scatters += float(dbg.to(torch.float64).sum().item())
+    fw_ms, bw_ms, adam_ms, scatters = fw_ms / steps, bw_ms / steps, adam_ms / steps, scatters / steps
+    samples = views * hw * hw * spp
+    return {
+        "workload": f"NEE inverse render step, one GPU's share: {res}^3 density + {ares}^3 RGB albedo, {views} of 32 "
+                    f"views {hw}x{hw}, {spp} spp, DT + NEE, max 8 bounces, replay backward, fused Adam",
+        "grad_voxel_updates_per_s": 8.0 * scatters / (bw_ms * 1e-3),
+        "trilinear_scatters_per_step": scatters,
+        "fw_ms": fw_ms, "bw_ms": bw_ms, "adam_ms": adam_ms,
+        "fw_samples_per_s": samples / (fw_ms * 1e-3), "bw_samples_per_s": samples / (bw_ms * 1e-3),
+        "loss": float(loss.item()),
+    }
+
+
+def time_to_rmse_leg(dev, radiance, camera, ref_spp=4096, rel_eps=0.05):
+    """Plain renderer (no radiance cache): wall time of the smallest single
+    capture whose image RMSE against a 4096-spp render of the same frame is at
+    most rel_eps x the reference's mean.  Captures of 8, 16, 32, ... spp."""
+    import torch
+    import rendervous_amd as rv
+
+    stream = torch.cuda.current_stream(dev)
+    with torch.no_grad():
+        rv.seed(12345)
+        ref = None
+        for _ in range(ref_spp // 64):
+            img = camera.capture(radiance, fw_samples=64)
+            ref = img if ref is None else ref + img
+        ref = ref / (ref_spp // 64)
+        eps = rel_eps * float(ref.mean())
+        rows = []
+        spp = 8
+        while spp <= ref_spp // 4:
+            rv.seed(spp)
+            e = _events(2)
+            e[0].record(stream)
+            img = camera.capture(radiance, fw_samples=spp)
+            e[1].record(stream)
+            torch.cuda.synchronize()
+            rmse = float(((img - ref) ** 2).mean().sqrt())
+            rows.append({"spp": spp, "ms": e[0].elapsed_time(e[1]), "rmse": rmse})
+            if rmse <= eps:
+                break
+            spp *= 2
+    hit = rows[-1] if rows[-1]["rmse"] <= eps else None
+    return {"workload": f"time to RMSE <= {rel_eps:g} x mean vs a {ref_spp}-spp render, same NEE frame, plain renderer",
+            "epsilon": eps, "time_to_rmse_ms": hit["ms"] if hit else None, "spp_needed": hit["spp"] if hit else None,
+            "ladder": rows}
+
+
 def dda_leg(dev, reps=20):
     """Deterministic DDA transmittance of the 64^3 Gaussian-blob grid, 128x128, 1 ray/pixel."""
     import torch
@@ -279,9 +393,14 @@ def main():
                      "traffic": profiled_traffic(args.res, args.width, args.height, args.spp)},
     }
     if not args.no_secondary:
+        rmse = time_to_rmse_leg(dev, radiance, camera)
         del img, radiance, camera
         torch.cuda.empty_cache()
-        result["secondary"] = {"gradient": gradient_leg(dev), "dda": dda_leg(dev)}
+        result["secondary"] = {"time_to_rmse": rmse, "gradient": gradient_leg(dev)}
+        torch.cuda.empty_cache()
+        result["secondary"]["nee_gradient"] = nee_gradient_leg(dev)
+        torch.cuda.empty_cache()
+        result["secondary"]["dda"] = dda_leg(dev)
         log(f"secondary: {json.dumps(result['secondary'])}")
     if not args.no_cpu_baseline:
         m = build_scene(args.res, args.width, args.height, dev)[1]

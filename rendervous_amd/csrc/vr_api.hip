@@ -318,24 +318,38 @@ __global__ void __launch_bounds__(kBlock) k_map_bw(MediumView M, const float* __
     field_bw<KIND, MAJ>(tr, a, o);
 }
 
+// Capture kernels: thread j handles sensor start + j % cnt and the samples
+// [c*chunk, min(spp, (c+1)*chunk)) with c = j / cnt, so that a pixel's samples
+// are spread over several threads (chunks = ceil(spp / chunk)).  With one chunk
+// the thread writes its pixel; with more, partial sums are added atomically
+// into rows zeroed by k_capture_prepare.
+struct CaptureSplit {
+    int spp, chunk, chunks;
+};
+
 template <int KIND, int MAJ>
-__global__ void __launch_bounds__(kBlock) k_capture_fw(MediumView M, CameraView cam, Seeds seeds, int spp, int64_t total,
+__global__ void __launch_bounds__(kBlock) k_capture_fw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int64_t total,
                                                        int64_t start, int64_t cnt, float* __restrict__ out, int* __restrict__ dbg) {
-    int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= start + cnt) return;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt * P.chunks) return;
+    const int64_t c = j / cnt;
+    const int64_t i = start + (j - c * cnt);
+    const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
     constexpr int OD = Dims<KIND>::out;
     int pose, row, col;
     sensor_index(cam, i, pose, row, col);
     if (pose < 0 || pose >= cam.num_poses || row < 0 || row >= cam.height || col < 0 || col >= cam.width) {
+        if (P.chunks == 1) {
 #pragma unroll
-        for (int k = 0; k < OD; k++) out[i * OD + k] = 0.0f;
+            for (int k = 0; k < OD; k++) out[i * OD + k] = 0.0f;
+        }
         return;
     }
     float acc[OD];
 #pragma unroll
     for (int k = 0; k < OD; k++) acc[k] = 0.0f;
     Tracer<MAJ> tr(M);
-    for (int s = 0; s < spp; s++) {
+    for (int s = s0; s < s1; s++) {
         tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
         f3 x, w;
         camera_ray(cam, pose, row, col, tr.rng, x, w);
@@ -344,18 +358,27 @@ __global__ void __launch_bounds__(kBlock) k_capture_fw(MediumView M, CameraView 
 #pragma unroll
         for (int k = 0; k < OD; k++) acc[k] += o[k];
     }
-    float inv = 1.0f / (float)spp;
+    const float inv = 1.0f / (float)P.spp;
+    if (P.chunks == 1) {
 #pragma unroll
-    for (int k = 0; k < OD; k++) out[i * OD + k] = spp == 1 ? acc[k] : acc[k] * inv;
-    if (dbg) dbg[i] = tr.lookups;
+        for (int k = 0; k < OD; k++) out[i * OD + k] = P.spp == 1 ? acc[k] : acc[k] * inv;
+        if (dbg) dbg[i] = tr.lookups;
+    } else {
+#pragma unroll
+        for (int k = 0; k < OD; k++) atomic_addf(out + i * OD + k, acc[k] * inv);
+        if (dbg) atomicAdd(dbg + i, tr.lookups);
+    }
 }
 
 template <int KIND, int MAJ>
-__global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView cam, Seeds seeds, int spp, int seed_mode,
+__global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int seed_mode,
                                                        int64_t total, int64_t start, int64_t cnt, const float* __restrict__ og,
                                                        int* __restrict__ dbg) {
-    int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= start + cnt) return;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt * P.chunks) return;
+    const int64_t c = j / cnt;
+    const int64_t i = start + (j - c * cnt);
+    const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
     constexpr int OD = Dims<KIND>::out;
     int pose, row, col;
     sensor_index(cam, i, pose, row, col);
@@ -363,21 +386,36 @@ __global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView 
     float o[OD];
     bool any = false;
 #pragma unroll
-    for (int k = 0; k < OD; k++) { o[k] = og[i * OD + k] / (float)spp; any |= o[k] != 0.0f; }
+    for (int k = 0; k < OD; k++) { o[k] = og[i * OD + k] / (float)P.spp; any |= o[k] != 0.0f; }
     if (!any) {
-        if (dbg) dbg[i] = 0;
+        if (dbg && P.chunks == 1) dbg[i] = 0;
         return;
     }
     Tracer<MAJ> tr(M);
+    // the reference's fresh-seed backward continues one stream over the samples (one chunk)
     if (seed_mode == VR_SEED_CAPTURE_BW) tr.rng = seed_capture_bw(seeds, (uint32_t)i);
-    for (int s = 0; s < spp; s++) {
+    for (int s = s0; s < s1; s++) {
         if (seed_mode != VR_SEED_CAPTURE_BW) tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
         f3 x, w;
         camera_ray(cam, pose, row, col, tr.rng, x, w);
         float a[6] = {x.x, x.y, x.z, w.x, w.y, w.z};
         field_bw<KIND, MAJ>(tr, a, o);
     }
-    if (dbg) dbg[i] = tr.scatters;
+    if (dbg) {
+        if (P.chunks == 1) dbg[i] = tr.scatters;
+        else atomicAdd(dbg + i, tr.scatters);
+    }
+}
+
+// Zeroes rows [start, start+cnt) of a capture output (OD floats each) and of the debug counters.
+__global__ void k_capture_prepare(float* __restrict__ out, int od, int64_t start, int64_t cnt, int* __restrict__ dbg) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < cnt; i += stride) {
+        if (out)
+            for (int k = 0; k < od; k++) out[(start + i) * od + k] = 0.0f;
+        if (dbg) dbg[start + i] = 0;
+    }
 }
 
 // torch.optim.Adam (no weight decay, no amsgrad), optionally clearing the gradient.
@@ -514,16 +552,40 @@ int launch_map_bw_maj(int kind, const MediumView& M, const float* in, const floa
     return check_hip(hipGetLastError(), "vr_map_bw launch");
 }
 
+// Samples per capture thread: a pixel's spp are spread over up to 8 threads
+// (VR_CAPTURE_CHUNK overrides); the fresh-seed backward keeps one chunk.
+CaptureSplit capture_split(int spp, bool one_chunk) {
+    CaptureSplit P;
+    P.spp = spp;
+    int chunk = spp <= 4 ? spp : (spp + 7) / 8;
+    chunk = env_int("VR_CAPTURE_CHUNK", chunk);
+    P.chunk = one_chunk ? spp : std::max(1, std::min(spp, chunk));
+    P.chunks = (spp + P.chunk - 1) / P.chunk;
+    return P;
+}
+
+int capture_prepare(const CaptureSplit& P, float* out, int od, int64_t start, int64_t cnt, int* dbg, hipStream_t st) {
+    if (P.chunks == 1 || (!out && !dbg)) return VR_OK;
+    unsigned prep = (unsigned)std::min<int64_t>((cnt + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_capture_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, od, start, cnt, dbg);
+    return check_hip(hipGetLastError(), "capture prepare");
+}
+
 template <int MAJ>
 int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
                           int64_t cnt, float* out, int* dbg, hipStream_t st) {
-    dim3 g(grid_for(cnt)), b(kBlock);
     // the NEE forward (also DRTPathIntegrator's) with a DT or RT shadow estimator runs persistent
     const bool nee = kind == VR_MAP_NEE_DT || kind == VR_MAP_NEE_DRT;
     if (nee && M.t_kind == VR_T_DT) return launch_nee_persistent<MAJ, VR_T_DT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
     if (nee && M.t_kind == VR_T_RT) return launch_nee_persistent<MAJ, VR_T_RT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+    int in_dim = 0, od = 0;
+    map_dims(kind, in_dim, od);
+    const CaptureSplit P = capture_split(spp, false);
+    int rc = capture_prepare(P, out, od, start, cnt, dbg, st);
+    if (rc) return rc;
+    dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
     switch (kind) {
-#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_fw<K, MAJ>), g, b, 0, st, M, C, sd, spp, total, start, cnt, out, dbg); break;
+#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_fw<K, MAJ>), g, b, 0, st, M, C, sd, P, total, start, cnt, out, dbg); break;
         VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_PATH_DT)
         VR_CASE(VR_MAP_NEE_DRT)
@@ -536,9 +598,12 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
 template <int MAJ>
 int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int seed_mode, int64_t total,
                           int64_t start, int64_t cnt, const float* og, int* dbg, hipStream_t st) {
-    dim3 g(grid_for(cnt)), b(kBlock);
+    const CaptureSplit P = capture_split(spp, seed_mode == VR_SEED_CAPTURE_BW);
+    int rc = capture_prepare(P, nullptr, 0, start, cnt, dbg, st);
+    if (rc) return rc;
+    dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
     switch (kind) {
-#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_bw<K, MAJ>), g, b, 0, st, M, C, sd, spp, seed_mode, total, start, cnt, og, dbg); break;
+#define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_bw<K, MAJ>), g, b, 0, st, M, C, sd, P, seed_mode, total, start, cnt, og, dbg); break;
         VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_PATH_DT)
         VR_CASE(VR_MAP_NEE_DRT)

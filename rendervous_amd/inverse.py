@@ -24,15 +24,21 @@ from . import objects as O
 
 
 def single_scatter_scene(sigma: torch.Tensor, *, albedo=(0.9, 0.9, 0.9), g: float = 0.0, env=(1.0, 1.0, 1.0),
-                         majorant: str = "grid", majorant_resolution: int = 16, ds_epsilon: float = 0.1):
+                         majorant: typing.Union[str, float] = "grid", majorant_resolution: int = 16,
+                         ds_epsilon: float = 0.1):
     """DependencySet with radiance = MCCollisionIntegrator(DT collision sampler,
-    SingleScatteredRadiance(RT transmittance), constant environment)."""
+    SingleScatteredRadiance(RT transmittance), constant environment).
+
+    majorant: "grid" (coarse max grid), "const" (max of sigma) or a number (a
+    fixed constant majorant, which must bound sigma)."""
     ds = O.DependencySet()
     ds.add_parameters(sigma_tensor=sigma, scattering_albedo_tensor=torch.tensor(list(albedo), dtype=torch.float32),
                       environment_tensor=torch.tensor(list(env), dtype=torch.float32),
                       phase_g_tensor=torch.tensor([g], dtype=torch.float32))
     if majorant == "grid":
         ds.requires(O.medium_majorant_coarse_grid, resolution=majorant_resolution)
+    elif not isinstance(majorant, str):
+        ds.add_parameters(majorant_tensor=torch.tensor([float(majorant), 1000000.0], dtype=torch.float32))
     ds.requires(O.medium_phase_HG, mixture=False)
     ds.requires(O.medium_transmittance_RT)
     ds.requires(O.medium_collision_sampler_DT, ds_epsilon=ds_epsilon)

@@ -3,9 +3,11 @@
 Each check compares the capture backward (through the map API and torch
 autograd, i.e. AutogradCaptureFunction -> vr_capture_bw) with a central
 difference of the expected loss.  Both sides of a difference are rendered with
-the same dispatch seeds (common random numbers): a perturbation of one voxel
-changes only the paths that reach it, so the difference keeps little of the
-Monte Carlo noise of either render.  The loss is a fixed linear functional of
+the same dispatch seeds (common random numbers) and the same fixed constant
+majorant (a majorant taken from max(sigma) would move with the perturbation
+and change every free flight): a perturbation of one voxel changes only the
+paths that reach it, so the difference keeps little of the Monte Carlo noise
+of either render.  The loss is a fixed linear functional of
 the image, L = sum(w * I), so dL/dI = w exactly.
 """
 import math
@@ -61,7 +63,7 @@ def test_mc_collision_single_scatter_gradient_matches_fd(gpu):
     sigma0 = _grid(6, seed=1)
 
     def make(sig):
-        ds = inverse.single_scatter_scene(sig, albedo=(0.8, 0.8, 0.8), g=0.3, env=(1.0, 0.8, 0.6), majorant="const")
+        ds = inverse.single_scatter_scene(sig, albedo=(0.8, 0.8, 0.8), g=0.3, env=(1.0, 0.8, 0.6), majorant=5.0)
         return ds.radiance
 
     with rv.backward_mode(seeds="replay", estimator="consistent"):
@@ -89,7 +91,8 @@ def test_nee_path_gradient_matches_fd(gpu):
     def make(sig):
         ds = rv.objects.DependencySet()
         ds.add_parameters(sigma_tensor=sig, scattering_albedo_tensor=torch.tensor([0.8, 0.8, 0.8]),
-                          environment_tensor=torch.tensor([1.0, 1.0, 1.0]), phase_g_tensor=torch.tensor([0.3]))
+                          environment_tensor=torch.tensor([1.0, 1.0, 1.0]), phase_g_tensor=torch.tensor([0.3]),
+                          majorant_tensor=torch.tensor([4.0, 1e6]))
         ds.requires(rv.objects.medium_phase_HG, mixture=False)
         ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
         ds.requires(rv.objects.medium_transmittance_DT)
@@ -98,6 +101,40 @@ def test_nee_path_gradient_matches_fd(gpu):
 
     with rv.backward_mode(seeds="replay", estimator="consistent"):
         _fd_check(make, sigma0, [(2, 3, 3, 0), (3, 2, 2, 0)], spp_grad=4096, spp_fd=16384, h=0.4, rel_tol=0.2)
+
+
+def test_drt_path_gradient_matches_fd(gpu):
+    """NEE forward, reservoir (DRT) backward."""
+    sigma0 = _grid(6, seed=3, lo=1.0, hi=3.0)
+
+    def make(sig):
+        ds = rv.objects.DependencySet()
+        ds.add_parameters(sigma_tensor=sig, scattering_albedo_tensor=torch.tensor([0.8, 0.8, 0.8]),
+                          environment_tensor=torch.tensor([1.0, 1.0, 1.0]), phase_g_tensor=torch.tensor([0.3]),
+                          majorant_tensor=torch.tensor([4.0, 1e6]))
+        ds.requires(rv.objects.medium_phase_HG, mixture=False)
+        ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
+        ds.requires(rv.objects.medium_transmittance_RT)
+        ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DRT, max_bounces=8)
+        return ds.radiance
+
+    _fd_check(make, sigma0, [(2, 3, 3, 0), (3, 2, 2, 0)], spp_grad=8192, spp_fd=16384, h=0.4, rel_tol=0.2)
+
+
+def test_dt_path_gradient_matches_fd(gpu):
+    """Delta-tracking path tracer without NEE, defensive replay backward."""
+    sigma0 = _grid(6, seed=5, lo=1.0, hi=3.0)
+
+    def make(sig):
+        ds = rv.objects.DependencySet()
+        ds.add_parameters(sigma_tensor=sig, scattering_albedo_tensor=torch.tensor([0.8, 0.8, 0.8]),
+                          environment_tensor=torch.tensor([1.0, 0.9, 0.8]), phase_g_tensor=torch.tensor([0.3]),
+                          majorant_tensor=torch.tensor([4.0, 1e6]))
+        ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
+        ds.requires(rv.objects.medium_radiance_path_integrator_DT, max_bounces=8)
+        return ds.radiance
+
+    _fd_check(make, sigma0, [(2, 3, 3, 0), (3, 2, 2, 0)], spp_grad=8192, spp_fd=16384, h=0.4, rel_tol=0.2)
 
 
 def test_dda_gradient_matches_fd_exactly(gpu):

@@ -294,9 +294,9 @@ def _capture_fd_check(kind, transmittance_kind, voxels, spp_bw, spp_fd, h=0.4, r
     cam = dict(poses=np.stack([pose((0.3, 0.4, 2.0)), pose((-1.8, 0.2, 0.9))]), width=8, height=8, jittered=True)
     w = (rng.uniform(size=(128, 3)) + 0.5).astype(np.float32)
 
-    def scene(s):
+    def scene(s):  # a fixed majorant: max(sigma) would move with the perturbation
         return Scene(sigma=s, albedo_c=(0.8, 0.8, 0.8), env_c=(1.0, 0.8, 0.6), phase_g=0.3,
-                     transmittance_kind=transmittance_kind, max_bounces=8)
+                     transmittance_kind=transmittance_kind, max_bounces=8, majorant_value=4.5)
 
     bw_kinds = bw_kinds or {kind: (spp_bw, rel_tol)}
     grads = {k: oracle_capture_bw(scene(g), k, cam, (1, 2, 3, 4), n, w, bw_mode=L.VR_BW_CONSISTENT)[0]
