@@ -287,8 +287,23 @@ __global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry,
     if (threadIdx.x == 0) out[cell] = red[0];
 }
 
+// Resident waves per SIMD the generic capture and map kernels are compiled for.
+// The nested estimators are bound by the latency of dependent gathers; left
+// alone the compiler gives the NEE backward ~240 VGPRs (2 waves per SIMD).
+// Capping the registers at 5 waves (96 VGPRs, the rest spilled to scratch)
+// halves the NEE backward's time on the 512^3 inverse-render step (4 and 6
+// waves are close, 8 is slower).
+#ifndef VR_CAPTURE_WAVES_PER_EU
+#define VR_CAPTURE_WAVES_PER_EU 5
+#endif
+#if VR_CAPTURE_WAVES_PER_EU > 0
+#define VR_CAPTURE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(VR_CAPTURE_WAVES_PER_EU)))
+#else
+#define VR_CAPTURE_OCCUPANCY
+#endif
+
 template <int KIND, int MAJ>
-__global__ void __launch_bounds__(kBlock) k_map_fw(MediumView M, const float* __restrict__ in, int64_t start, int64_t cnt,
+__global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_fw(MediumView M, const float* __restrict__ in, int64_t start, int64_t cnt,
                                                    float* __restrict__ out, Seeds seeds, int* __restrict__ dbg) {
     int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= start + cnt) return;
@@ -304,7 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_map_fw(MediumView M, const float* __
 }
 
 template <int KIND, int MAJ>
-__global__ void __launch_bounds__(kBlock) k_map_bw(MediumView M, const float* __restrict__ in, const float* __restrict__ og,
+__global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_bw(MediumView M, const float* __restrict__ in, const float* __restrict__ og,
                                                    int64_t start, int64_t cnt, Seeds seeds) {
     int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= start + cnt) return;
@@ -327,8 +342,9 @@ struct CaptureSplit {
     int spp, chunk, chunks;
 };
 
+
 template <int KIND, int MAJ>
-__global__ void __launch_bounds__(kBlock) k_capture_fw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int64_t total,
+__global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_fw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int64_t total,
                                                        int64_t start, int64_t cnt, float* __restrict__ out, int* __restrict__ dbg) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt * P.chunks) return;
@@ -371,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_capture_fw(MediumView M, CameraView 
 }
 
 template <int KIND, int MAJ>
-__global__ void __launch_bounds__(kBlock) k_capture_bw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int seed_mode,
+__global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int seed_mode,
                                                        int64_t total, int64_t start, int64_t cnt, const float* __restrict__ og,
                                                        int* __restrict__ dbg) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
