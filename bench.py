@@ -107,6 +107,75 @@ def _events(n):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
 
+def _inverse_step_timing(camera, field, target, spp, params, steps):
+    """Times `steps` inverse-render steps (after one warm-up): capture forward +
+    L2 loss, adjoint backward, fused bounded Adam on each (param, opt, bounds).
+    An untimed extra step counts the forward's and the backward's trilinear
+    lookups and the backward's scatters from the kernels' own counters."""
+    import torch
+    import rendervous_amd as rv
+
+    dev = target.device
+    n_sensors = camera.num_sensors()
+    dbg = torch.zeros(n_sensors, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(debug=None):
+        img = camera.capture(field, fw_samples=spp, bw_samples=spp, debug_out=debug)
+        loss = ((img - target) ** 2).mean()
+        return img, loss
+
+    def update():
+        for p, opt, bounds in params:
+            opt.step(p.grad, zero_grad=False, bounds=bounds)
+            p.grad = None
+
+    fw_ms = bw_ms = adam_ms = 0.0
+    scatters = 0.0
+    for k in range(steps + 1):  # the first step is a warm-up
+        e = _events(4)
+        e[0].record(stream)
+        img, loss = step(dbg)
+        e[1].record(stream)
+        loss.backward()
+        e[2].record(stream)
+        update()
+        e[3].record(stream)
+        torch.cuda.synchronize()
+        if k == 0:
+            continue
+        fw_ms += e[0].elapsed_time(e[1])
+        bw_ms += e[1].elapsed_time(e[2])
+        adam_ms += e[2].elapsed_time(e[3])
+        scatters += float(dbg.to(torch.float64).sum().item())
+    fw_ms, bw_ms, adam_ms, scatters = fw_ms / steps, bw_ms / steps, adam_ms / steps, scatters / steps
+    # lookup counts (untimed; the gradients of this step are dropped)
+    with rv.backward_mode(debug="lookups"):
+        img, loss = step(dbg)
+        torch.cuda.synchronize()
+        fw_lookups = float(dbg.to(torch.float64).sum().item())
+        loss.backward()
+        torch.cuda.synchronize()
+        bw_lookups = float(dbg.to(torch.float64).sum().item())
+    for p, _, _ in params:
+        p.grad = None
+    voxels = float(sum(p.numel() for p, _, _ in params))
+    out_dim = img.shape[-1]
+    # gather model: 32 B per lookup, 64 B per scatter, the image written and its
+    # gradient read (4 B x OUT per pixel each), 4 B per voxel to zero the fresh
+    # gradient and 28 B per voxel for Adam (p, g, m, v read; p, m, v written)
+    bytes_step = 32.0 * (fw_lookups + bw_lookups) + 64.0 * scatters + 8.0 * out_dim * n_sensors + 32.0 * voxels
+    t = (fw_ms + bw_ms + adam_ms) * 1e-3
+    return {
+        "grad_voxel_updates_per_s": 8.0 * scatters / (bw_ms * 1e-3),
+        "trilinear_scatters_per_step": scatters, "fw_lookups_per_step": fw_lookups, "bw_lookups_per_step": bw_lookups,
+        "fw_ms": fw_ms, "bw_ms": bw_ms, "adam_ms": adam_ms,
+        "roofline": {"bound": "hbm", "achieved": bytes_step / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_step / t / 1e9 / HBM_PEAK_GBS, "bytes_per_step": bytes_step},
+        "loss": float(loss.item()),
+    }
+
+
 def gradient_leg(dev, res=256, views=8, hw=256, spp=16, steps=3):
     """Inverse-render step of the 256^3 single-scatter configuration: forward
     capture (DT collision + single scatter with RT shadow rays), L2 loss,
@@ -122,45 +191,16 @@ def gradient_leg(dev, res=256, views=8, hw=256, spp=16, steps=3):
     rv.seed(3)
     with torch.no_grad():
         target = camera.capture(inverse.single_scatter_scene(target_sigma).radiance, fw_samples=spp)
+    del target_sigma
     sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
     field = inverse.single_scatter_scene(sigma).radiance
-    opt = inverse.FusedAdam(sigma, lr=1e-2)
-    dbg = torch.zeros(views * hw * hw, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
-    inverse.optimisation_step(camera, field, sigma, target, spp, opt)  # warm-up
-    torch.cuda.synchronize()
-    fw_ms = bw_ms = adam_ms = 0.0
-    scatters = 0.0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        e = _events(4)
-        e[0].record(stream)
-        img = camera.capture(field, fw_samples=spp, bw_samples=spp, debug_out=dbg)
-        loss = ((img - target) ** 2).mean()
-        e[1].record(stream)
-        loss.backward()
-        e[2].record(stream)
-        opt.step(sigma.grad, zero_grad=True)
-        with torch.no_grad():
-            sigma.clamp_(min=0.0)
-        e[3].record(stream)
-        torch.cuda.synchronize()
-        fw_ms += e[0].elapsed_time(e[1])
-        bw_ms += e[1].elapsed_time(e[2])
-        adam_ms += e[2].elapsed_time(e[3])
-        scatters += float(dbg.to(torch.float64).sum().item())
-    wall = (time.perf_counter() - t0) * 1000.0 / steps
-    fw_ms, bw_ms, adam_ms, scatters = fw_ms / steps, bw_ms / steps, adam_ms / steps, scatters / steps
+    r = _inverse_step_timing(camera, field, target, spp, [(sigma, inverse.FusedAdam(sigma, lr=1e-2), (0.0, math.inf))],
+                             steps)
     samples = views * hw * hw * spp
-    return {
-        "workload": f"inverse render step: {res}^3 density, {views} views {hw}x{hw}, {spp} spp, DT collision + "
-                    f"single scatter (RT shadow rays), L2 loss, adjoint backward, fused Adam",
-        "grad_voxel_updates_per_s": 8.0 * scatters / (bw_ms * 1e-3),
-        "trilinear_scatters_per_step": scatters,
-        "fw_ms": fw_ms, "bw_ms": bw_ms, "adam_ms": adam_ms, "step_ms_wall": wall,
-        "fw_bw_samples_per_s": 2.0 * samples / ((fw_ms + bw_ms) * 1e-3),
-        "loss": float(loss.item()),
-    }
+    r["workload"] = (f"inverse render step: {res}^3 density, {views} views {hw}x{hw}, {spp} spp, DT collision + "
+                     f"single scatter (RT shadow rays), L2 loss, adjoint backward, fused Adam")
+    r["fw_bw_samples_per_s"] = 2.0 * samples / ((r["fw_ms"] + r["bw_ms"]) * 1e-3)
+    return r
 
 
 def _nee_scene(sigma, albedo, max_bounces):
@@ -201,44 +241,15 @@ def nee_gradient_leg(dev, res=512, ares=128, views=4, hw=512, spp=32, steps=2):
     sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
     albedo = torch.nn.Parameter(torch.full((ares, ares, ares, 3), 0.7, device=dev), requires_grad=True)
     field = _nee_scene(sigma, albedo, 8)
-    opt_s = inverse.FusedAdam(sigma, lr=1e-2)
-    opt_a = inverse.FusedAdam(albedo, lr=1e-2)
-    dbg = torch.zeros(views * hw * hw, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
-    fw_ms = bw_ms = adam_ms = 0.0
-    scatters = 0.0
-    for k in range(steps + 1):  # the first step is a warm-up
-        e = _events(4)
-        e[0].record(stream)
-        img = camera.capture(field, fw_samples=spp, bw_samples=spp, debug_out=dbg)
-        loss = ((img - target) ** 2).mean()
-        e[1].record(stream)
-        loss.backward()
-        e[2].record(stream)
-        opt_s.step(sigma.grad, zero_grad=True)
-        opt_a.step(albedo.grad, zero_grad=True)
-        with torch.no_grad():
-            sigma.clamp_(min=0.0)
-            albedo.clamp_(0.0, 1.0)
-        e[3].record(stream)
-        torch.cuda.synchronize()
-        if k == 0:
-            continue
-        fw_ms += e[0].elapsed_time(e[1])
-        bw_ms += e[1].elapsed_time(e[2])
-        adam_ms += e[2].elapsed_time(e[3])
-        scatters += float(dbg.to(torch.float64).sum().item())
-    fw_ms, bw_ms, adam_ms, scatters = fw_ms / steps, bw_ms / steps, adam_ms / steps, scatters / steps
+    r = _inverse_step_timing(camera, field, target, spp,
+                             [(sigma, inverse.FusedAdam(sigma, lr=1e-2), (0.0, math.inf)),
+                              (albedo, inverse.FusedAdam(albedo, lr=1e-2), (0.0, 1.0))], steps)
     samples = views * hw * hw * spp
-    return {
-        "workload": f"NEE inverse render step, one GPU's share: {res}^3 density + {ares}^3 RGB albedo, {views} of 32 "
-                    f"views {hw}x{hw}, {spp} spp, DT + NEE, max 8 bounces, replay backward, fused Adam",
-        "grad_voxel_updates_per_s": 8.0 * scatters / (bw_ms * 1e-3),
-        "trilinear_scatters_per_step": scatters,
-        "fw_ms": fw_ms, "bw_ms": bw_ms, "adam_ms": adam_ms,
-        "fw_samples_per_s": samples / (fw_ms * 1e-3), "bw_samples_per_s": samples / (bw_ms * 1e-3),
-        "loss": float(loss.item()),
-    }
+    r["workload"] = (f"NEE inverse render step, one GPU's share: {res}^3 density + {ares}^3 RGB albedo, {views} of 32 "
+                     f"views {hw}x{hw}, {spp} spp, DT + NEE, max 8 bounces, replay backward, fused Adam")
+    r["fw_samples_per_s"] = samples / (r["fw_ms"] * 1e-3)
+    r["bw_samples_per_s"] = samples / (r["bw_ms"] * 1e-3)
+    return r
 
 
 def time_to_rmse_leg(dev, radiance, camera, ref_spp=4096, rel_eps=0.05):

@@ -162,11 +162,12 @@ typedef struct vr_launch {
     int32_t  samples;              /* spp (capture); ignored by maps */
     int32_t  bw_mode;              /* vr_bw_mode */
     int32_t  seed_mode;            /* vr_seed_mode used by vr_capture_bw (fw uses CAPTURE_FW) */
-    int32_t  reserved;
+    int32_t  debug_kind;           /* what debug_out counts in a backward: 0 = gradient scatters,
+                                      1 = trilinear lookups (forward entry points: lookups) */
     int64_t  start_index;          /* first thread index of this batch (batch_size chunking) */
     int64_t  count;                /* threads in this batch; < 0 = all remaining */
-    int32_t* debug_out;            /* optional per-thread counter: trilinear lookups (forward
-                                      entry points) or trilinear gradient scatters (backward) */
+    int32_t* debug_out;            /* optional per-thread counter (the reference's DEBUG_COUNTER
+                                      slot, _maps.py:368,611-615), see debug_kind */
 } vr_launch;
 
 /* ---- entry points ---------------------------------------------------------- */
@@ -217,6 +218,12 @@ int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera
  * reference leaves optimisation to torch. */
 int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
                  float eps, int32_t step, int32_t zero_grad, void* stream);
+
+/* vr_adam_step followed by the projection p = min(max(p, lo), hi) in the same
+ * pass (densities >= 0, albedos in [0, 1]); lo <= hi.  No reference
+ * counterpart, as vr_adam_step. */
+int vr_adam_step_bounded(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
+                         float eps, int32_t step, float lo, float hi, int32_t zero_grad, void* stream);
 
 #ifdef __cplusplus
 }

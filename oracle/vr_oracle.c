@@ -1846,8 +1846,8 @@ int oracle_capture_bw(const vr_medium* M, int32_t kind, const vr_camera* cam, co
     return VR_OK;
 }
 
-int oracle_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
-                     float eps, int32_t step, int32_t zero_grad) {
+int oracle_adam_step_bounded(float* p, float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                             float eps, int32_t step, float lo, float hi, int32_t zero_grad) {
     if (step < 1) return fail(VR_ERR_ARG, "step must be >= 1");
     double bc1 = 1.0 - pow((double)b1, step), bc2 = 1.0 - pow((double)b2, step);
     for (int64_t i = 0; i < n; i++) {
@@ -1856,10 +1856,15 @@ int oracle_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr
         double vi = b2 * (double)v[i] + (1.0 - b2) * gi * gi;
         m[i] = (float)mi; v[i] = (float)vi;
         double denom = sqrt(vi / bc2) + eps;
-        p[i] = (float)((double)p[i] - lr / bc1 * mi / denom);
+        p[i] = (float)dmin(dmax((double)p[i] - lr / bc1 * mi / denom, lo), hi);
         if (zero_grad) g[i] = 0.0f;
     }
     return VR_OK;
+}
+
+int oracle_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                     float eps, int32_t step, int32_t zero_grad) {
+    return oracle_adam_step_bounded(p, g, m, v, n, lr, b1, b2, eps, step, -INFINITY, INFINITY, zero_grad);
 }
 
 /* Raw RNG access for known-answer tests: n uniforms from a state. */

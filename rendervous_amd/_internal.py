@@ -48,17 +48,24 @@ class BackwardSettings:
                primal/replay majorants, single-scatter transmittance gradient
                along the sampled direction, full path weights in the collision
                sampler); 'reference' reproduces the reference formulas.
+    debug:     what a capture's debug_out counts in the backward: 'scatters'
+               (trilinear gradient scatters) or 'lookups' (trilinear lookups).
     """
     seeds: str = "replay"
     estimator: str = "consistent"
+    debug: str = "scatters"
 
 
 backward_settings = BackwardSettings()
 
 
 @contextlib.contextmanager
-def backward_mode(seeds: typing.Optional[str] = None, estimator: typing.Optional[str] = None):
+def backward_mode(seeds: typing.Optional[str] = None, estimator: typing.Optional[str] = None,
+                  debug: typing.Optional[str] = None):
     old = dataclasses.replace(backward_settings)
+    if debug is not None:
+        assert debug in ("scatters", "lookups")
+        backward_settings.debug = debug
     if seeds is not None:
         assert seeds in ("replay", "fresh")
         backward_settings.seeds = seeds
@@ -70,3 +77,4 @@ def backward_mode(seeds: typing.Optional[str] = None, estimator: typing.Optional
     finally:
         backward_settings.seeds = old.seeds
         backward_settings.estimator = old.estimator
+        backward_settings.debug = old.debug

@@ -117,7 +117,7 @@ class VrLaunch(ctypes.Structure):
         ("samples", ctypes.c_int32),
         ("bw_mode", ctypes.c_int32),
         ("seed_mode", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("debug_kind", ctypes.c_int32),
         ("start_index", ctypes.c_int64),
         ("count", ctypes.c_int64),
         ("debug_out", ctypes.c_void_p),
@@ -138,6 +138,10 @@ _SIGNATURES = {
     "vr_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                                     ctypes.c_int32, ctypes.c_void_p]),
+    "vr_adam_step_bounded": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                            ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
+                                            ctypes.c_void_p]),
 }
 
 # Symbols declared in include/vr_api.h (checked by the CPU test suite).
@@ -223,7 +227,8 @@ def grid_view(t: torch.Tensor, bmin, bmax, grad: torch.Tensor | None = None) -> 
 
 
 def launch(seeds, samples: int = 1, bw_mode: int = VR_BW_REFERENCE, seed_mode: int = VR_SEED_CAPTURE_FW,
-           start_index: int = 0, count: int = -1, debug_out: torch.Tensor | None = None) -> VrLaunch:
+           start_index: int = 0, count: int = -1, debug_out: torch.Tensor | None = None,
+           debug_kind: int = 0) -> VrLaunch:
     L = VrLaunch()
     for i in range(4):
         L.seeds[i] = int(seeds[i]) & 0xFFFFFFFF
@@ -233,4 +238,5 @@ def launch(seeds, samples: int = 1, bw_mode: int = VR_BW_REFERENCE, seed_mode: i
     L.start_index = int(start_index)
     L.count = int(count)
     L.debug_out = None if debug_out is None else debug_out.data_ptr()
+    L.debug_kind = int(debug_kind)
     return L

@@ -296,11 +296,12 @@ __global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry,
 #ifndef VR_CAPTURE_WAVES_PER_EU
 #define VR_CAPTURE_WAVES_PER_EU 5
 #endif
-#if VR_CAPTURE_WAVES_PER_EU > 0
-#define VR_CAPTURE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(VR_CAPTURE_WAVES_PER_EU)))
-#else
-#define VR_CAPTURE_OCCUPANCY
-#endif
+// The deterministic DDA (FP64 traversal, short launches) keeps its registers.
+template <int KIND>
+constexpr int capture_waves() {
+    return (KIND == VR_MAP_T_DDA || KIND == VR_MAP_RAYBOX || VR_CAPTURE_WAVES_PER_EU < 1) ? 1 : VR_CAPTURE_WAVES_PER_EU;
+}
+#define VR_CAPTURE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(capture_waves<KIND>())))
 
 template <int KIND, int MAJ>
 __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_fw(MediumView M, const float* __restrict__ in, int64_t start, int64_t cnt,
@@ -340,6 +341,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_bw(MediumVi
 // into rows zeroed by k_capture_prepare.
 struct CaptureSplit {
     int spp, chunk, chunks;
+    int debug_lookups;  // the backward's debug_out counts lookups instead of scatters
 };
 
 
@@ -418,8 +420,9 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
         field_bw<KIND, MAJ>(tr, a, o);
     }
     if (dbg) {
-        if (P.chunks == 1) dbg[i] = tr.scatters;
-        else atomicAdd(dbg + i, tr.scatters);
+        const int n = P.debug_lookups ? tr.lookups : tr.scatters;
+        if (P.chunks == 1) dbg[i] = n;
+        else atomicAdd(dbg + i, n);
     }
 }
 
@@ -437,7 +440,7 @@ __global__ void k_capture_prepare(float* __restrict__ out, int od, int64_t start
 // torch.optim.Adam (no weight decay, no amsgrad), optionally clearing the gradient.
 __global__ void __launch_bounds__(kBlock) k_adam(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m,
                                                  float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
-                                                 float bc1, float bc2_sqrt, int zero_grad) {
+                                                 float bc1, float bc2_sqrt, float lo, float hi, int zero_grad) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
@@ -447,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_adam(float* __restrict__ p, float* _
         m[i] = mi;
         v[i] = vi;
         float denom = sqrtf(vi) / bc2_sqrt + eps;
-        p[i] = p[i] - (lr / bc1) * mi / denom;
+        p[i] = fminf(fmaxf(p[i] - (lr / bc1) * mi / denom, lo), hi);
         if (zero_grad) g[i] = 0.0f;
     }
 }
@@ -577,6 +580,7 @@ CaptureSplit capture_split(int spp, bool one_chunk) {
     chunk = env_int("VR_CAPTURE_CHUNK", chunk);
     P.chunk = one_chunk ? spp : std::max(1, std::min(spp, chunk));
     P.chunks = (spp + P.chunk - 1) / P.chunk;
+    P.debug_lookups = 0;
     return P;
 }
 
@@ -612,9 +616,11 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
 }
 
 template <int MAJ>
-int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int seed_mode, int64_t total,
+int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int seed_mode, int debug_kind,
+                          int64_t total,
                           int64_t start, int64_t cnt, const float* og, int* dbg, hipStream_t st) {
-    const CaptureSplit P = capture_split(spp, seed_mode == VR_SEED_CAPTURE_BW);
+    CaptureSplit P = capture_split(spp, seed_mode == VR_SEED_CAPTURE_BW);
+    P.debug_lookups = debug_kind == 1;
     int rc = capture_prepare(P, nullptr, 0, start, cnt, dbg, st);
     if (rc) return rc;
     dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
@@ -756,14 +762,17 @@ int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera
     Seeds sd = make_seeds(L);
     hipStream_t s = (hipStream_t)stream;
     if (medium->majorant_kind == VR_MAJ_GRID)
-        return launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, L->debug_out, s);
-    return launch_capture_bw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, L->seed_mode, total, start, cnt, og, L->debug_out, s);
+        return launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, L->debug_kind, total, start, cnt, og,
+                                                  L->debug_out, s);
+    return launch_capture_bw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, L->seed_mode, L->debug_kind, total, start, cnt, og,
+                                               L->debug_out, s);
 }
 
-int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps, int32_t step,
-                 int32_t zero_grad, void* stream) {
+int vr_adam_step_bounded(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps,
+                         int32_t step, float lo, float hi, int32_t zero_grad, void* stream) {
     if (n < 0) return set_err(VR_ERR_ARG, "vr_adam_step: negative n");
     if (step < 1) return set_err(VR_ERR_ARG, "vr_adam_step: step must be >= 1");
+    if (!(lo <= hi)) return set_err(VR_ERR_ARG, "vr_adam_step: bounds must satisfy lo <= hi");
     if (n == 0) return VR_OK;
     if (!p || !g || !m || !v) return set_err(VR_ERR_ARG, "vr_adam_step: null buffer");
     float bc1 = (float)(1.0 - std::pow((double)beta1, step));
@@ -771,8 +780,13 @@ int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, fl
     int64_t blocks = (n + kBlock - 1) / kBlock;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(kBlock), 0, (hipStream_t)stream, p, g, m, v, n, lr, beta1, beta2, eps,
-                       bc1, bc2s, zero_grad);
+                       bc1, bc2s, lo, hi, zero_grad);
     return check_hip(hipGetLastError(), "vr_adam_step launch");
+}
+
+int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps, int32_t step,
+                 int32_t zero_grad, void* stream) {
+    return vr_adam_step_bounded(p, g, m, v, n, lr, beta1, beta2, eps, step, -INFINITY, INFINITY, zero_grad, stream);
 }
 
 }  // extern "C"

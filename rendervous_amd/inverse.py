@@ -12,6 +12,7 @@ gradient reset are one fused kernel (vr_adam_step).
 from __future__ import annotations
 
 import dataclasses
+import math
 import typing
 
 import torch
@@ -58,12 +59,14 @@ class FusedAdam:
         self.lr, self.b1, self.b2, self.eps = lr, betas[0], betas[1], eps
         self.t = 0
 
-    def step(self, grad: torch.Tensor, zero_grad: bool = True):
+    def step(self, grad: torch.Tensor, zero_grad: bool = True, bounds: typing.Optional[typing.Tuple[float, float]] = None):
+        """One update; bounds = (lo, hi) projects the parameters onto [lo, hi] in the same pass."""
         assert grad.is_contiguous() and grad.shape == self.p.shape
         self.t += 1
-        L.check(L.load().vr_adam_step(self.p.data_ptr(), grad.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
-                                      self.p.numel(), self.lr, self.b1, self.b2, self.eps, self.t, int(zero_grad),
-                                      L.stream_handle(self.p.device)), "vr_adam_step")
+        lo, hi = (-math.inf, math.inf) if bounds is None else (float(bounds[0]), float(bounds[1]))
+        L.check(L.load().vr_adam_step_bounded(self.p.data_ptr(), grad.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+                                              self.p.numel(), self.lr, self.b1, self.b2, self.eps, self.t, lo, hi,
+                                              int(zero_grad), L.stream_handle(self.p.device)), "vr_adam_step_bounded")
         # the kernel writes p and grad behind torch's back: bump their version
         # counters so caches keyed on them (the majorant grid) are rebuilt
         increment_version(self.p)
@@ -87,8 +90,6 @@ def optimisation_step(camera, radiance, sigma_param: torch.nn.Parameter, target:
     img = render(camera, radiance, spp)
     loss = ((img - target) ** 2).mean()
     loss.backward()
-    opt.step(sigma_param.grad, zero_grad=True)
-    if clamp_min is not None:
-        with torch.no_grad():
-            sigma_param.clamp_(min=clamp_min)
+    opt.step(sigma_param.grad, zero_grad=False, bounds=None if clamp_min is None else (clamp_min, math.inf))
+    sigma_param.grad = None  # the next backward's fresh gradient becomes .grad without an extra add
     return loss.detach()

@@ -812,7 +812,8 @@ class AutogradCaptureFunction(torch.autograd.Function):
     adds them into .grad once (the reference's backward both accumulates into
     .grad in place and returns it, _maps.py:1049-1058,1642).  A `debug_out`
     tensor receives per-sensor trilinear lookups in the forward and trilinear
-    gradient scatters in the backward.
+    gradient scatters (or lookups, backward_mode(debug="lookups")) in the
+    backward.
     """
 
     @staticmethod
@@ -864,7 +865,8 @@ class AutogradCaptureFunction(torch.autograd.Function):
             lib = L.load()
             for start in range(0, total, max(1, bs)):
                 launch = L.launch(seeds, samples=samples, bw_mode=_bw_mode(), seed_mode=seed_mode,
-                                  start_index=start, count=min(bs, total - start), debug_out=ctx.debug_out)
+                                  start_index=start, count=min(bs, total - start), debug_out=ctx.debug_out,
+                                  debug_kind=int(_internal.backward_settings.debug == "lookups"))
                 L.check(lib.vr_capture_bw(lw.medium, kind, cam, launch, L.ptr(og), L.stream_handle(og.device)),
                         "vr_capture_bw")
         return (None, None, None, None, None, None, None, *grads)

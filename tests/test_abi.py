@@ -55,6 +55,7 @@ def test_argument_errors_are_reported_without_a_device():
     lau = L.launch((1, 2, 3, 4))
     assert lib.vr_map_fw(None, 99, None, 0, None, lau, None) == L.VR_ERR_UNSUPPORTED
     assert lib.vr_adam_step(None, None, None, None, 4, 1e-3, 0.9, 0.999, 1e-8, 0, 0, None) == L.VR_ERR_ARG
+    assert lib.vr_adam_step_bounded(None, None, None, None, 4, 1e-3, 0.9, 0.999, 1e-8, 1, 1.0, 0.0, 0, None) == L.VR_ERR_ARG
 
 
 def test_nee_integrator_lowers_to_a_medium_descriptor():

@@ -262,3 +262,20 @@ def test_adam_matches_oracle(gpu):
     np.testing.assert_allclose(tm.cpu().numpy(), rm, rtol=1e-5, atol=1e-7)
     np.testing.assert_allclose(tv.cpu().numpy(), rv, rtol=1e-5, atol=1e-7)
     assert float(tg.abs().max()) == 0.0
+
+
+def test_bounded_adam_matches_oracle(gpu):
+    _print_case("adam_bounded")
+    rng = np.random.default_rng(27)
+    n = 4099
+    p, g, m, v = (rng.normal(size=n).astype(np.float32) for _ in range(4))
+    v = np.abs(v)
+    rp, rg, rm, rv = p.copy(), g.copy(), m.copy(), v.copy()
+    O.check(O.load().oracle_adam_step_bounded(O.nptr(rp), O.nptr(rg), O.nptr(rm), O.nptr(rv), n, 0.5, 0.9, 0.999, 1e-8,
+                                              2, 0.0, 0.75, 0), "adam_bounded")
+    tp, tg, tm, tv = (torch.from_numpy(a.copy()).cuda() for a in (p, g, m, v))
+    L.check(L.load().vr_adam_step_bounded(tp.data_ptr(), tg.data_ptr(), tm.data_ptr(), tv.data_ptr(), n, 0.5, 0.9, 0.999,
+                                          1e-8, 2, 0.0, 0.75, 0, None), "vr_adam_step_bounded")
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(tp.cpu().numpy(), rp, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(tg.cpu().numpy(), g)

@@ -360,3 +360,30 @@ def test_dt_path_and_nee_estimate_the_same_radiance():
     b = oracle_map_fw(sc, L.VR_MAP_NEE_DT, np.tile(ray, (n, 1)), (5, 6, 7, 8)).astype(np.float64)
     se = np.sqrt(a.var(0) / n + b.var(0) / n)
     assert np.all(np.abs(a.mean(0) - b.mean(0)) <= 4 * se), (a.mean(0), b.mean(0), se)
+
+
+# ---------------------------------------------------------------- Adam --
+def test_adam_matches_torch_optim_adam():
+    """Three steps of the oracle's Adam equal torch.optim.Adam (no weight decay)
+    on the same gradients; the bounded form is the projection of that update."""
+    rng = np.random.default_rng(26)
+    n = 1000
+    p0 = rng.normal(size=n).astype(np.float32)
+    grads = [rng.normal(size=n).astype(np.float32) for _ in range(3)]
+    tp = torch.nn.Parameter(torch.from_numpy(p0.copy()))
+    opt = torch.optim.Adam([tp], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    p, m, v = p0.copy(), np.zeros(n, np.float32), np.zeros(n, np.float32)
+    pb, mb, vb = p0.copy(), np.zeros(n, np.float32), np.zeros(n, np.float32)
+    for k, g in enumerate(grads):
+        tp.grad = torch.from_numpy(g.copy())
+        opt.step()
+        gg = g.copy()
+        O.check(O.load().oracle_adam_step(O.nptr(p), O.nptr(gg), O.nptr(m), O.nptr(v), n, 1e-2, 0.9, 0.999, 1e-8,
+                                          k + 1, 1), "adam")
+        assert np.all(gg == 0)
+        gb = g.copy()
+        O.check(O.load().oracle_adam_step_bounded(O.nptr(pb), O.nptr(gb), O.nptr(mb), O.nptr(vb), n, 1e-2, 0.9, 0.999,
+                                                  1e-8, k + 1, 0.0, 0.5, 0), "adam_bounded")
+        assert np.all(gb == g)
+    np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-5, atol=1e-6)
+    assert pb.min() >= 0.0 and pb.max() <= 0.5
