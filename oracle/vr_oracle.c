@@ -1780,6 +1780,11 @@ int oracle_capture_fw(const vr_medium* M, int32_t kind, const vr_camera* cam, co
         sensor_index(cam, i, &pose, &row, &col);
         double acc[16] = {0};
         int64_t lk = 0;
+        if (pose < 0 || pose >= cam->num_poses) {  /* outside the pose buffer: 0 (not read) */
+            for (int k = 0; k < od; k++) out[i * od + k] = 0.0f;
+            if (L->debug_out) L->debug_out[i] = 0;
+            continue;
+        }
         for (int s = 0; s < spp; s++) {
             ctx_t c;
             memset(&c, 0, sizeof(c));
@@ -1820,6 +1825,7 @@ int oracle_capture_bw(const vr_medium* M, int32_t kind, const vr_camera* cam, co
         for (int64_t i = start; i < start + cnt; i++) {
             int pose, row, col;
             sensor_index(cam, i, &pose, &row, &col);
+            if (pose < 0 || pose >= cam->num_poses) continue;  /* outside the pose buffer: no gradient */
             double o[16];
             for (int k = 0; k < od; k++) o[k] = og[i * od + k] / spp;
             ctx_t c;

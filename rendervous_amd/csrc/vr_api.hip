@@ -356,7 +356,9 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_fw(Medi
     constexpr int OD = Dims<KIND>::out;
     int pose, row, col;
     sensor_index(cam, i, pose, row, col);
-    if (pose < 0 || pose >= cam.num_poses || row < 0 || row >= cam.height || col < 0 || col >= cam.width) {
+    // a sensor's row and column are used as given (the reference does not check
+    // them); a pose index outside the pose buffer yields 0 instead of a fault
+    if (pose < 0 || pose >= cam.num_poses) {
         if (P.chunks == 1) {
 #pragma unroll
             for (int k = 0; k < OD; k++) out[i * OD + k] = 0.0f;
@@ -400,7 +402,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
     constexpr int OD = Dims<KIND>::out;
     int pose, row, col;
     sensor_index(cam, i, pose, row, col);
-    if (pose < 0 || pose >= cam.num_poses || row < 0 || row >= cam.height || col < 0 || col >= cam.width) return;
+    if (pose < 0 || pose >= cam.num_poses) return;
     float o[OD];
     bool any = false;
 #pragma unroll

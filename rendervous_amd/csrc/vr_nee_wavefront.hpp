@@ -271,7 +271,8 @@ VR_DEV void tiled_pixel(uint32_t k, uint32_t width, uint32_t height, int& row, i
     col = (int)(blk * 8u + (q - r * bw));
 }
 
-// Decodes item l of queue q; false if its sensor lies outside the camera.
+// Decodes item l of queue q; false if its pose index lies outside the pose
+// buffer (rows and columns are used as given, as the reference does).
 VR_DEV bool nee_item(const NeeSchedule& S, const CameraView& cam, int q, uint32_t l, uint32_t& sensor, int& s0, int& s1,
                      int& pose, int& row, int& col) {
     const uint32_t nqs = nee_queue_sensors(S, q);
@@ -284,7 +285,7 @@ VR_DEV bool nee_item(const NeeSchedule& S, const CameraView& cam, int q, uint32_
         pose = (int)cam.sensors[3 * (size_t)idx];
         row = (int)cam.sensors[3 * (size_t)idx + 1];
         col = (int)cam.sensors[3 * (size_t)idx + 2];
-        return pose >= 0 && pose < cam.num_poses && row >= 0 && row < cam.height && col >= 0 && col < cam.width;
+        return pose >= 0 && pose < cam.num_poses;
     }
     const uint32_t w = (uint32_t)cam.width, hw = w * (uint32_t)cam.height;
     const uint32_t p = idx / hw;

@@ -1,0 +1,118 @@
+"""Edge cases of the HIP path: sensor lists, batching, strided grids, empty
+launches, degenerate media and rays that miss or start inside the box."""
+
+import numpy as np
+import pytest
+import torch
+
+import rendervous_amd as rv
+from helpers import (L, Scene, Built, Camera, blob_grid, fbm_grid, gpu_capture_fw, gpu_map_fw, match_fraction,
+                     oracle_capture_fw, oracle_map_fw, pose, random_rays)
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = (12345, 67890, 13579, 24680)
+
+
+def _nee_scene(**kw):
+    args = dict(sigma=fbm_grid(24, seed=9, scale=20.0), albedo_c=(0.9, 0.8, 0.7), phase_kind=L.VR_PHASE_HG,
+                phase_g=0.6, env_c=(0.5, 0.5, 0.5), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(3.0, 3.0, 3.0),
+                sun_prob=0.5, majorant="grid", transmittance_kind=L.VR_T_DT, max_bounces=16)
+    args.update(kw)
+    return Scene(**args)
+
+
+@pytest.mark.parametrize("kind", [L.VR_MAP_NEE_DT, L.VR_MAP_MC_COLLISION])
+def test_capture_of_a_sensor_list_is_seed_matched(gpu, kind):
+    """Explicit (pose, row, col) sensors with duplicates, a row outside the image
+    (used as given, as the reference does) and a pose outside the pose buffer
+    (0): the persistent NEE kernel's sensor-list path and the generic one."""
+    print(f"[case] sensor_list kind={kind}", flush=True)
+    rng = np.random.default_rng(3)
+    sensors = np.stack([rng.integers(0, 2, 300), rng.integers(0, 32, 300), rng.integers(0, 40, 300)], 1)
+    sensors[7] = sensors[3]
+    sensors[11] = (2, 3, 5)   # pose out of range
+    sensors[12] = (0, 40, 5)  # row outside the image
+    poses = np.stack([pose((0.0, 0.3, 2.5)), pose((1.5, 0.2, -1.8))])
+    cam = dict(poses=poses, width=40, height=32, jittered=True, sensors=sensors)
+    sc = _nee_scene(transmittance_kind=L.VR_T_RT)
+    ref = oracle_capture_fw(sc, kind, cam, SEEDS, 3)
+    out = gpu_capture_fw(sc, kind, cam, SEEDS, 3)
+    assert np.all(out[11] == 0) and np.all(ref[11] == 0)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.97
+
+
+def _gpu_capture_range(sc, kind, cam_args, spp, start, count, out):
+    b = Built(sc, "cuda")
+    cam = Camera(device="cuda", **cam_args)
+    lau = L.launch(SEEDS, samples=spp, start_index=start, count=count)
+    L.check(L.load().vr_capture_fw(b.medium, kind, cam.c, lau, out.data_ptr(), None), "vr_capture_fw")
+
+
+@pytest.mark.parametrize("kind", [L.VR_MAP_NEE_DT, L.VR_MAP_PATH_DT])
+def test_batched_capture_equals_one_launch(gpu, kind):
+    """start_index/count batches (_maps.py:807-810) draw the same per-sample
+    streams as one launch, so the image is the same up to summation order."""
+    print(f"[case] batched kind={kind}", flush=True)
+    sc = _nee_scene()
+    cam = dict(poses=np.stack([pose((0.0, 0.3, 2.5)), pose((-2.0, 0.5, 1.0))]), width=37, height=29, jittered=True)
+    n = 2 * 37 * 29
+    whole = torch.zeros(n, 3, device="cuda")
+    _gpu_capture_range(sc, kind, cam, 8, 0, -1, whole)
+    parts = torch.full((n, 3), -1.0, device="cuda")
+    for start in range(0, n, 500):
+        _gpu_capture_range(sc, kind, cam, 8, start, min(500, n - start), parts)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(parts.cpu().numpy(), whole.cpu().numpy(), rtol=2e-5, atol=1e-6)
+
+
+def test_empty_launch_leaves_the_output_alone(gpu):
+    sc = _nee_scene()
+    cam = dict(poses=pose((0.0, 0.3, 2.5))[None], width=8, height=8)
+    out = torch.full((64, 3), 7.0, device="cuda")
+    _gpu_capture_range(sc, L.VR_MAP_NEE_DT, cam, 4, 10, 0, out)
+    torch.cuda.synchronize()
+    assert float(out.min()) == 7.0 and float(out.max()) == 7.0
+
+
+def test_strided_density_view_equals_its_contiguous_copy(gpu):
+    """A non-contiguous density (a transposed view, Parameter strides of
+    common.h:88-94) is read through its strides: identical DDA images."""
+    g = torch.from_numpy(blob_grid(20, seed=5, max_value=15.0)).cuda()
+    view = g.permute(2, 1, 0, 3)  # strides (1, 20, 400, 1) in (z, y, x, c) order
+    assert not view.is_contiguous()
+    cam = rv.PerspectiveCameraSensor(48, 48, rv.functions.look_at_pose((0.4, 0.3, 2.2))[None].cuda())
+    imgs = []
+    for t in (view, view.contiguous()):
+        bmin, bmax = rv.normalized_box(t)
+        field = rv.GridDDATransmittance(rv.Grid3D(t, bmin, bmax), rv.RayBoxIntersection(bmin, bmax))
+        with torch.no_grad():
+            imgs.append(cam.capture(field).cpu())
+    assert torch.equal(imgs[0], imgs[1])
+
+
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_T_DDA, L.VR_MAP_NEE_DT, L.VR_MAP_PATH_DT])
+def test_empty_medium(gpu, kind):
+    """sigma = 0: transmittance 1 and radiance = environment, exactly."""
+    sc = Scene(sigma=np.zeros((6, 6, 6, 1), np.float32), albedo_c=(0.8, 0.8, 0.8), env_c=(0.25, 0.5, 1.0),
+               phase_kind=L.VR_PHASE_HG, phase_g=0.3)
+    rays = random_rays(1024, seed=31)
+    out = gpu_map_fw(sc, kind, rays, SEEDS)
+    want = 1.0 if L.MAP_DIMS[kind][1] == 1 else np.array([0.25, 0.5, 1.0], np.float32)
+    np.testing.assert_allclose(out, np.broadcast_to(want, out.shape), rtol=1e-6)
+
+
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_NEE_DT, L.VR_MAP_MC_COLLISION])
+def test_rays_that_miss_or_start_inside_the_box(gpu, kind):
+    """Rays pointing away from the box (T = 1, radiance = env) and rays that
+    start inside it, seed-matched with the oracle."""
+    sc = _nee_scene(transmittance_kind=L.VR_T_RT, majorant="const")
+    away = random_rays(512, seed=32)
+    away[:, 3:] *= -1.0  # origins on a sphere of radius 1.2, pointing outwards
+    inside = random_rays(512, seed=33, inside=True)
+    rays = np.concatenate([away, inside]).astype(np.float32)
+    ref = oracle_map_fw(sc, kind, rays, SEEDS)
+    out = gpu_map_fw(sc, kind, rays, SEEDS)
+    want = 1.0 if L.MAP_DIMS[kind][1] == 1 else np.array([0.5, 0.5, 0.5], np.float32)
+    np.testing.assert_allclose(out[:512], np.broadcast_to(want, out[:512].shape), rtol=1e-6)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.98
