@@ -288,6 +288,94 @@ def time_to_rmse_leg(dev, radiance, camera, ref_spp=4096, rel_eps=0.05):
             "ladder": rows}
 
 
+def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32):
+    """Neural radiance cache vs plain NEE on the 512^3 cloud (fBm x60, albedo
+    0.99, HG g=0.8, sun + env 0.1), 1024x1024: epsilon = the RMSE of the plain
+    renderer at 256 spp against a 4096-spp plain render; the cache is trained
+    online (8 traces of 65536 paths, 8 Adam steps of 24576 records each) and its
+    renders (paths ended into the MLP after 2 bounces) are taken at doubling spp
+    until the RMSE is <= epsilon.  Times: HIP events around the GPU work."""
+    import torch
+    import rendervous_amd as rv
+    from rendervous_amd.nrc import NeuralRadianceCache
+
+    sigma = rv.functions.fbm_value_noise(res, seed=13, scale=60.0, sphere_radius=0.5, device=dev)
+    ds = rv.objects.DependencySet()
+    ds.add_parameters(sigma_tensor=sigma, scattering_albedo_tensor=torch.tensor([0.99, 0.99, 0.99]),
+                      environment_tensor=torch.tensor([0.1, 0.1, 0.1]), phase_g_tensor=torch.tensor([0.8]))
+    ds.requires(rv.objects.medium_majorant_coarse_grid, resolution=16)
+    ds.requires(rv.objects.medium_environment_sampler_sun, direction=(0.4, 0.8, 0.45), radiance=(3.0, 3.0, 3.0),
+                probability=0.5)
+    ds.requires(rv.objects.medium_phase_HG, mixture=False)
+    ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
+    ds.requires(rv.objects.medium_transmittance_DT)
+    ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DT, max_bounces=max_bounces)
+    field = ds.radiance
+    camera = rv.PerspectiveCameraSensor(hw, hw, rv.functions.look_at_pose((0.0, 0.3, 2.5))[None].to(dev),
+                                        jittered=True)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn):
+        e = _events(2)
+        e[0].record(stream)
+        out = fn()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        return out, e[0].elapsed_time(e[1])
+
+    with torch.no_grad():
+        ref = None
+        for k in range(ref_spp // 64):
+            rv.seed(100000 + k)
+            img = camera.capture(field, fw_samples=64)
+            ref = img if ref is None else ref + img
+        ref = ref / (ref_spp // 64)
+
+        def rmse(img):
+            return float(((img - ref) ** 2).mean().sqrt())
+
+        rv.seed(7)
+        img, plain_ms = timed(lambda: camera.capture(field, fw_samples=target_spp))
+        eps = rmse(img)
+        cache = NeuralRadianceCache(dev, seed=7, lr=1e-2)
+
+        def train():
+            for t in range(8):
+                rv.seed(5000 + t)
+                cache.train(field, camera, n_paths=65536, max_vertices=32, steps=8, batch=24576)
+        _, train_ms = timed(train)
+        ladder, hit = [], None
+        spp = 4
+        while spp <= target_spp:
+            rv.seed(77 + spp)
+            img, ms = timed(lambda: cache.render(field, camera, spp=spp))
+            ladder.append({"spp": spp, "ms": ms, "rmse": rmse(img)})
+            if ladder[-1]["rmse"] <= eps:
+                hit = ladder[-1]
+                break
+            spp *= 2
+        # MLP throughput: inference over the last render's queries and one training step
+        queries = cache.queries_last()
+        feats = torch.rand(queries, 96, device=dev) * 2.0 - 1.0
+        outq = torch.empty(queries, 3, device=dev)
+        from rendervous_amd import _lib as L
+        _, infer_ms = timed(lambda: L.check(L.load().vr_nrc_mlp_fw(feats.data_ptr(), queries, cache.weights.data_ptr(),
+                                                                    outq.data_ptr(), L.stream_handle(dev)), "mlp_fw"))
+    flops_q = 2.0 * (76 * 64 + 3 * 64 * 64 + 64 * 3)  # algorithmic FLOPs per query (no padding)
+    tflops = flops_q * queries / (infer_ms * 1e-3) / 1e12
+    nrc_ms = train_ms + hit["ms"] if hit else None
+    return {
+        "workload": f"time to RMSE <= plain {target_spp} spp RMSE, {res}^3 cloud fBm x60, albedo 0.99, g 0.8, "
+                    f"{hw}x{hw}, max {max_bounces} bounces, vs a {ref_spp}-spp plain render",
+        "epsilon": eps, "plain_ms": plain_ms, "plain_spp": target_spp,
+        "nrc_train_ms": train_ms, "nrc_render": hit, "nrc_ms": nrc_ms,
+        "speedup": plain_ms / nrc_ms if nrc_ms else None, "ladder": ladder,
+        "mlp_inference": {"queries": queries, "ms": infer_ms,
+                          "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                                       "frac": tflops / 2500.0}},
+    }
+
+
 def dda_leg(dev, reps=20):
     """Deterministic DDA transmittance of the 64^3 Gaussian-blob grid, 128x128, 1 ray/pixel."""
     import torch
@@ -410,6 +498,8 @@ def main():
         result["secondary"] = {"time_to_rmse": rmse, "gradient": gradient_leg(dev)}
         torch.cuda.empty_cache()
         result["secondary"]["nee_gradient"] = nee_gradient_leg(dev)
+        torch.cuda.empty_cache()
+        result["secondary"]["nrc"] = nrc_leg(dev)
         torch.cuda.empty_cache()
         result["secondary"]["dda"] = dda_leg(dev)
         log(f"secondary: {json.dumps(result['secondary'])}")

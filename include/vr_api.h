@@ -225,6 +225,49 @@ int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, fl
 int vr_adam_step_bounded(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
                          float eps, int32_t step, float lo, float hi, int32_t zero_grad, void* stream);
 
+/* ---- neural radiance cache ---------------------------------------------------
+ * No reference counterpart (the north star's cached renderer): a 96-64-64-64-64-3
+ * ReLU MLP in BF16 with FP32 accumulation.  Q(x, w) = radiance a NEE path collects
+ * from a real collision at x, reached along w, onwards.  Weights: VR_NRC_PARAMS
+ * values, per layer Wt[n][k] (output n, input k) for layers 96->64, 64->64 (x3),
+ * 64->16 (outputs 3..15 unused); `params` FP32 master copy, `weights` BF16 bits. */
+#define VR_NRC_PARAMS 19456
+#define VR_NRC_FEATURES 96
+
+/* Input features of n points x and directions w ([n][3] each): sin/cos of
+ * pi 2^f u_c (f < 12, u = x mapped from the box to [0,1]^3), w, 1, zeros. */
+int vr_nrc_encode(const float* x, const float* w, int64_t n, const float box_min[3], const float box_max[3],
+                  float* features, void* stream);
+
+/* MLP on explicit features [n][96] (rounded to BF16) -> out [n][3]. */
+int vr_nrc_mlp_fw(const float* features, int64_t n, const uint16_t* weights, float* out, void* stream);
+
+/* One training step on the records (x[3], w[3], target[3]) of rows
+ * [f, min(*count, f + max_records)) of `records`, f = first % *count (*count on
+ * the device: the rows written), so that windows first = k * max_records cycle
+ * through the records: relative L2 loss, gradient into `grad` (overwritten), Adam step
+ * `step` (>= 1) on `params`/`m`/`v`, BF16 copy into `weights`, mean loss into
+ * *loss (device).  lr = 0 leaves the weights. */
+int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first, int64_t max_records, const float box_min[3],
+                      const float box_max[3], float* params, uint16_t* weights, float* grad, float* m, float* v,
+                      int32_t step, float lr, float beta1, float beta2, float eps, float* loss, void* stream);
+
+/* Training samples: n_paths NEE paths from uniformly drawn sensors (seeded by
+ * launch->seeds), each collision vertex (at most max_vertices per path) giving a
+ * record (x, w, suffix radiance / throughput); *count (device) receives the
+ * number written (capped at max_records).  scratch: n_paths*max_vertices*12 floats. */
+int vr_nrc_paths(const vr_medium* medium, const vr_camera* camera, const vr_launch* launch, int32_t n_paths,
+                 int32_t max_vertices, float* scratch, float* records, int64_t max_records, uint32_t* count,
+                 void* stream);
+
+/* Capture of the NEE radiance (map kind VR_MAP_NEE_DT) whose paths stop at their
+ * (terminate_after + 1)-th collision and add W * max(Q, 0) there: out
+ * [sensors][3]; queries: scratch of max_queries * 10 floats, *count (device) the
+ * queries issued. */
+int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_launch* launch, int32_t terminate_after,
+                  const uint16_t* weights, float* out, float* queries, int64_t max_queries, uint32_t* count,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

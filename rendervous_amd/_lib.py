@@ -142,7 +142,22 @@ _SIGNATURES = {
                                             ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                             ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                                             ctypes.c_void_p]),
+    "vr_nrc_encode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, P(ctypes.c_float), P(ctypes.c_float),
+                                     ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_nrc_mlp_fw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_nrc_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, P(ctypes.c_float),
+                                         P(ctypes.c_float),
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_nrc_paths": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_nrc_render": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
 }
+
+NRC_PARAMS = 19456
+NRC_FEATURES = 96
 
 # Symbols declared in include/vr_api.h (checked by the CPU test suite).
 EXPORTED = tuple(_SIGNATURES)

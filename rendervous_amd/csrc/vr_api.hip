@@ -19,6 +19,7 @@
 
 #include "vr_device.hpp"
 #include "vr_nee_wavefront.hpp"
+#include "vr_nrc.hpp"
 
 using namespace vr;
 
@@ -789,6 +790,148 @@ int vr_adam_step_bounded(float* p, float* g, float* m, float* v, int64_t n, floa
 int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps, int32_t step,
                  int32_t zero_grad, void* stream) {
     return vr_adam_step_bounded(p, g, m, v, n, lr, beta1, beta2, eps, step, -INFINITY, INFINITY, zero_grad, stream);
+}
+
+// ------------------------------------------------------------------ cache --
+}  // extern "C"
+
+namespace {
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void k_zero_f32(float* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+int box_transform(const float bmin[3], const float bmax[3], f3& o, f3& ib) {
+    if (!bmin || !bmax) return set_err(VR_ERR_ARG, "nrc: box is NULL");
+    for (int a = 0; a < 3; a++)
+        if (!(bmax[a] > bmin[a])) return set_err(VR_ERR_ARG, "nrc: box must have positive extent");
+    o = F3(bmin[0], bmin[1], bmin[2]);
+    ib = F3(1.0f / (bmax[0] - bmin[0]), 1.0f / (bmax[1] - bmin[1]), 1.0f / (bmax[2] - bmin[2]));
+    return VR_OK;
+}
+
+unsigned nrc_blocks(int64_t rows, int waves_per_block, unsigned cap) {
+    int64_t b = (rows + (int64_t)nrc::kRows * waves_per_block - 1) / ((int64_t)nrc::kRows * waves_per_block);
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
+}
+
+}  // namespace
+
+extern "C" {
+
+int vr_nrc_encode(const float* x, const float* w, int64_t n, const float box_min[3], const float box_max[3],
+                  float* features, void* stream) {
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_nrc_encode: negative n");
+    if (n == 0) return VR_OK;
+    if (!x || !w || !features) return set_err(VR_ERR_ARG, "vr_nrc_encode: null buffer");
+    f3 o, ib;
+    int st = box_transform(box_min, box_max, o, ib);
+    if (st) return st;
+    hipLaunchKernelGGL(nrc::k_nrc_encode, dim3(grid_for(n * nrc::kIn)), dim3(kBlock), 0, (hipStream_t)stream, x, w, n, o, ib,
+                       features);
+    return check_hip(hipGetLastError(), "vr_nrc_encode launch");
+}
+
+int vr_nrc_mlp_fw(const float* features, int64_t n, const uint16_t* weights, float* out, void* stream) {
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: negative n");
+    if (n == 0) return VR_OK;
+    if (!features || !weights || !out) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: null buffer");
+    hipLaunchKernelGGL(nrc::k_nrc_mlp_fw, dim3(nrc_blocks(n, nrc::kInferWaves, 4096)), dim3(64 * nrc::kInferWaves), 0,
+                       (hipStream_t)stream, features, n, (const __bf16*)weights, out);
+    return check_hip(hipGetLastError(), "vr_nrc_mlp_fw launch");
+}
+
+int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first, int64_t max_records, const float box_min[3],
+                      const float box_max[3], float* params, uint16_t* weights, float* grad, float* m, float* v,
+                      int32_t step, float lr, float beta1, float beta2, float eps, float* loss, void* stream) {
+    if (max_records < 0 || first < 0) return set_err(VR_ERR_ARG, "vr_nrc_train_step: negative sizes");
+    if (step < 1) return set_err(VR_ERR_ARG, "vr_nrc_train_step: step must be >= 1");
+    if (!records || !count || !params || !weights || !grad || !m || !v || !loss)
+        return set_err(VR_ERR_ARG, "vr_nrc_train_step: null buffer");
+    f3 o, ib;
+    int st = box_transform(box_min, box_max, o, ib);
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(kBlock), 0, s, grad, (int64_t)nrc::kParams);
+    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(kBlock), 0, s, loss, (int64_t)1);
+    if (max_records > 0)
+        hipLaunchKernelGGL(nrc::k_nrc_train, dim3(nrc_blocks(max_records, nrc::kTrainWaves, 1024)), dim3(64 * nrc::kTrainWaves),
+                           0, s, records, count, first, max_records, o, ib, (const __bf16*)weights, grad, loss);
+    const float bc1 = (float)(1.0 - std::pow((double)beta1, step));
+    const float bc2s = (float)std::sqrt(1.0 - std::pow((double)beta2, step));
+    hipLaunchKernelGGL(nrc::k_nrc_adam, dim3((nrc::kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, s, params, grad, m, v,
+                       (__bf16*)weights, lr, beta1, beta2, eps, bc1, bc2s);
+    return check_hip(hipGetLastError(), "vr_nrc_train_step launch");
+}
+
+int vr_nrc_paths(const vr_medium* medium, const vr_camera* camera, const vr_launch* L, int32_t n_paths,
+                 int32_t max_vertices, float* scratch, float* records, int64_t max_records, uint32_t* count,
+                 void* stream) {
+    if (!L) return set_err(VR_ERR_ARG, "vr_nrc_paths: launch is NULL");
+    if (n_paths < 0 || max_vertices < 1 || max_records < 0) return set_err(VR_ERR_ARG, "vr_nrc_paths: bad sizes");
+    if (!scratch || !records || !count) return set_err(VR_ERR_ARG, "vr_nrc_paths: null buffer");
+    MediumView M;
+    int st = make_medium_view(medium, VR_MAP_NEE_DT, L->bw_mode, M);
+    if (st) return st;
+    CameraView C;
+    if ((st = make_camera_view(camera, C))) return st;
+    const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    if (total <= 0) return set_err(VR_ERR_ARG, "vr_nrc_paths: no sensors");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count, 1);
+    if (n_paths > 0) {
+        const Seeds sd = make_seeds(L);
+        if (medium->majorant_kind == VR_MAJ_GRID)
+            hipLaunchKernelGGL(nrc::k_nrc_paths<VR_MAJ_GRID>, dim3((n_paths + 63) / 64), dim3(64), 0, s, M, C, sd, total, n_paths,
+                               max_vertices, scratch, records, max_records, count);
+        else
+            hipLaunchKernelGGL(nrc::k_nrc_paths<VR_MAJ_CONST>, dim3((n_paths + 63) / 64), dim3(64), 0, s, M, C, sd, total, n_paths,
+                               max_vertices, scratch, records, max_records, count);
+    }
+    return check_hip(hipGetLastError(), "vr_nrc_paths launch");
+}
+
+int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_launch* L, int32_t terminate_after,
+                  const uint16_t* weights, float* out, float* queries, int64_t max_queries, uint32_t* count,
+                  void* stream) {
+    if (!L) return set_err(VR_ERR_ARG, "vr_nrc_render: launch is NULL");
+    if (terminate_after < 0 || max_queries < 0) return set_err(VR_ERR_ARG, "vr_nrc_render: bad sizes");
+    if (!weights || !out || !queries || !count) return set_err(VR_ERR_ARG, "vr_nrc_render: null buffer");
+    MediumView M;
+    int st = make_medium_view(medium, VR_MAP_NEE_DT, L->bw_mode, M);
+    if (st) return st;
+    if (M.t_kind == VR_T_DDA) return set_err(VR_ERR_UNSUPPORTED, "vr_nrc_render: DT or RT shadow rays only");
+    CameraView C;
+    if ((st = make_camera_view(camera, C))) return st;
+    const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    int64_t start, cnt;
+    if ((st = resolve_range(L, total, start, cnt))) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count, 1);
+    if (cnt == 0) return check_hip(hipGetLastError(), "vr_nrc_render launch");
+    const int spp = L->samples < 1 ? 1 : L->samples;
+    const CaptureSplit P = capture_split(spp, false);
+    if ((st = capture_prepare(P, out, 3, start, cnt, nullptr, s))) return st;
+    const Seeds sd = make_seeds(L);
+    const dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
+    if (medium->majorant_kind == VR_MAJ_GRID)
+        hipLaunchKernelGGL(nrc::k_nrc_render<VR_MAJ_GRID>, g, b, 0, s, M, C, sd, spp, P.chunk, P.chunks, (int)terminate_after,
+                           total, start, cnt, out, queries, max_queries, count);
+    else
+        hipLaunchKernelGGL(nrc::k_nrc_render<VR_MAJ_CONST>, g, b, 0, s, M, C, sd, spp, P.chunk, P.chunks, (int)terminate_after,
+                           total, start, cnt, out, queries, max_queries, count);
+    if ((st = check_hip(hipGetLastError(), "vr_nrc_render trace"))) return st;
+    f3 o, ib;
+    if ((st = box_transform(medium->box_min, medium->box_max, o, ib))) return st;
+    if (max_queries > 0)
+        hipLaunchKernelGGL(nrc::k_nrc_apply, dim3(nrc_blocks(max_queries, nrc::kInferWaves, 2048)), dim3(64 * nrc::kInferWaves), 0,
+                           s, queries, count, max_queries, o, ib, (const __bf16*)weights, out);
+    return check_hip(hipGetLastError(), "vr_nrc_render apply");
 }
 
 }  // extern "C"

@@ -1,0 +1,608 @@
+// vr_nrc.hpp -- neural radiance cache: a fully fused 96-64-64-64-64-16 ReLU MLP
+// on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation) for inference and
+// online training, plus the path-tracing kernels that feed it.
+//
+// The cache predicts Q(x, w): the radiance a NEE path collects from a real
+// collision at x (reached along direction w, throughput 1 on arrival) onwards,
+// i.e. the quantity the NEE integrator (include/vr_path_integrator_NEE.h:39-63)
+// adds from that vertex on.  Rendering terminates each path at its
+// (terminate_after + 1)-th collision and adds W * Q(x, w) there; training
+// targets are the suffix radiances of plain NEE paths, one per collision vertex.
+//
+// Layout.  Weights are stored per layer as Wt[n][k] (output neuron n, input k,
+// row-major), in FP32 (master, optimised by Adam) and BF16 (read by the MFMA
+// kernels).  Each wave owns 32 rows (two 16-row MFMA tiles); activations live in
+// LDS as BF16 rows padded by 8 elements (16 bytes) to spread LDS banks.
+//
+// MFMA 16x16x32 BF16 lane layout (checked on the device): lane l holds
+// A[l%16][8*(l/16) + j] and B[8*(l/16) + j][l%16] for j < 8, and returns
+// C[4*(l/16) + j][l%16] for j < 4.
+#pragma once
+
+#include "vr_device.hpp"
+
+namespace vr {
+namespace nrc {
+
+constexpr int kIn = 96;      // 72 Fourier features + direction (3) + constant 1 + zero padding
+constexpr int kH = 64;       // width of the 4 hidden layers
+constexpr int kOut = 16;     // output tile; the first 3 are RGB
+constexpr int kFreqs = 12;   // Fourier frequencies per position coordinate
+constexpr int kRows = 32;    // rows per wave
+constexpr int kPad = 8;      // LDS row padding (elements)
+
+// parameter offsets of Wt0 [64][96], Wt1..Wt3 [64][64], Wt4 [16][64]
+constexpr int kOff0 = 0;
+constexpr int kOff1 = kOff0 + kH * kIn;
+constexpr int kOff2 = kOff1 + kH * kH;
+constexpr int kOff3 = kOff2 + kH * kH;
+constexpr int kOff4 = kOff3 + kH * kH;
+constexpr int kParams = kOff4 + kOut * kH;  // 19456
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+VR_DEV float fract(float t) { return t - floorf(t); }
+
+// Orders a wave's LDS writes before its later LDS reads of other lanes' data
+// (a wave's LDS operations execute in order; this keeps the compiler from
+// moving them across).
+VR_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Features of a point u in the unit cube (the medium box mapped to [0,1]^3) and
+// a direction w: sin and cos of pi * 2^f * u_c for f < 12, then w, then 1.
+VR_DEV float feature(f3 u, f3 w, int i) {
+    if (i < 3 * 2 * kFreqs) {
+        const int c = i / (2 * kFreqs), f = (i / 2) % kFreqs;
+        const float uc = c == 0 ? u.x : (c == 1 ? u.y : u.z);
+        float s, co;
+        sincos_turns(fract(uc * __builtin_ldexpf(1.0f, f - 1)), s, co);  // pi 2^f u = 2 pi (2^(f-1) u)
+        return (i & 1) ? co : s;
+    }
+    if (i == 72) return w.x;
+    if (i == 73) return w.y;
+    if (i == 74) return w.z;
+    if (i == 75) return 1.0f;
+    return 0.0f;
+}
+
+VR_DEV f3 unit_coords(f3 x, f3 bmin, f3 ibsize) {
+    return F3((x.x - bmin.x) * ibsize.x, (x.y - bmin.y) * ibsize.y, (x.z - bmin.z) * ibsize.z);
+}
+
+// One wave's rows of feature vectors into LDS (row stride kIn + kPad).
+VR_DEV void encode_rows(__bf16* __restrict__ a0, const f3* u, const f3* w, int rows_valid) {
+    const int l = __lane_id();
+    for (int idx = l; idx < kRows * kIn; idx += 64) {
+        const int r = idx / kIn, i = idx - r * kIn;
+        const float v = r < rows_valid ? feature(u[r], w[r], i) : 0.0f;
+        a0[r * (kIn + kPad) + i] = (__bf16)v;
+    }
+}
+
+// Z[r][n] = sum_k A[r][k] Wt[n][k] for the wave's 32 rows; `ain` has row stride
+// K + kPad, `wt` row stride K + kPad.  The epilogue gets (row, col, value).
+template <int K, int N, typename Epi>
+VR_DEV void layer_fw(const __bf16* __restrict__ ain, const __bf16* __restrict__ wt, Epi epi) {
+    const int l = __lane_id();
+#pragma unroll
+    for (int mt = 0; mt < kRows / 16; mt++) {
+#pragma unroll
+        for (int nt = 0; nt < N / 16; nt++) {
+            floatx4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < K / 32; ks++) {
+                const bf16x8 a = *(const bf16x8*)(ain + (mt * 16 + l % 16) * (K + kPad) + ks * 32 + (l / 16) * 8);
+                const bf16x8 b = *(const bf16x8*)(wt + (nt * 16 + l % 16) * (K + kPad) + ks * 32 + (l / 16) * 8);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) epi(mt * 16 + (l / 16) * 4 + j, nt * 16 + l % 16, c[j]);
+        }
+    }
+}
+
+// Copies Wt [N][K] (global BF16) into LDS with row stride K + kPad and, if
+// `wT` is given, its transpose [K][N] with row stride N + kPad.  Block-wide.
+template <int K, int N>
+VR_DEV void load_layer(const __bf16* __restrict__ g, __bf16* __restrict__ wt, __bf16* __restrict__ wT) {
+    for (int i = threadIdx.x; i < N * K; i += blockDim.x) {
+        const int n = i / K, k = i - n * K;
+        const __bf16 v = g[i];
+        wt[n * (K + kPad) + k] = v;
+        if (wT) wT[k * (N + kPad) + n] = v;
+    }
+}
+
+// LDS footprint of the weights, one layout (elements).
+constexpr int kWLds = kH * (kIn + kPad) + 3 * kH * (kH + kPad) + kOut * (kH + kPad);
+// LDS offsets of the layers in the [N][K+pad] layout
+constexpr int kL0 = 0;
+constexpr int kL1 = kL0 + kH * (kIn + kPad);
+constexpr int kL2 = kL1 + kH * (kH + kPad);
+constexpr int kL3 = kL2 + kH * (kH + kPad);
+constexpr int kL4 = kL3 + kH * (kH + kPad);
+// LDS offsets in the transposed [K][N+pad] layout (Wt0 is never needed transposed)
+constexpr int kT1 = 0;
+constexpr int kT2 = kT1 + kH * (kH + kPad);
+constexpr int kT3 = kT2 + kH * (kH + kPad);
+constexpr int kT4 = kT3 + kH * (kH + kPad);
+constexpr int kWTLds = kT4 + kH * (kOut + kPad);
+
+VR_DEV void load_weights(const __bf16* __restrict__ g, __bf16* __restrict__ w, __bf16* __restrict__ wT) {
+    load_layer<kIn, kH>(g + kOff0, w + kL0, nullptr);
+    load_layer<kH, kH>(g + kOff1, w + kL1, wT ? wT + kT1 : nullptr);
+    load_layer<kH, kH>(g + kOff2, w + kL2, wT ? wT + kT2 : nullptr);
+    load_layer<kH, kH>(g + kOff3, w + kL3, wT ? wT + kT3 : nullptr);
+    load_layer<kH, kOut>(g + kOff4, w + kL4, wT ? wT + kT4 : nullptr);
+}
+
+// Per-wave activation scratch (elements): A0 [32][96+8] and A1..A4 [32][64+8].
+constexpr int kA0 = kRows * (kIn + kPad);
+constexpr int kAh = kRows * (kH + kPad);
+
+// Forward of the wave's rows from A0 (already encoded) through the hidden
+// layers into A1..A4 (ReLU, BF16), then the output layer into y[r][0..2]
+// (FP32, the lane holding column c < 3 of row r writes it).
+VR_DEV void mlp_forward(const __bf16* __restrict__ w, __bf16* __restrict__ acts, float* __restrict__ y) {
+    __bf16* a0 = acts;
+    __bf16* a1 = a0 + kA0;
+    __bf16* a2 = a1 + kAh;
+    __bf16* a3 = a2 + kAh;
+    __bf16* a4 = a3 + kAh;
+    auto relu_to = [](__bf16* dst) {
+        return [dst](int r, int c, float v) { dst[r * (kH + kPad) + c] = (__bf16)fmaxf(v, 0.0f); };
+    };
+    layer_fw<kIn, kH>(a0, w + kL0, relu_to(a1));
+    wave_sync();
+    layer_fw<kH, kH>(a1, w + kL1, relu_to(a2));
+    wave_sync();
+    layer_fw<kH, kH>(a2, w + kL2, relu_to(a3));
+    wave_sync();
+    layer_fw<kH, kH>(a3, w + kL3, relu_to(a4));
+    wave_sync();
+    layer_fw<kH, kOut>(a4, w + kL4, [y](int r, int c, float v) {
+        if (c < 3) y[r * 3 + c] = v;
+    });
+    wave_sync();
+}
+
+
+// dW[n][k] += sum over the wave's 32 rows of dz[r][n] * a[r][k] (global FP32 atomics).
+template <int N, int K>
+VR_DEV void layer_dw(const __bf16* __restrict__ dz, const __bf16* __restrict__ a, float* __restrict__ gw) {
+    const int l = __lane_id();
+#pragma unroll
+    for (int nt = 0; nt < N / 16; nt++) {
+#pragma unroll
+        for (int kt = 0; kt < K / 16; kt++) {
+            bf16x8 A, B;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int r = (l / 16) * 8 + j;
+                A[j] = dz[r * (kH + kPad) + nt * 16 + l % 16];
+                B[j] = a[r * (K + kPad) + kt * 16 + l % 16];
+            }
+            floatx4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, c, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) atomic_addf(gw + (nt * 16 + (l / 16) * 4 + j) * K + kt * 16 + l % 16, c[j]);
+        }
+    }
+}
+
+// dA[r][k] = sum_n dz[r][n] * Wt[n][k] over NIN inputs n (16 or 64), with the
+// transposed weights wT [K][NIN + kPad]; the epilogue gets (row, k, value).
+template <int NIN, int K, typename Epi>
+VR_DEV void layer_da(const __bf16* __restrict__ dz, const __bf16* __restrict__ wT, Epi epi) {
+    const int l = __lane_id();
+    const bf16x8 zero = {};
+#pragma unroll
+    for (int mt = 0; mt < kRows / 16; mt++) {
+#pragma unroll
+        for (int kt = 0; kt < K / 16; kt++) {
+            floatx4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < (NIN + 31) / 32; ks++) {
+                const int n0 = ks * 32 + (l / 16) * 8;
+                bf16x8 A = zero, B = zero;
+                if (n0 < NIN) {
+                    A = *(const bf16x8*)(dz + (mt * 16 + l % 16) * (kH + kPad) + n0);
+                    B = *(const bf16x8*)(wT + (kt * 16 + l % 16) * (NIN + kPad) + n0);
+                }
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, c, 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) epi(mt * 16 + (l / 16) * 4 + j, kt * 16 + l % 16, c[j]);
+        }
+    }
+}
+
+// LDS of one training block: weights in both layouts, then per wave A0..A4,
+// two gradient buffers and the rows' inputs, outputs and targets.
+struct TrainWave {
+    __bf16 acts[kA0 + 4 * kAh];
+    __bf16 dz[kAh];
+    __bf16 dz2[kAh];
+    float y[kRows * 3];
+    float t[kRows * 3];
+    f3 u[kRows];
+    f3 w[kRows];
+};
+constexpr int kTrainWaves = 2;
+
+// One optimisation step over the training records (x[3], w[3], target[3]) of
+// rows [f, min(count, f + max_records)) with f = first % count (so that
+// consecutive windows cycle through however many records were written):
+// relative L2 loss sum_c (q_c - t_c)^2 / (q_c^2 + 0.01) averaged over rows and
+// channels (q_c without gradient in the denominator), its gradient accumulated
+// into `grad` (FP32, zeroed by the caller) and the loss into *loss.
+__global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __restrict__ records, const uint32_t* __restrict__ count_p,
+                                                                int64_t first, int64_t max_records, f3 bmin, f3 ibsize,
+                                                                const __bf16* __restrict__ wg, float* __restrict__ grad,
+                                                                float* __restrict__ loss) {
+    __shared__ __bf16 w[kWLds];
+    __shared__ __bf16 wT[kWTLds];
+    __shared__ TrainWave waves[kTrainWaves];
+    load_weights(wg, w, wT);
+    __syncthreads();
+    const int64_t written = (int64_t)*count_p;
+    if (written <= 0) return;
+    const int64_t f = first % written;
+    const int64_t count = min(written - f, max_records);
+    records += f * 9;
+    const float inv = 1.0f / (3.0f * (float)count);
+    TrainWave& S = waves[threadIdx.x / 64];
+    const int l = __lane_id();
+    float lsum = 0.0f;
+    const int64_t nw = (int64_t)gridDim.x * kTrainWaves;
+    for (int64_t base = ((int64_t)blockIdx.x * kTrainWaves + threadIdx.x / 64) * kRows; base < count; base += nw * kRows) {
+        const int valid = (int)min((int64_t)kRows, count - base);
+        if (l < kRows) {
+            const float* rec = records + (base + (l < valid ? l : 0)) * 9;
+            S.u[l] = unit_coords(F3(rec[0], rec[1], rec[2]), bmin, ibsize);
+            S.w[l] = F3(rec[3], rec[4], rec[5]);
+            S.t[l * 3] = rec[6]; S.t[l * 3 + 1] = rec[7]; S.t[l * 3 + 2] = rec[8];
+        }
+        wave_sync();
+        encode_rows(S.acts, S.u, S.w, valid);
+        wave_sync();
+        mlp_forward(w, S.acts, S.y);
+        // output gradient (columns >= 3 and rows >= valid are zero)
+        for (int idx = l; idx < kRows * 16; idx += 64) {
+            const int r = idx / 16, c = idx - r * 16;
+            float g = 0.0f;
+            if (c < 3 && r < valid) {
+                const float q = S.y[r * 3 + c], d = q - S.t[r * 3 + c], den = q * q + 0.01f;
+                g = 2.0f * d / den * inv;
+                lsum += d * d / den * inv;
+            }
+            S.dz[r * (kH + kPad) + c] = (__bf16)g;
+        }
+        wave_sync();
+        const __bf16* a0 = S.acts;
+        const __bf16* a1 = a0 + kA0;
+        const __bf16* a2 = a1 + kAh;
+        const __bf16* a3 = a2 + kAh;
+        const __bf16* a4 = a3 + kAh;
+        auto masked_to = [](__bf16* dst, const __bf16* act) {
+            return [dst, act](int r, int k, float v) {
+                dst[r * (kH + kPad) + k] = (__bf16)((float)act[r * (kH + kPad) + k] > 0.0f ? v : 0.0f);
+            };
+        };
+        // output layer (16 x 64): dY -> dW4, dZ3
+        layer_dw<kOut, kH>(S.dz, a4, grad + kOff4);
+        layer_da<kOut, kH>(S.dz, wT + kT4, masked_to(S.dz2, a4));
+        wave_sync();
+        layer_dw<kH, kH>(S.dz2, a3, grad + kOff3);
+        layer_da<kH, kH>(S.dz2, wT + kT3, masked_to(S.dz, a3));
+        wave_sync();
+        layer_dw<kH, kH>(S.dz, a2, grad + kOff2);
+        layer_da<kH, kH>(S.dz, wT + kT2, masked_to(S.dz2, a2));
+        wave_sync();
+        layer_dw<kH, kH>(S.dz2, a1, grad + kOff1);
+        layer_da<kH, kH>(S.dz2, wT + kT1, masked_to(S.dz, a1));
+        wave_sync();
+        layer_dw<kH, kIn>(S.dz, a0, grad + kOff0);
+        wave_sync();
+    }
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+    if (l == 0) atomic_addf(loss, lsum);
+}
+
+// Adam on the FP32 master weights (torch.optim.Adam semantics) and the BF16 copy.
+__global__ void k_nrc_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                           __bf16* __restrict__ wg, float lr, float b1, float b2, float eps, float bc1, float bc2_sqrt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kParams; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float pi = p[i] - (lr / bc1) * mi / (sqrtf(vi) / bc2_sqrt + eps);
+        p[i] = pi;
+        wg[i] = (__bf16)pi;
+    }
+}
+
+// LDS of one inference block: weights (one layout) and per wave A0..A4.
+struct InferWave {
+    __bf16 acts[kA0 + 4 * kAh];
+    float y[kRows * 3];
+    f3 u[kRows];
+    f3 w[kRows];
+};
+constexpr int kInferWaves = 4;
+
+// MLP on explicit feature vectors [n][96] (FP32, rounded to BF16) -> out [n][3].
+__global__ void __launch_bounds__(64 * kInferWaves) k_nrc_mlp_fw(const float* __restrict__ feats, int64_t n,
+                                                                 const __bf16* __restrict__ wg, float* __restrict__ out) {
+    __shared__ __bf16 w[kWLds];
+    __shared__ InferWave waves[kInferWaves];
+    load_weights(wg, w, nullptr);
+    __syncthreads();
+    InferWave& S = waves[threadIdx.x / 64];
+    const int l = __lane_id();
+    const int64_t nw = (int64_t)gridDim.x * kInferWaves;
+    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * kRows; base < n; base += nw * kRows) {
+        const int valid = (int)min((int64_t)kRows, n - base);
+        for (int idx = l; idx < kRows * kIn; idx += 64) {
+            const int r = idx / kIn, i = idx - r * kIn;
+            S.acts[r * (kIn + kPad) + i] = (__bf16)(r < valid ? feats[(base + r) * kIn + i] : 0.0f);
+        }
+        wave_sync();
+        mlp_forward(w, S.acts, S.y);
+        for (int idx = l; idx < valid * 3; idx += 64) out[base * 3 + idx] = S.y[idx];
+        wave_sync();
+    }
+}
+
+// Features of explicit (x, w) pairs [n][3] -> [n][96] (FP32).
+__global__ void k_nrc_encode(const float* __restrict__ x, const float* __restrict__ w, int64_t n, f3 bmin, f3 ibsize,
+                             float* __restrict__ feats) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * kIn) return;
+    const int64_t r = i / kIn;
+    const int c = (int)(i - r * kIn);
+    feats[i] = feature(unit_coords(F3(x[3 * r], x[3 * r + 1], x[3 * r + 2]), bmin, ibsize),
+                       F3(w[3 * r], w[3 * r + 1], w[3 * r + 2]), c);
+}
+
+// Cache queries (x[3], w[3], weight[3], pixel bits) -> out[pixel] += weight * max(Q(x, w), 0).
+__global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __restrict__ queries, const uint32_t* __restrict__ count_p,
+                                                                int64_t max_queries, f3 bmin, f3 ibsize,
+                                                                const __bf16* __restrict__ wg, float* __restrict__ out) {
+    __shared__ __bf16 w[kWLds];
+    __shared__ InferWave waves[kInferWaves];
+    load_weights(wg, w, nullptr);
+    __syncthreads();
+    const int64_t count = min((int64_t)*count_p, max_queries);
+    InferWave& S = waves[threadIdx.x / 64];
+    const int l = __lane_id();
+    const int64_t nw = (int64_t)gridDim.x * kInferWaves;
+    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * kRows; base < count; base += nw * kRows) {
+        const int valid = (int)min((int64_t)kRows, count - base);
+        if (l < kRows) {
+            const float* q = queries + (base + (l < valid ? l : 0)) * 10;
+            S.u[l] = unit_coords(F3(q[0], q[1], q[2]), bmin, ibsize);
+            S.w[l] = F3(q[3], q[4], q[5]);
+        }
+        wave_sync();
+        encode_rows(S.acts, S.u, S.w, valid);
+        wave_sync();
+        mlp_forward(w, S.acts, S.y);
+        if (l < valid) {
+            const float* q = queries + (base + l) * 10;
+            const uint32_t pix = __float_as_uint(q[9]);
+            atomic_addf(out + 3 * (size_t)pix, q[6] * fmaxf(S.y[l * 3], 0.0f));
+            atomic_addf(out + 3 * (size_t)pix + 1, q[7] * fmaxf(S.y[l * 3 + 1], 0.0f));
+            atomic_addf(out + 3 * (size_t)pix + 2, q[8] * fmaxf(S.y[l * 3 + 2], 0.0f));
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- paths --
+// include/vr_path_integrator_NEE.h:1-66 up to the (K+1)-th real collision,
+// where the path stops and hands (x, w, W) to the cache.
+template <int MAJ>
+VR_DEV f3 nee_cached(Tracer<MAJ>& tr, f3 x, f3 w, int K, f3& qx, f3& qw, f3& qW, bool& emit) {
+    const MediumView& M = tr.M;
+    emit = false;
+    float tmin, tmax;
+    boundary(M, x, w, tmin, tmax);
+    x = x + w * tmin;
+    float d = tmax - tmin;
+    f3 R = M.env_c * tr.transmittance(x, w);
+    f3 W = F3(1.0f, 1.0f, 1.0f);
+    const float EPS = 0.00001f;
+    int bounces = 0;
+    Walk<MAJ> mj;
+    mj.begin(M, x, w);
+    while (d > EPS) {
+        float md;
+        float m = fmaxf(EPS, mj.query(M, md));
+        float t = tr.flight(m);
+        x = x + w * fminf(t, md);
+        if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+        if (t > d) break;
+        mj.move(t);
+        float s = tr.sigma(x);
+        if (tr.rnd() * m >= s) { d -= t; continue; }
+        if (bounces == K) {
+            qx = x; qw = w; qW = W;
+            emit = true;
+            return R;
+        }
+        f3 sa = tr.albedo(x);
+        R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+        W = W * sa;
+        f3 wnee;
+        f3 env = env_sample(M, tr.rng, wnee);
+        float Tn = tr.transmittance(x, wnee);
+        R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+        bounces++;
+        f3 wo;
+        float rho = phase_sample(M, tr.rng, w, wo);
+        W = W * rho;
+        if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
+        w = wo;
+        boundary(M, x, w, tmin, d);
+        mj.begin(M, x, w);
+    }
+    return R;
+}
+
+// The NEE path of a training sample, recording (x, w, R, W) at each real
+// collision before its terms are added (at most maxv vertices; the medium's
+// max_bounces caps the path).  Returns the vertex count; R_end is the path's
+// radiance.
+template <int MAJ>
+VR_DEV int nee_record(Tracer<MAJ>& tr, f3 x, f3 w, int maxv, float* __restrict__ tmp, f3& R_end) {
+    const MediumView& M = tr.M;
+    float tmin, tmax;
+    boundary(M, x, w, tmin, tmax);
+    x = x + w * tmin;
+    float d = tmax - tmin;
+    f3 R = M.env_c * tr.transmittance(x, w);
+    f3 W = F3(1.0f, 1.0f, 1.0f);
+    const float EPS = 0.00001f;
+    int bounces = 0, nv = 0;
+    Walk<MAJ> mj;
+    mj.begin(M, x, w);
+    while (d > EPS) {
+        float md;
+        float m = fmaxf(EPS, mj.query(M, md));
+        float t = tr.flight(m);
+        x = x + w * fminf(t, md);
+        if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+        if (t > d) break;
+        mj.move(t);
+        float s = tr.sigma(x);
+        if (tr.rnd() * m >= s) { d -= t; continue; }
+        if (nv < maxv) {
+            float* o = tmp + nv * 12;
+            o[0] = x.x; o[1] = x.y; o[2] = x.z; o[3] = w.x; o[4] = w.y; o[5] = w.z;
+            o[6] = R.x; o[7] = R.y; o[8] = R.z; o[9] = W.x; o[10] = W.y; o[11] = W.z;
+            nv++;
+        }
+        f3 sa = tr.albedo(x);
+        R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+        W = W * sa;
+        f3 wnee;
+        f3 env = env_sample(M, tr.rng, wnee);
+        float Tn = tr.transmittance(x, wnee);
+        R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+        bounces++;
+        if (M.max_bounces >= 0 && bounces >= M.max_bounces) break;
+        f3 wo;
+        float rho = phase_sample(M, tr.rng, w, wo);
+        W = W * rho;
+        if (W.x < EPS && W.y < EPS && W.z < EPS) break;
+        w = wo;
+        boundary(M, x, w, tmin, d);
+        mj.begin(M, x, w);
+    }
+    R_end = R;
+    return nv;
+}
+
+VR_DEV unsigned long long lanes_below() {
+    const unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Training paths: path j starts at a uniformly drawn sensor; each recorded
+// vertex becomes a record (x, w, (R_end - R_v) / W_v) appended at *count.
+// Few, long paths: one wave per block so that every CU gets work, and the
+// registers capped at 5 waves per SIMD as for the other nested estimators.
+template <int MAJ>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_nrc_paths(MediumView M, CameraView cam, Seeds seeds, int64_t total, int n_paths,
+                                                   int maxv, float* __restrict__ tmp, float* __restrict__ records,
+                                                   int64_t max_records, uint32_t* __restrict__ count) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_paths) return;
+    Tracer<MAJ> tr(M);
+    tr.rng = seed_map(seeds, (uint32_t)j);
+    const int64_t i = min(total - 1, (int64_t)(tr.rnd() * (float)total));
+    int pose, row, col;
+    sensor_index(cam, i, pose, row, col);
+    if (pose < 0 || pose >= cam.num_poses) return;
+    f3 x, w, R_end;
+    camera_ray(cam, pose, row, col, tr.rng, x, w);
+    float* t = tmp + (size_t)j * maxv * 12;
+    const int nv = nee_record(tr, x, w, maxv, t, R_end);
+    const uint32_t base = atomicAdd(count, (uint32_t)nv);
+    for (int v = 0; v < nv && (int64_t)base + v < max_records; v++) {
+        const float* o = t + v * 12;
+        float* r = records + ((size_t)base + v) * 9;
+        r[0] = o[0]; r[1] = o[1]; r[2] = o[2]; r[3] = o[3]; r[4] = o[4]; r[5] = o[5];
+        r[6] = o[9] > 0.0f ? (R_end.x - o[6]) / o[9] : 0.0f;
+        r[7] = o[10] > 0.0f ? (R_end.y - o[7]) / o[10] : 0.0f;
+        r[8] = o[11] > 0.0f ? (R_end.z - o[8]) / o[11] : 0.0f;
+    }
+}
+
+// Capture with cache termination: sensor start + j % cnt, samples [c*chunk, ...)
+// as in k_capture_fw; the partial radiance goes to `out` (rows zeroed first when
+// chunks > 1) and each terminated sample appends a query (x, w, W/spp, pixel).
+template <int MAJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_nrc_render(MediumView M, CameraView cam, Seeds seeds, int spp, int chunk, int chunks,
+                                                    int K, int64_t total, int64_t start, int64_t cnt,
+                                                    float* __restrict__ out, float* __restrict__ queries,
+                                                    int64_t max_queries, uint32_t* __restrict__ count) {
+    const int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = jj < cnt * chunks;
+    const int64_t c = live ? jj / cnt : 0;
+    const int64_t i = start + (live ? jj - c * cnt : 0);
+    int pose = 0, row = 0, col = 0;
+    if (live) sensor_index(cam, i, pose, row, col);
+    const bool ok = live && pose >= 0 && pose < cam.num_poses;
+    const int s0 = (int)c * chunk, s1 = ok ? min(spp, s0 + chunk) : s0;
+    const float inv = 1.0f / (float)spp;
+    f3 acc = F3(0.0f, 0.0f, 0.0f);
+    Tracer<MAJ> tr(M);
+    // every lane runs `chunk` iterations so that the query slots are taken by
+    // wave-wide ballots at the same point of the loop
+    for (int s = s0; s < s0 + chunk; s++) {
+        bool emit = false;
+        f3 qx, qw, qW;
+        if (s < s1) {
+            tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
+            f3 x, w;
+            camera_ray(cam, pose, row, col, tr.rng, x, w);
+            acc = acc + nee_cached(tr, x, w, K, qx, qw, qW, emit);
+        }
+        const unsigned long long mask = __ballot(emit);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader);
+            if (emit) {
+                const uint64_t slot = base + (uint64_t)__popcll(mask & lanes_below());
+                if ((int64_t)slot < max_queries) {
+                    float* q = queries + slot * 10;
+                    q[0] = qx.x; q[1] = qx.y; q[2] = qx.z; q[3] = qw.x; q[4] = qw.y; q[5] = qw.z;
+                    q[6] = qW.x * inv; q[7] = qW.y * inv; q[8] = qW.z * inv;
+                    q[9] = __uint_as_float((uint32_t)i);
+                }
+            }
+        }
+    }
+    if (!ok) return;
+    if (chunks == 1) {
+        out[3 * i] = acc.x * inv; out[3 * i + 1] = acc.y * inv; out[3 * i + 2] = acc.z * inv;
+    } else {
+        atomic_addf(out + 3 * i, acc.x * inv);
+        atomic_addf(out + 3 * i + 1, acc.y * inv);
+        atomic_addf(out + 3 * i + 2, acc.z * inv);
+    }
+}
+
+}  // namespace nrc
+}  // namespace vr

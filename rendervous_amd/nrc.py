@@ -1,0 +1,126 @@
+"""Neural radiance cache for the NEE volume path tracer.
+
+An MLP Q(x, w) -> RGB (96 Fourier/direction features, 4 hidden ReLU layers of
+64, BF16 weights and activations with FP32 accumulation on MFMA) learns, online
+from the path tracer's own samples, the radiance a NEE path collects from a real
+collision at x reached along w.  Rendering stops each path at its
+(terminate_after + 1)-th collision and adds the path throughput times Q there;
+the plain NEE integrator (DeltatrackingNEEPathIntegrator) stays the reference
+whose error the cached render is measured against.  No reference counterpart:
+rendervous has no radiance cache.  Kernels: vr_nrc_* in include/vr_api.h.
+"""
+from __future__ import annotations
+
+import math
+import typing
+
+import torch
+
+from . import _internal
+from . import _lib as L
+from . import maps as M
+
+# layer shapes (outputs, inputs) in parameter order; the last layer has 16
+# outputs of which the first 3 are RGB
+LAYERS = ((64, 96), (64, 64), (64, 64), (64, 64), (16, 64))
+
+
+def _f3p(v) -> "ctypes.Array":
+    return L.f3(v)
+
+
+class NeuralRadianceCache:
+    """Weights, Adam state and the training / query buffers of one cache."""
+
+    def __init__(self, device: typing.Optional[torch.device] = None, *, seed: int = 0, lr: float = 1e-2,
+                 betas=(0.9, 0.99), eps: float = 1e-8, terminate_after: int = 2):
+        device = torch.device("cuda", 0) if device is None else torch.device(device)
+        g = torch.Generator().manual_seed(seed)
+        chunks = []
+        for i, (n, k) in enumerate(LAYERS):
+            bound = math.sqrt(6.0 / k) * (0.1 if i == len(LAYERS) - 1 else 1.0)  # He-uniform for ReLU
+            w = (torch.rand(n, k, generator=g) * 2.0 - 1.0) * bound
+            if i == len(LAYERS) - 1:
+                w[3:] = 0.0
+            chunks.append(w.reshape(-1))
+        self.device = device
+        self.params = torch.cat(chunks).to(device)
+        assert self.params.numel() == L.NRC_PARAMS
+        self.weights = self.params.to(torch.bfloat16)  # round to nearest even, as the kernels
+        self.grad = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.loss = torch.zeros(1, device=device)
+        self.lr, self.betas, self.eps = lr, betas, eps
+        self.terminate_after = int(terminate_after)
+        self.steps = 0
+        self._records = None
+        self._scratch = None
+        self._count = torch.zeros(1, dtype=torch.int32, device=device)
+        self._queries = None
+        self._qcount = torch.zeros(1, dtype=torch.int32, device=device)
+
+    # --------------------------------------------------------------- helpers
+    @staticmethod
+    def _lower(field: M.MapBase):
+        kind, lw = field._lower_map()
+        if kind != L.VR_MAP_NEE_DT:
+            raise NotImplementedError("the radiance cache serves the NEE path integrator (DeltatrackingNEEPathIntegrator)")
+        return lw
+
+    def _stream(self):
+        return L.stream_handle(self.device)
+
+    # -------------------------------------------------------------- training
+    def train(self, field: M.MapBase, camera: M.PerspectiveCameraSensor, n_paths: int = 8192, max_vertices: int = 16,
+              steps: int = 1, batch: typing.Optional[int] = None) -> torch.Tensor:
+        """Traces n_paths plain NEE paths from random sensors and runs `steps`
+        Adam steps on their collision vertices, each on the next `batch` records
+        (all records when batch is None); returns the last mean loss (device)."""
+        lw = self._lower(field)
+        cam, keep = camera._camera(None)
+        cap = n_paths * max_vertices
+        if self._records is None or self._records.shape[0] < cap:
+            self._records = torch.empty(cap, 9, device=self.device)
+            self._scratch = torch.empty(cap * 12, device=self.device)
+        launch = L.launch(_internal.get_seeds(), bw_mode=L.VR_BW_CONSISTENT)
+        lib = L.load()
+        L.check(lib.vr_nrc_paths(lw.medium, cam, launch, int(n_paths), int(max_vertices), self._scratch.data_ptr(),
+                                 self._records.data_ptr(), int(cap), self._count.data_ptr(), self._stream()),
+                "vr_nrc_paths")
+        bmin, bmax = _f3p(lw.medium.box_min), _f3p(lw.medium.box_max)
+        size = cap if batch is None else int(batch)
+        for k in range(steps):
+            self.steps += 1
+            first = 0 if batch is None else k * size  # wrapped at the written count by the kernel
+            L.check(lib.vr_nrc_train_step(self._records.data_ptr(), self._count.data_ptr(), int(first), int(size), bmin, bmax,
+                                          self.params.data_ptr(), self.weights.data_ptr(), self.grad.data_ptr(),
+                                          self.m.data_ptr(), self.v.data_ptr(), self.steps, self.lr, self.betas[0],
+                                          self.betas[1], self.eps, self.loss.data_ptr(), self._stream()),
+                    "vr_nrc_train_step")
+        return self.loss
+
+    def samples_last(self) -> int:
+        """Training records written by the last train() (host sync)."""
+        return int(self._count.item())
+
+    # ------------------------------------------------------------- rendering
+    def render(self, field: M.MapBase, camera: M.PerspectiveCameraSensor, spp: int = 1) -> torch.Tensor:
+        """Capture of the NEE radiance with paths ended into the cache."""
+        lw = self._lower(field)
+        cam, keep = camera._camera(None)
+        total = camera.num_sensors()
+        need = total * max(1, spp)
+        if self._queries is None or self._queries.shape[0] < need:
+            self._queries = torch.empty(need, 10, device=self.device)
+        out = torch.empty(*camera.index_shape, 3, device=self.device)
+        launch = L.launch(_internal.get_seeds(), samples=spp, bw_mode=L.VR_BW_CONSISTENT)
+        with M._ENGINE_LOCK:
+            L.check(L.load().vr_nrc_render(lw.medium, cam, launch, self.terminate_after, self.weights.data_ptr(),
+                                           out.data_ptr(), self._queries.data_ptr(), int(need), self._qcount.data_ptr(),
+                                           self._stream()), "vr_nrc_render")
+        return out
+
+    def queries_last(self) -> int:
+        """Cache queries issued by the last render() (host sync)."""
+        return int(self._qcount.item())

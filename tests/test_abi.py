@@ -28,8 +28,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_oracle_exports_the_same_interface():
+    """Every entry point of the path has an oracle twin; the radiance-cache MLP
+    (vr_nrc_*) has no reference counterpart and is checked against PyTorch FP32."""
     lib = O.load()
     for n in _declared("vr_api.h", "vr_"):
+        if n.startswith("vr_nrc_"):
+            continue
         assert hasattr(lib, "oracle_" + n[3:]), n
 
 

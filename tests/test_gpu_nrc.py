@@ -1,0 +1,212 @@
+"""The neural radiance cache kernels (BF16 MFMA MLP, training, cached render)
+against plain PyTorch FP32 references of the same operations, and the cached
+render against the oracle's truncated NEE integrator."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import rendervous_amd as rv
+from rendervous_amd import _lib as L
+from rendervous_amd.nrc import LAYERS, NeuralRadianceCache
+from helpers import Built, Camera, Scene, fbm_grid, match_fraction, oracle_capture_fw, pose
+
+pytestmark = pytest.mark.gpu
+
+BOX = ((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+
+
+def _encode_ref(x, w, bmin, bmax):
+    # u in FP32 as the kernels form it (x - bmin) * (1 / extent); the phases pi 2^f u in FP64
+    ext = torch.tensor(bmax, dtype=torch.float32) - torch.tensor(bmin, dtype=torch.float32)
+    u = ((x.float() - torch.tensor(bmin, dtype=torch.float32)) * (1.0 / ext)).double()
+    cols = []
+    for c in range(3):
+        for f in range(12):
+            a = math.pi * (2.0 ** f) * u[:, c]
+            cols += [torch.sin(a), torch.cos(a)]
+    n = x.shape[0]
+    feats = torch.stack(cols, 1)
+    return torch.cat([feats, w.double(), torch.ones(n, 1, dtype=torch.float64), torch.zeros(n, 20, dtype=torch.float64)], 1)
+
+
+def _split(flat):
+    out, o = [], 0
+    for n, k in LAYERS:
+        out.append(flat[o:o + n * k].reshape(n, k))
+        o += n * k
+    return out
+
+
+def _mlp_ref(weights_bf16, feats, round_acts=True):
+    """PyTorch FP32 forward with the kernel's BF16 weights; round_acts rounds the
+    input and each hidden activation to BF16 as the kernels store them."""
+    Ws = [w.float() for w in _split(weights_bf16.cpu())]
+    a = feats.float()
+    if round_acts:
+        a = a.bfloat16().float()
+    for W in Ws[:-1]:
+        a = torch.relu(a @ W.t())
+        if round_acts:
+            a = a.bfloat16().float()
+    return (a @ Ws[-1].t())[:, :3]
+
+
+def _random_weights(seed, scale=1.0):
+    c = NeuralRadianceCache(torch.device("cuda", 0), seed=seed)
+    if scale != 1.0:
+        c.params.mul_(scale)
+        c.weights = c.params.to(torch.bfloat16)
+    return c
+
+
+def _box_ptrs():
+    return L.f3(BOX[0]), L.f3(BOX[1])
+
+
+def test_encoding_matches_reference(gpu):
+    rng = np.random.default_rng(1)
+    x = torch.from_numpy(rng.uniform(-0.5, 0.5, size=(500, 3)).astype(np.float32))
+    w = torch.nn.functional.normalize(torch.from_numpy(rng.normal(size=(500, 3)).astype(np.float32)), dim=1)
+    out = torch.zeros(500, L.NRC_FEATURES, device="cuda")
+    xd, wd = x.cuda(), w.cuda()  # kept alive until the kernel has run
+    L.check(L.load().vr_nrc_encode(xd.data_ptr(), wd.data_ptr(), 500, *_box_ptrs(), out.data_ptr(), None),
+            "vr_nrc_encode")
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(out.cpu().double().numpy(), _encode_ref(x, w, *BOX).numpy(), atol=2e-5)
+
+
+@pytest.mark.parametrize("n", [1, 31, 32, 1000])
+def test_mlp_forward_matches_torch(gpu, n):
+    cache = _random_weights(2)
+    feats = torch.from_numpy(np.random.default_rng(n).uniform(-1, 1, size=(n, 96)).astype(np.float32))
+    feats[:, 76:] = 0.0
+    out = torch.zeros(n, 3, device="cuda")
+    xf = feats.cuda()
+    L.check(L.load().vr_nrc_mlp_fw(xf.data_ptr(), n, cache.weights.data_ptr(), out.data_ptr(), None), "vr_nrc_mlp_fw")
+    torch.cuda.synchronize()
+    got = out.cpu()
+    ref = _mlp_ref(cache.weights, feats)
+    scale = float(ref.abs().max()) + 1e-6
+    assert float((got - ref).abs().max()) <= 1e-2 * scale, (got[:3], ref[:3])
+    # against the unrounded FP32 network: the BF16 activations cost ~1 %
+    ref32 = _mlp_ref(cache.weights, feats, round_acts=False)
+    assert float((got - ref32).norm() / ref32.norm()) <= 0.05
+
+
+def _records(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.5, 0.5, size=(n, 3))
+    w = rng.normal(size=(n, 3))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    t = np.stack([0.5 + 0.4 * np.sin(3 * x[:, 0]), 0.3 + 0.2 * x[:, 1] * w[:, 2], 0.2 * np.ones(n)], 1)
+    return torch.from_numpy(np.concatenate([x, w, t], 1).astype(np.float32))
+
+
+def test_training_gradient_matches_torch_autograd(gpu):
+    """lr = 0: the kernel's gradient of the relative L2 loss equals torch's autograd
+    of the same BF16-rounded network within the BF16 rounding of dZ."""
+    cache = _random_weights(3)
+    rec = _records(1000, 4)
+    recd = rec.cuda()
+    count = torch.tensor([1000], dtype=torch.int32, device="cuda")
+    lib = L.load()
+    L.check(lib.vr_nrc_train_step(recd.data_ptr(), count.data_ptr(), 0, 1024, *_box_ptrs(), cache.params.data_ptr(),
+                                  cache.weights.data_ptr(), cache.grad.data_ptr(), cache.m.data_ptr(), cache.v.data_ptr(),
+                                  1, 0.0, 0.9, 0.99, 1e-8, cache.loss.data_ptr(), None), "vr_nrc_train_step")
+    torch.cuda.synchronize()
+    Ws = [W.float().clone().requires_grad_(True) for W in _split(cache.weights.cpu())]
+    a = _encode_ref(rec[:, :3], rec[:, 3:6], *BOX).float().bfloat16().float()
+    for W in Ws[:-1]:
+        a = torch.relu(a @ W.t()).bfloat16().float()
+    q = (a @ Ws[-1].t())[:, :3]
+    loss = ((q - rec[:, 6:]) ** 2 / (q.detach() ** 2 + 0.01)).mean()
+    loss.backward()
+    assert abs(float(cache.loss.item()) - float(loss)) <= 2e-2 * float(loss)
+    got = _split(cache.grad.cpu())
+    for i, (W, g) in enumerate(zip(Ws, got)):
+        ref = W.grad
+        err = float((g - ref).norm() / (ref.norm() + 1e-12))
+        assert err <= 0.05, (i, err)
+
+
+def test_adam_step_matches_torch_optim(gpu):
+    cache = _random_weights(5)
+    rec = _records(256, 6).cuda()
+    count = torch.tensor([256], dtype=torch.int32, device="cuda")
+    lib = L.load()
+    p0 = cache.params.clone()
+    L.check(lib.vr_nrc_train_step(rec.data_ptr(), count.data_ptr(), 0, 256, *_box_ptrs(), cache.params.data_ptr(),
+                                  cache.weights.data_ptr(), cache.grad.data_ptr(), cache.m.data_ptr(), cache.v.data_ptr(),
+                                  1, 1e-2, 0.9, 0.99, 1e-8, cache.loss.data_ptr(), None), "vr_nrc_train_step")
+    torch.cuda.synchronize()
+    p = torch.nn.Parameter(p0.cpu())
+    opt = torch.optim.Adam([p], lr=1e-2, betas=(0.9, 0.99), eps=1e-8)
+    p.grad = cache.grad.cpu()
+    opt.step()
+    np.testing.assert_allclose(cache.params.cpu().numpy(), p.detach().numpy(), rtol=1e-5, atol=1e-6)
+    assert torch.equal(cache.weights.cpu(), cache.params.cpu().bfloat16())
+
+
+@pytest.mark.parametrize("tkind", [L.VR_T_DT, L.VR_T_RT])
+def test_zero_cache_render_is_the_truncated_nee_path(gpu, tkind):
+    """With Q = 0 the cached render adds nothing at the (K+1)-th collision, so it
+    equals the NEE integrator capped at K scatter events, sample by sample."""
+    K = 2
+    sc = Scene(sigma=fbm_grid(24, seed=9, scale=20.0), albedo_c=(0.9, 0.8, 0.7), phase_kind=L.VR_PHASE_HG,
+               phase_g=0.6, env_c=(0.5, 0.6, 0.7), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(3.0, 3.0, 3.0), sun_prob=0.5,
+               majorant="grid", transmittance_kind=tkind, max_bounces=K)
+    cam_args = dict(poses=pose((0.0, 0.3, 2.5))[None], width=40, height=40, jittered=True)
+    seeds = (12345, 67890, 13579, 24680)
+    ref = oracle_capture_fw(sc, L.VR_MAP_NEE_DT, cam_args, seeds, 2)
+    b = Built(sc, "cuda")
+    cam = Camera(device="cuda", **cam_args)
+    out = torch.zeros(1600, 3, device="cuda")
+    q = torch.zeros(3200, 10, device="cuda")
+    qc = torch.zeros(1, dtype=torch.int32, device="cuda")
+    wz = torch.zeros(L.NRC_PARAMS, dtype=torch.bfloat16, device="cuda")
+    lau = L.launch(seeds, samples=2)
+    L.check(L.load().vr_nrc_render(b.medium, cam.c, lau, K, wz.data_ptr(), out.data_ptr(), q.data_ptr(), 3200,
+                                   qc.data_ptr(), None), "vr_nrc_render")
+    torch.cuda.synchronize()
+    assert 0 < int(qc.item()) <= 3200
+    assert match_fraction(out.cpu().numpy(), ref, rtol=1e-3, atol=1e-4) >= 0.98
+
+
+def test_cache_learns_and_the_cached_render_approaches_the_plain_one(gpu):
+    """Online training on the scene's own NEE paths lowers the loss, and at equal
+    spp the cached render (paths ended after 2 bounces) is at least as close to
+    a high-spp plain render as the plain render itself."""
+    dev = torch.device("cuda", 0)
+    sigma = rv.functions.fbm_value_noise(48, seed=3, scale=20.0, device=dev)
+    ds = rv.objects.DependencySet()
+    ds.add_parameters(sigma_tensor=sigma, scattering_albedo_tensor=torch.tensor([0.9, 0.9, 0.9]),
+                      environment_tensor=torch.tensor([0.3, 0.3, 0.3]), phase_g_tensor=torch.tensor([0.5]))
+    ds.requires(rv.objects.medium_majorant_coarse_grid, resolution=8)
+    ds.requires(rv.objects.medium_environment_sampler_sun, direction=(0.4, 0.8, 0.45), radiance=(2.0, 2.0, 2.0),
+                probability=0.5)
+    ds.requires(rv.objects.medium_phase_HG, mixture=False)
+    ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
+    ds.requires(rv.objects.medium_transmittance_DT)
+    ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DT, max_bounces=16)
+    field = ds.radiance
+    camera = rv.PerspectiveCameraSensor(64, 64, rv.functions.look_at_pose((0.0, 0.3, 2.5))[None].to(dev), jittered=True)
+    cache = NeuralRadianceCache(dev, seed=7, lr=1e-2)
+    losses = []
+    for it in range(300):
+        rv.seed(1000 + it)
+        losses.append(float(cache.train(field, camera, n_paths=4096, max_vertices=16).item()))
+    print(f"[nrc] loss {losses[0]:.4f} -> {np.mean(losses[-20:]):.4f}", flush=True)
+    assert np.mean(losses[-20:]) < 0.5 * losses[0]
+    with torch.no_grad():
+        rv.seed(1)
+        ref = camera.capture(field, fw_samples=1024)
+        rv.seed(2)
+        plain = camera.capture(field, fw_samples=16)
+        rv.seed(3)
+        cached = cache.render(field, camera, spp=16)
+    e_plain = float(((plain - ref) ** 2).mean().sqrt())
+    e_cached = float(((cached - ref) ** 2).mean().sqrt())
+    print(f"[nrc] rmse plain {e_plain:.5f} cached {e_cached:.5f} (mean {float(ref.mean()):.4f})", flush=True)
+    assert e_cached <= 1.2 * e_plain
