@@ -60,7 +60,9 @@ enum vr_majorant_kind {
 enum vr_transmittance_kind { /* estimator behind transmittance() in NEE / single scatter */
     VR_T_DT = 0,             /* include/vr_transmittance_DT.h */
     VR_T_RT = 1,             /* include/vr_transmittance_RT.h */
-    VR_T_DDA = 2             /* include/vr_transmittance_DDA.h */
+    VR_T_DDA = 2,            /* include/vr_transmittance_DDA.h */
+    VR_T_GDT = 3,            /* include/vr_transmittance_GDT.h (no backward in the reference) */
+    VR_T_GRT = 4             /* include/vr_transmittance_GRT.h (no backward in the reference) */
 };
 
 enum vr_map_kind {           /* input -> output dims */
@@ -76,8 +78,10 @@ enum vr_map_kind {           /* input -> output dims */
     VR_MAP_MEDIUM_DT = 9,    /* 7 -> 12  DTMediumPathIntegrator      _scenes.py:1614-1632 */
     VR_MAP_PATH_DT = 10,     /* 6 -> 3   DeltatrackingPathIntegrator (no NEE; defensive replay bw)
                                          _maps.py:2886-2921, maps/radiance_DT.h */
-    VR_MAP_NEE_DRT = 11      /* 6 -> 3   DRTPathIntegrator: NEE forward, reservoir backward
+    VR_MAP_NEE_DRT = 11,     /* 6 -> 3   DRTPathIntegrator: NEE forward, reservoir backward
                                          _maps.py:2981-3022, maps/radiance_NEE_DRT.h */
+    VR_MAP_T_GDT = 12,       /* 6 -> 1   GridDeltatrackingTransmittance _maps.py:2619-2637 */
+    VR_MAP_T_GRT = 13        /* 6 -> 1   GridRatiotrackingTransmittance _maps.py:2588-2616 */
 };
 
 enum vr_seed_mode {          /* per-thread RNG state derivation from the 4 dispatch seeds */

@@ -580,6 +580,48 @@ static double dda_tau(ctx_t* c, const dda_t* s) {
     return tau;
 }
 
+/* include/vr_transmittance_GDT.h:1-84 / _GRT.h:1-83: over the DDA's cells, the
+ * cell's minorant (min of its 8 vertices) is integrated in closed form and the
+ * residual sigma - minorant tracked with the majorant max - min (delta tracking
+ * ends at a collision; ratio tracking multiplies by 1 - Pc). */
+static double t_grid_tracking(ctx_t* c, v3 x, v3 w, int ratio) {
+    dda_t s;
+    if (!dda_begin(c->M, x, w, &s)) return 1.0;
+    const vr_grid* g = &c->M->sigma;
+    double alpha[3] = {s.alpha0[0], s.alpha0[1], s.alpha0[2]};
+    int cell[3] = {s.cell[0], s.cell[1], s.cell[2]};
+    double cur = 0, T = 1.0;
+    while (cur < s.d - 0.0001) {
+        double next = dmin(s.d, dmin(alpha[0], dmin(alpha[1], alpha[2])));
+        double sg[8], mx = -INFINITY, mn = INFINITY;
+        for (int k = 0; k < 8; k++) {
+            sg[k] = dda_sigma_at(g, cell[0] + (k & 1), cell[1] + ((k >> 1) & 1), cell[2] + ((k >> 2) & 1));
+            mx = dmax(mx, sg[k]);
+            mn = dmin(mn, sg[k]);
+        }
+        double ct = cur;
+        T *= exp(-mn * (next - cur));
+        double r = dmax(0.000001, mx - mn);
+        for (;;) {
+            double dt = -log(1.0 - rnd(c)) / r;
+            if (ct + dt > next) break;
+            ct += dt;
+            double ax = s.vm[0] * ct + s.vn[0] - cell[0];
+            double ay = s.vm[1] * ct + s.vn[1] - cell[1];
+            double az = s.vm[2] * ct + s.vn[2] - cell[2];
+            double v = lerp(lerp(lerp(sg[0], sg[1], ax), lerp(sg[2], sg[3], ax), ay),
+                            lerp(lerp(sg[4], sg[5], ax), lerp(sg[6], sg[7], ax), ay), az);
+            c->lookups++;
+            double Pc = (v - mn) / r;
+            if (!ratio) { if (rnd(c) < Pc) return 0.0; }
+            else T *= 1 - Pc;
+        }
+        dda_step(alpha, cell, &s);
+        cur = next;
+    }
+    return T;
+}
+
 static double t_dda(ctx_t* c, v3 x, v3 w) {
     dda_t s;
     if (!dda_begin(c->M, x, w, &s)) return 1.0;
@@ -721,6 +763,8 @@ static double transmittance(ctx_t* c, v3 x, v3 w) {
     switch (c->M->transmittance_kind) {
     case VR_T_RT: return t_rt(c, x, w);
     case VR_T_DDA: return t_dda(c, x, w);
+    case VR_T_GDT: return t_grid_tracking(c, x, w, 0);
+    case VR_T_GRT: return t_grid_tracking(c, x, w, 1);
     default: return t_dt(c, x, w);
     }
 }
@@ -729,6 +773,7 @@ static void transmittance_bw(ctx_t* c, v3 x, v3 w, double dL_dT) {
     switch (c->M->transmittance_kind) {
     case VR_T_RT: t_rt_bw(c, x, w, dL_dT); break;
     case VR_T_DDA: t_dda_bw(c, x, w, dL_dT); break;
+    case VR_T_GDT: case VR_T_GRT: break;  /* the reference has no backward for them */
     default: t_dt_bw(c, x, w, dL_dT); break;
     }
 }
@@ -1529,7 +1574,8 @@ static void medium_dt_bw(ctx_t* c, v3 x, v3 w, double d, v3 dL_dW, v3 dL_dA) {
 static int map_dims(int kind, int* in_dim, int* out_dim) {
     switch (kind) {
     case VR_MAP_RAYBOX: *in_dim = 6; *out_dim = 2; return 1;
-    case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: *in_dim = 6; *out_dim = 1; return 1;
+    case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: case VR_MAP_T_GDT: case VR_MAP_T_GRT:
+        *in_dim = 6; *out_dim = 1; return 1;
     case VR_MAP_COLLISION_DT: case VR_MAP_SS_RADIANCE: case VR_MAP_MC_COLLISION: case VR_MAP_NEE_DT:
     case VR_MAP_PATH_DT: case VR_MAP_NEE_DRT:
         *in_dim = 6; *out_dim = 3; return 1;
@@ -1545,6 +1591,8 @@ static void field_fw(ctx_t* c, int kind, const double* in, double* out) {
     case VR_MAP_T_DDA: out[0] = t_dda(c, x, w); break;
     case VR_MAP_T_DT: out[0] = t_dt(c, x, w); break;
     case VR_MAP_T_RT: out[0] = t_rt(c, x, w); break;
+    case VR_MAP_T_GDT: out[0] = t_grid_tracking(c, x, w, 0); break;
+    case VR_MAP_T_GRT: out[0] = t_grid_tracking(c, x, w, 1); break;
     case VR_MAP_COLLISION_DT: { double t, T; out[0] = collision_dt(c, x, w, &t, &T); out[1] = t; out[2] = T; break; }
     case VR_MAP_SS_RADIANCE: { v3 r = ss_radiance(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }
     case VR_MAP_MC_COLLISION: { v3 r = mc_collision(c, x, w); out[0] = r.x; out[1] = r.y; out[2] = r.z; break; }

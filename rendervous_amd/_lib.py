@@ -21,7 +21,7 @@ VR_OK, VR_ERR_ARG, VR_ERR_UNSUPPORTED, VR_ERR_DEVICE = 0, 1, 2, 3
 VR_FIELD_CONST, VR_FIELD_GRID = 0, 1
 VR_PHASE_HG_MIX, VR_PHASE_HG, VR_PHASE_ISO = 0, 1, 2
 VR_MAJ_CONST, VR_MAJ_GRID = 0, 1
-VR_T_DT, VR_T_RT, VR_T_DDA = 0, 1, 2
+VR_T_DT, VR_T_RT, VR_T_DDA, VR_T_GDT, VR_T_GRT = 0, 1, 2, 3, 4
 VR_MAP_RAYBOX = 1
 VR_MAP_T_DDA = 2
 VR_MAP_T_DT = 3
@@ -33,6 +33,8 @@ VR_MAP_NEE_DT = 8
 VR_MAP_MEDIUM_DT = 9
 VR_MAP_PATH_DT = 10
 VR_MAP_NEE_DRT = 11
+VR_MAP_T_GDT = 12
+VR_MAP_T_GRT = 13
 VR_SEED_MAP, VR_SEED_CAPTURE_FW, VR_SEED_CAPTURE_BW = 0, 1, 2
 VR_BW_REFERENCE, VR_BW_CONSISTENT = 0, 1
 
@@ -48,6 +50,8 @@ MAP_DIMS = {
     VR_MAP_MEDIUM_DT: (7, 12),
     VR_MAP_PATH_DT: (6, 3),
     VR_MAP_NEE_DRT: (6, 3),
+    VR_MAP_T_GDT: (6, 1),
+    VR_MAP_T_GRT: (6, 1),
 }
 
 F3 = ctypes.c_float * 3

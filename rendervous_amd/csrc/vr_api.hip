@@ -82,7 +82,8 @@ int make_grid_view(const vr_grid* g, GridView& v, int want_c, const char* what) 
 int map_dims(int kind, int& in_dim, int& out_dim) {
     switch (kind) {
     case VR_MAP_RAYBOX: in_dim = 6; out_dim = 2; return 1;
-    case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: in_dim = 6; out_dim = 1; return 1;
+    case VR_MAP_T_DDA: case VR_MAP_T_DT: case VR_MAP_T_RT: case VR_MAP_T_GDT: case VR_MAP_T_GRT:
+        in_dim = 6; out_dim = 1; return 1;
     case VR_MAP_COLLISION_DT: case VR_MAP_SS_RADIANCE: case VR_MAP_MC_COLLISION: case VR_MAP_NEE_DT:
     case VR_MAP_PATH_DT: case VR_MAP_NEE_DRT:
         in_dim = 6; out_dim = 3; return 1;
@@ -132,7 +133,7 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
                          (m->sigma.bmax[2] - m->sigma.bmin[2]) / v.mrz);
         v.maj_icsize = F3(1.0f / v.maj_csize.x, 1.0f / v.maj_csize.y, 1.0f / v.maj_csize.z);
     } else if (m->majorant_kind != VR_MAJ_CONST) return set_err(VR_ERR_ARG, "unknown majorant kind");
-    if (m->transmittance_kind < VR_T_DT || m->transmittance_kind > VR_T_DDA) return set_err(VR_ERR_ARG, "unknown transmittance kind");
+    if (m->transmittance_kind < VR_T_DT || m->transmittance_kind > VR_T_GRT) return set_err(VR_ERR_ARG, "unknown transmittance kind");
     v.t_kind = m->transmittance_kind;
     v.ds_eps = m->ds_epsilon;
     v.max_bounces = m->max_bounces;
@@ -179,6 +180,8 @@ __device__ __forceinline__ void field_fw(Tracer<MAJ>& tr, const float* in, float
         out[0] = tr.t_dt(x, w);
     } else if constexpr (KIND == VR_MAP_T_RT) {
         out[0] = tr.t_rt(x, w);
+    } else if constexpr (KIND == VR_MAP_T_GDT || KIND == VR_MAP_T_GRT) {
+        out[0] = tr.t_grid_tracking(x, w, KIND == VR_MAP_T_GRT);
     } else if constexpr (KIND == VR_MAP_COLLISION_DT) {
         float t, T;
         out[0] = tr.collision_dt(x, w, t, T);
@@ -240,6 +243,8 @@ template <> struct Dims<VR_MAP_NEE_DT> { static constexpr int in = 6, out = 3; }
 template <> struct Dims<VR_MAP_MEDIUM_DT> { static constexpr int in = 7, out = 12; };
 template <> struct Dims<VR_MAP_PATH_DT> { static constexpr int in = 6, out = 3; };
 template <> struct Dims<VR_MAP_NEE_DRT> { static constexpr int in = 6, out = 3; };
+template <> struct Dims<VR_MAP_T_GDT> { static constexpr int in = 6, out = 1; };
+template <> struct Dims<VR_MAP_T_GRT> { static constexpr int in = 6, out = 1; };
 
 // ---------------------------------------------------------------- kernels --
 __global__ void __launch_bounds__(kBlock) k_grid_fw(GridView g, const float* __restrict__ x, int64_t n, float* __restrict__ out) {
@@ -551,7 +556,7 @@ int launch_map_fw_maj(int kind, const MediumView& M, const float* in, int64_t st
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_map_fw<K, MAJ>), g, b, 0, st, M, in, start, cnt, out, sd, dbg); break;
         VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_MEDIUM_DT)
-        VR_CASE(VR_MAP_PATH_DT) VR_CASE(VR_MAP_NEE_DRT)
+        VR_CASE(VR_MAP_PATH_DT) VR_CASE(VR_MAP_NEE_DRT) VR_CASE(VR_MAP_T_GDT) VR_CASE(VR_MAP_T_GRT)
 #undef VR_CASE
     default: return set_err(VR_ERR_UNSUPPORTED, "unknown map kind");
     }
@@ -569,6 +574,7 @@ int launch_map_bw_maj(int kind, const MediumView& M, const float* in, const floa
         VR_CASE(VR_MAP_PATH_DT) VR_CASE(VR_MAP_NEE_DRT)
 #undef VR_CASE
     case VR_MAP_RAYBOX: return VR_OK;  // no parameters
+    case VR_MAP_T_GDT: case VR_MAP_T_GRT: return VR_OK;  // no backward (maps/transmittance_gdt.h, _grt.h)
     default: return set_err(VR_ERR_UNSUPPORTED, "unknown map kind");
     }
     return check_hip(hipGetLastError(), "vr_map_bw launch");
@@ -611,7 +617,7 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_fw<K, MAJ>), g, b, 0, st, M, C, sd, P, total, start, cnt, out, dbg); break;
         VR_CASE(VR_MAP_RAYBOX) VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_PATH_DT)
-        VR_CASE(VR_MAP_NEE_DRT)
+        VR_CASE(VR_MAP_NEE_DRT) VR_CASE(VR_MAP_T_GDT) VR_CASE(VR_MAP_T_GRT)
 #undef VR_CASE
     default: return set_err(VR_ERR_UNSUPPORTED, "map kind cannot be captured (needs 6 inputs)");
     }
@@ -633,7 +639,7 @@ int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Se
         VR_CASE(VR_MAP_SS_RADIANCE) VR_CASE(VR_MAP_MC_COLLISION) VR_CASE(VR_MAP_NEE_DT) VR_CASE(VR_MAP_PATH_DT)
         VR_CASE(VR_MAP_NEE_DRT)
 #undef VR_CASE
-    case VR_MAP_RAYBOX: return VR_OK;
+    case VR_MAP_RAYBOX: case VR_MAP_T_GDT: case VR_MAP_T_GRT: return VR_OK;
     default: return set_err(VR_ERR_UNSUPPORTED, "map kind cannot be captured (needs 6 inputs)");
     }
     return check_hip(hipGetLastError(), "vr_capture_bw launch");
@@ -905,7 +911,7 @@ int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_lau
     MediumView M;
     int st = make_medium_view(medium, VR_MAP_NEE_DT, L->bw_mode, M);
     if (st) return st;
-    if (M.t_kind == VR_T_DDA) return set_err(VR_ERR_UNSUPPORTED, "vr_nrc_render: DT or RT shadow rays only");
+    if (M.t_kind != VR_T_DT && M.t_kind != VR_T_RT) return set_err(VR_ERR_UNSUPPORTED, "vr_nrc_render: DT or RT shadow rays only");
     CameraView C;
     if ((st = make_camera_view(camera, C))) return st;
     const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;

@@ -609,6 +609,54 @@ struct Tracer {
         return (float)exp(-dda_tau(s));
     }
 
+    // include/vr_transmittance_GDT.h:1-84 / _GRT.h:1-83 over the DDA's cells:
+    // the cell's minorant (min of its 8 vertices) in closed form, the residual
+    // tracked with majorant max - min (GDT ends at a collision, GRT multiplies
+    // by 1 - Pc).  Cell walk in FP64 as the DDA's.
+    VR_DEV float t_grid_tracking(f3 x, f3 w, bool ratio) {
+        Dda s;
+        if (!dda_begin(x, w, s)) return 1.0f;
+        double alpha[3] = {s.alpha0[0], s.alpha0[1], s.alpha0[2]};
+        int cell[3] = {s.cell[0], s.cell[1], s.cell[2]};
+        double cur = 0.0;
+        float T = 1.0f;
+        while (cur < s.d - 0.0001) {
+            const double nxt = fmin(s.d, fmin(alpha[0], fmin(alpha[1], alpha[2])));
+            float v[8];
+            float mx = -kInf, mn = kInf;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                v[k] = dda_sigma_at(cell[0] + (k & 1), cell[1] + ((k >> 1) & 1), cell[2] + ((k >> 2) & 1));
+                mx = fmaxf(mx, v[k]);
+                mn = fminf(mn, v[k]);
+            }
+            T *= expf(-mn * (float)(nxt - cur));
+            const float r = fmaxf(0.000001f, mx - mn);
+            double ct = cur;
+            for (;;) {
+                const float dt = flight(r);
+                if (ct + dt > nxt) break;
+                ct += dt;
+                const float ax = (float)(s.vm[0] * ct + s.vn[0] - cell[0]);
+                const float ay = (float)(s.vm[1] * ct + s.vn[1] - cell[1]);
+                const float az = (float)(s.vm[2] * ct + s.vn[2] - cell[2]);
+                const float bx = 1.0f - ax, by = 1.0f - ay;
+                const float sv = (1.0f - az) * (by * (bx * v[0] + ax * v[1]) + ay * (bx * v[2] + ax * v[3])) +
+                                 az * (by * (bx * v[4] + ax * v[5]) + ay * (bx * v[6] + ax * v[7]));
+                lookups++;
+                const float Pc = (sv - mn) / r;
+                if (!ratio) {
+                    if (rnd() < Pc) return 0.0f;
+                } else {
+                    T *= 1.0f - Pc;
+                }
+            }
+            dda_step(alpha, cell, s);
+            cur = nxt;
+        }
+        return T;
+    }
+
     // include/vr_transmittance_DDA.h:62-156
     VR_DEV void t_dda_bw(f3 x, f3 w, float dL_dT) {
         if (dL_dT == 0.0f || M.sigma.grad == nullptr) return;
@@ -742,13 +790,15 @@ struct Tracer {
     VR_DEV float transmittance(f3 x, f3 w) {
         if (M.t_kind == VR_T_RT) return t_rt(x, w);
         if (M.t_kind == VR_T_DDA) return t_dda(x, w);
+        if (M.t_kind == VR_T_GDT || M.t_kind == VR_T_GRT) return t_grid_tracking(x, w, M.t_kind == VR_T_GRT);
         return t_dt(x, w);
     }
 
+    // (GDT and GRT have no backward in the reference: no gradient)
     VR_DEV void transmittance_bw(f3 x, f3 w, float dL_dT) {
         if (M.t_kind == VR_T_RT) t_rt_bw(x, w, dL_dT);
         else if (M.t_kind == VR_T_DDA) t_dda_bw(x, w, dL_dT);
-        else t_dt_bw(x, w, dL_dT);
+        else if (M.t_kind == VR_T_DT) t_dt_bw(x, w, dL_dT);
     }
 
     // include/vr_collision_sampler_DT.h:5-44

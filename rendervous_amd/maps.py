@@ -21,7 +21,8 @@ from . import _lib as L
 
 __all__ = [
     "vec3", "MapBase", "ConstantMap", "const", "ZERO", "ONE", "Grid3D", "RayBoxIntersection", "ray_box_intersection",
-    "MajorantGrid", "GridDDATransmittance", "DeltatrackingTransmittance", "RatiotrackingTransmittance",
+    "MajorantGrid", "GridDDATransmittance", "GridDeltatrackingTransmittance", "GridRatiotrackingTransmittance",
+    "DeltatrackingTransmittance", "RatiotrackingTransmittance",
     "DeltatrackingCollisionSampler", "HGPhase", "VHGPhaseSampler", "UniformDirectionSampler", "_functionsampler",
     "DirectionalLightEnvironmentSampler", "SingleScatteredRadiance", "MCCollisionIntegrator",
     "DeltatrackingNEEPathIntegrator", "DeltatrackingPathIntegrator", "DRTPathIntegrator", "SensorsBase", "PerspectiveCameraSensor", "normalized_box",
@@ -479,13 +480,18 @@ class _Lowered:
             self.set_sigma(t.grid_model)
             self.medium.transmittance_kind = L.VR_T_DDA
             return
+        if isinstance(t, _GridTrackingTransmittance):
+            self.set_sigma(t.grid_model)
+            self.set_boundary(t.boundary)
+            self.medium.transmittance_kind = t._t_kind
+            return
         if isinstance(t, (DeltatrackingTransmittance, RatiotrackingTransmittance)):
             self.set_sigma(t.sigma)
             self.set_boundary(t.boundary)
             self.set_majorant(t.majorant)
             self.medium.transmittance_kind = L.VR_T_DT if isinstance(t, DeltatrackingTransmittance) else L.VR_T_RT
             return
-        raise NotImplementedError("transmittance must be a DT, RT or DDA transmittance map")
+        raise NotImplementedError("transmittance must be a DT, RT, DDA, GDT or GRT transmittance map")
 
 
 def _is_uniform_phase_sampler(s: MapBase) -> bool:
@@ -523,6 +529,37 @@ class GridDDATransmittance(_RayMap):
         lw.set_boundary(self.boundary)
         lw.medium.transmittance_kind = L.VR_T_DDA
         return L.VR_MAP_T_DDA, lw
+
+
+class _GridTrackingTransmittance(_RayMap):
+    """Grid-decomposition tracking over a Grid3D: per DDA cell, the minorant of
+    the cell's 8 vertices is integrated in closed form and the residual tracked
+    with the cell's max - min as majorant.  No backward (the reference has none):
+    the gradient with respect to the grid is 0."""
+    _t_kind = None
+    _map_kind = None
+
+    def __init__(self, grid: Grid3D, boundary: MapBase):
+        super().__init__(6, 1)
+        self.grid_model = grid
+        self.boundary = boundary
+
+    def _lower_map(self):
+        lw = _Lowered()
+        lw.set_transmittance(self)
+        return self._map_kind, lw
+
+
+class GridDeltatrackingTransmittance(_GridTrackingTransmittance):
+    """(_maps.py:2619-2637, vr_transmittance_GDT.h): 0 at the first residual collision."""
+    _t_kind = L.VR_T_GDT
+    _map_kind = L.VR_MAP_T_GDT
+
+
+class GridRatiotrackingTransmittance(_GridTrackingTransmittance):
+    """(_maps.py:2588-2616, vr_transmittance_GRT.h): the product of 1 - Pc over residual events."""
+    _t_kind = L.VR_T_GRT
+    _map_kind = L.VR_MAP_T_GRT
 
 
 class DeltatrackingTransmittance(_RayMap):

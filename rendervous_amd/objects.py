@@ -19,7 +19,7 @@ __all__ = [
     "DependencySet", "medium_box_AABB", "medium_box_normalized", "medium_box", "medium_boundary_box", "medium_boundary",
     "build_map_from_tensor", "medium_sigma_tensor", "medium_sigma", "medium_scattering_albedo_tensor",
     "medium_scattering_albedo", "medium_emission", "medium_phase_g_tensor", "medium_phase_g", "medium_majorant_tensor",
-    "medium_majorant_grid", "medium_majorant_coarse_grid", "medium_majorant", "medium_transmittance_DDA",
+    "medium_majorant_grid", "medium_majorant_coarse_grid", "medium_majorant", "medium_transmittance_DDA", "medium_transmittance_GDT", "medium_transmittance_GRT",
     "medium_transmittance_DT", "medium_transmittance_RT", "medium_transmittance", "medium_environment_tensor",
     "medium_environment", "medium_environment_sampler_uniform", "medium_environment_sampler_sun",
     "medium_environment_sampler", "medium_phase_iso", "medium_phase_HG", "medium_phase", "medium_phase_sampler_HG",
@@ -201,6 +201,22 @@ def medium_transmittance_DDA(ds: DependencySet):
     ds.requires(medium_boundary)
     ds.assert_ensures("sigma", _maps.Grid3D)
     ds.add_parameters(transmittance=_maps.GridDDATransmittance(ds.sigma, ds.boundary))
+
+
+def medium_transmittance_GDT(ds: DependencySet):
+    """_objects.py:116-120"""
+    ds.requires(medium_sigma)
+    ds.requires(medium_boundary)
+    ds.assert_ensures("sigma", _maps.Grid3D)
+    ds.add_parameters(transmittance=_maps.GridDeltatrackingTransmittance(ds.sigma, ds.boundary))
+
+
+def medium_transmittance_GRT(ds: DependencySet):
+    """_objects.py:122-126"""
+    ds.requires(medium_sigma)
+    ds.requires(medium_boundary)
+    ds.assert_ensures("sigma", _maps.Grid3D)
+    ds.add_parameters(transmittance=_maps.GridRatiotrackingTransmittance(ds.sigma, ds.boundary))
 
 
 def medium_transmittance_DT(ds: DependencySet):

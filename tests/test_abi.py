@@ -83,6 +83,21 @@ def test_nee_integrator_lowers_to_a_medium_descriptor():
     assert m.sigma.t.shape[0] == 9 and m.sigma.t.stride[2] == 1
 
 
+@pytest.mark.parametrize("recipe,tkind", [("medium_transmittance_GDT", L.VR_T_GDT), ("medium_transmittance_GRT", L.VR_T_GRT)])
+def test_grid_tracking_recipes_lower_to_their_transmittance(recipe, tkind):
+    sigma = torch.rand(7, 7, 7, 1)
+    ds = rv.objects.DependencySet()
+    ds.add_parameters(sigma_tensor=sigma, scattering_albedo_tensor=torch.tensor([0.9, 0.8, 0.7]),
+                      environment_tensor=torch.tensor([0.1, 0.1, 0.1]), phase_g_tensor=torch.tensor([0.6]))
+    ds.requires(getattr(rv.objects, recipe))
+    kind, lw = ds.transmittance._lower_map()
+    assert kind == (L.VR_MAP_T_GDT if tkind == L.VR_T_GDT else L.VR_MAP_T_GRT)
+    assert lw.medium.transmittance_kind == tkind
+    ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DT, max_bounces=4)
+    kind, lw = ds.radiance._lower_map()
+    assert kind == L.VR_MAP_NEE_DT and lw.medium.transmittance_kind == tkind
+
+
 def test_unsupported_graphs_raise():
     sigma = torch.rand(5, 5, 5, 1)
     g = rv.Grid3D(sigma, *rv.normalized_box(sigma))

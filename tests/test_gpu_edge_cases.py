@@ -91,7 +91,8 @@ def test_strided_density_view_equals_its_contiguous_copy(gpu):
     assert torch.equal(imgs[0], imgs[1])
 
 
-@pytest.mark.parametrize("kind", [L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_T_DDA, L.VR_MAP_NEE_DT, L.VR_MAP_PATH_DT])
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_T_DDA, L.VR_MAP_T_GDT, L.VR_MAP_T_GRT,
+                                  L.VR_MAP_NEE_DT, L.VR_MAP_PATH_DT])
 def test_empty_medium(gpu, kind):
     """sigma = 0: transmittance 1 and radiance = environment, exactly."""
     sc = Scene(sigma=np.zeros((6, 6, 6, 1), np.float32), albedo_c=(0.8, 0.8, 0.8), env_c=(0.25, 0.5, 1.0),

@@ -122,6 +122,21 @@ def test_trackers_seed_matched(gpu, kind, majorant):
     assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.995
 
 
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_GDT, L.VR_MAP_T_GRT])
+def test_grid_trackers_seed_matched(gpu, kind):
+    """Grid-decomposition delta / ratio tracking (vr_transmittance_GDT.h, _GRT.h),
+    rays from outside and from inside the box."""
+    _print_case("grid_trackers", kind=kind)
+    sc = Scene(sigma=fbm_grid(24, seed=7, scale=30.0))
+    rays = np.concatenate([random_rays(6144, seed=8), random_rays(2048, seed=9, inside=True)])
+    ref = oracle_map_fw(sc, kind, rays, SEEDS)
+    out = gpu_map_fw(sc, kind, rays, SEEDS)
+    assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.99
+    # no backward: the map's gradient call succeeds and leaves the grid gradient at 0
+    g, _ = gpu_map_bw(sc, kind, rays, np.ones((rays.shape[0], 1), np.float32), SEEDS)
+    assert not g.any()
+
+
 def test_collision_sampler_pins_rng_stream(gpu):
     """Homogeneous medium at the majorant: the first tentative collision is real, so
     t = tMin - log(1-u)/maj exposes the first uniform of every thread's stream."""
@@ -135,7 +150,7 @@ def test_collision_sampler_pins_rng_stream(gpu):
 
 
 @pytest.mark.parametrize("majorant", ["const", "grid"])
-@pytest.mark.parametrize("tkind", [L.VR_T_DT, L.VR_T_RT])
+@pytest.mark.parametrize("tkind", [L.VR_T_DT, L.VR_T_RT, L.VR_T_GDT, L.VR_T_GRT])
 def test_nee_capture_seed_matched(gpu, majorant, tkind):
     _print_case("nee_capture", majorant=majorant, tkind=tkind)
     sc = Scene(sigma=fbm_grid(24, seed=9, scale=20.0), albedo_c=(0.9, 0.8, 0.7), phase_kind=L.VR_PHASE_HG_MIX,

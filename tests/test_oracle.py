@@ -200,6 +200,52 @@ def test_dt_rt_expectation_equals_exact_transmittance(kind, majorant):
     assert abs(T.mean() - exact) <= 4 * se + 1e-4, (T.mean(), exact, se)
 
 
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_GDT, L.VR_MAP_T_GRT])
+def test_grid_tracking_constant_grid_is_exact(kind):
+    """sigma constant: every cell's minorant is sigma and the residual majorant
+    1e-6, so both trackers return exp(-sigma * chord) (up to the final 1e-4 piece
+    the cell walk drops, vr_transmittance_GDT.h:39)."""
+    sc = Scene(sigma=np.full((7, 7, 7, 1), 2.5, np.float32))
+    rays = np.concatenate([random_rays(200, seed=6), random_rays(50, seed=7, inside=True)])
+    T = oracle_map_fw(sc, kind, rays, (5, 6, 7, 8))[:, 0]
+    bx = oracle_map_fw(sc, L.VR_MAP_RAYBOX, rays, (1, 2, 3, 4))
+    chord = np.where(bx[:, 1] > np.maximum(bx[:, 0], 0), bx[:, 1] - np.clip(bx[:, 0], 0, None), 0.0)
+    lo, hi = np.exp(-2.5 * chord), np.exp(-2.5 * np.maximum(chord - 1e-4, 0))
+    assert np.all((T >= lo * (1 - 2e-6)) & (T <= hi * (1 + 2e-6)))
+
+
+@pytest.mark.parametrize("kind", [L.VR_MAP_T_GDT, L.VR_MAP_T_GRT])
+def test_grid_tracking_expectation_equals_exact_transmittance(kind):
+    """E[T] = exp(-integral of the trilinear field) by adaptive quadrature; a
+    closed-form part taken with the cell max, a wrong interpolation offset or a
+    dropped cell would each move the mean by many standard errors."""
+    g = blob_grid(10, seed=4, max_value=8.0)
+    sc = Scene(sigma=g)
+    ray = np.array([-0.1, 0.05, -1.5, 0.2, -0.1, 1.0], np.float32)
+    ray[3:] /= np.linalg.norm(ray[3:])
+    n = 100000
+    T = oracle_map_fw(sc, kind, np.tile(ray, (n, 1)), (31, 32, 33, 34))[:, 0]
+    exact = _exact_T(g, ray)
+    se = max(T.std(), 1e-3) / math.sqrt(n)
+    assert abs(T.mean() - exact) <= 4 * se + 1e-4, (T.mean(), exact, se)
+    if kind == L.VR_MAP_T_GDT:
+        # delta tracking returns 0 or exp(-integral of the cell minorants), one value per ray
+        nz = np.unique(T[T > 0])
+        assert np.ptp(nz) <= 1e-6 * nz.max()
+        # ratio tracking multiplies that value by factors 1 - Pc in [0, 1] (minorant <= sigma <= majorant)
+        Tr = oracle_map_fw(sc, L.VR_MAP_T_GRT, np.tile(ray, (1000, 1)), (1, 3, 5, 7))[:, 0]
+        assert Tr.min() >= 0 and Tr.max() <= nz.max() * (1 + 1e-6) and Tr.mean() < nz.max() * 0.99
+
+
+def test_grid_tracking_has_no_backward():
+    """maps/transmittance_gdt.h and _grt.h define FORWARD only: no gradient."""
+    g = blob_grid(6, seed=1, max_value=4.0)
+    rays = random_rays(32, seed=2)
+    for kind in (L.VR_MAP_T_GDT, L.VR_MAP_T_GRT):
+        grad, _ = oracle_map_bw(Scene(sigma=g), kind, rays, np.ones((32, 1), np.float32), (1, 2, 3, 4))
+        assert not grad.any()
+
+
 def test_collision_sampler_homogeneous_closed_form():
     s = 2.5
     sc = Scene(sigma=np.full((4, 4, 4, 1), s, np.float32), majorant_value=4.0)
