@@ -400,6 +400,48 @@ def dda_leg(dev, reps=20):
             "ms_per_frame": ms, "mean_T": float(img.mean().item())}
 
 
+def grid_tools_leg(dev, res=512, reps=10):
+    """TV regulariser fw/bw over a res^3 density and a 2x resample (res/2)^3 -> res^3,
+    the per-iteration grid passes of coarse-to-fine inverse rendering; HBM roofline
+    with algorithmic bytes 8 B/voxel (TV fw), 12 B/voxel (TV bw), 4 B per input and
+    output vertex (resample)."""
+    import torch
+    import rendervous_amd as rv
+    from rendervous_amd import _lib as L
+
+    V = res ** 3
+    x = rv.functions.fbm_value_noise(res, seed=3, scale=20.0, device=dev).contiguous()
+    og = torch.randn(res, res, res, 1, device=dev)
+    tv = torch.empty_like(og)
+    g = torch.empty_like(x)
+    half = rv.functions.resample_grid(x, (res // 2,) * 3)
+    up = torch.empty_like(x)
+    lib, st = L.load(), L.stream_handle(dev)
+    sh, hs = L.i64s(x.shape), L.i64s(half.shape[:-1])
+    dst = L.i64s((res,) * 3)
+    calls = {
+        "tv_fw": (lambda: lib.vr_total_variation_fw(x.data_ptr(), sh, 3, tv.data_ptr(), st), 8.0 * V),
+        "tv_bw": (lambda: lib.vr_total_variation_bw(x.data_ptr(), og.data_ptr(), sh, 3, g.data_ptr(), st), 12.0 * V),
+        "resample_2x": (lambda: lib.vr_resample_grid(half.data_ptr(), hs, dst, 3, 1, up.data_ptr(), st),
+                        4.0 * (V + half.numel())),
+    }
+    out = {"workload": f"{res}^3 x 1 float32 grid: TV forward, TV backward, {res // 2}^3 -> {res}^3 resample"}
+    stream = torch.cuda.current_stream(dev)
+    for name, (fn, nbytes) in calls.items():
+        L.check(fn(), name)
+        torch.cuda.synchronize()
+        e = _events(2)
+        e[0].record(stream)
+        for _ in range(reps):
+            fn()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        ms = e[0].elapsed_time(e[1]) / reps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "bytes": nbytes}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -502,6 +544,8 @@ def main():
         result["secondary"]["nrc"] = nrc_leg(dev)
         torch.cuda.empty_cache()
         result["secondary"]["dda"] = dda_leg(dev)
+        torch.cuda.empty_cache()
+        result["secondary"]["grid_tools"] = grid_tools_leg(dev)
         log(f"secondary: {json.dumps(result['secondary'])}")
     if not args.no_cpu_baseline:
         m = build_scene(args.res, args.width, args.height, dev)[1]

@@ -229,6 +229,37 @@ int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, fl
 int vr_adam_step_bounded(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
                          float eps, int32_t step, float lo, float hi, int32_t zero_grad, void* stream);
 
+/* ---- grid tools of the inverse-rendering loop --------------------------------
+ * Dense row-major float32 tensors [d0, .., d(dim-1), C] (C channels innermost);
+ * shapes are host arrays. */
+
+/* Replaces functions.random_ids (_functions.py:114-142, functions/tools/
+ * random_ids.comp.glsl), which SensorsBase.random_sensors (_maps.py:1751-1762)
+ * calls: out[n][dim] int64 with out[i][k] = int(u * shape[k]), u the k-th
+ * uniform of thread i seeded as the reference's compute functions
+ * (functions/common_functions.h:27-49) from the dispatch seeds[4].  1 <= dim <= 4. */
+int vr_random_ids(int64_t* out, int64_t n, const int64_t* shape, int32_t dim, const uint32_t seeds[4], void* stream);
+
+/* Replaces the total_variation forward (_functions.py:352-380, functions/
+ * regularizers/total_variation.comp.glsl): shape[dim + 1] = (spatial dims..., C),
+ * 1 <= dim <= 3; out [spatial dims..., 1] = per-voxel sum over the existing
+ * +-1 neighbours of sum_c |v - v_adj|. */
+int vr_total_variation_fw(const float* in, const int64_t* shape, int32_t dim, float* out, void* stream);
+
+/* Replaces the total_variation backward (_functions.py:383-413, total_variation_
+ * backward.comp.glsl): in_grad [shape] = sum over neighbours j of
+ * (out_grad[i] + out_grad[j]) * sign(v[i] - v[j]); in_grad is overwritten. */
+int vr_total_variation_bw(const float* in, const float* out_grad, const int64_t* shape, int32_t dim, float* in_grad,
+                          void* stream);
+
+/* One linear resampling step of functions.resample_grid (_functions.py:252-297,
+ * functions/tools/resample_grid.comp.glsl): in [in_shape..., C] -> out
+ * [out_shape..., C], source coordinate p * (in - 1) / (out - 1) per axis
+ * (vertex-aligned), 2^dim taps with the +1 tap clamped.  1 <= dim <= 3, every
+ * out_shape entry >= 2 (the shader divides by out - 1). */
+int vr_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                     float* out, void* stream);
+
 /* ---- neural radiance cache ---------------------------------------------------
  * No reference counterpart (the north star's cached renderer): a 96-64-64-64-64-3
  * ReLU MLP in BF16 with FP32 accumulation.  Q(x, w) = radiance a NEE path collects

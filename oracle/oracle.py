@@ -56,6 +56,13 @@ def load():
                                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
                                                         ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_float,
                                                         ctypes.c_float, ctypes.c_int32]),
+            "oracle_random_ids": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32,
+                                                 ctypes.c_void_p]),
+            "oracle_total_variation_fw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+            "oracle_total_variation_bw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                         ctypes.c_void_p]),
+            "oracle_resample_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                    ctypes.c_int32, ctypes.c_void_p]),
             "oracle_rng_uniforms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
             "oracle_rng_seed_states": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                                       ctypes.c_void_p]),

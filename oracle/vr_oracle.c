@@ -1938,3 +1938,143 @@ int oracle_rng_seed_states(int32_t mode, const uint32_t seeds[4], int64_t first,
     }
     return VR_OK;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Grid tools (compute functions of the reference)                            */
+/* ------------------------------------------------------------------------- */
+
+/* functions/tools/random_ids.comp.glsl:11-20; seeding of
+ * functions/common_functions.h:27-49 (seeds + (0x23F1, 0x3137, 129, i + 129), four
+ * discarded steps, then the bits of four uniforms as the new state). */
+int oracle_random_ids(int64_t* out, int64_t n, const int64_t* shape, int32_t dim, const uint32_t seeds[4]) {
+    if (n < 0 || dim < 1 || dim > 4 || !shape || !seeds || (n > 0 && !out)) return fail(VR_ERR_ARG, "bad random_ids arguments");
+    for (int64_t i = 0; i < n; i++) {
+        rng_t s;
+        s.x = seeds[0] + 0x23F1u;
+        s.y = seeds[1] + 0x3137u;
+        s.z = seeds[2] + 129u;
+        s.w = seeds[3] + ((uint32_t)i + 129u);
+        for (int k = 0; k < 4; k++) rng_random(&s);
+        float a = rng_random(&s), b = rng_random(&s), c = rng_random(&s), d = rng_random(&s);
+        s.x = float_to_bits(a); s.y = float_to_bits(b); s.z = float_to_bits(c); s.w = float_to_bits(d);
+        for (int k = 0; k < dim; k++) {
+            float u = rng_random(&s);
+            float prod = u * (float)(int)shape[k];
+            out[i * dim + k] = (int64_t)(int)prod;
+        }
+    }
+    return VR_OK;
+}
+
+/* total_variation*.comp.glsl: adjacent_index (lines 13-31) walks the
+ * coordinates innermost first; shape[0..dim-1] outermost first as stored. */
+static int tv_adjacent(int64_t index, const int64_t* shape, int dim, int axis, int delta, int64_t* adj) {
+    int64_t size = 1, a = 0;
+    for (int i = 0; i < dim; i++) {
+        int64_t id = shape[dim - i - 1];
+        int64_t pc = index % id;
+        if (i == axis) pc += delta;
+        if (pc < 0 || pc >= id) return 0;
+        a += pc * size;
+        index /= id;
+        size *= id;
+    }
+    *adj = a;
+    return 1;
+}
+
+static int tv_check(const int64_t* shape, int32_t dim, int64_t* voxels) {
+    if (!shape || dim < 1 || dim > 3) return fail(VR_ERR_ARG, "total variation: 1 to 3 spatial dimensions");
+    int64_t v = 1;
+    for (int i = 0; i <= dim; i++) {
+        if (shape[i] < 1) return fail(VR_ERR_ARG, "total variation: bad extent");
+        if (i < dim) v *= shape[i];
+    }
+    *voxels = v;
+    return VR_OK;
+}
+
+/* total_variation.comp.glsl:33-74 (float arithmetic, the shader's order). */
+int oracle_total_variation_fw(const float* in, const int64_t* shape, int32_t dim, float* out) {
+    int64_t V;
+    int st = tv_check(shape, dim, &V);
+    if (st) return st;
+    const int64_t C = shape[dim];
+    for (int64_t i = 0; i < V; i++) {
+        float tv = 0.0f;
+        for (int d = 0; d < dim; d++) {
+            for (int k = 0; k < 2; k++) {
+                int64_t j;
+                if (!tv_adjacent(i, shape, dim, d, k == 0 ? 1 : -1, &j)) continue;
+                float t = 0.0f;
+                for (int64_t c = 0; c < C; c++) t += fabsf(in[i * C + c] - in[j * C + c]);
+                tv += t;
+            }
+        }
+        out[i] = tv;
+    }
+    return VR_OK;
+}
+
+static float signf_glsl(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+/* total_variation_backward.comp.glsl:33-76. */
+int oracle_total_variation_bw(const float* in, const float* out_grad, const int64_t* shape, int32_t dim, float* in_grad) {
+    int64_t V;
+    int st = tv_check(shape, dim, &V);
+    if (st) return st;
+    const int64_t C = shape[dim];
+    for (int64_t i = 0; i < V; i++) {
+        for (int64_t c = 0; c < C; c++) in_grad[i * C + c] = 0.0f;
+        for (int d = 0; d < dim; d++) {
+            for (int k = 0; k < 2; k++) {
+                int64_t j;
+                if (!tv_adjacent(i, shape, dim, d, k == 0 ? 1 : -1, &j)) continue;
+                for (int64_t c = 0; c < C; c++)
+                    in_grad[i * C + c] += (out_grad[i] + out_grad[j]) * signf_glsl(in[i * C + c] - in[j * C + c]);
+            }
+        }
+    }
+    return VR_OK;
+}
+
+/* functions/tools/resample_grid.comp.glsl:14-86 (float arithmetic, taps and
+ * weights in the shader's order). */
+int oracle_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                         float* out) {
+    if (!in_shape || !out_shape || dim < 1 || dim > 3 || channels < 1) return fail(VR_ERR_ARG, "bad resample arguments");
+    int64_t V = 1;
+    for (int i = 0; i < dim; i++) {
+        if (in_shape[i] < 1 || out_shape[i] < 2) return fail(VR_ERR_ARG, "resample: output extents must be >= 2");
+        V *= out_shape[i];
+    }
+    for (int64_t index = 0; index < V; index++) {
+        int ic[4] = {0, 0, 0, 0};
+        float al[4] = {0, 0, 0, 0};
+        int64_t r = index;
+        for (int i = 0; i < dim; i++) {
+            int id = (int)in_shape[dim - i - 1], od = (int)out_shape[dim - i - 1];
+            int pc = (int)(r % od);
+            float coord = (float)(pc * (id - 1)) / (float)(od - 1);
+            ic[i] = (int)floorf(coord);
+            al[i] = coord - (float)ic[i];
+            r /= od;
+        }
+        for (int c = 0; c < channels; c++) out[index * channels + c] = 0.0f;
+        for (int t = 0; t < (1 << dim); t++) {
+            float w = 0.0f;
+            int64_t src = 0, u = 1;
+            for (int i = 0; i < dim; i++) {
+                int dx = (t >> i) & 1;
+                float f = dx ? al[i] : 1.0f - al[i];
+                w = i == 0 ? f : w * f;
+                int d = (int)in_shape[dim - i - 1];
+                int cc = ic[i] + dx < d - 1 ? ic[i] + dx : d - 1;
+                src += (int64_t)cc * u;
+                u *= d;
+            }
+            for (int c = 0; c < channels; c++) out[index * channels + c] += w * in[src * channels + c];
+        }
+    }
+    return VR_OK;
+}

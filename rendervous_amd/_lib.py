@@ -146,6 +146,14 @@ _SIGNATURES = {
                                             ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                             ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                                             ctypes.c_void_p]),
+    "vr_random_ids": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, P(ctypes.c_int64), ctypes.c_int32, P(ctypes.c_uint32),
+                                     ctypes.c_void_p]),
+    "vr_total_variation_fw": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int64), ctypes.c_int32, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
+    "vr_total_variation_bw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_int64), ctypes.c_int32,
+                                             ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_resample_grid": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
     "vr_nrc_encode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, P(ctypes.c_float), P(ctypes.c_float),
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "vr_nrc_mlp_fw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -159,6 +167,17 @@ _SIGNATURES = {
     "vr_nrc_render": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
 }
+
+def i64s(values) -> ctypes.Array:
+    """A host int64 array for the shape arguments."""
+    values = [int(v) for v in values]
+    return (ctypes.c_int64 * max(1, len(values)))(*values)
+
+
+def u32s(values) -> ctypes.Array:
+    values = [int(v) & 0xFFFFFFFF for v in values]
+    return (ctypes.c_uint32 * len(values))(*values)
+
 
 NRC_PARAMS = 19456
 NRC_FEATURES = 96

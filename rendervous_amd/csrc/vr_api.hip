@@ -20,6 +20,7 @@
 #include "vr_device.hpp"
 #include "vr_nee_wavefront.hpp"
 #include "vr_nrc.hpp"
+#include "vr_functions.hpp"
 
 using namespace vr;
 
@@ -798,9 +799,97 @@ int vr_adam_step(float* p, float* g, float* m, float* v, int64_t n, float lr, fl
     return vr_adam_step_bounded(p, g, m, v, n, lr, beta1, beta2, eps, step, -INFINITY, INFINITY, zero_grad, stream);
 }
 
-// ------------------------------------------------------------------ cache --
 }  // extern "C"
 
+// ------------------------------------------------------------- grid tools --
+
+namespace {
+
+// Spatial dims (outermost first, as given) -> x, y, z innermost first.
+int make_shape3(const int64_t* shape, int dim, int64_t channels, fn::Shape3& S, const char* what) {
+    if (!shape) return set_err(VR_ERR_ARG, "%s: shape is NULL", what);
+    if (dim < 1 || dim > 3) return set_err(VR_ERR_ARG, "%s: 1 to 3 spatial dimensions", what);
+    if (channels < 1 || channels > (1 << 20)) return set_err(VR_ERR_ARG, "%s: bad channel count", what);
+    int64_t total = 1;
+    for (int i = 0; i < 3; i++) {
+        const int64_t n = i < dim ? shape[dim - 1 - i] : 1;
+        if (n < 1 || n > (int64_t(1) << 30)) return set_err(VR_ERR_ARG, "%s: bad extent", what);
+        S.n[i] = (int)n;
+        S.s[i] = total;
+        total *= n;
+        if (total > (int64_t(1) << 40)) return set_err(VR_ERR_ARG, "%s: tensor too large", what);
+    }
+    S.C = (int)channels;
+    S.xb = (S.n[0] + fn::kRowBlock - 1) / fn::kRowBlock;
+    if ((int64_t)S.xb * S.n[1] * S.n[2] >= (int64_t(1) << 31)) return set_err(VR_ERR_ARG, "%s: too many rows", what);
+    return VR_OK;
+}
+
+inline dim3 shape_grid(const fn::Shape3& S) { return dim3((unsigned)((int64_t)S.xb * S.n[1] * S.n[2])); }
+
+}  // namespace
+
+extern "C" {
+
+int vr_random_ids(int64_t* out, int64_t n, const int64_t* shape, int32_t dim, const uint32_t seeds[4], void* stream) {
+    if (n < 0 || n >= (int64_t(1) << 31)) return set_err(VR_ERR_ARG, "vr_random_ids: n out of range");
+    if (dim < 1 || dim > 4) return set_err(VR_ERR_ARG, "vr_random_ids: 1 to 4 dimensions");
+    if (!shape || !seeds) return set_err(VR_ERR_ARG, "vr_random_ids: shape or seeds is NULL");
+    int sh[4] = {0, 0, 0, 0};
+    for (int k = 0; k < dim; k++) {
+        if (shape[k] < 1 || shape[k] >= (int64_t(1) << 31)) return set_err(VR_ERR_ARG, "vr_random_ids: bad extent");
+        sh[k] = (int)shape[k];
+    }
+    if (n == 0) return VR_OK;
+    if (!out) return set_err(VR_ERR_ARG, "vr_random_ids: out is NULL");
+    Seeds sd;
+    for (int k = 0; k < 4; k++) sd.s[k] = seeds[k];
+    hipLaunchKernelGGL(fn::k_random_ids, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, out, n,
+                       make_int4(sh[0], sh[1], sh[2], sh[3]), (int)dim, sd);
+    return check_hip(hipGetLastError(), "vr_random_ids launch");
+}
+
+int vr_total_variation_fw(const float* in, const int64_t* shape, int32_t dim, float* out, void* stream) {
+    fn::Shape3 S;
+    int st = make_shape3(shape, dim, shape && dim >= 1 && dim <= 3 ? shape[dim] : 0, S, "vr_total_variation_fw");
+    if (st) return st;
+    if (!in || !out) return set_err(VR_ERR_ARG, "vr_total_variation_fw: null buffer");
+    hipLaunchKernelGGL(fn::k_tv_fw, shape_grid(S), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, S, (int)dim, out);
+    return check_hip(hipGetLastError(), "vr_total_variation_fw launch");
+}
+
+int vr_total_variation_bw(const float* in, const float* out_grad, const int64_t* shape, int32_t dim, float* in_grad,
+                          void* stream) {
+    fn::Shape3 S;
+    int st = make_shape3(shape, dim, shape && dim >= 1 && dim <= 3 ? shape[dim] : 0, S, "vr_total_variation_bw");
+    if (st) return st;
+    if (!in || !out_grad || !in_grad) return set_err(VR_ERR_ARG, "vr_total_variation_bw: null buffer");
+    hipLaunchKernelGGL(fn::k_tv_bw, shape_grid(S), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, out_grad, S, (int)dim,
+                       in_grad);
+    return check_hip(hipGetLastError(), "vr_total_variation_bw launch");
+}
+
+int vr_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                     float* out, void* stream) {
+    fn::Resample R;
+    fn::Shape3 I;
+    int st = make_shape3(in_shape, dim, channels, I, "vr_resample_grid");
+    if (st) return st;
+    if ((st = make_shape3(out_shape, dim, channels, R.out, "vr_resample_grid"))) return st;
+    for (int i = 0; i < 3; i++) {
+        if (i < dim && R.out.n[i] < 2) return set_err(VR_ERR_ARG, "vr_resample_grid: output extents must be >= 2");
+        R.in_n[i] = I.n[i];
+        R.in_s[i] = I.s[i];
+    }
+    R.dim = dim;
+    if (!in || !out) return set_err(VR_ERR_ARG, "vr_resample_grid: null buffer");
+    hipLaunchKernelGGL(fn::k_resample, shape_grid(R.out), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, R, out);
+    return check_hip(hipGetLastError(), "vr_resample_grid launch");
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ cache --
 namespace {
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {

@@ -1,4 +1,11 @@
-"""Synthetic scene generators and camera helpers for the benchmark workloads.
+"""Grid tools of the inverse-rendering loop (HIP kernels through the C ABI) and
+synthetic scene generators and camera helpers for the benchmark workloads.
+
+Grid tools (``_functions.py`` of the reference):
+  * ``random_ids``: uniform random integer indices drawn with the reference's
+    compute-function RNG stream;
+  * ``total_variation``: the per-voxel anisotropic TV regulariser, differentiable;
+  * ``resample_grid``: vertex-aligned linear resampling, in steps of at most 2x.
 
 The density generators use only integer hashing and polynomial interpolation
 in torch, so CPU and GPU produce bit-identical grids for the same seed:
@@ -17,6 +24,8 @@ import typing
 
 import numpy as np
 import torch
+
+from . import _lib as L
 
 _M32 = 0xFFFFFFFF
 
@@ -121,12 +130,96 @@ def orbit_poses(n: int, radius: float = 2.5, elevation: float = 0.3, seed: int =
     return torch.stack(poses)
 
 
+def _dev(t: typing.Optional[torch.Tensor] = None) -> torch.device:
+    if t is not None and t.is_cuda:
+        return t.device
+    from . import _internal
+    return _internal.device()
+
+
 def random_ids(batch_size: int, shape: typing.Sequence[int], out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
-    """Uniform random integer indices [batch_size, len(shape)] (_functions.py:114-142)."""
-    dims = torch.tensor(list(shape), dtype=torch.float64)
-    r = (torch.rand(batch_size, len(shape), dtype=torch.float64) * dims).long()
-    r = torch.minimum(r, (dims - 1).long())
-    if out is not None:
-        out.copy_(r)
+    """Uniform random integer indices [batch_size, len(shape)], int64 on the device
+    (_functions.py:114-142): out[i][k] = int(u * shape[k]) with u from thread i's
+    stream of the dispatch seeds (one get_seeds() draw per call)."""
+    from . import _internal
+    shape = [int(v) for v in shape]
+    dim = len(shape)
+    dev = _dev(out)
+    if out is None:
+        out = torch.empty(batch_size, dim, dtype=torch.int64, device=dev)
+    if not out.is_cuda:
+        raise RuntimeError("random_ids writes a device tensor; no CPU fallback exists")
+    assert out.dtype == torch.int64 and out.is_contiguous() and out.numel() == batch_size * dim
+    seeds = _internal.get_seeds()
+    L.check(L.load().vr_random_ids(out.data_ptr(), batch_size, L.i64s(shape), dim, L.u32s(seeds), L.stream_handle(dev)),
+            "vr_random_ids")
+    return out
+
+
+def _spatial(t: torch.Tensor) -> typing.Tuple[typing.List[int], int]:
+    assert t.dtype == torch.float32, "grid tools take float32 tensors"
+    assert 2 <= t.dim() <= 4, "shape is (spatial dims..., channels) with 1 to 3 spatial dims"
+    return list(t.shape), t.dim() - 1
+
+
+class _TotalVariation(torch.autograd.Function):
+    """_functions.py:416-431: forward tv [..., 1], backward through the adjoint kernel."""
+
+    @staticmethod
+    def forward(ctx, x):
+        x = x.contiguous()
+        shape, dim = _spatial(x)
+        out = torch.empty(*shape[:-1], 1, dtype=torch.float32, device=x.device)
+        L.check(L.load().vr_total_variation_fw(x.data_ptr(), L.i64s(shape), dim, out.data_ptr(),
+                                               L.stream_handle(x.device)), "vr_total_variation_fw")
+        ctx.save_for_backward(x)
         return out
-    return r
+
+    @staticmethod
+    def backward(ctx, out_grad):
+        x, = ctx.saved_tensors
+        og = out_grad.contiguous()
+        shape, dim = _spatial(x)
+        grad = torch.empty_like(x)
+        L.check(L.load().vr_total_variation_bw(x.data_ptr(), og.data_ptr(), L.i64s(shape), dim, grad.data_ptr(),
+                                               L.stream_handle(x.device)), "vr_total_variation_bw")
+        return grad
+
+
+def total_variation(in_tensor: torch.Tensor) -> torch.Tensor:
+    """Per-voxel total variation [..., 1] of a grid [..., C] (_functions.py:434-435):
+    sum over the existing +-1 neighbours along every axis of sum_c |v - v_adj|."""
+    if not in_tensor.is_cuda:
+        raise RuntimeError("total_variation runs on the HIP device; no CPU fallback exists")
+    return _TotalVariation.apply(in_tensor)
+
+
+def _resample_step(x: torch.Tensor, dst: typing.Tuple[int, ...], out: typing.Optional[torch.Tensor]) -> torch.Tensor:
+    shape, dim = _spatial(x)
+    C = shape[-1]
+    if out is None:
+        out = torch.empty(*dst, C, dtype=torch.float32, device=x.device)
+    assert tuple(out.shape) == tuple(dst) + (C,) and out.is_contiguous()
+    L.check(L.load().vr_resample_grid(x.data_ptr(), L.i64s(shape[:-1]), L.i64s(dst), dim, C, out.data_ptr(),
+                                      L.stream_handle(x.device)), "vr_resample_grid")
+    return out
+
+
+def resample_grid(in_tensor: torch.Tensor, dst_shape: typing.Sequence[int],
+                  out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Vertex-aligned linear resampling of a grid [..., C] to dst_shape + (C,)
+    (_functions.py:287-295): steps of at most 2x up or down per axis until the
+    shape is reached; `out`, if given, receives the final step."""
+    if not in_tensor.is_cuda:
+        raise RuntimeError("resample_grid runs on the HIP device; no CPU fallback exists")
+    dst = tuple(int(v) for v in dst_shape)
+    x = in_tensor.contiguous()
+    while True:
+        cur = x.shape[:-1]
+        assert len(cur) == len(dst)
+        lo = tuple((d + 1) // 2 for d in cur)
+        hi = tuple(d * 2 for d in cur)
+        step = tuple(max(min(dst[i], hi[i]), lo[i]) for i in range(len(dst)))
+        if step == dst:
+            return _resample_step(x, dst, out)
+        x = _resample_step(x, step, None)

@@ -929,15 +929,12 @@ class SensorsBase(MapBase):
                                              *triggers)
 
     def random_sensors(self, batch_size: int, out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
-        """Uniform random sensor indices [batch_size, dims] (_maps.py:1751-1762)."""
-        dims = torch.tensor(self.index_shape, dtype=torch.float64)
-        r = (torch.rand(batch_size, len(self.index_shape), dtype=torch.float64) * dims).long()
-        r = torch.minimum(r, (dims - 1).long())
-        r = r.to(self.poses.device)
-        if out is not None:
-            out.copy_(r)
-            return out
-        return r
+        """Uniform random sensor indices [batch_size, dims] (_maps.py:1751-1762),
+        drawn on the device by functions.random_ids with the reference's stream."""
+        from . import functions
+        if out is None:
+            out = torch.empty(batch_size, len(self.index_shape), dtype=torch.int64, device=self.poses.device)
+        return functions.random_ids(batch_size, self.index_shape, out=out)
 
 
 class PerspectiveCameraSensor(SensorsBase):
