@@ -424,6 +424,8 @@ def grid_tools_leg(dev, res=512, reps=10):
         "tv_bw": (lambda: lib.vr_total_variation_bw(x.data_ptr(), og.data_ptr(), sh, 3, g.data_ptr(), st), 12.0 * V),
         "resample_2x": (lambda: lib.vr_resample_grid(half.data_ptr(), hs, dst, 3, 1, up.data_ptr(), st),
                         4.0 * (V + half.numel())),
+        # practical ceiling: a device copy of the same grid (reads and writes 4 B/voxel)
+        "copy": (lambda: (up.copy_(x), 0)[1], 8.0 * V),
     }
     out = {"workload": f"{res}^3 x 1 float32 grid: TV forward, TV backward, {res // 2}^3 -> {res}^3 resample"}
     stream = torch.cuda.current_stream(dev)

@@ -2046,6 +2046,7 @@ int oracle_resample_grid(const float* in, const int64_t* in_shape, const int64_t
     int64_t V = 1;
     for (int i = 0; i < dim; i++) {
         if (in_shape[i] < 1 || out_shape[i] < 2) return fail(VR_ERR_ARG, "resample: output extents must be >= 2");
+        if ((out_shape[i] - 1) * (in_shape[i] - 1) >= ((int64_t)1 << 31)) return fail(VR_ERR_ARG, "resample: sizes too large");
         V *= out_shape[i];
     }
     for (int64_t index = 0; index < V; index++) {

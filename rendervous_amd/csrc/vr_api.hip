@@ -827,6 +827,21 @@ int make_shape3(const int64_t* shape, int dim, int64_t channels, fn::Shape3& S, 
 
 inline dim3 shape_grid(const fn::Shape3& S) { return dim3((unsigned)((int64_t)S.xb * S.n[1] * S.n[2])); }
 
+// Single-channel 3-D tensors with rows of whole float4s take the x4 kernels.
+bool dense3_of(const fn::Shape3& S, int dim, const void* a, const void* b, const void* c, fn::Dense3& D) {
+    const auto aligned = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+    if (dim != 3 || S.C != 1 || (S.n[0] & 3) || !aligned(a) || !aligned(b) || !aligned(c)) return false;
+    if ((S.n[1] + fn::kRowsY - 1) / fn::kRowsY > 65535 || S.n[2] > 65535) return false;
+    D.nx = S.n[0]; D.ny = S.n[1]; D.nz = S.n[2];
+    D.sy = S.s[1]; D.sz = S.s[2];
+    return true;
+}
+
+inline dim3 dense3_grid(const fn::Dense3& D) {
+    return dim3((unsigned)((D.nx / 4 + fn::kQuadX - 1) / fn::kQuadX), (unsigned)((D.ny + fn::kRowsY - 1) / fn::kRowsY),
+                (unsigned)D.nz);
+}
+
 }  // namespace
 
 extern "C" {
@@ -854,6 +869,11 @@ int vr_total_variation_fw(const float* in, const int64_t* shape, int32_t dim, fl
     int st = make_shape3(shape, dim, shape && dim >= 1 && dim <= 3 ? shape[dim] : 0, S, "vr_total_variation_fw");
     if (st) return st;
     if (!in || !out) return set_err(VR_ERR_ARG, "vr_total_variation_fw: null buffer");
+    fn::Dense3 D;
+    if (dense3_of(S, dim, in, out, out, D)) {
+        hipLaunchKernelGGL(fn::k_tv_fw_x4, dense3_grid(D), dim3(fn::kQuadX, fn::kRowsY), 0, (hipStream_t)stream, in, D, out);
+        return check_hip(hipGetLastError(), "vr_total_variation_fw launch");
+    }
     hipLaunchKernelGGL(fn::k_tv_fw, shape_grid(S), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, S, (int)dim, out);
     return check_hip(hipGetLastError(), "vr_total_variation_fw launch");
 }
@@ -864,6 +884,12 @@ int vr_total_variation_bw(const float* in, const float* out_grad, const int64_t*
     int st = make_shape3(shape, dim, shape && dim >= 1 && dim <= 3 ? shape[dim] : 0, S, "vr_total_variation_bw");
     if (st) return st;
     if (!in || !out_grad || !in_grad) return set_err(VR_ERR_ARG, "vr_total_variation_bw: null buffer");
+    fn::Dense3 D;
+    if (dense3_of(S, dim, in, out_grad, in_grad, D)) {
+        hipLaunchKernelGGL(fn::k_tv_bw_x4, dense3_grid(D), dim3(fn::kQuadX, fn::kRowsY), 0, (hipStream_t)stream, in, out_grad,
+                           D, in_grad);
+        return check_hip(hipGetLastError(), "vr_total_variation_bw launch");
+    }
     hipLaunchKernelGGL(fn::k_tv_bw, shape_grid(S), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, out_grad, S, (int)dim,
                        in_grad);
     return check_hip(hipGetLastError(), "vr_total_variation_bw launch");
@@ -878,11 +904,23 @@ int vr_resample_grid(const float* in, const int64_t* in_shape, const int64_t* ou
     if ((st = make_shape3(out_shape, dim, channels, R.out, "vr_resample_grid"))) return st;
     for (int i = 0; i < 3; i++) {
         if (i < dim && R.out.n[i] < 2) return set_err(VR_ERR_ARG, "vr_resample_grid: output extents must be >= 2");
+        // the source coordinate is formed as the int p * (in - 1), as in the shader
+        if ((int64_t)(R.out.n[i] - 1) * (I.n[i] - 1) >= (int64_t(1) << 31))
+            return set_err(VR_ERR_ARG, "vr_resample_grid: (out - 1) * (in - 1) must stay below 2^31");
         R.in_n[i] = I.n[i];
         R.in_s[i] = I.s[i];
     }
     R.dim = dim;
     if (!in || !out) return set_err(VR_ERR_ARG, "vr_resample_grid: null buffer");
+    fn::Resample3 R3;
+    if (dense3_of(R.out, dim, out, out, out, R3.out)) {
+        for (int i = 0; i < 3; i++) R3.in_n[i] = I.n[i];
+        R3.in_sy = I.s[1];
+        R3.in_sz = I.s[2];
+        hipLaunchKernelGGL(fn::k_resample_x4, dense3_grid(R3.out), dim3(fn::kQuadX, fn::kRowsY), 0, (hipStream_t)stream, in, R3,
+                           out);
+        return check_hip(hipGetLastError(), "vr_resample_grid launch");
+    }
     hipLaunchKernelGGL(fn::k_resample, shape_grid(R.out), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, R, out);
     return check_hip(hipGetLastError(), "vr_resample_grid launch");
 }

@@ -46,8 +46,10 @@ def test_random_sensors_follow_the_seed(gpu):
     torch.testing.assert_close(img, full.reshape(-1, 1)[idx])
 
 
-@pytest.mark.parametrize("shape", [(4097, 1), (33, 300, 3), (17, 40, 300, 1), (9, 10, 11, 4), (5, 6, 7, 6)])
+@pytest.mark.parametrize("shape", [(4097, 1), (33, 300, 3), (17, 40, 300, 1), (9, 10, 11, 4), (5, 6, 7, 6),
+                                   (16, 24, 64, 1), (3, 9, 4, 1), (2, 2, 1028, 1), (7, 5, 3, 1)])
 def test_total_variation_bit_exact(gpu, shape):
+    """Generic kernels and the single-channel 3-D x4 kernels (rows of whole float4s)."""
     rng = np.random.default_rng(len(shape) + shape[-1])
     x = rng.normal(size=shape).astype(np.float32)
     x[..., :1][x[..., :1] > 1.5] = 0.0  # some ties
@@ -83,7 +85,8 @@ def test_total_variation_autograd_matches_torch(gpu):
 
 
 @pytest.mark.parametrize("src,dst,C", [((300,), (599,), 1), ((33, 40), (65, 21), 3), ((16, 20, 24), (31, 39, 47), 1),
-                                       ((64, 64, 64), (32, 33, 34), 2)])
+                                       ((64, 64, 64), (32, 33, 34), 2), ((16, 20, 24), (31, 39, 48), 1),
+                                       ((8, 8, 8), (16, 16, 16), 1), ((40, 30, 20), (20, 15, 12), 1)])
 def test_resample_step_matches_oracle(gpu, src, dst, C):
     x = np.random.default_rng(C + len(src)).normal(size=src + (C,)).astype(np.float32)
     xd = torch.from_numpy(x).cuda()
