@@ -85,10 +85,23 @@ def render(camera: M.PerspectiveCameraSensor, radiance: M.MapBase, spp: int) -> 
 
 
 def optimisation_step(camera, radiance, sigma_param: torch.nn.Parameter, target: torch.Tensor, spp: int,
-                      opt: FusedAdam, clamp_min: float = 0.0) -> torch.Tensor:
-    """One Adam step of the L2 image loss; returns the loss as a 0-d tensor (no host sync)."""
-    img = render(camera, radiance, spp)
-    loss = ((img - target) ** 2).mean()
+                      opt: FusedAdam, clamp_min: float = 0.0, batch: typing.Optional[int] = None,
+                      tv_weight: float = 0.0) -> torch.Tensor:
+    """One Adam step of the L2 image loss (plus tv_weight * mean total variation of
+    the density); returns the loss as a 0-d tensor (no host sync).  batch = B
+    renders B random sensors (camera.random_sensors) against the matching target
+    pixels instead of every view, as the reference's minibatched inverse renders."""
+    if batch is None:
+        img = render(camera, radiance, spp)
+        ref = target
+    else:
+        sensors = camera.random_sensors(batch)
+        img = camera.capture(radiance, sensors, fw_samples=spp, bw_samples=spp)
+        ref = target[sensors[:, 0], sensors[:, 1], sensors[:, 2]]
+    loss = ((img - ref) ** 2).mean()
+    if tv_weight:
+        from .functions import total_variation
+        loss = loss + tv_weight * total_variation(sigma_param).mean()
     loss.backward()
     opt.step(sigma_param.grad, zero_grad=False, bounds=None if clamp_min is None else (clamp_min, math.inf))
     sigma_param.grad = None  # the next backward's fresh gradient becomes .grad without an extra add
