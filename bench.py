@@ -456,6 +456,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the gradient and DDA legs")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--leg", choices=["gradient", "nee_gradient", "nrc", "dda", "grid_tools"],
+                    help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -470,6 +472,11 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.leg is not None:
+        legs = {"gradient": gradient_leg, "nee_gradient": nee_gradient_leg, "nrc": nrc_leg, "dda": dda_leg,
+                "grid_tools": grid_tools_leg}
+        print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
+        return
     if args.calibrate_copy:
         a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
