@@ -65,6 +65,18 @@ enum vr_transmittance_kind { /* estimator behind transmittance() in NEE / single
     VR_T_GRT = 4             /* include/vr_transmittance_GRT.h (no backward in the reference) */
 };
 
+enum vr_env_kind {           /* environment(x, w) */
+    VR_ENV_CONST = 0,        /* const[r,g,b] (env_c) */
+    VR_ENV_XR = 1            /* Image2D(env_image).after(xr_projection): maps/environment_xr.h,
+                                _objects.py:164-173 (no gradient into the image) */
+};
+
+enum vr_env_sampler_kind {   /* the environment part of environment_sampler(x, w) */
+    VR_ENVS_UNIFORM = 0,     /* _functionsampler(UniformDirectionSampler, environment) */
+    VR_ENVS_QUADTREE = 1     /* _functionsampler(XRQuadtreeRandomDirection, environment),
+                                _maps.py:2494-2549, _objects.py:180-199 */
+};
+
 enum vr_map_kind {           /* input -> output dims */
     VR_MAP_RAYBOX = 1,       /* 6 -> 2   RayBoxIntersection          _maps.py:4128-4154 */
     VR_MAP_T_DDA = 2,        /* 6 -> 1   GridDDATransmittance        _maps.py:2640-2667 */
@@ -145,6 +157,14 @@ typedef struct vr_medium {
     int32_t medium_filter;         /* DTMediumPathIntegrator MEDIUM_FILTER: bit0 scatter, bit1 gather */
     float   gathering_c[3];        /* constant gathering map of the 7->12 integrator */
     float   medium_majorant;       /* scalar majorant of the 7->12 integrator */
+    int32_t env_kind;              /* vr_env_kind */
+    int32_t env_sampler_kind;      /* vr_env_sampler_kind */
+    vr_tensor env_image;           /* VR_ENV_XR: [H, W, 3] (maps/image2d.h) */
+    float   env_image_bmin[2];     /* Image2D bmin, bmax over xr coordinates (-1, 1 in the recipes) */
+    float   env_image_bmax[2];
+    const float* env_quadtree;     /* VR_ENVS_QUADTREE: functions.create_density_quadtree output */
+    int32_t env_quadtree_levels;   /* 1 .. 15 */
+    int32_t env_reserved;
 } vr_medium;
 
 /* PerspectiveCameraSensor (_maps.py:1765-1786, include/maps/sensor_perspective_camera.h). */
@@ -259,6 +279,20 @@ int vr_total_variation_bw(const float* in, const float* out_grad, const int64_t*
  * out_shape entry >= 2 (the shader divides by out - 1). */
 int vr_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
                      float* out, void* stream);
+
+/* One pixel-centred linear resampling step of functions.resample_img
+ * (_functions.py:300-345, functions/tools/resample_img.comp.glsl): source
+ * coordinate (p + 1/2) / out * in - 1/2 per axis, both taps clamped to the
+ * extent.  1 <= dim <= 3; every out_shape entry >= 1. */
+int vr_resample_img(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                    float* out, void* stream);
+
+/* functions.copy_img_to_morton / copy_morton_to_img (_functions.py:434-506,
+ * functions/tools/copy_*_morton.comp.glsl): a square [R, R, C] image <-> [R*R, C]
+ * in Morton (Z) order, pixel (px, py) = (even bits, odd bits) of the index.
+ * R a power of two up to 2^15. */
+int vr_copy_img_to_morton(const float* in, int64_t resolution, int32_t channels, float* out, void* stream);
+int vr_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels, float* out, void* stream);
 
 /* ---- neural radiance cache ---------------------------------------------------
  * No reference counterpart (the north star's cached renderer): a 96-64-64-64-64-3

@@ -63,6 +63,11 @@ def load():
                                                          ctypes.c_void_p]),
             "oracle_resample_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                     ctypes.c_int32, ctypes.c_void_p]),
+            "oracle_resample_img": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                   ctypes.c_int32, ctypes.c_void_p]),
+            "oracle_copy_img_to_morton": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
+            "oracle_copy_morton_to_img": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
+            "oracle_env_sample": (ctypes.c_int, [P(L.VrMedium), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
             "oracle_rng_uniforms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
             "oracle_rng_seed_states": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                                       ctypes.c_void_p]),

@@ -365,11 +365,92 @@ static double phase_sample(ctx_t* c, v3 w, v3* wo) {
     }
 }
 
-/* environment_sampler(x, w) -> (L/pdf, wo):
- * _functionsampler(UniformDirectionSampler, const env) (_maps.py:2552-2585,
- * maps/uniform_direction_sampler.h), optionally mixed with a directional light
- * through MixtureSampler semantics (_maps.py:1541-1571; sub-samplers pre-scaled
- * by their selection probability). */
+/* maths.h:172-178 dir2xr */
+static void dir2xr(v3 w, double* u, double* v) {
+    if (w.x == 0.0 && w.z == 0.0) w.x += 0.0000001;
+    *u = atan2(w.x, w.z) / PI_D;
+    *v = -2.0 * asin(dmin(dmax(w.y, -1.0), 1.0)) / PI_D;
+}
+
+/* maps/image2d.h:1-25: pixel-centred bilinear lookup, coordinates clamped to the
+ * box, corner indices clamped to the image. */
+static v3 image_lookup(const vr_medium* M, double cx, double cy) {
+    const vr_tensor* t = &M->env_image;
+    int H = (int)t->shape[0], W = (int)t->shape[1];
+    double nx = dmin(dmax((cx - M->env_image_bmin[0]) / ((double)M->env_image_bmax[0] - M->env_image_bmin[0]), 0.0), 1.0);
+    double ny = dmin(dmax((cy - M->env_image_bmin[1]) / ((double)M->env_image_bmax[1] - M->env_image_bmin[1]), 0.0), 1.0);
+    double gx = nx * W - 0.5, gy = ny * H - 0.5;
+    int px = (int)floor(gx), py = (int)floor(gy);
+    double ax = gx - px, ay = gy - py;
+    int xs[2] = {iclamp(px, 0, W - 1), iclamp(px + 1, 0, W - 1)};
+    int ys[2] = {iclamp(py, 0, H - 1), iclamp(py + 1, 0, H - 1)};
+    double r[3];
+    for (int ch = 0; ch < 3; ch++) {
+        double v00 = t->data[ys[0] * t->stride[0] + xs[0] * t->stride[1] + ch * t->stride[2]];
+        double v01 = t->data[ys[0] * t->stride[0] + xs[1] * t->stride[1] + ch * t->stride[2]];
+        double v10 = t->data[ys[1] * t->stride[0] + xs[0] * t->stride[1] + ch * t->stride[2]];
+        double v11 = t->data[ys[1] * t->stride[0] + xs[1] * t->stride[1] + ch * t->stride[2]];
+        r[ch] = (v00 * (1 - ax) + v01 * ax) * (1 - ay) + (v10 * (1 - ax) + v11 * ax) * ay;
+    }
+    return V(r[0], r[1], r[2]);
+}
+
+/* environment(x, w): constant, or Image2D over dir2xr(w) (maps/environment_xr.h:3-9). */
+static v3 env_eval(const vr_medium* M, v3 w) {
+    if (M->env_kind == VR_ENV_XR) {
+        double u, v;
+        dir2xr(w, &u, &v);
+        return image_lookup(M, u, v);
+    }
+    return vf(M->env_c);
+}
+
+/* XRQuadtreeRandomDirection (_maps.py:2494-2549): quadtree descent with one
+ * uniform over the absolute node probabilities, then randomDirection(a0, a1,
+ * b0, b1) (randoms.h:186-199) in the selected pixel; returns 1 / pdf. */
+static double quadtree_direction(ctx_t* c, v3* wo) {
+    const vr_medium* M = c->M;
+    double sel = rnd(c);
+    int node = 0;
+    double p0x = 0, p0y = 0, p1x = 1, p1y = 1, prob = 1;
+    for (int i = 0; i < M->env_quadtree_levels; i++) {
+        const float* d = M->env_quadtree + node * 4;
+        int child = 3;
+        prob = d[3];
+        for (int k = 0; k < 3; k++) {
+            if (sel < d[k]) { child = k; prob = d[k]; break; }
+            sel -= d[k];
+        }
+        double xm = (p1x + p0x) / 2, ym = (p1y + p0y) / 2;
+        if (child % 2 == 0) p1x = xm; else p0x = xm;
+        if (child / 2 == 0) p1y = ym; else p0y = ym;
+        node = node * 4 + 1 + child;
+    }
+    double area = 2 * PI_D * (cos(p0y * PI_D) - cos(p1y * PI_D)) / (double)(1 << M->env_quadtree_levels);
+    double weight = area / dmax(0.000000001, prob);
+    double a0 = (p0x * 2 - 1) * PI_D, a1 = (p1x * 2 - 1) * PI_D, b0 = p0y * PI_D, b1 = p1y * PI_D;
+    double r1 = rnd(c) * (a1 - a0) + a0;
+    double r2 = cos(b0) - rnd(c) * (cos(b0) - cos(b1));
+    double s = sqrt(dmax(0.0, 1.0 - r2 * r2));
+    *wo = V(sin(r1) * s, r2, cos(r1) * s);
+    return weight;
+}
+
+/* _functionsampler(UniformDirectionSampler or XRQuadtreeRandomDirection,
+ * environment) (_maps.py:2552-2585): L(wo) / pdf. */
+static v3 env_sample_base(ctx_t* c, v3* wo) {
+    const vr_medium* M = c->M;
+    if (M->env_sampler_kind == VR_ENVS_QUADTREE) {
+        double wgt = quadtree_direction(c, wo);
+        return vs(env_eval(M, *wo), wgt);
+    }
+    *wo = random_direction(c, V(0, 1, 0));
+    return vs(env_eval(M, *wo), 4 * PI_D);
+}
+
+/* environment_sampler(x, w) -> (L/pdf, wo), optionally mixed with a
+ * directional light through MixtureSampler semantics (_maps.py:1541-1571;
+ * sub-samplers pre-scaled by their selection probability). */
 static v3 env_sample(ctx_t* c, v3* wo) {
     const vr_medium* M = c->M;
     if (M->sun_prob > 0) {
@@ -377,11 +458,9 @@ static v3 env_sample(ctx_t* c, v3* wo) {
             *wo = vf(M->sun_dir);
             return vs(vf(M->sun_radiance), 1.0 / M->sun_prob);
         }
-        *wo = random_direction(c, V(0, 1, 0));
-        return vs(vf(M->env_c), 4 * PI_D / (1.0 - M->sun_prob));
+        return vs(env_sample_base(c, wo), 1.0 / (1.0 - M->sun_prob));
     }
-    *wo = random_direction(c, V(0, 1, 0));
-    return vs(vf(M->env_c), 4 * PI_D);
+    return env_sample_base(c, wo);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -937,13 +1016,13 @@ static v3 mc_collision(ctx_t* c, v3 x, v3 w) {
     double T, t;
     double C = collision_dt(c, x, w, &t, &T);
     v3 xt = vadd(vs(w, t), x);
-    v3 R = vs(vf(c->M->env_c), T);
+    v3 R = vmul(env_eval(c->M, w), V(T, T, T));
     if (C != 0.0) R = vadd(R, vs(ss_radiance(c, xt, vs(w, -1.0)), C));
     return R;
 }
 
 static void mc_collision_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
-    v3 env = vf(c->M->env_c);
+    v3 env = env_eval(c->M, w);
     if (c->bw_mode == VR_BW_REFERENCE) {
         /* collision_integrator_mc.h:30-44: forward collision + exitance, then the
          * collision sampler's own defensive backward with dL_dC = <dL_dR, eR>. */
@@ -987,7 +1066,7 @@ static v3 nee_fw(ctx_t* c, v3 x, v3 w) {
     x = vadd(x, vs(w, tmin));
     double d = tmax - tmin;
     v3 R = V(0, 0, 0), W = V(1, 1, 1);
-    R = vadd(R, vs(vf(M->env_c), transmittance(c, x, w)));
+    R = vadd(R, vs(env_eval(M, w), transmittance(c, x, w)));
     const double EPS = 0.00001;
     int bounces = 0;
     maj_t mj;
@@ -1076,7 +1155,7 @@ static void nee_ds_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
         boundary(M, x, w, &tmin, &d);
         maj_begin(M, &mj, x, w);
     }
-    if (!scattered) R = vadd(R, vmul(vf(M->env_c), W));
+    if (!scattered) R = vadd(R, vmul(env_eval(M, w), W));
 
     /* replay */
     c->rng = before_path;
@@ -1180,7 +1259,7 @@ static v3 path_dt_fw(ctx_t* c, v3 x, v3 w) {
         boundary(M, x, w, &tmin, &d);
         maj_begin(M, &mj, x, w);
     }
-    return vadd(vmul(W, vf(M->env_c)), R);
+    return vadd(vmul(W, env_eval(M, w)), R);
 }
 
 /* vr_path_integrator_DT.h:59-235: a defensive-sampling pass (majorant + 2 eps,
@@ -1228,7 +1307,7 @@ static void path_dt_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
         boundary(M, x, w, &tmin, &d);
         maj_begin(M, &mj, x, w);
     }
-    v3 dL_dW = vmul(dL_dR, vf(M->env_c));
+    v3 dL_dW = vmul(dL_dR, env_eval(M, w));
     v3 W_out = W;
 
     c->rng = before_path;
@@ -1369,7 +1448,7 @@ static void nee_drt_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
 
     double direct_T = transmittance(c, x, w);
     (void)direct_T;  /* dL/denv = dL_dR * direct_T: a constant environment has no parameters */
-    double dL_ddT = vdot(dL_dR, vf(M->env_c));
+    double dL_ddT = vdot(dL_dR, env_eval(M, w));
     main_s = c->rng; c->rng = secondary;
     transmittance_bw(c, x, w, dL_ddT);
     secondary = c->rng; c->rng = main_s;
@@ -1630,6 +1709,15 @@ static int check_medium(const vr_medium* M, int kind) {
         for (int a = 0; a < 3; a++) if (M->sigma.t.shape[a] < 2) return fail(VR_ERR_ARG, "sigma grid needs >= 2 vertices per axis");
     }
     if (M->majorant_kind == VR_MAJ_GRID && !M->majorant_grid) return fail(VR_ERR_ARG, "majorant grid is NULL");
+    if (M->env_kind == VR_ENV_XR && (!M->env_image.data || M->env_image.ndim != 3 || M->env_image.shape[2] != 3 ||
+                                     M->env_image.shape[0] < 1 || M->env_image.shape[1] < 1))
+        return fail(VR_ERR_ARG, "environment image must be [H, W, 3]");
+    if (M->env_kind != VR_ENV_XR && M->env_kind != VR_ENV_CONST) return fail(VR_ERR_ARG, "unknown environment kind");
+    if (M->env_sampler_kind == VR_ENVS_QUADTREE &&
+        (!M->env_quadtree || M->env_quadtree_levels < 1 || M->env_quadtree_levels > 15))
+        return fail(VR_ERR_ARG, "bad environment quadtree");
+    if (M->env_sampler_kind != VR_ENVS_QUADTREE && M->env_sampler_kind != VR_ENVS_UNIFORM)
+        return fail(VR_ERR_ARG, "unknown environment sampler kind");
     if (M->albedo_kind == VR_FIELD_GRID && (!M->albedo.t.data || M->albedo.t.ndim != 4))
         return fail(VR_ERR_ARG, "albedo grid must be [D,H,W,C]");
     return VR_OK;
@@ -2038,14 +2126,15 @@ int oracle_total_variation_bw(const float* in, const float* out_grad, const int6
     return VR_OK;
 }
 
-/* functions/tools/resample_grid.comp.glsl:14-86 (float arithmetic, taps and
- * weights in the shader's order). */
-int oracle_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
-                         float* out) {
+/* functions/tools/resample_grid.comp.glsl:14-86 (vertex-aligned, cell == 0)
+ * and resample_img.comp.glsl:14-86 (pixel-centred, cell == 1): float
+ * arithmetic, taps and weights in the shaders' order. */
+static int resample_any(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                        float* out, int cell) {
     if (!in_shape || !out_shape || dim < 1 || dim > 3 || channels < 1) return fail(VR_ERR_ARG, "bad resample arguments");
     int64_t V = 1;
     for (int i = 0; i < dim; i++) {
-        if (in_shape[i] < 1 || out_shape[i] < 2) return fail(VR_ERR_ARG, "resample: output extents must be >= 2");
+        if (in_shape[i] < 1 || out_shape[i] < (cell ? 1 : 2)) return fail(VR_ERR_ARG, "resample: output extents too small");
         if ((out_shape[i] - 1) * (in_shape[i] - 1) >= ((int64_t)1 << 31)) return fail(VR_ERR_ARG, "resample: sizes too large");
         V *= out_shape[i];
     }
@@ -2056,7 +2145,7 @@ int oracle_resample_grid(const float* in, const int64_t* in_shape, const int64_t
         for (int i = 0; i < dim; i++) {
             int id = (int)in_shape[dim - i - 1], od = (int)out_shape[dim - i - 1];
             int pc = (int)(r % od);
-            float coord = (float)(pc * (id - 1)) / (float)(od - 1);
+            float coord = cell ? ((float)pc + 0.5f) / (float)od * (float)id - 0.5f : (float)(pc * (id - 1)) / (float)(od - 1);
             ic[i] = (int)floorf(coord);
             al[i] = coord - (float)ic[i];
             r /= od;
@@ -2071,11 +2160,79 @@ int oracle_resample_grid(const float* in, const int64_t* in_shape, const int64_t
                 w = i == 0 ? f : w * f;
                 int d = (int)in_shape[dim - i - 1];
                 int cc = ic[i] + dx < d - 1 ? ic[i] + dx : d - 1;
+                if (cc < 0) cc = 0;
                 src += (int64_t)cc * u;
                 u *= d;
             }
             for (int c = 0; c < channels; c++) out[index * channels + c] += w * in[src * channels + c];
         }
+    }
+    return VR_OK;
+}
+
+int oracle_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                         float* out) {
+    return resample_any(in, in_shape, out_shape, dim, channels, out, 0);
+}
+
+int oracle_resample_img(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                        float* out) {
+    return resample_any(in, in_shape, out_shape, dim, channels, out, 1);
+}
+
+/* maths.h:244-251 unpart1by1: the even bits of n, compacted. */
+static int unpart1by1(int n) {
+    n &= 0x55555555;
+    n = (n ^ (n >> 1)) & 0x33333333;
+    n = (n ^ (n >> 2)) & 0x0f0f0f0f;
+    n = (n ^ (n >> 4)) & 0x00ff00ff;
+    n = (n ^ (n >> 8)) & 0x0000ffff;
+    return n;
+}
+
+static int morton_check(int64_t resolution, int32_t channels) {
+    if (resolution < 1 || resolution > (1 << 15) || (resolution & (resolution - 1)) || channels < 1)
+        return fail(VR_ERR_ARG, "morton copy: resolution must be a power of two up to 2^15");
+    return VR_OK;
+}
+
+/* functions/tools/copy_img_to_morton.comp.glsl: out[i] = img[py][px], (px, py) = (even, odd) bits of i. */
+int oracle_copy_img_to_morton(const float* in, int64_t resolution, int32_t channels, float* out) {
+    int st = morton_check(resolution, channels);
+    if (st) return st;
+    for (int64_t i = 0; i < resolution * resolution; i++) {
+        int px = unpart1by1((int)i), py = unpart1by1((int)(i >> 1));
+        for (int c = 0; c < channels; c++) out[i * channels + c] = in[((int64_t)py * resolution + px) * channels + c];
+    }
+    return VR_OK;
+}
+
+/* functions/tools/copy_morton_to_img.comp.glsl: the inverse permutation. */
+int oracle_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels, float* out) {
+    int st = morton_check(resolution, channels);
+    if (st) return st;
+    for (int64_t i = 0; i < resolution * resolution; i++) {
+        int px = unpart1by1((int)i), py = unpart1by1((int)(i >> 1));
+        for (int c = 0; c < channels; c++) out[((int64_t)py * resolution + px) * channels + c] = in[i * channels + c];
+    }
+    return VR_OK;
+}
+
+/* Oracle-only probe of the environment sampler (no device twin): sample i
+ * draws from the map-eval stream of thread i and returns (wo, L(wo) / pdf). */
+int oracle_env_sample(const vr_medium* M, const uint32_t seeds[4], int64_t n, float* out) {
+    int st = check_medium(M, VR_MAP_RAYBOX);
+    if (st) return st;
+    if (!seeds || !out || n < 0) return fail(VR_ERR_ARG, "bad env_sample arguments");
+    for (int64_t i = 0; i < n; i++) {
+        ctx_t c;
+        memset(&c, 0, sizeof(c));
+        c.M = M;
+        c.rng = seed_map(seeds, (uint32_t)i);
+        v3 wo;
+        v3 L = env_sample(&c, &wo);
+        out[6 * i + 0] = (float)wo.x; out[6 * i + 1] = (float)wo.y; out[6 * i + 2] = (float)wo.z;
+        out[6 * i + 3] = (float)L.x; out[6 * i + 4] = (float)L.y; out[6 * i + 5] = (float)L.z;
     }
     return VR_OK;
 }

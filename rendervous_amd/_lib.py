@@ -22,6 +22,8 @@ VR_FIELD_CONST, VR_FIELD_GRID = 0, 1
 VR_PHASE_HG_MIX, VR_PHASE_HG, VR_PHASE_ISO = 0, 1, 2
 VR_MAJ_CONST, VR_MAJ_GRID = 0, 1
 VR_T_DT, VR_T_RT, VR_T_DDA, VR_T_GDT, VR_T_GRT = 0, 1, 2, 3, 4
+VR_ENV_CONST, VR_ENV_XR = 0, 1
+VR_ENVS_UNIFORM, VR_ENVS_QUADTREE = 0, 1
 VR_MAP_RAYBOX = 1
 VR_MAP_T_DDA = 2
 VR_MAP_T_DT = 3
@@ -98,6 +100,14 @@ class VrMedium(ctypes.Structure):
         ("medium_filter", ctypes.c_int32),
         ("gathering_c", F3),
         ("medium_majorant", ctypes.c_float),
+        ("env_kind", ctypes.c_int32),
+        ("env_sampler_kind", ctypes.c_int32),
+        ("env_image", VrTensor),
+        ("env_image_bmin", ctypes.c_float * 2),
+        ("env_image_bmax", ctypes.c_float * 2),
+        ("env_quadtree", ctypes.c_void_p),
+        ("env_quadtree_levels", ctypes.c_int32),
+        ("env_reserved", ctypes.c_int32),
     ]
 
 
@@ -154,6 +164,12 @@ _SIGNATURES = {
                                              ctypes.c_void_p, ctypes.c_void_p]),
     "vr_resample_grid": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_resample_img": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_copy_img_to_morton": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
+    "vr_copy_morton_to_img": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     "vr_nrc_encode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, P(ctypes.c_float), P(ctypes.c_float),
                                      ctypes.c_void_p, ctypes.c_void_p]),
     "vr_nrc_mlp_fw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -93,6 +93,40 @@ int map_dims(int kind, int& in_dim, int& out_dim) {
     }
 }
 
+int make_env_view(const vr_medium* m, MediumView& v) {
+    v.env_kind = m->env_kind;
+    if (m->env_kind == VR_ENV_XR) {
+        const vr_tensor& t = m->env_image;
+        if (!t.data || t.ndim != 3 || t.shape[2] != 3) return set_err(VR_ERR_ARG, "environment image must be [H, W, 3]");
+        if (t.shape[0] < 1 || t.shape[1] < 1 || t.shape[0] > (1 << 16) || t.shape[1] > (1 << 16))
+            return set_err(VR_ERR_ARG, "environment image extent out of range");
+        int64_t span = 1;
+        for (int i = 0; i < 3; i++) {
+            if (t.stride[i] < 0) return set_err(VR_ERR_ARG, "environment image: negative strides are not supported");
+            span += (t.shape[i] - 1) * t.stride[i];
+        }
+        if (span >= (int64_t(1) << 31)) return set_err(VR_ERR_ARG, "environment image spans more than 2^31 elements");
+        ImageView& I = v.env_img;
+        I.data = t.data;
+        I.h = (int)t.shape[0]; I.w = (int)t.shape[1];
+        I.sy = (int)t.stride[0]; I.sx = (int)t.stride[1]; I.sc = (int)t.stride[2];
+        for (int a = 0; a < 2; a++)
+            if (!(m->env_image_bmax[a] > m->env_image_bmin[a])) return set_err(VR_ERR_ARG, "environment image box is empty");
+        I.bmin_x = m->env_image_bmin[0]; I.bmin_y = m->env_image_bmin[1];
+        I.ibs_x = 1.0f / (m->env_image_bmax[0] - m->env_image_bmin[0]);
+        I.ibs_y = 1.0f / (m->env_image_bmax[1] - m->env_image_bmin[1]);
+    } else if (m->env_kind != VR_ENV_CONST) return set_err(VR_ERR_ARG, "unknown environment kind");
+    v.env_sampler = m->env_sampler_kind;
+    if (m->env_sampler_kind == VR_ENVS_QUADTREE) {
+        if (!m->env_quadtree) return set_err(VR_ERR_ARG, "environment quadtree is NULL");
+        if (m->env_quadtree_levels < 1 || m->env_quadtree_levels > 15)
+            return set_err(VR_ERR_ARG, "environment quadtree levels must be in [1, 15]");
+        v.quadtree = m->env_quadtree;
+        v.qt_levels = m->env_quadtree_levels;
+    } else if (m->env_sampler_kind != VR_ENVS_UNIFORM) return set_err(VR_ERR_ARG, "unknown environment sampler kind");
+    return VR_OK;
+}
+
 int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
     if (!m) return set_err(VR_ERR_ARG, "medium is NULL");
     std::memset(&v, 0, sizeof(v));
@@ -143,6 +177,7 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
     v.medium_majorant = m->medium_majorant;
     if (bw_mode != VR_BW_REFERENCE && bw_mode != VR_BW_CONSISTENT) return set_err(VR_ERR_ARG, "unknown bw mode");
     v.bw_mode = bw_mode;
+    if ((st = make_env_view(m, v))) return st;
     return VR_OK;
 }
 
@@ -508,7 +543,7 @@ int persistent_blocks(K kernel, int* blocks) {
     return VR_OK;
 }
 
-template <int MAJ, int TKIND>
+template <int MAJ, int TKIND, bool ENVMAP>
 int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
                           int64_t cnt, float* out, int* dbg, hipStream_t st) {
     if (total > (int64_t)UINT32_MAX) return set_err(VR_ERR_UNSUPPORTED, "capture has more than 2^32 sensors");
@@ -516,7 +551,7 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
     int rc = work_counter(&counter);
     if (rc) return rc;
     int blocks = 0;
-    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND>, &blocks))) return rc;
+    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP>, &blocks))) return rc;
     const int64_t lanes = (int64_t)blocks * kBlock;
     const int64_t hw = (int64_t)C.width * C.height;
     // launches of at most 2^30 sensors (whole images when possible) keep item indices in 32 bits
@@ -542,7 +577,8 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         unsigned prep = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
         hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, b0, n, dbg, counter);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
-        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, S, counter, out, dbg);
+        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, S, counter, out,
+                           dbg);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw launch"))) return rc;
     }
     return VR_OK;
@@ -606,8 +642,14 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
                           int64_t cnt, float* out, int* dbg, hipStream_t st) {
     // the NEE forward (also DRTPathIntegrator's) with a DT or RT shadow estimator runs persistent
     const bool nee = kind == VR_MAP_NEE_DT || kind == VR_MAP_NEE_DRT;
-    if (nee && M.t_kind == VR_T_DT) return launch_nee_persistent<MAJ, VR_T_DT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
-    if (nee && M.t_kind == VR_T_RT) return launch_nee_persistent<MAJ, VR_T_RT>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+    if (nee && (M.t_kind == VR_T_DT || M.t_kind == VR_T_RT)) {
+        const bool envmap = M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM;
+        if (M.t_kind == VR_T_DT)
+            return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true>(M, C, sd, spp, total, start, cnt, out, dbg, st)
+                          : launch_nee_persistent<MAJ, VR_T_DT, false>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+        return envmap ? launch_nee_persistent<MAJ, VR_T_RT, true>(M, C, sd, spp, total, start, cnt, out, dbg, st)
+                      : launch_nee_persistent<MAJ, VR_T_RT, false>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+    }
     int in_dim = 0, od = 0;
     map_dims(kind, in_dim, od);
     const CaptureSplit P = capture_split(spp, false);
@@ -895,34 +937,72 @@ int vr_total_variation_bw(const float* in, const float* out_grad, const int64_t*
     return check_hip(hipGetLastError(), "vr_total_variation_bw launch");
 }
 
-int vr_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
-                     float* out, void* stream) {
+}  // extern "C"
+
+namespace {
+
+template <bool CELL>
+int resample(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels, float* out,
+             void* stream, const char* what) {
     fn::Resample R;
     fn::Shape3 I;
-    int st = make_shape3(in_shape, dim, channels, I, "vr_resample_grid");
+    int st = make_shape3(in_shape, dim, channels, I, what);
     if (st) return st;
-    if ((st = make_shape3(out_shape, dim, channels, R.out, "vr_resample_grid"))) return st;
+    if ((st = make_shape3(out_shape, dim, channels, R.out, what))) return st;
     for (int i = 0; i < 3; i++) {
-        if (i < dim && R.out.n[i] < 2) return set_err(VR_ERR_ARG, "vr_resample_grid: output extents must be >= 2");
+        if (!CELL && i < dim && R.out.n[i] < 2) return set_err(VR_ERR_ARG, "%s: output extents must be >= 2", what);
         // the source coordinate is formed as the int p * (in - 1), as in the shader
         if ((int64_t)(R.out.n[i] - 1) * (I.n[i] - 1) >= (int64_t(1) << 31))
-            return set_err(VR_ERR_ARG, "vr_resample_grid: (out - 1) * (in - 1) must stay below 2^31");
+            return set_err(VR_ERR_ARG, "%s: (out - 1) * (in - 1) must stay below 2^31", what);
         R.in_n[i] = I.n[i];
         R.in_s[i] = I.s[i];
     }
     R.dim = dim;
-    if (!in || !out) return set_err(VR_ERR_ARG, "vr_resample_grid: null buffer");
+    if (!in || !out) return set_err(VR_ERR_ARG, "%s: null buffer", what);
     fn::Resample3 R3;
     if (dense3_of(R.out, dim, out, out, out, R3.out)) {
         for (int i = 0; i < 3; i++) R3.in_n[i] = I.n[i];
         R3.in_sy = I.s[1];
         R3.in_sz = I.s[2];
-        hipLaunchKernelGGL(fn::k_resample_x4, dense3_grid(R3.out), dim3(fn::kQuadX, fn::kRowsY), 0, (hipStream_t)stream, in, R3,
-                           out);
-        return check_hip(hipGetLastError(), "vr_resample_grid launch");
+        hipLaunchKernelGGL(fn::k_resample_x4<CELL>, dense3_grid(R3.out), dim3(fn::kQuadX, fn::kRowsY), 0, (hipStream_t)stream, in,
+                           R3, out);
+        return check_hip(hipGetLastError(), what);
     }
-    hipLaunchKernelGGL(fn::k_resample, shape_grid(R.out), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, R, out);
-    return check_hip(hipGetLastError(), "vr_resample_grid launch");
+    hipLaunchKernelGGL(fn::k_resample<CELL>, shape_grid(R.out), dim3(fn::kRowBlock), 0, (hipStream_t)stream, in, R, out);
+    return check_hip(hipGetLastError(), what);
+}
+
+template <bool TO_MORTON>
+int morton_copy(const float* in, int64_t resolution, int32_t channels, float* out, void* stream, const char* what) {
+    if (resolution < 1 || resolution > (1 << 15) || (resolution & (resolution - 1)))
+        return set_err(VR_ERR_ARG, "%s: resolution must be a power of two up to 2^15", what);
+    if (channels < 1 || channels > 64) return set_err(VR_ERR_ARG, "%s: 1 to 64 channels", what);
+    if (!in || !out) return set_err(VR_ERR_ARG, "%s: null buffer", what);
+    hipLaunchKernelGGL(fn::k_morton_copy<TO_MORTON>, dim3(grid_for(resolution * resolution)), dim3(kBlock), 0,
+                       (hipStream_t)stream, in, (int)resolution, (int)channels, out);
+    return check_hip(hipGetLastError(), what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vr_resample_grid(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                     float* out, void* stream) {
+    return resample<false>(in, in_shape, out_shape, dim, channels, out, stream, "vr_resample_grid");
+}
+
+int vr_resample_img(const float* in, const int64_t* in_shape, const int64_t* out_shape, int32_t dim, int32_t channels,
+                    float* out, void* stream) {
+    return resample<true>(in, in_shape, out_shape, dim, channels, out, stream, "vr_resample_img");
+}
+
+int vr_copy_img_to_morton(const float* in, int64_t resolution, int32_t channels, float* out, void* stream) {
+    return morton_copy<true>(in, resolution, channels, out, stream, "vr_copy_img_to_morton");
+}
+
+int vr_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels, float* out, void* stream) {
+    return morton_copy<false>(in, resolution, channels, out, stream, "vr_copy_morton_to_img");
 }
 
 }  // extern "C"

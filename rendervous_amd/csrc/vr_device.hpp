@@ -136,6 +136,15 @@ struct GridView {
     f3 bmin, bmax;
 };
 
+// Image2D over 2-D coordinates (include/maps/image2d.h): pixel-centred
+// bilinear lookup with border clamping of [H, W, 3].
+struct ImageView {
+    const float* data;
+    int sy, sx, sc;  // element strides of H, W, C
+    int w, h;
+    float bmin_x, bmin_y, ibs_x, ibs_y;  // bmin, 1 / (bmax - bmin)
+};
+
 struct MediumView {
     GridView sigma;
     GridView albedo;
@@ -157,6 +166,11 @@ struct MediumView {
     f3 gathering_c;
     float medium_majorant;
     int bw_mode;
+    int env_kind;          // VR_ENV_CONST / VR_ENV_XR
+    int env_sampler;       // VR_ENVS_UNIFORM / VR_ENVS_QUADTREE
+    ImageView env_img;
+    const float* quadtree; // XRQuadtreeRandomDirection densities
+    int qt_levels;
 };
 
 struct CameraView {
@@ -335,9 +349,105 @@ VR_DEV float phase_sample(const MediumView& M, Rng& rng, f3 w, f3& wo) {
     return 1.0f;
 }
 
-// _functionsampler(UniformDirectionSampler, const env) (_maps.py:2552-2585),
-// mixed with a directional light with MixtureSampler semantics (_maps.py:1541-1571).
-VR_DEV f3 env_sample(const MediumView& M, Rng& rng, f3& wo) {
+// include/maths.h:172-178 dir2xr: (atan2(x, z) / pi, -2 asin(y) / pi) in [-1, 1]^2.
+VR_DEV void dir2xr(f3 w, float& u, float& v) {
+    if (w.x == 0.0f && w.z == 0.0f) w.x += 0.0000001f;
+    u = atan2f(w.x, w.z) * (1.0f / kPi);
+    v = -2.0f * asinf(fminf(fmaxf(w.y, -1.0f), 1.0f)) * (1.0f / kPi);
+}
+
+// include/maps/image2d.h:1-25 FORWARD at c (3 channels).
+VR_DEV f3 image_lookup(const ImageView& I, float cx, float cy) {
+    const float nx = fminf(fmaxf((cx - I.bmin_x) * I.ibs_x, 0.0f), 1.0f);
+    const float ny = fminf(fmaxf((cy - I.bmin_y) * I.ibs_y, 0.0f), 1.0f);
+    const float gx = nx * (float)I.w - 0.5f, gy = ny * (float)I.h - 0.5f;
+    const float fx = floorf(gx), fy = floorf(gy);
+    const int px = (int)fx, py = (int)fy;
+    const float ax = gx - fx, ay = gy - fy;
+    const int x0 = min(max(px, 0), I.w - 1), x1 = min(max(px + 1, 0), I.w - 1);
+    const int y0 = min(max(py, 0), I.h - 1), y1 = min(max(py + 1, 0), I.h - 1);
+    const float* p00 = I.data + y0 * I.sy + x0 * I.sx;
+    const float* p01 = I.data + y0 * I.sy + x1 * I.sx;
+    const float* p10 = I.data + y1 * I.sy + x0 * I.sx;
+    const float* p11 = I.data + y1 * I.sy + x1 * I.sx;
+    float r[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float a = p00[c * I.sc] * (1.0f - ax) + p01[c * I.sc] * ax;
+        const float b = p10[c * I.sc] * (1.0f - ax) + p11[c * I.sc] * ax;
+        r[c] = a * (1.0f - ay) + b * ay;
+    }
+    return F3(r[0], r[1], r[2]);
+}
+
+// environment(x, w): a constant, or Image2D(...).after(xr_projection)
+// (include/maps/environment_xr.h:3-9, _objects.py:164-173).
+VR_DEV f3 env_eval(const MediumView& M, f3 w) {
+    if (M.env_kind == VR_ENV_XR) {
+        float u, v;
+        dir2xr(w, u, v);
+        return image_lookup(M.env_img, u, v);
+    }
+    return M.env_c;
+}
+
+// XRQuadtreeRandomDirection (_maps.py:2494-2549): descend the quadtree of
+// absolute pixel probabilities (Morton order per level, children of node n at
+// 4n .. 4n+3 below it) with one uniform, then a direction uniform in the
+// pixel's solid angle (randomDirection(a0, a1, b0, b1), randoms.h:186-199).
+// Returns 1 / pdf = pixel solid angle / probability.
+VR_DEV float quadtree_direction(const MediumView& M, Rng& rng, f3& wo) {
+    float sel = rng_uniform(rng);
+    int node = 0;
+    float x0 = 0.0f, y0 = 0.0f, x1 = 1.0f, y1 = 1.0f, prob = 1.0f;
+    for (int i = 0; i < M.qt_levels; i++) {
+        const float* d = M.quadtree + node * 4;
+        const float d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3];
+        int child = 3;
+        prob = d3;
+        if (sel < d0) { child = 0; prob = d0; }
+        else {
+            sel -= d0;
+            if (sel < d1) { child = 1; prob = d1; }
+            else {
+                sel -= d1;
+                if (sel < d2) { child = 2; prob = d2; }
+                else sel -= d2;
+            }
+        }
+        const float xm = (x1 + x0) * 0.5f, ym = (y1 + y0) * 0.5f;
+        if ((child & 1) == 0) x1 = xm; else x0 = xm;
+        if ((child >> 1) == 0) y1 = ym; else y0 = ym;
+        node = node * 4 + 1 + child;
+    }
+    // angles in turns: phi = (x * 2 - 1) pi -> x - 1/2, theta = y pi -> y / 2
+    const float cb0 = __builtin_amdgcn_cosf(y0 * 0.5f), cb1 = __builtin_amdgcn_cosf(y1 * 0.5f);
+    const float area = 2.0f * kPi * (cb0 - cb1) / (float)(1 << M.qt_levels);
+    const float weight = area / fmaxf(0.000000001f, prob);
+    const float a0 = x0 - 0.5f, a1 = x1 - 0.5f;
+    const float r1 = rng_uniform(rng) * (a1 - a0) + a0;
+    const float r2 = cb0 - rng_uniform(rng) * (cb0 - cb1);
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - r2 * r2));
+    float sn, cs;
+    sincos_turns(r1, sn, cs);
+    wo = F3(sn * s, r2, cs * s);
+    return weight;
+}
+
+// The environment part of the sampler: _functionsampler(UniformDirectionSampler
+// or XRQuadtreeRandomDirection, environment) (_maps.py:2552-2585): L(wo) / pdf.
+VR_DEV f3 env_sample_base(const MediumView& M, Rng& rng, f3& wo) {
+    if (M.env_sampler == VR_ENVS_QUADTREE) {
+        const float wgt = quadtree_direction(M, rng, wo);
+        return env_eval(M, wo) * wgt;
+    }
+    wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
+    return env_eval(M, wo) * (4.0f * kPi);
+}
+
+// The constant-environment sampler alone (uniform directions, optional sun):
+// what env_sample does when env_kind == VR_ENV_CONST and the sampler is uniform.
+VR_DEV f3 env_sample_const(const MediumView& M, Rng& rng, f3& wo) {
     if (M.sun_prob > 0.0f) {
         if (rng_uniform(rng) < M.sun_prob) {
             wo = M.sun_dir;
@@ -348,6 +458,18 @@ VR_DEV f3 env_sample(const MediumView& M, Rng& rng, f3& wo) {
     }
     wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
     return M.env_c * (4.0f * kPi);
+}
+
+// ... mixed with a directional light with MixtureSampler semantics (_maps.py:1541-1571).
+VR_DEV f3 env_sample(const MediumView& M, Rng& rng, f3& wo) {
+    if (M.sun_prob > 0.0f) {
+        if (rng_uniform(rng) < M.sun_prob) {
+            wo = M.sun_dir;
+            return M.sun_radiance * rcp(M.sun_prob);
+        }
+        return env_sample_base(M, rng, wo) * rcp(1.0f - M.sun_prob);
+    }
+    return env_sample_base(M, rng, wo);
 }
 
 // ---------------------------------------------------------------- majorant --
@@ -945,7 +1067,7 @@ struct Tracer {
         float T, t;
         float C = collision_dt(x, w, t, T);
         f3 xt = w * t + x;
-        f3 R = M.env_c * T;
+        f3 R = env_eval(M, w) * T;
         if (C != 0.0f) R = R + ss_radiance(xt, w * -1.0f) * C;
         return R;
     }
@@ -957,7 +1079,7 @@ struct Tracer {
             float C = collision_dt(x, w, t, T);
             f3 xt = w * t + x;
             f3 eR = ss_radiance(xt, w * -1.0f);
-            collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, M.env_c));
+            collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, env_eval(M, w)));
             ss_radiance_bw(xt, w * -1.0f, dL_dR * C);
             return;
         }
@@ -971,7 +1093,7 @@ struct Tracer {
         if (C != 0.0f) eR = ss_radiance(xt, w * -1.0f);
         Rng after = rng;
         rng = saved;
-        collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, M.env_c));
+        collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, env_eval(M, w)));
         rng = after;
         if (C != 0.0f) ss_radiance_bw(xt, w * -1.0f, dL_dR * Wc);
     }
@@ -982,7 +1104,7 @@ struct Tracer {
         boundary(M, x, w, tmin, tmax);
         x = x + w * tmin;
         float d = tmax - tmin;
-        f3 R = M.env_c * transmittance(x, w);
+        f3 R = env_eval(M, w) * transmittance(x, w);
         f3 W = F3(1.0f, 1.0f, 1.0f);
         const float EPS = 0.00001f;
         int bounces = 0;
@@ -1069,7 +1191,7 @@ struct Tracer {
             boundary(M, x, w, tmin, d);
             mj.begin(M, x, w);
         }
-        if (!scattered) R = R + M.env_c * W;
+        if (!scattered) R = R + env_eval(M, w) * W;
 
         rng = before_path;
         x = x0; w = w0;
@@ -1164,7 +1286,7 @@ struct Tracer {
             boundary(M, x, w, tmin, d);
             mj.begin(M, x, w);
         }
-        return W * M.env_c + R;
+        return W * env_eval(M, w) + R;
     }
 
     // include/vr_path_integrator_DT.h:59-235: defensive pass (majorant + 2 eps,
@@ -1209,7 +1331,7 @@ struct Tracer {
             boundary(M, x, w, tmin, d);
             mj.begin(M, x, w);
         }
-        const f3 dL_dW = dL_dR * M.env_c;
+        const f3 dL_dW = dL_dR * env_eval(M, w);
         const f3 W_out = W;
 
         rng = before_path;
@@ -1338,7 +1460,7 @@ struct Tracer {
         Rng secondary = create_branch_seed(), main_s;
         transmittance(x, w);  // the primary term's T: draws from the main stream
         main_s = rng; rng = secondary;
-        transmittance_bw(x, w, dot(dL_dR, M.env_c));
+        transmittance_bw(x, w, dot(dL_dR, env_eval(M, w)));
         secondary = rng; rng = main_s;
 
         f3 R = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);

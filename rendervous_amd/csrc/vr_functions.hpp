@@ -125,9 +125,20 @@ struct Resample {
     int dim;
 };
 
-// resample_grid.comp.glsl:30-86: the 2^dim linear taps (the +1 tap clamped to
-// the last vertex), weights formed and accumulated in the shader's order with
-// no fused multiply-add, so the result does not depend on contraction.
+// Source coordinate of output p along an axis: resample_grid.comp.glsl:14-27
+// (vertex-aligned, p (in - 1) / (out - 1)) or resample_img.comp.glsl:14-27
+// (pixel-centred, (p + 1/2) / out * in - 1/2, which is -1/2 .. in - 1/2).
+template <bool CELL>
+VR_DEV float source_coord(int p, int n_in, int n_out) {
+#pragma clang fp contract(off)
+    if (CELL) return ((float)p + 0.5f) / (float)n_out * (float)n_in - 0.5f;
+    return (float)(p * (n_in - 1)) / (float)(n_out - 1);
+}
+
+// resample_grid.comp.glsl / resample_img.comp.glsl:30-86: the 2^dim linear taps
+// (clamped to the extent), weights formed and accumulated in the shaders' order
+// with no fused multiply-add, so the result does not depend on contraction.
+template <bool CELL>
 __global__ void __launch_bounds__(kRowBlock) k_resample(const float* __restrict__ in, Resample R, float* __restrict__ out) {
 #pragma clang fp contract(off)
     const Voxel v = voxel_of_block(R.out);
@@ -137,7 +148,7 @@ __global__ void __launch_bounds__(kRowBlock) k_resample(const float* __restrict_
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         if (i >= R.dim) break;
-        const float coord = (float)(v.c[i] * (R.in_n[i] - 1)) / (float)(R.out.n[i] - 1);
+        const float coord = source_coord<CELL>(v.c[i], R.in_n[i], R.out.n[i]);
         ic[i] = (int)floorf(coord);
         a[i] = coord - (float)ic[i];
     }
@@ -155,7 +166,7 @@ __global__ void __launch_bounds__(kRowBlock) k_resample(const float* __restrict_
             const int dx = (t >> i) & 1;
             const float f = dx ? a[i] : 1.0f - a[i];
             wt = i == 0 ? f : wt * f;
-            sv += (int64_t)min(ic[i] + dx, R.in_n[i] - 1) * R.in_s[i];
+            sv += (int64_t)max(0, min(ic[i] + dx, R.in_n[i] - 1)) * R.in_s[i];
         }
         w[t] = wt;
         src[t] = sv * C;
@@ -282,15 +293,18 @@ struct Resample3 {
     int64_t in_sy, in_sz;
 };
 
+template <bool CELL>
 VR_DEV void axis_coord(int p, int n_in, int n_out, int& i0, int& i1, float& a) {
-    const float coord = (float)(p * (n_in - 1)) / (float)(n_out - 1);
-    i0 = (int)floorf(coord);
-    a = coord - (float)i0;
-    i1 = min(i0 + 1, n_in - 1);
+    const float coord = source_coord<CELL>(p, n_in, n_out);
+    const int f = (int)floorf(coord);
+    a = coord - (float)f;
+    i0 = max(0, min(f, n_in - 1));
+    i1 = max(0, min(f + 1, n_in - 1));
 }
 
 // The resample_grid taps for 4 consecutive x outputs of one row: y and z taps
 // shared, 32 independent loads per lane, one float4 store.
+template <bool CELL>
 __global__ void __launch_bounds__(kQuadX * kRowsY) k_resample_x4(const float* __restrict__ in, Resample3 R,
                                                                   float* __restrict__ out) {
 #pragma clang fp contract(off)
@@ -300,14 +314,14 @@ __global__ void __launch_bounds__(kQuadX * kRowsY) k_resample_x4(const float* __
     if (x0 >= R.out.nx || y >= R.out.ny) return;
     int y0, y1, z0, z1;
     float ay, az;
-    axis_coord(y, R.in_n[1], R.out.ny, y0, y1, ay);
-    axis_coord(z, R.in_n[2], R.out.nz, z0, z1, az);
+    axis_coord<CELL>(y, R.in_n[1], R.out.ny, y0, y1, ay);
+    axis_coord<CELL>(z, R.in_n[2], R.out.nz, z0, z1, az);
     const float* rows[4] = {in + y0 * R.in_sy + z0 * R.in_sz, in + y1 * R.in_sy + z0 * R.in_sz,
                             in + y0 * R.in_sy + z1 * R.in_sz, in + y1 * R.in_sy + z1 * R.in_sz};
     int xi[4][2];
     float ax[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) axis_coord(x0 + k, R.in_n[0], R.out.nx, xi[k][0], xi[k][1], ax[k]);
+    for (int k = 0; k < 4; k++) axis_coord<CELL>(x0 + k, R.in_n[0], R.out.nx, xi[k][0], xi[k][1], ax[k]);
     float v[4][8];
 #pragma unroll
     for (int k = 0; k < 4; k++)
@@ -328,6 +342,28 @@ __global__ void __launch_bounds__(kQuadX * kRowsY) k_resample_x4(const float* __
         r[k] = acc;
     }
     st4_stream(out + x0 + y * R.out.sy + z * R.out.sz, r[0], r[1], r[2], r[3]);
+}
+
+// maths.h:244-251 unpart1by1: compact the even bits.
+VR_DEV int unpart1by1(int n) {
+    n &= 0x55555555;
+    n = (n ^ (n >> 1)) & 0x33333333;
+    n = (n ^ (n >> 2)) & 0x0f0f0f0f;
+    n = (n ^ (n >> 4)) & 0x00ff00ff;
+    n = (n ^ (n >> 8)) & 0x0000ffff;
+    return n;
+}
+
+// functions/tools/copy_img_to_morton.comp.glsl (TO_MORTON) and
+// copy_morton_to_img.comp.glsl: Morton index i <-> pixel (even bits, odd bits).
+template <bool TO_MORTON>
+__global__ void k_morton_copy(const float* __restrict__ in, int resolution, int C, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)resolution * resolution) return;
+    const int px = unpart1by1((int)i), py = unpart1by1((int)(i >> 1));
+    const int64_t pix = (int64_t)py * resolution + px;
+    const int64_t src = (TO_MORTON ? pix : i) * C, dst = (TO_MORTON ? i : pix) * C;
+    for (int c = 0; c < C; c++) out[dst + c] = in[src + c];
 }
 
 }  // namespace fn

@@ -31,7 +31,9 @@ enum : int { kModePath = 0, kModeTrack = 1 };
 // Lane events: what a lane needs before it can take its next tracking step.
 enum : int { kEvNone = 0, kEvBegin = 1, kEvStart = 2, kEvCollision = 3, kEvTrackEnd = 4, kEvPathEnd = 5 };
 
-template <int MAJ, int TKIND>
+// ENVMAP: environment image and/or quadtree sampler; false compiles the
+// constant-environment (uniform + directional light) sampler only.
+template <int MAJ, int TKIND, bool ENVMAP>
 struct NeeLane {
     static constexpr float EPS = 0.00001f;  // vr_path_integrator_NEE.h:16
 
@@ -62,7 +64,7 @@ struct NeeLane {
         R = F3(0.0f, 0.0f, 0.0f);
         W = F3(1.0f, 1.0f, 1.0f);
         bounces = 0;
-        contrib = M.env_c;
+        contrib = ENVMAP ? env_eval(M, w) : M.env_c;
         primary = 1;
         rx = px;
         rw = pw;
@@ -188,7 +190,7 @@ struct NeeLane {
         R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
         W = W * sa;
         f3 wnee;
-        f3 env = env_sample(M, rng, wnee);
+        f3 env = ENVMAP ? env_sample(M, rng, wnee) : env_sample_const(M, rng, wnee);
         px = rx;
         pw = rw;
         contrib = (W * env) * phase_eval(M, rw, wnee);
@@ -326,12 +328,12 @@ VR_DEV unsigned long long lane_mask_lt() {
 // part of the image, and moves on to the next queue when that one is empty.
 // The output must be zero on entry (k_nee_prepare); item sums are added
 // atomically.
-template <int MAJ, int TKIND>
+template <int MAJ, int TKIND, bool ENVMAP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))
 k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsigned long long* __restrict__ counters,
                  float* __restrict__ out, int* __restrict__ dbg) {
     const float inv = S.spp == 1 ? 1.0f : 1.0f / (float)S.spp;
-    NeeLane<MAJ, TKIND> lane;
+    NeeLane<MAJ, TKIND, ENVMAP> lane;
     lane.lookups = 0;
     lane.event = kEvNone;
     bool have = false, done = false;

@@ -417,7 +417,7 @@ VR_DEV f3 nee_cached(Tracer<MAJ>& tr, f3 x, f3 w, int K, f3& qx, f3& qw, f3& qW,
     boundary(M, x, w, tmin, tmax);
     x = x + w * tmin;
     float d = tmax - tmin;
-    f3 R = M.env_c * tr.transmittance(x, w);
+    f3 R = env_eval(M, w) * tr.transmittance(x, w);
     f3 W = F3(1.0f, 1.0f, 1.0f);
     const float EPS = 0.00001f;
     int bounces = 0;
@@ -468,7 +468,7 @@ VR_DEV int nee_record(Tracer<MAJ>& tr, f3 x, f3 w, int maxv, float* __restrict__
     boundary(M, x, w, tmin, tmax);
     x = x + w * tmin;
     float d = tmax - tmin;
-    f3 R = M.env_c * tr.transmittance(x, w);
+    f3 R = env_eval(M, w) * tr.transmittance(x, w);
     f3 W = F3(1.0f, 1.0f, 1.0f);
     const float EPS = 0.00001f;
     int bounces = 0, nv = 0;

@@ -194,24 +194,22 @@ def total_variation(in_tensor: torch.Tensor) -> torch.Tensor:
     return _TotalVariation.apply(in_tensor)
 
 
-def _resample_step(x: torch.Tensor, dst: typing.Tuple[int, ...], out: typing.Optional[torch.Tensor]) -> torch.Tensor:
+def _resample_step(x: torch.Tensor, dst: typing.Tuple[int, ...], out: typing.Optional[torch.Tensor],
+                   cell: bool = False) -> torch.Tensor:
     shape, dim = _spatial(x)
     C = shape[-1]
     if out is None:
         out = torch.empty(*dst, C, dtype=torch.float32, device=x.device)
     assert tuple(out.shape) == tuple(dst) + (C,) and out.is_contiguous()
-    L.check(L.load().vr_resample_grid(x.data_ptr(), L.i64s(shape[:-1]), L.i64s(dst), dim, C, out.data_ptr(),
-                                      L.stream_handle(x.device)), "vr_resample_grid")
+    fn = L.load().vr_resample_img if cell else L.load().vr_resample_grid
+    L.check(fn(x.data_ptr(), L.i64s(shape[:-1]), L.i64s(dst), dim, C, out.data_ptr(), L.stream_handle(x.device)),
+            "vr_resample_img" if cell else "vr_resample_grid")
     return out
 
 
-def resample_grid(in_tensor: torch.Tensor, dst_shape: typing.Sequence[int],
-                  out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
-    """Vertex-aligned linear resampling of a grid [..., C] to dst_shape + (C,)
-    (_functions.py:287-295): steps of at most 2x up or down per axis until the
-    shape is reached; `out`, if given, receives the final step."""
+def _resample_chain(in_tensor, dst_shape, out, cell):
     if not in_tensor.is_cuda:
-        raise RuntimeError("resample_grid runs on the HIP device; no CPU fallback exists")
+        raise RuntimeError("resampling runs on the HIP device; no CPU fallback exists")
     dst = tuple(int(v) for v in dst_shape)
     x = in_tensor.contiguous()
     while True:
@@ -221,5 +219,72 @@ def resample_grid(in_tensor: torch.Tensor, dst_shape: typing.Sequence[int],
         hi = tuple(d * 2 for d in cur)
         step = tuple(max(min(dst[i], hi[i]), lo[i]) for i in range(len(dst)))
         if step == dst:
-            return _resample_step(x, dst, out)
-        x = _resample_step(x, step, None)
+            return _resample_step(x, dst, out, cell)
+        x = _resample_step(x, step, None, cell)
+
+
+def resample_img(in_tensor: torch.Tensor, dst_shape: typing.Sequence[int],
+                 out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Pixel-centred linear resampling of an image [..., C] (_functions.py:337-345),
+    in steps of at most 2x per axis; `out`, if given, receives the final step."""
+    return _resample_chain(in_tensor, dst_shape, out, True)
+
+
+def _morton(in_tensor: torch.Tensor, resolution: int, to_morton: bool, out: typing.Optional[torch.Tensor]):
+    if not in_tensor.is_cuda:
+        raise RuntimeError("Morton copies run on the HIP device; no CPU fallback exists")
+    x = in_tensor.contiguous()
+    C = x.shape[-1]
+    shape = (resolution * resolution, C) if to_morton else (resolution, resolution, C)
+    if out is None:
+        out = torch.empty(*shape, dtype=torch.float32, device=x.device)
+    assert tuple(out.shape) == shape and out.is_contiguous() and out.dtype == torch.float32
+    fn = L.load().vr_copy_img_to_morton if to_morton else L.load().vr_copy_morton_to_img
+    L.check(fn(x.data_ptr(), resolution, C, out.data_ptr(), L.stream_handle(x.device)), "morton copy")
+    return out
+
+
+def copy_img_to_morton(in_tensor: torch.Tensor, out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+    """[R, R, C] square image (R a power of two) -> [R*R, C] in Morton order (_functions.py:434-468)."""
+    assert in_tensor.dim() == 3 and in_tensor.shape[0] == in_tensor.shape[1], "in_tensor should be a square image"
+    return _morton(in_tensor, int(in_tensor.shape[0]), True, out)
+
+
+def copy_morton_to_img(in_tensor: torch.Tensor, out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+    """[R*R, C] in Morton order -> [R, R, C] (_functions.py:471-506)."""
+    assert in_tensor.dim() == 2
+    r = int(math.isqrt(int(in_tensor.shape[0])))
+    assert r * r == in_tensor.shape[0], "linearization does not correspond to a square image"
+    return _morton(in_tensor, r, False, out)
+
+
+def create_density_quadtree(densities: torch.Tensor) -> torch.Tensor:
+    """Quadtree of absolute pixel probabilities for XRQuadtreeRandomDirection
+    (_functions.py:509-530): levels 2x2, 4x4, .. RxR concatenated coarse to fine,
+    each in Morton order; a node's 4 children are consecutive one level below."""
+    resolution = densities.shape[0]
+    assert densities.dim() == 3 and densities.shape[-1] == 1
+    assert densities.shape[1] == resolution, "Must be square"
+    assert resolution >= 2 and resolution & (resolution - 1) == 0, "Resolution must be power of two"
+    sizes = []
+    size = resolution
+    while size > 1:
+        sizes.append(size * size)
+        size //= 2
+    sizes.reverse()
+    offsets = [0]
+    for n in sizes:
+        offsets.append(offsets[-1] + n)
+    pdfs = torch.zeros(offsets[-1], 1, dtype=torch.float32, device=densities.device)
+    for o in reversed(offsets[:-1]):
+        copy_img_to_morton(densities, out=pdfs[o:o + densities.numel()])
+        densities = resample_img(densities, (int(densities.shape[0]) // 2, int(densities.shape[1]) // 2)) * 4
+    return pdfs
+
+
+def resample_grid(in_tensor: torch.Tensor, dst_shape: typing.Sequence[int],
+                  out: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Vertex-aligned linear resampling of a grid [..., C] to dst_shape + (C,)
+    (_functions.py:287-295): steps of at most 2x up or down per axis until the
+    shape is reached; `out`, if given, receives the final step."""
+    return _resample_chain(in_tensor, dst_shape, out, False)

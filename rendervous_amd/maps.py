@@ -21,7 +21,8 @@ from . import _lib as L
 
 __all__ = [
     "vec3", "MapBase", "ConstantMap", "const", "ZERO", "ONE", "Grid3D", "RayBoxIntersection", "ray_box_intersection",
-    "MajorantGrid", "GridDDATransmittance", "GridDeltatrackingTransmittance", "GridRatiotrackingTransmittance",
+    "MajorantGrid", "CompositionMap", "Image2D", "XRProjection", "xr_projection", "XRQuadtreeRandomDirection",
+    "GridDDATransmittance", "GridDeltatrackingTransmittance", "GridRatiotrackingTransmittance",
     "DeltatrackingTransmittance", "RatiotrackingTransmittance",
     "DeltatrackingCollisionSampler", "HGPhase", "VHGPhaseSampler", "UniformDirectionSampler", "_functionsampler",
     "DirectionalLightEnvironmentSampler", "SingleScatteredRadiance", "MCCollisionIntegrator",
@@ -58,6 +59,14 @@ class MapBase(torch.nn.Module):
     @property
     def is_generic_input(self) -> bool:
         return self.input_dim is None
+
+    def after(self, prev_map: "MapBase") -> "MapBase":
+        """self(prev_map(x)) (_maps.py:1347-1348)."""
+        return CompositionMap(prev_map, self)
+
+    def then(self, next_map: "MapBase") -> "MapBase":
+        """next_map(self(x)) (_maps.py:1350-1355)."""
+        return CompositionMap(self, next_map)
 
     def cast(self, input_dim: typing.Optional[int] = None, output_dim: typing.Optional[int] = None) -> "MapBase":
         """Type a generic map (_maps.py:1260-1271)."""
@@ -153,6 +162,50 @@ class _ConstFactory:
 const = _ConstFactory()
 ZERO = const[0.0]
 ONE = const[1.0]
+
+
+class CompositionMap(MapBase):
+    """outer(inner(x)) (_maps.py:1804-1830, include/maps/composition.h); lowered by
+    pattern (Image2D after XRProjection as an environment), not evaluated alone."""
+
+    def __init__(self, inner: MapBase, outer: MapBase):
+        super().__init__(inner.input_dim, outer.output_dim)
+        self.inner = inner
+        self.outer = outer
+
+
+class Image2D(MapBase):
+    """Pixel-centred bilinear image [H,W,C] over [bmin, bmax] with border clamping
+    (_maps.py:2288-2316, include/maps/image2d.h)."""
+
+    def __init__(self, grid: torch.Tensor, bmin=(-1.0, -1.0), bmax=(1.0, 1.0)):
+        assert len(grid.shape) == 3
+        super().__init__(2, grid.shape[-1])
+        self.grid = grid if isinstance(grid, torch.nn.Parameter) else torch.nn.Parameter(grid, requires_grad=False)
+        self.bmin = [float(v) for v in bmin]
+        self.bmax = [float(v) for v in bmax]
+
+
+class XRProjection(MapBase):
+    """Direction (or ray) -> equirectangular coordinates in [-1, 1]^2 (dir2xr,
+    include/maths.h:172-178, include/maps/xr_projection.h)."""
+
+    def __init__(self, ray_input: bool = False):
+        super().__init__(6 if ray_input else 3, 2)
+        self.ray_input = ray_input
+
+
+xr_projection = XRProjection(False)
+
+
+class XRQuadtreeRandomDirection(MapBase):
+    """Importance-sampled direction from a quadtree of equirectangular pixel
+    probabilities (functions.create_density_quadtree), weight 1/pdf (_maps.py:2494-2549)."""
+
+    def __init__(self, input_dim: int, densities: torch.Tensor, levels: int):
+        super().__init__(input_dim, 4)
+        self.densities = densities
+        self.levels = int(levels)
 
 
 # ---------------------------------------------------------------- Grid3D --
@@ -427,26 +480,70 @@ class _Lowered:
             raise NotImplementedError("emission must be a constant")
         self.medium.emission_c = L.f3(emission.vec(3))
 
+    def _environment_of(self, environment: MapBase):
+        """('const', [r,g,b]) or ('xr', Image2D) for the supported environments."""
+        if isinstance(environment, ConstantMap):
+            return "const", environment.vec(3)
+        if isinstance(environment, CompositionMap) and isinstance(environment.inner, XRProjection) \
+                and isinstance(environment.outer, Image2D) and environment.outer.output_dim == 3:
+            if environment.outer.grid.requires_grad:
+                raise NotImplementedError("environment images are constants here (no gradient into the image)")
+            return "xr", environment.outer
+        raise NotImplementedError("environment must be a constant or Image2D(...).after(xr_projection)")
+
+    def _apply_environment(self, kind, value):
+        if kind == "const":
+            key = ("const", tuple(float(a) for a in L.f3(value)))
+        else:
+            key = ("xr", id(value))
+        prev = getattr(self, "_env_key", None)
+        if prev is not None and prev != key:
+            raise NotImplementedError("environment sampler and environment must share the environment")
+        self._env_key = key
+        if kind == "const":
+            self.medium.env_kind = L.VR_ENV_CONST
+            self.medium.env_c = L.f3(value)
+            return
+        img = value.grid.data
+        if img.dtype != torch.float32:
+            raise NotImplementedError("environment image must be float32")
+        self.keep.append(img)
+        self.medium.env_kind = L.VR_ENV_XR
+        self.medium.env_image = L.tensor_view(img)
+        for a in range(2):
+            self.medium.env_image_bmin[a] = value.bmin[a]
+            self.medium.env_image_bmax[a] = value.bmax[a]
+
     def set_environment(self, environment: MapBase):
-        if not isinstance(environment, ConstantMap):
-            raise NotImplementedError("environment must be a constant")
-        self.medium.env_c = L.f3(environment.vec(3))
+        self._apply_environment(*self._environment_of(environment))
 
     def set_environment_sampler(self, sampler: MapBase):
-        if isinstance(sampler, _functionsampler) and isinstance(sampler.point_sampler, UniformDirectionSampler) \
-                and isinstance(sampler.function_map, ConstantMap):
-            env = sampler.function_map.vec(3)
+        if isinstance(sampler, _functionsampler) and isinstance(sampler.point_sampler, UniformDirectionSampler):
+            self._apply_environment(*self._environment_of(sampler.function_map))
+            self.medium.env_sampler_kind = L.VR_ENVS_UNIFORM
             self.medium.sun_prob = 0.0
-        elif isinstance(sampler, DirectionalLightEnvironmentSampler) and isinstance(sampler.environment, ConstantMap):
-            env = sampler.environment.vec(3)
+        elif isinstance(sampler, _functionsampler) and isinstance(sampler.point_sampler, XRQuadtreeRandomDirection):
+            self._apply_environment(*self._environment_of(sampler.function_map))
+            q = sampler.point_sampler
+            t = q.densities
+            if not (isinstance(t, torch.Tensor) and t.dtype == torch.float32 and t.is_contiguous()):
+                raise NotImplementedError("quadtree densities must be a contiguous float32 tensor")
+            if t.numel() < (4 ** (q.levels + 1) - 4) // 3:
+                raise ValueError("quadtree densities hold fewer nodes than its levels need")
+            self.keep.append(t)
+            self.medium.env_sampler_kind = L.VR_ENVS_QUADTREE
+            self.medium.env_quadtree = t.data_ptr()
+            self.medium.env_quadtree_levels = q.levels
+            self.medium.sun_prob = 0.0
+        elif isinstance(sampler, DirectionalLightEnvironmentSampler):
+            self._apply_environment(*self._environment_of(sampler.environment))
+            self.medium.env_sampler_kind = L.VR_ENVS_UNIFORM
             self.medium.sun_dir = L.f3(sampler.direction)
             self.medium.sun_radiance = L.f3(sampler.radiance)
             self.medium.sun_prob = sampler.probability
         else:
-            raise NotImplementedError("environment sampler must sample a constant environment")
-        if list(self.medium.env_c) != [0.0, 0.0, 0.0] and [float(a) for a in self.medium.env_c] != [float(a) for a in L.f3(env)]:
-            raise NotImplementedError("environment sampler and environment must share the constant")
-        self.medium.env_c = L.f3(env)
+            raise NotImplementedError("environment sampler must be uniform or quadtree over the environment, "
+                                      "or a DirectionalLightEnvironmentSampler")
 
     def set_phase(self, phase: MapBase, sampler: MapBase):
         if isinstance(phase, HGPhase) and isinstance(sampler, VHGPhaseSampler):

@@ -20,7 +20,7 @@ __all__ = [
     "build_map_from_tensor", "medium_sigma_tensor", "medium_sigma", "medium_scattering_albedo_tensor",
     "medium_scattering_albedo", "medium_emission", "medium_phase_g_tensor", "medium_phase_g", "medium_majorant_tensor",
     "medium_majorant_grid", "medium_majorant_coarse_grid", "medium_majorant", "medium_transmittance_DDA", "medium_transmittance_GDT", "medium_transmittance_GRT",
-    "medium_transmittance_DT", "medium_transmittance_RT", "medium_transmittance", "medium_environment_tensor",
+    "medium_transmittance_DT", "medium_transmittance_RT", "medium_transmittance", "medium_environment_tensor", "medium_environment_sampler_quadtree",
     "medium_environment", "medium_environment_sampler_uniform", "medium_environment_sampler_sun",
     "medium_environment_sampler", "medium_phase_iso", "medium_phase_HG", "medium_phase", "medium_phase_sampler_HG",
     "medium_phase_sampler_uniform", "medium_phase_sampler", "medium_collision_sampler_DT", "medium_collision_sampler",
@@ -244,12 +244,16 @@ def medium_transmittance(ds: DependencySet):
         medium_transmittance_DDA(ds)
 
 
-def medium_environment_tensor(ds: DependencySet):
+def medium_environment_tensor(ds: DependencySet, *, projection: str = "xr"):
+    """_objects.py:164-173: a constant [3] or an equirectangular image [H, W, 3]."""
     ds.assert_ensures("environment_tensor", torch.Tensor)
     t = ds.environment_tensor
-    if len(t.shape) != 1:
-        raise NotImplementedError("only constant environments are supported")
-    ds.add_parameters(environment=_maps.const[t])
+    if len(t.shape) == 1:
+        ds.add_parameters(environment=_maps.const[t])
+        return
+    if projection != "xr":
+        raise NotImplementedError(f"environment projection {projection} (only 'xr')")
+    ds.add_parameters(environment=_maps.Image2D(t).after(_maps.xr_projection))
 
 
 def medium_environment(ds: DependencySet):
@@ -264,6 +268,32 @@ def medium_environment_sampler_uniform(ds: DependencySet):
     ds.add_parameters(environment_sampler=_maps._functionsampler(point_sampler=dir_sampling, function_map=ds.environment))
 
 
+def medium_environment_sampler_quadtree(ds: DependencySet, *, projection: str = "xr", levels: int = 10):
+    """_objects.py:180-199: importance sampling of an equirectangular environment
+    image by a quadtree of its solid-angle-weighted luminance sum."""
+    ds.requires(medium_environment_tensor, projection=projection)
+    if projection != "xr":
+        raise NotImplementedError(f"quadtree from projection {projection} (only 'xr')")
+    skybox = ds.environment_tensor
+    if len(skybox.shape) != 3:
+        raise NotImplementedError("the quadtree sampler needs an environment image")
+
+    def build_quadtree():
+        from . import functions
+        with torch.no_grad():
+            resolution = 1 << levels
+            densities = functions.resample_img(skybox.sum(-1, keepdim=True).contiguous(), (resolution, resolution))
+            py = torch.arange(resolution, dtype=torch.float64)
+            w = torch.cos(py * math.pi / resolution) - torch.cos((py + 1) * math.pi / resolution)
+            densities *= w.to(densities.device, torch.float32)[:, None, None]
+            densities /= max(float(densities.sum()), 0.00000001)
+            return functions.create_density_quadtree(densities)
+
+    ds.add_parameters(environment_sampler_quadtree=build_quadtree)
+    dir_sampling = _maps.XRQuadtreeRandomDirection(input_dim=6, densities=ds.environment_sampler_quadtree, levels=levels)
+    ds.add_parameters(environment_sampler=_maps._functionsampler(point_sampler=dir_sampling, function_map=ds.environment))
+
+
 def medium_environment_sampler_sun(ds: DependencySet, *, direction, radiance, probability: float = 0.5):
     ds.requires(medium_environment)
     ds.add_parameters(environment_sampler=_maps.DirectionalLightEnvironmentSampler(ds.environment, direction, radiance,
@@ -271,10 +301,15 @@ def medium_environment_sampler_sun(ds: DependencySet, *, direction, radiance, pr
 
 
 def medium_environment_sampler(ds: DependencySet):
-    """Constant environments are sampled uniformly (the reference's quadtree needs an image, _objects.py:204-212)."""
+    """_objects.py:204-212: the quadtree sampler for an environment image, uniform
+    sampling for a constant environment (where the reference's quadtree build fails
+    and its fallback sets nothing)."""
     if ds.ensures("environment_sampler", _maps.MapBase):
         return
-    medium_environment_sampler_uniform(ds)
+    if ds.ensures("environment_tensor", torch.Tensor) and len(ds.environment_tensor.shape) == 3:
+        medium_environment_sampler_quadtree(ds)
+    else:
+        medium_environment_sampler_uniform(ds)
 
 
 def medium_phase_iso(ds: DependencySet):

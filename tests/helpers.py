@@ -45,6 +45,9 @@ class Scene:
     medium_filter: int = 3
     gathering_c: tuple = (0.0, 0.0, 0.0)
     medium_majorant: float = 1.0
+    env_image: typing.Optional[np.ndarray] = None     # [H,W,3]: environment Image2D over xr coordinates
+    env_quadtree: typing.Optional[np.ndarray] = None  # flat quadtree (np_quadtree) for importance sampling
+    env_quadtree_levels: int = 0
 
 
 class Built:
@@ -109,6 +112,25 @@ class Built:
         m.medium_filter = scene.medium_filter
         m.gathering_c = L.f3(scene.gathering_c)
         m.medium_majorant = scene.medium_majorant
+        if scene.env_image is not None:
+            img = self._buf(scene.env_image)
+            m.env_kind = L.VR_ENV_XR
+            v = L.VrTensor()
+            v.data = self._addr(img)
+            v.ndim = 3
+            for i in range(3):
+                v.shape[i] = img.shape[i]
+                v.stride[i] = int(np.prod(img.shape[i + 1:]))
+            v.shape[3] = 1
+            v.stride[3] = 1
+            m.env_image = v
+            m.env_image_bmin[0] = m.env_image_bmin[1] = -1.0
+            m.env_image_bmax[0] = m.env_image_bmax[1] = 1.0
+        if scene.env_quadtree is not None:
+            q = self._buf(scene.env_quadtree)
+            m.env_sampler_kind = L.VR_ENVS_QUADTREE
+            m.env_quadtree = self._addr(q)
+            m.env_quadtree_levels = scene.env_quadtree_levels
         self.medium = m
 
     def _buf(self, a: np.ndarray):
@@ -144,6 +166,52 @@ class Built:
                 return None
             return x.copy() if self.device == "cpu" else x.detach().cpu().numpy()
         return host(self.sigma_grad), host(self.albedo_grad)
+
+
+def morton_pixels(n: int) -> typing.Tuple[np.ndarray, np.ndarray]:
+    """(px, py) of Morton indices 0..n-1: the even and the odd bits."""
+    i = np.arange(n, dtype=np.int64)
+    px = np.zeros(n, np.int64)
+    py = np.zeros(n, np.int64)
+    for b in range(16):
+        px |= ((i >> (2 * b)) & 1) << b
+        py |= ((i >> (2 * b + 1)) & 1) << b
+    return px, py
+
+
+def np_quadtree(density: np.ndarray) -> np.ndarray:
+    """The quadtree of create_density_quadtree (_functions.py:509-530) in numpy:
+    a normalised [R, R] pixel density, its 2x2 block sums level by level, each
+    level in Morton order, levels 2x2 .. RxR concatenated."""
+    R = density.shape[0]
+    levels = []
+    d = density.astype(np.float64)
+    while d.shape[0] >= 2:
+        n = d.shape[0]
+        px, py = morton_pixels(n * n)
+        levels.append(d[py, px])
+        d = d.reshape(n // 2, 2, n // 2, 2).sum(axis=(1, 3))
+    return np.concatenate(levels[::-1]).astype(np.float32)
+
+
+def env_density(img: np.ndarray) -> np.ndarray:
+    """Solid-angle-weighted, normalised luminance sum of an [R, R, 3] image
+    (the quadtree build of _objects.py:184-192 at the image's own resolution)."""
+    R = img.shape[0]
+    py = np.arange(R)
+    w = np.cos(py * np.pi / R) - np.cos((py + 1) * np.pi / R)
+    d = img.sum(-1).astype(np.float64) * w[:, None]
+    return d / d.sum()
+
+
+def sky_image(R: int, seed: int = 0) -> np.ndarray:
+    """A smooth sky [R, R, 3] with a bright sun-like blob (equirectangular)."""
+    rng = np.random.default_rng(seed)
+    v, u = np.meshgrid((np.arange(R) + 0.5) / R, (np.arange(R) + 0.5) / R, indexing="ij")
+    base = 0.2 + 0.3 * (1 - v)[..., None] * np.array([0.5, 0.7, 1.0])
+    cu, cv = rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.45)
+    blob = 20.0 * np.exp(-((u - cu) ** 2 + (v - cv) ** 2) / 0.003)[..., None] * np.array([1.0, 0.9, 0.7])
+    return (base + blob).astype(np.float32)
 
 
 def _np_view(a: np.ndarray) -> L.VrTensor:
