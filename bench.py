@@ -288,13 +288,16 @@ def time_to_rmse_leg(dev, radiance, camera, ref_spp=4096, rel_eps=0.05):
             "ladder": rows}
 
 
-def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32):
+def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32, traces=8, steps_per_trace=8,
+            batch=24576, lr=(1e-2, 1e-3), n_paths=65536, full_ladder=False):
     """Neural radiance cache vs plain NEE on the 512^3 cloud (fBm x60, albedo
     0.99, HG g=0.8, sun + env 0.1), 1024x1024: epsilon = the RMSE of the plain
     renderer at 256 spp against a 4096-spp plain render; the cache is trained
-    online (8 traces of 65536 paths, 8 Adam steps of 24576 records each) and its
-    renders (paths ended into the MLP after 2 bounces) are taken at doubling spp
-    until the RMSE is <= epsilon.  Times: HIP events around the GPU work."""
+    online (`traces` traces of `n_paths` paths, `steps_per_trace` Adam steps of
+    `batch` records after each, learning rate decaying geometrically from lr[0]
+    to lr[1]) and its renders (paths ended into the MLP after 2 bounces) are
+    taken at doubling spp until the RMSE is <= epsilon.  Times: HIP events
+    around the GPU work."""
     import torch
     import rendervous_amd as rv
     from rendervous_amd.nrc import NeuralRadianceCache
@@ -337,12 +340,16 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32)
         rv.seed(7)
         img, plain_ms = timed(lambda: camera.capture(field, fw_samples=target_spp))
         eps = rmse(img)
-        cache = NeuralRadianceCache(dev, seed=7, lr=1e-2)
+        cache = NeuralRadianceCache(dev, seed=7, lr=lr[0])
+        n_steps = traces * steps_per_trace
+        rates = [lr[0] * (lr[1] / lr[0]) ** (k / max(1, n_steps - 1)) for k in range(n_steps)]
+        cache.render(field, camera, spp=32)  # untimed: sizes the query buffer once
 
         def train():
-            for t in range(8):
+            for t in range(traces):
                 rv.seed(5000 + t)
-                cache.train(field, camera, n_paths=65536, max_vertices=32, steps=8, batch=24576)
+                cache.train(field, camera, n_paths=n_paths, max_vertices=32, steps=steps_per_trace, batch=batch,
+                            lr=rates[t * steps_per_trace:(t + 1) * steps_per_trace])
         _, train_ms = timed(train)
         ladder, hit = [], None
         spp = 4
@@ -350,9 +357,10 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32)
             rv.seed(77 + spp)
             img, ms = timed(lambda: cache.render(field, camera, spp=spp))
             ladder.append({"spp": spp, "ms": ms, "rmse": rmse(img)})
-            if ladder[-1]["rmse"] <= eps:
+            if ladder[-1]["rmse"] <= eps and hit is None:
                 hit = ladder[-1]
-                break
+                if not full_ladder:
+                    break
             spp *= 2
         # MLP throughput: inference over the last render's queries and one training step
         queries = cache.queries_last()

@@ -32,6 +32,8 @@ def _f3p(v) -> "ctypes.Array":
 class NeuralRadianceCache:
     """Weights, Adam state and the training / query buffers of one cache."""
 
+    MAX_QUERIES = 1 << 23  # rows of the render's query buffer (x, w, W/spp, pixel: 40 B each)
+
     def __init__(self, device: typing.Optional[torch.device] = None, *, seed: int = 0, lr: float = 1e-2,
                  betas=(0.9, 0.99), eps: float = 1e-8, terminate_after: int = 2):
         device = torch.device("cuda", 0) if device is None else torch.device(device)
@@ -59,6 +61,7 @@ class NeuralRadianceCache:
         self._count = torch.zeros(1, dtype=torch.int32, device=device)
         self._queries = None
         self._qcount = torch.zeros(1, dtype=torch.int32, device=device)
+        self._qtotal = torch.zeros(1, dtype=torch.int64, device=device)
 
     # --------------------------------------------------------------- helpers
     @staticmethod
@@ -73,10 +76,13 @@ class NeuralRadianceCache:
 
     # -------------------------------------------------------------- training
     def train(self, field: M.MapBase, camera: M.PerspectiveCameraSensor, n_paths: int = 8192, max_vertices: int = 16,
-              steps: int = 1, batch: typing.Optional[int] = None) -> torch.Tensor:
+              steps: int = 1, batch: typing.Optional[int] = None,
+              lr: typing.Union[None, float, typing.Sequence[float]] = None) -> torch.Tensor:
         """Traces n_paths plain NEE paths from random sensors and runs `steps`
         Adam steps on their collision vertices, each on the next `batch` records
-        (all records when batch is None); returns the last mean loss (device)."""
+        (all records when batch is None); returns the last mean loss (device).
+        `lr` overrides the cache's learning rate for these steps: one value, or
+        one per step (a schedule)."""
         lw = self._lower(field)
         cam, keep = camera._camera(None)
         cap = n_paths * max_vertices
@@ -90,12 +96,18 @@ class NeuralRadianceCache:
                 "vr_nrc_paths")
         bmin, bmax = _f3p(lw.medium.box_min), _f3p(lw.medium.box_max)
         size = cap if batch is None else int(batch)
+        if lr is None or isinstance(lr, (int, float)):
+            rates = [self.lr if lr is None else float(lr)] * steps
+        else:
+            rates = [float(r) for r in lr]
+            if len(rates) != steps:
+                raise ValueError(f"lr schedule has {len(rates)} entries for {steps} steps")
         for k in range(steps):
             self.steps += 1
             first = 0 if batch is None else k * size  # wrapped at the written count by the kernel
             L.check(lib.vr_nrc_train_step(self._records.data_ptr(), self._count.data_ptr(), int(first), int(size), bmin, bmax,
                                           self.params.data_ptr(), self.weights.data_ptr(), self.grad.data_ptr(),
-                                          self.m.data_ptr(), self.v.data_ptr(), self.steps, self.lr, self.betas[0],
+                                          self.m.data_ptr(), self.v.data_ptr(), self.steps, rates[k], self.betas[0],
                                           self.betas[1], self.eps, self.loss.data_ptr(), self._stream()),
                     "vr_nrc_train_step")
         return self.loss
@@ -106,21 +118,34 @@ class NeuralRadianceCache:
 
     # ------------------------------------------------------------- rendering
     def render(self, field: M.MapBase, camera: M.PerspectiveCameraSensor, spp: int = 1) -> torch.Tensor:
-        """Capture of the NEE radiance with paths ended into the cache."""
+        """Capture of the NEE radiance with paths ended into the cache.
+
+        A sample issues at most one query, so the sensors are rendered in
+        ranges of at most MAX_QUERIES // spp sensors: the query buffer never
+        exceeds MAX_QUERIES rows (40 B each) and is allocated once.  The ranges
+        use the same dispatch seeds and per-sample streams as one launch."""
         lw = self._lower(field)
         cam, keep = camera._camera(None)
         total = camera.num_sensors()
-        need = total * max(1, spp)
+        spp = max(1, int(spp))
+        per = max(1, min(total, self.MAX_QUERIES // spp))
+        need = per * spp
         if self._queries is None or self._queries.shape[0] < need:
-            self._queries = torch.empty(need, 10, device=self.device)
+            self._queries = torch.empty(max(need, min(self.MAX_QUERIES, total * spp)), 10, device=self.device)
         out = torch.empty(*camera.index_shape, 3, device=self.device)
-        launch = L.launch(_internal.get_seeds(), samples=spp, bw_mode=L.VR_BW_CONSISTENT)
+        seeds = _internal.get_seeds()
+        self._qtotal.zero_()
+        lib = L.load()
         with M._ENGINE_LOCK:
-            L.check(L.load().vr_nrc_render(lw.medium, cam, launch, self.terminate_after, self.weights.data_ptr(),
-                                           out.data_ptr(), self._queries.data_ptr(), int(need), self._qcount.data_ptr(),
-                                           self._stream()), "vr_nrc_render")
+            for start in range(0, total, per):
+                launch = L.launch(seeds, samples=spp, bw_mode=L.VR_BW_CONSISTENT, start_index=start,
+                                  count=min(per, total - start))
+                L.check(lib.vr_nrc_render(lw.medium, cam, launch, self.terminate_after, self.weights.data_ptr(),
+                                          out.data_ptr(), self._queries.data_ptr(), int(need), self._qcount.data_ptr(),
+                                          self._stream()), "vr_nrc_render")
+                self._qtotal += self._qcount.to(torch.int64)
         return out
 
     def queries_last(self) -> int:
         """Cache queries issued by the last render() (host sync)."""
-        return int(self._qcount.item())
+        return int(self._qtotal.item())
