@@ -194,6 +194,78 @@ typedef struct vr_launch {
                                       slot, _maps.py:368,611-615), see debug_kind */
 } vr_launch;
 
+/* ---- scene-level path tracing -------------------------------------------------
+ * PathtracedScene (_scenes.py:2251-2312, include/maps/pathtraced_scene.h,
+ * include/sc_path_tracing.h:12-236): a path tracer over triangle-mesh patches
+ * whose patches carry a surface scattering sampler, a surface emission and the
+ * media on their inner side (DTMediumPathIntegrator segments between surface
+ * hits).  The reference traces rays with Vulkan ray queries against a
+ * driver-built acceleration structure (maps/geometry_mesh.h); here a
+ * binned-SAH BVH2 built by vr_scene_build_bvh is traversed by the kernels. */
+
+enum vr_surface_sampler {    /* patch_info.surface_scattering_sampler */
+    VR_SURF_PASS = 0,        /* no sampler map: the path continues unchanged (sc_path_tracing.h:78-82) */
+    VR_SURF_LAMBERT = 1,     /* LambertSurfaceScatteringSampler   _scenes.py:906-937 */
+    VR_SURF_MIRROR = 2,      /* MirrorSurfaceScatteringSampler    _scenes.py:1003-1022 */
+    VR_SURF_FRESNEL = 3,     /* FresnelSurfaceScatteringSampler   _scenes.py:964-1000 */
+    VR_SURF_NULL = 4,        /* NullSurfaceScatteringSampler      _scenes.py:1308-1322 */
+    VR_SURF_BLOCK = 5        /* NoSurfaceScatteringSampler (zero) _scenes.py:1325-1340 */
+};
+
+enum vr_surface_emission {   /* patch_info.surface_gathering = the material's emission */
+    VR_EMIT_NONE = 0,
+    VR_EMIT_ISOTROPIC = 1,   /* IsotropicSurfaceEmission(const): E * 0.5/pi   _scenes.py:1392-1412 */
+    VR_EMIT_LAMBERT = 2      /* LambertSurfaceEmission(const): E * dot(-w, fN)/pi  _scenes.py:1415-1435 */
+};
+
+/* One BVH2 node (32 bytes).  Interior: `first` = index of the left child, the
+ * right child is first + 1, count == 0.  Leaf: triangles [first, first + count)
+ * of the BVH triangle order.  Node 0 is the root. */
+typedef struct vr_bvh_node {
+    float   bmin[3];
+    int32_t first;
+    float   bmax[3];
+    int32_t count;
+} vr_bvh_node;
+
+/* Per-patch information (PTPatchInfo, _scenes.py:2156-2162). */
+typedef struct vr_patch {
+    int32_t sampler;               /* vr_surface_sampler */
+    float   refraction_index;      /* VR_SURF_FRESNEL */
+    int32_t emission;              /* vr_surface_emission */
+    float   emission_c[3];         /* constant emission map value */
+    int32_t inside_medium;         /* index into vr_scene.media; -1 = none */
+    int32_t outside_medium;        /* kept as in the reference; the path tracer does not read it */
+} vr_patch;
+
+/* A scene: geometry (device buffers), patches (device), media (host array of
+ * DTMediumPathIntegrator descriptors, at most VR_SCENE_MAX_MEDIA) and the
+ * environment (env_kind / env_c / env_image* of vr_medium). */
+#define VR_SCENE_MAX_MEDIA 4
+typedef struct vr_scene {
+    const vr_bvh_node* nodes;      /* [num_nodes] */
+    const float*   tri_verts;      /* [num_tris, 9]: vertices of each triangle in BVH order */
+    const int32_t* tri_ids;        /* [num_tris]: triangle index of each BVH-ordered triangle */
+    int32_t        num_nodes;
+    int32_t        num_tris;
+    const float*   positions;      /* [num_vertices, 3] */
+    const float*   normals;        /* [num_vertices, 3] or NULL (face normals) */
+    const float*   uvs;            /* [num_vertices, 2] or NULL (C = 0) */
+    const int32_t* indices;        /* [num_tris, 3] vertex indices */
+    const int32_t* tri_patch;      /* [num_tris] patch index of each triangle */
+    const vr_patch* patches;       /* [num_patches] */
+    int32_t        num_patches;
+    int32_t        num_vertices;
+    const vr_medium* media;        /* host array [num_media]; medium_filter / medium_majorant /
+                                      gathering_c / sigma / albedo / phase used */
+    int32_t        num_media;
+    int32_t        env_kind;       /* vr_env_kind */
+    float          env_c[3];
+    float          env_image_bmin[2];
+    float          env_image_bmax[2];
+    vr_tensor      env_image;      /* VR_ENV_XR: [H, W, 3] */
+} vr_scene;
+
 /* ---- entry points ---------------------------------------------------------- */
 
 /* ABI version of the loaded library (VR_ABI_VERSION). No reference counterpart. */
@@ -336,6 +408,35 @@ int vr_nrc_paths(const vr_medium* medium, const vr_camera* camera, const vr_laun
 int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_launch* launch, int32_t terminate_after,
                   const uint16_t* weights, float* out, float* queries, int64_t max_queries, uint32_t* count,
                   void* stream);
+
+/* Builds the BVH of a triangle mesh on the host (binned SAH, at most 4
+ * triangles per leaf, depth <= 30).  positions [num_vertices,3], indices
+ * [num_tris,3] and every output are HOST buffers: nodes (2 * num_tris - 1
+ * entries at most; *num_nodes receives the count), tri_ids [num_tris] and
+ * tri_verts [num_tris,9] in BVH order.  Replaces the acceleration-structure
+ * build of MeshGeometry (_scenes.py:470-490, vk.ads_model / ads_scene). */
+int vr_scene_build_bvh(const float* positions, int64_t num_vertices, const int32_t* indices, int64_t num_tris,
+                       vr_bvh_node* nodes, int64_t max_nodes, int32_t* tri_ids, float* tri_verts, int32_t* num_nodes);
+
+/* Geometry map forward (6 -> 16, _scenes.py:93-100, maps/geometry_mesh.h:1-28):
+ * rays [n,6] -> hits [n,16] = (t, N, G, C, T, B, patch index as int bits; -1 = no hit)
+ * of the closest triangle with 0 <= t <= 10000 (common.h:297-333, 467-501). */
+int vr_scene_raycast(const vr_scene* scene, const float* rays, int64_t n, float* hits, void* stream);
+
+/* PathtracedScene on explicit rays (6 -> 3): eval_map_forward / eval_map_backward
+ * (_maps.py:885-976) with the map per-thread seeds.  The backward accumulates
+ * into the media's sigma / albedo gradients. */
+int vr_scene_map_fw(const vr_scene* scene, const float* rays, int64_t n, float* out, const vr_launch* launch,
+                    void* stream);
+int vr_scene_map_bw(const vr_scene* scene, const float* rays, const float* out_grad, int64_t n,
+                    const vr_launch* launch, void* stream);
+
+/* Capture of a PathtracedScene: eval_capture_forward / _backward (_maps.py:749-882)
+ * with map = PathtracedScene; out / out_grad [sensors][3]. */
+int vr_scene_capture_fw(const vr_scene* scene, const vr_camera* camera, const vr_launch* launch, float* out,
+                        void* stream);
+int vr_scene_capture_bw(const vr_scene* scene, const vr_camera* camera, const vr_launch* launch,
+                        const float* out_grad, void* stream);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,16 @@ def load():
             "oracle_copy_img_to_morton": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
             "oracle_copy_morton_to_img": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
             "oracle_env_sample": (ctypes.c_int, [P(L.VrMedium), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+            "oracle_scene_build_bvh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                                      P(ctypes.c_int32)]),
+            "oracle_scene_raycast": (ctypes.c_int, [P(L.VrScene), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+            "oracle_scene_map_fw": (ctypes.c_int, [P(L.VrScene), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                                   P(L.VrLaunch)]),
+            "oracle_scene_map_bw": (ctypes.c_int, [P(L.VrScene), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                   P(L.VrLaunch)]),
+            "oracle_scene_capture_fw": (ctypes.c_int, [P(L.VrScene), P(L.VrCamera), P(L.VrLaunch), ctypes.c_void_p]),
+            "oracle_scene_capture_bw": (ctypes.c_int, [P(L.VrScene), P(L.VrCamera), P(L.VrLaunch), ctypes.c_void_p]),
             "oracle_rng_uniforms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
             "oracle_rng_seed_states": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                                       ctypes.c_void_p]),

@@ -2236,3 +2236,487 @@ int oracle_env_sample(const vr_medium* M, const uint32_t seeds[4], int64_t n, fl
     }
     return VR_OK;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Scene-level path tracing: PathtracedScene (_scenes.py:2251-2312),          */
+/* include/sc_path_tracing.h:12-380, maps/pathtraced_scene.h.  The closest    */
+/* hit is found by brute force over every triangle (no BVH): it pins the      */
+/* library's BVH build and traversal.                                         */
+/* ------------------------------------------------------------------------- */
+#define SCENE_TMAX 10000.0     /* maps/geometry_mesh.h:15 */
+#define SCENE_PATH_CAP 4096    /* loop iterations per path (the reference has no cap) */
+
+typedef struct { v3 P, N, G; double cu, cv; } surfel_t;
+
+/* Moller-Trumbore in double; (u, v) weight vertices 1 and 2. */
+static int o_tri_hit(v3 v0, v3 v1, v3 v2, v3 o, v3 d, double tmax, double* t, double* bu, double* bv) {
+    v3 e1 = vsub(v1, v0), e2 = vsub(v2, v0);
+    v3 p = vcross(d, e2);
+    double det = vdot(e1, p);
+    if (fabs(det) < 1e-30) return 0;
+    double id = 1.0 / det;
+    v3 s = vsub(o, v0);
+    double u = vdot(s, p) * id;
+    if (u < 0 || u > 1) return 0;
+    v3 q = vcross(s, e1);
+    double w = vdot(d, q) * id;
+    if (w < 0 || u + w > 1) return 0;
+    double tt = vdot(e2, q) * id;
+    if (tt < 0 || tt >= tmax) return 0;
+    *t = tt; *bu = u; *bv = w;
+    return 1;
+}
+
+static v3 o_vertex(const vr_scene* S, int tri, int c) {
+    return vf(S->positions + 3 * (int64_t)S->indices[3 * (int64_t)tri + c]);
+}
+
+/* Closest triangle (original index) with 0 <= t < 10000, or -1. */
+static int o_scene_trace(const vr_scene* S, v3 o, v3 d, double* t, double* bu, double* bv) {
+    int best = -1;
+    double bt = SCENE_TMAX;
+    for (int k = 0; k < S->num_tris; k++) {
+        double tt, u, v;
+        if (o_tri_hit(o_vertex(S, k, 0), o_vertex(S, k, 1), o_vertex(S, k, 2), o, d, bt, &tt, &u, &v)) {
+            bt = tt; best = k; *bu = u; *bv = v;
+        }
+    }
+    *t = bt;
+    return best;
+}
+
+/* sample_surfel (include/common.h:467-501) + hit2surfel (:297-310). */
+static surfel_t o_scene_surfel(const vr_scene* S, int tri, double bu, double bv, v3 x, v3 w, double t, int* patch) {
+    surfel_t s;
+    int i0 = S->indices[3 * tri], i1 = S->indices[3 * tri + 1], i2 = S->indices[3 * tri + 2];
+    double a0 = 1 - bu - bv;
+    v3 p0 = vf(S->positions + 3 * i0), p1 = vf(S->positions + 3 * i1), p2 = vf(S->positions + 3 * i2);
+    s.P = vadd(vs(w, t), x);
+    s.G = vnorm(vcross(vsub(p1, p0), vsub(p2, p0)));
+    if (S->normals)
+        s.N = vnorm(vadd(vadd(vs(vf(S->normals + 3 * i0), a0), vs(vf(S->normals + 3 * i1), bu)), vs(vf(S->normals + 3 * i2), bv)));
+    else s.N = s.G;
+    if (S->uvs) {
+        s.cu = S->uvs[2 * i0] * a0 + S->uvs[2 * i1] * bu + S->uvs[2 * i2] * bv;
+        s.cv = S->uvs[2 * i0 + 1] * a0 + S->uvs[2 * i1 + 1] * bu + S->uvs[2 * i2 + 1] * bv;
+    } else s.cu = s.cv = 0;
+    *patch = S->tri_patch[tri];
+    return s;
+}
+
+/* INCLUDE_ADDITIONAL_COLLISION_VARS (include/common.h:509-512). */
+static void o_collision_vars(v3 win, const surfel_t* s, int* from_outside, v3* fN) {
+    *from_outside = vdot(win, s->G) < 0;
+    int correct = (vdot(win, s->N) < 0) == *from_outside;
+    v3 N = correct ? s->N : s->G;
+    *fN = *from_outside ? N : vs(N, -1.0);
+}
+
+static v3 o_reflect(v3 i, v3 n) { return vsub(i, vs(n, 2 * vdot(n, i))); }
+static v3 o_refract(v3 i, v3 n, double eta) {
+    double c = vdot(n, i);
+    double k = 1 - eta * eta * (1 - c * c);
+    if (k < 0) return V(0, 0, 0);
+    return vsub(vs(i, eta), vs(n, eta * c + sqrt(k)));
+}
+
+/* randomHSDirectionCosineWeighted (randoms.h:215-231), capped at 64 draws. */
+static v3 o_cosine_hemisphere(ctx_t* c, v3 N, double* ndotd) {
+    v3 t0, t1;
+    ortho_basis(N, &t0, &t1);
+    for (int it = 0; it < 64; it++) {
+        /* the GPU forms x, y and d2 in float; so does the oracle, so that the
+         * acceptance test takes the same branch */
+        float x = (float)rnd(c) * 2.0f - 1.0f;
+        float y = (float)rnd(c) * 2.0f - 1.0f;
+        float d2 = x * x + y * y;
+        if (d2 > 0.001f && d2 < 1.0f) {
+            double z = sqrt(1.0 - (double)d2);
+            *ndotd = z;
+            return vadd(vadd(vs(t0, x), vs(t1, y)), vs(N, z));
+        }
+    }
+    *ndotd = 1;
+    return N;
+}
+
+/* surface_scattering_sampler of a patch (_scenes.py:906-1340). */
+static void o_surface_sample(const vr_patch* P, ctx_t* c, v3 win, const surfel_t* s, v3* wout, v3* W, double* pdf) {
+    int kind = P->sampler;
+    if (kind == VR_SURF_PASS || kind == VR_SURF_NULL) { *wout = win; *W = V(1, 1, 1); *pdf = -1; return; }
+    if (kind == VR_SURF_BLOCK) { *wout = V(0, 0, 0); *W = V(0, 0, 0); *pdf = 0; return; }
+    int from_outside;
+    v3 fN;
+    o_collision_vars(win, s, &from_outside, &fN);
+    if (kind == VR_SURF_LAMBERT) {
+        if (!from_outside) { *wout = V(0, 0, 0); *pdf = 0; *W = V(0, 0, 0); }
+        else {
+            double ct;
+            *wout = o_cosine_hemisphere(c, fN, &ct);
+            *pdf = ct / PI_D;
+            *W = V(1, 1, 1);
+        }
+        return;
+    }
+    if (kind == VR_SURF_MIRROR) { *wout = o_reflect(win, fN); *W = V(1, 1, 1); *pdf = -1; return; }
+    /* Fresnel: the uniform is drawn only when the refraction exists (|| short-circuits) */
+    v3 rw = o_reflect(win, fN);
+    double eta = from_outside ? 1.0 / P->refraction_index : P->refraction_index;
+    v3 tw = o_refract(win, fN, eta);
+    double th = fabs(vdot(win, fN));
+    double f = pow((1 - eta) / (1 + eta), 2);
+    double R = f + (1 - f) * pow(1 - th, 5);
+    int no_t = tw.x == 0 && tw.y == 0 && tw.z == 0;
+    if (no_t || rnd(c) < R) *wout = rw;
+    else *wout = tw;
+    *W = V(1 / (eta * eta), 1 / (eta * eta), 1 / (eta * eta));
+    *pdf = -1;
+}
+
+/* The material's emission as the patch's surface gathering (_scenes.py:1392-1435, 2407-2423). */
+static v3 o_surface_emission(const vr_patch* P, v3 win, const surfel_t* s) {
+    if (P->emission == VR_EMIT_NONE) return V(0, 0, 0);
+    v3 E = vf(P->emission_c);
+    if (P->emission == VR_EMIT_ISOTROPIC) return vs(E, 0.5 / PI_D);
+    int from_outside;
+    v3 fN;
+    o_collision_vars(win, s, &from_outside, &fN);
+    return vs(E, vdot(vs(win, -1), fN) / PI_D);
+}
+
+static v3 o_scene_env(const vr_scene* S, v3 w) {
+    vr_medium m;
+    memset(&m, 0, sizeof(m));
+    m.env_kind = S->env_kind;
+    memcpy(m.env_c, S->env_c, sizeof(m.env_c));
+    m.env_image = S->env_image;
+    memcpy(m.env_image_bmin, S->env_image_bmin, sizeof(m.env_image_bmin));
+    memcpy(m.env_image_bmax, S->env_image_bmax, sizeof(m.env_image_bmax));
+    return env_eval(&m, w);
+}
+
+typedef struct { gacc_t acc[VR_SCENE_MAX_MEDIA]; } sacc_t;
+
+static int o_inside(const vr_scene* S, const vr_patch* P) {
+    return (P->inside_medium >= 0 && P->inside_medium < S->num_media) ? P->inside_medium : -1;
+}
+
+static void o_use_medium(ctx_t* c, const vr_scene* S, const sacc_t* a, int m) {
+    c->M = &S->media[m];
+    if (a) c->acc = a->acc[m];
+}
+
+typedef struct { int current, level, stacked[4]; } mstack_t;
+
+/* sc_path_tracing.h:84-101 */
+static void o_mstack_cross(mstack_t* m, int to_inside, int inside) {
+    if (to_inside) {
+        if (m->level >= 0) m->stacked[m->level] = m->current;
+        if (m->level < 3) m->level++;
+        m->current = inside;
+    } else {
+        m->level--;
+        m->current = m->level == -1 ? -1 : m->stacked[m->level < 0 ? 0 : m->level];
+    }
+}
+
+static int o_below(v3 W) { return W.x < 1e-7 && W.y < 1e-7 && W.z < 1e-7; }
+
+/* path_trace (sc_path_tracing.h:12-120); branch = path_trace_fw's 4 uniforms of
+ * branch set-up around the (zero) direct gathering (:124-128). */
+static v3 o_scene_path(const vr_scene* S, ctx_t* c, v3 x, v3 w, int branch) {
+    if (branch) { rnd(c); rnd(c); rnd(c); rnd(c); }
+    v3 A = V(0, 0, 0), W = V(1, 1, 1);
+    mstack_t ms = {-1, -1, {-1, -1, -1, -1}};
+    int bounces = 0;
+    for (int it = 0; it < SCENE_PATH_CAP; it++) {
+        if (o_below(W)) return A;
+        double d, bu, bv;
+        int tri = o_scene_trace(S, x, w, &d, &bu, &bv);
+        if (tri < 0) return vadd(A, vmul(W, o_scene_env(S, w)));
+        int patch;
+        surfel_t s = o_scene_surfel(S, tri, bu, bv, x, w, d, &patch);
+        const vr_patch* P = &S->patches[patch];
+        int from_inside = vdot(w, s.G) > 0;
+        if (ms.level == -1 && from_inside) { ms.level = 0; ms.current = o_inside(S, P); }
+        if (ms.current >= 0) {
+            double o[12];
+            o_use_medium(c, S, NULL, ms.current);
+            medium_dt_fw(c, x, w, d, o);
+            v3 xo = V(o[0], o[1], o[2]), wo = V(o[3], o[4], o[5]);
+            A = vadd(A, vmul(W, V(o[9], o[10], o[11])));
+            W = vmul(W, V(o[6], o[7], o[8]));
+            if (!(wo.x == w.x && wo.y == w.y && wo.z == w.z)) { x = xo; w = wo; continue; }
+        }
+        bounces++;
+        A = vadd(A, vmul(W, o_surface_emission(P, w, &s)));
+        v3 ws, Ws;
+        double pdf;
+        o_surface_sample(P, c, w, &s, &ws, &Ws, &pdf);
+        int to_inside = vdot(ws, s.G) < 0;
+        if (to_inside != from_inside) o_mstack_cross(&ms, to_inside, o_inside(S, P));
+        W = vmul(W, Ws);
+        w = ws;
+        if (pdf > 0 && bounces > 2) {
+            if (rnd(c) < 0.5) return A;
+            W = vs(W, 2);
+        }
+        x = vadd(s.P, vs(w, 0.0001));
+    }
+    return A;
+}
+
+/* path_trace_bw (sc_path_tracing.h:238-380). */
+static void o_scene_path_bw(const vr_scene* S, ctx_t* c, const sacc_t* acc, v3 x, v3 w, v3 out_A, v3 dL_dA) {
+    rnd(c); rnd(c); rnd(c); rnd(c);
+    v3 rem = out_A, W = V(1, 1, 1);
+    mstack_t ms = {-1, -1, {-1, -1, -1, -1}};
+    int bounces = 0;
+    for (int it = 0; it < SCENE_PATH_CAP; it++) {
+        if (o_below(W)) return;
+        double d, bu, bv;
+        int tri = o_scene_trace(S, x, w, &d, &bu, &bv);
+        if (tri < 0) return;
+        int patch;
+        surfel_t s = o_scene_surfel(S, tri, bu, bv, x, w, d, &patch);
+        const vr_patch* P = &S->patches[patch];
+        int from_inside = vdot(w, s.G) > 0;
+        if (ms.level == -1 && from_inside) { ms.level = 0; ms.current = o_inside(S, P); }
+        if (ms.current >= 0) {
+            double o[12];
+            o_use_medium(c, S, acc, ms.current);
+            rng_t before = c->rng;
+            medium_dt_fw(c, x, w, d, o);
+            v3 xo = V(o[0], o[1], o[2]), wo = V(o[3], o[4], o[5]), vW = V(o[6], o[7], o[8]), vA = V(o[9], o[10], o[11]);
+            rem = vsub(rem, vmul(W, vA));
+            v3 dL_dvA = vmul(dL_dA, W);
+            v3 g = vmul(dL_dA, rem);
+            double gv[3] = {g.x / vW.x, g.y / vW.y, g.z / vW.z};
+            for (int k = 0; k < 3; k++) if (!isfinite(gv[k])) gv[k] = 0;  /* vW == 0: dropped */
+            c->rng = before;
+            medium_dt_bw(c, x, w, d, V(gv[0], gv[1], gv[2]), dL_dvA);
+            W = vmul(W, vW);
+            if (!(wo.x == w.x && wo.y == w.y && wo.z == w.z)) { x = xo; w = wo; continue; }
+        }
+        bounces++;
+        rem = vsub(rem, vmul(W, o_surface_emission(P, w, &s)));
+        v3 ws, Ws;
+        double pdf;
+        o_surface_sample(P, c, w, &s, &ws, &Ws, &pdf);
+        int to_inside = vdot(ws, s.G) < 0;
+        if (to_inside != from_inside) o_mstack_cross(&ms, to_inside, o_inside(S, P));
+        W = vmul(W, Ws);
+        w = ws;
+        if (pdf > 0 && bounces > 2) {
+            if (rnd(c) < 0.5) return;
+            W = vs(W, 2);
+        }
+        x = vadd(s.P, vs(w, 0.0001));
+    }
+}
+
+/* PathtracedScene BACKWARD (maps/pathtraced_scene.h:13-25). */
+static void o_scene_map_bw(const vr_scene* S, ctx_t* c, const sacc_t* acc, v3 x, v3 w, v3 dL_dA) {
+    rng_t before = c->rng;
+    v3 A = o_scene_path(S, c, x, w, 1);
+    c->rng = before;
+    o_scene_path_bw(S, c, acc, x, w, A, dL_dA);
+}
+
+static int o_check_scene(const vr_scene* S) {
+    if (!S) return fail(VR_ERR_ARG, "scene is NULL");
+    if (S->num_tris < 0 || S->num_patches < 0) return fail(VR_ERR_ARG, "scene: negative sizes");
+    if (S->num_tris > 0 && (!S->positions || !S->indices || !S->tri_patch || !S->patches || S->num_patches < 1))
+        return fail(VR_ERR_ARG, "scene: NULL geometry buffer");
+    if (S->num_media < 0 || S->num_media > VR_SCENE_MAX_MEDIA) return fail(VR_ERR_UNSUPPORTED, "scene: too many media");
+    for (int k = 0; k < S->num_tris; k++)
+        if (S->tri_patch[k] < 0 || S->tri_patch[k] >= S->num_patches) return fail(VR_ERR_ARG, "scene: patch index out of range");
+    return VR_OK;
+}
+
+static void o_sacc_alloc(const vr_scene* S, sacc_t* a) {
+    memset(a, 0, sizeof(*a));
+    for (int m = 0; m < S->num_media; m++) {
+        const vr_medium* M = &S->media[m];
+        if (M->sigma.t.grad) a->acc[m].sig = (double*)calloc((size_t)grid_span(&M->sigma), sizeof(double));
+        if (M->albedo_kind == VR_FIELD_GRID && M->albedo.t.grad)
+            a->acc[m].alb = (double*)calloc((size_t)grid_span(&M->albedo), sizeof(double));
+    }
+}
+
+static void o_sacc_reduce_free(const vr_scene* S, sacc_t* a) {
+    for (int m = 0; m < S->num_media; m++) {
+        reduce_priv(&S->media[m].sigma, a->acc[m].sig);
+        if (a->acc[m].alb) reduce_priv(&S->media[m].albedo, a->acc[m].alb);
+        free(a->acc[m].sig);
+        free(a->acc[m].alb);
+    }
+}
+
+int oracle_scene_build_bvh(const float* positions, int64_t num_vertices, const int32_t* indices, int64_t num_tris,
+                           vr_bvh_node* nodes, int64_t max_nodes, int32_t* tri_ids, float* tri_verts, int32_t* num_nodes) {
+    /* The oracle needs no hierarchy: one leaf over every triangle in input order
+     * (its traversal is the brute-force loop above). */
+    if (!num_nodes || num_vertices < 0 || num_tris < 0) return fail(VR_ERR_ARG, "bad sizes");
+    *num_nodes = 0;
+    if (num_tris == 0) return VR_OK;
+    if (!positions || !indices || !nodes || !tri_ids || !tri_verts || max_nodes < 1) return fail(VR_ERR_ARG, "null buffer");
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    for (int64_t t = 0; t < num_tris; t++) {
+        tri_ids[t] = (int32_t)t;
+        for (int c = 0; c < 3; c++) {
+            int32_t v = indices[3 * t + c];
+            if (v < 0 || v >= num_vertices) return fail(VR_ERR_ARG, "vertex index out of range");
+            for (int a = 0; a < 3; a++) {
+                float p = positions[3 * (int64_t)v + a];
+                tri_verts[9 * t + 3 * c + a] = p;
+                if (p < lo[a]) lo[a] = p;
+                if (p > hi[a]) hi[a] = p;
+            }
+        }
+    }
+    for (int a = 0; a < 3; a++) { nodes[0].bmin[a] = lo[a]; nodes[0].bmax[a] = hi[a]; }
+    nodes[0].first = 0;
+    nodes[0].count = (int32_t)num_tris;
+    *num_nodes = 1;
+    return VR_OK;
+}
+
+int oracle_scene_raycast(const vr_scene* S, const float* rays, int64_t n, float* hits) {
+    int st = o_check_scene(S);
+    if (st) return st;
+    if (n > 0 && (!rays || !hits)) return fail(VR_ERR_ARG, "null buffer");
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; i++) {
+        v3 x = vf(rays + 6 * i), w = vf(rays + 6 * i + 3);
+        double t, bu, bv;
+        int tri = o_scene_trace(S, x, w, &t, &bu, &bv);
+        float* h = hits + 16 * i;
+        for (int k = 0; k < 15; k++) h[k] = 0;
+        if (tri < 0) { int32_t m1 = -1; memcpy(h + 15, &m1, 4); continue; }
+        int patch;
+        surfel_t s = o_scene_surfel(S, tri, bu, bv, x, w, t, &patch);
+        h[0] = (float)t;
+        h[1] = (float)s.N.x; h[2] = (float)s.N.y; h[3] = (float)s.N.z;
+        h[4] = (float)s.G.x; h[5] = (float)s.G.y; h[6] = (float)s.G.z;
+        h[7] = (float)s.cu; h[8] = (float)s.cv;
+        int32_t p32 = patch;
+        memcpy(h + 15, &p32, 4);
+    }
+    return VR_OK;
+}
+
+int oracle_scene_map_fw(const vr_scene* S, const float* rays, int64_t n, float* out, const vr_launch* L) {
+    int st = o_check_scene(S);
+    if (st) return st;
+    if (!L || (n > 0 && (!rays || !out))) return fail(VR_ERR_ARG, "null buffer");
+    int64_t start = L->start_index, cnt = L->count < 0 ? n - start : L->count;
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = start; i < start + cnt; i++) {
+        ctx_t c;
+        memset(&c, 0, sizeof(c));
+        c.bw_mode = L->bw_mode;
+        c.rng = seed_map(L->seeds, (uint32_t)i);
+        v3 A = o_scene_path(S, &c, vf(rays + 6 * i), vf(rays + 6 * i + 3), 0);
+        out[3 * i] = (float)A.x; out[3 * i + 1] = (float)A.y; out[3 * i + 2] = (float)A.z;
+        if (L->debug_out) L->debug_out[i] = (int32_t)c.lookups;
+    }
+    return VR_OK;
+}
+
+int oracle_scene_map_bw(const vr_scene* S, const float* rays, const float* og, int64_t n, const vr_launch* L) {
+    int st = o_check_scene(S);
+    if (st) return st;
+    if (!L || (n > 0 && (!rays || !og))) return fail(VR_ERR_ARG, "null buffer");
+    int64_t start = L->start_index, cnt = L->count < 0 ? n - start : L->count;
+    #pragma omp parallel
+    {
+        sacc_t acc;
+        o_sacc_alloc(S, &acc);
+        #pragma omp for schedule(dynamic, 16)
+        for (int64_t i = start; i < start + cnt; i++) {
+            v3 g = vf(og + 3 * i);
+            if (g.x == 0 && g.y == 0 && g.z == 0) continue;
+            ctx_t c;
+            memset(&c, 0, sizeof(c));
+            c.bw_mode = L->bw_mode;
+            c.rng = seed_map(L->seeds, (uint32_t)i);
+            o_scene_map_bw(S, &c, &acc, vf(rays + 6 * i), vf(rays + 6 * i + 3), g);
+        }
+        #pragma omp critical
+        o_sacc_reduce_free(S, &acc);
+    }
+    return VR_OK;
+}
+
+int oracle_scene_capture_fw(const vr_scene* S, const vr_camera* cam, const vr_launch* L, float* out) {
+    int st = o_check_scene(S);
+    if (st) return st;
+    if ((st = check_camera(cam))) return st;
+    if (!L || !out) return fail(VR_ERR_ARG, "null buffer");
+    int64_t total = camera_threads(cam);
+    int64_t start = L->start_index, cnt = L->count < 0 ? total - start : L->count;
+    int spp = L->samples < 1 ? 1 : L->samples;
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = start; i < start + cnt; i++) {
+        int pose, row, col;
+        sensor_index(cam, i, &pose, &row, &col);
+        if (pose < 0 || pose >= cam->num_poses) {
+            out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = 0;
+            if (L->debug_out) L->debug_out[i] = 0;
+            continue;
+        }
+        v3 acc = V(0, 0, 0);
+        int64_t lk = 0;
+        for (int s = 0; s < spp; s++) {
+            ctx_t c;
+            memset(&c, 0, sizeof(c));
+            c.bw_mode = L->bw_mode;
+            c.rng = seed_capture_fw(L->seeds, (uint32_t)((int64_t)s * total + i));
+            v3 x, w;
+            camera_ray(cam, pose, row, col, &c, &x, &w);
+            acc = vadd(acc, o_scene_path(S, &c, x, w, 0));
+            lk += c.lookups;
+        }
+        out[3 * i] = (float)(acc.x / spp); out[3 * i + 1] = (float)(acc.y / spp); out[3 * i + 2] = (float)(acc.z / spp);
+        if (L->debug_out) L->debug_out[i] = (int32_t)lk;
+    }
+    return VR_OK;
+}
+
+int oracle_scene_capture_bw(const vr_scene* S, const vr_camera* cam, const vr_launch* L, const float* og) {
+    int st = o_check_scene(S);
+    if (st) return st;
+    if ((st = check_camera(cam))) return st;
+    if (!L || !og) return fail(VR_ERR_ARG, "null buffer");
+    int64_t total = camera_threads(cam);
+    int64_t start = L->start_index, cnt = L->count < 0 ? total - start : L->count;
+    int spp = L->samples < 1 ? 1 : L->samples;
+    #pragma omp parallel
+    {
+        sacc_t acc;
+        o_sacc_alloc(S, &acc);
+        #pragma omp for schedule(dynamic, 4)
+        for (int64_t i = start; i < start + cnt; i++) {
+            int pose, row, col;
+            sensor_index(cam, i, &pose, &row, &col);
+            if (pose < 0 || pose >= cam->num_poses) continue;
+            v3 g = vs(vf(og + 3 * i), 1.0 / spp);
+            if (g.x == 0 && g.y == 0 && g.z == 0) continue;
+            ctx_t c;
+            memset(&c, 0, sizeof(c));
+            c.bw_mode = L->bw_mode;
+            if (L->seed_mode == VR_SEED_CAPTURE_BW) c.rng = seed_capture_bw(L->seeds, (uint32_t)i);
+            for (int s = 0; s < spp; s++) {
+                if (L->seed_mode != VR_SEED_CAPTURE_BW)
+                    c.rng = seed_capture_fw(L->seeds, (uint32_t)((int64_t)s * total + i));
+                v3 x, w;
+                camera_ray(cam, pose, row, col, &c, &x, &w);
+                o_scene_map_bw(S, &c, &acc, x, w, g);
+            }
+        }
+        #pragma omp critical
+        o_sacc_reduce_free(S, &acc);
+    }
+    return VR_OK;
+}

@@ -10,9 +10,11 @@ torch's current HIP device and the library is loaded on first use.
 from ._internal import seed, get_seeds, device, backward_mode, backward_settings
 from .maps import *  # noqa: F401,F403
 from .maps import __all__ as _maps_all
-from .scenes import HGPhaseSampler, DTMediumPathIntegrator
+from .scenes import *  # noqa: F401,F403
+from .scenes import __all__ as _scenes_all
 from . import functions
 from . import objects
+from . import scenes
 
-__all__ = ["seed", "get_seeds", "device", "backward_mode", "backward_settings", "HGPhaseSampler",
-           "DTMediumPathIntegrator", "functions", "objects", *_maps_all]
+__all__ = ["seed", "get_seeds", "device", "backward_mode", "backward_settings", "functions", "objects", "scenes",
+           *_maps_all, *_scenes_all]

@@ -39,6 +39,11 @@ VR_MAP_T_GDT = 12
 VR_MAP_T_GRT = 13
 VR_SEED_MAP, VR_SEED_CAPTURE_FW, VR_SEED_CAPTURE_BW = 0, 1, 2
 VR_BW_REFERENCE, VR_BW_CONSISTENT = 0, 1
+VR_SURF_PASS, VR_SURF_LAMBERT, VR_SURF_MIRROR, VR_SURF_FRESNEL, VR_SURF_NULL, VR_SURF_BLOCK = 0, 1, 2, 3, 4, 5
+VR_EMIT_NONE, VR_EMIT_ISOTROPIC, VR_EMIT_LAMBERT = 0, 1, 2
+VR_SCENE_MAX_MEDIA = 4
+# Python-side map kind of a PathtracedScene (evaluated by the vr_scene_* entry points)
+MAP_SCENE = 100
 
 MAP_DIMS = {
     VR_MAP_RAYBOX: (6, 2),
@@ -54,6 +59,7 @@ MAP_DIMS = {
     VR_MAP_NEE_DRT: (6, 3),
     VR_MAP_T_GDT: (6, 1),
     VR_MAP_T_GRT: (6, 1),
+    MAP_SCENE: (6, 3),
 }
 
 F3 = ctypes.c_float * 3
@@ -138,6 +144,46 @@ class VrLaunch(ctypes.Structure):
     ]
 
 
+class VrBvhNode(ctypes.Structure):
+    _fields_ = [("bmin", F3), ("first", ctypes.c_int32), ("bmax", F3), ("count", ctypes.c_int32)]
+
+
+class VrPatch(ctypes.Structure):
+    _fields_ = [
+        ("sampler", ctypes.c_int32),
+        ("refraction_index", ctypes.c_float),
+        ("emission", ctypes.c_int32),
+        ("emission_c", F3),
+        ("inside_medium", ctypes.c_int32),
+        ("outside_medium", ctypes.c_int32),
+    ]
+
+
+class VrScene(ctypes.Structure):
+    _fields_ = [
+        ("nodes", ctypes.c_void_p),
+        ("tri_verts", ctypes.c_void_p),
+        ("tri_ids", ctypes.c_void_p),
+        ("num_nodes", ctypes.c_int32),
+        ("num_tris", ctypes.c_int32),
+        ("positions", ctypes.c_void_p),
+        ("normals", ctypes.c_void_p),
+        ("uvs", ctypes.c_void_p),
+        ("indices", ctypes.c_void_p),
+        ("tri_patch", ctypes.c_void_p),
+        ("patches", ctypes.c_void_p),
+        ("num_patches", ctypes.c_int32),
+        ("num_vertices", ctypes.c_int32),
+        ("media", ctypes.c_void_p),
+        ("num_media", ctypes.c_int32),
+        ("env_kind", ctypes.c_int32),
+        ("env_c", F3),
+        ("env_image_bmin", ctypes.c_float * 2),
+        ("env_image_bmax", ctypes.c_float * 2),
+        ("env_image", VrTensor),
+    ]
+
+
 P = ctypes.POINTER
 _SIGNATURES = {
     "vr_abi_version": (ctypes.c_int, []),
@@ -182,6 +228,15 @@ _SIGNATURES = {
                                     ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
     "vr_nrc_render": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_scene_build_bvh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                          ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_int32)]),
+    "vr_scene_raycast": (ctypes.c_int, [P(VrScene), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_scene_map_fw": (ctypes.c_int, [P(VrScene), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, P(VrLaunch),
+                                       ctypes.c_void_p]),
+    "vr_scene_map_bw": (ctypes.c_int, [P(VrScene), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, P(VrLaunch),
+                                       ctypes.c_void_p]),
+    "vr_scene_capture_fw": (ctypes.c_int, [P(VrScene), P(VrCamera), P(VrLaunch), ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_scene_capture_bw": (ctypes.c_int, [P(VrScene), P(VrCamera), P(VrLaunch), ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 def i64s(values) -> ctypes.Array:

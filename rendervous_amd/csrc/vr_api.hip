@@ -16,11 +16,13 @@
 #include <cstring>
 #include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "vr_device.hpp"
 #include "vr_nee_wavefront.hpp"
 #include "vr_nrc.hpp"
 #include "vr_functions.hpp"
+#include "vr_scene.hpp"
 
 using namespace vr;
 
@@ -688,6 +690,263 @@ int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Se
     return check_hip(hipGetLastError(), "vr_capture_bw launch");
 }
 
+// ------------------------------------------------------------- scenes --
+constexpr int kSceneBlock = 128;  // LDS traversal stacks: 32 x 128 x 4 B = 16 KB per workgroup
+static_assert(sizeof(SceneView) <= 3072, "SceneView must fit the kernel argument segment");
+
+int make_scene_view(const vr_scene* sc, int bw_mode, SceneView& S) {
+    if (!sc) return set_err(VR_ERR_ARG, "scene is NULL");
+    std::memset(&S, 0, sizeof(S));
+    if (sc->num_tris < 0 || sc->num_nodes < 0 || sc->num_patches < 0 || sc->num_vertices < 0)
+        return set_err(VR_ERR_ARG, "scene: negative sizes");
+    if (sc->num_tris > 0) {
+        if (!sc->nodes || !sc->tri_verts || !sc->tri_ids || !sc->positions || !sc->indices || !sc->tri_patch || !sc->patches)
+            return set_err(VR_ERR_ARG, "scene: NULL geometry buffer");
+        if (sc->num_nodes < 1 || sc->num_nodes > 2 * sc->num_tris) return set_err(VR_ERR_ARG, "scene: node count out of range");
+        if (sc->num_patches < 1) return set_err(VR_ERR_ARG, "scene: no patches");
+    }
+    if (sc->num_media < 0 || sc->num_media > VR_SCENE_MAX_MEDIA)
+        return set_err(VR_ERR_UNSUPPORTED, "scene: at most VR_SCENE_MAX_MEDIA media");
+    if (sc->num_media > 0 && !sc->media) return set_err(VR_ERR_ARG, "scene: media is NULL");
+    S.nodes = reinterpret_cast<const float4*>(sc->nodes);
+    S.tri_verts = sc->tri_verts;
+    S.tri_ids = sc->tri_ids;
+    S.num_nodes = sc->num_nodes;
+    S.num_tris = sc->num_tris;
+    S.pos = sc->positions;
+    S.nrm = sc->normals;
+    S.uv = sc->uvs;
+    S.idx = sc->indices;
+    S.tri_patch = sc->tri_patch;
+    S.patches = sc->patches;
+    S.num_patches = sc->num_patches;
+    S.num_media = sc->num_media;
+    int st;
+    for (int m = 0; m < sc->num_media; m++)
+        if ((st = make_medium_view(&sc->media[m], VR_MAP_MEDIUM_DT, bw_mode, S.media[m]))) return st;
+    // environment: the same checks as a medium's environment
+    vr_medium env;
+    std::memset(&env, 0, sizeof(env));
+    env.env_kind = sc->env_kind;
+    env.env_image = sc->env_image;
+    for (int a = 0; a < 2; a++) { env.env_image_bmin[a] = sc->env_image_bmin[a]; env.env_image_bmax[a] = sc->env_image_bmax[a]; }
+    env.env_sampler_kind = VR_ENVS_UNIFORM;
+    MediumView ev;
+    std::memset(&ev, 0, sizeof(ev));
+    if ((st = make_env_view(&env, ev))) return st;
+    S.env_kind = sc->env_kind;
+    S.env_c = load3(sc->env_c);
+    S.env_img = ev.env_img;
+    return VR_OK;
+}
+
+__global__ void __launch_bounds__(kSceneBlock) k_scene_raycast(SceneView S, const float* __restrict__ rays, int64_t n,
+                                                               float* __restrict__ hits) {
+    __shared__ int stk[kSceneStack * kSceneBlock];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 x = load3(rays + 6 * i), w = load3(rays + 6 * i + 3);
+    float t, bu, bv;
+    const int k = scene_trace<kSceneBlock>(S, x, w, t, bu, bv, stk + threadIdx.x);
+    float* h = hits + 16 * i;
+    if (k < 0) {  // noHit2array (include/common.h:325-336)
+        for (int c = 0; c < 15; c++) h[c] = 0.0f;
+        h[15] = __int_as_float(-1);
+        return;
+    }
+    int patch;
+    const Surfel s = scene_surfel(S, k, bu, bv, x, w, t, patch);
+    // surfel2array (include/common.h:312-323)
+    h[0] = t;
+    h[1] = s.N.x; h[2] = s.N.y; h[3] = s.N.z;
+    h[4] = s.G.x; h[5] = s.G.y; h[6] = s.G.z;
+    h[7] = s.cu; h[8] = s.cv;
+    for (int c = 9; c < 15; c++) h[c] = 0.0f;
+    h[15] = __int_as_float(patch);
+}
+
+__global__ void __launch_bounds__(kSceneBlock) k_scene_map_fw(SceneView S, const float* __restrict__ rays, int64_t start,
+                                                              int64_t cnt, float* __restrict__ out, Seeds seeds,
+                                                              int* __restrict__ dbg) {
+    __shared__ int stk[kSceneStack * kSceneBlock];
+    const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= start + cnt) return;
+    Rng rng = seed_map(seeds, (uint32_t)i);
+    int lookups = 0;
+    const f3 A = scene_path<kSceneBlock>(S, rng, load3(rays + 6 * i), load3(rays + 6 * i + 3), false, stk + threadIdx.x, lookups);
+    out[3 * i] = A.x; out[3 * i + 1] = A.y; out[3 * i + 2] = A.z;
+    if (dbg) dbg[i] = lookups;
+}
+
+__global__ void __launch_bounds__(kSceneBlock) k_scene_map_bw(SceneView S, const float* __restrict__ rays,
+                                                              const float* __restrict__ og, int64_t start, int64_t cnt,
+                                                              Seeds seeds) {
+    __shared__ int stk[kSceneStack * kSceneBlock];
+    const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= start + cnt) return;
+    const f3 g = load3(og + 3 * i);
+    if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) return;
+    Rng rng = seed_map(seeds, (uint32_t)i);
+    int lookups = 0, scatters = 0;
+    scene_map_bw<kSceneBlock>(S, rng, load3(rays + 6 * i), load3(rays + 6 * i + 3), g, stk + threadIdx.x, lookups, scatters);
+}
+
+// Capture of a PathtracedScene: thread j takes sensor start + j % cnt and samples
+// [c*chunk, ...) with c = j / cnt, as k_capture_fw.
+__global__ void __launch_bounds__(kSceneBlock) k_scene_capture_fw(SceneView S, CameraView cam, Seeds seeds, CaptureSplit P,
+                                                                  int64_t total, int64_t start, int64_t cnt,
+                                                                  float* __restrict__ out, int* __restrict__ dbg) {
+    __shared__ int stk[kSceneStack * kSceneBlock];
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt * P.chunks) return;
+    const int64_t c = j / cnt;
+    const int64_t i = start + (j - c * cnt);
+    const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
+    int pose, row, col;
+    sensor_index(cam, i, pose, row, col);
+    if (pose < 0 || pose >= cam.num_poses) {
+        if (P.chunks == 1) { out[3 * i] = 0.0f; out[3 * i + 1] = 0.0f; out[3 * i + 2] = 0.0f; }
+        return;
+    }
+    f3 acc = F3(0.0f, 0.0f, 0.0f);
+    int lookups = 0;
+    for (int s = s0; s < s1; s++) {
+        Rng rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
+        f3 x, w;
+        camera_ray(cam, pose, row, col, rng, x, w);
+        acc = acc + scene_path<kSceneBlock>(S, rng, x, w, false, stk + threadIdx.x, lookups);
+    }
+    const float inv = 1.0f / (float)P.spp;
+    if (P.chunks == 1) {
+        out[3 * i] = acc.x * inv; out[3 * i + 1] = acc.y * inv; out[3 * i + 2] = acc.z * inv;
+        if (dbg) dbg[i] = lookups;
+    } else {
+        atomic_addf(out + 3 * i, acc.x * inv);
+        atomic_addf(out + 3 * i + 1, acc.y * inv);
+        atomic_addf(out + 3 * i + 2, acc.z * inv);
+        if (dbg) atomicAdd(dbg + i, lookups);
+    }
+}
+
+__global__ void __launch_bounds__(kSceneBlock) k_scene_capture_bw(SceneView S, CameraView cam, Seeds seeds, CaptureSplit P,
+                                                                  int seed_mode, int64_t total, int64_t start, int64_t cnt,
+                                                                  const float* __restrict__ og, int* __restrict__ dbg) {
+    __shared__ int stk[kSceneStack * kSceneBlock];
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt * P.chunks) return;
+    const int64_t c = j / cnt;
+    const int64_t i = start + (j - c * cnt);
+    const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
+    int pose, row, col;
+    sensor_index(cam, i, pose, row, col);
+    if (pose < 0 || pose >= cam.num_poses) return;
+    const f3 g = load3(og + 3 * i) * (1.0f / (float)P.spp);
+    if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) {
+        if (dbg && P.chunks == 1) dbg[i] = 0;
+        return;
+    }
+    int lookups = 0, scatters = 0;
+    Rng rng;
+    if (seed_mode == VR_SEED_CAPTURE_BW) rng = seed_capture_bw(seeds, (uint32_t)i);
+    for (int s = s0; s < s1; s++) {
+        if (seed_mode != VR_SEED_CAPTURE_BW) rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
+        f3 x, w;
+        camera_ray(cam, pose, row, col, rng, x, w);
+        scene_map_bw<kSceneBlock>(S, rng, x, w, g, stk + threadIdx.x, lookups, scatters);
+    }
+    if (dbg) {
+        const int n = P.debug_lookups ? lookups : scatters;
+        if (P.chunks == 1) dbg[i] = n;
+        else atomicAdd(dbg + i, n);
+    }
+}
+
+// ---- host BVH builder (binned SAH over triangle centroids) ----
+struct BvhBuilder {
+    struct Box {
+        float lo[3], hi[3];
+        void reset() { for (int a = 0; a < 3; a++) { lo[a] = 3.0e38f; hi[a] = -3.0e38f; } }
+        void grow(const float* p) { for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); } }
+        void grow(const Box& b) { for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], b.lo[a]); hi[a] = std::max(hi[a], b.hi[a]); } }
+        float area() const {
+            const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+            return (dx < 0 || dy < 0 || dz < 0) ? 0.0f : 2.0f * (dx * dy + dy * dz + dz * dx);
+        }
+    };
+    static constexpr int kBins = 16, kLeaf = 4, kMaxDepth = 30;
+    std::vector<Box> tri_box;
+    std::vector<float> cent;   // [T,3]
+    std::vector<int32_t> order;
+    vr_bvh_node* nodes;
+    int64_t max_nodes, used;
+
+    int build(int64_t node, int64_t b, int64_t e, int depth) {
+        Box bb, cb;
+        bb.reset(); cb.reset();
+        for (int64_t k = b; k < e; k++) { bb.grow(tri_box[order[k]]); cb.grow(&cent[3 * (size_t)order[k]]); }
+        vr_bvh_node& N = nodes[node];
+        for (int a = 0; a < 3; a++) { N.bmin[a] = bb.lo[a]; N.bmax[a] = bb.hi[a]; }
+        const int64_t n = e - b;
+        if (n <= kLeaf || depth >= kMaxDepth) { N.first = (int32_t)b; N.count = (int32_t)n; return VR_OK; }
+        // best binned SAH split over the three axes
+        int best_axis = -1, best_bin = -1;
+        float best_cost = 3.0e38f;
+        for (int a = 0; a < 3; a++) {
+            const float ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.0f)) continue;
+            Box bin_box[kBins];
+            int64_t bin_n[kBins] = {0};
+            for (int q = 0; q < kBins; q++) bin_box[q].reset();
+            const float sc = kBins / ext;
+            for (int64_t k = b; k < e; k++) {
+                const int t = order[k];
+                int q = (int)((cent[3 * (size_t)t + a] - cb.lo[a]) * sc);
+                q = std::min(std::max(q, 0), kBins - 1);
+                bin_n[q]++;
+                bin_box[q].grow(tri_box[t]);
+            }
+            float left_area[kBins];
+            int64_t left_n[kBins];
+            Box acc;
+            acc.reset();
+            int64_t cntl = 0;
+            for (int q = 0; q < kBins - 1; q++) {
+                acc.grow(bin_box[q]); cntl += bin_n[q];
+                left_area[q] = acc.area(); left_n[q] = cntl;
+            }
+            acc.reset();
+            int64_t cntr = 0;
+            for (int q = kBins - 1; q >= 1; q--) {
+                acc.grow(bin_box[q]); cntr += bin_n[q];
+                if (left_n[q - 1] == 0 || cntr == 0) continue;
+                const float cost = left_area[q - 1] * (float)left_n[q - 1] + acc.area() * (float)cntr;
+                if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = q; }
+            }
+        }
+        int64_t mid;
+        if (best_axis < 0) {
+            mid = b + n / 2;  // all centroids coincide: split by index
+        } else {
+            const float ext = cb.hi[best_axis] - cb.lo[best_axis], sc = kBins / ext;
+            const float lo = cb.lo[best_axis];
+            auto it = std::partition(order.begin() + b, order.begin() + e, [&](int32_t t) {
+                int q = (int)((cent[3 * (size_t)t + best_axis] - lo) * sc);
+                return std::min(std::max(q, 0), kBins - 1) < best_bin;
+            });
+            mid = it - order.begin();
+            if (mid == b || mid == e) mid = b + n / 2;
+        }
+        if (used + 2 > max_nodes) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: node buffer too small");
+        const int64_t l = used;
+        used += 2;
+        N.first = (int32_t)l;
+        N.count = 0;
+        int st = build(l, b, mid, depth + 1);
+        if (st) return st;
+        return build(l + 1, mid, e, depth + 1);
+    }
+};
+
 int resolve_range(const vr_launch* L, int64_t total, int64_t& start, int64_t& cnt) {
     start = L->start_index;
     if (start < 0 || start > total) return set_err(VR_ERR_ARG, "start_index out of range");
@@ -1145,6 +1404,144 @@ int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_lau
         hipLaunchKernelGGL(nrc::k_nrc_apply, dim3(nrc_blocks(max_queries, nrc::kInferWaves, 2048)), dim3(64 * nrc::kInferWaves), 0,
                            s, queries, count, max_queries, o, ib, (const __bf16*)weights, out);
     return check_hip(hipGetLastError(), "vr_nrc_render apply");
+}
+
+
+int vr_scene_build_bvh(const float* positions, int64_t num_vertices, const int32_t* indices, int64_t num_tris,
+                       vr_bvh_node* nodes, int64_t max_nodes, int32_t* tri_ids, float* tri_verts, int32_t* num_nodes) {
+    if (num_vertices < 0 || num_tris < 0) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: negative sizes");
+    if (num_tris > (int64_t(1) << 30)) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: more than 2^30 triangles");
+    if (!num_nodes) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: num_nodes is NULL");
+    *num_nodes = 0;
+    if (num_tris == 0) return VR_OK;
+    if (!positions || !indices || !nodes || !tri_ids || !tri_verts) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: NULL buffer");
+    if (max_nodes < 1) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: node buffer too small");
+    BvhBuilder B;
+    B.tri_box.resize((size_t)num_tris);
+    B.cent.resize(3 * (size_t)num_tris);
+    B.order.resize((size_t)num_tris);
+    for (int64_t t = 0; t < num_tris; t++) {
+        BvhBuilder::Box& bx = B.tri_box[(size_t)t];
+        bx.reset();
+        for (int c = 0; c < 3; c++) {
+            const int32_t v = indices[3 * t + c];
+            if (v < 0 || v >= num_vertices) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: vertex index out of range");
+            bx.grow(positions + 3 * (int64_t)v);
+        }
+        for (int a = 0; a < 3; a++) {
+            if (!std::isfinite(bx.lo[a]) || !std::isfinite(bx.hi[a])) return set_err(VR_ERR_ARG, "vr_scene_build_bvh: non-finite position");
+            B.cent[3 * (size_t)t + a] = 0.5f * (bx.lo[a] + bx.hi[a]);
+        }
+        B.order[(size_t)t] = (int32_t)t;
+    }
+    B.nodes = nodes;
+    B.max_nodes = max_nodes;
+    B.used = 1;
+    int st = B.build(0, 0, num_tris, 0);
+    if (st) return st;
+    for (int64_t k = 0; k < num_tris; k++) {
+        const int32_t t = B.order[(size_t)k];
+        tri_ids[k] = t;
+        for (int c = 0; c < 3; c++) {
+            const float* p = positions + 3 * (int64_t)indices[3 * (int64_t)t + c];
+            tri_verts[9 * k + 3 * c] = p[0];
+            tri_verts[9 * k + 3 * c + 1] = p[1];
+            tri_verts[9 * k + 3 * c + 2] = p[2];
+        }
+    }
+    *num_nodes = (int32_t)B.used;
+    return VR_OK;
+}
+
+int vr_scene_raycast(const vr_scene* scene, const float* rays, int64_t n, float* hits, void* stream) {
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_scene_raycast: negative n");
+    SceneView S;
+    int st = make_scene_view(scene, VR_BW_CONSISTENT, S);
+    if (st) return st;
+    if (n == 0) return VR_OK;
+    if (!rays || !hits) return set_err(VR_ERR_ARG, "vr_scene_raycast: NULL buffer");
+    hipLaunchKernelGGL(k_scene_raycast, dim3((unsigned)((n + kSceneBlock - 1) / kSceneBlock)), dim3(kSceneBlock), 0,
+                       (hipStream_t)stream, S, rays, n, hits);
+    return check_hip(hipGetLastError(), "vr_scene_raycast launch");
+}
+
+int vr_scene_map_fw(const vr_scene* scene, const float* rays, int64_t n, float* out, const vr_launch* L, void* stream) {
+    if (!L) return set_err(VR_ERR_ARG, "vr_scene_map_fw: launch is NULL");
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_scene_map_fw: negative n");
+    SceneView S;
+    int st = make_scene_view(scene, L->bw_mode, S);
+    if (st) return st;
+    int64_t start, cnt;
+    if ((st = resolve_range(L, n, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!rays || !out) return set_err(VR_ERR_ARG, "vr_scene_map_fw: NULL buffer");
+    hipLaunchKernelGGL(k_scene_map_fw, dim3((unsigned)((cnt + kSceneBlock - 1) / kSceneBlock)), dim3(kSceneBlock), 0,
+                       (hipStream_t)stream, S, rays, start, cnt, out, make_seeds(L), L->debug_out);
+    return check_hip(hipGetLastError(), "vr_scene_map_fw launch");
+}
+
+int vr_scene_map_bw(const vr_scene* scene, const float* rays, const float* out_grad, int64_t n, const vr_launch* L,
+                    void* stream) {
+    if (!L) return set_err(VR_ERR_ARG, "vr_scene_map_bw: launch is NULL");
+    if (n < 0) return set_err(VR_ERR_ARG, "vr_scene_map_bw: negative n");
+    SceneView S;
+    int st = make_scene_view(scene, L->bw_mode, S);
+    if (st) return st;
+    int64_t start, cnt;
+    if ((st = resolve_range(L, n, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!rays || !out_grad) return set_err(VR_ERR_ARG, "vr_scene_map_bw: NULL buffer");
+    hipLaunchKernelGGL(k_scene_map_bw, dim3((unsigned)((cnt + kSceneBlock - 1) / kSceneBlock)), dim3(kSceneBlock), 0,
+                       (hipStream_t)stream, S, rays, out_grad, start, cnt, make_seeds(L));
+    return check_hip(hipGetLastError(), "vr_scene_map_bw launch");
+}
+
+int vr_scene_capture_fw(const vr_scene* scene, const vr_camera* camera, const vr_launch* L, float* out, void* stream) {
+    if (!L) return set_err(VR_ERR_ARG, "vr_scene_capture_fw: launch is NULL");
+    SceneView S;
+    int st = make_scene_view(scene, L->bw_mode, S);
+    if (st) return st;
+    CameraView C;
+    if ((st = make_camera_view(camera, C))) return st;
+    const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    if (total > 0xFFFFFFFFll) return set_err(VR_ERR_ARG, "vr_scene_capture_fw: more than 2^32 sensors");
+    int64_t start, cnt;
+    if ((st = resolve_range(L, total, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!out) return set_err(VR_ERR_ARG, "vr_scene_capture_fw: out is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const int spp = L->samples < 1 ? 1 : L->samples;
+    const CaptureSplit P = capture_split(spp, false);
+    if ((st = capture_prepare(P, out, 3, start, cnt, L->debug_out, s))) return st;
+    const int64_t threads = cnt * P.chunks;
+    hipLaunchKernelGGL(k_scene_capture_fw, dim3((unsigned)((threads + kSceneBlock - 1) / kSceneBlock)), dim3(kSceneBlock), 0, s,
+                       S, C, make_seeds(L), P, total, start, cnt, out, L->debug_out);
+    return check_hip(hipGetLastError(), "vr_scene_capture_fw launch");
+}
+
+int vr_scene_capture_bw(const vr_scene* scene, const vr_camera* camera, const vr_launch* L, const float* out_grad,
+                        void* stream) {
+    if (!L) return set_err(VR_ERR_ARG, "vr_scene_capture_bw: launch is NULL");
+    SceneView S;
+    int st = make_scene_view(scene, L->bw_mode, S);
+    if (st) return st;
+    CameraView C;
+    if ((st = make_camera_view(camera, C))) return st;
+    const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    if (total > 0xFFFFFFFFll) return set_err(VR_ERR_ARG, "vr_scene_capture_bw: more than 2^32 sensors");
+    int64_t start, cnt;
+    if ((st = resolve_range(L, total, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!out_grad) return set_err(VR_ERR_ARG, "vr_scene_capture_bw: out_grad is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const int spp = L->samples < 1 ? 1 : L->samples;
+    CaptureSplit P = capture_split(spp, L->seed_mode == VR_SEED_CAPTURE_BW);
+    P.debug_lookups = L->debug_kind == 1;
+    if ((st = capture_prepare(P, nullptr, 0, start, cnt, L->debug_out, s))) return st;
+    const int64_t threads = cnt * P.chunks;
+    hipLaunchKernelGGL(k_scene_capture_bw, dim3((unsigned)((threads + kSceneBlock - 1) / kSceneBlock)), dim3(kSceneBlock), 0, s,
+                       S, C, make_seeds(L), P, L->seed_mode, total, start, cnt, out_grad, L->debug_out);
+    return check_hip(hipGetLastError(), "vr_scene_capture_bw launch");
 }
 
 }  // extern "C"

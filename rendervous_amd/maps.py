@@ -882,6 +882,8 @@ class DRTPathIntegrator(_RayMap):
 # ------------------------------------------------------------- autograd --
 def _grad_targets(lw: _Lowered, params: typing.Sequence[torch.Tensor]):
     """Fresh zero gradient buffers for the parameters that require grad."""
+    if hasattr(lw, "grad_targets"):  # a lowered scene: one buffer per media grid
+        return lw.grad_targets(params)
     grads = {}
     if lw.sigma is not None and lw.sigma.grid.requires_grad:
         grads["sigma"] = torch.zeros_like(lw.sigma.grid)
@@ -916,7 +918,12 @@ class AutogradMapFunction(torch.autograd.Function):
         out = torch.empty(*x.shape[:-1], out_dim, device=x.device, dtype=torch.float32)
         seeds = _internal.get_seeds()
         launch = L.launch(seeds, bw_mode=_bw_mode(), seed_mode=L.VR_SEED_MAP)
-        L.check(L.load().vr_map_fw(lw.medium, kind, L.ptr(xc), n, L.ptr(out), launch, L.stream_handle(x.device)), "vr_map_fw")
+        if kind == L.MAP_SCENE:
+            L.check(L.load().vr_scene_map_fw(lw.scene, L.ptr(xc), n, L.ptr(out), launch, L.stream_handle(x.device)),
+                    "vr_scene_map_fw")
+        else:
+            L.check(L.load().vr_map_fw(lw.medium, kind, L.ptr(xc), n, L.ptr(out), launch, L.stream_handle(x.device)),
+                    "vr_map_fw")
         ctx.save_for_backward(xc)
         ctx.map_object = map_object
         ctx.seeds = seeds
@@ -934,8 +941,12 @@ class AutogradMapFunction(torch.autograd.Function):
             seeds = ctx.seeds if _internal.backward_settings.seeds == "replay" else _internal.get_seeds()
             launch = L.launch(seeds, bw_mode=_bw_mode(), seed_mode=L.VR_SEED_MAP)
             og = og.to(torch.float32).contiguous()
-            L.check(L.load().vr_map_bw(lw.medium, kind, L.ptr(xc), L.ptr(og), xc.numel() // in_dim, launch,
-                                       L.stream_handle(xc.device)), "vr_map_bw")
+            if kind == L.MAP_SCENE:
+                L.check(L.load().vr_scene_map_bw(lw.scene, L.ptr(xc), L.ptr(og), xc.numel() // in_dim, launch,
+                                                 L.stream_handle(xc.device)), "vr_scene_map_bw")
+            else:
+                L.check(L.load().vr_map_bw(lw.medium, kind, L.ptr(xc), L.ptr(og), xc.numel() // in_dim, launch,
+                                           L.stream_handle(xc.device)), "vr_map_bw")
         return (None, None, *grads)
 
 
@@ -967,8 +978,12 @@ class AutogradCaptureFunction(torch.autograd.Function):
             for start in range(0, total, max(1, bs)):
                 launch = L.launch(seeds, samples=fw_samples, bw_mode=_bw_mode(), start_index=start,
                                   count=min(bs, total - start), debug_out=debug_out)
-                L.check(lib.vr_capture_fw(lw.medium, kind, cam, launch, L.ptr(out), L.stream_handle(out.device)),
-                        "vr_capture_fw")
+                if kind == L.MAP_SCENE:
+                    L.check(lib.vr_scene_capture_fw(lw.scene, cam, launch, L.ptr(out), L.stream_handle(out.device)),
+                            "vr_scene_capture_fw")
+                else:
+                    L.check(lib.vr_capture_fw(lw.medium, kind, cam, launch, L.ptr(out), L.stream_handle(out.device)),
+                            "vr_capture_fw")
         ctx.field = field
         ctx.capture_object = capture_object
         ctx.sensors = sensors
@@ -1001,8 +1016,12 @@ class AutogradCaptureFunction(torch.autograd.Function):
                 launch = L.launch(seeds, samples=samples, bw_mode=_bw_mode(), seed_mode=seed_mode,
                                   start_index=start, count=min(bs, total - start), debug_out=ctx.debug_out,
                                   debug_kind=int(_internal.backward_settings.debug == "lookups"))
-                L.check(lib.vr_capture_bw(lw.medium, kind, cam, launch, L.ptr(og), L.stream_handle(og.device)),
-                        "vr_capture_bw")
+                if kind == L.MAP_SCENE:
+                    L.check(lib.vr_scene_capture_bw(lw.scene, cam, launch, L.ptr(og), L.stream_handle(og.device)),
+                            "vr_scene_capture_bw")
+                else:
+                    L.check(lib.vr_capture_bw(lw.medium, kind, cam, launch, L.ptr(og), L.stream_handle(og.device)),
+                            "vr_capture_bw")
         return (None, None, None, None, None, None, None, *grads)
 
 

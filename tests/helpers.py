@@ -404,3 +404,194 @@ def match_fraction(a: np.ndarray, b: np.ndarray, rtol=1e-3, atol=1e-5) -> float:
 
 def rel_l1(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-30))
+
+
+# ------------------------------------------------------------------ scenes --
+def box_mesh(half: float = 1.0, center=(0.0, 0.0, 0.0)):
+    """positions [24,3], indices [12,3] of an axis-aligned cube with outward
+    winding (cross(p1 - p0, p2 - p0) points out of the box)."""
+    pos, idx = [], []
+    c = np.asarray(center, np.float64)
+    for axis in range(3):
+        for side in (-1.0, 1.0):
+            b = len(pos)
+            for a, bb in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                p = np.zeros(3)
+                p[axis] = side
+                p[(axis + 1) % 3] = a
+                p[(axis + 2) % 3] = bb
+                pos.append(c + half * p)
+            for t in ((b, b + 1, b + 2), (b, b + 2, b + 3)):
+                p0, p1, p2 = (np.asarray(pos[i]) - c for i in t)
+                n = np.cross(p1 - p0, p2 - p0)
+                idx.append(t if np.dot(n, p0 + p1 + p2) > 0 else (t[0], t[2], t[1]))
+    return np.asarray(pos, np.float32), np.asarray(idx, np.int32)
+
+
+def quad_mesh(y: float = 0.0, half: float = 50.0, up: bool = True):
+    """A large square at height y whose geometric normal points +y (up=True) or -y."""
+    pos = np.array([[-half, y, -half], [half, y, -half], [half, y, half], [-half, y, half]], np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2]], np.int32) if up else np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return pos, idx
+
+
+def patch(sampler=L.VR_SURF_PASS, emission=L.VR_EMIT_NONE, emission_c=(0.0, 0.0, 0.0), inside=-1,
+          refraction_index=1.5) -> dict:
+    return dict(sampler=sampler, emission=emission, emission_c=emission_c, inside=inside,
+                refraction_index=refraction_index)
+
+
+class SceneBuilt:
+    """A vr_scene over host (device="cpu", for the oracle) or device buffers.
+
+    meshes: list of (positions [V,3], indices [T,3]) -- one patch each; patches:
+    one dict (patch()) per mesh; media: helpers.Scene media (DTMediumPathIntegrator
+    fields: sigma grid, albedo, phase, medium_majorant, medium_filter)."""
+
+    def __init__(self, meshes, patches, media=(), env_c=(1.0, 1.0, 1.0), env_image=None, normals=None, uvs=None,
+                 device="cpu", grad_sigma=False, grad_albedo=False):
+        self.device = device
+        self.keep = []
+        pos, idx, tp = [], [], []
+        base = 0
+        for k, (p, ix) in enumerate(meshes):
+            pos.append(np.asarray(p, np.float32))
+            idx.append(np.asarray(ix, np.int64) + base)
+            tp.append(np.full(len(ix), k, np.int32))
+            base += len(p)
+        positions = np.ascontiguousarray(np.concatenate(pos) if pos else np.zeros((0, 3), np.float32))
+        indices = np.ascontiguousarray((np.concatenate(idx) if idx else np.zeros((0, 3))).astype(np.int32))
+        tri_patch = np.ascontiguousarray(np.concatenate(tp) if tp else np.zeros(0, np.int32))
+        T = indices.shape[0]
+        nodes = np.zeros((max(1, 2 * T), 8), np.float32)
+        tri_ids = np.zeros(max(1, T), np.int32)
+        tri_verts = np.zeros((max(1, T), 9), np.float32)
+        nn = L.ctypes.c_int32(0)
+        if T:
+            L.check(L.load().vr_scene_build_bvh(positions.ctypes.data, positions.shape[0], indices.ctypes.data, T,
+                                                 nodes.ctypes.data, nodes.shape[0], tri_ids.ctypes.data,
+                                                 tri_verts.ctypes.data, L.ctypes.byref(nn)), "build_bvh")
+        self.num_nodes = int(nn.value)
+        self.nodes_np, self.tri_ids_np, self.tri_verts_np = nodes, tri_ids, tri_verts
+        table = (L.VrPatch * max(1, len(patches)))()
+        for i, p in enumerate(patches):
+            table[i].sampler = p["sampler"]
+            table[i].refraction_index = p["refraction_index"]
+            table[i].emission = p["emission"]
+            table[i].emission_c = L.f3(p["emission_c"])
+            table[i].inside_medium = p["inside"]
+            table[i].outside_medium = -1
+        ptab = np.frombuffer(bytes(table), dtype=np.int32).copy()
+        self.media = [Built(m, device, grad_sigma=grad_sigma, grad_albedo=grad_albedo) for m in media]
+        sc = L.VrScene()
+        sc.nodes = self._addr(self._buf(nodes[:max(1, self.num_nodes)], np.float32))
+        sc.tri_verts = self._addr(self._buf(tri_verts, np.float32))
+        sc.tri_ids = self._addr(self._buf(tri_ids, np.int32))
+        sc.num_nodes = self.num_nodes
+        sc.num_tris = T
+        sc.positions = self._addr(self._buf(positions, np.float32)) if T else None
+        sc.normals = None if normals is None else self._addr(self._buf(normals, np.float32))
+        sc.uvs = None if uvs is None else self._addr(self._buf(uvs, np.float32))
+        sc.indices = self._addr(self._buf(indices, np.int32)) if T else None
+        sc.tri_patch = self._addr(self._buf(tri_patch, np.int32)) if T else None
+        sc.patches = self._addr(self._buf(ptab, np.int32))
+        sc.num_patches = len(patches)
+        sc.num_vertices = positions.shape[0]
+        self.media_array = (L.VrMedium * max(1, len(self.media)))(*[b.medium for b in self.media])
+        self.keep.append(self.media_array)
+        sc.media = L.ctypes.cast(self.media_array, L.ctypes.c_void_p).value if self.media else None
+        sc.num_media = len(self.media)
+        sc.env_c = L.f3(env_c)
+        if env_image is not None:
+            img = self._buf(env_image, np.float32)
+            sc.env_kind = L.VR_ENV_XR
+            v = L.VrTensor()
+            v.data = self._addr(img)
+            v.ndim = 3
+            for i in range(3):
+                v.shape[i] = img.shape[i]
+                v.stride[i] = int(np.prod(img.shape[i + 1:]))
+            v.shape[3] = v.stride[3] = 1
+            sc.env_image = v
+            sc.env_image_bmin[0] = sc.env_image_bmin[1] = -1.0
+            sc.env_image_bmax[0] = sc.env_image_bmax[1] = 1.0
+        else:
+            sc.env_kind = L.VR_ENV_CONST
+        self.scene = sc
+
+    def _buf(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if self.device == "cpu":
+            self.keep.append(a)
+            return a
+        t = torch.from_numpy(a).to(self.device)
+        self.keep.append(t)
+        return t
+
+    def _addr(self, b) -> int:
+        return O.nptr(b) if self.device == "cpu" else b.data_ptr()
+
+    def grads(self):
+        return [b.grads() for b in self.media]
+
+
+def scene_medium(sigma: np.ndarray, majorant: float, albedo_c=(1.0, 1.0, 1.0), phase_g=0.0, medium_filter=3,
+                 gathering_c=(0.0, 0.0, 0.0), albedo=None, bmin=(-0.5, -0.5, -0.5), bmax=(0.5, 0.5, 0.5)) -> Scene:
+    """A DTMediumPathIntegrator medium for SceneBuilt."""
+    return Scene(sigma=sigma, bmin=bmin, bmax=bmax, box_min=(-1e30,) * 3, box_max=(1e30,) * 3, albedo_c=albedo_c,
+                 albedo=albedo, phase_kind=L.VR_PHASE_HG, phase_g=phase_g, medium_majorant=majorant,
+                 medium_filter=medium_filter, gathering_c=gathering_c)
+
+
+def scene_raycast(sb: SceneBuilt, rays: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(rays, np.float32)
+    if sb.device == "cpu":
+        out = np.zeros((x.shape[0], 16), np.float32)
+        O.check(O.load().oracle_scene_raycast(sb.scene, O.nptr(x), x.shape[0], O.nptr(out)), "oracle_scene_raycast")
+        return out
+    xt = torch.from_numpy(x).cuda()
+    out = torch.zeros(x.shape[0], 16, device="cuda")
+    L.check(L.load().vr_scene_raycast(sb.scene, xt.data_ptr(), x.shape[0], out.data_ptr(), _stream()), "vr_scene_raycast")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def scene_map_fw(sb: SceneBuilt, rays: np.ndarray, seeds) -> np.ndarray:
+    x = np.ascontiguousarray(rays, np.float32)
+    lau = L.launch(seeds, bw_mode=L.VR_BW_CONSISTENT, seed_mode=L.VR_SEED_MAP)
+    if sb.device == "cpu":
+        out = np.zeros((x.shape[0], 3), np.float32)
+        O.check(O.load().oracle_scene_map_fw(sb.scene, O.nptr(x), x.shape[0], O.nptr(out), lau), "oracle_scene_map_fw")
+        return out
+    xt = torch.from_numpy(x).cuda()
+    out = torch.zeros(x.shape[0], 3, device="cuda")
+    L.check(L.load().vr_scene_map_fw(sb.scene, xt.data_ptr(), x.shape[0], out.data_ptr(), lau, _stream()), "vr_scene_map_fw")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def scene_capture_fw(sb: SceneBuilt, cam_args: dict, seeds, spp: int) -> np.ndarray:
+    cam = Camera(device=sb.device, **cam_args)
+    lau = L.launch(seeds, samples=spp, bw_mode=L.VR_BW_CONSISTENT)
+    if sb.device == "cpu":
+        out = np.zeros((cam.threads, 3), np.float32)
+        O.check(O.load().oracle_scene_capture_fw(sb.scene, cam.c, lau, O.nptr(out)), "oracle_scene_capture_fw")
+        return out
+    out = torch.zeros(cam.threads, 3, device="cuda")
+    L.check(L.load().vr_scene_capture_fw(sb.scene, cam.c, lau, out.data_ptr(), _stream()), "vr_scene_capture_fw")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def scene_capture_bw(sb: SceneBuilt, cam_args: dict, seeds, spp: int, out_grad: np.ndarray,
+                     seed_mode=L.VR_SEED_CAPTURE_FW):
+    cam = Camera(device=sb.device, **cam_args)
+    lau = L.launch(seeds, samples=spp, bw_mode=L.VR_BW_CONSISTENT, seed_mode=seed_mode)
+    og = np.ascontiguousarray(out_grad, np.float32)
+    if sb.device == "cpu":
+        O.check(O.load().oracle_scene_capture_bw(sb.scene, cam.c, lau, O.nptr(og)), "oracle_scene_capture_bw")
+    else:
+        ogt = torch.from_numpy(og).cuda()
+        L.check(L.load().vr_scene_capture_bw(sb.scene, cam.c, lau, ogt.data_ptr(), _stream()), "vr_scene_capture_bw")
+        torch.cuda.synchronize()
+    return sb.grads()

@@ -44,6 +44,8 @@ def test_struct_layouts_match_the_header():
     assert L.VrLaunch.start_index.offset == 32
     assert ctypes.sizeof(L.VrLaunch) == 56
     assert L.VrCamera.sensors.offset == 32
+    assert ctypes.sizeof(L.VrBvhNode) == 32 and ctypes.sizeof(L.VrPatch) == 32
+    assert ctypes.sizeof(L.VrScene) == 224 and L.VrScene.media.offset == 88 and L.VrScene.env_image.offset == 136
 
 
 def test_argument_errors_are_reported_without_a_device():
