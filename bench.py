@@ -452,6 +452,127 @@ def grid_tools_leg(dev, res=512, reps=10):
     return out
 
 
+def _icosphere(subdiv: int, radius: float, center):
+    """Triangle sphere by repeated 4:1 subdivision of an icosahedron."""
+    import numpy as np
+    t = (1.0 + 5 ** 0.5) / 2
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+         [8, 6, 7], [9, 8, 1]]
+    v = [list(np.asarray(p) / np.linalg.norm(p)) for p in v]
+    for _ in range(subdiv):
+        mid = {}
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = (np.asarray(v[a]) + np.asarray(v[b])) * 0.5
+                v.append(list(p / np.linalg.norm(p)))
+                mid[k] = len(v) - 1
+            return mid[k]
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    pos = np.asarray(v, np.float32) * radius + np.asarray(center, np.float32)
+    return pos, np.asarray(f, np.int32)
+
+
+def scene_leg(dev, res=256, hw=512, spp=16, reps=3):
+    """Scene-level path tracing (PathtracedScene): the 256^3 fBm cloud inside a
+    null box over a Lambert floor with an emitting panel, a mirror wall and a
+    glass (Fresnel) icosphere of 20480 triangles, constant sky; forward capture
+    and replay backward into the density; plus BVH ray casts against the
+    sphere and a 1M-triangle soup."""
+    import numpy as np
+    import torch
+    import rendervous_amd as rv
+
+    sigma = torch.nn.Parameter(rv.functions.fbm_value_noise(res, seed=7, scale=40.0, sphere_radius=0.5, device=dev),
+                               requires_grad=True)
+    bmin, bmax = rv.normalized_box(sigma)
+    medium = rv.HeterogeneousMediumMaterial(rv.Grid3D(sigma, bmin, bmax), scattering_albedo=rv.const[0.9, 0.9, 0.9],
+                                            phase_sampler=rv.HGPhaseSampler(0.6), majorant=float(sigma.detach().max()))
+    half = torch.diag(torch.tensor([0.5, 0.5, 0.5]))
+    cloud = rv.TransformedGeometry(rv.MeshGeometry.box(), torch.cat([half, torch.zeros(1, 3)]))
+    floor = rv.MeshGeometry(torch.tensor([[-4.0, -0.6, -4.0], [4.0, -0.6, -4.0], [4.0, -0.6, 4.0], [-4.0, -0.6, 4.0]]),
+                            indices=torch.tensor([[0, 2, 1], [0, 3, 2]], dtype=torch.int32), compute_normals=False)
+    mirror = rv.MeshGeometry(torch.tensor([[1.5, -0.6, -2.0], [1.5, 1.5, -2.0], [1.5, 1.5, 2.0], [1.5, -0.6, 2.0]]),
+                             indices=torch.tensor([[0, 1, 2], [0, 2, 3]], dtype=torch.int32), compute_normals=False)
+    sp, sf = _icosphere(5, 0.35, (-0.9, -0.25, 0.5))
+    glass = rv.MeshGeometry(torch.from_numpy(sp), indices=torch.from_numpy(sf))
+    geoms = {"cloud": cloud, "floor": floor, "mirror": mirror, "glass": glass}
+    mats = {"cloud": rv.SurfaceMaterial(rv.NullSurfaceScatteringSampler()),
+            "floor": rv.SurfaceMaterial(rv.LambertSurfaceScatteringSampler(),
+                                        rv.IsotropicSurfaceEmission(rv.const[0.2, 0.2, 0.2])),
+            "mirror": rv.SurfaceMaterial(rv.MirrorSurfaceScatteringSampler()),
+            "glass": rv.SurfaceMaterial(rv.FresnelSurfaceScatteringSampler(1.5))}
+    scene = rv.Scene.from_graph(geoms, rv.UniformEnvironmentSampler(rv.const[0.5, 0.5, 0.5]), materials=mats,
+                                inside_media={"cloud": medium})
+    field = scene.pathtrace()
+    camera = rv.PerspectiveCameraSensor(hw, hw, rv.functions.look_at_pose((0.0, 0.4, 2.5))[None].to(dev),
+                                        jittered=True)
+    stream = torch.cuda.current_stream(dev)
+    pixels = hw * hw
+    dbg = torch.zeros(pixels, dtype=torch.int32, device=dev)
+    with torch.no_grad():
+        camera.capture(field, fw_samples=spp, debug_out=dbg)
+    torch.cuda.synchronize()
+    lookups = float(dbg.to(torch.float64).sum().item())
+    fw_ms = bw_ms = 0.0
+    scatters = 0.0
+    for k in range(reps + 1):
+        e = _events(3)
+        e[0].record(stream)
+        img = camera.capture(field, fw_samples=spp, bw_samples=spp, debug_out=dbg)
+        e[1].record(stream)
+        img.sum().backward()
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        if k:
+            fw_ms += e[0].elapsed_time(e[1]) / reps
+            bw_ms += e[1].elapsed_time(e[2]) / reps
+            scatters += float(dbg.to(torch.float64).sum().item()) / reps
+        sigma.grad = None
+    samples = pixels * spp
+    # BVH ray casts: 1M rays against the sphere alone and against a 1M-triangle soup
+    from rendervous_amd import _lib as L
+    rng = np.random.default_rng(0)
+    casts = {}
+    soup_c = rng.uniform(-1, 1, size=(1 << 20, 1, 3))
+    soup = (soup_c + rng.normal(scale=0.004, size=(1 << 20, 3, 3))).reshape(-1, 3).astype(np.float32)
+    for name, geom in (("sphere_20480", glass),
+                       ("soup_1M", rv.MeshGeometry(torch.from_numpy(soup), compute_normals=False))):
+        n = 1 << 20
+        o = rng.normal(size=(n, 3))
+        o = o / np.linalg.norm(o, axis=1, keepdims=True) * 2.5
+        d = rng.uniform(-0.7, 0.7, size=(n, 3)) - o
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rays = torch.from_numpy(np.concatenate([o, d], 1).astype(np.float32)).to(dev)
+        geom(rays)  # builds the BVH and uploads it
+        torch.cuda.synchronize()
+        e = _events(2)
+        e[0].record(stream)
+        for _ in range(5):
+            hits = geom(rays)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        ms = e[0].elapsed_time(e[1]) / 5
+        casts[name] = {"rays": n, "ms": ms, "rays_per_s": n / (ms * 1e-3),
+                       "hit_fraction": float((hits[:, 15].view(torch.int32) >= 0).float().mean())}
+    return {
+        "workload": f"PathtracedScene: {res}^3 fBm cloud in a null box, Lambert floor with emission, mirror, glass "
+                    f"icosphere (20480 triangles), {hw}x{hw}, {spp} spp; forward + replay backward into the density",
+        "fw_ms": fw_ms, "bw_ms": bw_ms, "fw_samples_per_s": samples / (fw_ms * 1e-3),
+        "bw_samples_per_s": samples / (bw_ms * 1e-3), "fw_lookups": lookups,
+        "grad_voxel_updates_per_s": 8.0 * scatters / (bw_ms * 1e-3), "trilinear_scatters": scatters,
+        "mean": float(img.mean()), "bvh_casts": casts,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -464,7 +585,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the gradient and DDA legs")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--leg", choices=["gradient", "nee_gradient", "nrc", "dda", "grid_tools"],
+    ap.add_argument("--leg", choices=["gradient", "nee_gradient", "nrc", "dda", "grid_tools", "scene"],
                     help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM byte counters)")
@@ -482,7 +603,7 @@ def main():
     torch.cuda.set_device(dev)
     if args.leg is not None:
         legs = {"gradient": gradient_leg, "nee_gradient": nee_gradient_leg, "nrc": nrc_leg, "dda": dda_leg,
-                "grid_tools": grid_tools_leg}
+                "grid_tools": grid_tools_leg, "scene": scene_leg}
         print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
         return
     if args.calibrate_copy:
@@ -563,6 +684,8 @@ def main():
         result["secondary"]["dda"] = dda_leg(dev)
         torch.cuda.empty_cache()
         result["secondary"]["grid_tools"] = grid_tools_leg(dev)
+        torch.cuda.empty_cache()
+        result["secondary"]["scene"] = scene_leg(dev)
         log(f"secondary: {json.dumps(result['secondary'])}")
     if not args.no_cpu_baseline:
         m = build_scene(args.res, args.width, args.height, dev)[1]
