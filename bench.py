@@ -203,6 +203,166 @@ def gradient_leg(dev, res=256, views=8, hw=256, spp=16, steps=3):
     return r
 
 
+def _directional_check(camera, field_of, param, spp, n_dirs=4, h=0.25, batches=4, seed=0):
+    """Adjoint gradient of sum(w * image) (replayed seeds, consistent estimator)
+    against central differences of the same functional along smooth positive
+    directions v = 0.5 + |a random 8^3 field resampled to the grid|, both sides
+    of a difference rendered with the same dispatch seeds.  Each of `batches`
+    independent seed batches gives one adjoint and one difference per
+    direction; returns per direction the means, their standard errors and the
+    discrepancy in standard errors."""
+    import torch
+    import rendervous_amd as rv
+
+    dev = param.device
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    w = (torch.rand(*camera.index_shape, 3, generator=gen) + 0.5).to(dev)
+    dirs = []
+    for _ in range(n_dirs):
+        coarse = torch.randn(8, 8, 8, param.shape[-1], generator=gen).abs().to(dev)
+        dirs.append((0.5 + rv.functions.resample_grid(coarse, tuple(param.shape[:-1]))).double())
+    ad = [[] for _ in dirs]
+    fd = [[] for _ in dirs]
+    for b in range(batches):
+        rv.seed(1000 + b)
+        img = camera.capture(field_of(param), fw_samples=spp, bw_samples=spp)
+        (img * w).sum().backward()
+        g = param.grad.detach().double()
+        param.grad = None
+        for k, v in enumerate(dirs):
+            ad[k].append(float((g * v).sum()))
+            vals = []
+            for sgn in (1.0, -1.0):
+                with torch.no_grad():
+                    p2 = (param.detach() + sgn * h * v.float()).contiguous()
+                    rv.seed(1000 + b)
+                    vals.append(float((camera.capture(field_of(p2), fw_samples=spp) * w).double().sum()))
+            fd[k].append((vals[0] - vals[1]) / (2 * h))
+    out = []
+    for k in range(n_dirs):
+        a, f = torch.tensor(ad[k], dtype=torch.float64), torch.tensor(fd[k], dtype=torch.float64)
+        sa, sf = float(a.std() / batches ** 0.5), float(f.std() / batches ** 0.5)
+        ma, mf = float(a.mean()), float(f.mean())
+        out.append({"fd": mf, "fd_se": sf, "adjoint": ma, "adjoint_se": sa,
+                    "rel_err": abs(ma - mf) / max(abs(mf), 1e-30),
+                    "z": abs(ma - mf) / max((sa * sa + sf * sf) ** 0.5, 1e-30)})
+    return out
+
+
+def convergence_leg(dev, res=256, views=8, hw=256, spp=16, steps=50):
+    """The 256^3 single-scatter inverse render run to completion: 50 Adam steps
+    (lr 1e-2) from the constant density 5 towards 8 views of a seeded fBm target;
+    the loss at every step and the wall time of the loop; then the adjoint
+    gradient against central differences (h = 0.25, same seeds) along 4 smooth
+    directions at the full grid size, with a fixed majorant so that both sides
+    of each difference draw the same random numbers."""
+    import torch
+    import rendervous_amd as rv
+    from rendervous_amd import inverse
+
+    target_sigma = rv.functions.fbm_value_noise(res, seed=11, scale=40.0, device=dev)
+    poses = rv.functions.orbit_poses(views, radius=2.5).to(dev)
+    camera = rv.PerspectiveCameraSensor(hw, hw, poses, jittered=True)
+    rv.seed(3)
+    with torch.no_grad():
+        target = camera.capture(inverse.single_scatter_scene(target_sigma).radiance, fw_samples=spp)
+    del target_sigma
+    sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
+    # gradient check first, at the initial density (fixed majorant 7 >= 5 + 0.25 max v)
+    checks = _directional_check(camera, lambda s: inverse.single_scatter_scene(s, majorant=7.0).radiance, sigma, 64)
+    field = inverse.single_scatter_scene(sigma).radiance
+    with torch.no_grad():  # low-noise loss: 256-spp renders of target and estimate with fixed seeds
+        rv.seed(99)
+        target_hi = camera.capture(inverse.single_scatter_scene(
+            rv.functions.fbm_value_noise(res, seed=11, scale=40.0, device=dev)).radiance, fw_samples=256)
+
+        def eval_loss():
+            with torch.no_grad():
+                rv.seed(98)
+                return float(((camera.capture(field, fw_samples=256) - target_hi) ** 2).mean())
+        eval_first = eval_loss()
+    opt = inverse.FusedAdam(sigma, lr=1e-2)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        losses.append(inverse.optimisation_step(camera, field, sigma, target, spp, opt))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    losses = [float(x) for x in losses]
+    return {"workload": f"{res}^3 density from 5.0, {views} views {hw}x{hw}, {spp} spp, single scatter (RT shadow rays), "
+                        f"L2 loss, Adam lr 1e-2, {steps} steps; gradient check at 64 spp, 4 seed batches",
+            "loss_curve": losses, "loss_first": losses[0], "loss_last": losses[-1],
+            "loss_256spp_first": eval_first, "loss_256spp_last": eval_loss(), "wall_s": wall,
+            "ms_per_step": wall * 1e3 / steps, "directional_fd": checks,
+            "fd_rel_err_max": max(c["rel_err"] for c in checks), "fd_z_max": max(c["z"] for c in checks)}
+
+
+def nee_convergence_leg(dev, res=512, ares=128, views=4, hw=512, spp=32, steps=100):
+    """One GPU's share (4 of 32 views) of the 512^3 + 128^3-albedo multiple-
+    scattering inverse render run for 100 Adam steps; loss curve and the adjoint
+    (density) against central differences along 4 smooth directions."""
+    import torch
+    import rendervous_amd as rv
+    from rendervous_amd import inverse
+
+    t_sigma = rv.functions.fbm_value_noise(res, seed=21, scale=60.0, sphere_radius=0.5, device=dev)
+    t_albedo = (0.55 + 0.4 * rv.functions.fbm_value_noise(ares, seed=22, scale=1.0, device=dev)).expand(
+        ares, ares, ares, 3).contiguous()
+    poses = rv.functions.orbit_poses(32, radius=2.5)[:views].to(dev)
+    camera = rv.PerspectiveCameraSensor(hw, hw, poses, jittered=True)
+    rv.seed(5)
+    with torch.no_grad():
+        target = camera.capture(_nee_scene(t_sigma, t_albedo, 8), fw_samples=spp)
+    del t_sigma, t_albedo
+    sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
+    albedo = torch.nn.Parameter(torch.full((ares, ares, ares, 3), 0.7, device=dev), requires_grad=True)
+    albedo.requires_grad_(False)
+    checks = _directional_check(camera, lambda s: _nee_scene_fixed(s, albedo, 8, 7.0), sigma, 64)
+    albedo.requires_grad_(True)
+    field = _nee_scene(sigma, albedo, 8)
+    opts = [(sigma, inverse.FusedAdam(sigma, lr=1e-2), (0.0, math.inf)),
+            (albedo, inverse.FusedAdam(albedo, lr=1e-2), (0.0, 1.0))]
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        img = camera.capture(field, fw_samples=spp, bw_samples=spp)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        for p, opt, bounds in opts:
+            opt.step(p.grad, zero_grad=False, bounds=bounds)
+            p.grad = None
+        losses.append(loss.detach())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    losses = [float(x) for x in losses]
+    return {"workload": f"{res}^3 density + {ares}^3 RGB albedo, {views} of 32 views {hw}x{hw}, {spp} spp, NEE + DT, "
+                        f"max 8 bounces, replay backward, Adam lr 1e-2, {steps} steps; gradient check at 64 spp, "
+                        f"4 seed batches",
+            "loss_curve": losses[::5] + [losses[-1]], "loss_first": losses[0], "loss_last": losses[-1], "wall_s": wall,
+            "ms_per_step": wall * 1e3 / steps, "directional_fd": checks,
+            "fd_rel_err_max": max(c["rel_err"] for c in checks), "fd_z_max": max(c["z"] for c in checks)}
+
+
+def _nee_scene_fixed(sigma, albedo, max_bounces, majorant):
+    """_nee_scene with a fixed constant majorant (finite-difference checks)."""
+    import torch
+    import rendervous_amd as rv
+
+    ds = rv.objects.DependencySet()
+    ds.add_parameters(sigma_tensor=sigma, scattering_albedo_tensor=albedo,
+                      environment_tensor=torch.tensor([0.1, 0.1, 0.1]), phase_g_tensor=torch.tensor([0.6]),
+                      majorant_tensor=torch.tensor([float(majorant), 1000000.0]))
+    ds.requires(rv.objects.medium_environment_sampler_sun, direction=(0.4, 0.8, 0.45), radiance=(3.0, 3.0, 3.0),
+                probability=0.5)
+    ds.requires(rv.objects.medium_phase_HG, mixture=False)
+    ds.requires(rv.objects.medium_phase_sampler_HG, mixture=False)
+    ds.requires(rv.objects.medium_transmittance_DT)
+    ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DT, max_bounces=max_bounces)
+    return ds.radiance
+
+
 def _nee_scene(sigma, albedo, max_bounces):
     import torch
     import rendervous_amd as rv
@@ -585,7 +745,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the gradient and DDA legs")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--leg", choices=["gradient", "nee_gradient", "nrc", "dda", "grid_tools", "scene"],
+    ap.add_argument("--leg", choices=["gradient", "nee_gradient", "nrc", "dda", "grid_tools", "scene", "convergence",
+                                      "nee_convergence"],
                     help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM byte counters)")
@@ -603,7 +764,8 @@ def main():
     torch.cuda.set_device(dev)
     if args.leg is not None:
         legs = {"gradient": gradient_leg, "nee_gradient": nee_gradient_leg, "nrc": nrc_leg, "dda": dda_leg,
-                "grid_tools": grid_tools_leg, "scene": scene_leg}
+                "grid_tools": grid_tools_leg, "scene": scene_leg, "convergence": convergence_leg,
+                "nee_convergence": nee_convergence_leg}
         print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
         return
     if args.calibrate_copy:
@@ -686,6 +848,8 @@ def main():
         result["secondary"]["grid_tools"] = grid_tools_leg(dev)
         torch.cuda.empty_cache()
         result["secondary"]["scene"] = scene_leg(dev)
+        torch.cuda.empty_cache()
+        result["secondary"]["convergence"] = convergence_leg(dev)
         log(f"secondary: {json.dumps(result['secondary'])}")
     if not args.no_cpu_baseline:
         m = build_scene(args.res, args.width, args.height, dev)[1]

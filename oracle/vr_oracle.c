@@ -1593,6 +1593,8 @@ static void medium_dt_fw(ctx_t* c, v3 x, v3 w, double d, double* out) {
     out[9] = A.x; out[10] = A.y; out[11] = A.z;
 }
 
+static void medium_dt_replay(ctx_t* c, v3 x, v3 w, double d, v3 W, v3 A, v3 dL_dW, v3 dL_dA);
+
 static void medium_dt_bw(ctx_t* c, v3 x, v3 w, double d, v3 dL_dW, v3 dL_dA) {
     const vr_medium* M = c->M;
     v3 W = V(1, 1, 1), A = V(0, 0, 0);
@@ -1615,7 +1617,14 @@ static void medium_dt_bw(ctx_t* c, v3 x, v3 w, double d, v3 dL_dW, v3 dL_dA) {
         }
     }
     c->rng = before;
-    x = xi; w = wi; d = di;
+    medium_dt_replay(c, xi, wi, di, W, A, dL_dW, dL_dA);
+}
+
+/* The replay half of medium_dt_bw given the primal pass's (W, A). */
+static void medium_dt_replay(ctx_t* c, v3 x, v3 w, double d, v3 W, v3 A, v3 dL_dW, v3 dL_dA) {
+    const vr_medium* M = c->M;
+    double maj = dmax(0.00001, M->medium_majorant);
+    int f = M->medium_filter;
     /* `#if MEDIUM_FILTER & 1 == 1` and `& 2 == 2` both parse as `& 1`
      * (sc_medium_path_integrator_DT_bw.h:85,95,109); VR_BW_CONSISTENT uses the
      * intended bit tests. */
@@ -2494,7 +2503,7 @@ static void o_scene_path_bw(const vr_scene* S, ctx_t* c, const sacc_t* acc, v3 x
             double gv[3] = {g.x / vW.x, g.y / vW.y, g.z / vW.z};
             for (int k = 0; k < 3; k++) if (!isfinite(gv[k])) gv[k] = 0;  /* vW == 0: dropped */
             c->rng = before;
-            medium_dt_bw(c, x, w, d, V(gv[0], gv[1], gv[2]), dL_dvA);
+            medium_dt_replay(c, x, w, d, vW, vA, V(gv[0], gv[1], gv[2]), dL_dvA);
             W = vmul(W, vW);
             if (!(wo.x == w.x && wo.y == w.y && wo.z == w.z)) { x = xo; w = wo; continue; }
         }

@@ -576,6 +576,9 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         S.chunks = (spp + S.chunk - 1) / S.chunk;
         S.tiled = !C.sensors && b0 % hw == 0 && n % hw == 0;
         S.ksteps = std::max(1, env_int("VR_NEE_STEPS", 3));
+        // more steps while at least VR_NEE_TRACK_MIN lanes still track (default: never)
+        S.kmax = std::min(256, std::max(S.ksteps, env_int("VR_NEE_STEPS_MAX", S.ksteps)));
+        S.track_min = std::max(0, std::min(65, env_int("VR_NEE_TRACK_MIN", 65)));
         unsigned prep = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
         hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, b0, n, dbg, counter);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;

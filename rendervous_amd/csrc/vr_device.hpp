@@ -1618,7 +1618,16 @@ struct Tracer {
             }
         }
         rng = before;
-        x = xi; w = wi; d = di;
+        medium_dt_replay(xi, wi, di, W, A, dL_dW, dL_dA);
+    }
+
+    // The replay half of medium_dt_bw given the primal pass's (W, A), which
+    // equal the forward's outputs: a caller that has just run medium_dt on the
+    // same state (the scene backward, sc_path_tracing.h:281-296) skips the
+    // primal pass.
+    VR_DEV void medium_dt_replay(f3 x, f3 w, float d, f3 W, f3 A, f3 dL_dW, f3 dL_dA) {
+        const float maj = fmaxf(0.00001f, M.medium_majorant);
+        const int f = M.medium_filter;
         const bool scat_blk = (f & 1) != 0;
         const bool acc_blk = M.bw_mode == VR_BW_REFERENCE ? (f & 1) != 0 : (f & 2) != 0;
         for (;;) {

@@ -250,6 +250,8 @@ struct NeeSchedule {
     int chunks;     // ceil(spp / chunk)
     int tiled;      // whole images without a sensor list: visit each pose in 8x8 tiles
     int ksteps;     // tracking steps per iteration of the event loop
+    int kmax;       // at most this many steps when ...
+    int track_min;  // ... at least this many lanes of the wave are still tracking after ksteps
 };
 
 // Sensors and items of queue q.
@@ -409,7 +411,10 @@ k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsig
         if (have && lane.event == kEvStart) lane.start_ray(M);
         for (int k = 0;;) {
             if (have && lane.event == kEvNone) lane.step(M);
-            if (++k >= S.ksteps || __ballot(have && lane.event == kEvNone) == 0ull) break;
+            ++k;
+            const unsigned long long tracking = __ballot(have && lane.event == kEvNone);
+            if (tracking == 0ull || k >= S.kmax) break;
+            if (k >= S.ksteps && __popcll(tracking) < S.track_min) break;
         }
         if (have && lane.event == kEvCollision) lane.on_collision(M);
         if (have && lane.event == kEvTrackEnd) lane.on_track_end(M);

@@ -272,11 +272,14 @@ VR_DEV void medium_traverse(const SceneView& S, int m, Rng& rng, f3 x, f3 w, flo
     A = F3(out[9], out[10], out[11]);
 }
 
-VR_DEV void medium_traverse_bw(const SceneView& S, int m, Rng& rng, f3 x, f3 w, float d, f3 dL_dW, f3 dL_dA, int& lookups,
-                               int& scatters) {
+// medium_traversal_bw (dyn_medium_traversal.h:20-32) after medium_traverse from
+// the same state: (W, A) are that forward's outputs, which the DT backward's
+// primal pass would recompute bit for bit, so only its replay runs.
+VR_DEV void medium_traverse_bw(const SceneView& S, int m, Rng& rng, f3 x, f3 w, float d, f3 W, f3 A, f3 dL_dW, f3 dL_dA,
+                               int& lookups, int& scatters) {
     Tracer<VR_MAJ_CONST> tr(S.media[m]);
     tr.rng = rng;
-    tr.medium_dt_bw(x, w, d, dL_dW, dL_dA);
+    tr.medium_dt_replay(x, w, d, W, A, dL_dW, dL_dA);
     rng = tr.rng;
     lookups += tr.lookups;
     scatters += tr.scatters;
@@ -393,7 +396,7 @@ VR_DEV void scene_path_bw(const SceneView& S, Rng& rng, f3 x, f3 w, f3 out_A, f3
             for (int c = 0; c < 3; c++)
                 if (!isfinite(gv[c])) gv[c] = 0.0f;
             rng = before;
-            medium_traverse_bw(S, ms.current, rng, x, w, d, F3(gv[0], gv[1], gv[2]), dL_dvA, lookups, scatters);
+            medium_traverse_bw(S, ms.current, rng, x, w, d, vW, vA, F3(gv[0], gv[1], gv[2]), dL_dvA, lookups, scatters);
             W = W * vW;
             if (!same_dir(wo, w)) { x = xo; w = wo; continue; }
         }
