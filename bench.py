@@ -271,6 +271,7 @@ def convergence_leg(dev, res=256, views=8, hw=256, spp=16, steps=50):
     # gradient check first, at the initial density (fixed majorant 7 >= 5 + 0.25 max v)
     checks = _directional_check(camera, lambda s: inverse.single_scatter_scene(s, majorant=7.0).radiance, sigma, 64)
     field = inverse.single_scatter_scene(sigma).radiance
+    init = sigma.detach().clone()
     with torch.no_grad():  # low-noise loss: 256-spp renders of target and estimate with fixed seeds
         rv.seed(99)
         target_hi = camera.capture(inverse.single_scatter_scene(
@@ -281,20 +282,28 @@ def convergence_leg(dev, res=256, views=8, hw=256, spp=16, steps=50):
                 rv.seed(98)
                 return float(((camera.capture(field, fw_samples=256) - target_hi) ** 2).mean())
         eval_first = eval_loss()
-    opt = inverse.FusedAdam(sigma, lr=1e-2)
-    losses = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        losses.append(inverse.optimisation_step(camera, field, sigma, target, spp, opt))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    losses = [float(x) for x in losses]
+    runs = {}
+    for mode in ("replay", "fresh"):  # backward seeds: the forward's (replay) or new ones (the reference's)
+        with torch.no_grad():
+            sigma.copy_(init)
+        opt = inverse.FusedAdam(sigma, lr=1e-2)
+        losses = []
+        rv.seed(4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with rv.backward_mode(seeds=mode):
+            for _ in range(steps):
+                losses.append(inverse.optimisation_step(camera, field, sigma, target, spp, opt))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        losses = [float(x) for x in losses]
+        runs[mode] = {"loss_curve": losses, "loss_256spp_last": eval_loss(), "wall_s": wall,
+                      "ms_per_step": wall * 1e3 / steps}
     return {"workload": f"{res}^3 density from 5.0, {views} views {hw}x{hw}, {spp} spp, single scatter (RT shadow rays), "
-                        f"L2 loss, Adam lr 1e-2, {steps} steps; gradient check at 64 spp, 4 seed batches",
-            "loss_curve": losses, "loss_first": losses[0], "loss_last": losses[-1],
-            "loss_256spp_first": eval_first, "loss_256spp_last": eval_loss(), "wall_s": wall,
-            "ms_per_step": wall * 1e3 / steps, "directional_fd": checks,
+                        f"L2 loss, Adam lr 1e-2, {steps} steps (backward seeds replayed or fresh); gradient check at "
+                        f"64 spp, 4 seed batches",
+            "loss_256spp_first": eval_first, "runs": runs, "ms_per_step": runs["replay"]["ms_per_step"],
+            "directional_fd": checks,
             "fd_rel_err_max": max(c["rel_err"] for c in checks), "fd_z_max": max(c["z"] for c in checks)}
 
 

@@ -143,3 +143,40 @@ def test_scene_api_capture_and_gradient(gpu):
     gref = scene_capture_bw(direct, CAM, seeds, 4, w.reshape(-1, 3).cpu().numpy())[0][0]
     assert sigma.grad is not None
     assert rel_l1(sigma.grad.cpu().numpy(), gref) <= 1e-4
+
+
+def test_raycast_interpolates_normals_and_uvs(gpu):
+    """Shading normals and coordinates interpolated with the hit's barycentrics
+    (sample_surfel) equal the oracle's."""
+    pos, idx = box_mesh(0.8)
+    rng = np.random.default_rng(7)
+    nrm = rng.normal(size=pos.shape).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    uvs = rng.uniform(size=(pos.shape[0], 2)).astype(np.float32)
+    rays = _rays(8192, 8)
+    ref = scene_raycast(SceneBuilt([(pos, idx)], [patch()], normals=nrm, uvs=uvs, device="cpu"), rays)
+    out = scene_raycast(SceneBuilt([(pos, idx)], [patch()], normals=nrm, uvs=uvs, device="cuda"), rays)
+    hit = ref[:, 15].view(np.int32) >= 0
+    assert hit.sum() > 4000 and (out[:, 15].view(np.int32) == ref[:, 15].view(np.int32)).mean() >= 0.999
+    same = hit & (np.abs(out[:, 4:7] - ref[:, 4:7]).max(1) < 1e-4)  # same face
+    assert same[hit].mean() >= 0.999
+    np.testing.assert_allclose(out[same, 1:4], ref[same, 1:4], atol=2e-4)
+    np.testing.assert_allclose(out[same, 7:9], ref[same, 7:9], atol=2e-4)
+
+
+def test_nested_media_scene_is_seed_matched(gpu):
+    """Two scattering media in nested null boxes (the medium stack pushes and pops)
+    and a Fresnel box inside the inner one."""
+    def build(device):
+        ga = blob_grid(8, seed=1, max_value=2.0)
+        gb = blob_grid(8, seed=2, max_value=5.0)
+        ma = scene_medium(ga, majorant=2.5, albedo_c=(0.9, 0.9, 0.9), phase_g=0.2, bmin=(-1.0,) * 3, bmax=(1.0,) * 3)
+        mb = scene_medium(gb, majorant=5.5, albedo_c=(0.7, 0.8, 0.9), phase_g=-0.3, bmin=(-0.5,) * 3, bmax=(0.5,) * 3)
+        return SceneBuilt([box_mesh(1.0), box_mesh(0.5), box_mesh(0.15, center=(0.1, 0.1, 0.0))],
+                          [patch(L.VR_SURF_NULL, inside=0), patch(L.VR_SURF_NULL, inside=1),
+                           patch(L.VR_SURF_FRESNEL, refraction_index=1.3)],
+                          media=[ma, mb], env_c=(1.0, 0.9, 0.8), device=device)
+    ref = scene_capture_fw(build("cpu"), CAM, SEEDS, 4)
+    out = scene_capture_fw(build("cuda"), CAM, SEEDS, 4)
+    frac = match_fraction(out, ref, rtol=1e-3, atol=1e-4)
+    assert frac >= 0.97, frac

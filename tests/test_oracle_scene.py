@@ -228,3 +228,53 @@ def test_oracle_scene_backward_matches_finite_differences():
             r.append(float((scene_capture_fw(build(s2), cam, (7, 8, 9, 10), 65536).astype(np.float64) * w).sum()))
         fd = (r[0] - r[1]) / (2 * h)
         assert abs(grad[v] - fd) <= 0.08 * abs(fd), (v, grad[v], fd)
+
+
+def _chord(ray, half, center=(0.0, 0.0, 0.0)):
+    o, d = ray[:3].astype(np.float64) - np.asarray(center), ray[3:].astype(np.float64)
+    t0 = np.minimum((-half - o) / d, (half - o) / d).max()
+    t1 = np.maximum((-half - o) / d, (half - o) / d).min()
+    return max(0.0, t1 - max(t0, 0.0))
+
+
+def test_oracle_nested_media_follow_the_medium_stack():
+    """A null box (medium A, constant sigma_A) around a smaller null box (medium B,
+    sigma_B), both absorbing: inside B only B's medium acts, and after leaving B
+    the stack returns to A, so E[A] = exp(-sigma_A (L_A - L_B) - sigma_B L_B)."""
+    sa, sb = 1.3, 3.1
+    med_a = scene_medium(np.full((2, 2, 2, 1), sa, np.float32), majorant=sa + 0.5, medium_filter=0,
+                         bmin=(-1.0,) * 3, bmax=(1.0,) * 3)
+    med_b = scene_medium(np.full((2, 2, 2, 1), sb, np.float32), majorant=sb + 0.5, medium_filter=0,
+                         bmin=(-0.4,) * 3, bmax=(0.4,) * 3)
+    sbuilt = SceneBuilt([box_mesh(1.0), box_mesh(0.4)], [patch(L.VR_SURF_NULL, inside=0), patch(L.VR_SURF_NULL, inside=1)],
+                        media=[med_a, med_b], env_c=(1, 1, 1), device="cpu")
+    for ray in _rays_through_centre(4, 12, spread=0.3):
+        n = 40000
+        A = scene_map_fw(sbuilt, np.tile(ray, (n, 1)), SEEDS)[:, 0]
+        la, lb = _chord(ray, 1.0), _chord(ray, 0.4)
+        exact = math.exp(-sa * (la - lb) - sb * lb)
+        se = A.std() / math.sqrt(n)
+        assert abs(A.mean() - exact) <= 4 * se + 1e-4, (A.mean(), exact, se, la, lb)
+
+
+def test_mesh_geometry_api_obj_group_and_transform(tmp_path):
+    """load_obj normalises into [-1, 1] and fans polygons; GroupGeometry concatenates
+    patches; TransformedGeometry moves positions by p @ R + t (host side, no GPU)."""
+    import rendervous_amd as rv
+    obj = tmp_path / "quad.obj"
+    obj.write_text("v 0 0 0\nv 4 0 0\nv 4 2 0\nv 0 2 0\nf 1 2 3 4\n")
+    m = rv.MeshGeometry.load_obj(str(obj))
+    assert m.indices.tolist() == [[0, 1, 2], [0, 2, 3]]
+    np.testing.assert_allclose(m.positions.detach().numpy(), [[-1, -0.5, 0], [1, -0.5, 0], [1, 0.5, 0], [-1, 0.5, 0]])
+    box = rv.MeshGeometry.box()
+    p = box.positions.detach().numpy()
+    assert np.allclose(np.abs(p), 1.0)
+    T = torch.tensor([[0.0, 1.0, 0.0], [-1.0, 0.0, 0.0], [0.0, 0.0, 2.0], [0.5, -1.0, 3.0]])
+    moved = rv.TransformedGeometry(box, T)
+    pm, nm, _, _ = moved._patches()[0]
+    np.testing.assert_allclose(pm.numpy(), p @ T[:3].numpy() + T[3].numpy(), atol=1e-6)
+    n0 = box.normals.detach().numpy()
+    expect = n0 @ np.linalg.inv(T[:3].numpy()).T
+    np.testing.assert_allclose(nm.numpy(), expect / np.linalg.norm(expect, axis=1, keepdims=True), atol=1e-6)
+    g = rv.GroupGeometry(box, m, moved)
+    assert len(g) == 3 and [len(pp[3]) for pp in g._patches()] == [12, 2, 12]
