@@ -277,10 +277,10 @@ def convergence_leg(dev, res=256, views=8, hw=256, spp=16, steps=50):
         target_hi = camera.capture(inverse.single_scatter_scene(
             rv.functions.fbm_value_noise(res, seed=11, scale=40.0, device=dev)).radiance, fw_samples=256)
 
-        def eval_loss():
+        def eval_loss(f=None):
             with torch.no_grad():
                 rv.seed(98)
-                return float(((camera.capture(field, fw_samples=256) - target_hi) ** 2).mean())
+                return float(((camera.capture(field if f is None else f, fw_samples=256) - target_hi) ** 2).mean())
         eval_first = eval_loss()
     runs = {}
     for mode in ("replay", "fresh"):  # backward seeds: the forward's (replay) or new ones (the reference's)
@@ -299,6 +299,28 @@ def convergence_leg(dev, res=256, views=8, hw=256, spp=16, steps=50):
         losses = [float(x) for x in losses]
         runs[mode] = {"loss_curve": losses, "loss_256spp_last": eval_loss(), "wall_s": wall,
                       "ms_per_step": wall * 1e3 / steps}
+    # the same 50 steps coarse to fine (32^3 -> 64^3 -> 128^3 -> 256^3 with
+    # resample_grid between stages), as the reference's inverse-rendering
+    # examples do: each coarse voxel collects many gradient samples per step
+    stages = [(32, 13), (64, 13), (128, 12), (res, 12)]
+    cur = torch.full((stages[0][0],) * 3 + (1,), 5.0, device=dev)
+    losses = []
+    rv.seed(4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, (r, n) in enumerate(stages):
+        if cur.shape[0] != r:
+            cur = rv.functions.resample_grid(cur, (r, r, r))
+        p = torch.nn.Parameter(cur.contiguous(), requires_grad=True)
+        f = inverse.single_scatter_scene(p).radiance
+        opt = inverse.FusedAdam(p, lr=1e-2)
+        for _ in range(n):
+            losses.append(inverse.optimisation_step(camera, f, p, target, spp, opt))
+        cur = p.detach()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    runs["coarse_to_fine"] = {"loss_curve": [float(x) for x in losses], "loss_256spp_last": eval_loss(f),
+                              "wall_s": wall, "ms_per_step": wall * 1e3 / len(losses)}
     return {"workload": f"{res}^3 density from 5.0, {views} views {hw}x{hw}, {spp} spp, single scatter (RT shadow rays), "
                         f"L2 loss, Adam lr 1e-2, {steps} steps (backward seeds replayed or fresh); gradient check at "
                         f"64 spp, 4 seed batches",
