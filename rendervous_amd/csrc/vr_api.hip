@@ -386,7 +386,32 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_bw(MediumVi
 struct CaptureSplit {
     int spp, chunk, chunks;
     int debug_lookups;  // the backward's debug_out counts lookups instead of scatters
+    int tiled;          // whole images: 1 = consecutive threads take 8x8 pixel tiles (tiled_pixel),
+                        // 2 = the 64 lanes of a wave take pixels hw/64 apart
 };
+
+// Sensor of capture position i: the position itself, or (whole images
+// without a sensor list, P.tiled) the pixel at position i of its image visited
+// in 8x8 tiles, so that a wave's 64 rays come from one tile.  i becomes the
+// sensor's own index (its seeds, output row and counter).
+VR_DEV void capture_sensor(const CameraView& cam, const CaptureSplit& P, int64_t& i, int& pose, int& row, int& col) {
+    if (!P.tiled) {
+        sensor_index(cam, i, pose, row, col);
+        return;
+    }
+    const int64_t hw = (int64_t)cam.width * cam.height;
+    const int64_t p = i / hw;
+    const uint32_t k = (uint32_t)(i - p * hw);
+    if (P.tiled == 1) {
+        tiled_pixel(k, (uint32_t)cam.width, (uint32_t)cam.height, row, col);
+    } else {
+        const uint32_t q = (k & 63u) * (uint32_t)(hw >> 6) + (k >> 6);
+        row = (int)(q / (uint32_t)cam.width);
+        col = (int)(q - (uint32_t)row * (uint32_t)cam.width);
+    }
+    pose = (int)p;
+    i = p * hw + (int64_t)row * cam.width + col;
+}
 
 
 template <int KIND, int MAJ>
@@ -395,11 +420,11 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_fw(Medi
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt * P.chunks) return;
     const int64_t c = j / cnt;
-    const int64_t i = start + (j - c * cnt);
+    int64_t i = start + (j - c * cnt);
     const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
     constexpr int OD = Dims<KIND>::out;
     int pose, row, col;
-    sensor_index(cam, i, pose, row, col);
+    capture_sensor(cam, P, i, pose, row, col);
     // a sensor's row and column are used as given (the reference does not check
     // them); a pose index outside the pose buffer yields 0 instead of a fault
     if (pose < 0 || pose >= cam.num_poses) {
@@ -441,11 +466,11 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt * P.chunks) return;
     const int64_t c = j / cnt;
-    const int64_t i = start + (j - c * cnt);
+    int64_t i = start + (j - c * cnt);
     const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
     constexpr int OD = Dims<KIND>::out;
     int pose, row, col;
-    sensor_index(cam, i, pose, row, col);
+    capture_sensor(cam, P, i, pose, row, col);
     if (pose < 0 || pose >= cam.num_poses) return;
     float o[OD];
     bool any = false;
@@ -632,7 +657,22 @@ CaptureSplit capture_split(int spp, bool one_chunk) {
     P.chunk = one_chunk ? spp : std::max(1, std::min(spp, chunk));
     P.chunks = (spp + P.chunk - 1) / P.chunk;
     P.debug_lookups = 0;
+    P.tiled = 0;
     return P;
+}
+
+// Pixel visiting order of whole images without a sensor list: 0 row-major, 1
+// 8x8 tiles, 2 waves spread over the image (environment `var` overrides
+// `fallback`).  Measured on the bench workloads: row-major is fastest for the
+// medium captures (tiles: NEE backward 152 -> 214 ms, spread: -> 180 ms);
+// spread is fastest for scene captures (forward 28.4 -> 25 ms, backward
+// 124 -> 107 ms).
+void capture_tiling(CaptureSplit& P, const CameraView& C, int64_t start, int64_t cnt, const char* var = "VR_CAPTURE_TILED",
+                    int fallback = 0) {
+    const int64_t hw = (int64_t)C.width * C.height;
+    const int mode = env_int(var, fallback);
+    const bool whole = !C.sensors && hw > 0 && hw <= (int64_t)UINT32_MAX && start % hw == 0 && cnt % hw == 0;
+    P.tiled = whole && (mode == 1 || (mode == 2 && hw % 64 == 0)) ? mode : 0;
 }
 
 int capture_prepare(const CaptureSplit& P, float* out, int od, int64_t start, int64_t cnt, int* dbg, hipStream_t st) {
@@ -657,7 +697,8 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
     }
     int in_dim = 0, od = 0;
     map_dims(kind, in_dim, od);
-    const CaptureSplit P = capture_split(spp, false);
+    CaptureSplit P = capture_split(spp, false);
+    capture_tiling(P, C, start, cnt);
     int rc = capture_prepare(P, out, od, start, cnt, dbg, st);
     if (rc) return rc;
     dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
@@ -677,6 +718,7 @@ int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Se
                           int64_t total,
                           int64_t start, int64_t cnt, const float* og, int* dbg, hipStream_t st) {
     CaptureSplit P = capture_split(spp, seed_mode == VR_SEED_CAPTURE_BW);
+    capture_tiling(P, C, start, cnt);
     P.debug_lookups = debug_kind == 1;
     int rc = capture_prepare(P, nullptr, 0, start, cnt, dbg, st);
     if (rc) return rc;
@@ -803,10 +845,10 @@ __global__ void __launch_bounds__(kSceneBlock) k_scene_capture_fw(SceneView S, C
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt * P.chunks) return;
     const int64_t c = j / cnt;
-    const int64_t i = start + (j - c * cnt);
+    int64_t i = start + (j - c * cnt);
     const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
     int pose, row, col;
-    sensor_index(cam, i, pose, row, col);
+    capture_sensor(cam, P, i, pose, row, col);
     if (pose < 0 || pose >= cam.num_poses) {
         if (P.chunks == 1) { out[3 * i] = 0.0f; out[3 * i + 1] = 0.0f; out[3 * i + 2] = 0.0f; }
         return;
@@ -838,10 +880,10 @@ __global__ void __launch_bounds__(kSceneBlock) k_scene_capture_bw(SceneView S, C
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt * P.chunks) return;
     const int64_t c = j / cnt;
-    const int64_t i = start + (j - c * cnt);
+    int64_t i = start + (j - c * cnt);
     const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
     int pose, row, col;
-    sensor_index(cam, i, pose, row, col);
+    capture_sensor(cam, P, i, pose, row, col);
     if (pose < 0 || pose >= cam.num_poses) return;
     const f3 g = load3(og + 3 * i) * (1.0f / (float)P.spp);
     if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) {
@@ -1514,7 +1556,8 @@ int vr_scene_capture_fw(const vr_scene* scene, const vr_camera* camera, const vr
     if (!out) return set_err(VR_ERR_ARG, "vr_scene_capture_fw: out is NULL");
     hipStream_t s = (hipStream_t)stream;
     const int spp = L->samples < 1 ? 1 : L->samples;
-    const CaptureSplit P = capture_split(spp, false);
+    CaptureSplit P = capture_split(spp, false);
+    capture_tiling(P, C, start, cnt, "VR_SCENE_TILED", 2);
     if ((st = capture_prepare(P, out, 3, start, cnt, L->debug_out, s))) return st;
     const int64_t threads = cnt * P.chunks;
     hipLaunchKernelGGL(k_scene_capture_fw, dim3((unsigned)((threads + kSceneBlock - 1) / kSceneBlock)), dim3(kSceneBlock), 0, s,
@@ -1539,6 +1582,7 @@ int vr_scene_capture_bw(const vr_scene* scene, const vr_camera* camera, const vr
     hipStream_t s = (hipStream_t)stream;
     const int spp = L->samples < 1 ? 1 : L->samples;
     CaptureSplit P = capture_split(spp, L->seed_mode == VR_SEED_CAPTURE_BW);
+    capture_tiling(P, C, start, cnt, "VR_SCENE_TILED", 2);
     P.debug_lookups = L->debug_kind == 1;
     if ((st = capture_prepare(P, nullptr, 0, start, cnt, L->debug_out, s))) return st;
     const int64_t threads = cnt * P.chunks;
