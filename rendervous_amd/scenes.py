@@ -303,16 +303,15 @@ class _GeometryBuffers:
     """Device buffers and BVH of a geometry, rebuilt when a position tensor or
     transform changes (MeshGeometry.update_ads_if_necessary, _scenes.py:485-490)."""
 
-    _cache: "dict" = {}
-
     @staticmethod
     def of(geometry: Geometry, device: torch.device) -> "_GeometryBuffers":
+        """The geometry's buffers on `device`, cached on the geometry object."""
         key = (geometry._version_key(), str(device))
-        b = _GeometryBuffers._cache.get(id(geometry))
+        b = geometry.__dict__.get("_bvh_cache")
         if b is None or b.key != key:
             b = _GeometryBuffers(geometry, device)
             b.key = key
-            _GeometryBuffers._cache[id(geometry)] = b
+            object.__setattr__(geometry, "_bvh_cache", b)
         return b
 
     def __init__(self, geometry: Geometry, device: torch.device):
