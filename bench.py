@@ -28,11 +28,12 @@ HBM_PEAK_GBS = 8000.0
 
 
 # HBM traffic of one launch of the persistent NEE kernel at the default workload
-# (256^3, 512x512, 64 spp), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in KiB
-# per dispatch, each scaled by what the counter reads for a 256 MiB device copy
-# (FETCH_SIZE reads 128 MiB for it, WRITE_SIZE 256 MiB: factors 2 and 1).  The
-# raw figures and the commands are in profiles/nee_persistent_pmc_256cube_512x512x64spp.txt.
-NEE_PROFILED = {"workload": (256, 512, 512, 64), "fetch_kib": 1.8694e7, "write_kib": 1.10636e5,
+# (256^3, 512x512, 64 spp, density read through its cell table), from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by what the
+# counter reads for a 256 MiB device copy (FETCH_SIZE reads 128 MiB for it,
+# WRITE_SIZE 256 MiB: factors 2 and 1).  The raw figures and the commands are in
+# profiles/nee_persistent_pmc_256cube_512x512x64spp.txt.
+NEE_PROFILED = {"workload": (256, 512, 512, 64), "fetch_kib": 5.63486e6, "write_kib": 1.10572e5,
                 "fetch_scale": 2.0, "write_scale": 1.0}
 
 

@@ -165,6 +165,8 @@ typedef struct vr_medium {
     const float* env_quadtree;     /* VR_ENVS_QUADTREE: functions.create_density_quadtree output */
     int32_t env_quadtree_levels;   /* 1 .. 15 */
     int32_t env_reserved;
+    const float* sigma_cells;      /* optional cell table of sigma (vr_grid_pack_cells); NULL = read
+                                      the grid.  Lookups give bit-identical values either way. */
 } vr_medium;
 
 /* PerspectiveCameraSensor (_maps.py:1765-1786, include/maps/sensor_perspective_camera.h). */
@@ -408,6 +410,13 @@ int vr_nrc_paths(const vr_medium* medium, const vr_camera* camera, const vr_laun
 int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_launch* launch, int32_t terminate_after,
                   const uint16_t* weights, float* out, float* queries, int64_t max_queries, uint32_t* count,
                   void* stream);
+
+/* Cell table of a single-channel grid [D,H,W,1]: for each of the (D-1)(H-1)(W-1)
+ * cells (x fastest) its 8 corner values in the order x, then y, then z fastest
+ * (32 bytes per cell), so that a trilinear lookup reads one cell with two 16-byte
+ * loads.  out: (D-1)(H-1)(W-1) * 8 floats, 16-byte aligned.  No reference
+ * counterpart (a layout of the Grid3D tensor, include/maps/grid3d.h:6-37). */
+int vr_grid_pack_cells(const vr_grid* grid, float* out, void* stream);
 
 /* Builds the BVH of a triangle mesh on the host (binned SAH, at most 4
  * triangles per leaf, depth <= 30).  positions [num_vertices,3], indices

@@ -2729,3 +2729,19 @@ int oracle_scene_capture_bw(const vr_scene* S, const vr_camera* cam, const vr_la
     }
     return VR_OK;
 }
+
+/* Cell table of a single-channel grid (vr_grid_pack_cells): cell c = (pz*(ny-1)
+ * + py)*(nx-1) + px holds its 8 corners, x fastest, then y, then z. */
+int oracle_grid_pack_cells(const vr_grid* g, float* out) {
+    if (!g || !out || g->t.ndim != 4 || g->t.shape[3] != 1) return fail(VR_ERR_ARG, "bad grid arguments");
+    const vr_tensor* t = &g->t;
+    int64_t nx = t->shape[2], ny = t->shape[1], nz = t->shape[0];
+    if (nx < 2 || ny < 2 || nz < 2) return fail(VR_ERR_ARG, "grid needs 2 vertices per axis");
+    int64_t c = 0;
+    for (int64_t z = 0; z < nz - 1; z++)
+        for (int64_t y = 0; y < ny - 1; y++)
+            for (int64_t x = 0; x < nx - 1; x++, c++)
+                for (int k = 0; k < 8; k++)
+                    out[8 * c + k] = t->data[goff(t, (int)(z + ((k >> 2) & 1)), (int)(y + ((k >> 1) & 1)), (int)(x + (k & 1)))];
+    return VR_OK;
+}

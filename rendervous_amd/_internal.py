@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import contextlib
 import dataclasses
+import os
 import typing
 
 import torch
@@ -57,6 +58,20 @@ class BackwardSettings:
 
 
 backward_settings = BackwardSettings()
+
+
+@dataclasses.dataclass
+class Settings:
+    """cell_tables: give a medium a cell table of its density (8 corners per
+    cell, 8x the grid's memory) so that each trilinear lookup reads one 32-byte
+    cell; the values are the same bit for bit as reading the grid.  "auto"
+    builds one for densities that do not require a gradient (a density that an
+    optimiser updates every step would rebuild its table every step); True /
+    False force it on / off."""
+    cell_tables: typing.Union[str, bool] = "auto"
+
+
+settings = Settings(cell_tables={"0": False, "1": True}.get(os.environ.get("VR_CELL_TABLES", ""), "auto"))
 
 
 @contextlib.contextmanager

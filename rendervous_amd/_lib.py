@@ -114,6 +114,7 @@ class VrMedium(ctypes.Structure):
         ("env_quadtree", ctypes.c_void_p),
         ("env_quadtree_levels", ctypes.c_int32),
         ("env_reserved", ctypes.c_int32),
+        ("sigma_cells", ctypes.c_void_p),
     ]
 
 
@@ -228,6 +229,7 @@ _SIGNATURES = {
                                     ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
     "vr_nrc_render": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_grid_pack_cells": (ctypes.c_int, [P(VrGrid), ctypes.c_void_p, ctypes.c_void_p]),
     "vr_scene_build_bvh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                           ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_int32)]),
     "vr_scene_raycast": (ctypes.c_int, [P(VrScene), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),

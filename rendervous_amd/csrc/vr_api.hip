@@ -177,6 +177,11 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
     v.medium_filter = m->medium_filter;
     v.gathering_c = load3(m->gathering_c);
     v.medium_majorant = m->medium_majorant;
+    if (m->sigma_cells && kind != VR_MAP_RAYBOX) {
+        if (reinterpret_cast<uintptr_t>(m->sigma_cells) % 16 != 0)
+            return set_err(VR_ERR_ARG, "sigma cell table must be 16-byte aligned");
+        v.cells = reinterpret_cast<const float4*>(m->sigma_cells);
+    }
     if (bw_mode != VR_BW_REFERENCE && bw_mode != VR_BW_CONSISTENT) return set_err(VR_ERR_ARG, "unknown bw mode");
     v.bw_mode = bw_mode;
     if ((st = make_env_view(m, v))) return st;
@@ -495,6 +500,17 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
         if (P.chunks == 1) dbg[i] = n;
         else atomicAdd(dbg + i, n);
     }
+}
+
+// Cell table of a single-channel grid: thread c writes cell c's 8 corners.
+__global__ void __launch_bounds__(kBlock) k_pack_cells(GridView g, float4* __restrict__ out, int64_t cells) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int cx = g.nx - 1, cy = g.ny - 1;
+    const int px = (int)(c % cx), py = (int)((c / cx) % cy), pz = (int)(c / ((int64_t)cx * cy));
+    const float* p = g.data + (pz * g.sz + py * g.sy + px * g.sx);
+    out[2 * c] = make_float4(p[0], p[g.sx], p[g.sy], p[g.sx + g.sy]);
+    out[2 * c + 1] = make_float4(p[g.sz], p[g.sz + g.sx], p[g.sz + g.sy], p[g.sz + g.sy + g.sx]);
 }
 
 // Zeroes rows [start, start+cnt) of a capture output (OD floats each) and of the debug counters.
@@ -1451,6 +1467,19 @@ int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_lau
     return check_hip(hipGetLastError(), "vr_nrc_render apply");
 }
 
+
+int vr_grid_pack_cells(const vr_grid* grid, float* out, void* stream) {
+    GridView g;
+    int st = make_grid_view(grid, g, 1, "vr_grid_pack_cells");
+    if (st) return st;
+    if (!out) return set_err(VR_ERR_ARG, "vr_grid_pack_cells: out is NULL");
+    if (reinterpret_cast<uintptr_t>(out) % 16 != 0) return set_err(VR_ERR_ARG, "vr_grid_pack_cells: out must be 16-byte aligned");
+    const int64_t cells = (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1);
+    if (2 * cells >= (int64_t(1) << 31)) return set_err(VR_ERR_ARG, "vr_grid_pack_cells: more than 2^30 cells");
+    hipLaunchKernelGGL(k_pack_cells, dim3(grid_for(cells)), dim3(kBlock), 0, (hipStream_t)stream, g,
+                       reinterpret_cast<float4*>(out), cells);
+    return check_hip(hipGetLastError(), "vr_grid_pack_cells launch");
+}
 
 int vr_scene_build_bvh(const float* positions, int64_t num_vertices, const int32_t* indices, int64_t num_tris,
                        vr_bvh_node* nodes, int64_t max_nodes, int32_t* tri_ids, float* tri_verts, int32_t* num_nodes) {

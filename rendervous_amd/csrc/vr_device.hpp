@@ -171,6 +171,7 @@ struct MediumView {
     ImageView env_img;
     const float* quadtree; // XRQuadtreeRandomDirection densities
     int qt_levels;
+    const float4* cells;   // optional sigma cell table (2 float4 per cell), else null
 };
 
 struct CameraView {
@@ -225,6 +226,37 @@ VR_DEV float grid_lookup1(const GridView& g, f3 c) {
     r += v011 * (bx * ay * az);
     r += v111 * (ax * ay * az);
     return r;
+}
+
+// grid_lookup1 reading the cell table: the cell's 8 corners are two float4
+// (x fastest, then y, then z) at 32-byte index (pz*(ny-1) + py)*(nx-1) + px.
+// Same coordinates, weights and summation order as grid_lookup1, so the same
+// value bit for bit.
+VR_DEV float cells_lookup1(const GridView& g, const float4* __restrict__ cells, f3 c) {
+    float gx = fmaf(c.x, g.scale.x, g.offset.x);
+    float gy = fmaf(c.y, g.scale.y, g.offset.y);
+    float gz = fmaf(c.z, g.scale.z, g.offset.z);
+    if (!(gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx < g.dimf.x && gy < g.dimf.y && gz < g.dimf.z)) return 0.0f;
+    float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float ax = gx - fx, ay = gy - fy, az = gz - fz;
+    int px = min((int)fx, g.nx - 2), py = min((int)fy, g.ny - 2), pz = min((int)fz, g.nz - 2);
+    const float4* q = cells + 2 * ((pz * (g.ny - 1) + py) * (g.nx - 1) + px);
+    const float4 lo = q[0], hi = q[1];
+    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    float r = lo.x * (bx * by * bz);
+    r += lo.y * (ax * by * bz);
+    r += lo.z * (bx * ay * bz);
+    r += lo.w * (ax * ay * bz);
+    r += hi.x * (bx * by * az);
+    r += hi.y * (ax * by * az);
+    r += hi.z * (bx * ay * az);
+    r += hi.w * (ax * ay * az);
+    return r;
+}
+
+// The medium's sigma at x: from the cell table when the medium has one.
+VR_DEV float sigma_lookup(const MediumView& M, f3 x) {
+    return M.cells ? cells_lookup1(M.sigma, M.cells, x) : grid_lookup1(M.sigma, x);
 }
 
 // Multi-channel lookup (C <= 4), grid3d.h:6-37.
@@ -558,7 +590,7 @@ struct Tracer {
     VR_DEV float rnd() { return rng_uniform(rng); }
     VR_DEV float flight(float maj) { return neg_log1m(rnd()) * rcp(maj); }
 
-    VR_DEV float sigma(f3 x) { lookups++; return grid_lookup1(M.sigma, x); }
+    VR_DEV float sigma(f3 x) { lookups++; return sigma_lookup(M, x); }
     VR_DEV void sigma_bw(f3 x, float g) {
         if (g != 0.0f && M.sigma.grad != nullptr) scatters++;
         grid_scatter1(M.sigma, x, g);

@@ -159,7 +159,7 @@ struct NeeLane {
 
         if (need_sigma) {
             // the shared trilinear gather
-            float sg = grid_lookup1(M.sigma, rx);
+            float sg = sigma_lookup(M, rx);
             lookups++;
             if (mode == kModePath) {
                 if (rng_uniform(rng) * m >= sg) rd -= t;  // null collision

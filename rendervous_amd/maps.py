@@ -227,6 +227,20 @@ class Grid3D(MapBase):
     def view(self, grad: typing.Optional[torch.Tensor] = None) -> L.VrGrid:
         return L.grid_view(self.grid.data, self.bmin, self.bmax, grad)
 
+    def cells_tensor(self) -> torch.Tensor:
+        """The grid's cell table (vr_grid_pack_cells: 8 corners per cell, 32 bytes),
+        rebuilt when the grid tensor changes (its pointer, version or layout)."""
+        g = self.grid
+        key = (g.data_ptr(), g._version, tuple(g.shape), tuple(g.stride()))
+        cache = self.__dict__.get("_cells_cache")
+        if cache is None or cache[0] != key:
+            n = (g.shape[0] - 1) * (g.shape[1] - 1) * (g.shape[2] - 1)
+            out = torch.empty(max(1, n) * 8, device=g.device, dtype=torch.float32)
+            L.check(L.load().vr_grid_pack_cells(self.view(), L.ptr(out), L.stream_handle(g.device)), "vr_grid_pack_cells")
+            cache = (key, out)
+            object.__setattr__(self, "_cells_cache", cache)
+        return cache[1]
+
     def get_maximum(self) -> float:
         return self.grid.max().item()
 
@@ -427,6 +441,11 @@ class _Lowered:
         self.sigma = sigma
         self.device = sigma.grid.device
         self.medium.sigma = sigma.view()
+        ct = _internal.settings.cell_tables
+        if sigma.grid.is_cuda and (ct is True or (ct == "auto" and not sigma.grid.requires_grad)):
+            cells = sigma.cells_tensor()
+            self.keep.append(cells)
+            self.medium.sigma_cells = cells.data_ptr()
 
     def set_boundary(self, boundary: MapBase):
         if not isinstance(boundary, RayBoxIntersection):

@@ -117,3 +117,43 @@ def test_rays_that_miss_or_start_inside_the_box(gpu, kind):
     want = 1.0 if L.MAP_DIMS[kind][1] == 1 else np.array([0.5, 0.5, 0.5], np.float32)
     np.testing.assert_allclose(out[:512], np.broadcast_to(want, out[:512].shape), rtol=1e-6)
     assert match_fraction(out, ref, rtol=1e-3, atol=1e-4) >= 0.98
+
+
+def test_cell_table_lookups_are_bit_identical(gpu):
+    """Captures reading the sigma cell table (vr_grid_pack_cells) equal captures
+    reading the grid bit for bit: the persistent NEE kernel and a generic capture
+    kernel, on a strided density view; the table itself equals the oracle's."""
+    import torch
+    from helpers import O, np_grid, pose
+    g = fbm_grid(24, seed=3, scale=25.0)
+    wide = np.zeros(g.shape[:3] + (3,), np.float32)
+    wide[..., 1:2] = g
+    sc = Scene(sigma=g, albedo_c=(0.8, 0.8, 0.8), phase_g=0.5, env_c=(0.2, 0.3, 0.4), sun_dir=(0.3, 0.9, 0.3),
+               sun_radiance=(2.0, 2.0, 2.0), sun_prob=0.5, majorant="grid", majorant_res=(4, 4, 4), max_bounces=8)
+    b = Built(sc, "cuda")
+    dev_wide = torch.from_numpy(wide).cuda()
+    view = dev_wide[..., 1:2]  # strided: channel stride 3
+    b.medium.sigma.t = L.tensor_view(view)
+    n = 23 ** 3
+    cells = torch.empty(n * 8, device="cuda")
+    L.check(L.load().vr_grid_pack_cells(b.medium.sigma, cells.data_ptr(), torch.cuda.current_stream().cuda_stream),
+            "pack")
+    ref = np.zeros((n, 8), np.float32)
+    O.check(O.load().oracle_grid_pack_cells(np_grid(np.ascontiguousarray(g)), O.nptr(ref)), "oracle pack")
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(cells.cpu().numpy().reshape(n, 8), ref)
+    cam = Camera(np.stack([pose((0.2, 0.4, 2.2)), pose((-1.9, 0.1, 1.0))]), 24, 24, jittered=True, device="cuda")
+    for kind in (L.VR_MAP_NEE_DT, L.VR_MAP_MC_COLLISION, L.VR_MAP_PATH_DT):
+        outs = []
+        for use in (False, True):
+            b.medium.sigma_cells = cells.data_ptr() if use else None
+            out = torch.zeros(cam.threads, 3, device="cuda")
+            # one sample per pixel for the persistent kernel (its per-item partial
+            # sums are added atomically, in no fixed order, when spp > 1)
+            spp = 1 if kind == L.VR_MAP_NEE_DT else 4
+            lau = L.launch((5, 6, 7, 8), samples=spp, bw_mode=L.VR_BW_CONSISTENT)
+            L.check(L.load().vr_capture_fw(b.medium, kind, cam.c, lau, out.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream), "capture")
+            torch.cuda.synchronize()
+            outs.append(out.cpu().numpy())
+        np.testing.assert_array_equal(outs[0], outs[1])

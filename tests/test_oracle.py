@@ -433,3 +433,16 @@ def test_adam_matches_torch_optim_adam():
         assert np.all(gb == g)
     np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-5, atol=1e-6)
     assert pb.min() >= 0.0 and pb.max() <= 0.5
+
+
+def test_cell_table_layout():
+    """vr_grid_pack_cells' layout (oracle twin): cell (z, y, x) of a strided
+    [D,H,W,1] view holds its 8 corners, x fastest, then y, then z."""
+    rng = np.random.default_rng(4)
+    full = rng.normal(size=(5, 7, 9, 2)).astype(np.float32)
+    g = np.ascontiguousarray(full[..., :1])
+    out = np.zeros((4 * 6 * 8, 8), np.float32)
+    O.check(O.load().oracle_grid_pack_cells(np_grid(g), O.nptr(out)), "pack")
+    v = g[..., 0]
+    expect = np.stack([v[z:z + 4, y:y + 6, x:x + 8] for z in (0, 1) for y in (0, 1) for x in (0, 1)], -1)
+    np.testing.assert_array_equal(out, expect.reshape(-1, 8))
