@@ -442,10 +442,12 @@ class _Lowered:
         self.device = sigma.grid.device
         self.medium.sigma = sigma.view()
         ct = _internal.settings.cell_tables
-        if sigma.grid.is_cuda and (ct is True or (ct == "auto" and not sigma.grid.requires_grad)):
-            cells = sigma.cells_tensor()
-            self.keep.append(cells)
-            self.medium.sigma_cells = cells.data_ptr()
+        g = sigma.grid
+        n_cells = (g.shape[0] - 1) * (g.shape[1] - 1) * (g.shape[2] - 1)
+        if g.is_cuda and n_cells < (1 << 30) and (ct is True or (ct == "auto" and not g.requires_grad)):
+            table = sigma.cells_tensor()
+            self.keep.append(table)
+            self.medium.sigma_cells = table.data_ptr()
 
     def set_boundary(self, boundary: MapBase):
         if not isinstance(boundary, RayBoxIntersection):
