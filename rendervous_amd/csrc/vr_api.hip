@@ -23,6 +23,7 @@
 #include "vr_nrc.hpp"
 #include "vr_functions.hpp"
 #include "vr_scene.hpp"
+#include "vr_nee_bw_flat.hpp"
 
 using namespace vr;
 
@@ -502,6 +503,65 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
     }
 }
 
+// The NEE capture backward (defensive replay, DT shadow rays) with the flat
+// lane state machine of vr_nee_bw_flat.hpp: same thread -> (sensor, samples)
+// split and seeds as k_capture_bw, so the same gradients sample by sample.
+// Resident waves per SIMD the flat NEE backward is compiled for: 4 leave 128
+// VGPRs per lane and spill a few, yet beat 3 waves (148 against 158 ms for the
+// backward of a 256^3 / 512x512 / 64 spp frame).
+#ifndef VR_NEE_BW_WAVES_PER_EU
+#define VR_NEE_BW_WAVES_PER_EU 4
+#endif
+template <int MAJ>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VR_NEE_BW_WAVES_PER_EU)))
+k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int seed_mode, int64_t total,
+                      int64_t start, int64_t cnt, const float* __restrict__ og, int* __restrict__ dbg) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt * P.chunks) return;
+    const int64_t c = j / cnt;
+    int64_t i = start + (j - c * cnt);
+    const int s0 = (int)c * P.chunk, s1 = min(P.spp, s0 + P.chunk);
+    int pose, row, col;
+    capture_sensor(cam, P, i, pose, row, col);
+    if (pose < 0 || pose >= cam.num_poses) return;
+    const f3 g = F3(og[3 * i] / (float)P.spp, og[3 * i + 1] / (float)P.spp, og[3 * i + 2] / (float)P.spp);
+    if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) {
+        if (dbg && P.chunks == 1) dbg[i] = 0;
+        return;
+    }
+    NeeDsBwLane<MAJ> L(M);
+    L.eps = M.ds_eps;
+    L.dL_dR = g;
+    // the reference's fresh-seed backward continues one stream over the samples (one chunk)
+    if (seed_mode == VR_SEED_CAPTURE_BW) L.tr.rng = seed_capture_bw(seeds, (uint32_t)i);
+    int s = s0;
+    bool live = s < s1;
+    if (live) {
+        if (seed_mode != VR_SEED_CAPTURE_BW) L.tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
+        camera_ray(cam, pose, row, col, L.tr.rng, L.x0, L.w0);
+        L.begin_pass(1);
+    }
+    // every sample ends (its paths end); the cap only bounds a thread's loop
+    for (int it = 0; live && it < (1 << 24); it++) {
+        if (L.state == kBwDone) {
+            if (++s >= s1) break;
+            if (seed_mode != VR_SEED_CAPTURE_BW) L.tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
+            camera_ray(cam, pose, row, col, L.tr.rng, L.x0, L.w0);
+            L.begin_pass(1);
+        }
+        L.events();
+        for (int k = 0; k < 3; k++) {
+            if (L.tracking()) L.step();
+            if (__ballot(L.tracking()) == 0ull) break;
+        }
+    }
+    if (dbg) {
+        const int n = P.debug_lookups ? L.tr.lookups : L.tr.scatters;
+        if (P.chunks == 1) dbg[i] = n;
+        else atomicAdd(dbg + i, n);
+    }
+}
+
 // Cell table of a single-channel grid: thread c writes cell c's 8 corners.
 __global__ void __launch_bounds__(kBlock) k_pack_cells(GridView g, float4* __restrict__ out, int64_t cells) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -739,6 +799,10 @@ int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Se
     int rc = capture_prepare(P, nullptr, 0, start, cnt, dbg, st);
     if (rc) return rc;
     dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
+    if (kind == VR_MAP_NEE_DT && M.t_kind == VR_T_DT && env_int("VR_NEE_BW_FLAT", 1) != 0) {
+        hipLaunchKernelGGL((k_capture_bw_nee_flat<MAJ>), g, b, 0, st, M, C, sd, P, seed_mode, total, start, cnt, og, dbg);
+        return check_hip(hipGetLastError(), "vr_capture_bw launch");
+    }
     switch (kind) {
 #define VR_CASE(K) case K: hipLaunchKernelGGL((k_capture_bw<K, MAJ>), g, b, 0, st, M, C, sd, P, seed_mode, total, start, cnt, og, dbg); break;
         VR_CASE(VR_MAP_T_DDA) VR_CASE(VR_MAP_T_DT) VR_CASE(VR_MAP_T_RT) VR_CASE(VR_MAP_COLLISION_DT)

@@ -157,3 +157,49 @@ def test_cell_table_lookups_are_bit_identical(gpu):
             torch.cuda.synchronize()
             outs.append(out.cpu().numpy())
         np.testing.assert_array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("bw_mode", [L.VR_BW_CONSISTENT, L.VR_BW_REFERENCE])
+@pytest.mark.parametrize("majorant", ["grid", "const"])
+def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant):
+    """The NEE capture backward as a flat lane state machine (the default for DT
+    shadow rays) scatters the same gradients, sample by sample, as the nested
+    estimator (VR_NEE_BW_FLAT=0): same scatter counts, density and albedo
+    gradients equal up to the order of the float atomics.  Sensors with a zero
+    output gradient trace nothing."""
+    import os
+    import torch
+    g = fbm_grid(20, seed=5, scale=30.0)
+    alb = (0.5 + 0.45 * fbm_grid(8, seed=6, scale=1.0)).astype(np.float32)
+    sc = Scene(sigma=g, albedo=np.repeat(alb, 3, -1), phase_g=0.4, env_c=(0.3, 0.4, 0.5), sun_dir=(0.3, 0.9, 0.3),
+               sun_radiance=(2.0, 2.0, 2.0), sun_prob=0.4, majorant=majorant, majorant_res=(4, 4, 4), max_bounces=6,
+               transmittance_kind=L.VR_T_DT)
+    cam = dict(poses=np.stack([pose((0.2, 0.4, 2.2)), pose((-1.9, 0.1, 1.0))]), width=20, height=20, jittered=True)
+    og = (np.random.default_rng(3).uniform(size=(800, 3)) + 0.5).astype(np.float32)
+    og[::7] = 0.0
+    res = []
+    old = os.environ.get("VR_NEE_BW_FLAT")
+    try:
+        for flat in ("0", "1"):
+            os.environ["VR_NEE_BW_FLAT"] = flat
+            b = Built(sc, "cuda", grad_sigma=True, grad_albedo=True)
+            c = Camera(device="cuda", **cam)
+            dbg = torch.zeros(c.threads, dtype=torch.int32, device="cuda")
+            o = torch.from_numpy(og).cuda()
+            lau = L.launch((9, 8, 7, 6), samples=8, bw_mode=bw_mode, seed_mode=L.VR_SEED_CAPTURE_FW, debug_out=dbg)
+            L.check(L.load().vr_capture_bw(b.medium, L.VR_MAP_NEE_DT, c.c, lau, o.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream), "capture_bw")
+            torch.cuda.synchronize()
+            res.append((b.grads(), dbg.cpu().numpy()))
+    finally:
+        if old is None:
+            os.environ.pop("VR_NEE_BW_FLAT", None)
+        else:
+            os.environ["VR_NEE_BW_FLAT"] = old
+    from helpers import rel_l1
+    (gs0, ga0), d0 = res[0]
+    assert np.abs(gs0).sum() > 0 and np.abs(ga0).sum() > 0
+    assert (d0[::7] == 0).all() and (d0 > 0).mean() > 0.2
+    for (gs1, ga1), d1 in res[1:]:
+        np.testing.assert_array_equal(d0, d1)
+        assert rel_l1(gs1, gs0) <= 1e-5 and rel_l1(ga1, ga0) <= 1e-5, (rel_l1(gs1, gs0), rel_l1(ga1, ga0))
