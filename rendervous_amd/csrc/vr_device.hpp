@@ -33,7 +33,13 @@ constexpr float kCrossEps = 1e-6f;
 VR_DEV float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 VR_DEV float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 VR_DEV float rsqrt_fast(float x) { return __builtin_amdgcn_rsqf(x); }
-VR_DEV float neg_log1m(float u) { return -__logf(1.0f - u); }  // -log(1-u) of a free-flight draw
+// v_sqrt_f32 for a sine from a cosine (argument in [0, 1]; sqrtf lowers to a
+// correctly rounded sequence with denormal rescaling)
+VR_DEV float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+// -log(1-u) of a free-flight draw as -ln2 * log2(1-u): u in [0, 1) on a 2^-23
+// lattice keeps 1-u a normal float, so v_log_f32 needs no denormal rescaling
+// (__logf lowers to the correctly rounded libm sequence).
+VR_DEV float neg_log1m(float u) { return -0.693147182f * __builtin_amdgcn_logf(1.0f - u); }
 // sine and cosine of 2*pi*turns
 VR_DEV void sincos_turns(float turns, float& s, float& c) {
     s = __builtin_amdgcn_sinf(turns);
@@ -311,7 +317,7 @@ VR_DEV void grid_scatter1(const GridView& g, f3 c, float og) { grid_scatterC(g, 
 // include/randoms.h:144-151
 VR_DEV void ortho_basis(f3 n, f3& t, f3& b) {
     float sign = n.z >= 0.0f ? 1.0f : -1.0f;
-    float a = -1.0f / (sign + n.z);
+    float a = -rcp(sign + n.z);  // |sign + n.z| >= 1
     float bb = n.x * n.y * a;
     t = F3(1.0f + sign * n.x * n.x * a, sign * bb, -sign * n.x);
     b = F3(bb, sign + n.y * n.y * a, -n.y);
@@ -321,7 +327,7 @@ VR_DEV void ortho_basis(f3 n, f3& t, f3& b) {
 VR_DEV f3 random_direction(Rng& rng, f3 D) {
     float r1 = rng_uniform(rng);
     float r2 = rng_uniform(rng) * 2.0f - 1.0f;
-    float s = sqrtf(fmaxf(0.0f, 1.0f - r2 * r2));
+    float s = sqrt_fast(fmaxf(0.0f, 1.0f - r2 * r2));
     float sn, cs;
     sincos_turns(r1, sn, cs);
     f3 t0, t1;
@@ -340,7 +346,7 @@ VR_DEV f3 hg_sample(Rng& rng, f3 win, float g) {
         float t = fdiv(1.0f - g2, 1.0f - g + 2.0f * g * xi);
         cos_t = fdiv(0.5f, g) * (1.0f + g2 - t * t);
     }
-    float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    float sin_t = sqrt_fast(fmaxf(0.0f, 1.0f - cos_t * cos_t));
     float sp, cp;
     sincos_turns(phi_turns, sp, cp);
     f3 t0, t1;
@@ -459,7 +465,7 @@ VR_DEV float quadtree_direction(const MediumView& M, Rng& rng, f3& wo) {
     const float a0 = x0 - 0.5f, a1 = x1 - 0.5f;
     const float r1 = rng_uniform(rng) * (a1 - a0) + a0;
     const float r2 = cb0 - rng_uniform(rng) * (cb0 - cb1);
-    const float s = sqrtf(fmaxf(0.0f, 1.0f - r2 * r2));
+    const float s = sqrt_fast(fmaxf(0.0f, 1.0f - r2 * r2));
     float sn, cs;
     sincos_turns(r1, sn, cs);
     wo = F3(sn * s, r2, cs * s);
