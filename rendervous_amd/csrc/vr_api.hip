@@ -1348,6 +1348,17 @@ int resample(const float* in, const int64_t* in_shape, const int64_t* out_shape,
         for (int i = 0; i < 3; i++) R3.in_n[i] = I.n[i];
         R3.in_sy = I.s[1];
         R3.in_sz = I.s[2];
+        // source rows a band of kBandY output rows reads: (kBandY - 1) * ratio + 2,
+        // one more for rounding; the kernel itself checks the exact count
+        const double ratio = CELL ? (double)I.n[1] / R.out.n[1] : (double)(I.n[1] - 1) / (R.out.n[1] - 1);
+        const int64_t band_rows = (int64_t)std::ceil((fn::kBandY - 1) * ratio) + 3;
+        const int64_t lds_floats = 3 * (int64_t)R3.out.nx + 2 * band_rows * I.n[0];
+        if (env_int("VR_RESAMPLE_LDS", 1) != 0 && lds_floats * 4 <= fn::kResampleLdsBytes) {
+            const dim3 grid((unsigned)((R3.out.ny + fn::kBandY - 1) / fn::kBandY), (unsigned)R3.out.nz);
+            hipLaunchKernelGGL(fn::k_resample_lds<CELL>, grid, dim3(256), (unsigned)(lds_floats * 4), (hipStream_t)stream,
+                               in, R3, out, (int)lds_floats);
+            return check_hip(hipGetLastError(), what);
+        }
         hipLaunchKernelGGL(fn::k_resample_x4<CELL>, dense3_grid(R3.out), dim3(fn::kQuadX, fn::kRowsY), 0, (hipStream_t)stream, in,
                            R3, out);
         return check_hip(hipGetLastError(), what);

@@ -344,6 +344,104 @@ __global__ void __launch_bounds__(kQuadX * kRowsY) k_resample_x4(const float* __
     st4_stream(out + x0 + y * R.out.sy + z * R.out.sz, r[0], r[1], r[2], r[3]);
 }
 
+// LDS-staged form of k_resample_x4 for a dense 3-D grid: a workgroup covers a
+// band of kBandY output rows of one output plane.  It first copies the source
+// rows the band reads (planes z0 and z1, rows y0(first row) .. y1(last row);
+// both are monotone in the output row) into LDS with coalesced loads and tabulates
+// the x taps of every output column (two indices and the fraction), then each
+// wave forms rows of the band, 4 outputs per lane, from LDS with the taps and
+// arithmetic order of k_resample_x4, so the results are bit-identical; each
+// source value is read from memory about once per band instead of once per tap.
+// Dynamic LDS: [xi0 | xi1 | ax] (nxo each, nxo a multiple of 4) then the rows.
+constexpr int kBandY = 8;
+constexpr int kResampleLdsBytes = 64 * 1024;  // most dynamic LDS a workgroup asks for
+
+// One output row: 4 consecutive outputs per lane from the source rows (y0,z0),
+// (y1,z0), (y0,z1), (y1,z1) and the tabulated x taps.
+template <typename RowPtr>
+__attribute__((always_inline)) VR_DEV void resample_row4(RowPtr r0, RowPtr r1, RowPtr r2, RowPtr r3, const int* tx0,
+                                                         const int* tx1, const float* tax, int nxo, float ay, float az,
+                                                         int lane, float* o) {
+#pragma clang fp contract(off)
+    for (int x0 = lane * 4; x0 < nxo; x0 += 256) {
+        const int4 i0 = *reinterpret_cast<const int4*>(tx0 + x0), i1 = *reinterpret_cast<const int4*>(tx1 + x0);
+        const float4 a4 = *reinterpret_cast<const float4*>(tax + x0);
+        const int xi[4][2] = {{i0.x, i1.x}, {i0.y, i1.y}, {i0.z, i1.z}, {i0.w, i1.w}};
+        const float ax[4] = {a4.x, a4.y, a4.z, a4.w};
+        float r[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float v[8] = {r0[xi[k][0]], r0[xi[k][1]], r1[xi[k][0]], r1[xi[k][1]],
+                                r2[xi[k][0]], r2[xi[k][1]], r3[xi[k][0]], r3[xi[k][1]]};
+            float acc = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const float fx = (t & 1) ? ax[k] : 1.0f - ax[k];
+                const float fy = (t & 2) ? ay : 1.0f - ay;
+                const float fz = (t & 4) ? az : 1.0f - az;
+                const float w = (fx * fy) * fz;
+                acc += w * v[t];
+            }
+            r[k] = acc;
+        }
+        st4_stream(o + x0, r[0], r[1], r[2], r[3]);
+    }
+}
+
+template <bool CELL>
+__global__ void __launch_bounds__(256) k_resample_lds(const float* __restrict__ in, Resample3 R, float* __restrict__ out,
+                                                      int lds_floats) {
+#pragma clang fp contract(off)
+    extern __shared__ float lds[];
+    const int nxo = R.out.nx, nxi = R.in_n[0];
+    int* tx0 = reinterpret_cast<int*>(lds);
+    int* tx1 = tx0 + nxo;
+    float* tax = lds + 2 * nxo;
+    float* tile = lds + 3 * nxo;
+    const int Y0 = blockIdx.x * kBandY;
+    const int Y1 = min(Y0 + kBandY, R.out.ny) - 1;
+    const int z = blockIdx.y;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    int z0, z1, ya, yb, unused;
+    float az, fu;
+    axis_coord<CELL>(z, R.in_n[2], R.out.nz, z0, z1, az);
+    axis_coord<CELL>(Y0, R.in_n[1], R.out.ny, ya, unused, fu);
+    axis_coord<CELL>(Y1, R.in_n[1], R.out.ny, unused, yb, fu);
+    const int nrows = yb - ya + 1;
+    const bool staged = 3 * nxo + 2 * nrows * nxi <= lds_floats;  // else the taps are read from memory
+    for (int x = threadIdx.x; x < nxo; x += 256) {
+        int a, b;
+        float f;
+        axis_coord<CELL>(x, nxi, nxo, a, b, f);
+        tx0[x] = a;
+        tx1[x] = b;
+        tax[x] = f;
+    }
+    if (staged) {
+        // row r of plane p (0: z0, 1: z1) at tile[(p * nrows + r) * nxi]
+        for (int row = wave; row < 2 * nrows; row += 4) {
+            const int p = row >= nrows, r = row - p * nrows;
+            const float* src = in + (ya + r) * R.in_sy + (p ? z1 : z0) * R.in_sz;
+            float* dst = tile + row * nxi;
+            for (int x = lane; x < nxi; x += 64) dst[x] = src[x];
+        }
+    }
+    __syncthreads();
+    for (int y = Y0 + wave; y <= Y1; y += 4) {
+        int y0, y1;
+        float ay;
+        axis_coord<CELL>(y, R.in_n[1], R.out.ny, y0, y1, ay);
+        float* o = out + y * R.out.sy + z * R.out.sz;
+        if (staged)  // separate calls keep the LDS rows in the LDS address space (ds_read)
+            resample_row4(tile + (y0 - ya) * nxi, tile + (y1 - ya) * nxi, tile + (nrows + y0 - ya) * nxi,
+                          tile + (nrows + y1 - ya) * nxi, tx0, tx1, tax, nxo, ay, az, lane, o);
+        else
+            resample_row4(in + y0 * R.in_sy + z0 * R.in_sz, in + y1 * R.in_sy + z0 * R.in_sz,
+                          in + y0 * R.in_sy + z1 * R.in_sz, in + y1 * R.in_sy + z1 * R.in_sz, tx0, tx1, tax, nxo, ay,
+                          az, lane, o);
+    }
+}
+
 // maths.h:244-251 unpart1by1: compact the even bits.
 VR_DEV int unpart1by1(int n) {
     n &= 0x55555555;
