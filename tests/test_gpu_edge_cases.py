@@ -159,21 +159,29 @@ def test_cell_table_lookups_are_bit_identical(gpu):
         np.testing.assert_array_equal(outs[0], outs[1])
 
 
-@pytest.mark.parametrize("bw_mode", [L.VR_BW_CONSISTENT, L.VR_BW_REFERENCE])
-@pytest.mark.parametrize("majorant", ["grid", "const"])
-def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant):
+@pytest.mark.parametrize("bw_mode,majorant,seed_mode,env", [
+    (L.VR_BW_CONSISTENT, "grid", L.VR_SEED_CAPTURE_FW, "sun"), (L.VR_BW_REFERENCE, "grid", L.VR_SEED_CAPTURE_FW, "sun"),
+    (L.VR_BW_CONSISTENT, "const", L.VR_SEED_CAPTURE_FW, "sun"), (L.VR_BW_REFERENCE, "const", L.VR_SEED_CAPTURE_FW, "sun"),
+    (L.VR_BW_CONSISTENT, "grid", L.VR_SEED_CAPTURE_BW, "sun"), (L.VR_BW_CONSISTENT, "grid", L.VR_SEED_CAPTURE_FW, "image")])
+def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant, seed_mode, env):
     """The NEE capture backward as a flat lane state machine (the default for DT
     shadow rays) scatters the same gradients, sample by sample, as the nested
     estimator (VR_NEE_BW_FLAT=0): same scatter counts, density and albedo
     gradients equal up to the order of the float atomics.  Sensors with a zero
-    output gradient trace nothing."""
+    output gradient trace nothing.  Also with fresh backward seeds (one stream over
+    a sensor's samples) and an environment image sampled through its quadtree."""
     import os
     import torch
+    from helpers import env_density, np_quadtree, sky_image
+    print(f"[case] flat nee bw mode={bw_mode} majorant={majorant} seeds={seed_mode} env={env}", flush=True)
     g = fbm_grid(20, seed=5, scale=30.0)
     alb = (0.5 + 0.45 * fbm_grid(8, seed=6, scale=1.0)).astype(np.float32)
-    sc = Scene(sigma=g, albedo=np.repeat(alb, 3, -1), phase_g=0.4, env_c=(0.3, 0.4, 0.5), sun_dir=(0.3, 0.9, 0.3),
-               sun_radiance=(2.0, 2.0, 2.0), sun_prob=0.4, majorant=majorant, majorant_res=(4, 4, 4), max_bounces=6,
-               transmittance_kind=L.VR_T_DT)
+    light = dict(env_c=(0.3, 0.4, 0.5), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(2.0, 2.0, 2.0), sun_prob=0.4)
+    if env == "image":
+        img = sky_image(16, seed=2)
+        light = dict(env_image=img, env_quadtree=np_quadtree(env_density(img)), env_quadtree_levels=4)
+    sc = Scene(sigma=g, albedo=np.repeat(alb, 3, -1), phase_g=0.4, majorant=majorant, majorant_res=(4, 4, 4),
+               max_bounces=6, transmittance_kind=L.VR_T_DT, **light)
     cam = dict(poses=np.stack([pose((0.2, 0.4, 2.2)), pose((-1.9, 0.1, 1.0))]), width=20, height=20, jittered=True)
     og = (np.random.default_rng(3).uniform(size=(800, 3)) + 0.5).astype(np.float32)
     og[::7] = 0.0
@@ -186,7 +194,7 @@ def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant):
             c = Camera(device="cuda", **cam)
             dbg = torch.zeros(c.threads, dtype=torch.int32, device="cuda")
             o = torch.from_numpy(og).cuda()
-            lau = L.launch((9, 8, 7, 6), samples=8, bw_mode=bw_mode, seed_mode=L.VR_SEED_CAPTURE_FW, debug_out=dbg)
+            lau = L.launch((9, 8, 7, 6), samples=8, bw_mode=bw_mode, seed_mode=seed_mode, debug_out=dbg)
             L.check(L.load().vr_capture_bw(b.medium, L.VR_MAP_NEE_DT, c.c, lau, o.data_ptr(),
                                            torch.cuda.current_stream().cuda_stream), "capture_bw")
             torch.cuda.synchronize()

@@ -47,7 +47,7 @@ def test_random_sensors_follow_the_seed(gpu):
 
 
 @pytest.mark.parametrize("shape", [(4097, 1), (33, 300, 3), (17, 40, 300, 1), (9, 10, 11, 4), (5, 6, 7, 6),
-                                   (16, 24, 64, 1), (3, 9, 4, 1), (2, 2, 1028, 1), (7, 5, 3, 1)])
+                                   (16, 24, 64, 1), (3, 9, 4, 1), (2, 2, 1028, 1), (7, 5, 3, 1), (40, 33, 36, 1)])
 def test_total_variation_bit_exact(gpu, shape):
     """Generic kernels and the single-channel 3-D x4 kernels (rows of whole float4s)."""
     rng = np.random.default_rng(len(shape) + shape[-1])
