@@ -112,7 +112,9 @@ def _inverse_step_timing(camera, field, target, spp, params, steps):
     """Times `steps` inverse-render steps (after one warm-up): capture forward +
     L2 loss, adjoint backward, fused bounded Adam on each (param, opt, bounds).
     An untimed extra step counts the forward's and the backward's trilinear
-    lookups and the backward's scatters from the kernels' own counters."""
+    lookups and the backward's scatters from the kernels' own counters.  Means
+    over the timed steps, with each step's forward and backward times and the
+    host time of the forward / backward calls (launch work, not completion)."""
     import torch
     import rendervous_amd as rv
 
@@ -132,22 +134,31 @@ def _inverse_step_timing(camera, field, target, spp, params, steps):
             p.grad = None
 
     fw_ms = bw_ms = adam_ms = 0.0
+    fw_steps, bw_steps = [], []
+    fw_host = bw_host = 0.0  # host time of the forward / backward calls (launches, not completion)
     scatters = 0.0
     for k in range(steps + 1):  # the first step is a warm-up
         e = _events(4)
         e[0].record(stream)
+        h0 = time.perf_counter()
         img, loss = step(dbg)
+        h1 = time.perf_counter()
         e[1].record(stream)
         loss.backward()
+        h2 = time.perf_counter()
         e[2].record(stream)
         update()
         e[3].record(stream)
         torch.cuda.synchronize()
         if k == 0:
             continue
-        fw_ms += e[0].elapsed_time(e[1])
-        bw_ms += e[1].elapsed_time(e[2])
+        fw_steps.append(e[0].elapsed_time(e[1]))
+        bw_steps.append(e[1].elapsed_time(e[2]))
+        fw_ms += fw_steps[-1]
+        bw_ms += bw_steps[-1]
         adam_ms += e[2].elapsed_time(e[3])
+        fw_host += (h1 - h0) * 1e3
+        bw_host += (h2 - h1) * 1e3
         scatters += float(dbg.to(torch.float64).sum().item())
     fw_ms, bw_ms, adam_ms, scatters = fw_ms / steps, bw_ms / steps, adam_ms / steps, scatters / steps
     # lookup counts (untimed; the gradients of this step are dropped)
@@ -171,6 +182,7 @@ def _inverse_step_timing(camera, field, target, spp, params, steps):
         "grad_voxel_updates_per_s": 8.0 * scatters / (bw_ms * 1e-3),
         "trilinear_scatters_per_step": scatters, "fw_lookups_per_step": fw_lookups, "bw_lookups_per_step": bw_lookups,
         "fw_ms": fw_ms, "bw_ms": bw_ms, "adam_ms": adam_ms,
+        "fw_host_ms": fw_host / steps, "bw_host_ms": bw_host / steps, "fw_ms_steps": fw_steps, "bw_ms_steps": bw_steps,
         "roofline": {"bound": "hbm", "achieved": bytes_step / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bytes_step / t / 1e9 / HBM_PEAK_GBS, "bytes_per_step": bytes_step},
         "loss": float(loss.item()),
@@ -412,7 +424,7 @@ def _nee_scene(sigma, albedo, max_bounces):
     return ds.radiance
 
 
-def nee_gradient_leg(dev, res=512, ares=128, views=4, hw=512, spp=32, steps=2):
+def nee_gradient_leg(dev, res=512, ares=128, views=4, hw=512, spp=32, steps=3):
     """One GPU's share (4 of 32 views) of the multiple-scattering inverse-render
     step: 512^3 density + 128^3 RGB albedo, 512x512, 32 spp, NEE + DT, max 8
     bounces; L2 loss, backward by the defensive replay estimator replaying the
