@@ -646,7 +646,7 @@ int persistent_blocks(K kernel, int* blocks) {
     return VR_OK;
 }
 
-template <int MAJ, int TKIND, bool ENVMAP>
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false>
 int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
                           int64_t cnt, float* out, int* dbg, hipStream_t st) {
     if (total > (int64_t)UINT32_MAX) return set_err(VR_ERR_UNSUPPORTED, "capture has more than 2^32 sensors");
@@ -654,7 +654,7 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
     int rc = work_counter(&counter);
     if (rc) return rc;
     int blocks = 0;
-    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP>, &blocks))) return rc;
+    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN>, &blocks))) return rc;
     const int64_t lanes = (int64_t)blocks * kBlock;
     const int64_t hw = (int64_t)C.width * C.height;
     // launches of at most 2^30 sensors (whole images when possible) keep item indices in 32 bits
@@ -683,7 +683,7 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         unsigned prep = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
         hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, b0, n, dbg, counter);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
-        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, S, counter, out,
+        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, S, counter, out,
                            dbg);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw launch"))) return rc;
     }
@@ -765,6 +765,11 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
     const bool nee = kind == VR_MAP_NEE_DT || kind == VR_MAP_NEE_DRT;
     if (nee && (M.t_kind == VR_T_DT || M.t_kind == VR_T_RT)) {
         const bool envmap = M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM;
+        // cell table + constant albedo (DT shadow rays): the lean specialisation
+        const bool lean = M.cells && !M.albedo_grid && env_int("VR_NEE_LEAN", 1) != 0;
+        if (M.t_kind == VR_T_DT && lean)
+            return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true, true>(M, C, sd, spp, total, start, cnt, out, dbg, st)
+                          : launch_nee_persistent<MAJ, VR_T_DT, false, true>(M, C, sd, spp, total, start, cnt, out, dbg, st);
         if (M.t_kind == VR_T_DT)
             return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true>(M, C, sd, spp, total, start, cnt, out, dbg, st)
                           : launch_nee_persistent<MAJ, VR_T_DT, false>(M, C, sd, spp, total, start, cnt, out, dbg, st);

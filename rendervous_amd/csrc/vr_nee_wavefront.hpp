@@ -33,7 +33,10 @@ enum : int { kEvNone = 0, kEvBegin = 1, kEvStart = 2, kEvCollision = 3, kEvTrack
 
 // ENVMAP: environment image and/or quadtree sampler; false compiles the
 // constant-environment (uniform + directional light) sampler only.
-template <int MAJ, int TKIND, bool ENVMAP>
+// LEAN: the medium has a sigma cell table and a constant albedo, so the kernel
+// compiles neither the strided grid read nor the albedo grid (fewer uniform
+// values held in scalar registers).
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false>
 struct NeeLane {
     static constexpr float EPS = 0.00001f;  // vr_path_integrator_NEE.h:16
 
@@ -159,7 +162,7 @@ struct NeeLane {
 
         if (need_sigma) {
             // the shared trilinear gather
-            float sg = sigma_lookup(M, rx);
+            float sg = LEAN ? cells_lookup1(M.sigma, M.cells, rx) : sigma_lookup(M, rx);
             lookups++;
             if (mode == kModePath) {
                 if (rng_uniform(rng) * m >= sg) rd -= t;  // null collision
@@ -181,7 +184,7 @@ struct NeeLane {
     // albedo, light sample; the shadow ray starts at the collision.
     VR_DEV void on_collision(const MediumView& M) {
         f3 sa;
-        if (!M.albedo_grid) sa = M.albedo_c;
+        if (LEAN || !M.albedo_grid) sa = M.albedo_c;
         else {
             float v[4];
             grid_lookupC(M.albedo, rx, v);
@@ -330,12 +333,12 @@ VR_DEV unsigned long long lane_mask_lt() {
 // part of the image, and moves on to the next queue when that one is empty.
 // The output must be zero on entry (k_nee_prepare); item sums are added
 // atomically.
-template <int MAJ, int TKIND, bool ENVMAP>
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))
 k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsigned long long* __restrict__ counters,
                  float* __restrict__ out, int* __restrict__ dbg) {
     const float inv = S.spp == 1 ? 1.0f : 1.0f / (float)S.spp;
-    NeeLane<MAJ, TKIND, ENVMAP> lane;
+    NeeLane<MAJ, TKIND, ENVMAP, LEAN> lane;
     lane.lookups = 0;
     lane.event = kEvNone;
     bool have = false, done = false;
