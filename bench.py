@@ -15,6 +15,7 @@ baseline (the oracle on a bounded pixel sample of the same frame).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -137,6 +138,9 @@ def _inverse_step_timing(camera, field, target, spp, params, steps):
     fw_steps, bw_steps = [], []
     fw_host = bw_host = 0.0  # host time of the forward / backward calls (launches, not completion)
     scatters = 0.0
+    gc.collect()
+    enabled = gc.isenabled()
+    gc.disable()  # no collector pause inside a timed step (a stalled launch idles the GPU)
     for k in range(steps + 1):  # the first step is a warm-up
         e = _events(4)
         e[0].record(stream)
@@ -160,6 +164,8 @@ def _inverse_step_timing(camera, field, target, spp, params, steps):
         fw_host += (h1 - h0) * 1e3
         bw_host += (h2 - h1) * 1e3
         scatters += float(dbg.to(torch.float64).sum().item())
+    if enabled:
+        gc.enable()
     fw_ms, bw_ms, adam_ms, scatters = fw_ms / steps, bw_ms / steps, adam_ms / steps, scatters / steps
     # lookup counts (untimed; the gradients of this step are dropped)
     with rv.backward_mode(debug="lookups"):
@@ -777,6 +783,19 @@ def scene_leg(dev, res=256, hw=512, spp=16, reps=3):
     }
 
 
+def _quiet(leg, *args):
+    """Runs a leg with Python's cyclic collector paused (collected first), so
+    that no collector pause lands inside its timed regions."""
+    gc.collect()
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        return leg(*args)
+    finally:
+        if enabled:
+            gc.enable()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -810,7 +829,7 @@ def main():
         legs = {"gradient": gradient_leg, "nee_gradient": nee_gradient_leg, "nrc": nrc_leg, "dda": dda_leg,
                 "grid_tools": grid_tools_leg, "scene": scene_leg, "convergence": convergence_leg,
                 "nee_convergence": nee_convergence_leg}
-        print(json.dumps({args.leg: legs[args.leg](dev)}), flush=True)
+        print(json.dumps({args.leg: _quiet(legs[args.leg], dev)}), flush=True)
         return
     if args.calibrate_copy:
         a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
@@ -842,6 +861,8 @@ def main():
         stream = torch.cuda.current_stream(dev)
         starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
         ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        gc.collect()
+        gc.disable()  # no collector pause between the timed launches
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
@@ -850,6 +871,7 @@ def main():
             ends[k].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        gc.enable()
     kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
     ms_per_step = wall * 1000.0 / args.steps
     assert torch.isfinite(img).all()
@@ -878,22 +900,22 @@ def main():
                      "traffic": profiled_traffic(args.res, args.width, args.height, args.spp)},
     }
     if not args.no_secondary:
-        rmse = time_to_rmse_leg(dev, radiance, camera)
+        rmse = _quiet(time_to_rmse_leg, dev, radiance, camera)
         del img, radiance, camera
         torch.cuda.empty_cache()
-        result["secondary"] = {"time_to_rmse": rmse, "gradient": gradient_leg(dev)}
+        result["secondary"] = {"time_to_rmse": rmse, "gradient": _quiet(gradient_leg, dev)}
         torch.cuda.empty_cache()
-        result["secondary"]["nee_gradient"] = nee_gradient_leg(dev)
+        result["secondary"]["nee_gradient"] = _quiet(nee_gradient_leg, dev)
         torch.cuda.empty_cache()
-        result["secondary"]["nrc"] = nrc_leg(dev)
+        result["secondary"]["nrc"] = _quiet(nrc_leg, dev)
         torch.cuda.empty_cache()
-        result["secondary"]["dda"] = dda_leg(dev)
+        result["secondary"]["dda"] = _quiet(dda_leg, dev)
         torch.cuda.empty_cache()
-        result["secondary"]["grid_tools"] = grid_tools_leg(dev)
+        result["secondary"]["grid_tools"] = _quiet(grid_tools_leg, dev)
         torch.cuda.empty_cache()
-        result["secondary"]["scene"] = scene_leg(dev)
+        result["secondary"]["scene"] = _quiet(scene_leg, dev)
         torch.cuda.empty_cache()
-        result["secondary"]["convergence"] = convergence_leg(dev)
+        result["secondary"]["convergence"] = _quiet(convergence_leg, dev)
         log(f"secondary: {json.dumps(result['secondary'])}")
     if not args.no_cpu_baseline:
         m = build_scene(args.res, args.width, args.height, dev)[1]
