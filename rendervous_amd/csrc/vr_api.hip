@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
 #ifndef VR_NEE_BW_WAVES_PER_EU
 #define VR_NEE_BW_WAVES_PER_EU 4
 #endif
-template <int MAJ>
+template <int MAJ, bool ENVMAP>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VR_NEE_BW_WAVES_PER_EU)))
 k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int seed_mode, int64_t total,
                       int64_t start, int64_t cnt, const float* __restrict__ og, int* __restrict__ dbg) {
@@ -529,7 +529,7 @@ k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P,
         if (dbg && P.chunks == 1) dbg[i] = 0;
         return;
     }
-    NeeDsBwLane<MAJ> L(M);
+    NeeDsBwLane<MAJ, ENVMAP> L(M);
     L.eps = M.ds_eps;
     L.dL_dR = g;
     // the reference's fresh-seed backward continues one stream over the samples (one chunk)
@@ -805,7 +805,13 @@ int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Se
     if (rc) return rc;
     dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
     if (kind == VR_MAP_NEE_DT && M.t_kind == VR_T_DT && env_int("VR_NEE_BW_FLAT", 1) != 0) {
-        hipLaunchKernelGGL((k_capture_bw_nee_flat<MAJ>), g, b, 0, st, M, C, sd, P, seed_mode, total, start, cnt, og, dbg);
+        // constant environments compile the uniform + sun sampler only (as the forward)
+        if (M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM)
+            hipLaunchKernelGGL((k_capture_bw_nee_flat<MAJ, true>), g, b, 0, st, M, C, sd, P, seed_mode, total, start, cnt, og,
+                               dbg);
+        else
+            hipLaunchKernelGGL((k_capture_bw_nee_flat<MAJ, false>), g, b, 0, st, M, C, sd, P, seed_mode, total, start, cnt, og,
+                               dbg);
         return check_hip(hipGetLastError(), "vr_capture_bw launch");
     }
     switch (kind) {

@@ -33,7 +33,10 @@ enum : int {
     kBwCollision = 9      // the path ray has a real collision (s, m, Pc saved)
 };
 
-template <int MAJ>
+// ENVMAP: environment image and/or quadtree sampler; false compiles the
+// constant-environment (uniform + directional light) sampler only, which draws
+// the same random numbers for a constant environment.
+template <int MAJ, bool ENVMAP = true>
 struct NeeDsBwLane {
     static constexpr float EPS = 0.00000001f;  // vr_path_integrator_NEE_DS.h:8
 
@@ -202,7 +205,7 @@ struct NeeDsBwLane {
         added = (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
         W = W * sa;
         f3 wnee;
-        env = env_sample(M, tr.rng, wnee);
+        env = ENVMAP ? env_sample(M, tr.rng, wnee) : env_sample_const(M, tr.rng, wnee);
         px = rx;
         pw = rw;
         rw = wnee;
@@ -238,7 +241,7 @@ struct NeeDsBwLane {
         tr.albedo_bw(rx, dL_dsa);
         W = W * sa;
         f3 wnee;
-        env = env_sample(M, tr.rng, wnee);
+        env = ENVMAP ? env_sample(M, tr.rng, wnee) : env_sample_const(M, tr.rng, wnee);
         before_T = tr.rng;
         px = rx;
         pw = rw;
@@ -269,7 +272,7 @@ struct NeeDsBwLane {
 
     VR_DEV void path_end() {
         if (pass == 1) {
-            if (!scattered) R = R + env_eval(tr.M, rw) * W;
+            if (!scattered) R = R + (ENVMAP ? env_eval(tr.M, rw) : tr.M.env_c) * W;
             begin_pass(2);
         } else state = kBwDone;
     }
