@@ -65,9 +65,11 @@ class Settings:
     """cell_tables: give a medium a cell table of its density (8 corners per
     cell, 8x the grid's memory) so that each trilinear lookup reads one 32-byte
     cell; the values are the same bit for bit as reading the grid.  "auto"
-    builds one for densities that do not require a gradient (a density that an
-    optimiser updates every step would rebuild its table every step); True /
-    False force it on / off."""
+    builds one for densities that do not require a gradient, and for densities
+    that do when the integrator scatters many times per sample (NEE, DT path,
+    DRT): a density an optimiser updates rebuilds its table at every update,
+    which a single-scatter step does not win back; True / False force it on /
+    off."""
     cell_tables: typing.Union[str, bool] = "auto"
 
 

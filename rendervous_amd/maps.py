@@ -433,7 +433,10 @@ class _Lowered:
         self.majorant_set = False
         self.device: typing.Optional[torch.device] = None
 
-    def set_sigma(self, sigma: MapBase):
+    def set_sigma(self, sigma: MapBase, multiple_scattering: bool = False):
+        """multiple_scattering: the integrator traces paths of many tracking steps
+        per sample (NEE / DT path / DRT), where a cell table pays for its rebuild
+        after every update of a density that requires a gradient."""
         if not isinstance(sigma, Grid3D) or sigma.output_dim != 1:
             raise NotImplementedError("sigma must be a single-channel Grid3D")
         if self.sigma is not None and self.sigma is not sigma:
@@ -444,7 +447,8 @@ class _Lowered:
         ct = _internal.settings.cell_tables
         g = sigma.grid
         n_cells = (g.shape[0] - 1) * (g.shape[1] - 1) * (g.shape[2] - 1)
-        if g.is_cuda and n_cells < (1 << 30) and (ct is True or (ct == "auto" and not g.requires_grad)):
+        auto = not g.requires_grad or multiple_scattering
+        if g.is_cuda and n_cells < (1 << 30) and (ct is True or (ct == "auto" and auto)):
             table = sigma.cells_tensor()
             self.keep.append(table)
             self.medium.sigma_cells = table.data_ptr()
@@ -812,7 +816,7 @@ class DeltatrackingNEEPathIntegrator(_RayMap):
 
     def _lower_map(self):
         lw = _Lowered()
-        lw.set_sigma(self.sigma)
+        lw.set_sigma(self.sigma, multiple_scattering=True)
         lw.set_boundary(self.boundary)
         lw.set_majorant(self.majorant)
         lw.set_albedo(self.scattering_albedo)
@@ -850,7 +854,7 @@ class DeltatrackingPathIntegrator(_RayMap):
 
     def _lower_map(self):
         lw = _Lowered()
-        lw.set_sigma(self.sigma)
+        lw.set_sigma(self.sigma, multiple_scattering=True)
         lw.set_boundary(self.boundary)
         lw.set_majorant(self.majorant)
         lw.set_albedo(self.scattering_albedo)
@@ -887,7 +891,7 @@ class DRTPathIntegrator(_RayMap):
 
     def _lower_map(self):
         lw = _Lowered()
-        lw.set_sigma(self.sigma)
+        lw.set_sigma(self.sigma, multiple_scattering=True)
         lw.set_boundary(self.boundary)
         lw.set_majorant(self.majorant)
         lw.set_albedo(self.scattering_albedo)
