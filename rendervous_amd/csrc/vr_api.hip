@@ -927,9 +927,18 @@ __global__ void __launch_bounds__(kSceneBlock) k_scene_map_bw(SceneView S, const
     scene_map_bw<kSceneBlock>(S, rng, load3(rays + 6 * i), load3(rays + 6 * i + 3), g, stk + threadIdx.x, lookups, scatters);
 }
 
+// Resident waves per SIMD the scene capture kernels are compiled for: 4 cap
+// them at 128 VGPRs (137-141 uncapped, 3 waves) and 8 workgroups' traversal
+// stacks use 128 KB of LDS; forward 25.6 -> 22.7 ms, backward 109 -> 99 ms on
+// the 256^3 cloud scene at 512x512, 16 spp, same images.
+#ifndef VR_SCENE_WAVES_PER_EU
+#define VR_SCENE_WAVES_PER_EU 4
+#endif
+#define VR_SCENE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(VR_SCENE_WAVES_PER_EU)))
+
 // Capture of a PathtracedScene: thread j takes sensor start + j % cnt and samples
 // [c*chunk, ...) with c = j / cnt, as k_capture_fw.
-__global__ void __launch_bounds__(kSceneBlock) k_scene_capture_fw(SceneView S, CameraView cam, Seeds seeds, CaptureSplit P,
+__global__ void __launch_bounds__(kSceneBlock) VR_SCENE_OCCUPANCY k_scene_capture_fw(SceneView S, CameraView cam, Seeds seeds, CaptureSplit P,
                                                                   int64_t total, int64_t start, int64_t cnt,
                                                                   float* __restrict__ out, int* __restrict__ dbg) {
     __shared__ int stk[kSceneStack * kSceneBlock];
@@ -964,7 +973,7 @@ __global__ void __launch_bounds__(kSceneBlock) k_scene_capture_fw(SceneView S, C
     }
 }
 
-__global__ void __launch_bounds__(kSceneBlock) k_scene_capture_bw(SceneView S, CameraView cam, Seeds seeds, CaptureSplit P,
+__global__ void __launch_bounds__(kSceneBlock) VR_SCENE_OCCUPANCY k_scene_capture_bw(SceneView S, CameraView cam, Seeds seeds, CaptureSplit P,
                                                                   int seed_mode, int64_t total, int64_t start, int64_t cnt,
                                                                   const float* __restrict__ og, int* __restrict__ dbg) {
     __shared__ int stk[kSceneStack * kSceneBlock];
