@@ -516,12 +516,22 @@ VR_DEV unsigned long long lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Resident waves per SIMD the render kernel and the training-path kernel
+// (nested estimators, one thread per sample or path) are compiled for: 5 for the
+// render (11.8 ms against 12.7 at 4 and 14.6 at 3 for 1024^2 x 16 spp), 3 for the
+// few, long training paths (training 17.75 -> 17.3 ms).
+#ifndef VR_NRC_WAVES_PER_EU
+#define VR_NRC_WAVES_PER_EU 5
+#endif
+#ifndef VR_NRC_PATHS_WAVES_PER_EU
+#define VR_NRC_PATHS_WAVES_PER_EU 3
+#endif
+
 // Training paths: path j starts at a uniformly drawn sensor; each recorded
 // vertex becomes a record (x, w, (R_end - R_v) / W_v) appended at *count.
-// Few, long paths: one wave per block so that every CU gets work, and the
-// registers capped at 5 waves per SIMD as for the other nested estimators.
+// Few, long paths: one wave per block so that every CU gets work.
 template <int MAJ>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_nrc_paths(MediumView M, CameraView cam, Seeds seeds, int64_t total, int n_paths,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VR_NRC_PATHS_WAVES_PER_EU))) k_nrc_paths(MediumView M, CameraView cam, Seeds seeds, int64_t total, int n_paths,
                                                    int maxv, float* __restrict__ tmp, float* __restrict__ records,
                                                    int64_t max_records, uint32_t* __restrict__ count) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -551,7 +561,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
 // as in k_capture_fw; the partial radiance goes to `out` (rows zeroed first when
 // chunks > 1) and each terminated sample appends a query (x, w, W/spp, pixel).
 template <int MAJ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_nrc_render(MediumView M, CameraView cam, Seeds seeds, int spp, int chunk, int chunks,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NRC_WAVES_PER_EU))) k_nrc_render(MediumView M, CameraView cam, Seeds seeds, int spp, int chunk, int chunks,
                                                     int K, int64_t total, int64_t start, int64_t cnt,
                                                     float* __restrict__ out, float* __restrict__ queries,
                                                     int64_t max_queries, uint32_t* __restrict__ count) {
