@@ -927,12 +927,13 @@ __global__ void __launch_bounds__(kSceneBlock) k_scene_map_bw(SceneView S, const
     scene_map_bw<kSceneBlock>(S, rng, load3(rays + 6 * i), load3(rays + 6 * i + 3), g, stk + threadIdx.x, lookups, scatters);
 }
 
-// Resident waves per SIMD the scene capture kernels are compiled for: 4 cap
-// them at 128 VGPRs (137-141 uncapped, 3 waves) and 8 workgroups' traversal
-// stacks use 128 KB of LDS; forward 25.6 -> 22.7 ms, backward 109 -> 99 ms on
-// the 256^3 cloud scene at 512x512, 16 spp, same images.
+// Resident waves per SIMD the scene capture kernels are compiled for: 5 cap
+// them at 96 VGPRs (137-141 uncapped, 3 waves) and 10 workgroups' traversal
+// stacks fill the CU's 160 KB of LDS; on the 256^3 cloud scene at 512x512,
+// 16 spp: forward 25.6 ms uncapped -> 22.0 ms (4 or 5 waves), backward 109 ->
+// 98.6 (4) -> 96.5 ms (5), same images.
 #ifndef VR_SCENE_WAVES_PER_EU
-#define VR_SCENE_WAVES_PER_EU 4
+#define VR_SCENE_WAVES_PER_EU 5
 #endif
 #define VR_SCENE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(VR_SCENE_WAVES_PER_EU)))
 
