@@ -517,11 +517,11 @@ VR_DEV unsigned long long lanes_below() {
 }
 
 // Resident waves per SIMD the render kernel and the training-path kernel
-// (nested estimators, one thread per sample or path) are compiled for: 5 for the
-// render (11.8 ms against 12.7 at 4 and 14.6 at 3 for 1024^2 x 16 spp), 3 for the
-// few, long training paths (training 17.75 -> 17.3 ms).
+// (nested estimators, one thread per sample or path) are compiled for: 6 for the
+// render (11.3 ms for 1024^2 x 16 spp against 12.3 at 8, 11.75 at 5, 12.7 at 4
+// and 14.6 at 3), 3 for the few, long training paths (training 17.75 -> 17.3 ms).
 #ifndef VR_NRC_WAVES_PER_EU
-#define VR_NRC_WAVES_PER_EU 5
+#define VR_NRC_WAVES_PER_EU 6
 #endif
 #ifndef VR_NRC_PATHS_WAVES_PER_EU
 #define VR_NRC_PATHS_WAVES_PER_EU 3
