@@ -68,6 +68,11 @@ def load():
             "oracle_copy_img_to_morton": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
             "oracle_copy_morton_to_img": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
             "oracle_env_sample": (ctypes.c_int, [P(L.VrMedium), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+            "oracle_phase_sample": (ctypes.c_int, [P(L.VrMedium), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                   ctypes.c_void_p]),
+            "oracle_phase_eval": (ctypes.c_int, [P(L.VrMedium), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.c_void_p]),
+            "oracle_camera_rays": (ctypes.c_int, [P(L.VrCamera), ctypes.c_void_p, ctypes.c_void_p]),
             "oracle_grid_pack_cells": (ctypes.c_int, [P(L.VrGrid), ctypes.c_void_p]),
             "oracle_scene_build_bvh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
                                                       ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,

@@ -17,7 +17,16 @@
  * pointers (no stream argument).
  *
  * Parity pins (what checks each function; see tests/):
- *   rng_*                golden LCG sequence, Tausworthe step structure, uniformity
+ *   rng_*                golden LCG sequence, Tausworthe step structure, maximal
+ *                        period of each Tausworthe component (primitive transition
+ *                        over GF(2)), uniformity; the seed formulas stay unpinned
+ *   hg_eval / hg_sample  normalisation and mean cosine g by quadrature, KS test
+ *                        of sampled cosines against the analytic CDF, uniform
+ *                        azimuth, sampler histogram = evaluated phase
+ *   camera_ray           numpy restatement of the reference's torch camera
+ *                        (_maps.py:1788-1801), edge-pixel angles from the FOV
+ *   env_sample (sun)     single scattering in a homogeneous box lit by the sun
+ *                        alone = 1-D quadrature, two selection probabilities
  *   ray_box / boundary   slab-test special cases, axis-parallel rays
  *   grid_lookup / _bw    closed form (affine fields are reproduced exactly),
  *                        torch grid_sample(align_corners=True) in the interior,
@@ -2242,6 +2251,59 @@ int oracle_env_sample(const vr_medium* M, const uint32_t seeds[4], int64_t n, fl
         v3 L = env_sample(&c, &wo);
         out[6 * i + 0] = (float)wo.x; out[6 * i + 1] = (float)wo.y; out[6 * i + 2] = (float)wo.z;
         out[6 * i + 3] = (float)L.x; out[6 * i + 4] = (float)L.y; out[6 * i + 5] = (float)L.z;
+    }
+    return VR_OK;
+}
+
+/* Oracle-only probe of the phase sampler and the phase function (no device
+ * twin): sample i draws from the map-eval stream of thread i, samples wo for the
+ * propagation direction w[i] and returns (wo, rho, phase_eval(w, wo)). */
+int oracle_phase_sample(const vr_medium* M, const float* w, const uint32_t seeds[4], int64_t n, float* out) {
+    int st = check_medium(M, VR_MAP_RAYBOX);
+    if (st) return st;
+    if (!w || !seeds || !out || n < 0) return fail(VR_ERR_ARG, "bad phase_sample arguments");
+    for (int64_t i = 0; i < n; i++) {
+        ctx_t c;
+        memset(&c, 0, sizeof(c));
+        c.M = M;
+        c.rng = seed_map(seeds, (uint32_t)i);
+        v3 wi = vf(w + 3 * i), wo;
+        double rho = phase_sample(&c, wi, &wo);
+        out[5 * i + 0] = (float)wo.x; out[5 * i + 1] = (float)wo.y; out[5 * i + 2] = (float)wo.z;
+        out[5 * i + 3] = (float)rho;
+        out[5 * i + 4] = (float)phase_eval(M, wi, wo);
+    }
+    return VR_OK;
+}
+
+/* Oracle-only probe: phase(x, w, wo) for n direction pairs, in double. */
+int oracle_phase_eval(const vr_medium* M, const double* w, const double* wo, int64_t n, double* out) {
+    int st = check_medium(M, VR_MAP_RAYBOX);
+    if (st) return st;
+    if (!w || !wo || !out || n < 0) return fail(VR_ERR_ARG, "bad phase_eval arguments");
+    for (int64_t i = 0; i < n; i++)
+        out[i] = phase_eval(M, V(w[3 * i], w[3 * i + 1], w[3 * i + 2]), V(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]));
+    return VR_OK;
+}
+
+/* Oracle-only probe of the camera: (x, w) of sample 0 of every sensor, drawn
+ * (when jittered) from that sample's capture-forward stream, as the capture
+ * drivers do. */
+int oracle_camera_rays(const vr_camera* cam, const uint32_t seeds[4], double* out) {
+    int st = check_camera(cam);
+    if (st) return st;
+    if (!seeds || !out) return fail(VR_ERR_ARG, "bad camera_rays arguments");
+    const int64_t n = camera_threads(cam);
+    for (int64_t i = 0; i < n; i++) {
+        ctx_t c;
+        memset(&c, 0, sizeof(c));
+        c.rng = seed_capture_fw(seeds, (uint32_t)i);
+        int pose, row, col;
+        sensor_index(cam, i, &pose, &row, &col);
+        v3 x, w;
+        camera_ray(cam, pose, row, col, &c, &x, &w);
+        out[6 * i + 0] = x.x; out[6 * i + 1] = x.y; out[6 * i + 2] = x.z;
+        out[6 * i + 3] = w.x; out[6 * i + 4] = w.y; out[6 * i + 5] = w.z;
     }
     return VR_OK;
 }

@@ -2,8 +2,10 @@
 
 rng.json: HybridTaus streams from fixed states and the per-thread initial
 states of the three seeding formulas (capture fw, capture bw, map eval) for a
-few thread indices.  The GPU tests check that the HIP kernels draw these exact
-streams; the CPU tests check that the oracle keeps producing them.
+few thread indices.  The CPU tests check that the oracle keeps producing them
+(tests/test_oracle.py); the GPU parity tests compare the HIP kernels' streams
+with the oracle's, not with this file (tests/test_gpu_parity.py exposes the
+first uniform of every thread through a homogeneous collision capture).
 
 Run: python tests/golden/make_golden.py
 """
