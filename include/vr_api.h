@@ -6,8 +6,14 @@
  * Every entry point is extern "C", takes plain pointers and sizes, and returns
  * an int status (VR_OK == 0).  The caller owns every buffer; the library never
  * allocates or frees caller memory.  All device work is ordered on the caller's
- * HIP stream (`stream`, a hipStream_t; NULL = the default stream) and no entry
- * point synchronises the device.
+ * HIP stream (`stream`, a hipStream_t; NULL = the default stream).  No entry
+ * point synchronises the device; vr_majorant_build alone waits for its stream,
+ * to report a non-finite density.
+ *
+ * Limits: a capture draws the random stream of sample s of sensor i from the
+ * 32-bit flat index s * sensors + i, so spp * sensors must not exceed 2^32
+ * (VR_ERR_UNSUPPORTED otherwise: the streams would repeat).  Constant majorant
+ * values must be finite and >= 0, majorant distances > 0.
  *
  * Reference interfaces replaced (rendervous, files relative to its root):
  *   - the vulky-dispatched generated GLSL kernels of DispatcherEngine
@@ -28,7 +34,7 @@
 extern "C" {
 #endif
 
-#define VR_ABI_VERSION 1
+#define VR_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum vr_status {
@@ -194,7 +200,17 @@ typedef struct vr_launch {
     int64_t  count;                /* threads in this batch; < 0 = all remaining */
     int32_t* debug_out;            /* optional per-thread counter (the reference's DEBUG_COUNTER
                                       slot, _maps.py:368,611-615), see debug_kind */
+    void*    workspace;            /* optional device scratch for vr_capture_fw (the work-queue
+                                      counters of the persistent NEE kernel): at least
+                                      VR_CAPTURE_WORKSPACE_BYTES, 256-byte aligned, used in stream
+                                      order (so captures on different streams never share it).
+                                      NULL = the library allocates it for the call on `stream`
+                                      (hipMallocAsync / hipFreeAsync). */
+    int64_t  workspace_bytes;
 } vr_launch;
+
+/* Size of vr_launch.workspace (8 queue counters, one 128-byte line each). */
+#define VR_CAPTURE_WORKSPACE_BYTES 1024
 
 /* ---- scene-level path tracing -------------------------------------------------
  * PathtracedScene (_scenes.py:2251-2312, include/maps/pathtraced_scene.h,
@@ -288,7 +304,9 @@ int vr_grid_bw(const vr_grid* grid, const float* x, const float* out_grad, int64
 /* Coarse majorant grid: out[rz,ry,rx] = max of sigma over the vertices whose
  * trilinear footprint touches each coarse cell.  Implements the 6->2 majorant
  * contract (include/vr_majorant.h:1-7) that the reference fills with
- * const[max(sigma), 1e6] (_objects.py:96-101). */
+ * const[max(sigma), 1e6] (_objects.py:96-101).  Waits for `stream` and returns
+ * VR_ERR_ARG if sigma holds a NaN or an infinity (an infinite majorant would
+ * give zero-length free flights that never end a ray). */
 int vr_majorant_build(const vr_grid* sigma, const int32_t res[3], float* out, void* stream);
 
 /* Map forward on rays: in[n,IN] -> out[n,OUT] for the map `kind`

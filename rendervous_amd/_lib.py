@@ -142,7 +142,15 @@ class VrLaunch(ctypes.Structure):
         ("start_index", ctypes.c_int64),
         ("count", ctypes.c_int64),
         ("debug_out", ctypes.c_void_p),
+        ("workspace", ctypes.c_void_p),
+        ("workspace_bytes", ctypes.c_int64),
     ]
+
+
+# VR_ABI_VERSION of include/vr_api.h
+ABI_VERSION = 2
+# vr_launch.workspace of a capture: the persistent NEE kernel's queue counters
+CAPTURE_WORKSPACE_BYTES = 1024
 
 
 class VrBvhNode(ctypes.Structure):
@@ -285,7 +293,7 @@ def load() -> ctypes.CDLL:
                     "(there is no CPU fallback for the HIP path)")
             lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
             bind(lib)
-            if lib.vr_abi_version() != 1:
+            if lib.vr_abi_version() != ABI_VERSION:
                 raise ImportError("librendervous_amd.so ABI version mismatch")
             _lib = lib
         return _lib
@@ -339,7 +347,7 @@ def grid_view(t: torch.Tensor, bmin, bmax, grad: torch.Tensor | None = None) -> 
 
 def launch(seeds, samples: int = 1, bw_mode: int = VR_BW_REFERENCE, seed_mode: int = VR_SEED_CAPTURE_FW,
            start_index: int = 0, count: int = -1, debug_out: torch.Tensor | None = None,
-           debug_kind: int = 0) -> VrLaunch:
+           debug_kind: int = 0, workspace: torch.Tensor | None = None) -> VrLaunch:
     L = VrLaunch()
     for i in range(4):
         L.seeds[i] = int(seeds[i]) & 0xFFFFFFFF
@@ -350,4 +358,15 @@ def launch(seeds, samples: int = 1, bw_mode: int = VR_BW_REFERENCE, seed_mode: i
     L.count = int(count)
     L.debug_out = None if debug_out is None else debug_out.data_ptr()
     L.debug_kind = int(debug_kind)
+    if workspace is not None:
+        assert workspace.is_contiguous() and workspace.numel() * workspace.element_size() >= CAPTURE_WORKSPACE_BYTES
+        L.workspace = workspace.data_ptr()
+        L.workspace_bytes = workspace.numel() * workspace.element_size()
     return L
+
+
+def capture_workspace(device) -> torch.Tensor:
+    """A capture's workspace from torch's stream-ordered allocator (freed when
+    the caller drops it; later work on the same stream reuses it only after
+    the capture)."""
+    return torch.empty(CAPTURE_WORKSPACE_BYTES // 8, dtype=torch.int64, device=device)

@@ -156,6 +156,19 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
     v.g = m->phase_g;
     v.maj_value = m->majorant_value;
     v.maj_distance = m->majorant_distance;
+    // Trackers advance by free flights -log(1-u)/majorant within the majorant's
+    // distance: an infinite or NaN majorant, or a distance <= 0, would give steps
+    // of length 0 and a tracking loop that never ends.  (The DDA-based maps do
+    // not read the majorant; the 7->12 medium integrator reads medium_majorant.)
+    const bool reads_majorant = kind != VR_MAP_RAYBOX && kind != VR_MAP_T_DDA && kind != VR_MAP_T_GDT &&
+                                kind != VR_MAP_T_GRT && kind != VR_MAP_MEDIUM_DT;
+    if (reads_majorant && m->majorant_kind == VR_MAJ_CONST) {
+        if (!(m->majorant_value >= 0.0f && m->majorant_value <= 3.402823466e38f))
+            return set_err(VR_ERR_ARG, "majorant value must be finite and >= 0");
+        if (!(m->majorant_distance > 0.0f)) return set_err(VR_ERR_ARG, "majorant distance must be > 0");
+    }
+    if (kind == VR_MAP_MEDIUM_DT && !(m->medium_majorant >= 0.0f && m->medium_majorant <= 3.402823466e38f))
+        return set_err(VR_ERR_ARG, "medium majorant must be finite and >= 0");
     if (m->majorant_kind == VR_MAJ_GRID) {
         if (!m->majorant_grid) return set_err(VR_ERR_ARG, "majorant grid is NULL");
         for (int a = 0; a < 3; a++)
@@ -311,8 +324,10 @@ __global__ void __launch_bounds__(kBlock) k_grid_bw(GridView g, const float* __r
 }
 
 // One workgroup per coarse cell: max over the vertices whose trilinear
-// footprint can reach the cell (widened by one vertex per side).
-__global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry, int rx, float* __restrict__ out) {
+// footprint can reach the cell (widened by one vertex per side).  A NaN or
+// infinite vertex sets *bad.
+__global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry, int rx, float* __restrict__ out,
+                                                     int* __restrict__ bad) {
     int cell = blockIdx.x;
     int cx = cell % rx, cy = (cell / rx) % ry, cz = cell / (rx * ry);
     int n[3] = {g.nx - 1, g.ny - 1, g.nz - 1}, r[3] = {rx, ry, rz}, c[3] = {cx, cy, cz}, lo[3], hi[3];
@@ -323,13 +338,16 @@ __global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry,
     int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
     int total = ex * ey * ez;
     float m = 0.0f;
+    int nonfinite = 0;
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
         int xx = lo[0] + k % ex, yy = lo[1] + (k / ex) % ey, zz = lo[2] + k / (ex * ey);
-        m = fmaxf(m, g.data[zz * g.sz + yy * g.sy + xx * g.sx]);
+        const float v = g.data[zz * g.sz + yy * g.sy + xx * g.sx];
+        nonfinite |= !(fabsf(v) <= 3.402823466e38f);
+        m = fmaxf(m, v);
     }
     __shared__ float red[kBlock];
     red[threadIdx.x] = m;
-    __syncthreads();
+    if (__syncthreads_or(nonfinite) && threadIdx.x == 0) atomicOr(bad, 1);
     for (int s = kBlock / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
@@ -573,6 +591,13 @@ __global__ void __launch_bounds__(kBlock) k_pack_cells(GridView g, float4* __res
     out[2 * c + 1] = make_float4(p[g.sz], p[g.sz + g.sx], p[g.sz + g.sy], p[g.sz + g.sy + g.sx]);
 }
 
+// Zeroes p[0, n) (a kernel rather than hipMemsetAsync, so that callers may
+// capture the work in a graph).
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 // Zeroes rows [start, start+cnt) of a capture output (OD floats each) and of the debug counters.
 __global__ void k_capture_prepare(float* __restrict__ out, int od, int64_t start, int64_t cnt, int* __restrict__ dbg) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -604,29 +629,42 @@ __global__ void __launch_bounds__(kBlock) k_adam(float* __restrict__ p, float* _
 
 // --------------------------------------------------- persistent NEE queue --
 // The persistent NEE kernel's queue counters (kMaxQueues 64-bit words, one per
-// 128-byte line), one set per device, allocated on first use and kept for the
-// life of the process.  Launches on one device are ordered by their streams;
-// two captures running concurrently on different streams of one device would
-// share the counters, so the map layer serialises captures.
-constexpr int kMaxDevices = 64;
-std::mutex g_counter_mu;
-unsigned long long* g_counters[kMaxDevices] = {};
+// 128-byte line) live in the launch's workspace: the caller's buffer
+// (vr_launch.workspace, used in stream order) or one allocated on the stream
+// for the call, so captures on different streams never share counters.
+constexpr int64_t kCounterBytes = (int64_t)kMaxQueues * kQueueStride * sizeof(unsigned long long);
+static_assert(kCounterBytes == VR_CAPTURE_WORKSPACE_BYTES, "workspace size of the C ABI");
 
-int work_counter(unsigned long long** out) {
-    int dev = 0;
-    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
-    if (st) return st;
-    if (dev < 0 || dev >= kMaxDevices) return set_err(VR_ERR_DEVICE, "device index out of range");
-    std::lock_guard<std::mutex> lock(g_counter_mu);
-    if (!g_counters[dev]) {
+// Work counters of one capture: the caller's workspace, else a stream-ordered
+// allocation released (stream-ordered) by release().
+struct Counters {
+    unsigned long long* ptr = nullptr;
+    hipStream_t owned_on = nullptr;
+    bool owned = false;
+
+    int acquire(const vr_launch* L, hipStream_t st) {
+        if (L->workspace) {
+            if (L->workspace_bytes < kCounterBytes)
+                return set_err(VR_ERR_ARG, "vr_capture_fw: workspace smaller than VR_CAPTURE_WORKSPACE_BYTES");
+            if (reinterpret_cast<uintptr_t>(L->workspace) % 256 != 0)
+                return set_err(VR_ERR_ARG, "vr_capture_fw: workspace must be 256-byte aligned");
+            ptr = static_cast<unsigned long long*>(L->workspace);
+            return VR_OK;
+        }
         void* p = nullptr;
-        if ((st = check_hip(hipMalloc(&p, kMaxQueues * kQueueStride * sizeof(unsigned long long)), "hipMalloc(work counters)")))
-            return st;
-        g_counters[dev] = (unsigned long long*)p;
+        int rc = check_hip(hipMallocAsync(&p, (size_t)kCounterBytes, st), "hipMallocAsync(work counters)");
+        if (rc) return rc;
+        ptr = static_cast<unsigned long long*>(p);
+        owned = true;
+        owned_on = st;
+        return VR_OK;
     }
-    *out = g_counters[dev];
-    return VR_OK;
-}
+    int release() {
+        if (!owned) return VR_OK;
+        owned = false;
+        return check_hip(hipFreeAsync(ptr, owned_on), "hipFreeAsync(work counters)");
+    }
+};
 
 // Tuning knob read from the environment (unset: the built-in default).
 int env_int(const char* name, int fallback) {
@@ -648,11 +686,10 @@ int persistent_blocks(K kernel, int* blocks) {
 
 template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false>
 int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
-                          int64_t cnt, float* out, int* dbg, hipStream_t st) {
+                          int64_t cnt, float* out, int* dbg, unsigned long long* counter, hipStream_t st) {
     if (total > (int64_t)UINT32_MAX) return set_err(VR_ERR_UNSUPPORTED, "capture has more than 2^32 sensors");
-    unsigned long long* counter = nullptr;
-    int rc = work_counter(&counter);
-    if (rc) return rc;
+    if (!counter) return set_err(VR_ERR_ARG, "persistent NEE capture without work counters");
+    int rc = VR_OK;
     int blocks = 0;
     if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN>, &blocks))) return rc;
     const int64_t lanes = (int64_t)blocks * kBlock;
@@ -758,23 +795,29 @@ int capture_prepare(const CaptureSplit& P, float* out, int od, int64_t start, in
     return check_hip(hipGetLastError(), "capture prepare");
 }
 
+// The NEE forward (also DRTPathIntegrator's) with a DT or RT shadow estimator
+// runs as the persistent lane state machine, which needs work counters.
+bool capture_is_persistent(int kind, const MediumView& M) {
+    return (kind == VR_MAP_NEE_DT || kind == VR_MAP_NEE_DRT) && (M.t_kind == VR_T_DT || M.t_kind == VR_T_RT);
+}
+
 template <int MAJ>
 int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
-                          int64_t cnt, float* out, int* dbg, hipStream_t st) {
-    // the NEE forward (also DRTPathIntegrator's) with a DT or RT shadow estimator runs persistent
-    const bool nee = kind == VR_MAP_NEE_DT || kind == VR_MAP_NEE_DRT;
-    if (nee && (M.t_kind == VR_T_DT || M.t_kind == VR_T_RT)) {
+                          int64_t cnt, float* out, int* dbg, unsigned long long* counter, hipStream_t st) {
+    if (capture_is_persistent(kind, M)) {
         const bool envmap = M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM;
         // cell table + constant albedo (DT shadow rays): the lean specialisation
         const bool lean = M.cells && !M.albedo_grid && env_int("VR_NEE_LEAN", 1) != 0;
+#define VR_NEE_ARGS M, C, sd, spp, total, start, cnt, out, dbg, counter, st
         if (M.t_kind == VR_T_DT && lean)
-            return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true, true>(M, C, sd, spp, total, start, cnt, out, dbg, st)
-                          : launch_nee_persistent<MAJ, VR_T_DT, false, true>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+            return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true, true>(VR_NEE_ARGS)
+                          : launch_nee_persistent<MAJ, VR_T_DT, false, true>(VR_NEE_ARGS);
         if (M.t_kind == VR_T_DT)
-            return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true>(M, C, sd, spp, total, start, cnt, out, dbg, st)
-                          : launch_nee_persistent<MAJ, VR_T_DT, false>(M, C, sd, spp, total, start, cnt, out, dbg, st);
-        return envmap ? launch_nee_persistent<MAJ, VR_T_RT, true>(M, C, sd, spp, total, start, cnt, out, dbg, st)
-                      : launch_nee_persistent<MAJ, VR_T_RT, false>(M, C, sd, spp, total, start, cnt, out, dbg, st);
+            return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true>(VR_NEE_ARGS)
+                          : launch_nee_persistent<MAJ, VR_T_DT, false>(VR_NEE_ARGS);
+        return envmap ? launch_nee_persistent<MAJ, VR_T_RT, true>(VR_NEE_ARGS)
+                      : launch_nee_persistent<MAJ, VR_T_RT, false>(VR_NEE_ARGS);
+#undef VR_NEE_ARGS
     }
     int in_dim = 0, od = 0;
     map_dims(kind, in_dim, od);
@@ -1101,6 +1144,29 @@ int resolve_range(const vr_launch* L, int64_t total, int64_t& start, int64_t& cn
     return VR_OK;
 }
 
+// Sample s of sensor i draws the stream seeded from the 32-bit flat index
+// s * total + i (the fresh-seed backward: one stream per sensor, seeded from i).
+// Indices that would wrap, and so repeat streams, are rejected.
+int check_sample_indices(int64_t total, int spp, int seed_mode, const char* what) {
+    const int64_t limit = int64_t(1) << 32;
+    if (total > limit) return set_err(VR_ERR_UNSUPPORTED, "%s: more than 2^32 sensors", what);
+    if (seed_mode != VR_SEED_CAPTURE_BW && (int64_t)spp * total > limit)
+        return set_err(VR_ERR_UNSUPPORTED, "%s: samples x sensors exceeds 2^32 (per-sample random streams would repeat)",
+                       what);
+    return VR_OK;
+}
+
+int64_t camera_sensors(const vr_camera* c) {
+    return c->sensors ? c->num_sensors : (int64_t)c->num_poses * c->width * c->height;
+}
+
+// Flag set by k_majorant for a non-finite density, read back after the build:
+// one device word and one pinned host word per device, used under the mutex.
+constexpr int kMaxDevices = 64;
+std::mutex g_flag_mu;
+int* g_flag_dev[kMaxDevices] = {};
+int* g_flag_host[kMaxDevices] = {};
+
 }  // namespace
 
 // ================================================================== C ABI ==
@@ -1139,8 +1205,32 @@ int vr_majorant_build(const vr_grid* sigma, const int32_t res[3], float* out, vo
     if (!res || !out) return set_err(VR_ERR_ARG, "vr_majorant_build: null argument");
     if (res[0] < 1 || res[1] < 1 || res[2] < 1 || (int64_t)res[0] * res[1] * res[2] > (1 << 24))
         return set_err(VR_ERR_ARG, "vr_majorant_build: resolution out of range");
-    hipLaunchKernelGGL(k_majorant, dim3(res[0] * res[1] * res[2]), dim3(kBlock), 0, (hipStream_t)stream, g, res[0], res[1], res[2], out);
-    return check_hip(hipGetLastError(), "vr_majorant_build launch");
+    int dev = 0;
+    if ((st = check_hip(hipGetDevice(&dev), "hipGetDevice"))) return st;
+    if (dev < 0 || dev >= kMaxDevices) return set_err(VR_ERR_DEVICE, "vr_majorant_build: device index out of range");
+    std::lock_guard<std::mutex> lock(g_flag_mu);
+    if (!g_flag_dev[dev]) {
+        void* d = nullptr;
+        void* h = nullptr;
+        if ((st = check_hip(hipMalloc(&d, sizeof(int)), "hipMalloc(majorant flag)"))) return st;
+        if ((st = check_hip(hipHostMalloc(&h, sizeof(int), hipHostMallocDefault), "hipHostMalloc(majorant flag)"))) {
+            (void)hipFree(d);
+            return st;
+        }
+        g_flag_dev[dev] = (int*)d;
+        g_flag_host[dev] = (int*)h;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t*)g_flag_dev[dev], 1);
+    hipLaunchKernelGGL(k_majorant, dim3(res[0] * res[1] * res[2]), dim3(kBlock), 0, s, g, res[0], res[1], res[2], out,
+                       g_flag_dev[dev]);
+    if ((st = check_hip(hipGetLastError(), "vr_majorant_build launch"))) return st;
+    if ((st = check_hip(hipMemcpyAsync(g_flag_host[dev], g_flag_dev[dev], sizeof(int), hipMemcpyDeviceToHost, s),
+                        "vr_majorant_build flag copy")))
+        return st;
+    if ((st = check_hip(hipStreamSynchronize(s), "vr_majorant_build synchronize"))) return st;
+    if (*g_flag_host[dev]) return set_err(VR_ERR_ARG, "vr_majorant_build: sigma holds NaN or infinite values");
+    return VR_OK;
 }
 
 int vr_map_fw(const vr_medium* medium, int32_t kind, const float* in, int64_t n, float* out, const vr_launch* L, void* stream) {
@@ -1187,17 +1277,23 @@ int vr_capture_fw(const vr_medium* medium, int32_t kind, const vr_camera* camera
     if (st) return st;
     CameraView C;
     if ((st = make_camera_view(camera, C))) return st;
-    int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    const int64_t total = camera_sensors(camera);
     int64_t start, cnt;
     if ((st = resolve_range(L, total, start, cnt))) return st;
     if (cnt == 0) return VR_OK;
     if (!out) return set_err(VR_ERR_ARG, "vr_capture_fw: out is NULL");
     int spp = L->samples < 1 ? 1 : L->samples;
+    if ((st = check_sample_indices(total, spp, VR_SEED_CAPTURE_FW, "vr_capture_fw"))) return st;
     Seeds sd = make_seeds(L);
     hipStream_t s = (hipStream_t)stream;
+    Counters ctr;
+    if (capture_is_persistent(kind, M) && (st = ctr.acquire(L, s))) return st;
     if (medium->majorant_kind == VR_MAJ_GRID)
-        return launch_capture_fw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, s);
-    return launch_capture_fw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, s);
+        st = launch_capture_fw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, ctr.ptr, s);
+    else
+        st = launch_capture_fw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, ctr.ptr, s);
+    const int rel = ctr.release();
+    return st ? st : rel;
 }
 
 int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera, const vr_launch* L, const float* og, void* stream) {
@@ -1210,12 +1306,13 @@ int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera
     if (st) return st;
     CameraView C;
     if ((st = make_camera_view(camera, C))) return st;
-    int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    const int64_t total = camera_sensors(camera);
     int64_t start, cnt;
     if ((st = resolve_range(L, total, start, cnt))) return st;
     if (cnt == 0) return VR_OK;
     if (!og) return set_err(VR_ERR_ARG, "vr_capture_bw: out_grad is NULL");
     int spp = L->samples < 1 ? 1 : L->samples;
+    if ((st = check_sample_indices(total, spp, L->seed_mode, "vr_capture_bw"))) return st;
     Seeds sd = make_seeds(L);
     hipStream_t s = (hipStream_t)stream;
     if (medium->majorant_kind == VR_MAJ_GRID)
@@ -1426,11 +1523,6 @@ int vr_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels,
 // ------------------------------------------------------------------ cache --
 namespace {
 
-__global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
 __global__ void k_zero_f32(float* __restrict__ p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
@@ -1537,13 +1629,14 @@ int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_lau
     if (M.t_kind != VR_T_DT && M.t_kind != VR_T_RT) return set_err(VR_ERR_UNSUPPORTED, "vr_nrc_render: DT or RT shadow rays only");
     CameraView C;
     if ((st = make_camera_view(camera, C))) return st;
-    const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
+    const int64_t total = camera_sensors(camera);
     int64_t start, cnt;
     if ((st = resolve_range(L, total, start, cnt))) return st;
+    const int spp = L->samples < 1 ? 1 : L->samples;
+    if ((st = check_sample_indices(total, spp, VR_SEED_CAPTURE_FW, "vr_nrc_render"))) return st;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count, 1);
     if (cnt == 0) return check_hip(hipGetLastError(), "vr_nrc_render launch");
-    const int spp = L->samples < 1 ? 1 : L->samples;
     const CaptureSplit P = capture_split(spp, false);
     if ((st = capture_prepare(P, out, 3, start, cnt, nullptr, s))) return st;
     const Seeds sd = make_seeds(L);
@@ -1673,14 +1766,14 @@ int vr_scene_capture_fw(const vr_scene* scene, const vr_camera* camera, const vr
     if (st) return st;
     CameraView C;
     if ((st = make_camera_view(camera, C))) return st;
-    const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
-    if (total > 0xFFFFFFFFll) return set_err(VR_ERR_ARG, "vr_scene_capture_fw: more than 2^32 sensors");
+    const int64_t total = camera_sensors(camera);
     int64_t start, cnt;
     if ((st = resolve_range(L, total, start, cnt))) return st;
     if (cnt == 0) return VR_OK;
     if (!out) return set_err(VR_ERR_ARG, "vr_scene_capture_fw: out is NULL");
     hipStream_t s = (hipStream_t)stream;
     const int spp = L->samples < 1 ? 1 : L->samples;
+    if ((st = check_sample_indices(total, spp, VR_SEED_CAPTURE_FW, "vr_scene_capture_fw"))) return st;
     CaptureSplit P = capture_split(spp, false);
     capture_tiling(P, C, start, cnt, "VR_SCENE_TILED", 2);
     if ((st = capture_prepare(P, out, 3, start, cnt, L->debug_out, s))) return st;
@@ -1698,14 +1791,14 @@ int vr_scene_capture_bw(const vr_scene* scene, const vr_camera* camera, const vr
     if (st) return st;
     CameraView C;
     if ((st = make_camera_view(camera, C))) return st;
-    const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
-    if (total > 0xFFFFFFFFll) return set_err(VR_ERR_ARG, "vr_scene_capture_bw: more than 2^32 sensors");
+    const int64_t total = camera_sensors(camera);
     int64_t start, cnt;
     if ((st = resolve_range(L, total, start, cnt))) return st;
     if (cnt == 0) return VR_OK;
     if (!out_grad) return set_err(VR_ERR_ARG, "vr_scene_capture_bw: out_grad is NULL");
     hipStream_t s = (hipStream_t)stream;
     const int spp = L->samples < 1 ? 1 : L->samples;
+    if ((st = check_sample_indices(total, spp, L->seed_mode, "vr_scene_capture_bw"))) return st;
     CaptureSplit P = capture_split(spp, L->seed_mode == VR_SEED_CAPTURE_BW);
     capture_tiling(P, C, start, cnt, "VR_SCENE_TILED", 2);
     P.debug_lookups = L->debug_kind == 1;

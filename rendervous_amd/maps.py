@@ -12,6 +12,7 @@ from __future__ import annotations
 import math
 import threading
 import typing
+import weakref
 
 import numpy as np
 import torch
@@ -209,6 +210,22 @@ class XRQuadtreeRandomDirection(MapBase):
 
 
 # ---------------------------------------------------------------- Grid3D --
+class _TensorStamp:
+    """Identity, storage address, version and layout of a tensor, for caches of
+    tensors derived from it (cell tables, majorant grids): a cache entry is
+    reused only while all of them match, as the reference's TensorCheck compares
+    identity and version (_maps.py:1061-1073).  In-place writes through `.data`
+    do not bump a tensor's version; follow them with
+    torch.autograd.graph.increment_version(t)."""
+
+    def __init__(self, t: torch.Tensor):
+        self.ref = weakref.ref(t)
+        self.key = (t.data_ptr(), t._version, tuple(t.shape), tuple(t.stride()))
+
+    def matches(self, t: torch.Tensor) -> bool:
+        return self.ref() is t and self.key == (t.data_ptr(), t._version, tuple(t.shape), tuple(t.stride()))
+
+
 class Grid3D(MapBase):
     """Vertex-centred trilinear grid [D,H,W,C] over [bmin,bmax] (_maps.py:2319-2365)."""
 
@@ -231,13 +248,12 @@ class Grid3D(MapBase):
         """The grid's cell table (vr_grid_pack_cells: 8 corners per cell, 32 bytes),
         rebuilt when the grid tensor changes (its pointer, version or layout)."""
         g = self.grid
-        key = (g.data_ptr(), g._version, tuple(g.shape), tuple(g.stride()))
         cache = self.__dict__.get("_cells_cache")
-        if cache is None or cache[0] != key:
+        if cache is None or not cache[0].matches(g):
             n = (g.shape[0] - 1) * (g.shape[1] - 1) * (g.shape[2] - 1)
             out = torch.empty(max(1, n) * 8, device=g.device, dtype=torch.float32)
             L.check(L.load().vr_grid_pack_cells(self.view(), L.ptr(out), L.stream_handle(g.device)), "vr_grid_pack_cells")
-            cache = (key, out)
+            cache = (_TensorStamp(g), out)
             object.__setattr__(self, "_cells_cache", cache)
         return cache[1]
 
@@ -336,14 +352,13 @@ class MajorantGrid(MapBase):
 
     def majorant_tensor(self) -> torch.Tensor:
         g = self.sigma_grid.grid
-        key = (g.data_ptr(), g._version, tuple(g.shape), tuple(g.stride()))
-        if self._cache is None or self._cache_key != key:
+        if self._cache is None or not self._cache_key.matches(g):
             out = torch.empty(*self.resolution, device=g.device, dtype=torch.float32)
             res = (L.ctypes.c_int32 * 3)(*self.resolution)
             L.check(L.load().vr_majorant_build(self.sigma_grid.view(), res, L.ptr(out), L.stream_handle(g.device)),
                     "vr_majorant_build")
             self._cache = out
-            self._cache_key = key
+            self._cache_key = _TensorStamp(g)
         return self._cache
 
 
@@ -1000,9 +1015,10 @@ class AutogradCaptureFunction(torch.autograd.Function):
             seeds = _internal.get_seeds()
             bs = total if batch_size is None else int(batch_size)
             lib = L.load()
+            workspace = L.capture_workspace(out.device)
             for start in range(0, total, max(1, bs)):
                 launch = L.launch(seeds, samples=fw_samples, bw_mode=_bw_mode(), start_index=start,
-                                  count=min(bs, total - start), debug_out=debug_out)
+                                  count=min(bs, total - start), debug_out=debug_out, workspace=workspace)
                 if kind == L.MAP_SCENE:
                     L.check(lib.vr_scene_capture_fw(lw.scene, cam, launch, L.ptr(out), L.stream_handle(out.device)),
                             "vr_scene_capture_fw")

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import L, O, ROOT
+from helpers import L, O, ROOT, Built, Camera, Scene, pose
 
 import rendervous_amd as rv
 
@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(L.EXPORTED)
-    assert lib.vr_abi_version() == 1
+    assert lib.vr_abi_version() == L.ABI_VERSION == 2
 
 
 def test_oracle_exports_the_same_interface():
@@ -42,7 +42,7 @@ def test_struct_layouts_match_the_header():
     assert ctypes.sizeof(L.VrTensor) == 8 + 8 + 32 + 32 + 8
     assert ctypes.sizeof(L.VrGrid) == ctypes.sizeof(L.VrTensor) + 24
     assert L.VrLaunch.start_index.offset == 32
-    assert ctypes.sizeof(L.VrLaunch) == 56
+    assert L.VrLaunch.workspace.offset == 56 and ctypes.sizeof(L.VrLaunch) == 72
     assert L.VrCamera.sensors.offset == 32
     assert ctypes.sizeof(L.VrBvhNode) == 32 and ctypes.sizeof(L.VrPatch) == 32
     assert ctypes.sizeof(L.VrScene) == 224 and L.VrScene.media.offset == 88 and L.VrScene.env_image.offset == 136
@@ -121,3 +121,75 @@ def test_seed_contract():
     b = rv.get_seeds()
     assert a == b
     assert all(129 <= s < (1 << 30) for s in a)
+
+
+# ------------------------------------------- host-side rejection of bad input --
+# Each case is rejected while validating arguments, before any device work: the
+# buffers are host arrays that the library never dereferences.
+def _nee_args(**scene_kw):
+    sc = Scene(sigma=np.ones((4, 4, 4, 1), np.float32), **scene_kw)
+    b = Built(sc, "cpu")
+    cam = Camera(pose((0.0, 0.0, 2.5))[None], 8, 8, device="cpu")
+    out = np.zeros((64, 3), np.float32)
+    return b, cam, out
+
+
+@pytest.mark.parametrize("value,distance", [(float("inf"), 1e6), (float("nan"), 1e6), (-1.0, 1e6), (2.0, 0.0),
+                                            (2.0, float("nan")), (2.0, -3.0)])
+def test_bad_constant_majorants_are_rejected(value, distance):
+    lib = L.load()
+    b, cam, out = _nee_args(majorant_value=value, majorant_distance=distance)
+    for kind in (L.VR_MAP_NEE_DT, L.VR_MAP_T_DT, L.VR_MAP_T_RT, L.VR_MAP_COLLISION_DT):
+        st = lib.vr_capture_fw(b.medium, kind, cam.c, L.launch((1, 2, 3, 4), samples=1), O.nptr(out), None)
+        assert st == L.VR_ERR_ARG, kind
+        assert b"majorant" in lib.vr_last_error()
+        st = lib.vr_map_fw(b.medium, kind, O.nptr(out), 4, O.nptr(out), L.launch((1, 2, 3, 4)), None)
+        assert st == L.VR_ERR_ARG
+
+
+def test_bad_medium_majorant_is_rejected():
+    lib = L.load()
+    b, cam, out = _nee_args(medium_majorant=float("inf"))
+    st = lib.vr_map_fw(b.medium, L.VR_MAP_MEDIUM_DT, O.nptr(out), 2, O.nptr(out), L.launch((1, 2, 3, 4)), None)
+    assert st == L.VR_ERR_ARG and b"medium majorant" in lib.vr_last_error()
+
+
+def test_sample_indices_beyond_32_bits_are_rejected():
+    """Sample s of sensor i draws the stream of flat index s * sensors + i (32
+    bits): 64 x 2^26 = 2^32 samples is the largest capture, one more sensor's
+    worth is rejected (replayed backward too; the fresh-seed backward seeds one
+    stream per sensor)."""
+    lib = L.load()
+    b, _, out = _nee_args()
+    p = np.ascontiguousarray(pose((0.0, 0.0, 2.5))[None], np.float32)
+    cam = L.VrCamera()
+    cam.poses = O.nptr(p)
+    cam.num_poses = 1
+    cam.width = 1 << 13
+    cam.height = (1 << 13) + 1  # 2^26 + 2^13 sensors
+    cam.fov = 0.7
+    cam.znear = 0.001
+    for kind in (L.VR_MAP_NEE_DT, L.VR_MAP_SS_RADIANCE, L.VR_MAP_T_DT):
+        st = lib.vr_capture_fw(b.medium, kind, cam, L.launch((1, 2, 3, 4), samples=64), O.nptr(out), None)
+        assert st == L.VR_ERR_UNSUPPORTED and b"2^32" in lib.vr_last_error(), kind
+        st = lib.vr_capture_bw(b.medium, kind, cam, L.launch((1, 2, 3, 4), samples=64, seed_mode=L.VR_SEED_CAPTURE_FW),
+                               O.nptr(out), None)
+        assert st == L.VR_ERR_UNSUPPORTED, kind
+    cam.height = 1 << 19  # 2^32 sensors + ... : too many sensors for any seed mode
+    cam.width = (1 << 13) + 1
+    st = lib.vr_capture_bw(b.medium, L.VR_MAP_NEE_DT, cam, L.launch((1, 2, 3, 4), samples=1, seed_mode=L.VR_SEED_CAPTURE_BW),
+                           O.nptr(out), None)
+    assert st == L.VR_ERR_UNSUPPORTED and b"2^32 sensors" in lib.vr_last_error()
+
+
+@pytest.mark.parametrize("nbytes,offset", [(512, 0), (1024, 64)])
+def test_capture_workspace_must_fit(nbytes, offset):
+    lib = L.load()
+    b, cam, out = _nee_args(majorant_value=1.0)
+    ws = np.zeros(4096, np.uint8)
+    base = (O.nptr(ws) + 255) // 256 * 256 + offset
+    lau = L.launch((1, 2, 3, 4), samples=2)
+    lau.workspace = base
+    lau.workspace_bytes = nbytes
+    st = lib.vr_capture_fw(b.medium, L.VR_MAP_NEE_DT, cam.c, lau, O.nptr(out), None)
+    assert st == L.VR_ERR_ARG and b"workspace" in lib.vr_last_error()
