@@ -324,6 +324,19 @@ int vr_map_bw(const vr_medium* medium, int32_t kind, const float* in, const floa
 int vr_capture_fw(const vr_medium* medium, int32_t kind, const vr_camera* camera,
                   const vr_launch* launch, float* out, void* stream);
 
+/* vr_capture_fw that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The persistent NEE kernel's
+ * work counters (the only scratch a capture uses: VR_CAPTURE_WORKSPACE_BYTES for
+ * at most 8 work queues, whatever the capture's size) live in a file-scope device
+ * array of the library, one per GPU, held until the process ends;
+ * launch->workspace is not used.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (concurrent streams or threads) race and must be
+ * serialised by the caller.  Each call clears the counters before use.  Same
+ * arguments, limits and results as vr_capture_fw. */
+int vr_capture_fw_noalloc(const vr_medium* medium, int32_t kind, const vr_camera* camera,
+                          const vr_launch* launch, float* out, void* stream);
+
 /* Capture backward: accumulates into sigma.grad / albedo.grad the gradient of
  * sum(out_grad * out) (eval_capture_backward, _maps.py:824-882). */
 int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera,

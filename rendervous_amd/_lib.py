@@ -203,6 +203,8 @@ _SIGNATURES = {
     "vr_map_fw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, P(VrLaunch), ctypes.c_void_p]),
     "vr_map_bw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, P(VrLaunch), ctypes.c_void_p]),
     "vr_capture_fw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, P(VrCamera), P(VrLaunch), ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_capture_fw_noalloc": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, P(VrCamera), P(VrLaunch), ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     "vr_capture_bw": (ctypes.c_int, [P(VrMedium), ctypes.c_int32, P(VrCamera), P(VrLaunch), ctypes.c_void_p, ctypes.c_void_p]),
     "vr_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32,

@@ -1167,6 +1167,68 @@ std::mutex g_flag_mu;
 int* g_flag_dev[kMaxDevices] = {};
 int* g_flag_host[kMaxDevices] = {};
 
+// Work counters of vr_capture_fw_noalloc: a file-scope device array (one copy
+// per GPU, held until the process ends) instead of a workspace.  In host code
+// `scratch` only names the symbol; its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr int kScratchWords = kMaxQueues * kQueueStride;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+static_assert(sizeof(scratch) >= (size_t)kCounterBytes, "scratch holds the work counters of every queue");
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_counters(unsigned long long** out) {
+    int dev = 0;
+    int st = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (st) return st;
+    if (dev < 0 || dev >= kMaxDevices) return set_err(VR_ERR_DEVICE, "vr_capture_fw_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if ((st = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))) return st;
+        g_scratch_ptr[dev] = static_cast<unsigned long long*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return VR_OK;
+}
+
+// vr_capture_fw and vr_capture_fw_noalloc: the persistent NEE kernel's work
+// counters come from the launch's workspace (or a stream-ordered allocation),
+// or, with `static_counters`, from the device array above.
+int capture_fw(const vr_medium* medium, int32_t kind, const vr_camera* camera, const vr_launch* L, float* out, void* stream,
+               bool static_counters, const char* who) {
+    int id, od;
+    if (!map_dims(kind, id, od)) return set_err(VR_ERR_UNSUPPORTED, "%s: unknown map kind", who);
+    if (id != 6) return set_err(VR_ERR_UNSUPPORTED, "%s: capture needs a 6-input map", who);
+    if (!L) return set_err(VR_ERR_ARG, "%s: launch is NULL", who);
+    MediumView M;
+    int st = make_medium_view(medium, kind, L->bw_mode, M);
+    if (st) return st;
+    CameraView C;
+    if ((st = make_camera_view(camera, C))) return st;
+    const int64_t total = camera_sensors(camera);
+    int64_t start, cnt;
+    if ((st = resolve_range(L, total, start, cnt))) return st;
+    if (cnt == 0) return VR_OK;
+    if (!out) return set_err(VR_ERR_ARG, "%s: out is NULL", who);
+    int spp = L->samples < 1 ? 1 : L->samples;
+    if ((st = check_sample_indices(total, spp, VR_SEED_CAPTURE_FW, who))) return st;
+    Seeds sd = make_seeds(L);
+    hipStream_t s = (hipStream_t)stream;
+    Counters ctr;
+    if (capture_is_persistent(kind, M)) {
+        if (static_counters) st = scratch_counters(&ctr.ptr);
+        else st = ctr.acquire(L, s);
+        if (st) return st;
+    }
+    if (medium->majorant_kind == VR_MAJ_GRID)
+        st = launch_capture_fw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, ctr.ptr, s);
+    else
+        st = launch_capture_fw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, ctr.ptr, s);
+    const int rel = ctr.release();
+    return st ? st : rel;
+}
+
 }  // namespace
 
 // ================================================================== C ABI ==
@@ -1268,32 +1330,12 @@ int vr_map_bw(const vr_medium* medium, int32_t kind, const float* in, const floa
 }
 
 int vr_capture_fw(const vr_medium* medium, int32_t kind, const vr_camera* camera, const vr_launch* L, float* out, void* stream) {
-    int id, od;
-    if (!map_dims(kind, id, od)) return set_err(VR_ERR_UNSUPPORTED, "vr_capture_fw: unknown map kind");
-    if (id != 6) return set_err(VR_ERR_UNSUPPORTED, "vr_capture_fw: capture needs a 6-input map");
-    if (!L) return set_err(VR_ERR_ARG, "vr_capture_fw: launch is NULL");
-    MediumView M;
-    int st = make_medium_view(medium, kind, L->bw_mode, M);
-    if (st) return st;
-    CameraView C;
-    if ((st = make_camera_view(camera, C))) return st;
-    const int64_t total = camera_sensors(camera);
-    int64_t start, cnt;
-    if ((st = resolve_range(L, total, start, cnt))) return st;
-    if (cnt == 0) return VR_OK;
-    if (!out) return set_err(VR_ERR_ARG, "vr_capture_fw: out is NULL");
-    int spp = L->samples < 1 ? 1 : L->samples;
-    if ((st = check_sample_indices(total, spp, VR_SEED_CAPTURE_FW, "vr_capture_fw"))) return st;
-    Seeds sd = make_seeds(L);
-    hipStream_t s = (hipStream_t)stream;
-    Counters ctr;
-    if (capture_is_persistent(kind, M) && (st = ctr.acquire(L, s))) return st;
-    if (medium->majorant_kind == VR_MAJ_GRID)
-        st = launch_capture_fw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, ctr.ptr, s);
-    else
-        st = launch_capture_fw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, total, start, cnt, out, L->debug_out, ctr.ptr, s);
-    const int rel = ctr.release();
-    return st ? st : rel;
+    return capture_fw(medium, kind, camera, L, out, stream, false, "vr_capture_fw");
+}
+
+int vr_capture_fw_noalloc(const vr_medium* medium, int32_t kind, const vr_camera* camera, const vr_launch* L, float* out,
+                          void* stream) {
+    return capture_fw(medium, kind, camera, L, out, stream, true, "vr_capture_fw_noalloc");
 }
 
 int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera, const vr_launch* L, const float* og, void* stream) {

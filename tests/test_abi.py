@@ -29,10 +29,11 @@ def test_library_exports_every_declared_symbol():
 
 def test_oracle_exports_the_same_interface():
     """Every entry point of the path has an oracle twin; the radiance-cache MLP
-    (vr_nrc_*) has no reference counterpart and is checked against PyTorch FP32."""
+    (vr_nrc_*) has no reference counterpart and is checked against PyTorch FP32,
+    and the allocation-free capture is checked against vr_capture_fw."""
     lib = O.load()
     for n in _declared("vr_api.h", "vr_"):
-        if n.startswith("vr_nrc_"):
+        if n.startswith("vr_nrc_") or n == "vr_capture_fw_noalloc":
             continue
         assert hasattr(lib, "oracle_" + n[3:]), n
 
