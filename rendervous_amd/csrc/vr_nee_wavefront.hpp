@@ -105,6 +105,10 @@ struct NeeLane {
     // One tracking step of the current ray; sets `event` when the lane needs a
     // handler (collision, end of a transmittance ray, end of the path).
     VR_DEV void step(const MediumView& M) {
+        if (TKIND == VR_T_DT) {
+            step_dt(M);
+            return;
+        }
         float md;
         float mq = walk.query(M, md);
         bool need_sigma = false;
@@ -177,6 +181,54 @@ struct NeeLane {
                     else Tt = 1.0f;
                 }
             }
+        }
+    }
+
+    // step() with DT shadow rays: the path's free flight (vr_path_integrator_NEE.h:18-38)
+    // and the DT shadow ray's (vr_transmittance_DT.h:10-27) differ only in the
+    // majorant floor, the end-of-ray tests and when the remaining distance is
+    // decremented, so both modes run one code path with per-lane selects (a wave
+    // holding both would otherwise execute both branches).  Same draws and
+    // arithmetic per lane as the two branches.
+    VR_DEV void step_dt(const MediumView& M) {
+        const bool path = mode == kModePath;
+        if (path && !(rd > EPS)) {
+            event = kEvPathEnd;
+            return;
+        }
+        float md;
+        const float mq = walk.query(M, md);
+        const float m = fmaxf(path ? EPS : 0.0000001f, mq);
+        const float t = neg_log1m(rng_uniform(rng)) * rcp(m);
+        rx = rx + rw * fminf(t, md);
+        if (t > md) {  // majorant-cell crossing
+            rd -= md;
+            walk.cross(M, rw);
+            if (rd <= kCrossEps) {
+                event = path ? kEvPathEnd : kEvTrackEnd;
+                track_T = 1.0f;
+            }
+            return;
+        }
+        // path: the tentative collision lies beyond the exit (rd is reduced only
+        // at a null collision); shadow ray: rd is reduced first, then tested
+        const float rest = rd - t;
+        if (path ? t > rd : rest <= 0.0000001f) {
+            event = path ? kEvPathEnd : kEvTrackEnd;
+            track_T = 1.0f;
+            return;
+        }
+        walk.move(t);
+        if (!path) rd = rest;
+        const float sg = LEAN ? cells_lookup1(M.sigma, M.cells, rx) : sigma_lookup(M, rx);
+        lookups++;
+        const bool real = rng_uniform(rng) * m < sg;
+        if (path) {
+            if (real) event = kEvCollision;
+            else rd = rest;  // null collision
+        } else if (real) {
+            event = kEvTrackEnd;
+            track_T = 0.0f;
         }
     }
 
