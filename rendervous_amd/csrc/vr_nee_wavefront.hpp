@@ -307,6 +307,7 @@ struct NeeSchedule {
     int ksteps;     // tracking steps per iteration of the event loop
     int kmax;       // at most this many steps when ...
     int track_min;  // ... at least this many lanes of the wave are still tracking after ksteps
+    int min_begin;  // new samples start once this many lanes wait for one (or no lane tracks)
 };
 
 // Sensors and items of queue q.
@@ -461,8 +462,13 @@ k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsig
             if (__ballot(!done) == 0ull) break;  // every lane of the wave has drained the queues
             continue;
         }
-        if (have && lane.event == kEvBegin)
-            lane.begin_sample(M, cam, seeds, (uint32_t)s * S.total + sensor, pose, row, col);
+        // Starting a sample (seeding, camera ray, box entry) is the costliest
+        // handler; lanes waiting for one idle while the others track until
+        // S.min_begin of them can start together.
+        const bool tracks = __ballot(have && lane.event == kEvNone) != 0ull;
+        if (!tracks || __popcll(__ballot(have && lane.event == kEvBegin)) >= S.min_begin)
+            if (have && lane.event == kEvBegin)
+                lane.begin_sample(M, cam, seeds, (uint32_t)s * S.total + sensor, pose, row, col);
         if (have && lane.event == kEvStart) lane.start_ray(M);
         for (int k = 0;;) {
             if (have && lane.event == kEvNone) lane.step(M);
