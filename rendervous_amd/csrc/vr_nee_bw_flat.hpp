@@ -30,7 +30,17 @@ enum : int {
     kBwAfterT2 = 6,       // pass 2: T known, start the shadow-ray backward
     kBwAfterTbw = 7,      // pass 2: the shadow-ray backward ended
     kBwDone = 8,          // the sample ended
-    kBwCollision = 9      // the path ray has a real collision (s, m, Pc saved)
+    kBwCollision = 9,     // the path ray has a real collision (s, m, Pc saved)
+    kBwStart = 10         // a ray waits for its set-up (start_kind) in the shared start block
+};
+
+// Ray set-ups of the shared start block.
+enum : int {
+    kStPath = 0,      // a pass's first path segment: clipped to the medium box
+    kStContinue = 1,  // the path continues from a collision to the box exit
+    kStShadowFw = 2,  // the shadow ray's DT estimate (t_dt)
+    kStShadowBw = 3,  // the shadow ray's DT backward (t_dt_bw)
+    kStReplay = 4     // the shadow-ray backward's replay from its saved start
 };
 
 // ENVMAP: environment image and/or quadtree sampler; false compiles the
@@ -52,7 +62,7 @@ struct NeeDsBwLane {
     float ds0, null_W, rho, Tn, dL_dTn, eps;
     float cs, cm, cpc;         // sigma, majorant and collision probability of a pending collision
     f3 dL_dR;
-    int state, pass, scattered, bounces;
+    int state, pass, scattered, bounces, start_kind, after;
 
     VR_DEV explicit NeeDsBwLane(const MediumView& m) : tr(m) {}
 
@@ -72,173 +82,155 @@ struct NeeDsBwLane {
         W = F3(1.0f, 1.0f, 1.0f);
         rx = x0;
         rw = w0;
-        float tmin, tmax;
-        boundary(tr.M, rx, rw, tmin, tmax);
-        rx = rx + rw * tmin;
-        rd = tmax - tmin;
         bounces = 0;
-        walk.begin(tr.M, rx, rw);
-        state = kBwPath;
+        pend(kStPath);
     }
 
-    // Start of the shadow ray from the path vertex px along rw (t_dt, :616-635).
-    VR_DEV void start_shadow_fw(int after) {
-        rx = px;
+    VR_DEV void pend(int kind) {
+        start_kind = kind;
+        state = kBwStart;
+    }
+
+    // The shared ray set-up (one block per iteration for every waiting lane):
+    // the box interval of (rx, rw) and the majorant walk; the pass's first path
+    // segment and the shadow rays (t_dt, t_dt_bw) are clipped to the box, a
+    // continued path runs to the box exit, a shadow ray that misses the box has
+    // T = 1 (its backward, nothing to do).
+    VR_DEV void start() {
+        const MediumView& M = tr.M;
         float tmin, tmax;
-        if (!boundary(tr.M, rx, rw, tmin, tmax)) { Tn = 1.0f; state = after; return; }
-        rx = rx + rw * tmin;
-        rd = tmax - tmin;
-        walk.begin(tr.M, rx, rw);
-        state = kBwShadowFw;
+        const bool hit = boundary(M, rx, rw, tmin, tmax);
+        int next = kBwPath;
+        if (start_kind == kStContinue) {
+            rd = tmax;
+        } else if (start_kind == kStReplay) {
+            rx = xs0;
+            rd = ds0;
+            next = kBwShadowReplay;
+        } else {
+            if (start_kind == kStShadowFw) next = hit ? kBwShadowFw : after;
+            else if (start_kind == kStShadowBw) next = hit ? kBwShadowPrimal : kBwAfterTbw;
+            if (start_kind == kStShadowFw && !hit) Tn = 1.0f;
+            rx = rx + rw * tmin;
+            rd = tmax - tmin;
+            if (start_kind == kStShadowBw) {
+                xs0 = rx;
+                ds0 = rd;
+            }
+        }
+        state = next;
+        if (state <= kBwShadowReplay) walk.begin(M, rx, rw);
     }
 
-    // Start of the shadow ray's DT backward (t_dt_bw), with tr.rng = before_T.
-    VR_DEV void start_shadow_bw() {
-        if (dL_dTn == 0.0f) { state = kBwAfterTbw; return; }
-        rx = px;
-        float tmin, tmax;
-        if (!boundary(tr.M, rx, rw, tmin, tmax)) { state = kBwAfterTbw; return; }
-        rx = rx + rw * tmin;
-        rd = tmax - tmin;
-        xs0 = rx;
-        ds0 = rd;
-        walk.begin(tr.M, rx, rw);
-        state = kBwShadowPrimal;
+    // The end of a shadow ray (it left the medium): its DT estimate is 1, its
+    // backward's primal pass was not absorbed (replay it from the start), or
+    // its replay is done.
+    VR_DEV void shadow_end() {
+        if (state == kBwShadowFw) {
+            Tn = 1.0f;
+            state = pass == 1 ? kBwAfterT1 : kBwAfterT2;
+        } else if (state == kBwShadowPrimal) {
+            tr.rng = before_T;
+            pend(kStReplay);
+        } else {
+            state = kBwAfterTbw;
+        }
     }
 
-    // The next path segment from the collision px along wo.
-    VR_DEV void continue_path() {
-        rw = wo;
-        rx = px;
-        float tmin;
-        boundary(tr.M, rx, rw, tmin, rd);
-        walk.begin(tr.M, rx, rw);
-        state = kBwPath;
-    }
-
-    // One tracking step of the current ray.
+    // One tracking step of the current ray.  The path's flight (either pass)
+    // and the shadow ray's (its estimate, its backward's primal pass and
+    // replay) differ in the majorant, the end-of-ray tests and when the
+    // remaining distance is decremented, so all kinds run one code path with
+    // per-lane selects, one uniform after the lookup and one scatter site (a
+    // wave holding several kinds would otherwise run each branch).  Every
+    // lane's draws and arithmetic are those of Tracer::nee_ds_bw.
     VR_DEV void step() {
         const MediumView& M = tr.M;
+        const bool path = state == kBwPath;
+        if (path && !(rd > EPS)) { state = kBwPathEnd; return; }
         float md;
         const float q = walk.query(M, md);
-        float m;
-        float t = 0.0f;
-        if (state == kBwPath) {
-            if (!(rd > EPS)) { state = kBwPathEnd; return; }
-            const bool ref = M.bw_mode == VR_BW_REFERENCE;
-            m = (pass == 2 && ref) ? fmaxf(EPS, q * 1.2f + 2.0f * eps) : fmaxf(EPS, q + 2.0f * eps + 0.1f);
-            t = tr.flight(m);
-            rx = rx + rw * fminf(t, md);
-            if (t > md) { rd -= md; walk.cross(M, rw); if (rd <= kCrossEps) state = kBwPathEnd; return; }
-            if (t > rd) { state = kBwPathEnd; return; }
-            walk.move(t);
+        const bool ref2 = pass == 2 && M.bw_mode == VR_BW_REFERENCE;
+        const float m_path = ref2 ? fmaxf(EPS, q * 1.2f + 2.0f * eps) : fmaxf(EPS, q + 2.0f * eps + 0.1f);
+        const float m_shadow = fmaxf(0.0000001f, state == kBwShadowFw ? q : q + 0.1f);
+        const float m = path ? m_path : m_shadow;
+        const float t = tr.flight(m);
+        rx = rx + rw * fminf(t, md);
+        bool ended;
+        if (t > md) {  // majorant-cell crossing
+            rd -= md;
+            walk.cross(M, rw);
+            ended = rd <= kCrossEps;
         } else {
-            m = fmaxf(0.0000001f, state == kBwShadowFw ? q : q + 0.1f);
-            const float dt = tr.flight(m);
-            const float tt = fminf(md, dt);
-            rx = rx + rw * tt;
-            rd -= tt;
-            bool end = false;
-            if (dt > md) {
-                walk.cross(M, rw);
-                if (rd > kCrossEps) return;
-                end = true;
-            } else {
-                walk.move(tt);
-                end = rd <= 0.0000001f;
+            // path: the tentative collision lies beyond the exit (rd is reduced
+            // only at a null collision); shadow ray: rd is reduced, then tested
+            const float rest = rd - t;
+            ended = path ? t > rd : rest <= 0.0000001f;
+            if (!ended) {
+                walk.move(t);
+                if (!path) rd = rest;
+                // the shared trilinear lookup and the decision of each ray kind
+                const float s = tr.sigma(rx);
+                const float u = tr.rnd();
+                float g = 0.0f;
+                bool scatter = false;
+                if (state == kBwShadowFw) {
+                    if (u * m < s) { Tn = 0.0f; state = pass == 1 ? kBwAfterT1 : kBwAfterT2; }
+                } else if (state == kBwShadowPrimal) {
+                    if (u < s / m) state = kBwAfterTbw;  // absorbed: no gradient
+                } else if (state == kBwShadowReplay) {
+                    const float Pc = s / m;
+                    g = -dL_dTn / m / (1.0f - Pc);
+                    scatter = true;
+                } else {
+                    const float Pc = (s + eps) / m;
+                    if (u >= Pc) {  // null collision
+                        if (pass == 2) {
+                            g = dot(dL_dR, R * (-1.0f / m / (1.0f - s / m)));
+                            scatter = true;
+                        }
+                        W = W * ((1.0f - s / m) / (1.0f - Pc));
+                        rd = rest;
+                    } else {
+                        cs = s;
+                        cm = m;
+                        cpc = Pc;
+                        state = kBwCollision;
+                    }
+                }
+                if (scatter) tr.sigma_bw(rx, g);
             }
-            if (end) {  // the shadow ray left the medium
-                if (state == kBwShadowFw) { Tn = 1.0f; state = pass == 1 ? kBwAfterT1 : kBwAfterT2; }
-                else if (state == kBwShadowPrimal) {  // not absorbed: replay from the start
-                    rx = xs0;
-                    rd = ds0;
-                    tr.rng = before_T;
-                    walk.begin(M, rx, rw);
-                    state = kBwShadowReplay;
-                } else state = kBwAfterTbw;
-                return;
-            }
         }
-        // the shared trilinear lookup and the decision of each ray kind
-        const float s = tr.sigma(rx);
-        if (state == kBwShadowFw) {
-            if (tr.rnd() * m < s) { Tn = 0.0f; state = pass == 1 ? kBwAfterT1 : kBwAfterT2; }
-            return;
+        if (ended) {
+            if (path) state = kBwPathEnd;
+            else shadow_end();
         }
-        if (state == kBwShadowPrimal) {
-            if (tr.rnd() < s / m) state = kBwAfterTbw;  // absorbed: no gradient
-            return;
-        }
-        if (state == kBwShadowReplay) {
-            tr.rnd();
-            const float Pc = s / m;
-            tr.sigma_bw(rx, -dL_dTn / m / (1.0f - Pc));
-            return;
-        }
-        const float Pc = (s + eps) / m;
-        if (tr.rnd() >= Pc) {  // null collision
-            if (pass == 2) tr.sigma_bw(rx, dot(dL_dR, R * (-1.0f / m / (1.0f - s / m))));
-            W = W * ((1.0f - s / m) / (1.0f - Pc));
-            rd -= t;
-            return;
-        }
-        cs = s;
-        cm = m;
-        cpc = Pc;
-        state = kBwCollision;
     }
 
+    // A real collision of the path: pass 1 (vr_path_integrator_NEE_DS.h:40-72)
+    // or pass 2 (:140-200; the albedo gradient is sent before the shadow ray,
+    // it uses no random number), then the light sample and the shadow ray.
     VR_DEV void collision() {
-        if (pass == 1) collision1(cs, cm, cpc);
-        else collision2(cs, cm, cpc);
-    }
-
-    // Pass-1 real collision (vr_path_integrator_NEE_DS.h:40-72).
-    VR_DEV void collision1(float s, float m, float Pc) {
         const MediumView& M = tr.M;
-        scattered = 1;
-        if (s < EPS) {
-            if (null_W < EPS) { W = F3(0.0f, 0.0f, 0.0f); state = kBwPathEnd; return; }
-            null_W = 0.0f;
-        } else W = W * ((s / m) / Pc);
+        const float s = cs, m = cm, Pc = cpc;
         const f3 sa = tr.albedo(rx);
-        added = (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
-        W = W * sa;
-        f3 wnee;
-        env = ENVMAP ? env_sample(M, tr.rng, wnee) : env_sample_const(M, tr.rng, wnee);
-        px = rx;
-        pw = rw;
-        rw = wnee;
-        start_shadow_fw(kBwAfterT1);
-    }
-
-    VR_DEV void after_t1() {
-        const MediumView& M = tr.M;
-        const float phn = phase_eval(M, pw, rw);
-        added = added + (W * env) * (Tn * phn);
-        if (null_W < EPS) R_sp = R_sp + added;
-        else R = R + added;
-        bounces++;
-        if (M.max_bounces >= 0 && bounces >= M.max_bounces) { W = F3(0.0f, 0.0f, 0.0f); state = kBwPathEnd; return; }
-        rho = phase_sample(M, tr.rng, pw, wo);
-        W = W * rho;
-        if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); state = kBwPathEnd; return; }
-        continue_path();
-    }
-
-    // Pass-2 real collision (vr_path_integrator_NEE_DS.h:140-200): the albedo
-    // gradient is sent before the shadow ray (it uses no random number).
-    VR_DEV void collision2(float s, float m, float Pc) {
-        const MediumView& M = tr.M;
-        const float gs = s < EPS ? dot(dL_dR, R_sp * (1.0f / eps)) : dot(dL_dR, R * (1.0f / s));
-        tr.sigma_bw(rx, gs);
-        W = W * (s / m / Pc);
-        if (s < EPS) { state = kBwDone; return; }
-        const f3 sa = tr.albedo(rx);
-        f3 dL_dsa = (dL_dR * W) * (M.emission_c * -1.0f);
-        R = R - (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
-        dL_dsa = dL_dsa + dL_dR * F3(R.x / fmaxf(EPS, sa.x), R.y / fmaxf(EPS, sa.y), R.z / fmaxf(EPS, sa.z));
-        tr.albedo_bw(rx, dL_dsa);
+        if (pass == 1) {
+            scattered = 1;
+            if (s < EPS) {
+                if (null_W < EPS) { W = F3(0.0f, 0.0f, 0.0f); state = kBwPathEnd; return; }
+                null_W = 0.0f;
+            } else W = W * ((s / m) / Pc);
+            added = (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+        } else {
+            const float gs = s < EPS ? dot(dL_dR, R_sp * (1.0f / eps)) : dot(dL_dR, R * (1.0f / s));
+            tr.sigma_bw(rx, gs);
+            W = W * (s / m / Pc);
+            if (s < EPS) { state = kBwDone; return; }
+            f3 dL_dsa = (dL_dR * W) * (M.emission_c * -1.0f);
+            R = R - (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            dL_dsa = dL_dsa + dL_dR * F3(R.x / fmaxf(EPS, sa.x), R.y / fmaxf(EPS, sa.y), R.z / fmaxf(EPS, sa.z));
+            tr.albedo_bw(rx, dL_dsa);
+        }
         W = W * sa;
         f3 wnee;
         env = ENVMAP ? env_sample(M, tr.rng, wnee) : env_sample_const(M, tr.rng, wnee);
@@ -246,18 +238,52 @@ struct NeeDsBwLane {
         px = rx;
         pw = rw;
         rw = wnee;
-        start_shadow_fw(kBwAfterT2);
+        after = pass == 1 ? kBwAfterT1 : kBwAfterT2;
+        pend(kStShadowFw);
     }
 
-    VR_DEV void after_t2() {
+    // The shadow ray's estimate Tn is known: pass 1 adds the NEE term and
+    // samples the next direction (:53-72); pass 2 samples the next direction,
+    // subtracts the term and starts the shadow ray's backward (:170-196).
+    VR_DEV void after_t() {
         const MediumView& M = tr.M;
         const float phn = phase_eval(M, pw, rw);
+        bool more = true;
+        if (pass == 1) {
+            added = added + (W * env) * (Tn * phn);
+            if (null_W < EPS) R_sp = R_sp + added;
+            else R = R + added;
+            bounces++;
+            if (M.max_bounces >= 0 && bounces >= M.max_bounces) {
+                W = F3(0.0f, 0.0f, 0.0f);
+                state = kBwPathEnd;
+                more = false;
+            }
+        }
+        if (!more) return;
         rho = phase_sample(M, tr.rng, pw, wo);
-        before_next = tr.rng;
-        dL_dTn = dot(dL_dR, (W * env) * phn);
-        R = R - (W * env) * (Tn * phn);
-        tr.rng = before_T;
-        start_shadow_bw();
+        if (pass == 1) {
+            W = W * rho;
+            if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); state = kBwPathEnd; return; }
+            continue_path();
+        } else {
+            before_next = tr.rng;
+            dL_dTn = dot(dL_dR, (W * env) * phn);
+            R = R - (W * env) * (Tn * phn);
+            tr.rng = before_T;
+            if (dL_dTn == 0.0f) state = kBwAfterTbw;
+            else {
+                rx = px;
+                pend(kStShadowBw);
+            }
+        }
+    }
+
+    // The next path segment from the collision px along wo.
+    VR_DEV void continue_path() {
+        rw = wo;
+        rx = px;
+        pend(kStContinue);
     }
 
     VR_DEV void after_tbw() {
@@ -284,10 +310,10 @@ struct NeeDsBwLane {
     // a shadow ray that misses the medium goes on to its continuation, ...).
     VR_DEV void events() {
         if (state == kBwCollision) collision();
-        if (state == kBwAfterT1) after_t1();
-        if (state == kBwAfterT2) after_t2();
+        if (state == kBwAfterT1 || state == kBwAfterT2) after_t();
         if (state == kBwAfterTbw) after_tbw();
         if (state == kBwPathEnd) path_end();
+        if (state == kBwStart) start();
     }
 };
 
