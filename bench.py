@@ -242,6 +242,7 @@ def _directional_check(camera, field_of, param, spp, n_dirs=4, h=0.25, batches=4
         dirs.append((0.5 + rv.functions.resample_grid(coarse, tuple(param.shape[:-1]))).double())
     ad = [[] for _ in dirs]
     fd = [[] for _ in dirs]
+    fd2 = [[] for _ in dirs]  # step h / 2, for Richardson extrapolation
     for b in range(batches):
         rv.seed(1000 + b)
         img = camera.capture(field_of(param), fw_samples=spp, bw_samples=spp)
@@ -250,22 +251,146 @@ def _directional_check(camera, field_of, param, spp, n_dirs=4, h=0.25, batches=4
         param.grad = None
         for k, v in enumerate(dirs):
             ad[k].append(float((g * v).sum()))
-            vals = []
-            for sgn in (1.0, -1.0):
-                with torch.no_grad():
-                    p2 = (param.detach() + sgn * h * v.float()).contiguous()
-                    rv.seed(1000 + b)
-                    vals.append(float((camera.capture(field_of(p2), fw_samples=spp) * w).double().sum()))
-            fd[k].append((vals[0] - vals[1]) / (2 * h))
+            for step, acc in ((h, fd[k]), (0.5 * h, fd2[k])):
+                vals = []
+                for sgn in (1.0, -1.0):
+                    with torch.no_grad():
+                        p2 = (param.detach() + sgn * step * v.float()).contiguous()
+                        rv.seed(1000 + b)
+                        vals.append(float((camera.capture(field_of(p2), fw_samples=spp) * w).double().sum()))
+                acc.append((vals[0] - vals[1]) / (2 * step))
     out = []
     for k in range(n_dirs):
-        a, f = torch.tensor(ad[k], dtype=torch.float64), torch.tensor(fd[k], dtype=torch.float64)
-        sa, sf = float(a.std() / batches ** 0.5), float(f.std() / batches ** 0.5)
-        ma, mf = float(a.mean()), float(f.mean())
-        out.append({"fd": mf, "fd_se": sf, "adjoint": ma, "adjoint_se": sa,
+        a = torch.tensor(ad[k], dtype=torch.float64)
+        f = torch.tensor(fd[k], dtype=torch.float64)
+        f2 = torch.tensor(fd2[k], dtype=torch.float64)
+        # central differences err by O(h^2): (4 D(h/2) - D(h)) / 3 removes that term
+        fr = (4.0 * f2 - f) / 3.0
+        sa, sf, sr = (float(x.std() / batches ** 0.5) for x in (a, f, fr))
+        ma, mf, mr = float(a.mean()), float(f.mean()), float(fr.mean())
+        out.append({"fd": mf, "fd_se": sf, "fd_half_step": float(f2.mean()), "fd_richardson": mr, "fd_richardson_se": sr,
+                    "adjoint": ma, "adjoint_se": sa,
                     "rel_err": abs(ma - mf) / max(abs(mf), 1e-30),
-                    "z": abs(ma - mf) / max((sa * sa + sf * sf) ** 0.5, 1e-30)})
+                    "z": abs(ma - mf) / max((sa * sa + sf * sf) ** 0.5, 1e-30),
+                    "rel_err_richardson": abs(ma - mr) / max(abs(mr), 1e-30),
+                    "z_richardson": abs(ma - mr) / max((sa * sa + sr * sr) ** 0.5, 1e-30)})
     return out
+
+
+def gradient_gate_leg(dev, res=256, views=8, hw=256, spp=16, voxels=64, h=1e-3, batches=256, ref_spp=1024):
+    """The gradient check of the 256^3 inverse-render configuration: central
+    finite differences (h = 1e-3) on 64 random voxels against the adjoint
+    gradient, at the initial density 5 (8 views of the seeded fBm target,
+    256x256, 16 spp, single scatter with RT shadow rays, fixed constant majorant
+    7 so that both sides of a difference draw the same random numbers).
+
+    The checked quantity is the L2 loss linearised at the current density,
+    Lambda = sum(w * I_hat) with w = dL/dI = 2 (I - T) / N and I a 1024-spp
+    render: its derivative is the loss gradient, and for a linear functional the
+    differences (same samples on both sides) and the adjoint estimate the same
+    thing.  For the L2 loss itself they do not: the differences of
+    (I_hat - T)^2 estimate d E[(I_hat - T)^2] = d (E[I_hat] - T)^2 + d Var(I_hat),
+    the fresh-seed adjoint (the reference's) d (E[I_hat] - T)^2 and the
+    replayed-seed adjoint adds 2 Cov(I_hat, gradient estimate); those three are
+    reported beside the gate as the measured variance terms.  Each of `batches`
+    dispatch seeds gives one adjoint and one difference per voxel; z = (mean
+    adjoint - mean difference) / the combined standard error.  Pixels are
+    rendered by one thread each (summation order fixed) and differences are
+    formed in float64 from the two images, so unchanged pixels cancel exactly."""
+    import torch
+    import rendervous_amd as rv
+    from rendervous_amd import inverse
+
+    old_chunk = os.environ.get("VR_CAPTURE_CHUNK")
+    os.environ["VR_CAPTURE_CHUNK"] = str(spp)
+    try:
+        target_sigma = rv.functions.fbm_value_noise(res, seed=11, scale=40.0, device=dev)
+        poses = rv.functions.orbit_poses(views, radius=2.5).to(dev)
+        camera = rv.PerspectiveCameraSensor(hw, hw, poses, jittered=True)
+        rv.seed(3)
+        with torch.no_grad():
+            target = camera.capture(inverse.single_scatter_scene(target_sigma).radiance, fw_samples=spp)
+        del target_sigma
+        target64 = target.double()
+        n_out = target.numel()
+        sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
+        field = inverse.single_scatter_scene(sigma, majorant=7.0).radiance
+        work = torch.full((res, res, res, 1), 5.0, device=dev)
+        wfield = inverse.single_scatter_scene(work, majorant=7.0).radiance
+        os.environ.pop("VR_CAPTURE_CHUNK", None)
+        rv.seed(77)
+        with torch.no_grad():
+            ref_img = camera.capture(wfield, fw_samples=ref_spp)
+        os.environ["VR_CAPTURE_CHUNK"] = str(spp)
+        w = (2.0 * (ref_img - target) / n_out).contiguous()
+        w64 = w.double()
+        gen = torch.Generator(device="cpu").manual_seed(17)
+        idx = torch.randint(8, res - 8, (voxels, 3), generator=gen)
+        vox = [tuple(int(a) for a in r) + (0,) for r in idx]
+        base = torch.tensor(5.0, dtype=torch.float32)
+        up, dn = float(base + h), float(base - h)  # float32 values of the perturbed voxel
+        h_eff = 0.5 * (up - dn)
+        adj = {"linear": [], "l2_replay": [], "l2_fresh": []}
+        fd_lin, fd_l2 = [], []
+        t0 = time.perf_counter()
+        for b in range(batches):
+            for name, mode, lin in (("linear", "fresh", True), ("l2_replay", "replay", False), ("l2_fresh", "fresh", False)):
+                rv.seed(2000 + b)
+                with rv.backward_mode(seeds=mode):
+                    img = camera.capture(field, fw_samples=spp, bw_samples=spp)
+                    ((img * w).sum() if lin else ((img - target) ** 2).mean()).backward()
+                g = sigma.grad.detach()
+                adj[name].append([float(g[v]) for v in vox])
+                sigma.grad = None
+            rl, r2 = [], []
+            for v in vox:
+                imgs = []
+                for val in (up, dn):
+                    with torch.no_grad():
+                        work[v] = val
+                        rv.seed(2000 + b)
+                        imgs.append(camera.capture(wfield, fw_samples=spp).double())
+                with torch.no_grad():
+                    work[v] = 5.0
+                a, c = imgs
+                rl.append(float(((a - c) * w64).sum()) / (2.0 * h_eff))
+                r2.append(float(((a - c) * (a + c - 2.0 * target64)).sum()) / n_out / (2.0 * h_eff))
+            fd_lin.append(rl)
+            fd_l2.append(r2)
+        wall = time.perf_counter() - t0
+    finally:
+        if old_chunk is None:
+            os.environ.pop("VR_CAPTURE_CHUNK", None)
+        else:
+            os.environ["VR_CAPTURE_CHUNK"] = old_chunk
+    sq = batches ** 0.5
+    A = {m: torch.tensor(x, dtype=torch.float64) for m, x in adj.items()}
+    F = {"linear": torch.tensor(fd_lin, dtype=torch.float64), "l2": torch.tensor(fd_l2, dtype=torch.float64)}
+
+    def compare(a, f):
+        ma, sa, mf, sf = a.mean(0), a.std(0) / sq, f.mean(0), f.std(0) / sq
+        z = (ma - mf) / (sa * sa + sf * sf).sqrt().clamp_min(1e-300)
+        za = z.abs()
+        return z, {"abs_z_max": float(za.max()), "abs_z_median": float(za.median()),
+                   "frac_abs_z_below_3": float((za < 3).double().mean()), "chi2_per_voxel": float((z * z).mean()),
+                   "corr_of_means": float(torch.corrcoef(torch.stack([ma, mf]))[0, 1]),
+                   "rel_err_of_sum": float((ma.sum() - mf.sum()).abs() / mf.abs().sum()),
+                   "median_rel_se_fd": float((sf / mf.abs().clamp_min(1e-300)).median())}
+
+    z_lin, gate = compare(A["linear"], F["linear"])
+    _, rep = compare(A["l2_replay"], F["l2"])
+    _, fre = compare(A["l2_fresh"], F["l2"])
+    per_voxel = [{"voxel": list(v[:3]), "fd": float(F["linear"][:, k].mean()), "fd_se": float(F["linear"][:, k].std() / sq),
+                  "adjoint": float(A["linear"][:, k].mean()), "adjoint_se": float(A["linear"][:, k].std() / sq),
+                  "z": float(z_lin[k])} for k, v in enumerate(vox)]
+    return {"workload": f"{res}^3 density 5.0, {views} views {hw}x{hw}, {spp} spp, single scatter (RT shadow rays); "
+                        f"central differences h = {h} on {voxels} random voxels of the L2 loss linearised at a "
+                        f"{ref_spp}-spp render, fixed majorant 7, {batches} seed batches", "h_float32": h_eff,
+            "wall_s": wall, "gate": gate,
+            "l2_loss_fd_vs_replay_adjoint": rep, "l2_loss_fd_vs_fresh_adjoint": fre,
+            "l2_replay_minus_fresh_over_sum_abs": float((A["l2_replay"].mean(0) - A["l2_fresh"].mean(0)).abs().sum()
+                                                       / A["l2_fresh"].mean(0).abs().sum()),
+            "per_voxel": per_voxel}
 
 
 def convergence_leg(dev, res=256, views=8, hw=256, spp=16, steps=50):
@@ -809,7 +934,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true", help="skip the gradient and DDA legs")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--leg", choices=["gradient", "nee_gradient", "nrc", "dda", "grid_tools", "scene", "convergence",
-                                      "nee_convergence"],
+                                      "nee_convergence", "gradient_gate"],
                     help="run only this secondary leg and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM byte counters)")
@@ -828,7 +953,7 @@ def main():
     if args.leg is not None:
         legs = {"gradient": gradient_leg, "nee_gradient": nee_gradient_leg, "nrc": nrc_leg, "dda": dda_leg,
                 "grid_tools": grid_tools_leg, "scene": scene_leg, "convergence": convergence_leg,
-                "nee_convergence": nee_convergence_leg}
+                "nee_convergence": nee_convergence_leg, "gradient_gate": gradient_gate_leg}
         print(json.dumps({args.leg: _quiet(legs[args.leg], dev)}), flush=True)
         return
     if args.calibrate_copy:

@@ -41,10 +41,17 @@ def get_seeds() -> typing.Tuple[int, int, int, int]:
 class BackwardSettings:
     """How capture/map backward passes draw randoms and which estimators they use.
 
-    seeds:     'replay' reuses the forward dispatch seeds (each sample's stream is
-               re-derived from its flat index, so the backward replays the forward's
-               randoms); 'fresh' draws new seeds with the capture-backward
-               per-thread init (_maps.py:864, :676-685), as the reference does.
+    seeds:     'fresh' (default) draws new dispatch seeds for the backward, as the
+               reference does (_maps.py:864): its paths are independent of the
+               forward's, so the gradient of a loss that is nonlinear in the image
+               (the L2 loss) is unbiased; each of its bw_samples samples per sensor
+               draws its own stream (the per-sample seeding of the forward), so a
+               sensor's samples can run on several threads.  'reference' also draws
+               new seeds but continues one stream per sensor over its samples, the
+               reference's capture-backward init (_maps.py:676-685).  'replay' reuses
+               the forward dispatch seeds (the backward replays the forward's
+               samples and needs no new ones), which for such a loss adds
+               2 Cov(image, gradient estimate) to the expected gradient.
     estimator: 'consistent' fixes the reference backward defects (matched
                primal/replay majorants, single-scatter transmittance gradient
                along the sampled direction, full path weights in the collision
@@ -52,7 +59,7 @@ class BackwardSettings:
     debug:     what a capture's debug_out counts in the backward: 'scatters'
                (trilinear gradient scatters) or 'lookups' (trilinear lookups).
     """
-    seeds: str = "replay"
+    seeds: str = "fresh"
     estimator: str = "consistent"
     debug: str = "scatters"
 
@@ -84,7 +91,7 @@ def backward_mode(seeds: typing.Optional[str] = None, estimator: typing.Optional
         assert debug in ("scatters", "lookups")
         backward_settings.debug = debug
     if seeds is not None:
-        assert seeds in ("replay", "fresh")
+        assert seeds in ("replay", "fresh", "reference")
         backward_settings.seeds = seeds
     if estimator is not None:
         assert estimator in ("consistent", "reference")

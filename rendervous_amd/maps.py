@@ -1045,9 +1045,11 @@ class AutogradCaptureFunction(torch.autograd.Function):
             grads = _grad_targets(lw, params)
             cam, keep = ctx.capture_object._camera(ctx.sensors)
             total = cam.num_sensors if ctx.sensors is not None else ctx.capture_object.num_sensors()
-            replay = _internal.backward_settings.seeds == "replay"
-            if replay:
+            mode = _internal.backward_settings.seeds
+            if mode == "replay":
                 seeds, samples, seed_mode = ctx.seeds, ctx.fw_samples, L.VR_SEED_CAPTURE_FW
+            elif mode == "fresh":
+                seeds, samples, seed_mode = _internal.get_seeds(), ctx.bw_samples, L.VR_SEED_CAPTURE_FW
             else:
                 seeds, samples, seed_mode = _internal.get_seeds(), ctx.bw_samples, L.VR_SEED_CAPTURE_BW
             og = og.to(torch.float32).contiguous()

@@ -209,3 +209,51 @@ def test_minibatched_inverse_render_with_tv_reduces_loss(gpu):
               flush=True)
         assert after < 0.85 * before
     assert tvs[1] < 0.5 * tvs[0]
+
+
+def test_l2_loss_gradient_fresh_seeds_are_unbiased_replayed_seeds_are_not(gpu):
+    """The L2 image loss L = sum((I_hat - T)^2) is quadratic in the estimate: a
+    backward that replays the forward's samples estimates
+    2 E[(I_hat - T) G] = d (E I_hat - T)^2 + 2 Cov(I_hat, G), a backward with
+    fresh seeds (the reference's, the default) 2 (E I_hat - T) E[G] = d J with
+    J = sum((E I_hat - T)^2).  At 1 spp, along the uniform direction of the
+    density: the mean fresh gradient equals the central difference of J (E
+    I_hat from 32768-spp renders with the same seeds on both sides), the mean
+    replayed one does not."""
+    dev = torch.device("cuda", 0)
+    sigma0 = _grid(6, seed=4, lo=1.0, hi=3.0)
+    poses = torch.stack([rv.functions.look_at_pose((0.3, 0.4, 2.0)), rv.functions.look_at_pose((-1.8, 0.2, 0.9))]).to(dev)
+    camera = rv.PerspectiveCameraSensor(16, 16, poses, jittered=True)
+
+    def make(sig):
+        return inverse.single_scatter_scene(sig, albedo=(0.8, 0.8, 0.8), g=0.3, env=(1.0, 0.8, 0.6), majorant=5.0).radiance
+
+    target = torch.full((2, 16, 16, 3), 0.1, device=dev, dtype=torch.float64)
+    h = 0.05
+    J = []
+    for sgn in (1.0, -1.0):
+        f = make(torch.nn.Parameter((sigma0 + sgn * h).to(dev), requires_grad=False))
+        rv.seed(77)
+        with torch.no_grad():
+            img = camera.capture(f, fw_samples=32768).double()
+        J.append(float(((img - target) ** 2).sum()))
+    fd = (J[0] - J[1]) / (2 * h)
+    sigma = torch.nn.Parameter(sigma0.clone().to(dev), requires_grad=True)
+    field = make(sigma)
+    means, ses = {}, {}
+    for mode in ("fresh", "reference", "replay"):
+        vals = []
+        with rv.backward_mode(seeds=mode):
+            for b in range(1500):
+                rv.seed(5000 + b)
+                img = camera.capture(field, fw_samples=1, bw_samples=1)
+                ((img.double() - target) ** 2).sum().backward()
+                vals.append(sigma.grad.sum())
+                sigma.grad = None
+        v = torch.stack(vals).double().cpu()
+        means[mode], ses[mode] = float(v.mean()), float(v.std() / len(vals) ** 0.5)
+    print(f"[l2 bias] fd of J {fd:.4f}; " + "; ".join(f"{m} {means[m]:.4f} +- {ses[m]:.4f}" for m in means), flush=True)
+    # fresh seeds: per-sample streams (the default) or the reference's per-sensor stream
+    for m in ("fresh", "reference"):
+        assert abs(means[m] - fd) <= 4.0 * ses[m] + 0.02 * abs(fd), m
+    assert abs(means["replay"] - fd) >= 4.0 * ses["replay"]
