@@ -473,10 +473,17 @@ def convergence_leg(dev, res=256, views=8, hw=256, spp=16, steps=50):
             "fd_rel_err_max": max(c["rel_err"] for c in checks), "fd_z_max": max(c["z"] for c in checks)}
 
 
-def nee_convergence_leg(dev, res=512, ares=128, views=4, hw=512, spp=32, steps=100):
-    """One GPU's share (4 of 32 views) of the 512^3 + 128^3-albedo multiple-
-    scattering inverse render run for 100 Adam steps; loss curve and the adjoint
-    (density) against central differences along 4 smooth directions."""
+def nee_convergence_leg(dev, res=512, ares=128, views=32, hw=512, spp=32, steps=100, check_views=8, check_spp=32):
+    """The multiple-scattering inverse render at its stated size: a 512^3 density
+    and a 128^3 RGB albedo optimised from 32 views at 512x512, 32 spp, NEE with
+    delta tracking (max 8 bounces), backward by defensive path replay from the
+    dispatch seeds (fresh seeds, no stored vertices), targets rendered from
+    seeded fBm grids, Adam (lr 1e-2) for 100 steps.  Reports the training loss
+    of every step, a low-noise loss (4 views at 256 spp against a 256-spp
+    target) before and after, the step time split into forward, backward and
+    Adam, and the adjoint against central differences (h and h/2 with Richardson
+    extrapolation, same seeds on both sides, fixed majorant) along smooth
+    directions of the density and of the albedo on 8 of the views."""
     import torch
     import rendervous_amd as rv
     from rendervous_amd import inverse
@@ -486,38 +493,71 @@ def nee_convergence_leg(dev, res=512, ares=128, views=4, hw=512, spp=32, steps=1
         ares, ares, ares, 3).contiguous()
     poses = rv.functions.orbit_poses(32, radius=2.5)[:views].to(dev)
     camera = rv.PerspectiveCameraSensor(hw, hw, poses, jittered=True)
+    eval_camera = rv.PerspectiveCameraSensor(hw, hw, poses[::max(1, views // 4)][:4].contiguous(), jittered=True)
     rv.seed(5)
     with torch.no_grad():
-        target = camera.capture(_nee_scene(t_sigma, t_albedo, 8), fw_samples=spp)
-    del t_sigma, t_albedo
+        target_field = _nee_scene(t_sigma, t_albedo, 8)
+        target = camera.capture(target_field, fw_samples=spp)
+        rv.seed(6)
+        target_hi = eval_camera.capture(target_field, fw_samples=256)
+    del t_sigma, t_albedo, target_field
     sigma = torch.nn.Parameter(torch.full((res, res, res, 1), 5.0, device=dev), requires_grad=True)
     albedo = torch.nn.Parameter(torch.full((ares, ares, ares, 3), 0.7, device=dev), requires_grad=True)
+    check_camera = rv.PerspectiveCameraSensor(hw, hw, poses[:check_views].contiguous(), jittered=True)
     albedo.requires_grad_(False)
-    checks = _directional_check(camera, lambda s: _nee_scene_fixed(s, albedo, 8, 7.0), sigma, 64)
+    checks = _directional_check(check_camera, lambda s: _nee_scene_fixed(s, albedo, 8, 7.0), sigma, check_spp, n_dirs=2,
+                                h=0.25)
     albedo.requires_grad_(True)
+    sigma.requires_grad_(False)
+    checks_albedo = _directional_check(check_camera, lambda a: _nee_scene_fixed(sigma, a, 8, 7.0), albedo, check_spp,
+                                       n_dirs=2, h=0.02)
+    sigma.requires_grad_(True)
     field = _nee_scene(sigma, albedo, 8)
+
+    def eval_loss():
+        with torch.no_grad():
+            rv.seed(7)
+            return float(((eval_camera.capture(field, fw_samples=256) - target_hi) ** 2).mean())
+
+    eval_first = eval_loss()
     opts = [(sigma, inverse.FusedAdam(sigma, lr=1e-2), (0.0, math.inf)),
             (albedo, inverse.FusedAdam(albedo, lr=1e-2), (0.0, 1.0))]
-    losses = []
+    stream = torch.cuda.current_stream(dev)
+    losses, fw, bw, ad = [], [], [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
+        e = _events(4)
+        e[0].record(stream)
         img = camera.capture(field, fw_samples=spp, bw_samples=spp)
         loss = ((img - target) ** 2).mean()
+        e[1].record(stream)
         loss.backward()
+        e[2].record(stream)
         for p, opt, bounds in opts:
             opt.step(p.grad, zero_grad=False, bounds=bounds)
             p.grad = None
+        e[3].record(stream)
         losses.append(loss.detach())
+        fw.append(e)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    fw_ms = sum(e[0].elapsed_time(e[1]) for e in fw) / steps
+    bw_ms = sum(e[1].elapsed_time(e[2]) for e in fw) / steps
+    adam_ms = sum(e[2].elapsed_time(e[3]) for e in fw) / steps
     losses = [float(x) for x in losses]
-    return {"workload": f"{res}^3 density + {ares}^3 RGB albedo, {views} of 32 views {hw}x{hw}, {spp} spp, NEE + DT, "
-                        f"max 8 bounces, replay backward, Adam lr 1e-2, {steps} steps; gradient check at 64 spp, "
-                        f"4 seed batches",
-            "loss_curve": losses[::5] + [losses[-1]], "loss_first": losses[0], "loss_last": losses[-1], "wall_s": wall,
-            "ms_per_step": wall * 1e3 / steps, "directional_fd": checks,
-            "fd_rel_err_max": max(c["rel_err"] for c in checks), "fd_z_max": max(c["z"] for c in checks)}
+    samples = views * hw * hw * spp
+    return {"workload": f"{res}^3 density + {ares}^3 RGB albedo from 5.0 / 0.7, {views} views {hw}x{hw}, {spp} spp, "
+                        f"NEE + DT, max 8 bounces, defensive-replay backward (fresh seeds), Adam lr 1e-2, {steps} steps; "
+                        f"gradient checks on {check_views} views at {check_spp} spp, 4 seed batches",
+            "loss_curve": losses, "loss_first": losses[0], "loss_last": losses[-1],
+            "eval_loss_256spp_first": eval_first, "eval_loss_256spp_last": eval_loss(),
+            "wall_s": wall, "ms_per_step": wall * 1e3 / steps, "fw_ms": fw_ms, "bw_ms": bw_ms, "adam_ms": adam_ms,
+            "fw_samples_per_s": samples / (fw_ms * 1e-3), "bw_samples_per_s": samples / (bw_ms * 1e-3),
+            "directional_fd_density": checks, "directional_fd_albedo": checks_albedo,
+            "fd_rel_err_max": max(c["rel_err"] for c in checks + checks_albedo),
+            "fd_rel_err_richardson_max": max(c["rel_err_richardson"] for c in checks + checks_albedo),
+            "fd_z_richardson_max": max(c["z_richardson"] for c in checks + checks_albedo)}
 
 
 def _nee_scene_fixed(sigma, albedo, max_bounces, majorant):
@@ -1041,6 +1081,10 @@ def main():
         result["secondary"]["scene"] = _quiet(scene_leg, dev)
         torch.cuda.empty_cache()
         result["secondary"]["convergence"] = _quiet(convergence_leg, dev)
+        torch.cuda.empty_cache()
+        result["secondary"]["gradient_gate"] = _quiet(gradient_gate_leg, dev)
+        torch.cuda.empty_cache()
+        result["secondary"]["nee_inverse_32_views"] = _quiet(nee_convergence_leg, dev)
         log(f"secondary: {json.dumps(result['secondary'])}")
     if not args.no_cpu_baseline:
         m = build_scene(args.res, args.width, args.height, dev)[1]
