@@ -6,9 +6,8 @@
  * Every entry point is extern "C", takes plain pointers and sizes, and returns
  * an int status (VR_OK == 0).  The caller owns every buffer; the library never
  * allocates or frees caller memory.  All device work is ordered on the caller's
- * HIP stream (`stream`, a hipStream_t; NULL = the default stream).  No entry
- * point synchronises the device; vr_majorant_build alone waits for its stream,
- * to report a non-finite density.
+ * HIP stream (`stream`, a hipStream_t; NULL = the default stream) and no entry
+ * point synchronises the device.
  *
  * Limits: a capture draws the random stream of sample s of sensor i from the
  * 32-bit flat index s * sensors + i, so spp * sensors must not exceed 2^32
@@ -304,9 +303,10 @@ int vr_grid_bw(const vr_grid* grid, const float* x, const float* out_grad, int64
 /* Coarse majorant grid: out[rz,ry,rx] = max of sigma over the vertices whose
  * trilinear footprint touches each coarse cell.  Implements the 6->2 majorant
  * contract (include/vr_majorant.h:1-7) that the reference fills with
- * const[max(sigma), 1e6] (_objects.py:96-101).  Waits for `stream` and returns
- * VR_ERR_ARG if sigma holds a NaN or an infinity (an infinite majorant would
- * give zero-length free flights that never end a ray). */
+ * const[max(sigma), 1e6] (_objects.py:96-101).  A cell whose footprint holds a
+ * NaN or infinite density gets a NaN bound, which every tracker treats as an
+ * empty cell (an infinite bound would give zero-length free flights that never
+ * end a ray); such densities are the caller's error, not reported here. */
 int vr_majorant_build(const vr_grid* sigma, const int32_t res[3], float* out, void* stream);
 
 /* Map forward on rays: in[n,IN] -> out[n,OUT] for the map `kind`

@@ -690,6 +690,7 @@ static double t_grid_tracking(ctx_t* c, v3 x, v3 w, int ratio) {
         double ct = cur;
         T *= exp(-mn * (next - cur));
         double r = dmax(0.000001, mx - mn);
+        if (!(r < INFINITY)) return 0.0;  /* an infinite density in the cell absorbs */
         for (;;) {
             double dt = -log(1.0 - rnd(c)) / r;
             if (ct + dt > next) break;
@@ -1805,13 +1806,17 @@ int oracle_majorant_build(const vr_grid* g, const int32_t res[3], float* out) {
                     hi[a] = imin(n[a], (int)ceil((double)(c[a] + 1) * n[a] / r[a]) + 1);
                 }
                 float m = 0.0f;
+                int nonfinite = 0;
                 for (int z = lo[2]; z <= hi[2]; z++)
                     for (int y = lo[1]; y <= hi[1]; y++)
                         for (int x = lo[0]; x <= hi[0]; x++) {
                             float v = t->data[goff(t, z, y, x)];
-                            if (v > m) m = v;
+                            if (!isfinite(v)) nonfinite = 1;
+                            else if (v > m) m = v;
                         }
-                out[((int64_t)cz * ry + cy) * rx + cx] = m;
+                /* a NaN bound for a footprint with a NaN or infinite density: the
+                 * trackers floor the majorant, so the cell is crossed as empty */
+                out[((int64_t)cz * ry + cy) * rx + cx] = nonfinite ? NAN : m;
             }
     return VR_OK;
 }

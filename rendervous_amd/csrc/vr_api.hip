@@ -324,10 +324,12 @@ __global__ void __launch_bounds__(kBlock) k_grid_bw(GridView g, const float* __r
 }
 
 // One workgroup per coarse cell: max over the vertices whose trilinear
-// footprint can reach the cell (widened by one vertex per side).  A NaN or
-// infinite vertex sets *bad.
-__global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry, int rx, float* __restrict__ out,
-                                                     int* __restrict__ bad) {
+// footprint can reach the cell (widened by one vertex per side).  A cell whose
+// footprint holds a NaN or infinite vertex gets a NaN bound: every tracker
+// floors the majorant (fmaxf(floor, bound) is the floor for a NaN bound), so
+// its free flights cross such a cell in one step instead of stalling on an
+// infinite majorant.
+__global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry, int rx, float* __restrict__ out) {
     int cell = blockIdx.x;
     int cx = cell % rx, cy = (cell / rx) % ry, cz = cell / (rx * ry);
     int n[3] = {g.nx - 1, g.ny - 1, g.nz - 1}, r[3] = {rx, ry, rz}, c[3] = {cx, cy, cz}, lo[3], hi[3];
@@ -347,12 +349,12 @@ __global__ void __launch_bounds__(kBlock) k_majorant(GridView g, int rz, int ry,
     }
     __shared__ float red[kBlock];
     red[threadIdx.x] = m;
-    if (__syncthreads_or(nonfinite) && threadIdx.x == 0) atomicOr(bad, 1);
+    nonfinite = __syncthreads_or(nonfinite);
     for (int s = kBlock / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[cell] = red[0];
+    if (threadIdx.x == 0) out[cell] = nonfinite ? __builtin_nanf("") : red[0];
 }
 
 // Resident waves per SIMD the generic capture and map kernels are compiled for.
@@ -1163,12 +1165,7 @@ int64_t camera_sensors(const vr_camera* c) {
     return c->sensors ? c->num_sensors : (int64_t)c->num_poses * c->width * c->height;
 }
 
-// Flag set by k_majorant for a non-finite density, read back after the build:
-// one device word and one pinned host word per device, used under the mutex.
 constexpr int kMaxDevices = 64;
-std::mutex g_flag_mu;
-int* g_flag_dev[kMaxDevices] = {};
-int* g_flag_host[kMaxDevices] = {};
 
 // Work counters of vr_capture_fw_noalloc: a file-scope device array (one copy
 // per GPU, held until the process ends) instead of a workspace.  In host code
@@ -1270,32 +1267,9 @@ int vr_majorant_build(const vr_grid* sigma, const int32_t res[3], float* out, vo
     if (!res || !out) return set_err(VR_ERR_ARG, "vr_majorant_build: null argument");
     if (res[0] < 1 || res[1] < 1 || res[2] < 1 || (int64_t)res[0] * res[1] * res[2] > (1 << 24))
         return set_err(VR_ERR_ARG, "vr_majorant_build: resolution out of range");
-    int dev = 0;
-    if ((st = check_hip(hipGetDevice(&dev), "hipGetDevice"))) return st;
-    if (dev < 0 || dev >= kMaxDevices) return set_err(VR_ERR_DEVICE, "vr_majorant_build: device index out of range");
-    std::lock_guard<std::mutex> lock(g_flag_mu);
-    if (!g_flag_dev[dev]) {
-        void* d = nullptr;
-        void* h = nullptr;
-        if ((st = check_hip(hipMalloc(&d, sizeof(int)), "hipMalloc(majorant flag)"))) return st;
-        if ((st = check_hip(hipHostMalloc(&h, sizeof(int), hipHostMallocDefault), "hipHostMalloc(majorant flag)"))) {
-            (void)hipFree(d);
-            return st;
-        }
-        g_flag_dev[dev] = (int*)d;
-        g_flag_host[dev] = (int*)h;
-    }
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (uint32_t*)g_flag_dev[dev], 1);
-    hipLaunchKernelGGL(k_majorant, dim3(res[0] * res[1] * res[2]), dim3(kBlock), 0, s, g, res[0], res[1], res[2], out,
-                       g_flag_dev[dev]);
-    if ((st = check_hip(hipGetLastError(), "vr_majorant_build launch"))) return st;
-    if ((st = check_hip(hipMemcpyAsync(g_flag_host[dev], g_flag_dev[dev], sizeof(int), hipMemcpyDeviceToHost, s),
-                        "vr_majorant_build flag copy")))
-        return st;
-    if ((st = check_hip(hipStreamSynchronize(s), "vr_majorant_build synchronize"))) return st;
-    if (*g_flag_host[dev]) return set_err(VR_ERR_ARG, "vr_majorant_build: sigma holds NaN or infinite values");
-    return VR_OK;
+    hipLaunchKernelGGL(k_majorant, dim3(res[0] * res[1] * res[2]), dim3(kBlock), 0, (hipStream_t)stream, g, res[0], res[1],
+                       res[2], out);
+    return check_hip(hipGetLastError(), "vr_majorant_build launch");
 }
 
 int vr_map_fw(const vr_medium* medium, int32_t kind, const float* in, int64_t n, float* out, const vr_launch* L, void* stream) {

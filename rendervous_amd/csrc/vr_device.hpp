@@ -792,6 +792,7 @@ struct Tracer {
             }
             T *= expf(-mn * (float)(nxt - cur));
             const float r = fmaxf(0.000001f, mx - mn);
+            if (!(r < kInf)) return 0.0f;  // an infinite density in the cell absorbs (and ends the walk)
             double ct = cur;
             for (;;) {
                 const float dt = flight(r);
@@ -1513,7 +1514,7 @@ struct Tracer {
         mj.begin(M, x, w);
         while (d > EPS) {
             float md;
-            float m = mj.query(M, md) * 1.05f + 0.1f;
+            float m = fmaxf(0.1f, mj.query(M, md) * 1.05f + 0.1f);  // a NaN cell bound counts as empty
             float t = flight(m);
             x = x + w * fminf(t, md);
             if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
@@ -1548,7 +1549,7 @@ struct Tracer {
         mj.begin(M, x, w);
         while (d > EPS) {
             float md;
-            float m = mj.query(M, md) * 1.05f + 0.1f;
+            float m = fmaxf(0.1f, mj.query(M, md) * 1.05f + 0.1f);  // a NaN cell bound counts as empty
             float t = flight(m);
             x = x + w * fminf(t, md);
             if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }

@@ -353,6 +353,12 @@ class MajorantGrid(MapBase):
     def majorant_tensor(self) -> torch.Tensor:
         g = self.sigma_grid.grid
         if self._cache is None or not self._cache_key.matches(g):
+            if self._cache_key is None or self._cache_key.ref() is not g:
+                # a density first seen here: NaN or infinite values would leave the
+                # coarse bounds NaN (cells crossed as empty) -- refuse them once
+                # (one host synchronisation per new tensor, none per update)
+                if not bool(torch.isfinite(g.detach()).all()):
+                    raise ValueError("the density holds NaN or infinite values")
             out = torch.empty(*self.resolution, device=g.device, dtype=torch.float32)
             res = (L.ctypes.c_int32 * 3)(*self.resolution)
             L.check(L.load().vr_majorant_build(self.sigma_grid.view(), res, L.ptr(out), L.stream_handle(g.device)),

@@ -194,3 +194,14 @@ def test_capture_workspace_must_fit(nbytes, offset):
     lau.workspace_bytes = nbytes
     st = lib.vr_capture_fw(b.medium, L.VR_MAP_NEE_DT, cam.c, lau, O.nptr(out), None)
     assert st == L.VR_ERR_ARG and b"workspace" in lib.vr_last_error()
+
+
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+def test_map_layer_refuses_a_non_finite_density(bad):
+    """The map layer checks a density once, when it first builds its majorant
+    grid, before any library call."""
+    sigma = torch.rand(9, 9, 9, 1)
+    sigma[4, 4, 4, 0] = bad
+    g = rv.Grid3D(sigma, *rv.normalized_box(sigma))
+    with pytest.raises(ValueError, match="NaN or infinite"):
+        rv.MajorantGrid(g, 4).majorant_tensor()
