@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define VR_ABI_VERSION 2
+#define VR_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum vr_status {
@@ -172,6 +172,13 @@ typedef struct vr_medium {
     int32_t env_reserved;
     const float* sigma_cells;      /* optional cell table of sigma (vr_grid_pack_cells); NULL = read
                                       the grid.  Lookups give bit-identical values either way. */
+    int32_t phase_g_kind;          /* vr_field_kind: VR_FIELD_CONST = phase_g everywhere;
+                                      VR_FIELD_GRID = phase_g_grid read at each scattering
+                                      position (HGPhase / VHGPhaseSampler evaluate their phase_g
+                                      map there, _maps.py:4449,4473; no gradient flows into it,
+                                      the reference's HGPhase has no backward) */
+    int32_t phase_reserved;
+    vr_grid phase_g_grid;          /* C == 1, used when phase_g_kind == VR_FIELD_GRID */
 } vr_medium;
 
 /* PerspectiveCameraSensor (_maps.py:1765-1786, include/maps/sensor_perspective_camera.h). */

@@ -345,31 +345,42 @@ static double hg_eval(double cos_t, double g) {
     return 0.25 / PI_D * (1.0 - g2) / pow(1.0 + g2 - 2 * g * cos_t, 1.5);
 }
 
+/* The HG asymmetry at scattering position x: the constant phase_g or the
+ * phase_g grid read there (HGPhase / VHGPhaseSampler evaluate their phase_g map
+ * at the first three inputs, _maps.py:4449,4473). */
+static double phase_g_at(const vr_medium* M, v3 x) {
+    if (M->phase_g_kind != VR_FIELD_GRID || M->phase_kind == VR_PHASE_ISO) return M->phase_g;
+    double g;
+    grid_lookup(&M->phase_g_grid, x, &g);
+    return g;
+}
+
 /* phase(x, w, wo): HGPhase (_maps.py:4446-4454) / single HG / isotropic. */
-static double phase_eval(const vr_medium* M, v3 w, v3 wo) {
+static double phase_eval(const vr_medium* M, v3 x, v3 w, v3 wo) {
     double ct = vdot(w, wo);
+    double g = phase_g_at(M, x);
     switch (M->phase_kind) {
     case VR_PHASE_HG_MIX: {
-        double a = hg_eval(ct, M->phase_g), b = hg_eval(ct, M->phase_g / 5);
+        double a = hg_eval(ct, g), b = hg_eval(ct, g / 5);
         return a * (1 - 0.2) + b * 0.2;
     }
-    case VR_PHASE_HG: return hg_eval(ct, M->phase_g);
+    case VR_PHASE_HG: return hg_eval(ct, g);
     default: return 0.25 / PI_D;
     }
 }
 
 /* phase_sampler(x, w) -> (rho, wo): VHGPhaseSampler (_maps.py:4463-4481) /
  * HGPhaseSampler (_scenes.py:1984-2004) / uniform. */
-static double phase_sample(ctx_t* c, v3 w, v3* wo) {
+static double phase_sample(ctx_t* c, v3 x, v3 w, v3* wo) {
     const vr_medium* M = c->M;
+    double g = phase_g_at(M, x);
     switch (M->phase_kind) {
     case VR_PHASE_HG_MIX: {
-        double g = M->phase_g;
         if (rnd(c) < 0.8) g /= 5;
         *wo = hg_sample(c, w, g);
         return 1.0;
     }
-    case VR_PHASE_HG: *wo = hg_sample(c, w, M->phase_g); return 1.0;
+    case VR_PHASE_HG: *wo = hg_sample(c, w, g); return 1.0;
     default: *wo = random_direction(c, V(0, 1, 0)); return 1.0;
     }
 }
@@ -989,7 +1000,7 @@ static void collision_dt_bw(ctx_t* c, v3 x, v3 w, double eps, double dL_dC, doub
 static v3 ss_radiance(ctx_t* c, v3 x, v3 w) {
     v3 wo;
     v3 env = env_sample(c, &wo);
-    double ph = phase_eval(c->M, vs(w, -1.0), wo);
+    double ph = phase_eval(c->M, x, vs(w, -1.0), wo);
     v3 sa = albedo_at(c, x);
     double T = transmittance(c, x, wo);
     return vs(vmul(env, sa), ph * T);
@@ -999,7 +1010,7 @@ static void ss_radiance_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
     rng_t saved = c->rng;
     v3 wo;
     v3 env = env_sample(c, &wo);
-    double ph = phase_eval(c->M, vs(w, -1.0), wo);
+    double ph = phase_eval(c->M, x, vs(w, -1.0), wo);
     v3 sa = albedo_at(c, x);
     rng_t before_T = c->rng;
     double T = transmittance(c, x, wo);
@@ -1098,11 +1109,11 @@ static v3 nee_fw(ctx_t* c, v3 x, v3 w) {
         v3 wnee;
         v3 env = env_sample(c, &wnee);
         double Tn = transmittance(c, x, wnee);
-        R = vadd(R, vs(vmul(W, env), Tn * phase_eval(M, w, wnee)));
+        R = vadd(R, vs(vmul(W, env), Tn * phase_eval(M, x, w, wnee)));
         bounces++;
         if (M->max_bounces >= 0 && bounces >= M->max_bounces) return R;
         v3 wo;
-        double rho = phase_sample(c, w, &wo);
+        double rho = phase_sample(c, x, w, &wo);
         W = vs(W, rho);
         if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
         w = wo;
@@ -1151,14 +1162,14 @@ static void nee_ds_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
         v3 wnee;
         v3 env = env_sample(c, &wnee);
         double Tn = transmittance(c, x, wnee);
-        double phn = phase_eval(M, w, wnee);
+        double phn = phase_eval(M, x, w, wnee);
         added = vadd(added, vs(vmul(W, env), Tn * phn));
         if (null_W < EPS) R_after_sp = vadd(R_after_sp, added);
         else R = vadd(R, added);
         bounces++;
         if (M->max_bounces >= 0 && bounces >= M->max_bounces) { W = V(0, 0, 0); break; }
         v3 wo;
-        double rho = phase_sample(c, w, &wo);
+        double rho = phase_sample(c, x, w, &wo);
         W = vs(W, rho);
         if (W.x < EPS && W.y < EPS && W.z < EPS) { W = V(0, 0, 0); break; }
         w = wo;
@@ -1207,10 +1218,10 @@ static void nee_ds_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
         v3 env = env_sample(c, &wnee);
         rng_t before_T = c->rng;
         double Tn = transmittance(c, x, wnee);
-        double phn = phase_eval(M, w, wnee);
+        double phn = phase_eval(M, x, w, wnee);
         rng_t before_ps = c->rng;
         v3 wo;
-        double rho = phase_sample(c, w, &wo);
+        double rho = phase_sample(c, x, w, &wo);
         rng_t before_next = c->rng;
         W = vmul(W, sa);
         double dL_dTn = vdot(dL_dR, vs(vmul(W, env), phn));
@@ -1260,7 +1271,7 @@ static v3 path_dt_fw(ctx_t* c, v3 x, v3 w) {
         R = vadd(R, vmul(vmul(W, V(1 - sa.x, 1 - sa.y, 1 - sa.z)), em));
         W = vmul(W, sa);
         v3 wo;
-        double rho = phase_sample(c, w, &wo);
+        double rho = phase_sample(c, x, w, &wo);
         W = vs(W, rho);
         bounces++;
         if (M->max_bounces >= 0 && bounces >= M->max_bounces) return R;
@@ -1308,7 +1319,7 @@ static void path_dt_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
         } else W = vs(W, (s / m) / Pc);
         v3 sa = albedo_at(c, x);
         v3 wo;
-        double rho = phase_sample(c, w, &wo);
+        double rho = phase_sample(c, x, w, &wo);
         W = vs(vmul(W, sa), rho);
         bounces++;
         if (M->max_bounces >= 0 && bounces >= M->max_bounces) { W = V(0, 0, 0); break; }
@@ -1345,7 +1356,7 @@ static void path_dt_bw(ctx_t* c, v3 x, v3 w, double eps, v3 dL_dR) {
         sigma_bw(c, x, vdot(dL_dW, s == 0.0 ? W_out : vs(W_out, null_W / s)));
         if (s == 0.0) break;
         v3 wo;
-        phase_sample(c, w, &wo);
+        phase_sample(c, x, w, &wo);
         bounces++;
         if (M->max_bounces >= 0 && bounces >= M->max_bounces) return;
         w = wo;
@@ -1398,11 +1409,11 @@ static v3 nee_scatters(ctx_t* c, v3 x, v3 w) {
         v3 wnee;
         v3 env = env_sample(c, &wnee);
         double Tn = transmittance(c, x, wnee);
-        R = vadd(R, vs(vmul(W, env), Tn * phase_eval(M, w, wnee)));
+        R = vadd(R, vs(vmul(W, env), Tn * phase_eval(M, x, w, wnee)));
         bounces++;
         if (M->max_bounces >= 0 && bounces >= M->max_bounces) return R;
         v3 wo;
-        double rho = phase_sample(c, w, &wo);
+        double rho = phase_sample(c, x, w, &wo);
         W = vs(W, rho);
         if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
         w = wo;
@@ -1488,12 +1499,12 @@ static void nee_drt_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
         v3 wnee;
         v3 env = env_sample(c, &wnee);
         double Tn = transmittance(c, x, wnee);
-        double phn = phase_eval(M, w, wnee);
+        double phn = phase_eval(M, x, w, wnee);
         R = vadd(R, vs(vmul(W, env), Tn * phn));
         bounces++;
         if (M->max_bounces >= 0 && bounces >= M->max_bounces) break;
         v3 wo;
-        double rho = phase_sample(c, w, &wo);
+        double rho = phase_sample(c, x, w, &wo);
         W = vs(W, rho);
         w = wo;
         boundary(M, x, w, &tmin, &d);
@@ -1534,11 +1545,11 @@ static void nee_drt_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
         v3 wnee;
         v3 env = env_sample(c, &wnee);
         double Tn = transmittance(c, x, wnee);
-        double phn = phase_eval(M, w, wnee);
+        double phn = phase_eval(M, x, w, wnee);
         bounces++;
         const int last = M->max_bounces >= 0 && bounces >= M->max_bounces;
         v3 wo = w;
-        double rho = last ? 1.0 : phase_sample(c, w, &wo);
+        double rho = last ? 1.0 : phase_sample(c, x, w, &wo);
         double dL_dTn = vdot(dL_dR, vs(vmul(W, env), phn));
         R = vsub(R, vs(vmul(W, env), Tn * phn));
         main_s = c->rng; c->rng = secondary;
@@ -1561,9 +1572,9 @@ static void nee_drt_bw(ctx_t* c, v3 x, v3 w, v3 dL_dR) {
     v3 xs = vadd(res_x, vs(res_w, s_t));
     v3 lw;
     v3 Lenv = env_sample(c, &lw);
-    v3 Ls = vs(Lenv, transmittance(c, xs, lw) * phase_eval(M, res_w, lw));
+    v3 Ls = vs(Lenv, transmittance(c, xs, lw) * phase_eval(M, xs, res_w, lw));
     v3 ws;
-    double rho_s = phase_sample(c, res_w, &ws);
+    double rho_s = phase_sample(c, xs, res_w, &ws);
     Ls = vadd(Ls, vs(nee_scatters(c, xs, ws), rho_s));
     Ls = vmul(vs(vmul(Ls, res_W), weight_t), albedo_at(c, xs));
     double sxs = sigma_at(c, xs);
@@ -1590,7 +1601,7 @@ static void medium_dt_fw(ctx_t* c, v3 x, v3 w, double d, double* out) {
         if (rnd(c) < s / maj) {
             if (M->medium_filter & 1) {
                 xo = x;
-                (void)phase_sample(c, w, &wo);
+                (void)phase_sample(c, x, w, &wo);
                 W = albedo_at(c, x);
             } else W = V(0, 0, 0);
             if (M->medium_filter & 2) A = vf(M->gathering_c);
@@ -1620,7 +1631,7 @@ static void medium_dt_bw(ctx_t* c, v3 x, v3 w, double d, v3 dL_dW, v3 dL_dA) {
         d -= t;
         double s = sigma_at(c, x);
         if (rnd(c) < s / maj) {
-            if (f & 1) { v3 wo; double wt = phase_sample(c, w, &wo); W = vs(albedo_at(c, x), wt); }
+            if (f & 1) { v3 wo; double wt = phase_sample(c, x, w, &wo); W = vs(albedo_at(c, x), wt); }
             else W = V(0, 0, 0);
             if (f & 2) A = vf(M->gathering_c);
             break;
@@ -1651,7 +1662,7 @@ static void medium_dt_replay(ctx_t* c, v3 x, v3 w, double d, v3 W, v3 A, v3 dL_d
             double g = 0;
             if (scat_blk) {
                 v3 wo;
-                (void)phase_sample(c, w, &wo);
+                (void)phase_sample(c, x, w, &wo);
                 g += vdot(dL_dW, vs(W, 1.0 / s));
                 albedo_bw(c, x, dL_dW);
             }
@@ -1739,6 +1750,11 @@ static int check_medium(const vr_medium* M, int kind) {
         return fail(VR_ERR_ARG, "unknown environment sampler kind");
     if (M->albedo_kind == VR_FIELD_GRID && (!M->albedo.t.data || M->albedo.t.ndim != 4))
         return fail(VR_ERR_ARG, "albedo grid must be [D,H,W,C]");
+    if (M->phase_g_kind != VR_FIELD_CONST && M->phase_g_kind != VR_FIELD_GRID)
+        return fail(VR_ERR_ARG, "unknown phase_g kind");
+    if (M->phase_g_kind == VR_FIELD_GRID && M->phase_kind != VR_PHASE_ISO &&
+        (!M->phase_g_grid.t.data || M->phase_g_grid.t.ndim != 4 || M->phase_g_grid.t.shape[3] != 1))
+        return fail(VR_ERR_ARG, "phase_g grid must be [D,H,W,1]");
     return VR_OK;
 }
 
@@ -2273,10 +2289,10 @@ int oracle_phase_sample(const vr_medium* M, const float* w, const uint32_t seeds
         c.M = M;
         c.rng = seed_map(seeds, (uint32_t)i);
         v3 wi = vf(w + 3 * i), wo;
-        double rho = phase_sample(&c, wi, &wo);
+        double rho = phase_sample(&c, V(0, 0, 0), wi, &wo);
         out[5 * i + 0] = (float)wo.x; out[5 * i + 1] = (float)wo.y; out[5 * i + 2] = (float)wo.z;
         out[5 * i + 3] = (float)rho;
-        out[5 * i + 4] = (float)phase_eval(M, wi, wo);
+        out[5 * i + 4] = (float)phase_eval(M, V(0, 0, 0), wi, wo);
     }
     return VR_OK;
 }
@@ -2287,7 +2303,7 @@ int oracle_phase_eval(const vr_medium* M, const double* w, const double* wo, int
     if (st) return st;
     if (!w || !wo || !out || n < 0) return fail(VR_ERR_ARG, "bad phase_eval arguments");
     for (int64_t i = 0; i < n; i++)
-        out[i] = phase_eval(M, V(w[3 * i], w[3 * i + 1], w[3 * i + 2]), V(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]));
+        out[i] = phase_eval(M, V(0, 0, 0), V(w[3 * i], w[3 * i + 1], w[3 * i + 2]), V(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]));
     return VR_OK;
 }
 

@@ -115,6 +115,9 @@ class VrMedium(ctypes.Structure):
         ("env_quadtree_levels", ctypes.c_int32),
         ("env_reserved", ctypes.c_int32),
         ("sigma_cells", ctypes.c_void_p),
+        ("phase_g_kind", ctypes.c_int32),
+        ("phase_reserved", ctypes.c_int32),
+        ("phase_g_grid", VrGrid),
     ]
 
 
@@ -148,7 +151,7 @@ class VrLaunch(ctypes.Structure):
 
 
 # VR_ABI_VERSION of include/vr_api.h
-ABI_VERSION = 2
+ABI_VERSION = 3
 # vr_launch.workspace of a capture: the persistent NEE kernel's queue counters
 CAPTURE_WORKSPACE_BYTES = 1024
 

@@ -154,6 +154,12 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
     if (m->phase_kind < VR_PHASE_HG_MIX || m->phase_kind > VR_PHASE_ISO) return set_err(VR_ERR_ARG, "unknown phase kind");
     v.phase_kind = m->phase_kind;
     v.g = m->phase_g;
+    v.phase_g_grid = m->phase_g_kind == VR_FIELD_GRID && m->phase_kind != VR_PHASE_ISO;
+    if (v.phase_g_grid) {
+        if ((st = make_grid_view(&m->phase_g_grid, v.phase_g, 1, "phase_g"))) return st;
+        v.phase_g.grad = nullptr;  // the phase asymmetry receives no gradient
+    } else if (m->phase_g_kind != VR_FIELD_CONST && m->phase_g_kind != VR_FIELD_GRID)
+        return set_err(VR_ERR_ARG, "unknown phase_g kind");
     v.maj_value = m->majorant_value;
     v.maj_distance = m->majorant_distance;
     // Trackers advance by free flights -log(1-u)/majorant within the majorant's
@@ -811,8 +817,8 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
                           int64_t cnt, float* out, int* dbg, unsigned long long* counter, hipStream_t st) {
     if (capture_is_persistent(kind, M)) {
         const bool envmap = M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM;
-        // cell table + constant albedo (DT shadow rays): the lean specialisation
-        const bool lean = M.cells && !M.albedo_grid && env_int("VR_NEE_LEAN", 1) != 0;
+        // cell table + constant albedo and phase asymmetry (DT shadow rays): the lean specialisation
+        const bool lean = M.cells && !M.albedo_grid && !M.phase_g_grid && env_int("VR_NEE_LEAN", 1) != 0;
 #define VR_NEE_ARGS M, C, sd, spp, total, start, cnt, out, dbg, counter, st
         if (M.t_kind == VR_T_DT && lean)
             return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true, true>(VR_NEE_ARGS)

@@ -178,6 +178,8 @@ struct MediumView {
     const float* quadtree; // XRQuadtreeRandomDirection densities
     int qt_levels;
     const float4* cells;   // optional sigma cell table (2 float4 per cell), else null
+    GridView phase_g;      // HG asymmetry field, read when phase_g_grid
+    int phase_g_grid;
 };
 
 struct CameraView {
@@ -363,28 +365,40 @@ VR_DEV float hg_eval(float cos_t, float g) {
     return (0.25f / kPi) * (1.0f - g2) * (r * r * r);
 }
 
-// HGPhase (_maps.py:4446-4454) / single HG / isotropic.
-VR_DEV float phase_eval(const MediumView& M, f3 w, f3 wo) {
+// The HG asymmetry at a scattering position x: the constant g, or the phase_g
+// Grid3D read at x (HGPhase and VHGPhaseSampler evaluate their phase_g map at
+// the first three inputs, _maps.py:4449,4473).
+VR_DEV float phase_g_at(const MediumView& M, f3 x) {
+    return M.phase_g_grid ? grid_lookup1(M.phase_g, x) : M.g;
+}
+
+// HGPhase (_maps.py:4446-4454) / single HG / isotropic, for asymmetry g.
+VR_DEV float phase_eval_g(const MediumView& M, float g, f3 w, f3 wo) {
     float ct = dot(w, wo);
     if (M.phase_kind == VR_PHASE_HG_MIX) {
-        float a = hg_eval(ct, M.g), b = hg_eval(ct, M.g / 5.0f);
+        float a = hg_eval(ct, g), b = hg_eval(ct, g / 5.0f);
         return a * (1.0f - 0.2f) + b * 0.2f;
     }
-    if (M.phase_kind == VR_PHASE_HG) return hg_eval(ct, M.g);
+    if (M.phase_kind == VR_PHASE_HG) return hg_eval(ct, g);
     return 0.25f / kPi;
 }
 
 // VHGPhaseSampler (_maps.py:4463-4481) / HGPhaseSampler (_scenes.py:1984-2004) / uniform.
-VR_DEV float phase_sample(const MediumView& M, Rng& rng, f3 w, f3& wo) {
+VR_DEV float phase_sample_g(const MediumView& M, Rng& rng, float g, f3 w, f3& wo) {
     if (M.phase_kind == VR_PHASE_HG_MIX) {
-        float g = M.g;
         if (rng_uniform(rng) < 0.8f) g /= 5.0f;
         wo = hg_sample(rng, w, g);
         return 1.0f;
     }
-    if (M.phase_kind == VR_PHASE_HG) { wo = hg_sample(rng, w, M.g); return 1.0f; }
+    if (M.phase_kind == VR_PHASE_HG) { wo = hg_sample(rng, w, g); return 1.0f; }
     wo = random_direction(rng, F3(0.0f, 1.0f, 0.0f));
     return 1.0f;
+}
+
+// The same at scattering position x.
+VR_DEV float phase_eval(const MediumView& M, f3 x, f3 w, f3 wo) { return phase_eval_g(M, phase_g_at(M, x), w, wo); }
+VR_DEV float phase_sample(const MediumView& M, Rng& rng, f3 x, f3 w, f3& wo) {
+    return phase_sample_g(M, rng, phase_g_at(M, x), w, wo);
 }
 
 // include/maths.h:172-178 dir2xr: (atan2(x, z) / pi, -2 asin(y) / pi) in [-1, 1]^2.
@@ -1074,7 +1088,7 @@ struct Tracer {
     VR_DEV f3 ss_radiance(f3 x, f3 w) {
         f3 wo;
         f3 env = env_sample(M, rng, wo);
-        float ph = phase_eval(M, w * -1.0f, wo);
+        float ph = phase_eval(M, x, w * -1.0f, wo);
         f3 sa = albedo(x);
         float T = transmittance(x, wo);
         return (env * sa) * (ph * T);
@@ -1084,7 +1098,7 @@ struct Tracer {
     VR_DEV void ss_radiance_bw(f3 x, f3 w, f3 dL_dR) {
         f3 wo;
         f3 env = env_sample(M, rng, wo);
-        float ph = phase_eval(M, w * -1.0f, wo);
+        float ph = phase_eval(M, x, w * -1.0f, wo);
         f3 sa = albedo(x);
         Rng before_T = rng;
         float T = transmittance(x, wo);
@@ -1165,11 +1179,11 @@ struct Tracer {
             f3 wnee;
             f3 env = env_sample(M, rng, wnee);
             float Tn = transmittance(x, wnee);
-            R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+            R = R + (W * env) * (Tn * phase_eval(M, x, w, wnee));
             bounces++;
             if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
             f3 wo;
-            float rho = phase_sample(M, rng, w, wo);
+            float rho = phase_sample(M, rng, x, w, wo);
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
             w = wo;
@@ -1216,14 +1230,14 @@ struct Tracer {
             f3 wnee;
             f3 env = env_sample(M, rng, wnee);
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, w, wnee);
+            float phn = phase_eval(M, x, w, wnee);
             added = added + (W * env) * (Tn * phn);
             if (null_W < EPS) R_sp = R_sp + added;
             else R = R + added;
             bounces++;
             if (M.max_bounces >= 0 && bounces >= M.max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
             f3 wo;
-            float rho = phase_sample(M, rng, w, wo);
+            float rho = phase_sample(M, rng, x, w, wo);
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); break; }
             w = wo;
@@ -1270,9 +1284,9 @@ struct Tracer {
             f3 env = env_sample(M, rng, wnee);
             Rng before_T = rng;
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, w, wnee);
+            float phn = phase_eval(M, x, w, wnee);
             f3 wo;
-            float rho = phase_sample(M, rng, w, wo);
+            float rho = phase_sample(M, rng, x, w, wo);
             Rng before_next = rng;
             W = W * sa;
             float dL_dTn = dot(dL_dR, (W * env) * phn);
@@ -1316,7 +1330,7 @@ struct Tracer {
             R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
             W = W * sa;
             f3 wo;
-            float rho = phase_sample(M, rng, w, wo);
+            float rho = phase_sample(M, rng, x, w, wo);
             W = W * rho;
             bounces++;
             if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
@@ -1361,7 +1375,7 @@ struct Tracer {
             } else W = W * ((s / m) / Pc);
             f3 sa = albedo(x);
             f3 wo;
-            float rho = phase_sample(M, rng, w, wo);
+            float rho = phase_sample(M, rng, x, w, wo);
             W = (W * sa) * rho;
             bounces++;
             if (M.max_bounces >= 0 && bounces >= M.max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
@@ -1398,7 +1412,7 @@ struct Tracer {
             sigma_bw(x, dot(dL_dW, s == 0.0f ? W_out : W_out * (null_W / s)));
             if (s == 0.0f) break;
             f3 wo;
-            phase_sample(M, rng, w, wo);
+            phase_sample(M, rng, x, w, wo);
             bounces++;
             if (M.max_bounces >= 0 && bounces >= M.max_bounces) return;
             w = wo;
@@ -1445,11 +1459,11 @@ struct Tracer {
             f3 wnee;
             f3 env = env_sample(M, rng, wnee);
             float Tn = transmittance(x, wnee);
-            R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+            R = R + (W * env) * (Tn * phase_eval(M, x, w, wnee));
             bounces++;
             if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
             f3 wo;
-            float rho = phase_sample(M, rng, w, wo);
+            float rho = phase_sample(M, rng, x, w, wo);
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
             w = wo;
@@ -1526,12 +1540,12 @@ struct Tracer {
             f3 wnee;
             f3 env = env_sample(M, rng, wnee);
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, w, wnee);
+            float phn = phase_eval(M, x, w, wnee);
             R = R + (W * env) * (Tn * phn);
             bounces++;
             if (M.max_bounces >= 0 && bounces >= M.max_bounces) break;
             f3 wo;
-            float rho = phase_sample(M, rng, w, wo);
+            float rho = phase_sample(M, rng, x, w, wo);
             W = W * rho;
             w = wo;
             boundary(M, x, w, tmin, d);
@@ -1570,11 +1584,11 @@ struct Tracer {
             f3 wnee;
             f3 env = env_sample(M, rng, wnee);
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, w, wnee);
+            float phn = phase_eval(M, x, w, wnee);
             bounces++;
             const bool last = M.max_bounces >= 0 && bounces >= M.max_bounces;
             f3 wo = w;
-            float rho = last ? 1.0f : phase_sample(M, rng, w, wo);
+            float rho = last ? 1.0f : phase_sample(M, rng, x, w, wo);
             float dL_dTn = dot(dL_dR, (W * env) * phn);
             R = R - (W * env) * (Tn * phn);
             main_s = rng; rng = secondary;
@@ -1596,9 +1610,9 @@ struct Tracer {
         f3 xs = res_x + res_w * s_t;
         f3 lw;
         f3 Lenv = env_sample(M, rng, lw);
-        f3 Ls = Lenv * (transmittance(xs, lw) * phase_eval(M, res_w, lw));
+        f3 Ls = Lenv * (transmittance(xs, lw) * phase_eval(M, xs, res_w, lw));
         f3 ws;
-        float rho_s = phase_sample(M, rng, res_w, ws);
+        float rho_s = phase_sample(M, rng, xs, res_w, ws);
         Ls = Ls + nee_scatters(xs, ws) * rho_s;
         Ls = ((Ls * res_W) * weight_t) * albedo(xs);
         float sxs = sigma(xs);
@@ -1622,7 +1636,7 @@ struct Tracer {
             if (rnd() < s / maj) {
                 if (M.medium_filter & 1) {
                     xo = x;
-                    phase_sample(M, rng, w, wo);
+                    phase_sample(M, rng, x, w, wo);
                     W = albedo(x);
                 } else W = F3(0.0f, 0.0f, 0.0f);
                 if (M.medium_filter & 2) A = M.gathering_c;
@@ -1650,7 +1664,7 @@ struct Tracer {
             d -= t;
             float s = sigma(x);
             if (rnd() < s / maj) {
-                if (f & 1) { f3 wo; float wt = phase_sample(M, rng, w, wo); W = albedo(x) * wt; }
+                if (f & 1) { f3 wo; float wt = phase_sample(M, rng, x, w, wo); W = albedo(x) * wt; }
                 else W = F3(0.0f, 0.0f, 0.0f);
                 if (f & 2) A = M.gathering_c;
                 break;
@@ -1680,7 +1694,7 @@ struct Tracer {
                 float g = 0.0f;
                 if (scat_blk) {
                     f3 wo;
-                    phase_sample(M, rng, w, wo);
+                    phase_sample(M, rng, x, w, wo);
                     g += dot(dL_dW, W * (1.0f / s));
                     albedo_bw(x, dL_dW);
                 }

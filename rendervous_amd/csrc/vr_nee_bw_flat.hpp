@@ -247,7 +247,7 @@ struct NeeDsBwLane {
     // subtracts the term and starts the shadow ray's backward (:170-196).
     VR_DEV void after_t() {
         const MediumView& M = tr.M;
-        const float phn = phase_eval(M, pw, rw);
+        const float phn = phase_eval(M, px, pw, rw);
         bool more = true;
         if (pass == 1) {
             added = added + (W * env) * (Tn * phn);
@@ -261,7 +261,7 @@ struct NeeDsBwLane {
             }
         }
         if (!more) return;
-        rho = phase_sample(M, tr.rng, pw, wo);
+        rho = phase_sample(M, tr.rng, px, pw, wo);
         if (pass == 1) {
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); state = kBwPathEnd; return; }

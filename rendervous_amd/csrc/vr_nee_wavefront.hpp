@@ -33,9 +33,9 @@ enum : int { kEvNone = 0, kEvBegin = 1, kEvStart = 2, kEvCollision = 3, kEvTrack
 
 // ENVMAP: environment image and/or quadtree sampler; false compiles the
 // constant-environment (uniform + directional light) sampler only.
-// LEAN: the medium has a sigma cell table and a constant albedo, so the kernel
-// compiles neither the strided grid read nor the albedo grid (fewer uniform
-// values held in scalar registers).
+// LEAN: the medium has a sigma cell table, a constant albedo and a constant
+// phase asymmetry, so the kernel compiles neither the strided grid read nor the
+// albedo and phase_g grids (fewer uniform values held in scalar registers).
 template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false>
 struct NeeLane {
     static constexpr float EPS = 0.00001f;  // vr_path_integrator_NEE.h:16
@@ -248,7 +248,7 @@ struct NeeLane {
         f3 env = ENVMAP ? env_sample(M, rng, wnee) : env_sample_const(M, rng, wnee);
         px = rx;
         pw = rw;
-        contrib = (W * env) * phase_eval(M, rw, wnee);
+        contrib = (W * env) * phase_eval_g(M, LEAN ? M.g : phase_g_at(M, px), rw, wnee);
         rw = wnee;
         mode = kModeTrack;
         bnd = 1;
@@ -272,7 +272,7 @@ struct NeeLane {
         bounces++;
         if (M.max_bounces >= 0 && bounces >= M.max_bounces) { event = kEvPathEnd; return; }
         f3 wo;
-        float rho = phase_sample(M, rng, pw, wo);
+        float rho = phase_sample_g(M, rng, LEAN ? M.g : phase_g_at(M, px), pw, wo);
         W = W * rho;
         if (W.x < EPS && W.y < EPS && W.z < EPS) { event = kEvPathEnd; return; }
         rw = wo;

@@ -444,10 +444,10 @@ VR_DEV f3 nee_cached(Tracer<MAJ>& tr, f3 x, f3 w, int K, f3& qx, f3& qw, f3& qW,
         f3 wnee;
         f3 env = env_sample(M, tr.rng, wnee);
         float Tn = tr.transmittance(x, wnee);
-        R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+        R = R + (W * env) * (Tn * phase_eval(M, x, w, wnee));
         bounces++;
         f3 wo;
-        float rho = phase_sample(M, tr.rng, w, wo);
+        float rho = phase_sample(M, tr.rng, x, w, wo);
         W = W * rho;
         if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
         w = wo;
@@ -496,11 +496,11 @@ VR_DEV int nee_record(Tracer<MAJ>& tr, f3 x, f3 w, int maxv, float* __restrict__
         f3 wnee;
         f3 env = env_sample(M, tr.rng, wnee);
         float Tn = tr.transmittance(x, wnee);
-        R = R + (W * env) * (Tn * phase_eval(M, w, wnee));
+        R = R + (W * env) * (Tn * phase_eval(M, x, w, wnee));
         bounces++;
         if (M.max_bounces >= 0 && bounces >= M.max_bounces) break;
         f3 wo;
-        float rho = phase_sample(M, tr.rng, w, wo);
+        float rho = phase_sample(M, tr.rng, x, w, wo);
         W = W * rho;
         if (W.x < EPS && W.y < EPS && W.z < EPS) break;
         w = wo;

@@ -591,15 +591,37 @@ class _Lowered:
             raise NotImplementedError("environment sampler must be uniform or quadtree over the environment, "
                                       "or a DirectionalLightEnvironmentSampler")
 
+    def set_phase_g(self, *phases: MapBase):
+        """The asymmetry shared by an HG phase and its sampler: a constant, or a
+        single-channel Grid3D that the kernels read at each scattering position
+        (HGPhase and VHGPhaseSampler evaluate phase_g at x, _maps.py:4449,4473).
+        No gradient flows into it: the reference's HGPhase has no backward."""
+        gs = [p.phase_g for p in phases]
+        if isinstance(gs[0], Grid3D):
+            grid = gs[0]
+            if any(g is not grid for g in gs):
+                raise NotImplementedError("phase and phase sampler must share one phase_g grid")
+            if grid.output_dim != 1:
+                raise NotImplementedError("phase_g grid must have one channel")
+            self.medium.phase_g_kind = L.VR_FIELD_GRID
+            self.medium.phase_g_grid = grid.view()
+            self.medium.phase_g = 0.0
+            self.keep.append(grid.grid)
+            return
+        if not all(isinstance(g, ConstantMap) for g in gs):
+            raise NotImplementedError("phase g must be a constant or a Grid3D")
+        values = [g.vec(1)[0] for g in gs]
+        if any(v != values[0] for v in values):
+            raise NotImplementedError("phase and phase sampler must use the same HG parameters")
+        self.medium.phase_g_kind = L.VR_FIELD_CONST
+        self.medium.phase_g = values[0]
+
     def set_phase(self, phase: MapBase, sampler: MapBase):
         if isinstance(phase, HGPhase) and isinstance(sampler, VHGPhaseSampler):
-            if not (isinstance(phase.phase_g, ConstantMap) and isinstance(sampler.phase_g, ConstantMap)):
-                raise NotImplementedError("phase g must be constant")
-            g1, g2 = phase.phase_g.vec(1)[0], sampler.phase_g.vec(1)[0]
-            if g1 != g2 or phase.mixture != sampler.mixture:
+            if phase.mixture != sampler.mixture:
                 raise NotImplementedError("phase and phase sampler must use the same HG parameters")
+            self.set_phase_g(phase, sampler)
             self.medium.phase_kind = L.VR_PHASE_HG_MIX if phase.mixture else L.VR_PHASE_HG
-            self.medium.phase_g = g1
         elif isinstance(phase, ConstantMap) and abs(phase.vec(1)[0] - 1.0 / (4 * math.pi)) < 1e-7 \
                 and _is_uniform_phase_sampler(sampler):
             self.medium.phase_kind = L.VR_PHASE_ISO
@@ -609,10 +631,8 @@ class _Lowered:
     def set_phase_sampler(self, sampler: MapBase):
         """A phase sampler alone (integrators that never evaluate the phase)."""
         if isinstance(sampler, VHGPhaseSampler):
-            if not isinstance(sampler.phase_g, ConstantMap):
-                raise NotImplementedError("phase g must be constant")
+            self.set_phase_g(sampler)
             self.medium.phase_kind = L.VR_PHASE_HG_MIX if sampler.mixture else L.VR_PHASE_HG
-            self.medium.phase_g = sampler.phase_g.vec(1)[0]
         elif _is_uniform_phase_sampler(sampler):
             self.medium.phase_kind = L.VR_PHASE_ISO
         else:
@@ -772,9 +792,8 @@ class SingleScatteredRadiance(_RayMap):
         lw.set_albedo(self.scattering_albedo)
         lw.set_environment_sampler(self.environment_sampler)
         if isinstance(self.phase, HGPhase):
-            g = self.phase.phase_g.vec(1)[0]
+            lw.set_phase_g(self.phase)
             lw.medium.phase_kind = L.VR_PHASE_HG_MIX if self.phase.mixture else L.VR_PHASE_HG
-            lw.medium.phase_g = g
         elif isinstance(self.phase, ConstantMap):
             lw.medium.phase_kind = L.VR_PHASE_ISO
         else:

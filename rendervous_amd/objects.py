@@ -8,12 +8,16 @@ majorant of this library.
 """
 from __future__ import annotations
 
+import collections
 import math
 import typing
+import warnings
 
 import torch
 
 from . import maps as _maps
+
+_ABSENT = object()
 
 __all__ = [
     "DependencySet", "medium_box_AABB", "medium_box_normalized", "medium_box", "medium_boundary_box", "medium_boundary",
@@ -29,61 +33,99 @@ __all__ = [
 ]
 
 
-class _Lazy:
-    """A tensor recomputed from a callable on forward_dependency (Reparameter, _internal.py:223-254)."""
+class _Derived:
+    """A field given as a callable (the reference's Reparameter, _internal.py:224-250).
+
+    The maps hold ``leaf``, a tensor of its own storage.  ``refresh`` re-runs the
+    callable and copies its value into the leaf; when that value carries a graph,
+    ``push_grad`` hands the gradient accumulated on the leaf back into it, so a
+    parameter behind the callable receives its share."""
 
     def __init__(self, fn: typing.Callable[[], torch.Tensor]):
         self.fn = fn
-        self.value = fn()
+        self.graph: typing.Optional[torch.Tensor] = None
+        value = fn()
+        self.leaf = value.detach().clone().requires_grad_(value.requires_grad)
 
     def refresh(self):
+        value = self.fn()
+        self.graph = value if value.requires_grad else None
         with torch.no_grad():
-            self.value.copy_(self.fn())
+            self.leaf.copy_(value)
+        self.leaf.requires_grad_(value.requires_grad)
+        if self.leaf.grad is not None:
+            self.leaf.grad.zero_()
+
+    def push_grad(self):
+        if self.graph is not None and self.leaf.grad is not None:
+            self.graph.backward(self.leaf.grad)
 
 
 class DependencySet:
-    """Named fields filled by builder functions (_internal.py:256-333)."""
+    """Named fields that the recipes below read and add (the interface of
+    _internal.py:256-333: add_parameters, requires, ensures, assert_ensures,
+    branch, copy_parameters, forward_dependency, backward_dependency, attribute
+    access).
+
+    The fields of a set and of its ancestors form one ``ChainMap``: a branch
+    pushes an empty layer of its own, so what it adds shadows its parent and
+    stays invisible to it, while the parent's later additions show through.
+    Fields given as callables become ``_Derived`` values kept in one registry
+    shared by the whole tree; forward_dependency from any set re-evaluates all
+    of them in the order they were added, and backward_dependency hands their
+    gradients back."""
 
     def __init__(self, parent: typing.Optional["DependencySet"] = None):
-        object.__setattr__(self, "_fields", {})
-        object.__setattr__(self, "_lazy", [])
-        object.__setattr__(self, "_parent", parent)
+        if parent is None:
+            scope, registry = collections.ChainMap(), []
+        else:
+            scope, registry = parent._scope.new_child(), parent._registry
+        object.__setattr__(self, "_scope", scope)
+        object.__setattr__(self, "_registry", registry)
 
     def ensures(self, field_name: str, field_type) -> bool:
-        return (field_name in self._fields and isinstance(self._fields[field_name], field_type)) or \
-            (self._parent is not None and self._parent.ensures(field_name, field_type))
+        """True when this set or an ancestor holds ``field_name`` as a ``field_type``."""
+        return any(isinstance(layer.get(field_name, _ABSENT), field_type) for layer in self._scope.maps)
 
     def assert_ensures(self, field_name: str, field_type):
-        assert self.ensures(field_name, field_type), f"Required field {field_name} of type {field_type} not found."
+        if not self.ensures(field_name, field_type):
+            raise AssertionError(f"the set holds no field {field_name!r} of type {field_type}")
 
     def branch(self) -> "DependencySet":
         return DependencySet(parent=self)
 
     def add_parameters(self, **kwargs):
-        for k, v in kwargs.items():
-            if not isinstance(v, torch.nn.Module) and callable(v):
-                lz = _Lazy(v)
-                self._lazy.append(lz)
-                v = lz.value
-            self._fields[k] = v
+        layer = self._scope.maps[0]
+        for name, value in kwargs.items():
+            if callable(value) and not isinstance(value, torch.nn.Module):
+                derived = _Derived(value)
+                self._registry.append(derived)
+                value = derived.leaf
+            layer[name] = value
+
+    def copy_parameters(self, ds: "DependencySet"):
+        """Adds every field visible from ``ds`` (its own layer wins over its ancestors')."""
+        self.add_parameters(**dict(ds._scope))
 
     def requires(self, builder, **kwargs):
         builder(self, **kwargs)
 
     def forward_dependency(self):
-        if self._parent is not None:
-            self._parent.forward_dependency()
-        for lz in self._lazy:
-            lz.refresh()
+        for derived in self._registry:
+            derived.refresh()
+
+    def backward_dependency(self):
+        for derived in self._registry:
+            derived.push_grad()
 
     def __getattr__(self, item):
-        fields = object.__getattribute__(self, "_fields")
-        if item in fields:
-            return fields[item]
-        parent = object.__getattribute__(self, "_parent")
-        if parent is not None:
-            return getattr(parent, item)
-        raise AttributeError(item)
+        try:
+            return object.__getattribute__(self, "_scope")[item]
+        except KeyError:
+            raise AttributeError(item) from None
+
+    def __repr__(self):
+        return "".join(f"{k}: {v!r}\n" for k, v in dict(self._scope).items())
 
 
 def medium_box_AABB(ds: DependencySet, *, bmin=None, bmax=None):
@@ -103,7 +145,7 @@ def medium_box(ds: DependencySet):
     elif ds.ensures("sigma", _maps.Grid3D):
         medium_box_normalized(ds, t=ds.sigma.grid)
     else:
-        print("[WARNING] Neither Grid3D sigma nor tensor was found, -1..1 box used instead.")
+        warnings.warn("no sigma grid or sigma tensor in the set: the medium box is [-1, 1]^3", stacklevel=2)
         medium_box_AABB(ds)
 
 
@@ -240,7 +282,7 @@ def medium_transmittance(ds: DependencySet):
     try:
         medium_transmittance_RT(ds)
     except Exception:
-        print("[WARNING] fields for ratio-tracking transmittance not found, trying DDA instead")
+        warnings.warn("ratio tracking needs a majorant the set cannot build: falling back to DDA transmittance", stacklevel=2)
         medium_transmittance_DDA(ds)
 
 
@@ -268,30 +310,40 @@ def medium_environment_sampler_uniform(ds: DependencySet):
     ds.add_parameters(environment_sampler=_maps._functionsampler(point_sampler=dir_sampling, function_map=ds.environment))
 
 
-def medium_environment_sampler_quadtree(ds: DependencySet, *, projection: str = "xr", levels: int = 10):
-    """_objects.py:180-199: importance sampling of an equirectangular environment
-    image by a quadtree of its solid-angle-weighted luminance sum."""
-    ds.requires(medium_environment_tensor, projection=projection)
-    if projection != "xr":
-        raise NotImplementedError(f"quadtree from projection {projection} (only 'xr')")
-    skybox = ds.environment_tensor
-    if len(skybox.shape) != 3:
-        raise NotImplementedError("the quadtree sampler needs an environment image")
+def _xr_direction_density(image: torch.Tensor, levels: int) -> torch.Tensor:
+    """Probability of each cell of an n x n equirectangular grid (n = 2^levels)
+    for sampling directions towards an environment image: the sum of the image's
+    channels, resampled to the grid, times the cell's solid angle.  Row r spans
+    polar angles [r pi/n, (r+1) pi/n], so each of its cells subtends
+    (2 pi/n)(cos(r pi/n) - cos((r+1) pi/n)); the factor 2 pi/n cancels in the
+    normalisation.  The reference weights its rows the same way
+    (_objects.py:185-191)."""
+    from . import functions
+    n = 1 << levels
+    edges = torch.cos(torch.linspace(0.0, math.pi, n + 1, dtype=torch.float64))
+    band = (edges[:-1] - edges[1:]).to(image.device, torch.float32)
+    radiance = functions.resample_img(image.sum(-1, keepdim=True).contiguous(), (n, n))
+    density = radiance * band.view(n, 1, 1)
+    return density / max(float(density.sum()), 1e-8)
 
-    def build_quadtree():
+
+def medium_environment_sampler_quadtree(ds: DependencySet, *, projection: str = "xr", levels: int = 10):
+    """Environment directions drawn from a density quadtree over an
+    equirectangular image (the recipe of _objects.py:180-199: the quadtree is a
+    derived field, rebuilt by forward_dependency)."""
+    ds.requires(medium_environment_tensor, projection=projection)
+    image = ds.environment_tensor
+    if projection != "xr" or image.dim() != 3:
+        raise NotImplementedError("the quadtree sampler needs an equirectangular ('xr') environment image [H, W, C]")
+
+    def quadtree():
         from . import functions
         with torch.no_grad():
-            resolution = 1 << levels
-            densities = functions.resample_img(skybox.sum(-1, keepdim=True).contiguous(), (resolution, resolution))
-            py = torch.arange(resolution, dtype=torch.float64)
-            w = torch.cos(py * math.pi / resolution) - torch.cos((py + 1) * math.pi / resolution)
-            densities *= w.to(densities.device, torch.float32)[:, None, None]
-            densities /= max(float(densities.sum()), 0.00000001)
-            return functions.create_density_quadtree(densities)
+            return functions.create_density_quadtree(_xr_direction_density(image, levels))
 
-    ds.add_parameters(environment_sampler_quadtree=build_quadtree)
-    dir_sampling = _maps.XRQuadtreeRandomDirection(input_dim=6, densities=ds.environment_sampler_quadtree, levels=levels)
-    ds.add_parameters(environment_sampler=_maps._functionsampler(point_sampler=dir_sampling, function_map=ds.environment))
+    ds.add_parameters(environment_sampler_quadtree=quadtree)
+    directions = _maps.XRQuadtreeRandomDirection(input_dim=6, densities=ds.environment_sampler_quadtree, levels=levels)
+    ds.add_parameters(environment_sampler=_maps._functionsampler(point_sampler=directions, function_map=ds.environment))
 
 
 def medium_environment_sampler_sun(ds: DependencySet, *, direction, radiance, probability: float = 0.5):
@@ -327,7 +379,7 @@ def medium_phase(ds: DependencySet):
     try:
         medium_phase_HG(ds)
     except Exception:
-        print("[WARNING] a HG phase could not be created, assuming isotropic instead.")
+        warnings.warn("no phase_g in the set: the phase function is isotropic", stacklevel=2)
         medium_phase_iso(ds)
 
 
@@ -347,7 +399,7 @@ def medium_phase_sampler(ds: DependencySet):
     try:
         medium_phase_sampler_HG(ds)
     except Exception:
-        print("[WARNING] a HG phase sampler could not be created, assuming isotropic instead.")
+        warnings.warn("no phase_g in the set: phase directions are sampled uniformly", stacklevel=2)
         medium_phase_sampler_uniform(ds)
 
 

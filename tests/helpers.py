@@ -48,6 +48,7 @@ class Scene:
     env_image: typing.Optional[np.ndarray] = None     # [H,W,3]: environment Image2D over xr coordinates
     env_quadtree: typing.Optional[np.ndarray] = None  # flat quadtree (np_quadtree) for importance sampling
     env_quadtree_levels: int = 0
+    phase_g_grid: typing.Optional[np.ndarray] = None  # [D,H,W,1] over the sigma box: HG asymmetry per position
 
 
 class Built:
@@ -87,6 +88,14 @@ class Built:
         m.sun_prob = scene.sun_prob
         m.phase_kind = scene.phase_kind
         m.phase_g = scene.phase_g
+        if scene.phase_g_grid is not None:
+            pg = self._buf(scene.phase_g_grid)
+            self.phase_g_grid = pg
+            m.phase_g_kind = L.VR_FIELD_GRID
+            m.phase_g_grid = L.VrGrid()
+            m.phase_g_grid.t = self._view(pg, None)
+            m.phase_g_grid.bmin = L.f3(scene.bmin)
+            m.phase_g_grid.bmax = L.f3(scene.bmax)
         if scene.majorant == "grid":
             res = np.zeros(scene.majorant_res, dtype=np.float32)
             g = L.VrGrid()

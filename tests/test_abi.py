@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(L.EXPORTED)
-    assert lib.vr_abi_version() == L.ABI_VERSION == 2
+    assert lib.vr_abi_version() == L.ABI_VERSION == 3
 
 
 def test_oracle_exports_the_same_interface():

@@ -258,13 +258,30 @@ def _hg(mu, g):
     return (1 - g * g) / (4 * math.pi * (1 + g * g - 2 * g * mu) ** 1.5)
 
 
+def _linear_phase_g(n: int) -> np.ndarray:
+    """A phase_g grid [n, n, n, 1] over [-0.5, 0.5]^3 holding g(x) = 0.1 + 0.4 x -
+    0.25 y + 0.15 z at its vertices: trilinear interpolation reproduces a linear
+    field exactly, so the estimator's g at any point is g(x) up to rounding."""
+    c = np.linspace(-0.5, 0.5, n)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    return (0.1 + 0.4 * x - 0.25 * y + 0.15 * z)[..., None].astype(np.float32)
+
+
+def _linear_g(x: np.ndarray) -> np.ndarray:
+    return 0.1 + 0.4 * x[..., 0] - 0.25 * x[..., 1] + 0.15 * x[..., 2]
+
+
 @pytest.mark.parametrize("sun_prob", [0.3, 0.9])
-def test_directional_light_single_scatter_equals_quadrature(sun_prob):
+@pytest.mark.parametrize("phase_field", [False, True])
+def test_directional_light_single_scatter_equals_quadrature(sun_prob, phase_field):
     """Homogeneous box (sigma 3), albedo 0.8, HG g = 0.6, environment 0, sun
     radiance E along s, one scattering event (max_bounces 1): the NEE estimate's
     mean is the 1-D integral over the ray of
-    sigma e^{-sigma t} * albedo * E * e^{-sigma l(x_t)} * p(w . s),
-    l = distance from x_t to the box along s (Simpson, 2e5 intervals)."""
+    sigma e^{-sigma t} * albedo * E * e^{-sigma l(x_t)} * p(w . s; g),
+    l = distance from x_t to the box along s (Simpson, 2e5 intervals).  With
+    phase_field, g is the linear field of _linear_phase_g, read at the collision
+    x_t (a read at another point, or with the grid's axes swapped, moves the
+    mean by many standard errors)."""
     sig, alb, g = 3.0, 0.8, 0.6
     E = np.array([2.0, 1.0, 0.5])
     s_dir = _unit([0.35, 0.8, 0.3])
@@ -272,7 +289,7 @@ def test_directional_light_single_scatter_equals_quadrature(sun_prob):
     w = _unit([-0.05, 0.03, -1.0])
     sc = Scene(sigma=np.full((4, 4, 4, 1), sig, np.float32), albedo_c=(alb, alb, alb), env_c=(0.0, 0.0, 0.0),
                sun_dir=tuple(s_dir), sun_radiance=tuple(E), sun_prob=sun_prob, phase_kind=L.VR_PHASE_HG, phase_g=g,
-               max_bounces=1)
+               max_bounces=1, phase_g_grid=_linear_phase_g(5) if phase_field else None)
     n = 200000
     ray = np.concatenate([x0, w]).astype(np.float32)
     R = oracle_map_fw(sc, L.VR_MAP_NEE_DT, np.tile(ray, (n, 1)), (21, 22, 23, 24)).astype(np.float64)
@@ -284,7 +301,8 @@ def test_directional_light_single_scatter_equals_quadrature(sun_prob):
     m = 200000
     t = np.linspace(tmin, tmax, m + 1)
     xt = x0f + t[:, None] * wf
-    f = sig * np.exp(-sig * (t - tmin)) * alb * np.exp(-sig * _exit_distance(xt, s_dir)) * _hg(wf @ s_dir, g)
+    gt = _linear_g(xt) if phase_field else g
+    f = sig * np.exp(-sig * (t - tmin)) * alb * np.exp(-sig * _exit_distance(xt, s_dir)) * _hg(wf @ s_dir, gt)
     h = (tmax - tmin) / m
     integral = h / 3 * (f[0] + f[-1] + 4 * f[1:-1:2].sum() + 2 * f[2:-1:2].sum())
     expect = integral * E
