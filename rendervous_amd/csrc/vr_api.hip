@@ -731,8 +731,8 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         unsigned prep = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
         hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, b0, n, dbg, counter);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
-        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN>), dim3(blocks), dim3(kBlock), 0, st, M, C, sd, S, counter, out,
-                           dbg);
+        const NeeArgs args{M, C, sd, S, counter, out, dbg};
+        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN>), dim3(blocks), dim3(kBlock), 0, st, args);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw launch"))) return rc;
     }
     return VR_OK;

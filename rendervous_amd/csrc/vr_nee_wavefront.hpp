@@ -386,10 +386,39 @@ VR_DEV unsigned long long lane_mask_lt() {
 // part of the image, and moves on to the next queue when that one is empty.
 // The output must be zero on entry (k_nee_prepare); item sums are added
 // atomically.
+// Every argument of the persistent kernel, passed as its only argument, so the
+// structure starts at offset 0 of the kernel-argument segment.
+struct NeeArgs {
+    MediumView M;
+    CameraView cam;
+    Seeds seeds;
+    NeeSchedule S;
+    unsigned long long* counters;
+    float* out;
+    int* dbg;
+};
+
+// The kernel's arguments read through an address the compiler cannot follow:
+// the kernel-argument segment pointer passes through an empty asm statement, so
+// the loads of the fields a handler reads stay in that handler instead of being
+// hoisted out of the persistent loop.  Hoisted, every field of the medium, the
+// camera and the seeds stays live in scalar registers for the whole kernel,
+// more than the 102 a wave has, and the excess spills to vector-register lanes
+// (a v_readlane per reload on the vector pipe).  Re-reading them costs scalar
+// loads from the constant segment, which the scalar cache serves.
+VR_DEV const NeeArgs& nee_args() {
+    auto p = (const __attribute__((address_space(4))) NeeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const NeeArgs*)p;
+}
+
 template <int MAJ, int TKIND, bool ENVMAP, bool LEAN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))
-k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsigned long long* __restrict__ counters,
-                 float* __restrict__ out, int* __restrict__ dbg) {
+k_nee_persistent(NeeArgs args) {
+    const NeeSchedule& S = args.S;
+    unsigned long long* __restrict__ counters = args.counters;
+    float* __restrict__ out = args.out;
+    int* __restrict__ dbg = args.dbg;
     const float inv = S.spp == 1 ? 1.0f : 1.0f / (float)S.spp;
     NeeLane<MAJ, TKIND, ENVMAP, LEAN> lane;
     lane.lookups = 0;
@@ -444,7 +473,7 @@ k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsig
             if (need) {
                 if (rank < avail) {
                     int s0;
-                    if (nee_item(S, cam, q, pool_next + rank, sensor, s0, s1, pose, row, col)) {
+                    if (nee_item(S, nee_args().cam, q, pool_next + rank, sensor, s0, s1, pose, row, col)) {
                         have = true;
                         s = s0;
                         acc = F3(0.0f, 0.0f, 0.0f);
@@ -467,18 +496,21 @@ k_nee_persistent(MediumView M, CameraView cam, Seeds seeds, NeeSchedule S, unsig
         // S.min_begin of them can start together.
         const bool tracks = __ballot(have && lane.event == kEvNone) != 0ull;
         if (!tracks || __popcll(__ballot(have && lane.event == kEvBegin)) >= S.min_begin)
-            if (have && lane.event == kEvBegin)
-                lane.begin_sample(M, cam, seeds, (uint32_t)s * S.total + sensor, pose, row, col);
-        if (have && lane.event == kEvStart) lane.start_ray(M);
+            if (have && lane.event == kEvBegin) {
+                const NeeArgs& A = nee_args();
+                lane.begin_sample(A.M, A.cam, A.seeds, (uint32_t)s * S.total + sensor, pose, row, col);
+            }
+        if (have && lane.event == kEvStart) lane.start_ray(nee_args().M);
+        const MediumView& Mt = nee_args().M;
         for (int k = 0;;) {
-            if (have && lane.event == kEvNone) lane.step(M);
+            if (have && lane.event == kEvNone) lane.step(Mt);
             ++k;
             const unsigned long long tracking = __ballot(have && lane.event == kEvNone);
             if (tracking == 0ull || k >= S.kmax) break;
             if (k >= S.ksteps && __popcll(tracking) < S.track_min) break;
         }
-        if (have && lane.event == kEvCollision) lane.on_collision(M);
-        if (have && lane.event == kEvTrackEnd) lane.on_track_end(M);
+        if (have && lane.event == kEvCollision) lane.on_collision(nee_args().M);
+        if (have && lane.event == kEvTrackEnd) lane.on_track_end(nee_args().M);
     }
 }
 
