@@ -236,7 +236,7 @@ template <int KIND, int MAJ>
 __device__ __forceinline__ void field_fw(Tracer<MAJ>& tr, const float* in, float* out) {
     f3 x = F3(in[0], in[1], in[2]), w = F3(in[3], in[4], in[5]);
     if constexpr (KIND == VR_MAP_RAYBOX) {
-        ray_box_intersection(x, w, tr.M.box_min, tr.M.box_max, out[0], out[1]);
+        ray_box_intersection(x, w, tr.med().box_min, tr.med().box_max, out[0], out[1]);
     } else if constexpr (KIND == VR_MAP_T_DDA) {
         out[0] = tr.t_dda(x, w);
     } else if constexpr (KIND == VR_MAP_T_DT) {
@@ -277,15 +277,15 @@ __device__ __forceinline__ void field_bw(Tracer<MAJ>& tr, const float* in, const
     } else if constexpr (KIND == VR_MAP_T_RT) {
         tr.t_rt_bw(x, w, og[0]);
     } else if constexpr (KIND == VR_MAP_COLLISION_DT) {
-        tr.collision_dt_bw(x, w, tr.M.ds_eps, og[0], og[2]);
+        tr.collision_dt_bw(x, w, tr.med().ds_eps, og[0], og[2]);
     } else if constexpr (KIND == VR_MAP_SS_RADIANCE) {
         tr.ss_radiance_bw(x, w, F3(og[0], og[1], og[2]));
     } else if constexpr (KIND == VR_MAP_MC_COLLISION) {
         tr.mc_collision_bw(x, w, F3(og[0], og[1], og[2]));
     } else if constexpr (KIND == VR_MAP_NEE_DT) {
-        tr.nee_ds_bw(x, w, tr.M.ds_eps, F3(og[0], og[1], og[2]));
+        tr.nee_ds_bw(x, w, tr.med().ds_eps, F3(og[0], og[1], og[2]));
     } else if constexpr (KIND == VR_MAP_PATH_DT) {
-        tr.path_dt_bw(x, w, tr.M.ds_eps, F3(og[0], og[1], og[2]));
+        tr.path_dt_bw(x, w, tr.med().ds_eps, F3(og[0], og[1], og[2]));
     } else if constexpr (KIND == VR_MAP_NEE_DRT) {
         tr.nee_drt_bw(x, w, F3(og[0], og[1], og[2]));
     } else if constexpr (KIND == VR_MAP_MEDIUM_DT) {
@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_fw(MediumVi
                                                    float* __restrict__ out, Seeds seeds, int* __restrict__ dbg) {
     int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= start + cnt) return;
-    Tracer<MAJ> tr(M);
+    Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     tr.rng = seed_map(seeds, (uint32_t)i);
     float a[Dims<KIND>::in], o[Dims<KIND>::out];
 #pragma unroll
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_bw(MediumVi
                                                    int64_t start, int64_t cnt, Seeds seeds) {
     int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= start + cnt) return;
-    Tracer<MAJ> tr(M);
+    Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     tr.rng = seed_map(seeds, (uint32_t)i);
     float a[Dims<KIND>::in], o[Dims<KIND>::out];
 #pragma unroll
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_fw(Medi
     float acc[OD];
 #pragma unroll
     for (int k = 0; k < OD; k++) acc[k] = 0.0f;
-    Tracer<MAJ> tr(M);
+    Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     for (int s = s0; s < s1; s++) {
         tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
         f3 x, w;
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
         if (dbg && P.chunks == 1) dbg[i] = 0;
         return;
     }
-    Tracer<MAJ> tr(M);
+    Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     // the reference's fresh-seed backward continues one stream over the samples (one chunk)
     if (seed_mode == VR_SEED_CAPTURE_BW) tr.rng = seed_capture_bw(seeds, (uint32_t)i);
     for (int s = s0; s < s1; s++) {
@@ -555,7 +555,7 @@ k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P,
         if (dbg && P.chunks == 1) dbg[i] = 0;
         return;
     }
-    NeeDsBwLane<MAJ, ENVMAP> L(M);
+    NeeDsBwLane<MAJ, ENVMAP> L(first_kernel_arg<MediumView>());
     L.eps = M.ds_eps;
     L.dL_dR = g;
     // the reference's fresh-seed backward continues one stream over the samples (one chunk)
@@ -569,6 +569,7 @@ k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P,
     }
     // every sample ends (its paths end); the cap only bounds a thread's loop
     for (int it = 0; live && it < (1 << 24); it++) {
+        L.tr.reread();
         if (L.state == kBwDone) {
             if (++s >= s1) break;
             if (seed_mode != VR_SEED_CAPTURE_BW) L.tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));

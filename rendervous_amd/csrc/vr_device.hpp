@@ -597,57 +597,91 @@ struct Walk<VR_MAJ_GRID> {
     }
 };
 
+// ------------------------------------------------- kernel-argument access --
+// Read-only data in the constant address space: loads through such a pointer
+// are scalar loads when the address is wave-uniform.
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+
+// The running kernel's first parameter, read in place in the kernel-argument
+// segment (explicit arguments start at its offset 0).  Valid only in a kernel
+// whose first parameter has type T.
+template <typename T>
+VR_DEV ConstPtr<T> first_kernel_arg() {
+    return (ConstPtr<T>)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
+// p, passed through an empty asm statement: the compiler can no longer tell
+// the result from another address, so the loads made through it stay after
+// this point.  Kernels whose loops read many uniform fields (medium, camera,
+// seeds) call it at the loop head; hoisted out of the loop instead, every field
+// stays live in scalar registers for the whole kernel, more than the 102 a wave
+// has, and the excess spills to vector-register lanes (a v_readlane on the
+// vector pipe per reload).  Re-reading costs scalar loads that the scalar cache
+// serves.
+template <typename T>
+VR_DEV ConstPtr<T> reread(ConstPtr<T> p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // -------------------------------------------------------------- estimators --
 template <int MAJ>
 struct Tracer {
-    const MediumView& M;
+    ConstPtr<MediumView> Mp;  // the medium, in the constant address space (scalar loads)
     Rng rng;
     int lookups;   // trilinear sigma lookups
     int scatters;  // trilinear sigma gradient scatters
 
-    VR_DEV explicit Tracer(const MediumView& m) : M(m), lookups(0), scatters(0) {}
+    VR_DEV explicit Tracer(ConstPtr<MediumView> m) : Mp(m), lookups(0), scatters(0) {}
+
+    VR_DEV const MediumView& med() const { return *(const MediumView*)Mp; }
+    // Loads of the medium's fields after this call are not hoisted above it
+    // (reread): a loop that calls it at its head keeps them out of the scalar
+    // registers live across the loop.
+    VR_DEV void reread() { Mp = vr::reread(Mp); }
 
     VR_DEV float rnd() { return rng_uniform(rng); }
     VR_DEV float flight(float maj) { return neg_log1m(rnd()) * rcp(maj); }
 
-    VR_DEV float sigma(f3 x) { lookups++; return sigma_lookup(M, x); }
+    VR_DEV float sigma(f3 x) { lookups++; return sigma_lookup(med(), x); }
     VR_DEV void sigma_bw(f3 x, float g) {
-        if (g != 0.0f && M.sigma.grad != nullptr) scatters++;
-        grid_scatter1(M.sigma, x, g);
+        if (g != 0.0f && med().sigma.grad != nullptr) scatters++;
+        grid_scatter1(med().sigma, x, g);
     }
 
     VR_DEV f3 albedo(f3 x) {
-        if (!M.albedo_grid) return M.albedo_c;
+        if (!med().albedo_grid) return med().albedo_c;
         float v[4];
-        grid_lookupC(M.albedo, x, v);
-        if (M.albedo.C == 1) return F3(v[0], v[0], v[0]);
+        grid_lookupC(med().albedo, x, v);
+        if (med().albedo.C == 1) return F3(v[0], v[0], v[0]);
         return F3(v[0], v[1], v[2]);
     }
     VR_DEV void albedo_bw(f3 x, f3 g) {
-        if (!M.albedo_grid || M.albedo.grad == nullptr) return;
+        if (!med().albedo_grid || med().albedo.grad == nullptr) return;
         if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) return;
         float og[3];
-        if (M.albedo.C == 1) og[0] = g.x + g.y + g.z;
+        if (med().albedo.C == 1) og[0] = g.x + g.y + g.z;
         else { og[0] = g.x; og[1] = g.y; og[2] = g.z; }
-        grid_scatterC(M.albedo, x, og);
+        grid_scatterC(med().albedo, x, og);
     }
 
     // include/vr_transmittance_DT.h:1-28
     VR_DEV float t_dt(f3 x, f3 w) {
         float tmin, tmax;
-        if (!boundary(M, x, w, tmin, tmax)) return 1.0f;
+        if (!boundary(med(), x, w, tmin, tmax)) return 1.0f;
         x = x + w * tmin;
         float d = tmax - tmin;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(0.0000001f, mj.query(M, md));
+            float maj = fmaxf(0.0000001f, mj.query(med(), md));
             float dt = flight(maj);
             float t = fminf(md, dt);
             x = x + w * t;
             d -= t;
-            if (dt > md) { mj.cross(M, w); if (d <= kCrossEps) return 1.0f; continue; }
+            if (dt > md) { mj.cross(med(), w); if (d <= kCrossEps) return 1.0f; continue; }
             mj.move(t);
             if (d <= 0.0000001f) return 1.0f;
             if (rnd() * maj < sigma(x)) return 0.0f;
@@ -657,16 +691,16 @@ struct Tracer {
     // include/vr_transmittance_RT.h:1-49
     VR_DEV float t_rt(f3 x, f3 w) {
         float T = 1.0f, tmin, tmax;
-        if (!boundary(M, x, w, tmin, tmax)) return T;
+        if (!boundary(med(), x, w, tmin, tmax)) return T;
         x = x + w * tmin;
         float d = tmax - tmin;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(0.0000001f, mj.query(M, md));
+            float maj = fmaxf(0.0000001f, mj.query(med(), md));
             float s = flight(maj);
-            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(M, w); if (d <= kCrossEps) return T; continue; }
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(med(), w); if (d <= kCrossEps) return T; continue; }
             float dt = fminf(s, fminf(d, md));
             x = x + w * dt;
             d -= dt;
@@ -689,11 +723,11 @@ struct Tracer {
 
     VR_DEV bool dda_begin(f3 xf, f3 wf, Dda& s) {
         float tminf, tmaxf;
-        if (!boundary(M, xf, wf, tminf, tmaxf)) return false;
+        if (!boundary(med(), xf, wf, tminf, tmaxf)) return false;
         // intersect_ray_box (maths.h:11-26) with the grid box, in double
         double x[3] = {xf.x, xf.y, xf.z}, w[3] = {wf.x, wf.y, wf.z};
-        double bmin[3] = {M.sigma.bmin.x, M.sigma.bmin.y, M.sigma.bmin.z};
-        double bmax[3] = {M.sigma.bmax.x, M.sigma.bmax.y, M.sigma.bmax.z};
+        double bmin[3] = {med().sigma.bmin.x, med().sigma.bmin.y, med().sigma.bmin.z};
+        double bmax[3] = {med().sigma.bmax.x, med().sigma.bmax.y, med().sigma.bmax.z};
         double gmin = -1e300, gmax = 1e300;
         for (int a = 0; a < 3; a++) {
             double wa = fabs(w[a]) <= 0.000001 ? 0.000001 : w[a];
@@ -704,7 +738,7 @@ struct Tracer {
         gmin = fmax(0.0, gmin);
         if (gmax <= gmin || gmax <= 0.0) return false;
         // the boundary map's interval, recomputed in double (maths.h:28-38)
-        double bmn[3] = {M.box_min.x, M.box_min.y, M.box_min.z}, bmx[3] = {M.box_max.x, M.box_max.y, M.box_max.z};
+        double bmn[3] = {med().box_min.x, med().box_min.y, med().box_min.z}, bmx[3] = {med().box_max.x, med().box_max.y, med().box_max.z};
         double tmin = -1e300, tmax = 1e300;
         for (int a = 0; a < 3; a++) {
             double wa = fabs(w[a]) <= 0.000001 ? 0.000001 : w[a];
@@ -718,7 +752,7 @@ struct Tracer {
         tmax = fmin(tmax, gmax);
         if (tmax <= tmin) return false;
         s.d = tmax - tmin;
-        s.dim[0] = M.sigma.nx - 1; s.dim[1] = M.sigma.ny - 1; s.dim[2] = M.sigma.nz - 1;
+        s.dim[0] = med().sigma.nx - 1; s.dim[1] = med().sigma.ny - 1; s.dim[2] = med().sigma.nz - 1;
         for (int a = 0; a < 3; a++) {
             double xa = x[a] + w[a] * tmin;
             s.x[a] = xa;
@@ -739,7 +773,7 @@ struct Tracer {
     }
 
     VR_DEV float dda_sigma_at(int x, int y, int z) {
-        const GridView& g = M.sigma;
+        const GridView& g = med().sigma;
         x = max(0, min(g.nx - 1, x)); y = max(0, min(g.ny - 1, y)); z = max(0, min(g.nz - 1, z));
         return g.data[z * g.sz + y * g.sy + x * g.sx];
     }
@@ -834,7 +868,7 @@ struct Tracer {
 
     // include/vr_transmittance_DDA.h:62-156
     VR_DEV void t_dda_bw(f3 x, f3 w, float dL_dT) {
-        if (dL_dT == 0.0f || M.sigma.grad == nullptr) return;
+        if (dL_dT == 0.0f || med().sigma.grad == nullptr) return;
         Dda s;
         if (!dda_begin(x, w, s)) return;
         double T = exp(-dda_tau(s));
@@ -842,7 +876,7 @@ struct Tracer {
         double alpha[3] = {s.alpha0[0], s.alpha0[1], s.alpha0[2]};
         int cell[3] = {s.cell[0], s.cell[1], s.cell[2]};
         double cur = 0.0;
-        const GridView& g = M.sigma;
+        const GridView& g = med().sigma;
         while (cur < s.d - 0.0001) {
             double nxt = fmin(s.d, fmin(alpha[0], fmin(alpha[1], alpha[2])));
             double tm = lerpd(cur, nxt, 0.5);
@@ -868,37 +902,37 @@ struct Tracer {
     VR_DEV void t_dt_bw(f3 x, f3 w, float dL_dT) {
         if (dL_dT == 0.0f) return;
         float tmin, tmax;
-        if (!boundary(M, x, w, tmin, tmax)) return;
+        if (!boundary(med(), x, w, tmin, tmax)) return;
         x = x + w * tmin;
         float d = tmax - tmin;
         f3 x0 = x;
         float d0 = d;
         Rng saved = rng;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(0.0000001f, mj.query(M, md) + 0.1f);
+            float maj = fmaxf(0.0000001f, mj.query(med(), md) + 0.1f);
             float dt = flight(maj);
             float t = fminf(md, dt);
             x = x + w * t;
             d -= t;
-            if (dt > md) { mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (dt > md) { mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             mj.move(t);
             if (d <= 0.0000001f) break;
             if (rnd() < sigma(x) / maj) return;
         }
         x = x0; d = d0;
         rng = saved;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(0.0000001f, mj.query(M, md) + 0.1f);
+            float maj = fmaxf(0.0000001f, mj.query(med(), md) + 0.1f);
             float dt = flight(maj);
             float t = fminf(md, dt);
             x = x + w * t;
             d -= t;
-            if (dt > md) { mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (dt > md) { mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             mj.move(t);
             if (d <= 0.0000001f) break;
             rnd();
@@ -911,7 +945,7 @@ struct Tracer {
     VR_DEV void t_rt_bw(f3 x, f3 w, float dL_dT) {
         if (dL_dT == 0.0f) return;
         float tmin, tmax;
-        if (!boundary(M, x, w, tmin, tmax)) return;
+        if (!boundary(med(), x, w, tmin, tmax)) return;
         const float EPS = 0.0000001f;
         float T = 1.0f;
         x = x + w * tmin;
@@ -920,12 +954,12 @@ struct Tracer {
         float d0 = d;
         Rng saved = rng;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(EPS, mj.query(M, md) + 0.1f);
+            float maj = fmaxf(EPS, mj.query(med(), md) + 0.1f);
             float s = flight(maj);
-            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             float dt = fminf(s, fminf(d, md));
             x = x + w * dt;
             d -= dt;
@@ -941,12 +975,12 @@ struct Tracer {
         T = 1.0f;
         rng = saved;
         x = x0; d = d0;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(EPS, mj.query(M, md) + 0.1f);
+            float maj = fmaxf(EPS, mj.query(med(), md) + 0.1f);
             float s = flight(maj);
-            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (Walk<MAJ>::grid && md < d && s > md) { x = x + w * md; d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             float dt = fminf(s, fminf(d, md));
             x = x + w * dt;
             d -= dt;
@@ -963,37 +997,37 @@ struct Tracer {
     }
 
     VR_DEV float transmittance(f3 x, f3 w) {
-        if (M.t_kind == VR_T_RT) return t_rt(x, w);
-        if (M.t_kind == VR_T_DDA) return t_dda(x, w);
-        if (M.t_kind == VR_T_GDT || M.t_kind == VR_T_GRT) return t_grid_tracking(x, w, M.t_kind == VR_T_GRT);
+        if (med().t_kind == VR_T_RT) return t_rt(x, w);
+        if (med().t_kind == VR_T_DDA) return t_dda(x, w);
+        if (med().t_kind == VR_T_GDT || med().t_kind == VR_T_GRT) return t_grid_tracking(x, w, med().t_kind == VR_T_GRT);
         return t_dt(x, w);
     }
 
     // (GDT and GRT have no backward in the reference: no gradient)
     VR_DEV void transmittance_bw(f3 x, f3 w, float dL_dT) {
-        if (M.t_kind == VR_T_RT) t_rt_bw(x, w, dL_dT);
-        else if (M.t_kind == VR_T_DDA) t_dda_bw(x, w, dL_dT);
-        else if (M.t_kind == VR_T_DT) t_dt_bw(x, w, dL_dT);
+        if (med().t_kind == VR_T_RT) t_rt_bw(x, w, dL_dT);
+        else if (med().t_kind == VR_T_DDA) t_dda_bw(x, w, dL_dT);
+        else if (med().t_kind == VR_T_DT) t_dt_bw(x, w, dL_dT);
     }
 
     // include/vr_collision_sampler_DT.h:5-44
     VR_DEV float collision_dt(f3 x, f3 w, float& out_t, float& out_T) {
         out_T = 1.0f; out_t = 0.0f;
         float C = 0.0f, tmin, tmax;
-        if (!boundary(M, x, w, tmin, tmax)) return C;
+        if (!boundary(med(), x, w, tmin, tmax)) return C;
         float d = tmax - tmin;
         x = x + w * tmin;
         float t = 0.0f;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = mj.query(M, md);
+            float maj = mj.query(med(), md);
             if (Walk<MAJ>::grid) maj = fmaxf(0.0000001f, maj);
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(med(), w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
             if (rnd() * maj < sigma(x)) { out_T = 0.0f; C = 1.0f; break; }
@@ -1009,14 +1043,14 @@ struct Tracer {
         float t = 0.0f, W = 1.0f;
         C = 0.0f;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(0.0000000001f, mj.query(M, md) + 2.0f * eps);
+            float maj = fmaxf(0.0000000001f, mj.query(med(), md) + 2.0f * eps);
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(med(), w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
             float s = sigma(x);
@@ -1031,7 +1065,7 @@ struct Tracer {
     // include/vr_collision_sampler_DT.h:46-134 (VR_BW_CONSISTENT: full path weight)
     VR_DEV void collision_dt_bw(f3 x, f3 w, float eps, float dL_dC, float dL_dT) {
         float tmin, tmax;
-        if (!boundary(M, x, w, tmin, tmax)) return;
+        if (!boundary(med(), x, w, tmin, tmax)) return;
         Rng saved = rng;
         float out_T = 1.0f, C = 0.0f, s_t = 0.0f, Wpre = 1.0f;
         float d = tmax - tmin;
@@ -1039,14 +1073,14 @@ struct Tracer {
         f3 x0 = x;
         float t = 0.0f;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(0.0000000001f, mj.query(M, md) + 2.0f * eps);
+            float maj = fmaxf(0.0000000001f, mj.query(med(), md) + 2.0f * eps);
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(med(), w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
             float s = sigma(x);
@@ -1056,18 +1090,18 @@ struct Tracer {
             out_T *= r;
             Wpre *= r;
         }
-        bool consistent = M.bw_mode == VR_BW_CONSISTENT;
+        bool consistent = med().bw_mode == VR_BW_CONSISTENT;
         if (consistent) s_t *= Wpre;
         rng = saved;
         x = x0; t = 0.0f;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float maj = fmaxf(0.0000000001f, mj.query(M, md) + 2.0f * eps);
+            float maj = fmaxf(0.0000000001f, mj.query(med(), md) + 2.0f * eps);
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
             t += dt;
-            if (dt >= md) { mj.cross(M, w); if (t >= d - kCrossEps) break; continue; }
+            if (dt >= md) { mj.cross(med(), w); if (t >= d - kCrossEps) break; continue; }
             mj.move(dt);
             if (t >= d - 0.0000001f) break;
             float s = sigma(x);
@@ -1087,8 +1121,8 @@ struct Tracer {
     // include/maps/single_scattered_radiance_mc.h:6-16
     VR_DEV f3 ss_radiance(f3 x, f3 w) {
         f3 wo;
-        f3 env = env_sample(M, rng, wo);
-        float ph = phase_eval(M, x, w * -1.0f, wo);
+        f3 env = env_sample(med(), rng, wo);
+        float ph = phase_eval(med(), x, w * -1.0f, wo);
         f3 sa = albedo(x);
         float T = transmittance(x, wo);
         return (env * sa) * (ph * T);
@@ -1097,15 +1131,15 @@ struct Tracer {
     // include/maps/single_scattered_radiance_mc.h:19-38
     VR_DEV void ss_radiance_bw(f3 x, f3 w, f3 dL_dR) {
         f3 wo;
-        f3 env = env_sample(M, rng, wo);
-        float ph = phase_eval(M, x, w * -1.0f, wo);
+        f3 env = env_sample(med(), rng, wo);
+        float ph = phase_eval(med(), x, w * -1.0f, wo);
         f3 sa = albedo(x);
         Rng before_T = rng;
         float T = transmittance(x, wo);
         f3 dL_dsa = (dL_dR * env) * (ph * T);
         float dL_dT = dot(dL_dR, (env * sa) * ph);
         albedo_bw(x, dL_dsa);
-        if (M.bw_mode == VR_BW_CONSISTENT) {
+        if (med().bw_mode == VR_BW_CONSISTENT) {
             Rng after = rng;
             rng = before_T;
             transmittance_bw(x, wo, dL_dT);
@@ -1120,33 +1154,33 @@ struct Tracer {
         float T, t;
         float C = collision_dt(x, w, t, T);
         f3 xt = w * t + x;
-        f3 R = env_eval(M, w) * T;
+        f3 R = env_eval(med(), w) * T;
         if (C != 0.0f) R = R + ss_radiance(xt, w * -1.0f) * C;
         return R;
     }
 
     // include/maps/collision_integrator_mc.h:30-44
     VR_DEV void mc_collision_bw(f3 x, f3 w, f3 dL_dR) {
-        if (M.bw_mode == VR_BW_REFERENCE) {
+        if (med().bw_mode == VR_BW_REFERENCE) {
             float T, t;
             float C = collision_dt(x, w, t, T);
             f3 xt = w * t + x;
             f3 eR = ss_radiance(xt, w * -1.0f);
-            collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, env_eval(M, w)));
+            collision_dt_bw(x, w, med().ds_eps, dot(dL_dR, eR), dot(dL_dR, env_eval(med(), w)));
             ss_radiance_bw(xt, w * -1.0f, dL_dR * C);
             return;
         }
         float tmin, tmax;
-        if (!boundary(M, x, w, tmin, tmax)) return;
+        if (!boundary(med(), x, w, tmin, tmax)) return;
         Rng saved = rng;
         float C, t, Wc;
-        collision_ds_primal(x, w, M.ds_eps, tmin, tmax, C, t, Wc);
+        collision_ds_primal(x, w, med().ds_eps, tmin, tmax, C, t, Wc);
         f3 xt = x + w * (tmin + t);
         f3 eR = F3(0.0f, 0.0f, 0.0f);
         if (C != 0.0f) eR = ss_radiance(xt, w * -1.0f);
         Rng after = rng;
         rng = saved;
-        collision_dt_bw(x, w, M.ds_eps, dot(dL_dR, eR), dot(dL_dR, env_eval(M, w)));
+        collision_dt_bw(x, w, med().ds_eps, dot(dL_dR, eR), dot(dL_dR, env_eval(med(), w)));
         rng = after;
         if (C != 0.0f) ss_radiance_bw(xt, w * -1.0f, dL_dR * Wc);
     }
@@ -1154,41 +1188,41 @@ struct Tracer {
     // include/vr_path_integrator_NEE.h:1-66 (max_bounces >= 0 caps scatter events)
     VR_DEV f3 nee(f3 x, f3 w) {
         float tmin, tmax;
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         float d = tmax - tmin;
-        f3 R = env_eval(M, w) * transmittance(x, w);
+        f3 R = env_eval(med(), w) * transmittance(x, w);
         f3 W = F3(1.0f, 1.0f, 1.0f);
         const float EPS = 0.00001f;
         int bounces = 0;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > EPS) {
             float md;
-            float m = fmaxf(EPS, mj.query(M, md));
+            float m = fmaxf(EPS, mj.query(med(), md));
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
             if (rnd() * m >= s) { d -= t; continue; }
             f3 sa = albedo(x);
-            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * med().emission_c;
             W = W * sa;
             f3 wnee;
-            f3 env = env_sample(M, rng, wnee);
+            f3 env = env_sample(med(), rng, wnee);
             float Tn = transmittance(x, wnee);
-            R = R + (W * env) * (Tn * phase_eval(M, x, w, wnee));
+            R = R + (W * env) * (Tn * phase_eval(med(), x, w, wnee));
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) return R;
             f3 wo;
-            float rho = phase_sample(M, rng, x, w, wo);
+            float rho = phase_sample(med(), rng, x, w, wo);
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
         }
         return R;
     }
@@ -1198,7 +1232,7 @@ struct Tracer {
         Rng before_path = rng;
         f3 x0 = x, w0 = w;
         float tmin, tmax;
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         float d = tmax - tmin;
         f3 R = F3(0.0f, 0.0f, 0.0f), R_sp = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
@@ -1207,13 +1241,13 @@ struct Tracer {
         bool scattered = false;
         int bounces = 0;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > EPS) {
             float md;
-            float m = fmaxf(EPS, mj.query(M, md) + 2.0f * eps + 0.1f);
+            float m = fmaxf(EPS, mj.query(med(), md) + 2.0f * eps + 0.1f);
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
@@ -1225,43 +1259,43 @@ struct Tracer {
                 null_W = 0.0f;
             } else W = W * ((s / m) / Pc);
             f3 sa = albedo(x);
-            f3 added = (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            f3 added = (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * med().emission_c;
             W = W * sa;
             f3 wnee;
-            f3 env = env_sample(M, rng, wnee);
+            f3 env = env_sample(med(), rng, wnee);
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, x, w, wnee);
+            float phn = phase_eval(med(), x, w, wnee);
             added = added + (W * env) * (Tn * phn);
             if (null_W < EPS) R_sp = R_sp + added;
             else R = R + added;
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
             f3 wo;
-            float rho = phase_sample(M, rng, x, w, wo);
+            float rho = phase_sample(med(), rng, x, w, wo);
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); break; }
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
         }
-        if (!scattered) R = R + env_eval(M, w) * W;
+        if (!scattered) R = R + env_eval(med(), w) * W;
 
         rng = before_path;
         x = x0; w = w0;
         W = F3(1.0f, 1.0f, 1.0f);
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         d = tmax - tmin;
         bounces = 0;
-        mj.begin(M, x, w);
-        const bool ref = M.bw_mode == VR_BW_REFERENCE;
+        mj.begin(med(), x, w);
+        const bool ref = med().bw_mode == VR_BW_REFERENCE;
         while (d > EPS) {
             float md;
-            float mq = mj.query(M, md);
+            float mq = mj.query(med(), md);
             float m = ref ? fmaxf(EPS, mq * 1.2f + 2.0f * eps) : fmaxf(EPS, mq + 2.0f * eps + 0.1f);
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
@@ -1277,16 +1311,16 @@ struct Tracer {
             W = W * (s / m / Pc);
             if (s < EPS) return;
             f3 sa = albedo(x);
-            f3 dL_dsa = (dL_dR * W) * (M.emission_c * -1.0f);
-            R = R - (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            f3 dL_dsa = (dL_dR * W) * (med().emission_c * -1.0f);
+            R = R - (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * med().emission_c;
             dL_dsa = dL_dsa + dL_dR * F3(R.x / fmaxf(EPS, sa.x), R.y / fmaxf(EPS, sa.y), R.z / fmaxf(EPS, sa.z));
             f3 wnee;
-            f3 env = env_sample(M, rng, wnee);
+            f3 env = env_sample(med(), rng, wnee);
             Rng before_T = rng;
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, x, w, wnee);
+            float phn = phase_eval(med(), x, w, wnee);
             f3 wo;
-            float rho = phase_sample(M, rng, x, w, wo);
+            float rho = phase_sample(med(), rng, x, w, wo);
             Rng before_next = rng;
             W = W * sa;
             float dL_dTn = dot(dL_dR, (W * env) * phn);
@@ -1296,12 +1330,12 @@ struct Tracer {
             transmittance_bw(x, wnee, dL_dTn);
             rng = before_next;
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return;
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) return;
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) return;
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
         }
     }
 
@@ -1309,37 +1343,37 @@ struct Tracer {
     // that many scatter events, with no escape term)
     VR_DEV f3 path_dt(f3 x, f3 w) {
         float tmin, tmax;
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         float d = tmax - tmin;
         f3 R = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
         int bounces = 0;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > 0.00000001f) {
             float md;
-            float m = fmaxf(0.00000001f, mj.query(M, md));
+            float m = fmaxf(0.00000001f, mj.query(med(), md));
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
             if (rnd() * m >= s) { d -= t; continue; }
             f3 sa = albedo(x);
-            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * med().emission_c;
             W = W * sa;
             f3 wo;
-            float rho = phase_sample(M, rng, x, w, wo);
+            float rho = phase_sample(med(), rng, x, w, wo);
             W = W * rho;
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) return R;
             if (W.x < 0.0000001f && W.y < 0.0000001f && W.z < 0.0000001f) return R;
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
         }
-        return W * env_eval(M, w) + R;
+        return W * env_eval(med(), w) + R;
     }
 
     // include/vr_path_integrator_DT.h:59-235: defensive pass (majorant + 2 eps,
@@ -1349,7 +1383,7 @@ struct Tracer {
         Rng before_path = rng;
         f3 x0 = x, w0 = w;
         float tmin, tmax;
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         float d = tmax - tmin;
         const float EPS = 0.00000001f;
@@ -1357,13 +1391,13 @@ struct Tracer {
         float null_W = 1.0f;
         int bounces = 0;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > EPS) {
             float md;
-            float m = fmaxf(EPS, mj.query(M, md) + 2.0f * eps);
+            float m = fmaxf(EPS, mj.query(med(), md) + 2.0f * eps);
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
@@ -1375,31 +1409,31 @@ struct Tracer {
             } else W = W * ((s / m) / Pc);
             f3 sa = albedo(x);
             f3 wo;
-            float rho = phase_sample(M, rng, x, w, wo);
+            float rho = phase_sample(med(), rng, x, w, wo);
             W = (W * sa) * rho;
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) { W = F3(0.0f, 0.0f, 0.0f); break; }
             if (W.x < EPS && W.y < EPS && W.z < EPS) { W = F3(0.0f, 0.0f, 0.0f); break; }
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
         }
-        const f3 dL_dW = dL_dR * env_eval(M, w);
+        const f3 dL_dW = dL_dR * env_eval(med(), w);
         const f3 W_out = W;
 
         rng = before_path;
         x = x0; w = w0;
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         d = tmax - tmin;
         bounces = 0;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > EPS) {
             float md;
-            float m = fmaxf(EPS, mj.query(M, md) + 2.0f * eps);
+            float m = fmaxf(EPS, mj.query(med(), md) + 2.0f * eps);
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
@@ -1412,12 +1446,12 @@ struct Tracer {
             sigma_bw(x, dot(dL_dW, s == 0.0f ? W_out : W_out * (null_W / s)));
             if (s == 0.0f) break;
             f3 wo;
-            phase_sample(M, rng, x, w, wo);
+            phase_sample(med(), rng, x, w, wo);
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return;
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) return;
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
         }
     }
 
@@ -1435,40 +1469,40 @@ struct Tracer {
     // include/vr_path_integrator_NEE.h:1-66 with only_scatters = true
     VR_DEV f3 nee_scatters(f3 x, f3 w) {
         float tmin, tmax;
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         float d = tmax - tmin;
         f3 R = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
         const float EPS = 0.00001f;
         int bounces = 0;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > EPS) {
             float md;
-            float m = fmaxf(EPS, mj.query(M, md));
+            float m = fmaxf(EPS, mj.query(med(), md));
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
             if (rnd() * m >= s) { d -= t; continue; }
             f3 sa = albedo(x);
-            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
+            R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * med().emission_c;
             W = W * sa;
             f3 wnee;
-            f3 env = env_sample(M, rng, wnee);
+            f3 env = env_sample(med(), rng, wnee);
             float Tn = transmittance(x, wnee);
-            R = R + (W * env) * (Tn * phase_eval(M, x, w, wnee));
+            R = R + (W * env) * (Tn * phase_eval(med(), x, w, wnee));
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) return R;
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) return R;
             f3 wo;
-            float rho = phase_sample(M, rng, x, w, wo);
+            float rho = phase_sample(med(), rng, x, w, wo);
             W = W * rho;
             if (W.x < EPS && W.y < EPS && W.z < EPS) return R;
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
         }
         return R;
     }
@@ -1480,17 +1514,17 @@ struct Tracer {
         float T = 1.0f, t = 0.0f, res_t = 0.0f, res_dt = 0.0f, C = 0.0f;
         const float EPS = 0.0000001f;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         for (;;) {
             float md;
-            float m = fmaxf(EPS, mj.query(M, md));
+            float m = fmaxf(EPS, mj.query(med(), md));
             float dt = fminf(flight(m), fminf(md, d - t));
             float w_step = T * dt;
             C += w_step;
             if (rnd() < w_step / C) { res_t = t; res_dt = dt; }
             t += dt;
             if (t >= d - EPS) break;  // before a crossing: the last cell's exit can coincide with the end
-            if (dt >= md - EPS) { mj.cross(M, w); continue; }
+            if (dt >= md - EPS) { mj.cross(med(), w); continue; }
             mj.move(dt);
             T *= 1.0f - sigma(x + w * t) / m;
         }
@@ -1506,14 +1540,14 @@ struct Tracer {
     // every transmittance backward.
     VR_DEV void nee_drt_bw(f3 x, f3 w, f3 dL_dR) {
         float tmin, tmax;
-        boundary(M, x, w, tmin, tmax);
+        boundary(med(), x, w, tmin, tmax);
         x = x + w * tmin;
         float d = tmax - tmin;
         if (d <= 0.0f) return;
         Rng secondary = create_branch_seed(), main_s;
         transmittance(x, w);  // the primary term's T: draws from the main stream
         main_s = rng; rng = secondary;
-        transmittance_bw(x, w, dot(dL_dR, env_eval(M, w)));
+        transmittance_bw(x, w, dot(dL_dR, env_eval(med(), w)));
         secondary = rng; rng = main_s;
 
         f3 R = F3(0.0f, 0.0f, 0.0f), W = F3(1.0f, 1.0f, 1.0f);
@@ -1525,31 +1559,31 @@ struct Tracer {
         const Rng before_path = rng;
         int bounces = 0;
         Walk<MAJ> mj;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > EPS) {
             float md;
-            float m = fmaxf(0.1f, mj.query(M, md) * 1.05f + 0.1f);  // a NaN cell bound counts as empty
+            float m = fmaxf(0.1f, mj.query(med(), md) * 1.05f + 0.1f);  // a NaN cell bound counts as empty
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
             if (rnd() >= s / m) { d -= t; continue; }
             W = W * albedo(x);
             f3 wnee;
-            f3 env = env_sample(M, rng, wnee);
+            f3 env = env_sample(med(), rng, wnee);
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, x, w, wnee);
+            float phn = phase_eval(med(), x, w, wnee);
             R = R + (W * env) * (Tn * phn);
             bounces++;
-            if (M.max_bounces >= 0 && bounces >= M.max_bounces) break;
+            if (med().max_bounces >= 0 && bounces >= med().max_bounces) break;
             f3 wo;
-            float rho = phase_sample(M, rng, x, w, wo);
+            float rho = phase_sample(med(), rng, x, w, wo);
             W = W * rho;
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
             float cur_P = W.x + W.y + W.z;
             total_weight += cur_P;
             if (rnd() < cur_P / total_weight) { res_P = cur_P; res_W = W; res_x = x; res_w = w; }
@@ -1560,13 +1594,13 @@ struct Tracer {
         x = ini_x; w = ini_w; d = ini_d;
         rng = before_path;
         bounces = 0;
-        mj.begin(M, x, w);
+        mj.begin(med(), x, w);
         while (d > EPS) {
             float md;
-            float m = fmaxf(0.1f, mj.query(M, md) * 1.05f + 0.1f);  // a NaN cell bound counts as empty
+            float m = fmaxf(0.1f, mj.query(med(), md) * 1.05f + 0.1f);  // a NaN cell bound counts as empty
             float t = flight(m);
             x = x + w * fminf(t, md);
-            if (t > md) { d -= md; mj.cross(M, w); if (d <= kCrossEps) break; continue; }
+            if (t > md) { d -= md; mj.cross(med(), w); if (d <= kCrossEps) break; continue; }
             if (t > d) break;
             mj.move(t);
             float s = sigma(x);
@@ -1582,13 +1616,13 @@ struct Tracer {
             float g = dot(dL_dR, R * ((s * s / (1.0f + s * s)) / s));
             W = W * albedo(x);
             f3 wnee;
-            f3 env = env_sample(M, rng, wnee);
+            f3 env = env_sample(med(), rng, wnee);
             float Tn = transmittance(x, wnee);
-            float phn = phase_eval(M, x, w, wnee);
+            float phn = phase_eval(med(), x, w, wnee);
             bounces++;
-            const bool last = M.max_bounces >= 0 && bounces >= M.max_bounces;
+            const bool last = med().max_bounces >= 0 && bounces >= med().max_bounces;
             f3 wo = w;
-            float rho = last ? 1.0f : phase_sample(M, rng, x, w, wo);
+            float rho = last ? 1.0f : phase_sample(med(), rng, x, w, wo);
             float dL_dTn = dot(dL_dR, (W * env) * phn);
             R = R - (W * env) * (Tn * phn);
             main_s = rng; rng = secondary;
@@ -1598,21 +1632,21 @@ struct Tracer {
             if (last) break;
             W = W * rho;
             w = wo;
-            boundary(M, x, w, tmin, d);
-            mj.begin(M, x, w);
+            boundary(med(), x, w, tmin, d);
+            mj.begin(med(), x, w);
             rnd();  // the reservoir draw of the primal pass
             if (W.x < EPS && W.y < EPS && W.z < EPS) break;
         }
 
-        boundary(M, res_x, res_w, tmin, d);
+        boundary(med(), res_x, res_w, tmin, d);
         float s_t, weight_t;
         sample_transmittance_position(res_x, res_w, d, s_t, weight_t);
         f3 xs = res_x + res_w * s_t;
         f3 lw;
-        f3 Lenv = env_sample(M, rng, lw);
-        f3 Ls = Lenv * (transmittance(xs, lw) * phase_eval(M, xs, res_w, lw));
+        f3 Lenv = env_sample(med(), rng, lw);
+        f3 Ls = Lenv * (transmittance(xs, lw) * phase_eval(med(), xs, res_w, lw));
         f3 ws;
-        float rho_s = phase_sample(M, rng, xs, res_w, ws);
+        float rho_s = phase_sample(med(), rng, xs, res_w, ws);
         Ls = Ls + nee_scatters(xs, ws) * rho_s;
         Ls = ((Ls * res_W) * weight_t) * albedo(xs);
         float sxs = sigma(xs);
@@ -1626,7 +1660,7 @@ struct Tracer {
     VR_DEV void medium_dt(f3 x, f3 w, float d, float* out) {
         f3 W = F3(1.0f, 1.0f, 1.0f), A = F3(0.0f, 0.0f, 0.0f);
         f3 xo = w * d + x, wo = w;
-        float maj = fmaxf(0.00001f, M.medium_majorant);
+        float maj = fmaxf(0.00001f, med().medium_majorant);
         for (;;) {
             float t = flight(maj);
             if (t > d - 0.00001f) break;
@@ -1634,12 +1668,12 @@ struct Tracer {
             d -= t;
             float s = sigma(x);
             if (rnd() < s / maj) {
-                if (M.medium_filter & 1) {
+                if (med().medium_filter & 1) {
                     xo = x;
-                    phase_sample(M, rng, x, w, wo);
+                    phase_sample(med(), rng, x, w, wo);
                     W = albedo(x);
                 } else W = F3(0.0f, 0.0f, 0.0f);
-                if (M.medium_filter & 2) A = M.gathering_c;
+                if (med().medium_filter & 2) A = med().gathering_c;
                 break;
             }
         }
@@ -1655,8 +1689,8 @@ struct Tracer {
         f3 xi = x, wi = w;
         float di = d;
         Rng before = rng;
-        float maj = fmaxf(0.00001f, M.medium_majorant);
-        int f = M.medium_filter;
+        float maj = fmaxf(0.00001f, med().medium_majorant);
+        int f = med().medium_filter;
         for (;;) {
             float t = flight(maj);
             if (t > d - 0.00001f) break;
@@ -1664,9 +1698,9 @@ struct Tracer {
             d -= t;
             float s = sigma(x);
             if (rnd() < s / maj) {
-                if (f & 1) { f3 wo; float wt = phase_sample(M, rng, x, w, wo); W = albedo(x) * wt; }
+                if (f & 1) { f3 wo; float wt = phase_sample(med(), rng, x, w, wo); W = albedo(x) * wt; }
                 else W = F3(0.0f, 0.0f, 0.0f);
-                if (f & 2) A = M.gathering_c;
+                if (f & 2) A = med().gathering_c;
                 break;
             }
         }
@@ -1679,10 +1713,10 @@ struct Tracer {
     // same state (the scene backward, sc_path_tracing.h:281-296) skips the
     // primal pass.
     VR_DEV void medium_dt_replay(f3 x, f3 w, float d, f3 W, f3 A, f3 dL_dW, f3 dL_dA) {
-        const float maj = fmaxf(0.00001f, M.medium_majorant);
-        const int f = M.medium_filter;
+        const float maj = fmaxf(0.00001f, med().medium_majorant);
+        const int f = med().medium_filter;
         const bool scat_blk = (f & 1) != 0;
-        const bool acc_blk = M.bw_mode == VR_BW_REFERENCE ? (f & 1) != 0 : (f & 2) != 0;
+        const bool acc_blk = med().bw_mode == VR_BW_REFERENCE ? (f & 1) != 0 : (f & 2) != 0;
         for (;;) {
             float t = flight(maj);
             if (t > d - 0.00001f) break;
@@ -1694,7 +1728,7 @@ struct Tracer {
                 float g = 0.0f;
                 if (scat_blk) {
                     f3 wo;
-                    phase_sample(M, rng, x, w, wo);
+                    phase_sample(med(), rng, x, w, wo);
                     g += dot(dL_dW, W * (1.0f / s));
                     albedo_bw(x, dL_dW);
                 }

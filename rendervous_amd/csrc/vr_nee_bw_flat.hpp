@@ -64,7 +64,7 @@ struct NeeDsBwLane {
     f3 dL_dR;
     int state, pass, scattered, bounces, start_kind, after;
 
-    VR_DEV explicit NeeDsBwLane(const MediumView& m) : tr(m) {}
+    VR_DEV explicit NeeDsBwLane(ConstPtr<MediumView> m) : tr(m) {}
 
     // Pass 1 (or 2) of the sample from its camera ray: nee_ds_bw's start
     // (vr_path_integrator_NEE_DS.h:6-30) and its replay restart (:110-125).
@@ -97,7 +97,7 @@ struct NeeDsBwLane {
     // continued path runs to the box exit, a shadow ray that misses the box has
     // T = 1 (its backward, nothing to do).
     VR_DEV void start() {
-        const MediumView& M = tr.M;
+        const MediumView& M = tr.med();
         float tmin, tmax;
         const bool hit = boundary(M, rx, rw, tmin, tmax);
         int next = kBwPath;
@@ -145,7 +145,7 @@ struct NeeDsBwLane {
     // wave holding several kinds would otherwise run each branch).  Every
     // lane's draws and arithmetic are those of Tracer::nee_ds_bw.
     VR_DEV void step() {
-        const MediumView& M = tr.M;
+        const MediumView& M = tr.med();
         const bool path = state == kBwPath;
         if (path && !(rd > EPS)) { state = kBwPathEnd; return; }
         float md;
@@ -211,7 +211,7 @@ struct NeeDsBwLane {
     // or pass 2 (:140-200; the albedo gradient is sent before the shadow ray,
     // it uses no random number), then the light sample and the shadow ray.
     VR_DEV void collision() {
-        const MediumView& M = tr.M;
+        const MediumView& M = tr.med();
         const float s = cs, m = cm, Pc = cpc;
         const f3 sa = tr.albedo(rx);
         if (pass == 1) {
@@ -246,7 +246,7 @@ struct NeeDsBwLane {
     // samples the next direction (:53-72); pass 2 samples the next direction,
     // subtracts the term and starts the shadow ray's backward (:170-196).
     VR_DEV void after_t() {
-        const MediumView& M = tr.M;
+        const MediumView& M = tr.med();
         const float phn = phase_eval(M, px, pw, rw);
         bool more = true;
         if (pass == 1) {
@@ -287,7 +287,7 @@ struct NeeDsBwLane {
     }
 
     VR_DEV void after_tbw() {
-        const MediumView& M = tr.M;
+        const MediumView& M = tr.med();
         tr.rng = before_next;
         bounces++;
         if (M.max_bounces >= 0 && bounces >= M.max_bounces) { state = kBwDone; return; }
@@ -298,7 +298,7 @@ struct NeeDsBwLane {
 
     VR_DEV void path_end() {
         if (pass == 1) {
-            if (!scattered) R = R + (ENVMAP ? env_eval(tr.M, rw) : tr.M.env_c) * W;
+            if (!scattered) R = R + (ENVMAP ? env_eval(tr.med(), rw) : tr.med().env_c) * W;
             begin_pass(2);
         } else state = kBwDone;
     }

@@ -398,19 +398,9 @@ struct NeeArgs {
     int* dbg;
 };
 
-// The kernel's arguments read through an address the compiler cannot follow:
-// the kernel-argument segment pointer passes through an empty asm statement, so
-// the loads of the fields a handler reads stay in that handler instead of being
-// hoisted out of the persistent loop.  Hoisted, every field of the medium, the
-// camera and the seeds stays live in scalar registers for the whole kernel,
-// more than the 102 a wave has, and the excess spills to vector-register lanes
-// (a v_readlane per reload on the vector pipe).  Re-reading them costs scalar
-// loads from the constant segment, which the scalar cache serves.
-VR_DEV const NeeArgs& nee_args() {
-    auto p = (const __attribute__((address_space(4))) NeeArgs*)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(p));
-    return *(const NeeArgs*)p;
-}
+// The kernel's arguments, re-read where a handler uses them (reread): the
+// loads of the fields each handler reads stay in that handler.
+VR_DEV const NeeArgs& nee_args() { return *(const NeeArgs*)reread(first_kernel_arg<NeeArgs>()); }
 
 template <int MAJ, int TKIND, bool ENVMAP, bool LEAN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))

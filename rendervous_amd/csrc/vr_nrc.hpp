@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __r
 // where the path stops and hands (x, w, W) to the cache.
 template <int MAJ>
 VR_DEV f3 nee_cached(Tracer<MAJ>& tr, f3 x, f3 w, int K, f3& qx, f3& qw, f3& qW, bool& emit) {
-    const MediumView& M = tr.M;
+    const MediumView& M = tr.med();
     emit = false;
     float tmin, tmax;
     boundary(M, x, w, tmin, tmax);
@@ -463,7 +463,7 @@ VR_DEV f3 nee_cached(Tracer<MAJ>& tr, f3 x, f3 w, int K, f3& qx, f3& qw, f3& qW,
 // radiance.
 template <int MAJ>
 VR_DEV int nee_record(Tracer<MAJ>& tr, f3 x, f3 w, int maxv, float* __restrict__ tmp, f3& R_end) {
-    const MediumView& M = tr.M;
+    const MediumView& M = tr.med();
     float tmin, tmax;
     boundary(M, x, w, tmin, tmax);
     x = x + w * tmin;
@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VR_NRC_
                                                    int64_t max_records, uint32_t* __restrict__ count) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n_paths) return;
-    Tracer<MAJ> tr(M);
+    Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     tr.rng = seed_map(seeds, (uint32_t)j);
     const int64_t i = min(total - 1, (int64_t)(tr.rnd() * (float)total));
     int pose, row, col;
@@ -575,7 +575,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NRC
     const int s0 = (int)c * chunk, s1 = ok ? min(spp, s0 + chunk) : s0;
     const float inv = 1.0f / (float)spp;
     f3 acc = F3(0.0f, 0.0f, 0.0f);
-    Tracer<MAJ> tr(M);
+    Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     // every lane runs `chunk` iterations so that the query slots are taken by
     // wave-wide ballots at the same point of the loop
     for (int s = s0; s < s0 + chunk; s++) {

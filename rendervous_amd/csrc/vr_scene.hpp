@@ -258,9 +258,11 @@ VR_DEV f3 surface_emission(const vr_patch& P, f3 win, const Surfel& s) {
 }
 
 // Medium segment (include/dyn_medium_traversal.h, DTMediumPathIntegrator fw).
+// The medium is read in the kernel-argument segment: only called from kernels
+// whose first parameter is the SceneView S.
 VR_DEV void medium_traverse(const SceneView& S, int m, Rng& rng, f3 x, f3 w, float d, f3& xo, f3& wo, f3& W, f3& A,
                             int& lookups) {
-    Tracer<VR_MAJ_CONST> tr(S.media[m]);
+    Tracer<VR_MAJ_CONST> tr(&first_kernel_arg<SceneView>()->media[m]);
     tr.rng = rng;
     float out[12];
     tr.medium_dt(x, w, d, out);
@@ -277,7 +279,7 @@ VR_DEV void medium_traverse(const SceneView& S, int m, Rng& rng, f3 x, f3 w, flo
 // primal pass would recompute bit for bit, so only its replay runs.
 VR_DEV void medium_traverse_bw(const SceneView& S, int m, Rng& rng, f3 x, f3 w, float d, f3 W, f3 A, f3 dL_dW, f3 dL_dA,
                                int& lookups, int& scatters) {
-    Tracer<VR_MAJ_CONST> tr(S.media[m]);
+    Tracer<VR_MAJ_CONST> tr(&first_kernel_arg<SceneView>()->media[m]);
     tr.rng = rng;
     tr.medium_dt_replay(x, w, d, W, A, dL_dW, dL_dA);
     rng = tr.rng;
