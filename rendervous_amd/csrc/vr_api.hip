@@ -402,6 +402,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_bw(MediumVi
     if (i >= start + cnt) return;
     Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     tr.rng = seed_map(seeds, (uint32_t)i);
+    tr.stream_ends = true;
     float a[Dims<KIND>::in], o[Dims<KIND>::out];
 #pragma unroll
     for (int k = 0; k < Dims<KIND>::in; k++) a[k] = in[i * Dims<KIND>::in + k];
@@ -515,6 +516,7 @@ __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_bw(Medi
     Tracer<MAJ> tr(first_kernel_arg<MediumView>());
     // the reference's fresh-seed backward continues one stream over the samples (one chunk)
     if (seed_mode == VR_SEED_CAPTURE_BW) tr.rng = seed_capture_bw(seeds, (uint32_t)i);
+    tr.stream_ends = seed_mode != VR_SEED_CAPTURE_BW;  // per-sample streams
     for (int s = s0; s < s1; s++) {
         if (seed_mode != VR_SEED_CAPTURE_BW) tr.rng = seed_capture_fw(seeds, (uint32_t)((int64_t)s * total + i));
         f3 x, w;

@@ -632,8 +632,12 @@ struct Tracer {
     Rng rng;
     int lookups;   // trilinear sigma lookups
     int scatters;  // trilinear sigma gradient scatters
+    // the caller draws nothing from rng after the estimator returns (each
+    // sample or ray seeds its own stream): a backward may then skip primal
+    // estimates whose only use is a gradient that is not requested
+    bool stream_ends;
 
-    VR_DEV explicit Tracer(ConstPtr<MediumView> m) : Mp(m), lookups(0), scatters(0) {}
+    VR_DEV explicit Tracer(ConstPtr<MediumView> m) : Mp(m), lookups(0), scatters(0), stream_ends(false) {}
 
     VR_DEV const MediumView& med() const { return *(const MediumView*)Mp; }
     // Loads of the medium's fields after this call are not hoisted above it
@@ -657,6 +661,7 @@ struct Tracer {
         if (med().albedo.C == 1) return F3(v[0], v[0], v[0]);
         return F3(v[0], v[1], v[2]);
     }
+    VR_DEV bool albedo_has_grad() const { return med().albedo_grid && med().albedo.grad != nullptr; }
     VR_DEV void albedo_bw(f3 x, f3 g) {
         if (!med().albedo_grid || med().albedo.grad == nullptr) return;
         if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) return;
@@ -1036,12 +1041,19 @@ struct Tracer {
         return C;
     }
 
-    // Defensive primal pass of vr_collision_sampler_DT.h:52-90.
-    VR_DEV void collision_ds_primal(f3 x, f3 w, float eps, float tmin, float tmax, float& C, float& t_out, float& Wc) {
+    // Defensive primal pass of vr_collision_sampler_DT.h:52-90 (majorant
+    // m + 2 eps, collision with Pc = (s + eps) / m) from the box entry: whether
+    // it collides (C), the distance t from the entry, the product Wpre of the
+    // null-collision weights (1 - s/m) / (1 - Pc) before the collision and
+    // s_t = (s/m) / Pc at the collision (0 without one).
+    struct CollisionPrimal {
+        float C, t, Wpre, s_t;
+    };
+    VR_DEV CollisionPrimal collision_ds_walk(f3 x, f3 w, float eps, float tmin, float tmax) {
         float d = tmax - tmin;
         x = x + w * tmin;
-        float t = 0.0f, W = 1.0f;
-        C = 0.0f;
+        CollisionPrimal P;
+        P.C = 0.0f; P.t = 0.0f; P.Wpre = 1.0f; P.s_t = 0.0f;
         Walk<MAJ> mj;
         mj.begin(med(), x, w);
         for (;;) {
@@ -1049,28 +1061,28 @@ struct Tracer {
             float maj = fmaxf(0.0000000001f, mj.query(med(), md) + 2.0f * eps);
             float dt = fminf(flight(maj), fminf(d, md));
             x = x + w * dt;
-            t += dt;
-            if (dt >= md) { mj.cross(med(), w); if (t >= d - kCrossEps) break; continue; }
+            P.t += dt;
+            if (dt >= md) { mj.cross(med(), w); if (P.t >= d - kCrossEps) break; continue; }
             mj.move(dt);
-            if (t >= d - 0.0000001f) break;
+            if (P.t >= d - 0.0000001f) break;
             float s = sigma(x);
             float Pc = (s + eps) / maj;
-            if (rnd() < Pc) { C = 1.0f; W *= (s / maj) / Pc; break; }
-            W *= (1.0f - s / maj) / (1.0f - Pc);
+            if (rnd() < Pc) { P.C = 1.0f; P.s_t = (s / maj) / Pc; break; }
+            P.Wpre *= (1.0f - s / maj) / (1.0f - Pc);
         }
-        t_out = t;
-        Wc = W;
+        return P;
     }
 
-    // include/vr_collision_sampler_DT.h:46-134 (VR_BW_CONSISTENT: full path weight)
-    VR_DEV void collision_dt_bw(f3 x, f3 w, float eps, float dL_dC, float dL_dT) {
-        float tmin, tmax;
-        if (!boundary(med(), x, w, tmin, tmax)) return;
-        Rng saved = rng;
-        float out_T = 1.0f, C = 0.0f, s_t = 0.0f, Wpre = 1.0f;
+    // Replay pass of vr_collision_sampler_DT.h:92-134 given the primal pass P
+    // drawn from the current random state (VR_BW_CONSISTENT: full path weight).
+    VR_DEV void collision_dt_replay(f3 x, f3 w, float eps, float tmin, float tmax, const CollisionPrimal& P,
+                                    float dL_dC, float dL_dT) {
+        const float C = P.C;
+        const float out_T = C != 0.0f ? 0.0f : P.Wpre;
+        const bool consistent = med().bw_mode == VR_BW_CONSISTENT;
+        const float s_t = consistent ? P.s_t * P.Wpre : P.s_t;
         float d = tmax - tmin;
         x = x + w * tmin;
-        f3 x0 = x;
         float t = 0.0f;
         Walk<MAJ> mj;
         mj.begin(med(), x, w);
@@ -1085,30 +1097,9 @@ struct Tracer {
             if (t >= d - 0.0000001f) break;
             float s = sigma(x);
             float Pc = (s + eps) / maj;
-            if (rnd() < Pc) { out_T = 0.0f; C = 1.0f; s_t = s / maj / Pc; break; }
-            float r = (1.0f - s / maj) / (1.0f - Pc);
-            out_T *= r;
-            Wpre *= r;
-        }
-        bool consistent = med().bw_mode == VR_BW_CONSISTENT;
-        if (consistent) s_t *= Wpre;
-        rng = saved;
-        x = x0; t = 0.0f;
-        mj.begin(med(), x, w);
-        for (;;) {
-            float md;
-            float maj = fmaxf(0.0000000001f, mj.query(med(), md) + 2.0f * eps);
-            float dt = fminf(flight(maj), fminf(d, md));
-            x = x + w * dt;
-            t += dt;
-            if (dt >= md) { mj.cross(med(), w); if (t >= d - kCrossEps) break; continue; }
-            mj.move(dt);
-            if (t >= d - 0.0000001f) break;
-            float s = sigma(x);
-            float Pc = (s + eps) / maj;
             if (rnd() < Pc) {
                 float tr = C / Pc;
-                if (consistent) tr *= Wpre;
+                if (consistent) tr *= P.Wpre;
                 sigma_bw(x, dL_dC * tr * (1.0f / maj));
                 break;
             } else {
@@ -1116,6 +1107,17 @@ struct Tracer {
                 sigma_bw(x, q * (-1.0f / maj));
             }
         }
+    }
+
+    // include/vr_collision_sampler_DT.h:46-134: the primal pass, then the
+    // replay from the same random state.
+    VR_DEV void collision_dt_bw(f3 x, f3 w, float eps, float dL_dC, float dL_dT) {
+        float tmin, tmax;
+        if (!boundary(med(), x, w, tmin, tmax)) return;
+        Rng saved = rng;
+        CollisionPrimal P = collision_ds_walk(x, w, eps, tmin, tmax);
+        rng = saved;
+        collision_dt_replay(x, w, eps, tmin, tmax, P, dL_dC, dL_dT);
     }
 
     // include/maps/single_scattered_radiance_mc.h:6-16
@@ -1134,10 +1136,16 @@ struct Tracer {
         f3 env = env_sample(med(), rng, wo);
         float ph = phase_eval(med(), x, w * -1.0f, wo);
         f3 sa = albedo(x);
+        float dL_dT = dot(dL_dR, (env * sa) * ph);
+        if (med().bw_mode == VR_BW_CONSISTENT && stream_ends && !albedo_has_grad()) {
+            // T enters only the albedo gradient, and no draw follows this
+            // estimator: the shadow ray's primal estimate is not needed
+            transmittance_bw(x, wo, dL_dT);
+            return;
+        }
         Rng before_T = rng;
         float T = transmittance(x, wo);
         f3 dL_dsa = (dL_dR * env) * (ph * T);
-        float dL_dT = dot(dL_dR, (env * sa) * ph);
         albedo_bw(x, dL_dsa);
         if (med().bw_mode == VR_BW_CONSISTENT) {
             Rng after = rng;
@@ -1172,15 +1180,19 @@ struct Tracer {
         }
         float tmin, tmax;
         if (!boundary(med(), x, w, tmin, tmax)) return;
+        // one primal pass serves the exitance estimate and the sampler's replay
+        // (the nested form of the reference walks it twice from the same state)
         Rng saved = rng;
-        float C, t, Wc;
-        collision_ds_primal(x, w, med().ds_eps, tmin, tmax, C, t, Wc);
-        f3 xt = x + w * (tmin + t);
+        const float eps = med().ds_eps;
+        CollisionPrimal P = collision_ds_walk(x, w, eps, tmin, tmax);
+        const float C = P.C;
+        const float Wc = C != 0.0f ? P.Wpre * P.s_t : P.Wpre;
+        f3 xt = x + w * (tmin + P.t);
         f3 eR = F3(0.0f, 0.0f, 0.0f);
         if (C != 0.0f) eR = ss_radiance(xt, w * -1.0f);
         Rng after = rng;
         rng = saved;
-        collision_dt_bw(x, w, med().ds_eps, dot(dL_dR, eR), dot(dL_dR, env_eval(med(), w)));
+        collision_dt_replay(x, w, eps, tmin, tmax, P, dot(dL_dR, eR), dot(dL_dR, env_eval(med(), w)));
         rng = after;
         if (C != 0.0f) ss_radiance_bw(xt, w * -1.0f, dL_dR * Wc);
     }
