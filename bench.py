@@ -720,11 +720,16 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32,
         rates = [lr[0] * (lr[1] / lr[0]) ** (k / max(1, n_steps - 1)) for k in range(n_steps)]
         cache.render(field, camera, spp=32)  # untimed: sizes the query buffer once
 
+        trace_seeds = []
+        for t in range(traces):
+            rv.seed(5000 + t)
+            trace_seeds.append(rv._internal.get_seeds())
+
         def train():
-            for t in range(traces):
-                rv.seed(5000 + t)
-                cache.train(field, camera, n_paths=n_paths, max_vertices=32, steps=steps_per_trace, batch=batch,
-                            lr=rates[t * steps_per_trace:(t + 1) * steps_per_trace])
+            # the traces' paths (independent of the cache) in one launch, then
+            # steps_per_trace Adam steps on each trace's records in turn
+            cache.train(field, camera, n_paths=n_paths, max_vertices=32, steps=steps_per_trace, batch=batch,
+                        lr=rates, traces=traces, seeds=trace_seeds)
         _, train_ms = timed(train)
         ladder, hit = [], None
         spp = 4
@@ -749,7 +754,9 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32,
     nrc_ms = train_ms + hit["ms"] if hit else None
     return {
         "workload": f"time to RMSE <= plain {target_spp} spp RMSE, {res}^3 cloud fBm x60, albedo 0.99, g 0.8, "
-                    f"{hw}x{hw}, max {max_bounces} bounces, vs a {ref_spp}-spp plain render",
+                    f"{hw}x{hw}, max {max_bounces} bounces, vs a {ref_spp}-spp plain render; cache trained on "
+                    f"{traces} traces of {n_paths} NEE training paths (<= 32 records each), {steps_per_trace} Adam "
+                    f"steps of {batch} records per trace; cached render ends paths after 2 bounces",
         "epsilon": eps, "plain_ms": plain_ms, "plain_spp": target_spp,
         "nrc_train_ms": train_ms, "nrc_render": hit, "nrc_ms": nrc_ms,
         "speedup": plain_ms / nrc_ms if nrc_ms else None, "ladder": ladder,

@@ -411,9 +411,12 @@ int vr_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels,
  * ReLU MLP in BF16 with FP32 accumulation.  Q(x, w) = radiance a NEE path collects
  * from a real collision at x, reached along w, onwards.  Weights: VR_NRC_PARAMS
  * values, per layer Wt[n][k] (output n, input k) for layers 96->64, 64->64 (x3),
- * 64->16 (outputs 3..15 unused); `params` FP32 master copy, `weights` BF16 bits. */
+ * 64->16 (outputs 3..15 unused); `params` FP32 master copy, `weights` BF16 bits
+ * (16-byte aligned). */
 #define VR_NRC_PARAMS 19456
 #define VR_NRC_FEATURES 96
+/* Copies of the weight gradient a training step accumulates (one per XCD). */
+#define VR_NRC_GRAD_COPIES 8
 
 /* Input features of n points x and directions w ([n][3] each): sin/cos of
  * pi 2^f u_c (f < 12, u = x mapped from the box to [0,1]^3), w, 1, zeros. */
@@ -426,9 +429,11 @@ int vr_nrc_mlp_fw(const float* features, int64_t n, const uint16_t* weights, flo
 /* One training step on the records (x[3], w[3], target[3]) of rows
  * [f, min(*count, f + max_records)) of `records`, f = first % *count (*count on
  * the device: the rows written), so that windows first = k * max_records cycle
- * through the records: relative L2 loss, gradient into `grad` (overwritten), Adam step
- * `step` (>= 1) on `params`/`m`/`v`, BF16 copy into `weights`, mean loss into
- * *loss (device).  lr = 0 leaves the weights. */
+ * through the records: relative L2 loss, its gradient accumulated in `grad`
+ * (VR_NRC_GRAD_COPIES * VR_NRC_PARAMS floats, overwritten; on return the first
+ * VR_NRC_PARAMS hold the step's gradient), Adam step `step` (>= 1) on
+ * `params`/`m`/`v`, BF16 copy into `weights`, mean loss into *loss (device).
+ * lr = 0 leaves the weights. */
 int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first, int64_t max_records, const float box_min[3],
                       const float box_max[3], float* params, uint16_t* weights, float* grad, float* m, float* v,
                       int32_t step, float lr, float beta1, float beta2, float eps, float* loss, void* stream);
@@ -440,6 +445,15 @@ int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first
 int vr_nrc_paths(const vr_medium* medium, const vr_camera* camera, const vr_launch* launch, int32_t n_paths,
                  int32_t max_vertices, float* scratch, float* records, int64_t max_records, uint32_t* count,
                  void* stream);
+
+/* vr_nrc_paths for `traces` (1..16) independent traces in one launch: trace t
+ * draws its paths from launches[t].seeds and writes records
+ * [t * max_records, (t + 1) * max_records) of `records` (9 floats each) and
+ * counts[t]; scratch: traces * n_paths * max_vertices * 12 floats.  The paths do
+ * not read the cache, so this equals `traces` calls of vr_nrc_paths. */
+int vr_nrc_paths_traces(const vr_medium* medium, const vr_camera* camera, const vr_launch* launches, int32_t traces,
+                        int32_t n_paths, int32_t max_vertices, float* scratch, float* records, int64_t max_records,
+                        uint32_t* counts, void* stream);
 
 /* Capture of the NEE radiance (map kind VR_MAP_NEE_DT) whose paths stop at their
  * (terminate_after + 1)-th collision and add W * max(Q, 0) there: out

@@ -240,6 +240,9 @@ _SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_void_p]),
     "vr_nrc_paths": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "vr_nrc_paths_traces": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                           ctypes.c_void_p]),
     "vr_nrc_render": (ctypes.c_int, [P(VrMedium), P(VrCamera), P(VrLaunch), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
     "vr_grid_pack_cells": (ctypes.c_int, [P(VrGrid), ctypes.c_void_p, ctypes.c_void_p]),
@@ -266,6 +269,7 @@ def u32s(values) -> ctypes.Array:
 
 
 NRC_PARAMS = 19456
+NRC_GRAD_COPIES = 8  # weight-gradient copies of a training step (VR_NRC_GRAD_COPIES)
 NRC_FEATURES = 96
 
 # Symbols declared in include/vr_api.h (checked by the CPU test suite).

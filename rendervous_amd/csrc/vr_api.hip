@@ -1590,6 +1590,7 @@ int vr_nrc_mlp_fw(const float* features, int64_t n, const uint16_t* weights, flo
     if (n < 0) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: negative n");
     if (n == 0) return VR_OK;
     if (!features || !weights || !out) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: null buffer");
+    if (((uintptr_t)weights & 15u) != 0) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: weights must be 16-byte aligned");
     hipLaunchKernelGGL(nrc::k_nrc_mlp_fw, dim3(nrc_blocks(n, nrc::kInferWaves, 4096)), dim3(64 * nrc::kInferWaves), 0,
                        (hipStream_t)stream, features, n, (const __bf16*)weights, out);
     return check_hip(hipGetLastError(), "vr_nrc_mlp_fw launch");
@@ -1605,8 +1606,9 @@ int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first
     f3 o, ib;
     int st = box_transform(box_min, box_max, o, ib);
     if (st) return st;
+    if (((uintptr_t)weights & 15u) != 0) return set_err(VR_ERR_ARG, "vr_nrc_train_step: weights must be 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(kBlock), 0, s, grad, (int64_t)nrc::kParams);
+    hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(kBlock), 0, s, grad, (int64_t)nrc::kGradCopies * nrc::kParams);
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(kBlock), 0, s, loss, (int64_t)1);
     if (max_records > 0)
         hipLaunchKernelGGL(nrc::k_nrc_train, dim3(nrc_blocks(max_records, nrc::kTrainWaves, 1024)), dim3(64 * nrc::kTrainWaves),
@@ -1618,31 +1620,40 @@ int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first
     return check_hip(hipGetLastError(), "vr_nrc_train_step launch");
 }
 
-int vr_nrc_paths(const vr_medium* medium, const vr_camera* camera, const vr_launch* L, int32_t n_paths,
-                 int32_t max_vertices, float* scratch, float* records, int64_t max_records, uint32_t* count,
-                 void* stream) {
-    if (!L) return set_err(VR_ERR_ARG, "vr_nrc_paths: launch is NULL");
+int vr_nrc_paths_traces(const vr_medium* medium, const vr_camera* camera, const vr_launch* launches, int32_t traces,
+                        int32_t n_paths, int32_t max_vertices, float* scratch, float* records, int64_t max_records,
+                        uint32_t* counts, void* stream) {
+    if (!launches) return set_err(VR_ERR_ARG, "vr_nrc_paths: launch is NULL");
+    if (traces < 1 || traces > nrc::kMaxTraces) return set_err(VR_ERR_ARG, "vr_nrc_paths_traces: traces must be in [1, 16]");
     if (n_paths < 0 || max_vertices < 1 || max_records < 0) return set_err(VR_ERR_ARG, "vr_nrc_paths: bad sizes");
-    if (!scratch || !records || !count) return set_err(VR_ERR_ARG, "vr_nrc_paths: null buffer");
+    if (!scratch || !records || !counts) return set_err(VR_ERR_ARG, "vr_nrc_paths: null buffer");
     MediumView M;
-    int st = make_medium_view(medium, VR_MAP_NEE_DT, L->bw_mode, M);
+    int st = make_medium_view(medium, VR_MAP_NEE_DT, launches[0].bw_mode, M);
     if (st) return st;
     CameraView C;
     if ((st = make_camera_view(camera, C))) return st;
     const int64_t total = camera->sensors ? camera->num_sensors : (int64_t)camera->num_poses * camera->width * camera->height;
     if (total <= 0) return set_err(VR_ERR_ARG, "vr_nrc_paths: no sensors");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count, 1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counts, traces);
     if (n_paths > 0) {
-        const Seeds sd = make_seeds(L);
+        nrc::TraceSeeds sd = {};
+        for (int t = 0; t < traces; t++) sd.s[t] = make_seeds(&launches[t]);
+        const dim3 g((n_paths + 63) / 64, traces);
         if (medium->majorant_kind == VR_MAJ_GRID)
-            hipLaunchKernelGGL(nrc::k_nrc_paths<VR_MAJ_GRID>, dim3((n_paths + 63) / 64), dim3(64), 0, s, M, C, sd, total, n_paths,
-                               max_vertices, scratch, records, max_records, count);
+            hipLaunchKernelGGL(nrc::k_nrc_paths<VR_MAJ_GRID>, g, dim3(64), 0, s, M, C, sd, total, n_paths, max_vertices,
+                               scratch, records, max_records, counts);
         else
-            hipLaunchKernelGGL(nrc::k_nrc_paths<VR_MAJ_CONST>, dim3((n_paths + 63) / 64), dim3(64), 0, s, M, C, sd, total, n_paths,
-                               max_vertices, scratch, records, max_records, count);
+            hipLaunchKernelGGL(nrc::k_nrc_paths<VR_MAJ_CONST>, g, dim3(64), 0, s, M, C, sd, total, n_paths, max_vertices,
+                               scratch, records, max_records, counts);
     }
     return check_hip(hipGetLastError(), "vr_nrc_paths launch");
+}
+
+int vr_nrc_paths(const vr_medium* medium, const vr_camera* camera, const vr_launch* L, int32_t n_paths,
+                 int32_t max_vertices, float* scratch, float* records, int64_t max_records, uint32_t* count,
+                 void* stream) {
+    return vr_nrc_paths_traces(medium, camera, L, 1, n_paths, max_vertices, scratch, records, max_records, count, stream);
 }
 
 int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_launch* L, int32_t terminate_after,
@@ -1651,6 +1662,7 @@ int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_lau
     if (!L) return set_err(VR_ERR_ARG, "vr_nrc_render: launch is NULL");
     if (terminate_after < 0 || max_queries < 0) return set_err(VR_ERR_ARG, "vr_nrc_render: bad sizes");
     if (!weights || !out || !queries || !count) return set_err(VR_ERR_ARG, "vr_nrc_render: null buffer");
+    if (((uintptr_t)weights & 15u) != 0) return set_err(VR_ERR_ARG, "vr_nrc_render: weights must be 16-byte aligned");
     MediumView M;
     int st = make_medium_view(medium, VR_MAP_NEE_DT, L->bw_mode, M);
     if (st) return st;

@@ -106,15 +106,21 @@ VR_DEV void layer_fw(const __bf16* __restrict__ ain, const __bf16* __restrict__ 
     }
 }
 
-// Copies Wt [N][K] (global BF16) into LDS with row stride K + kPad and, if
-// `wT` is given, its transpose [K][N] with row stride N + kPad.  Block-wide.
+// Copies Wt [N][K] (global BF16, 16-byte aligned) into LDS with row stride
+// K + kPad and, if `wT` is given, its transpose [K][N] with row stride N + kPad.
+// Block-wide; 8 consecutive k of one row per 16-byte load (K is a multiple of 8).
 template <int K, int N>
 VR_DEV void load_layer(const __bf16* __restrict__ g, __bf16* __restrict__ wt, __bf16* __restrict__ wT) {
-    for (int i = threadIdx.x; i < N * K; i += blockDim.x) {
-        const int n = i / K, k = i - n * K;
-        const __bf16 v = g[i];
-        wt[n * (K + kPad) + k] = v;
-        if (wT) wT[k * (N + kPad) + n] = v;
+    static_assert(K % 8 == 0, "rows are copied 8 elements at a time");
+    for (int i = threadIdx.x; i < N * K / 8; i += blockDim.x) {
+        const int e = i * 8;
+        const int n = e / K, k = e - n * K;
+        const bf16x8 v = *(const bf16x8*)(g + e);
+        *(bf16x8*)(wt + n * (K + kPad) + k) = v;
+        if (wT) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) wT[(k + j) * (N + kPad) + n] = v[j];
+        }
     }
 }
 
@@ -171,6 +177,11 @@ VR_DEV void mlp_forward(const __bf16* __restrict__ w, __bf16* __restrict__ acts,
     wave_sync();
 }
 
+
+// Copies of the FP32 weight gradient the training blocks add into: block b adds
+// into copy b % kGradCopies, which (workgroups being dealt to the 8 XCDs in
+// turn) keeps each copy's atomics inside one XCD's L2; the Adam kernel sums them.
+constexpr int kGradCopies = 8;
 
 // dW[n][k] += sum over the wave's 32 rows of dz[r][n] * a[r][k] (global FP32 atomics).
 template <int N, int K>
@@ -240,7 +251,8 @@ constexpr int kTrainWaves = 2;
 // consecutive windows cycle through however many records were written):
 // relative L2 loss sum_c (q_c - t_c)^2 / (q_c^2 + 0.01) averaged over rows and
 // channels (q_c without gradient in the denominator), its gradient accumulated
-// into `grad` (FP32, zeroed by the caller) and the loss into *loss.
+// into the kGradCopies copies at `grad` (FP32 [kGradCopies][kParams], zeroed by
+// the caller) and the loss into *loss.
 __global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __restrict__ records, const uint32_t* __restrict__ count_p,
                                                                 int64_t first, int64_t max_records, f3 bmin, f3 ibsize,
                                                                 const __bf16* __restrict__ wg, float* __restrict__ grad,
@@ -250,6 +262,7 @@ __global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __r
     __shared__ TrainWave waves[kTrainWaves];
     load_weights(wg, w, wT);
     __syncthreads();
+    grad += (size_t)(blockIdx.x % kGradCopies) * kParams;
     const int64_t written = (int64_t)*count_p;
     if (written <= 0) return;
     const int64_t f = first % written;
@@ -314,11 +327,15 @@ __global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __r
     if (l == 0) atomic_addf(loss, lsum);
 }
 
-// Adam on the FP32 master weights (torch.optim.Adam semantics) and the BF16 copy.
-__global__ void k_nrc_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+// Adam on the FP32 master weights (torch.optim.Adam semantics) and the BF16
+// copy, on the sum of the gradient copies g[c][i]; the sum is left in copy 0.
+__global__ void k_nrc_adam(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
                            __bf16* __restrict__ wg, float lr, float b1, float b2, float eps, float bc1, float bc2_sqrt) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kParams; i += gridDim.x * blockDim.x) {
-        const float gi = g[i];
+        float gi = g[i];
+#pragma unroll
+        for (int c = 1; c < kGradCopies; c++) gi += g[(size_t)c * kParams + i];
+        g[i] = gi;
         const float mi = b1 * m[i] + (1.0f - b1) * gi;
         const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
         m[i] = mi;
@@ -527,24 +544,36 @@ VR_DEV unsigned long long lanes_below() {
 #define VR_NRC_PATHS_WAVES_PER_EU 3
 #endif
 
-// Training paths: path j starts at a uniformly drawn sensor; each recorded
-// vertex becomes a record (x, w, (R_end - R_v) / W_v) appended at *count.
-// Few, long paths: one wave per block so that every CU gets work.
+// Dispatch seeds of the traces of one path launch (kernel-argument array).
+constexpr int kMaxTraces = 16;
+struct TraceSeeds {
+    Seeds s[kMaxTraces];
+};
+
+// Training paths: path j of trace blockIdx.y starts at a uniformly drawn sensor
+// (the trace's own seeds); each recorded vertex becomes a record (x, w,
+// (R_end - R_v) / W_v) appended to the trace's records at its count.  The
+// traces of a launch are independent (the paths do not read the cache), so a
+// launch of T traces equals T launches of one.  Few, long paths: one wave per
+// block so that every CU gets work.
 template <int MAJ>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VR_NRC_PATHS_WAVES_PER_EU))) k_nrc_paths(MediumView M, CameraView cam, Seeds seeds, int64_t total, int n_paths,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VR_NRC_PATHS_WAVES_PER_EU))) k_nrc_paths(MediumView M, CameraView cam, TraceSeeds seeds, int64_t total, int n_paths,
                                                    int maxv, float* __restrict__ tmp, float* __restrict__ records,
                                                    int64_t max_records, uint32_t* __restrict__ count) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n_paths) return;
+    const int trace = blockIdx.y;
+    records += (size_t)trace * (size_t)max_records * 9;
+    count += trace;
     Tracer<MAJ> tr(first_kernel_arg<MediumView>());
-    tr.rng = seed_map(seeds, (uint32_t)j);
+    tr.rng = seed_map(seeds.s[trace], (uint32_t)j);
     const int64_t i = min(total - 1, (int64_t)(tr.rnd() * (float)total));
     int pose, row, col;
     sensor_index(cam, i, pose, row, col);
     if (pose < 0 || pose >= cam.num_poses) return;
     f3 x, w, R_end;
     camera_ray(cam, pose, row, col, tr.rng, x, w);
-    float* t = tmp + (size_t)j * maxv * 12;
+    float* t = tmp + ((size_t)trace * (size_t)n_paths + (size_t)j) * (size_t)maxv * 12;
     const int nv = nee_record(tr, x, w, maxv, t, R_end);
     const uint32_t base = atomicAdd(count, (uint32_t)nv);
     for (int v = 0; v < nv && (int64_t)base + v < max_records; v++) {

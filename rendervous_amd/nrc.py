@@ -49,7 +49,10 @@ class NeuralRadianceCache:
         self.params = torch.cat(chunks).to(device)
         assert self.params.numel() == L.NRC_PARAMS
         self.weights = self.params.to(torch.bfloat16)  # round to nearest even, as the kernels
-        self.grad = torch.zeros_like(self.params)
+        # a training step accumulates into NRC_GRAD_COPIES copies; the first
+        # holds the step's gradient afterwards (self.grad views it)
+        self.grad_copies = torch.zeros(L.NRC_GRAD_COPIES * L.NRC_PARAMS, device=device)
+        self.grad = self.grad_copies[:L.NRC_PARAMS]
         self.m = torch.zeros_like(self.params)
         self.v = torch.zeros_like(self.params)
         self.loss = torch.zeros(1, device=device)
@@ -57,6 +60,8 @@ class NeuralRadianceCache:
         self.terminate_after = int(terminate_after)
         self.steps = 0
         self._records = None
+        self._records_cap = 0
+        self._last_traces = 1
         self._scratch = None
         self._count = torch.zeros(1, dtype=torch.int32, device=device)
         self._queries = None
@@ -77,44 +82,64 @@ class NeuralRadianceCache:
     # -------------------------------------------------------------- training
     def train(self, field: M.MapBase, camera: M.PerspectiveCameraSensor, n_paths: int = 8192, max_vertices: int = 16,
               steps: int = 1, batch: typing.Optional[int] = None,
-              lr: typing.Union[None, float, typing.Sequence[float]] = None) -> torch.Tensor:
-        """Traces n_paths plain NEE paths from random sensors and runs `steps`
-        Adam steps on their collision vertices, each on the next `batch` records
-        (all records when batch is None); returns the last mean loss (device).
-        `lr` overrides the cache's learning rate for these steps: one value, or
-        one per step (a schedule)."""
+              lr: typing.Union[None, float, typing.Sequence[float]] = None, traces: int = 1,
+              seeds: typing.Optional[typing.Sequence[typing.Sequence[int]]] = None) -> torch.Tensor:
+        """Traces `traces` sets of n_paths plain NEE paths from random sensors
+        and, for each set in turn, runs `steps` Adam steps on its collision
+        vertices, each on the next `batch` records (all records when batch is
+        None); returns the last mean loss (device).  The paths do not read the
+        cache, so the sets are traced in one launch (up to 16), with the result
+        of `traces` calls of one set each.  `seeds` gives each set's dispatch
+        seeds (drawn with get_seeds() when None).  `lr` overrides the cache's
+        learning rate: one value, or one per step of all sets (a schedule)."""
+        traces = int(traces)
+        if not 1 <= traces <= 16:
+            raise ValueError("traces must be in [1, 16]")
         lw = self._lower(field)
         cam, keep = camera._camera(None)
         cap = n_paths * max_vertices
-        if self._records is None or self._records.shape[0] < cap:
-            self._records = torch.empty(cap, 9, device=self.device)
-            self._scratch = torch.empty(cap * 12, device=self.device)
-        launch = L.launch(_internal.get_seeds(), bw_mode=L.VR_BW_CONSISTENT)
+        if self._records is None or self._records.shape[0] < traces * cap or self._records_cap != cap:
+            self._records = torch.empty(traces * cap, 9, device=self.device)
+            self._scratch = torch.empty(traces * cap * 12, device=self.device)
+            self._records_cap = cap
+        if self._count.numel() < traces:
+            self._count = torch.zeros(traces, dtype=torch.int32, device=self.device)
+        if seeds is None:
+            seeds = [_internal.get_seeds() for _ in range(traces)]
+        if len(seeds) != traces:
+            raise ValueError(f"{len(seeds)} seed tuples for {traces} traces")
+        self._last_traces = traces
+        launches = (L.VrLaunch * traces)(*[L.launch(sd, bw_mode=L.VR_BW_CONSISTENT) for sd in seeds])
         lib = L.load()
-        L.check(lib.vr_nrc_paths(lw.medium, cam, launch, int(n_paths), int(max_vertices), self._scratch.data_ptr(),
-                                 self._records.data_ptr(), int(cap), self._count.data_ptr(), self._stream()),
-                "vr_nrc_paths")
+        L.check(lib.vr_nrc_paths_traces(lw.medium, cam, launches, traces, int(n_paths), int(max_vertices),
+                                        self._scratch.data_ptr(), self._records.data_ptr(), int(cap),
+                                        self._count.data_ptr(), self._stream()), "vr_nrc_paths_traces")
         bmin, bmax = _f3p(lw.medium.box_min), _f3p(lw.medium.box_max)
         size = cap if batch is None else int(batch)
+        n_steps = traces * steps
         if lr is None or isinstance(lr, (int, float)):
-            rates = [self.lr if lr is None else float(lr)] * steps
+            rates = [self.lr if lr is None else float(lr)] * n_steps
         else:
             rates = [float(r) for r in lr]
-            if len(rates) != steps:
-                raise ValueError(f"lr schedule has {len(rates)} entries for {steps} steps")
-        for k in range(steps):
-            self.steps += 1
-            first = 0 if batch is None else k * size  # wrapped at the written count by the kernel
-            L.check(lib.vr_nrc_train_step(self._records.data_ptr(), self._count.data_ptr(), int(first), int(size), bmin, bmax,
-                                          self.params.data_ptr(), self.weights.data_ptr(), self.grad.data_ptr(),
-                                          self.m.data_ptr(), self.v.data_ptr(), self.steps, rates[k], self.betas[0],
-                                          self.betas[1], self.eps, self.loss.data_ptr(), self._stream()),
-                    "vr_nrc_train_step")
+            if len(rates) != n_steps:
+                raise ValueError(f"lr schedule has {len(rates)} entries for {n_steps} steps")
+        row_bytes = 9 * self._records.element_size()
+        for t in range(traces):
+            records = self._records.data_ptr() + t * cap * row_bytes
+            count = self._count.data_ptr() + t * self._count.element_size()
+            for k in range(steps):
+                self.steps += 1
+                first = 0 if batch is None else k * size  # wrapped at the written count by the kernel
+                L.check(lib.vr_nrc_train_step(records, count, int(first), int(size), bmin, bmax,
+                                              self.params.data_ptr(), self.weights.data_ptr(),
+                                              self.grad_copies.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+                                              self.steps, rates[t * steps + k], self.betas[0], self.betas[1], self.eps,
+                                              self.loss.data_ptr(), self._stream()), "vr_nrc_train_step")
         return self.loss
 
     def samples_last(self) -> int:
-        """Training records written by the last train() (host sync)."""
-        return int(self._count.item())
+        """Training records written by the last train(), over its traces (host sync)."""
+        return int(self._count[:self._last_traces].to(torch.int64).sum().item())
 
     # ------------------------------------------------------------- rendering
     def render(self, field: M.MapBase, camera: M.PerspectiveCameraSensor, spp: int = 1) -> torch.Tensor:

@@ -113,7 +113,7 @@ def test_training_gradient_matches_torch_autograd(gpu):
     count = torch.tensor([1000], dtype=torch.int32, device="cuda")
     lib = L.load()
     L.check(lib.vr_nrc_train_step(recd.data_ptr(), count.data_ptr(), 0, 1024, *_box_ptrs(), cache.params.data_ptr(),
-                                  cache.weights.data_ptr(), cache.grad.data_ptr(), cache.m.data_ptr(), cache.v.data_ptr(),
+                                  cache.weights.data_ptr(), cache.grad_copies.data_ptr(), cache.m.data_ptr(), cache.v.data_ptr(),
                                   1, 0.0, 0.9, 0.99, 1e-8, cache.loss.data_ptr(), None), "vr_nrc_train_step")
     torch.cuda.synchronize()
     Ws = [W.float().clone().requires_grad_(True) for W in _split(cache.weights.cpu())]
@@ -138,7 +138,7 @@ def test_adam_step_matches_torch_optim(gpu):
     lib = L.load()
     p0 = cache.params.clone()
     L.check(lib.vr_nrc_train_step(rec.data_ptr(), count.data_ptr(), 0, 256, *_box_ptrs(), cache.params.data_ptr(),
-                                  cache.weights.data_ptr(), cache.grad.data_ptr(), cache.m.data_ptr(), cache.v.data_ptr(),
+                                  cache.weights.data_ptr(), cache.grad_copies.data_ptr(), cache.m.data_ptr(), cache.v.data_ptr(),
                                   1, 1e-2, 0.9, 0.99, 1e-8, cache.loss.data_ptr(), None), "vr_nrc_train_step")
     torch.cuda.synchronize()
     p = torch.nn.Parameter(p0.cpu())
@@ -174,10 +174,8 @@ def test_zero_cache_render_is_the_truncated_nee_path(gpu, tkind):
     assert match_fraction(out.cpu().numpy(), ref, rtol=1e-3, atol=1e-4) >= 0.98
 
 
-def test_cache_learns_and_the_cached_render_approaches_the_plain_one(gpu):
-    """Online training on the scene's own NEE paths lowers the loss, and at equal
-    spp the cached render (paths ended after 2 bounces) is at least as close to
-    a high-spp plain render as the plain render itself."""
+def _nrc_scene():
+    """48^3 fBm cloud, sun + environment, HG g = 0.5, NEE with DT shadow rays; 64x64 camera."""
     dev = torch.device("cuda", 0)
     sigma = rv.functions.fbm_value_noise(48, seed=3, scale=20.0, device=dev)
     ds = rv.objects.DependencySet()
@@ -192,6 +190,15 @@ def test_cache_learns_and_the_cached_render_approaches_the_plain_one(gpu):
     ds.requires(rv.objects.medium_radiance_path_integrator_NEE_DT, max_bounces=16)
     field = ds.radiance
     camera = rv.PerspectiveCameraSensor(64, 64, rv.functions.look_at_pose((0.0, 0.3, 2.5))[None].to(dev), jittered=True)
+    return field, camera
+
+
+def test_cache_learns_and_the_cached_render_approaches_the_plain_one(gpu):
+    """Online training on the scene's own NEE paths lowers the loss, and at equal
+    spp the cached render (paths ended after 2 bounces) is at least as close to
+    a high-spp plain render as the plain render itself."""
+    dev = torch.device("cuda", 0)
+    field, camera = _nrc_scene()
     cache = NeuralRadianceCache(dev, seed=7, lr=1e-2)
     losses = []
     for it in range(300):
@@ -210,3 +217,35 @@ def test_cache_learns_and_the_cached_render_approaches_the_plain_one(gpu):
     e_cached = float(((cached - ref) ** 2).mean().sqrt())
     print(f"[nrc] rmse plain {e_plain:.5f} cached {e_cached:.5f} (mean {float(ref.mean()):.4f})", flush=True)
     assert e_cached <= 1.2 * e_plain
+
+
+def test_paths_of_several_traces_in_one_launch_equal_separate_launches(gpu):
+    """vr_nrc_paths_traces over 3 traces writes, per trace, the records that
+    vr_nrc_paths writes with that trace's seeds (rows compared as sorted sets:
+    the append order follows the atomics)."""
+    import rendervous_amd as rv
+    field, camera = _nrc_scene()
+    lw = NeuralRadianceCache._lower(field)
+    cam, keep = camera._camera(None)
+    lib = L.load()
+    n_paths, maxv, traces = 2048, 8, 3
+    cap = n_paths * maxv
+    seeds = [(1001 + 17 * t, 2002 + t, 3003 + 5 * t, 4004 + t) for t in range(traces)]
+    rec = torch.zeros(traces * cap, 9, device="cuda")
+    scratch = torch.empty(traces * cap * 12, device="cuda")
+    counts = torch.zeros(traces, dtype=torch.int32, device="cuda")
+    launches = (L.VrLaunch * traces)(*[L.launch(sd, bw_mode=L.VR_BW_CONSISTENT) for sd in seeds])
+    L.check(lib.vr_nrc_paths_traces(lw.medium, cam, launches, traces, n_paths, maxv, scratch.data_ptr(), rec.data_ptr(),
+                                    cap, counts.data_ptr(), None), "vr_nrc_paths_traces")
+    torch.cuda.synchronize()
+    for t in range(traces):
+        one = torch.zeros(cap, 9, device="cuda")
+        cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+        L.check(lib.vr_nrc_paths(lw.medium, cam, L.launch(seeds[t], bw_mode=L.VR_BW_CONSISTENT), n_paths, maxv,
+                                 scratch.data_ptr(), one.data_ptr(), cap, cnt.data_ptr(), None), "vr_nrc_paths")
+        torch.cuda.synchronize()
+        n = int(cnt.item())
+        assert n == int(counts[t].item()) and n > n_paths // 2
+        a = np.unique(rec[t * cap:t * cap + n].cpu().numpy(), axis=0)
+        b = np.unique(one[:n].cpu().numpy(), axis=0)
+        np.testing.assert_array_equal(a, b)
