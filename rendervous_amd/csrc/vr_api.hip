@@ -591,6 +591,123 @@ k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P,
     }
 }
 
+// The same backward with the lanes of a wave pulling work items (a sensor and
+// `chunk` consecutive samples of it) from a launch-wide counter, as the
+// persistent NEE forward does: a lane whose samples end takes the next item,
+// so the wave's lanes keep tracking until the counter drains instead of
+// waiting for the lane with the longest paths.  Per-sample streams only (the
+// fresh or replayed dispatch seeds); each sample's draws and gradients are
+// those of k_capture_bw_nee_flat.  Every argument sits in one structure at the
+// start of the kernel-argument segment (the lane reads the medium there).
+struct NeeBwArgs {
+    MediumView M;
+    CameraView cam;
+    Seeds seeds;
+    NeeSchedule S;
+    unsigned long long* counter;
+    const float* og;
+    int* dbg;
+    int debug_lookups;
+};
+VR_DEV const NeeBwArgs& nee_bw_args() { return *(const NeeBwArgs*)reread(first_kernel_arg<NeeBwArgs>()); }
+
+#ifndef VR_NEE_BW_POOL_WAVES_PER_EU
+#define VR_NEE_BW_POOL_WAVES_PER_EU 4
+#endif
+template <int MAJ, bool ENVMAP>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VR_NEE_BW_POOL_WAVES_PER_EU)))
+k_capture_bw_nee_pool(NeeBwArgs args) {
+    NeeDsBwLane<MAJ, ENVMAP> L(first_kernel_arg<MediumView>());
+    L.eps = args.M.ds_eps;
+    // A lane without work sits in kBwDone with s == s1 (a per-lane "has work"
+    // flag guarding the handlers cost ~150 spilled VGPRs).
+    L.state = kBwDone;
+    bool done = false, drained = false;
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform pool of item indices
+    uint32_t sensor = 0;
+    int s = 0, s1 = 0, pose = 0, row = 0, col = 0, count0 = 0;
+    // every sample ends (its paths end); the cap only bounds a wave's loop
+    for (uint32_t it = 0; it < (1u << 28); it++) {
+        L.tr.reread();
+        // end of a sample: the next sample of the item, or the item ends
+        bool fresh = false, need = false;
+        if (L.state == kBwDone) {
+            if (s < s1) {
+                if (++s < s1) {
+                    fresh = true;
+                } else {
+                    const NeeBwArgs& A = nee_bw_args();
+                    if (A.dbg) atomicAdd(A.dbg + sensor, (A.debug_lookups ? L.tr.lookups : L.tr.scatters) - count0);
+                }
+            }
+            need = !fresh && !done;
+        }
+        // lanes that need an item take one from the wave's pool of consecutive
+        // item indices; an empty pool is refilled with kPool items by one atomic
+        const unsigned long long mask = __ballot(need);
+        if (mask) {
+            const NeeBwArgs& A = nee_bw_args();
+            const uint32_t items = A.S.cnt * (uint32_t)A.S.chunks;
+            while (pool_next >= pool_end && !drained) {
+                unsigned long long base = 0;
+                if (__lane_id() == 0) base = atomicAdd(A.counter, (unsigned long long)kPool);
+                base = __shfl(base, 0);
+                if (base < items) {
+                    pool_next = (uint32_t)base;
+                    pool_end = items - pool_next > (uint32_t)kPool ? pool_next + (uint32_t)kPool : items;
+                } else {
+                    drained = true;
+                }
+            }
+            const uint32_t avail = pool_end - pool_next;
+            const uint32_t rank = (uint32_t)__popcll(mask & lane_mask_lt());
+            if (need) {
+                if (rank < avail) {
+                    int s0;
+                    s = s1 = 0;  // no samples unless the item has a gradient
+                    if (nee_item(A.S, A.cam, 0, pool_next + rank, sensor, s0, s1, pose, row, col)) {
+                        const float inv = (float)A.S.spp;
+                        const float* o = A.og + 3 * (size_t)sensor;
+                        const f3 g = F3(o[0] / inv, o[1] / inv, o[2] / inv);
+                        if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
+                            L.dL_dR = g;
+                            s = s0;
+                            count0 = A.debug_lookups ? L.tr.lookups : L.tr.scatters;
+                            fresh = true;
+                        }
+                    }
+                    if (!fresh) s = s1;
+                } else if (drained) {
+                    done = true;
+                }
+            }
+            const uint32_t taken = (uint32_t)__popcll(mask);
+            pool_next += taken < avail ? taken : avail;
+        }
+        if (fresh) {
+            const NeeBwArgs& A = nee_bw_args();
+            L.tr.rng = seed_capture_fw(A.seeds, (uint32_t)s * A.S.total + sensor);
+            camera_ray(A.cam, pose, row, col, L.tr.rng, L.x0, L.w0);
+            L.begin_pass(1);
+        }
+        if (__ballot(L.state != kBwDone || !done) == 0ull) break;  // the wave drained the counter
+        L.events();
+        for (int k = 0; k < 3; k++) {
+            if (L.tracking()) L.step();
+            if (__ballot(L.tracking()) == 0ull) break;
+        }
+    }
+}
+
+// Zeroes the pooled backward's work counter and the debug rows [start, start+cnt).
+__global__ void k_nee_bw_prepare(unsigned long long* __restrict__ counter, int64_t start, int64_t cnt, int* __restrict__ dbg) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (i == 0) counter[0] = 0ull;
+    if (!dbg) return;
+    for (; i < cnt; i += stride) dbg[start + i] = 0;
+}
+
 // Cell table of a single-channel grid: thread c writes cell c's 8 corners.
 __global__ void __launch_bounds__(kBlock) k_pack_cells(GridView g, float4* __restrict__ out, int64_t cells) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -851,10 +968,74 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
     return check_hip(hipGetLastError(), "vr_capture_fw launch");
 }
 
+// The NEE backward with DT shadow rays and per-sample streams runs as the
+// pooled lane state machine (k_capture_bw_nee_pool), which needs a work counter.
+bool capture_bw_is_pooled(int kind, const MediumView& M, int seed_mode) {
+    return kind == VR_MAP_NEE_DT && M.t_kind == VR_T_DT && seed_mode != VR_SEED_CAPTURE_BW &&
+           env_int("VR_NEE_BW_FLAT", 1) != 0 && env_int("VR_NEE_BW_POOL", 1) != 0;
+}
+
+template <int MAJ, bool ENVMAP>
+int launch_nee_bw_pool(const MediumView& M, const CameraView& C, Seeds sd, int spp, int debug_kind, int64_t total,
+                       int64_t start, int64_t cnt, const float* og, int* dbg, unsigned long long* counter, hipStream_t st) {
+    if (total > (int64_t)UINT32_MAX) return set_err(VR_ERR_UNSUPPORTED, "capture has more than 2^32 sensors");
+    int rc = VR_OK;
+    int blocks = 0;
+    if ((rc = persistent_blocks(k_capture_bw_nee_pool<MAJ, ENVMAP>, &blocks))) return rc;
+    const int64_t lanes = (int64_t)blocks * kBlock;
+    const int64_t hw = (int64_t)C.width * C.height;
+    CaptureSplit P = capture_split(spp, false);
+    capture_tiling(P, C, start, cnt);
+    // launches of at most 2^30 sensors (whole images when possible) keep item indices in 32 bits
+    int64_t batch = int64_t(1) << 30;
+    if (!C.sensors && hw <= batch) batch = batch / hw * hw;
+    for (int64_t b0 = start; b0 < start + cnt; b0 += batch) {
+        const int64_t n = std::min(batch, start + cnt - b0);
+        // samples per work item: ~8 items per lane, at most 8 samples (short items balance the drain)
+        int64_t chunk = (n * (int64_t)spp) / (8 * lanes);
+        chunk = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(chunk, 8), spp));
+        chunk = env_int("VR_NEE_BW_CHUNK", (int)chunk);
+        chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, spp));
+        while (n * ((spp + chunk - 1) / chunk) > (int64_t)UINT32_MAX - kPool) chunk *= 2;
+        NeeBwArgs A;
+        A.M = M;
+        A.cam = C;
+        A.seeds = sd;
+        NeeSchedule& S = A.S;
+        S.total = (uint32_t)total;
+        S.start = (uint32_t)b0;
+        S.cnt = (uint32_t)n;
+        S.qpix = (uint32_t)n;
+        S.nq = 1;
+        S.spp = spp;
+        S.chunk = (int)chunk;
+        S.chunks = (int)((spp + chunk - 1) / chunk);
+        S.tiled = P.tiled == 1 && n % hw == 0 && b0 % hw == 0 ? 1 : 0;
+        S.ksteps = 3;
+        S.kmax = 3;
+        S.track_min = 0;
+        S.min_begin = 1;
+        A.counter = counter;
+        A.og = og;
+        A.dbg = dbg;
+        A.debug_lookups = debug_kind == 1;
+        const unsigned prep = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 4096));
+        hipLaunchKernelGGL(k_nee_bw_prepare, dim3(prep), dim3(kBlock), 0, st, counter, b0, n, dbg);
+        hipLaunchKernelGGL((k_capture_bw_nee_pool<MAJ, ENVMAP>), dim3(blocks), dim3(kBlock), 0, st, A);
+        if ((rc = check_hip(hipGetLastError(), "vr_capture_bw launch"))) return rc;
+    }
+    return VR_OK;
+}
+
 template <int MAJ>
 int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Seeds sd, int spp, int seed_mode, int debug_kind,
                           int64_t total,
-                          int64_t start, int64_t cnt, const float* og, int* dbg, hipStream_t st) {
+                          int64_t start, int64_t cnt, const float* og, int* dbg, unsigned long long* counter, hipStream_t st) {
+    if (counter && capture_bw_is_pooled(kind, M, seed_mode)) {
+        if (M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM)
+            return launch_nee_bw_pool<MAJ, true>(M, C, sd, spp, debug_kind, total, start, cnt, og, dbg, counter, st);
+        return launch_nee_bw_pool<MAJ, false>(M, C, sd, spp, debug_kind, total, start, cnt, og, dbg, counter, st);
+    }
     CaptureSplit P = capture_split(spp, seed_mode == VR_SEED_CAPTURE_BW);
     capture_tiling(P, C, start, cnt);
     P.debug_lookups = debug_kind == 1;
@@ -1343,11 +1524,16 @@ int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera
     if ((st = check_sample_indices(total, spp, L->seed_mode, "vr_capture_bw"))) return st;
     Seeds sd = make_seeds(L);
     hipStream_t s = (hipStream_t)stream;
+    Counters ctr;
+    if (capture_bw_is_pooled(kind, M, L->seed_mode) && (st = ctr.acquire(L, s))) return st;
     if (medium->majorant_kind == VR_MAJ_GRID)
-        return launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, L->debug_kind, total, start, cnt, og,
-                                                  L->debug_out, s);
-    return launch_capture_bw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, L->seed_mode, L->debug_kind, total, start, cnt, og,
-                                               L->debug_out, s);
+        st = launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, L->debug_kind, total, start, cnt, og,
+                                                L->debug_out, ctr.ptr, s);
+    else
+        st = launch_capture_bw_maj<VR_MAJ_CONST>(kind, M, C, sd, spp, L->seed_mode, L->debug_kind, total, start, cnt, og,
+                                                 L->debug_out, ctr.ptr, s);
+    const int rel = ctr.release();
+    return st ? st : rel;
 }
 
 int vr_adam_step_bounded(float* p, float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps,

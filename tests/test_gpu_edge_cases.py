@@ -164,12 +164,15 @@ def test_cell_table_lookups_are_bit_identical(gpu):
     (L.VR_BW_CONSISTENT, "const", L.VR_SEED_CAPTURE_FW, "sun"), (L.VR_BW_REFERENCE, "const", L.VR_SEED_CAPTURE_FW, "sun"),
     (L.VR_BW_CONSISTENT, "grid", L.VR_SEED_CAPTURE_BW, "sun"), (L.VR_BW_CONSISTENT, "grid", L.VR_SEED_CAPTURE_FW, "image")])
 def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant, seed_mode, env):
-    """The NEE capture backward as a flat lane state machine (the default for DT
-    shadow rays) scatters the same gradients, sample by sample, as the nested
-    estimator (VR_NEE_BW_FLAT=0): same scatter counts, density and albedo
-    gradients equal up to the order of the float atomics.  Sensors with a zero
-    output gradient trace nothing.  Also with fresh backward seeds (one stream over
-    a sensor's samples) and an environment image sampled through its quadtree."""
+    """The NEE capture backward as a flat lane state machine, with each thread's
+    own samples (VR_NEE_BW_POOL=0) and with lanes pulling work items from a
+    counter (the default for DT shadow rays and per-sample streams), scatters the
+    same gradients, sample by sample, as the nested estimator
+    (VR_NEE_BW_FLAT=0): same scatter counts, density and albedo gradients equal
+    up to the order of the float atomics.  Sensors with a zero output gradient
+    trace nothing.  Also with the reference's backward seeds (one stream over a
+    sensor's samples: the pooled form does not apply) and an environment image
+    sampled through its quadtree."""
     import os
     import torch
     from helpers import env_density, np_quadtree, sky_image
@@ -186,10 +189,11 @@ def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant, seed_mo
     og = (np.random.default_rng(3).uniform(size=(800, 3)) + 0.5).astype(np.float32)
     og[::7] = 0.0
     res = []
-    old = os.environ.get("VR_NEE_BW_FLAT")
+    old = {k: os.environ.get(k) for k in ("VR_NEE_BW_FLAT", "VR_NEE_BW_POOL")}
     try:
-        for flat in ("0", "1"):
+        for flat, pool in (("0", "0"), ("1", "0"), ("1", "1")):
             os.environ["VR_NEE_BW_FLAT"] = flat
+            os.environ["VR_NEE_BW_POOL"] = pool
             b = Built(sc, "cuda", grad_sigma=True, grad_albedo=True)
             c = Camera(device="cuda", **cam)
             dbg = torch.zeros(c.threads, dtype=torch.int32, device="cuda")
@@ -200,10 +204,11 @@ def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant, seed_mo
             torch.cuda.synchronize()
             res.append((b.grads(), dbg.cpu().numpy()))
     finally:
-        if old is None:
-            os.environ.pop("VR_NEE_BW_FLAT", None)
-        else:
-            os.environ["VR_NEE_BW_FLAT"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     from helpers import rel_l1
     (gs0, ga0), d0 = res[0]
     assert np.abs(gs0).sum() > 0 and np.abs(ga0).sum() > 0
