@@ -557,7 +557,9 @@ k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P,
         if (dbg && P.chunks == 1) dbg[i] = 0;
         return;
     }
+    __shared__ uint16_t tn_rec[NeeDsBwLane<MAJ, ENVMAP>::kTnRecords * kBlock];
     NeeDsBwLane<MAJ, ENVMAP> L(first_kernel_arg<MediumView>());
+    L.tn_rec = tn_rec + threadIdx.x;
     L.eps = M.ds_eps;
     L.dL_dR = g;
     // the reference's fresh-seed backward continues one stream over the samples (one chunk)
@@ -617,7 +619,9 @@ VR_DEV const NeeBwArgs& nee_bw_args() { return *(const NeeBwArgs*)reread(first_k
 template <int MAJ, bool ENVMAP>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VR_NEE_BW_POOL_WAVES_PER_EU)))
 k_capture_bw_nee_pool(NeeBwArgs args) {
+    __shared__ uint16_t tn_rec[NeeDsBwLane<MAJ, ENVMAP>::kTnRecords * kBlock];
     NeeDsBwLane<MAJ, ENVMAP> L(first_kernel_arg<MediumView>());
+    L.tn_rec = tn_rec + threadIdx.x;
     L.eps = args.M.ds_eps;
     // A lane without work sits in kBwDone with s == s1 (a per-lane "has work"
     // flag guarding the handlers cost ~150 spilled VGPRs).
@@ -829,6 +833,7 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         const int64_t n = std::min(batch, start + cnt - b0);
         // samples per work item: enough items for ~8 per lane in flight, at most spp
         int64_t chunk = (n * (int64_t)spp) / (8 * lanes);
+        chunk = env_int("VR_NEE_CHUNK", (int)std::min<int64_t>(chunk, INT32_MAX));
         chunk = chunk < 1 ? 1 : (chunk > spp ? spp : chunk);
         while (n * ((spp + chunk - 1) / chunk) > (int64_t)UINT32_MAX - kPool) chunk *= 2;
         NeeSchedule S;
@@ -982,7 +987,6 @@ int launch_nee_bw_pool(const MediumView& M, const CameraView& C, Seeds sd, int s
     int rc = VR_OK;
     int blocks = 0;
     if ((rc = persistent_blocks(k_capture_bw_nee_pool<MAJ, ENVMAP>, &blocks))) return rc;
-    const int64_t lanes = (int64_t)blocks * kBlock;
     const int64_t hw = (int64_t)C.width * C.height;
     CaptureSplit P = capture_split(spp, false);
     capture_tiling(P, C, start, cnt);
@@ -991,11 +995,9 @@ int launch_nee_bw_pool(const MediumView& M, const CameraView& C, Seeds sd, int s
     if (!C.sensors && hw <= batch) batch = batch / hw * hw;
     for (int64_t b0 = start; b0 < start + cnt; b0 += batch) {
         const int64_t n = std::min(batch, start + cnt - b0);
-        // samples per work item: ~8 items per lane, at most 8 samples (short items balance the drain)
-        int64_t chunk = (n * (int64_t)spp) / (8 * lanes);
-        chunk = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(chunk, 8), spp));
-        chunk = env_int("VR_NEE_BW_CHUNK", (int)chunk);
-        chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, spp));
+        // one sample per work item (measured on the 512^3 NEE step backward:
+        // 50.6 ms with 1, 50.9 with 2, 52.7 with 4, 56.7 with 8 samples per item)
+        int64_t chunk = std::max(1, std::min(spp, env_int("VR_NEE_BW_CHUNK", 1)));
         while (n * ((spp + chunk - 1) / chunk) > (int64_t)UINT32_MAX - kPool) chunk *= 2;
         NeeBwArgs A;
         A.M = M;

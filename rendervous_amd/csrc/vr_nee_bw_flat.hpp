@@ -63,8 +63,18 @@ struct NeeDsBwLane {
     float cs, cm, cpc;         // sigma, majorant and collision probability of a pending collision
     f3 dL_dR;
     int state, pass, scattered, bounces, start_kind, after;
+    // Pass 1's shadow-ray estimates, for pass 2 to reuse (VR_BW_CONSISTENT: both
+    // passes draw the same collisions and light samples, so pass 2's estimate
+    // at collision b equals pass 1's): per collision, the estimate (0 or 1, top
+    // bit) and the uniforms its walk drew (the low 15 bits), kTnRecords
+    // collisions per lane in LDS with a stride of the block size; kTnRedo marks
+    // one pass 2 walks again.
+    static constexpr int kTnRecords = 16;
+    static constexpr uint16_t kTnRedo = 0xFFFFu;
+    uint16_t* tn_rec;  // this lane's first record (nullptr: pass 2 walks every shadow ray)
+    int tn_draws;      // uniforms drawn by the current shadow-ray estimate
 
-    VR_DEV explicit NeeDsBwLane(ConstPtr<MediumView> m) : tr(m) {}
+    VR_DEV explicit NeeDsBwLane(ConstPtr<MediumView> m) : tr(m), tn_rec(nullptr) {}
 
     // Pass 1 (or 2) of the sample from its camera ray: nee_ds_bw's start
     // (vr_path_integrator_NEE_DS.h:6-30) and its replay restart (:110-125).
@@ -156,6 +166,8 @@ struct NeeDsBwLane {
         const float m = path ? m_path : m_shadow;
         const float t = tr.flight(m);
         rx = rx + rw * fminf(t, md);
+        const bool shadow_fw = state == kBwShadowFw;
+        tn_draws += shadow_fw ? 1 : 0;
         bool ended;
         if (t > md) {  // majorant-cell crossing
             rd -= md;
@@ -172,6 +184,7 @@ struct NeeDsBwLane {
                 // the shared trilinear lookup and the decision of each ray kind
                 const float s = tr.sigma(rx);
                 const float u = tr.rnd();
+                tn_draws += shadow_fw ? 1 : 0;
                 float g = 0.0f;
                 bool scatter = false;
                 if (state == kBwShadowFw) {
@@ -239,6 +252,17 @@ struct NeeDsBwLane {
         pw = rw;
         rw = wnee;
         after = pass == 1 ? kBwAfterT1 : kBwAfterT2;
+        tn_draws = 0;
+        if (pass == 2 && tn_rec && bounces < kTnRecords && M.bw_mode == VR_BW_CONSISTENT) {
+            const uint16_t r = tn_rec[bounces * (int)blockDim.x];
+            if (r != kTnRedo) {
+                // pass 1's estimate at this collision; the stream moves past its walk
+                Tn = (r & 0x8000u) ? 1.0f : 0.0f;
+                for (int k = (int)(r & 0x7FFFu); k > 0; k--) rng_advance(tr.rng);
+                state = kBwAfterT2;
+                return;
+            }
+        }
         pend(kStShadowFw);
     }
 
@@ -250,6 +274,9 @@ struct NeeDsBwLane {
         const float phn = phase_eval(M, px, pw, rw);
         bool more = true;
         if (pass == 1) {
+            if (tn_rec && bounces < kTnRecords)
+                tn_rec[bounces * (int)blockDim.x] =
+                    tn_draws <= 0x7FFE ? (uint16_t)((Tn != 0.0f ? 0x8000u : 0u) | (uint32_t)tn_draws) : kTnRedo;
             added = added + (W * env) * (Tn * phn);
             if (null_W < EPS) R_sp = R_sp + added;
             else R = R + added;
