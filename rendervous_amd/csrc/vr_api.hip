@@ -581,8 +581,10 @@ k_capture_bw_nee_flat(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P,
             L.begin_pass(1);
         }
         L.events();
+        L.flush_collision_lane();
         for (int k = 0; k < 3; k++) {
             if (L.tracking()) L.step();
+            L.flush_step_lane();
             if (__ballot(L.tracking()) == 0ull) break;
         }
     }
@@ -620,6 +622,13 @@ template <int MAJ, bool ENVMAP>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VR_NEE_BW_POOL_WAVES_PER_EU)))
 k_capture_bw_nee_pool(NeeBwArgs args) {
     __shared__ uint16_t tn_rec[NeeDsBwLane<MAJ, ENVMAP>::kTnRecords * kBlock];
+    // per wave: the sigma scatter records (kScatterRecords x 8 floats) and one
+    // collision's albedo records (64 x 8 floats)
+    constexpr int kSRec = NeeDsBwLane<MAJ, ENVMAP>::kScatterRecords * 8;
+    __shared__ float scat_records[kBlock / 64 * (kSRec + 512)];
+    float* srec = scat_records + (threadIdx.x / 64) * (kSRec + 512);
+    float* arec = srec + kSRec;
+    int nrec = 0;  // wave-uniform count of buffered sigma records
     NeeDsBwLane<MAJ, ENVMAP> L(first_kernel_arg<MediumView>());
     L.tn_rec = tn_rec + threadIdx.x;
     L.eps = args.M.ds_eps;
@@ -696,11 +705,14 @@ k_capture_bw_nee_pool(NeeBwArgs args) {
         }
         if (__ballot(L.state != kBwDone || !done) == 0ull) break;  // the wave drained the counter
         L.events();
+        nrec = L.post_collision(srec, arec, nrec);
         for (int k = 0; k < 3; k++) {
             if (L.tracking()) L.step();
+            nrec = L.post_step(srec, nrec);
             if (__ballot(L.tracking()) == 0ull) break;
         }
     }
+    L.flush_sigma(srec, nrec);
 }
 
 // Zeroes the pooled backward's work counter and the debug rows [start, start+cnt).

@@ -315,6 +315,61 @@ VR_DEV void grid_scatterC(const GridView& g, f3 c, const float* og) {
 
 VR_DEV void grid_scatter1(const GridView& g, f3 c, float og) { grid_scatterC(g, c, &og); }
 
+// Wave-cooperative form of grid_scatterC for a grid of C = 1 or 3 channels.
+// scatter_post (called by every lane of the wave; lanes with `active` false
+// contribute nothing) appends each contributing lane's cell, fractions and og
+// to the wave's record buffer `rec` (8 floats per record) after its `n`
+// records and returns the new count.  scatter_issue then adds the buffered
+// records' corners with the C x 8 adds of one record on consecutive lanes
+// (corner major, channel minor), so that the x-adjacent corners of a
+// channel-last grid (2C consecutive floats) come from neighbouring lanes of one
+// instruction and travel as one memory request instead of one per lane and
+// float, and a wave issues its atomics in bursts.  The adds and their values
+// are those of grid_scatterC.
+template <int C>
+VR_DEV int scatter_post(const GridView& g, float* __restrict__ rec, int n, bool active, f3 c, f3 og) {
+    static_assert(C == 1 || C == 3, "one or three channels");
+    const float gx = fmaf(c.x, g.scale.x, g.offset.x);
+    const float gy = fmaf(c.y, g.scale.y, g.offset.y);
+    const float gz = fmaf(c.z, g.scale.z, g.offset.z);
+    const bool ok = active && gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx < g.dimf.x && gy < g.dimf.y && gz < g.dimf.z;
+    const unsigned long long mask = __ballot(ok);
+    if (!mask) return n;
+    if (ok) {
+        const unsigned lane = __lane_id();
+        const unsigned rank = (unsigned)__popcll(mask & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
+        const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+        const int px = min((int)fx, g.nx - 2), py = min((int)fy, g.ny - 2), pz = min((int)fz, g.nz - 2);
+        float4* r = reinterpret_cast<float4*>(rec + (n + (int)rank) * 8);
+        r[0] = make_float4(__int_as_float(pz * g.sz + py * g.sy + px * g.sx), gx - fx, gy - fy, gz - fz);
+        r[1] = make_float4(og.x, og.y, og.z, 0.0f);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return n + __popcll(mask);
+}
+
+template <int C>
+VR_DEV void scatter_issue(const GridView& g, const float* __restrict__ rec, int n) {
+    constexpr int kPer = 8 * C;        // lanes per record
+    constexpr int kGroup = 64 / kPer;  // records per instruction
+    const int lane = (int)__lane_id();
+    const int j = lane / kPer, m = lane - j * kPer;
+    const int k = m / C, ch = m - k * C;
+    const int coff = ((k & 1) ? g.sx : 0) + ((k & 2) ? g.sy : 0) + ((k & 4) ? g.sz : 0) + ch * g.sc;
+    for (int q0 = 0; q0 < n; q0 += kGroup) {
+        const int q = q0 + j;
+        if (j < kGroup && q < n) {
+            const float4 r0 = reinterpret_cast<const float4*>(rec + q * 8)[0];
+            const float v = rec[q * 8 + 4 + ch];
+            const float wx = (k & 1) ? r0.y : 1.0f - r0.y;
+            const float wy = (k & 2) ? r0.z : 1.0f - r0.z;
+            const float wz = (k & 4) ? r0.w : 1.0f - r0.w;
+            atomic_addf(g.grad + (__float_as_int(r0.x) + coff), v * (wx * wy * wz));
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // ------------------------------------------------------------- directions --
 // include/randoms.h:144-151
 VR_DEV void ortho_basis(f3 n, f3& t, f3& b) {

@@ -73,8 +73,67 @@ struct NeeDsBwLane {
     static constexpr uint16_t kTnRedo = 0xFFFFu;
     uint16_t* tn_rec;  // this lane's first record (nullptr: pass 2 walks every shadow ray)
     int tn_draws;      // uniforms drawn by the current shadow-ray estimate
+    // Gradient scatters are issued by the whole wave (wave_scatter): a tracking
+    // step leaves its sigma gradient at rx in pend_s, a pass-2 collision its
+    // sigma and albedo gradients at px in pend_cs / pend_ca (zero: nothing).
+    float pend_s, pend_cs;
+    f3 pend_ca;
 
-    VR_DEV explicit NeeDsBwLane(ConstPtr<MediumView> m) : tr(m), tn_rec(nullptr) {}
+    VR_DEV explicit NeeDsBwLane(ConstPtr<MediumView> m)
+        : tr(m), tn_rec(nullptr), pend_s(0.0f), pend_cs(0.0f), pend_ca(F3(0.0f, 0.0f, 0.0f)) {}
+
+    // Wave-issued scatters (every lane of the wave calls these).  The sigma
+    // gradients of the tracking steps and collisions gather in the wave's
+    // record buffer `srec` (kScatterRecords records, count n wave-uniform) and
+    // are issued when it fills and when the wave ends (flush_sigma); the albedo
+    // gradients of a pass-2 collision are issued at once through `arec`.
+    static constexpr int kScatterRecords = 128;
+    VR_DEV int post_step(float* srec, int n) {
+        const MediumView& M = tr.med();
+        if (M.sigma.grad == nullptr) { pend_s = 0.0f; return n; }
+        if (n > kScatterRecords - 64) { scatter_issue<1>(M.sigma, srec, n); n = 0; }
+        if (pend_s != 0.0f) tr.scatters++;
+        n = scatter_post<1>(M.sigma, srec, n, pend_s != 0.0f, rx, F3(pend_s, 0.0f, 0.0f));
+        pend_s = 0.0f;
+        return n;
+    }
+    VR_DEV int post_collision(float* srec, float* arec, int n) {
+        const MediumView& M = tr.med();
+        if (M.sigma.grad != nullptr) {
+            if (n > kScatterRecords - 64) { scatter_issue<1>(M.sigma, srec, n); n = 0; }
+            if (pend_cs != 0.0f) tr.scatters++;
+            n = scatter_post<1>(M.sigma, srec, n, pend_cs != 0.0f, px, F3(pend_cs, 0.0f, 0.0f));
+        }
+        pend_cs = 0.0f;
+        if (M.albedo_grid && M.albedo.grad != nullptr) {
+            const bool a = pend_ca.x != 0.0f || pend_ca.y != 0.0f || pend_ca.z != 0.0f;
+            if (M.albedo.C == 1) {
+                const int na = scatter_post<1>(M.albedo, arec, 0, a, px, F3(pend_ca.x + pend_ca.y + pend_ca.z, 0.0f, 0.0f));
+                if (na) scatter_issue<1>(M.albedo, arec, na);
+            } else {
+                const int na = scatter_post<3>(M.albedo, arec, 0, a, px, pend_ca);
+                if (na) scatter_issue<3>(M.albedo, arec, na);
+            }
+        }
+        pend_ca = F3(0.0f, 0.0f, 0.0f);
+        return n;
+    }
+    VR_DEV void flush_sigma(float* srec, int n) {
+        if (n) scatter_issue<1>(tr.med().sigma, srec, n);
+    }
+
+    // The same scatters issued by this lane alone (a kernel whose lanes leave
+    // the loop at different times cannot share the issue).
+    VR_DEV void flush_step_lane() {
+        if (pend_s != 0.0f) tr.sigma_bw(rx, pend_s);
+        pend_s = 0.0f;
+    }
+    VR_DEV void flush_collision_lane() {
+        if (pend_cs != 0.0f) tr.sigma_bw(px, pend_cs);
+        pend_cs = 0.0f;
+        tr.albedo_bw(px, pend_ca);
+        pend_ca = F3(0.0f, 0.0f, 0.0f);
+    }
 
     // Pass 1 (or 2) of the sample from its camera ray: nee_ds_bw's start
     // (vr_path_integrator_NEE_DS.h:6-30) and its replay restart (:110-125).
@@ -211,7 +270,7 @@ struct NeeDsBwLane {
                         state = kBwCollision;
                     }
                 }
-                if (scatter) tr.sigma_bw(rx, g);
+                if (scatter) pend_s = g;
             }
         }
         if (ended) {
@@ -236,13 +295,14 @@ struct NeeDsBwLane {
             added = (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
         } else {
             const float gs = s < EPS ? dot(dL_dR, R_sp * (1.0f / eps)) : dot(dL_dR, R * (1.0f / s));
-            tr.sigma_bw(rx, gs);
+            pend_cs = gs;  // scattered at the collision point px by flush_collision
+            px = rx;
             W = W * (s / m / Pc);
             if (s < EPS) { state = kBwDone; return; }
             f3 dL_dsa = (dL_dR * W) * (M.emission_c * -1.0f);
             R = R - (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
             dL_dsa = dL_dsa + dL_dR * F3(R.x / fmaxf(EPS, sa.x), R.y / fmaxf(EPS, sa.y), R.z / fmaxf(EPS, sa.z));
-            tr.albedo_bw(rx, dL_dsa);
+            pend_ca = dL_dsa;
         }
         W = W * sa;
         f3 wnee;
