@@ -652,6 +652,32 @@ struct Walk<VR_MAJ_GRID> {
     }
 };
 
+// The coarse majorant grid staged in LDS: an internal majorant kind for
+// kernels that copy a grid of at most kMajLdsCells cells into maj_lds at their
+// start (majorant queries then read LDS instead of global memory); the walk is
+// that of VR_MAJ_GRID.
+constexpr int kMajGridLds = 2;
+constexpr int kMajLdsCells = 4096;
+__shared__ float maj_lds[kMajLdsCells];
+
+template <>
+struct Walk<kMajGridLds> : Walk<VR_MAJ_GRID> {
+    VR_DEV float query(const MediumView& M, float& md) const {
+        if (cell < 0) { md = kInf; return 0.0f; }
+        md = fmaxf(0.0f, fminf(tnx, fminf(tny, tnz)) - t);
+        int cx = cell & 1023, cy = (cell >> 10) & 1023, cz = cell >> 20;
+        return maj_lds[(cz * M.mry + cy) * M.mrx + cx];
+    }
+};
+
+// Copies the medium's majorant grid into maj_lds (block-wide; the caller has
+// checked that it fits).
+VR_DEV void stage_majorant(const MediumView& M) {
+    const int n = M.mrx * M.mry * M.mrz;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) maj_lds[i] = M.maj_grid[i];
+    __syncthreads();
+}
+
 // ------------------------------------------------- kernel-argument access --
 // Read-only data in the constant address space: loads through such a pointer
 // are scalar loads when the address is wave-uniform.

@@ -405,6 +405,7 @@ VR_DEV const NeeArgs& nee_args() { return *(const NeeArgs*)reread(first_kernel_a
 template <int MAJ, int TKIND, bool ENVMAP, bool LEAN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))
 k_nee_persistent(NeeArgs args) {
+    if constexpr (MAJ == kMajGridLds) stage_majorant(args.M);
     const NeeSchedule& S = args.S;
     unsigned long long* __restrict__ counters = args.counters;
     float* __restrict__ out = args.out;

@@ -159,6 +159,46 @@ def test_cell_table_lookups_are_bit_identical(gpu):
         np.testing.assert_array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("tkind,cells", [(L.VR_T_DT, True), (L.VR_T_DT, False), (L.VR_T_RT, False)])
+def test_lds_majorant_equals_global(gpu, tkind, cells):
+    """The persistent NEE kernel with its coarse majorant grid staged in LDS
+    (VR_MAJ_LDS=1, grids of at most 4096 cells and constant environments)
+    renders the same frame, bit for bit at one sample per pixel, as with the
+    grid read from global memory (the default): lean (cell table) and generic
+    forms, DT and RT shadow rays."""
+    import os
+    print(f"[case] lds majorant t_kind={tkind} cells={cells}", flush=True)
+    sc = Scene(sigma=fbm_grid(24, seed=4, scale=25.0), albedo_c=(0.8, 0.8, 0.8), phase_g=0.5, env_c=(0.2, 0.3, 0.4),
+               sun_dir=(0.3, 0.9, 0.3), sun_radiance=(2.0, 2.0, 2.0), sun_prob=0.5, majorant="grid",
+               majorant_res=(16, 16, 16), max_bounces=8, transmittance_kind=tkind)
+    b = Built(sc, "cuda")
+    if cells:
+        n = 23 ** 3
+        table = torch.empty(n * 8, device="cuda")
+        L.check(L.load().vr_grid_pack_cells(b.medium.sigma, table.data_ptr(), torch.cuda.current_stream().cuda_stream),
+                "pack")
+        b.medium.sigma_cells = table.data_ptr()
+    cam = Camera(np.stack([pose((0.2, 0.4, 2.2)), pose((-1.9, 0.1, 1.0))]), 24, 24, jittered=True, device="cuda")
+    outs = []
+    old = os.environ.get("VR_MAJ_LDS")
+    try:
+        for flag in ("0", "1"):
+            os.environ["VR_MAJ_LDS"] = flag
+            out = torch.zeros(cam.threads, 3, device="cuda")
+            lau = L.launch((5, 6, 7, 8), samples=1, bw_mode=L.VR_BW_CONSISTENT)
+            L.check(L.load().vr_capture_fw(b.medium, L.VR_MAP_NEE_DT, cam.c, lau, out.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream), "capture")
+            torch.cuda.synchronize()
+            outs.append(out.cpu().numpy())
+    finally:
+        if old is None:
+            os.environ.pop("VR_MAJ_LDS", None)
+        else:
+            os.environ["VR_MAJ_LDS"] = old
+    assert outs[0].mean() > 0.0
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("bw_mode,majorant,seed_mode,env", [
     (L.VR_BW_CONSISTENT, "grid", L.VR_SEED_CAPTURE_FW, "sun"), (L.VR_BW_REFERENCE, "grid", L.VR_SEED_CAPTURE_FW, "sun"),
     (L.VR_BW_CONSISTENT, "const", L.VR_SEED_CAPTURE_FW, "sun"), (L.VR_BW_REFERENCE, "const", L.VR_SEED_CAPTURE_FW, "sun"),
