@@ -857,7 +857,10 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         S.total = (uint32_t)total;
         S.start = (uint32_t)b0;
         S.cnt = (uint32_t)n;
-        S.nq = std::max(1, std::min(kMaxQueues, env_int("VR_NEE_QUEUES", 1)));
+        // one work queue per XCD: each XCD's L2 serves a contiguous eighth of the
+        // images (headline frame 4.90 ms with one queue, 4.79 with 2, 4.72 with 4,
+        // 4.54 with 8; the 1024^2 x 256 spp render 96.3 -> 92.0 ms)
+        S.nq = std::max(1, std::min(kMaxQueues, env_int("VR_NEE_QUEUES", kMaxQueues)));
         S.qpix = (uint32_t)((n + S.nq - 1) / S.nq);
         S.spp = spp;
         S.chunk = (int)std::min<int64_t>(chunk, spp);
