@@ -828,14 +828,23 @@ int persistent_blocks(K kernel, int* blocks) {
     return VR_OK;
 }
 
-template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false>
+// Cached-render extras of a persistent launch (CACHE): the query buffer.
+struct NeeCache {
+    float* queries = nullptr;
+    int64_t max_queries = 0;
+    uint32_t* qcount = nullptr;
+    int k = 0;
+};
+
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false, bool CACHE = false>
 int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
-                          int64_t cnt, float* out, int* dbg, unsigned long long* counter, hipStream_t st) {
+                          int64_t cnt, float* out, int* dbg, unsigned long long* counter, hipStream_t st,
+                          const NeeCache& cache = NeeCache()) {
     if (total > (int64_t)UINT32_MAX) return set_err(VR_ERR_UNSUPPORTED, "capture has more than 2^32 sensors");
     if (!counter) return set_err(VR_ERR_ARG, "persistent NEE capture without work counters");
     int rc = VR_OK;
     int blocks = 0;
-    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN>, &blocks))) return rc;
+    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN, CACHE>, &blocks))) return rc;
     const int64_t lanes = (int64_t)blocks * kBlock;
     const int64_t hw = (int64_t)C.width * C.height;
     // launches of at most 2^30 sensors (whole images when possible) keep item indices in 32 bits
@@ -876,8 +885,8 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         unsigned prep = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
         hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, b0, n, dbg, counter);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
-        const NeeArgs args{M, C, sd, S, counter, out, dbg};
-        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN>), dim3(blocks), dim3(kBlock), 0, st, args);
+        const NeeArgs args{M, C, sd, S, counter, out, dbg, cache.queries, cache.max_queries, cache.qcount, cache.k};
+        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN, CACHE>), dim3(blocks), dim3(kBlock), 0, st, args);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw launch"))) return rc;
     }
     return VR_OK;
@@ -1897,17 +1906,46 @@ int vr_nrc_render(const vr_medium* medium, const vr_camera* camera, const vr_lau
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count, 1);
     if (cnt == 0) return check_hip(hipGetLastError(), "vr_nrc_render launch");
-    const CaptureSplit P = capture_split(spp, false);
-    if ((st = capture_prepare(P, out, 3, start, cnt, nullptr, s))) return st;
     const Seeds sd = make_seeds(L);
-    const dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
-    if (medium->majorant_kind == VR_MAJ_GRID)
-        hipLaunchKernelGGL(nrc::k_nrc_render<VR_MAJ_GRID>, g, b, 0, s, M, C, sd, spp, P.chunk, P.chunks, (int)terminate_after,
-                           total, start, cnt, out, queries, max_queries, count);
-    else
-        hipLaunchKernelGGL(nrc::k_nrc_render<VR_MAJ_CONST>, g, b, 0, s, M, C, sd, spp, P.chunk, P.chunks, (int)terminate_after,
-                           total, start, cnt, out, queries, max_queries, count);
-    if ((st = check_hip(hipGetLastError(), "vr_nrc_render trace"))) return st;
+    const bool envmap = M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM;
+    if (!envmap && env_int("VR_NRC_PERSISTENT", 1) != 0) {
+        // the persistent lane state machine with paths ending into cache queries;
+        // the cached estimator has no bounce cap of its own
+        MediumView Mc = M;
+        Mc.max_bounces = -1;
+        const bool lean = Mc.cells && !Mc.albedo_grid && !Mc.phase_g_grid && env_int("VR_NEE_LEAN", 1) != 0;
+        NeeCache cache;
+        cache.queries = queries;
+        cache.max_queries = max_queries;
+        cache.qcount = count;
+        cache.k = terminate_after;
+        Counters ctr;
+        if ((st = ctr.acquire(L, s))) return st;
+#define VR_NRC_ARGS Mc, C, sd, spp, total, start, cnt, out, nullptr, ctr.ptr, s, cache
+        if (medium->majorant_kind == VR_MAJ_GRID) {
+            if (Mc.t_kind == VR_T_DT && lean) st = launch_nee_persistent<VR_MAJ_GRID, VR_T_DT, false, true, true>(VR_NRC_ARGS);
+            else if (Mc.t_kind == VR_T_DT) st = launch_nee_persistent<VR_MAJ_GRID, VR_T_DT, false, false, true>(VR_NRC_ARGS);
+            else st = launch_nee_persistent<VR_MAJ_GRID, VR_T_RT, false, false, true>(VR_NRC_ARGS);
+        } else {
+            if (Mc.t_kind == VR_T_DT && lean) st = launch_nee_persistent<VR_MAJ_CONST, VR_T_DT, false, true, true>(VR_NRC_ARGS);
+            else if (Mc.t_kind == VR_T_DT) st = launch_nee_persistent<VR_MAJ_CONST, VR_T_DT, false, false, true>(VR_NRC_ARGS);
+            else st = launch_nee_persistent<VR_MAJ_CONST, VR_T_RT, false, false, true>(VR_NRC_ARGS);
+        }
+#undef VR_NRC_ARGS
+        const int rel = ctr.release();
+        if (st || (st = rel)) return st;
+    } else {
+        const CaptureSplit P = capture_split(spp, false);
+        if ((st = capture_prepare(P, out, 3, start, cnt, nullptr, s))) return st;
+        const dim3 g(grid_for(cnt * P.chunks)), b(kBlock);
+        if (medium->majorant_kind == VR_MAJ_GRID)
+            hipLaunchKernelGGL(nrc::k_nrc_render<VR_MAJ_GRID>, g, b, 0, s, M, C, sd, spp, P.chunk, P.chunks, (int)terminate_after,
+                               total, start, cnt, out, queries, max_queries, count);
+        else
+            hipLaunchKernelGGL(nrc::k_nrc_render<VR_MAJ_CONST>, g, b, 0, s, M, C, sd, spp, P.chunk, P.chunks, (int)terminate_after,
+                               total, start, cnt, out, queries, max_queries, count);
+        if ((st = check_hip(hipGetLastError(), "vr_nrc_render trace"))) return st;
+    }
     f3 o, ib;
     if ((st = box_transform(medium->box_min, medium->box_max, o, ib))) return st;
     if (max_queries > 0)

@@ -396,13 +396,22 @@ struct NeeArgs {
     unsigned long long* counters;
     float* out;
     int* dbg;
+    // cached render (CACHE): a path stops at its (cache_k + 1)-th real collision
+    // and appends the query (x, w, W / spp, sensor) at *qcount
+    float* queries;
+    int64_t max_queries;
+    uint32_t* qcount;
+    int cache_k;
 };
 
 // The kernel's arguments, re-read where a handler uses them (reread): the
 // loads of the fields each handler reads stay in that handler.
 VR_DEV const NeeArgs& nee_args() { return *(const NeeArgs*)reread(first_kernel_arg<NeeArgs>()); }
 
-template <int MAJ, int TKIND, bool ENVMAP, bool LEAN>
+// CACHE: the radiance-cache render -- paths end at their (cache_k + 1)-th real
+// collision, which becomes a query of the cache (the medium's max_bounces must
+// be -1: the cached estimator has no other cap), as nrc::nee_cached.
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN, bool CACHE = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))
 k_nee_persistent(NeeArgs args) {
     if constexpr (MAJ == kMajGridLds) stage_majorant(args.M);
@@ -499,6 +508,30 @@ k_nee_persistent(NeeArgs args) {
             const unsigned long long tracking = __ballot(have && lane.event == kEvNone);
             if (tracking == 0ull || k >= S.kmax) break;
             if (k >= S.ksteps && __popcll(tracking) < S.track_min) break;
+        }
+        if constexpr (CACHE) {
+            // the (cache_k + 1)-th collision: the lanes there append their queries
+            // (one atomic per wave) and end their paths
+            const bool query = have && lane.event == kEvCollision && lane.bounces == nee_args().cache_k;
+            const unsigned long long qm = __ballot(query);
+            if (qm) {
+                const NeeArgs& A = nee_args();
+                const int leader = __ffsll((long long)qm) - 1;
+                unsigned long long base = 0;
+                if ((int)__lane_id() == leader) base = atomicAdd(A.qcount, (uint32_t)__popcll(qm));
+                base = __shfl(base, leader);
+                if (query) {
+                    const uint64_t slot = base + (uint64_t)__popcll(qm & lane_mask_lt());
+                    if ((int64_t)slot < A.max_queries) {
+                        float* q = A.queries + slot * 10;
+                        q[0] = lane.rx.x; q[1] = lane.rx.y; q[2] = lane.rx.z;
+                        q[3] = lane.rw.x; q[4] = lane.rw.y; q[5] = lane.rw.z;
+                        q[6] = lane.W.x * inv; q[7] = lane.W.y * inv; q[8] = lane.W.z * inv;
+                        q[9] = __uint_as_float(sensor);
+                    }
+                    lane.event = kEvPathEnd;
+                }
+            }
         }
         if (have && lane.event == kEvCollision) lane.on_collision(nee_args().M);
         if (have && lane.event == kEvTrackEnd) lane.on_track_end(nee_args().M);

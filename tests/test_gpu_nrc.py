@@ -174,6 +174,55 @@ def test_zero_cache_render_is_the_truncated_nee_path(gpu, tkind):
     assert match_fraction(out.cpu().numpy(), ref, rtol=1e-3, atol=1e-4) >= 0.98
 
 
+@pytest.mark.parametrize("tkind,cells", [(L.VR_T_DT, True), (L.VR_T_DT, False), (L.VR_T_RT, False)])
+def test_persistent_cached_render_equals_the_per_thread_one(gpu, tkind, cells):
+    """The cached render as the persistent lane state machine (the default for
+    constant environments) traces the same samples as the per-thread kernel
+    (VR_NRC_PERSISTENT=0): at one sample per pixel the pre-cache images agree to
+    float rounding (the two kernels contract a few products into FMAs
+    differently) and so do the queries, matched by their pixel."""
+    import os
+    K = 2
+    sc = Scene(sigma=fbm_grid(24, seed=9, scale=20.0), albedo_c=(0.9, 0.8, 0.7), phase_kind=L.VR_PHASE_HG,
+               phase_g=0.6, env_c=(0.5, 0.6, 0.7), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(3.0, 3.0, 3.0), sun_prob=0.5,
+               majorant="grid", transmittance_kind=tkind, max_bounces=8)
+    b = Built(sc, "cuda")
+    if cells:
+        table = torch.empty(23 ** 3 * 8, device="cuda")
+        L.check(L.load().vr_grid_pack_cells(b.medium.sigma, table.data_ptr(), None), "pack")
+        b.medium.sigma_cells = table.data_ptr()
+    cam = Camera(device="cuda", poses=pose((0.0, 0.3, 2.5))[None], width=40, height=40, jittered=True)
+    wz = torch.zeros(L.NRC_PARAMS, dtype=torch.bfloat16, device="cuda")
+    res = []
+    old = os.environ.get("VR_NRC_PERSISTENT")
+    try:
+        for flag in ("0", "1"):
+            os.environ["VR_NRC_PERSISTENT"] = flag
+            out = torch.zeros(1600, 3, device="cuda")
+            q = torch.zeros(1600, 10, device="cuda")
+            qc = torch.zeros(1, dtype=torch.int32, device="cuda")
+            L.check(L.load().vr_nrc_render(b.medium, cam.c, L.launch((3, 1, 4, 1), samples=1), K, wz.data_ptr(),
+                                           out.data_ptr(), q.data_ptr(), 1600, qc.data_ptr(), None), "vr_nrc_render")
+            torch.cuda.synchronize()
+            n = int(qc.item())
+            rows = q[:n].cpu().numpy()
+            pix = rows[:, 9].copy().view(np.uint32)
+            assert len(np.unique(pix)) == n  # one query per pixel at most
+            res.append((out.cpu().numpy(), rows[np.argsort(pix)], np.sort(pix)))
+    finally:
+        if old is None:
+            os.environ.pop("VR_NRC_PERSISTENT", None)
+        else:
+            os.environ["VR_NRC_PERSISTENT"] = old
+    (o0, q0, p0), (o1, q1, p1) = res
+    assert len(p0) > 100 and abs(len(p0) - len(p1)) <= len(p0) // 200
+    assert match_fraction(o1, o0, rtol=1e-5, atol=1e-6) >= 0.995
+    both, i0, i1 = np.intersect1d(p0, p1, return_indices=True)
+    assert len(both) >= 0.995 * len(p0)
+    close = np.all(np.abs(q0[i0, :9] - q1[i1, :9]) <= 1e-5 * np.abs(q0[i0, :9]) + 1e-6, axis=1)
+    assert close.mean() >= 0.995, close.mean()
+
+
 def _nrc_scene():
     """48^3 fBm cloud, sun + environment, HG g = 0.5, NEE with DT shadow rays; 64x64 camera."""
     dev = torch.device("cuda", 0)
