@@ -423,7 +423,7 @@ int vr_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels,
 int vr_nrc_encode(const float* x, const float* w, int64_t n, const float box_min[3], const float box_max[3],
                   float* features, void* stream);
 
-/* MLP on explicit features [n][96] (rounded to BF16) -> out [n][3]. */
+/* MLP on explicit features [n][96] (rounded to BF16; 16-byte aligned) -> out [n][3]. */
 int vr_nrc_mlp_fw(const float* features, int64_t n, const uint16_t* weights, float* out, void* stream);
 
 /* One training step on the records (x[3], w[3], target[3]) of rows

@@ -1819,7 +1819,8 @@ int vr_nrc_mlp_fw(const float* features, int64_t n, const uint16_t* weights, flo
     if (n < 0) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: negative n");
     if (n == 0) return VR_OK;
     if (!features || !weights || !out) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: null buffer");
-    if (((uintptr_t)weights & 15u) != 0) return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: weights must be 16-byte aligned");
+    if (((uintptr_t)weights & 15u) != 0 || ((uintptr_t)features & 15u) != 0)
+        return set_err(VR_ERR_ARG, "vr_nrc_mlp_fw: weights and features must be 16-byte aligned");
     hipLaunchKernelGGL(nrc::k_nrc_mlp_fw, dim3(nrc_blocks(n, nrc::kInferWaves, 4096)), dim3(64 * nrc::kInferWaves), 0,
                        (hipStream_t)stream, features, n, (const __bf16*)weights, out);
     return check_hip(hipGetLastError(), "vr_nrc_mlp_fw launch");

@@ -346,35 +346,120 @@ __global__ void k_nrc_adam(float* __restrict__ p, float* __restrict__ g, float* 
     }
 }
 
-// LDS of one inference block: weights (one layout) and per wave A0..A4.
-struct InferWave {
-    __bf16 acts[kA0 + 4 * kAh];
-    float y[kRows * 3];
-    f3 u[kRows];
-    f3 w[kRows];
-};
-constexpr int kInferWaves = 4;
+// --------------------------------------------------- inference (registers) --
+// Inference keeps a tile's activations in registers from layer to layer.  A
+// wave runs 16 rows per tile: lane l owns row r = l % 16, and its group
+// g = l / 16 holds a quarter of each layer's columns.  Each layer is computed
+// transposed, Z^T = Wt X^T: the MFMA's A operand is a 16 x 32 block of Wt (from
+// LDS) and its B operand the rows' inputs, so the output tile leaves lane (g, r)
+// holding Z[r][16 nt + 4 g + j] (j < 4) -- the layout of the next layer's B
+// operand once the 32 inputs of a K-block are taken in the slot order
+// pi(8 g + j) = 4 g + j (j < 4), 16 + 4 g + j - 4 (j >= 4).  The next layer's
+// weights are read in that order (two 8-byte LDS reads per fragment), so no
+// activation passes through LDS.  Layer 0's inputs are the features, computed
+// per lane in natural order.
+
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+
+// A fragment of layer 0 (K = 96, natural slot order): Wt0[16 nt + r][32 ks + 8 g + j].
+VR_DEV bf16x8 wfrag_in(const __bf16* __restrict__ w, int nt, int ks) {
+    const int l = __lane_id();
+    return *(const bf16x8*)(w + (nt * 16 + l % 16) * (kIn + kPad) + ks * 32 + (l / 16) * 8);
+}
+
+// A fragment of a layer fed by a previous layer (K = 64, slot order pi).
+VR_DEV bf16x8 wfrag_hidden(const __bf16* __restrict__ w, int nt, int ks) {
+    const int l = __lane_id();
+    const __bf16* row = w + (nt * 16 + l % 16) * (kH + kPad) + ks * 32 + (l / 16) * 4;
+    const bf16x4 lo = *(const bf16x4*)row, hi = *(const bf16x4*)(row + 16);
+    bf16x8 v;
+    v[0] = lo[0]; v[1] = lo[1]; v[2] = lo[2]; v[3] = lo[3];
+    v[4] = hi[0]; v[5] = hi[1]; v[6] = hi[2]; v[7] = hi[3];
+    return v;
+}
+
+// ReLU of two output tiles (columns 32 kb + 4 g + j and 32 kb + 16 + 4 g + j)
+// packed as the next layer's B fragment of K-block kb.
+VR_DEV bf16x8 relu_pack(floatx4 a, floatx4 b) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        v[j] = (__bf16)fmaxf(a[j], 0.0f);
+        v[4 + j] = (__bf16)fmaxf(b[j], 0.0f);
+    }
+    return v;
+}
+
+// The network on one 16-row tile whose features are x[0..2] (B fragments of
+// the 3 input K-blocks, natural order); returns the output tile: lanes of
+// group 0 hold Y[r][0..3] (RGB = the first 3).
+VR_DEV floatx4 mlp_tile(const __bf16* __restrict__ w, const bf16x8* x) {
+    floatx4 c[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        c[nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 3; ks++) c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag_in(w + kL0, nt, ks), x[ks], c[nt], 0, 0, 0);
+    }
+    bf16x8 h[2];
+    h[0] = relu_pack(c[0], c[1]);
+    h[1] = relu_pack(c[2], c[3]);
+    const int offs[3] = {kL1, kL2, kL3};
+#pragma unroll
+    for (int L = 0; L < 3; L++) {
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            c[nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+                c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag_hidden(w + offs[L], nt, ks), h[ks], c[nt], 0, 0, 0);
+        }
+        h[0] = relu_pack(c[0], c[1]);
+        h[1] = relu_pack(c[2], c[3]);
+    }
+    floatx4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag_hidden(w + kL4, 0, ks), h[ks], y, 0, 0, 0);
+    return y;
+}
+
+// Feature fragments of this lane's row: slots 8 g + j of the 3 K-blocks.
+VR_DEV void encode_frags(f3 u, f3 wd, bool valid, bf16x8* x) {
+    const int g = (int)__lane_id() / 16;
+#pragma unroll
+    for (int ks = 0; ks < 3; ks++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[ks][j] = (__bf16)(valid ? feature(u, wd, ks * 32 + g * 8 + j) : 0.0f);
+}
+
+constexpr int kInferWaves = 8;  // waves per block (the block shares the weights in LDS)
 
 // MLP on explicit feature vectors [n][96] (FP32, rounded to BF16) -> out [n][3].
 __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_mlp_fw(const float* __restrict__ feats, int64_t n,
                                                                  const __bf16* __restrict__ wg, float* __restrict__ out) {
     __shared__ __bf16 w[kWLds];
-    __shared__ InferWave waves[kInferWaves];
     load_weights(wg, w, nullptr);
     __syncthreads();
-    InferWave& S = waves[threadIdx.x / 64];
-    const int l = __lane_id();
+    const int l = __lane_id(), r = l % 16, g = l / 16;
     const int64_t nw = (int64_t)gridDim.x * kInferWaves;
-    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * kRows; base < n; base += nw * kRows) {
-        const int valid = (int)min((int64_t)kRows, n - base);
-        for (int idx = l; idx < kRows * kIn; idx += 64) {
-            const int r = idx / kIn, i = idx - r * kIn;
-            S.acts[r * (kIn + kPad) + i] = (__bf16)(r < valid ? feats[(base + r) * kIn + i] : 0.0f);
+    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * 16; base < n; base += nw * 16) {
+        const int64_t row = base + r;
+        const bool valid = row < n;
+        bf16x8 x[3];
+#pragma unroll
+        for (int ks = 0; ks < 3; ks++) {
+            const float* f = feats + (valid ? row : 0) * kIn + ks * 32 + g * 8;
+            const float4 a = *(const float4*)f, b = *(const float4*)(f + 4);
+            const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int j = 0; j < 8; j++) x[ks][j] = (__bf16)(valid ? v[j] : 0.0f);
         }
-        wave_sync();
-        mlp_forward(w, S.acts, S.y);
-        for (int idx = l; idx < valid * 3; idx += 64) out[base * 3 + idx] = S.y[idx];
-        wave_sync();
+        const floatx4 y = mlp_tile(w, x);
+        if (g == 0 && valid) {
+            out[row * 3] = y[0];
+            out[row * 3 + 1] = y[1];
+            out[row * 3 + 2] = y[2];
+        }
     }
 }
 
@@ -394,32 +479,24 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __r
                                                                 int64_t max_queries, f3 bmin, f3 ibsize,
                                                                 const __bf16* __restrict__ wg, float* __restrict__ out) {
     __shared__ __bf16 w[kWLds];
-    __shared__ InferWave waves[kInferWaves];
     load_weights(wg, w, nullptr);
     __syncthreads();
     const int64_t count = min((int64_t)*count_p, max_queries);
-    InferWave& S = waves[threadIdx.x / 64];
-    const int l = __lane_id();
+    const int l = __lane_id(), r = l % 16;
     const int64_t nw = (int64_t)gridDim.x * kInferWaves;
-    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * kRows; base < count; base += nw * kRows) {
-        const int valid = (int)min((int64_t)kRows, count - base);
-        if (l < kRows) {
-            const float* q = queries + (base + (l < valid ? l : 0)) * 10;
-            S.u[l] = unit_coords(F3(q[0], q[1], q[2]), bmin, ibsize);
-            S.w[l] = F3(q[3], q[4], q[5]);
-        }
-        wave_sync();
-        encode_rows(S.acts, S.u, S.w, valid);
-        wave_sync();
-        mlp_forward(w, S.acts, S.y);
-        if (l < valid) {
-            const float* q = queries + (base + l) * 10;
+    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * 16; base < count; base += nw * 16) {
+        const int64_t row = base + r;
+        const bool valid = row < count;
+        const float* q = queries + (valid ? row : 0) * 10;
+        bf16x8 x[3];
+        encode_frags(unit_coords(F3(q[0], q[1], q[2]), bmin, ibsize), F3(q[3], q[4], q[5]), valid, x);
+        const floatx4 y = mlp_tile(w, x);
+        if (l < 16 && valid) {
             const uint32_t pix = __float_as_uint(q[9]);
-            atomic_addf(out + 3 * (size_t)pix, q[6] * fmaxf(S.y[l * 3], 0.0f));
-            atomic_addf(out + 3 * (size_t)pix + 1, q[7] * fmaxf(S.y[l * 3 + 1], 0.0f));
-            atomic_addf(out + 3 * (size_t)pix + 2, q[8] * fmaxf(S.y[l * 3 + 2], 0.0f));
+            atomic_addf(out + 3 * (size_t)pix, q[6] * fmaxf(y[0], 0.0f));
+            atomic_addf(out + 3 * (size_t)pix + 1, q[7] * fmaxf(y[1], 0.0f));
+            atomic_addf(out + 3 * (size_t)pix + 2, q[8] * fmaxf(y[2], 0.0f));
         }
-        wave_sync();
     }
 }
 
