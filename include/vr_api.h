@@ -415,8 +415,8 @@ int vr_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels,
  * (16-byte aligned). */
 #define VR_NRC_PARAMS 19456
 #define VR_NRC_FEATURES 96
-/* Copies of the weight gradient a training step accumulates (one per XCD). */
-#define VR_NRC_GRAD_COPIES 8
+/* Copies of the weight gradient a training step accumulates into. */
+#define VR_NRC_GRAD_COPIES 64
 
 /* Input features of n points x and directions w ([n][3] each): sin/cos of
  * pi 2^f u_c (f < 12, u = x mapped from the box to [0,1]^3), w, 1, zeros. */

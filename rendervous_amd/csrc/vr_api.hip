@@ -1793,8 +1793,10 @@ int box_transform(const float bmin[3], const float bmax[3], f3& o, f3& ib) {
     return VR_OK;
 }
 
+// Blocks of `waves_per_block` waves for one 16-row tile per wave (at most cap;
+// the kernels loop over further tiles).
 unsigned nrc_blocks(int64_t rows, int waves_per_block, unsigned cap) {
-    int64_t b = (rows + (int64_t)nrc::kRows * waves_per_block - 1) / ((int64_t)nrc::kRows * waves_per_block);
+    int64_t b = (rows + (int64_t)nrc::kTileRows * waves_per_block - 1) / ((int64_t)nrc::kTileRows * waves_per_block);
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
 }
 
@@ -1838,7 +1840,7 @@ int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first
     if (st) return st;
     if (((uintptr_t)weights & 15u) != 0) return set_err(VR_ERR_ARG, "vr_nrc_train_step: weights must be 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(kBlock), 0, s, grad, (int64_t)nrc::kGradCopies * nrc::kParams);
+    hipLaunchKernelGGL(k_zero_f32, dim3(512), dim3(kBlock), 0, s, grad, (int64_t)nrc::kGradCopies * nrc::kParams);
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(kBlock), 0, s, loss, (int64_t)1);
     if (max_records > 0)
         hipLaunchKernelGGL(nrc::k_nrc_train, dim3(nrc_blocks(max_records, nrc::kTrainWaves, 1024)), dim3(64 * nrc::kTrainWaves),

@@ -28,7 +28,7 @@ constexpr int kIn = 96;      // 72 Fourier features + direction (3) + constant 1
 constexpr int kH = 64;       // width of the 4 hidden layers
 constexpr int kOut = 16;     // output tile; the first 3 are RGB
 constexpr int kFreqs = 12;   // Fourier frequencies per position coordinate
-constexpr int kRows = 32;    // rows per wave
+constexpr int kTileRows = 16;  // rows per MFMA tile (one row per lane of a 16-lane group)
 constexpr int kPad = 8;      // LDS row padding (elements)
 
 // parameter offsets of Wt0 [64][96], Wt1..Wt3 [64][64], Wt4 [16][64]
@@ -74,38 +74,6 @@ VR_DEV f3 unit_coords(f3 x, f3 bmin, f3 ibsize) {
     return F3((x.x - bmin.x) * ibsize.x, (x.y - bmin.y) * ibsize.y, (x.z - bmin.z) * ibsize.z);
 }
 
-// One wave's rows of feature vectors into LDS (row stride kIn + kPad).
-VR_DEV void encode_rows(__bf16* __restrict__ a0, const f3* u, const f3* w, int rows_valid) {
-    const int l = __lane_id();
-    for (int idx = l; idx < kRows * kIn; idx += 64) {
-        const int r = idx / kIn, i = idx - r * kIn;
-        const float v = r < rows_valid ? feature(u[r], w[r], i) : 0.0f;
-        a0[r * (kIn + kPad) + i] = (__bf16)v;
-    }
-}
-
-// Z[r][n] = sum_k A[r][k] Wt[n][k] for the wave's 32 rows; `ain` has row stride
-// K + kPad, `wt` row stride K + kPad.  The epilogue gets (row, col, value).
-template <int K, int N, typename Epi>
-VR_DEV void layer_fw(const __bf16* __restrict__ ain, const __bf16* __restrict__ wt, Epi epi) {
-    const int l = __lane_id();
-#pragma unroll
-    for (int mt = 0; mt < kRows / 16; mt++) {
-#pragma unroll
-        for (int nt = 0; nt < N / 16; nt++) {
-            floatx4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int ks = 0; ks < K / 32; ks++) {
-                const bf16x8 a = *(const bf16x8*)(ain + (mt * 16 + l % 16) * (K + kPad) + ks * 32 + (l / 16) * 8);
-                const bf16x8 b = *(const bf16x8*)(wt + (nt * 16 + l % 16) * (K + kPad) + ks * 32 + (l / 16) * 8);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++) epi(mt * 16 + (l / 16) * 4 + j, nt * 16 + l % 16, c[j]);
-        }
-    }
-}
-
 // Copies Wt [N][K] (global BF16, 16-byte aligned) into LDS with row stride
 // K + kPad and, if `wT` is given, its transpose [K][N] with row stride N + kPad.
 // Block-wide; 8 consecutive k of one row per 16-byte load (K is a multiple of 8).
@@ -147,185 +115,12 @@ VR_DEV void load_weights(const __bf16* __restrict__ g, __bf16* __restrict__ w, _
     load_layer<kH, kOut>(g + kOff4, w + kL4, wT ? wT + kT4 : nullptr);
 }
 
-// Per-wave activation scratch (elements): A0 [32][96+8] and A1..A4 [32][64+8].
-constexpr int kA0 = kRows * (kIn + kPad);
-constexpr int kAh = kRows * (kH + kPad);
-
-// Forward of the wave's rows from A0 (already encoded) through the hidden
-// layers into A1..A4 (ReLU, BF16), then the output layer into y[r][0..2]
-// (FP32, the lane holding column c < 3 of row r writes it).
-VR_DEV void mlp_forward(const __bf16* __restrict__ w, __bf16* __restrict__ acts, float* __restrict__ y) {
-    __bf16* a0 = acts;
-    __bf16* a1 = a0 + kA0;
-    __bf16* a2 = a1 + kAh;
-    __bf16* a3 = a2 + kAh;
-    __bf16* a4 = a3 + kAh;
-    auto relu_to = [](__bf16* dst) {
-        return [dst](int r, int c, float v) { dst[r * (kH + kPad) + c] = (__bf16)fmaxf(v, 0.0f); };
-    };
-    layer_fw<kIn, kH>(a0, w + kL0, relu_to(a1));
-    wave_sync();
-    layer_fw<kH, kH>(a1, w + kL1, relu_to(a2));
-    wave_sync();
-    layer_fw<kH, kH>(a2, w + kL2, relu_to(a3));
-    wave_sync();
-    layer_fw<kH, kH>(a3, w + kL3, relu_to(a4));
-    wave_sync();
-    layer_fw<kH, kOut>(a4, w + kL4, [y](int r, int c, float v) {
-        if (c < 3) y[r * 3 + c] = v;
-    });
-    wave_sync();
-}
-
-
-// Copies of the FP32 weight gradient the training blocks add into: block b adds
-// into copy b % kGradCopies, which (workgroups being dealt to the 8 XCDs in
-// turn) keeps each copy's atomics inside one XCD's L2; the Adam kernel sums them.
-constexpr int kGradCopies = 8;
-
-// dW[n][k] += sum over the wave's 32 rows of dz[r][n] * a[r][k] (global FP32 atomics).
-template <int N, int K>
-VR_DEV void layer_dw(const __bf16* __restrict__ dz, const __bf16* __restrict__ a, float* __restrict__ gw) {
-    const int l = __lane_id();
-#pragma unroll
-    for (int nt = 0; nt < N / 16; nt++) {
-#pragma unroll
-        for (int kt = 0; kt < K / 16; kt++) {
-            bf16x8 A, B;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int r = (l / 16) * 8 + j;
-                A[j] = dz[r * (kH + kPad) + nt * 16 + l % 16];
-                B[j] = a[r * (K + kPad) + kt * 16 + l % 16];
-            }
-            floatx4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, c, 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < 4; j++) atomic_addf(gw + (nt * 16 + (l / 16) * 4 + j) * K + kt * 16 + l % 16, c[j]);
-        }
-    }
-}
-
-// dA[r][k] = sum_n dz[r][n] * Wt[n][k] over NIN inputs n (16 or 64), with the
-// transposed weights wT [K][NIN + kPad]; the epilogue gets (row, k, value).
-template <int NIN, int K, typename Epi>
-VR_DEV void layer_da(const __bf16* __restrict__ dz, const __bf16* __restrict__ wT, Epi epi) {
-    const int l = __lane_id();
-    const bf16x8 zero = {};
-#pragma unroll
-    for (int mt = 0; mt < kRows / 16; mt++) {
-#pragma unroll
-        for (int kt = 0; kt < K / 16; kt++) {
-            floatx4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int ks = 0; ks < (NIN + 31) / 32; ks++) {
-                const int n0 = ks * 32 + (l / 16) * 8;
-                bf16x8 A = zero, B = zero;
-                if (n0 < NIN) {
-                    A = *(const bf16x8*)(dz + (mt * 16 + l % 16) * (kH + kPad) + n0);
-                    B = *(const bf16x8*)(wT + (kt * 16 + l % 16) * (NIN + kPad) + n0);
-                }
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, c, 0, 0, 0);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++) epi(mt * 16 + (l / 16) * 4 + j, kt * 16 + l % 16, c[j]);
-        }
-    }
-}
-
-// LDS of one training block: weights in both layouts, then per wave A0..A4,
-// two gradient buffers and the rows' inputs, outputs and targets.
-struct TrainWave {
-    __bf16 acts[kA0 + 4 * kAh];
-    __bf16 dz[kAh];
-    __bf16 dz2[kAh];
-    float y[kRows * 3];
-    float t[kRows * 3];
-    f3 u[kRows];
-    f3 w[kRows];
-};
-constexpr int kTrainWaves = 2;
-
-// One optimisation step over the training records (x[3], w[3], target[3]) of
-// rows [f, min(count, f + max_records)) with f = first % count (so that
-// consecutive windows cycle through however many records were written):
-// relative L2 loss sum_c (q_c - t_c)^2 / (q_c^2 + 0.01) averaged over rows and
-// channels (q_c without gradient in the denominator), its gradient accumulated
-// into the kGradCopies copies at `grad` (FP32 [kGradCopies][kParams], zeroed by
-// the caller) and the loss into *loss.
-__global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __restrict__ records, const uint32_t* __restrict__ count_p,
-                                                                int64_t first, int64_t max_records, f3 bmin, f3 ibsize,
-                                                                const __bf16* __restrict__ wg, float* __restrict__ grad,
-                                                                float* __restrict__ loss) {
-    __shared__ __bf16 w[kWLds];
-    __shared__ __bf16 wT[kWTLds];
-    __shared__ TrainWave waves[kTrainWaves];
-    load_weights(wg, w, wT);
-    __syncthreads();
-    grad += (size_t)(blockIdx.x % kGradCopies) * kParams;
-    const int64_t written = (int64_t)*count_p;
-    if (written <= 0) return;
-    const int64_t f = first % written;
-    const int64_t count = min(written - f, max_records);
-    records += f * 9;
-    const float inv = 1.0f / (3.0f * (float)count);
-    TrainWave& S = waves[threadIdx.x / 64];
-    const int l = __lane_id();
-    float lsum = 0.0f;
-    const int64_t nw = (int64_t)gridDim.x * kTrainWaves;
-    for (int64_t base = ((int64_t)blockIdx.x * kTrainWaves + threadIdx.x / 64) * kRows; base < count; base += nw * kRows) {
-        const int valid = (int)min((int64_t)kRows, count - base);
-        if (l < kRows) {
-            const float* rec = records + (base + (l < valid ? l : 0)) * 9;
-            S.u[l] = unit_coords(F3(rec[0], rec[1], rec[2]), bmin, ibsize);
-            S.w[l] = F3(rec[3], rec[4], rec[5]);
-            S.t[l * 3] = rec[6]; S.t[l * 3 + 1] = rec[7]; S.t[l * 3 + 2] = rec[8];
-        }
-        wave_sync();
-        encode_rows(S.acts, S.u, S.w, valid);
-        wave_sync();
-        mlp_forward(w, S.acts, S.y);
-        // output gradient (columns >= 3 and rows >= valid are zero)
-        for (int idx = l; idx < kRows * 16; idx += 64) {
-            const int r = idx / 16, c = idx - r * 16;
-            float g = 0.0f;
-            if (c < 3 && r < valid) {
-                const float q = S.y[r * 3 + c], d = q - S.t[r * 3 + c], den = q * q + 0.01f;
-                g = 2.0f * d / den * inv;
-                lsum += d * d / den * inv;
-            }
-            S.dz[r * (kH + kPad) + c] = (__bf16)g;
-        }
-        wave_sync();
-        const __bf16* a0 = S.acts;
-        const __bf16* a1 = a0 + kA0;
-        const __bf16* a2 = a1 + kAh;
-        const __bf16* a3 = a2 + kAh;
-        const __bf16* a4 = a3 + kAh;
-        auto masked_to = [](__bf16* dst, const __bf16* act) {
-            return [dst, act](int r, int k, float v) {
-                dst[r * (kH + kPad) + k] = (__bf16)((float)act[r * (kH + kPad) + k] > 0.0f ? v : 0.0f);
-            };
-        };
-        // output layer (16 x 64): dY -> dW4, dZ3
-        layer_dw<kOut, kH>(S.dz, a4, grad + kOff4);
-        layer_da<kOut, kH>(S.dz, wT + kT4, masked_to(S.dz2, a4));
-        wave_sync();
-        layer_dw<kH, kH>(S.dz2, a3, grad + kOff3);
-        layer_da<kH, kH>(S.dz2, wT + kT3, masked_to(S.dz, a3));
-        wave_sync();
-        layer_dw<kH, kH>(S.dz, a2, grad + kOff2);
-        layer_da<kH, kH>(S.dz, wT + kT2, masked_to(S.dz2, a2));
-        wave_sync();
-        layer_dw<kH, kH>(S.dz2, a1, grad + kOff1);
-        layer_da<kH, kH>(S.dz2, wT + kT1, masked_to(S.dz, a1));
-        wave_sync();
-        layer_dw<kH, kIn>(S.dz, a0, grad + kOff0);
-        wave_sync();
-    }
-    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-    if (l == 0) atomic_addf(loss, lsum);
-}
+// Copies of the FP32 weight gradient the training waves add into: wave w adds
+// into copy w % kGradCopies.  Float atomics are performed memory-side, and adds
+// to one address serialise there: with every tile of a step adding into one
+// copy, a step's 1536 tiles queue 1536 deep on each weight; 64 copies cut the
+// queues to 24 (the Adam kernel sums the copies).
+constexpr int kGradCopies = 64;
 
 // Adam on the FP32 master weights (torch.optim.Adam semantics) and the BF16
 // copy, on the sum of the gradient copies g[c][i]; the sum is left in copy 0.
@@ -498,6 +293,230 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __r
             atomic_addf(out + 3 * (size_t)pix + 2, q[8] * fmaxf(y[2], 0.0f));
         }
     }
+}
+
+// ------------------------------------------------------------- training --
+// A training step runs the inference tile (above) forward, keeping each
+// layer's input fragments, then back-propagates in the same transposed form:
+// dX^T = Wt^T dZ^T with the transposed weights as the A operand (read in the
+// K-slot order of dZ's tiles) and dZ packed as the B operand, so dX lands in
+// the output layout and the ReLU masks are lane-local.  The weight gradient
+// dW = dZ^T X reduces over the tile's 16 rows: the wave writes dZ and X to LDS
+// transposed (one row of 16 values per column) and takes 16 x 16 x 16 MFMAs
+// (v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+j], B[4(l/16)+j][l%16]
+// and returns C[4(l/16)+j][l%16], j < 4) whose rows are the 16 rows, adding
+// each 16 x 16 block into the gradient copy (16 consecutive floats per group).
+
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short short4_t;
+
+constexpr int kTrainWaves = 8;  // waves per block (they share both weight layouts in LDS)
+constexpr int kTrStride = 20;   // LDS row stride (elements) of a transposed column: 16 rows + pad
+
+// A fragment of a layer's transposed weights wT [K][NIN + kPad] (K-tile kt,
+// dZ block kb): lane (g, i) holds wT[16 kt + i][32 kb + pi(8 g + j)], zeros
+// past NIN.
+template <int NIN>
+VR_DEV bf16x8 wtfrag(const __bf16* __restrict__ wT, int kt, int kb) {
+    const int l = __lane_id();
+    const __bf16* row = wT + (kt * 16 + l % 16) * (NIN + kPad) + kb * 32 + (l / 16) * 4;
+    const bf16x4 lo = *(const bf16x4*)row;
+    bf16x8 v;
+    v[0] = lo[0]; v[1] = lo[1]; v[2] = lo[2]; v[3] = lo[3];
+    if (kb * 32 + 16 < NIN) {
+        const bf16x4 hi = *(const bf16x4*)(row + 16);
+        v[4] = hi[0]; v[5] = hi[1]; v[6] = hi[2]; v[7] = hi[3];
+    } else {
+        v[4] = v[5] = v[6] = v[7] = (__bf16)0.0f;
+    }
+    return v;
+}
+
+// Two output-layout tiles as one B fragment (columns 32 kb + pi(8 g + j)).
+VR_DEV bf16x8 pack2(floatx4 a, floatx4 b) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        v[j] = (__bf16)a[j];
+        v[4 + j] = (__bf16)b[j];
+    }
+    return v;
+}
+
+// dZ of the layer below: dX (4 output-layout tiles, 64 columns) masked by that
+// layer's ReLU outputs h (its B fragments), packed as B fragments.
+VR_DEV void mask_pack(const floatx4* dx, const bf16x8* h, bf16x8* dz) {
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            dz[kb][j] = (__bf16)((float)h[kb][j] > 0.0f ? dx[2 * kb][j] : 0.0f);
+            dz[kb][4 + j] = (__bf16)((float)h[kb][4 + j] > 0.0f ? dx[2 * kb + 1][j] : 0.0f);
+        }
+}
+
+// dX^T = Wt^T dZ^T for the 64 inputs of a layer with NIN outputs (NIN = 16: the
+// output layer, one dZ block; 64: a hidden layer, two).
+template <int NIN>
+VR_DEV void back_layer(const __bf16* __restrict__ wT, const bf16x8* dz, floatx4* dx) {
+    constexpr int KB = (NIN + 31) / 32;
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) {
+        dx[kt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) dx[kt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wtfrag<NIN>(wT, kt, kb), dz[kb], dx[kt], 0, 0, 0);
+    }
+}
+
+// dW[n][k] += sum over the tile's rows of dZ[r][n] X[r][k] for a layer of N
+// outputs (dZ: N/32 or one half-filled B fragment) and K inputs (X: K/32 B
+// fragments, natural slot order when NATURAL, else pi), into gw (row stride K).
+// tz, tx: the wave's LDS scratch for the transposed columns.
+template <int N, int K, bool NATURAL>
+VR_DEV void weight_grad(const bf16x8* dz, const bf16x8* x, __bf16* __restrict__ tz, __bf16* __restrict__ tx,
+                        float* __restrict__ gw) {
+    const int l = __lane_id(), r = l % 16, g = l / 16;
+    constexpr int ZB = (N + 31) / 32, XB = K / 32;
+#pragma unroll
+    for (int kb = 0; kb < ZB; kb++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int n = kb * 32 + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
+            if (n < N) tz[n * kTrStride + r] = dz[kb][j];
+        }
+#pragma unroll
+    for (int kb = 0; kb < XB; kb++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = kb * 32 + (NATURAL ? 8 * g + j : (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4));
+            tx[k * kTrStride + r] = x[kb][j];
+        }
+    wave_sync();
+    // one output tile row at a time (unrolled, the addresses of all 4 x K/16
+    // blocks' atomics would be held in registers at once)
+#pragma unroll 1
+    for (int nt = 0; nt < N / 16; nt++) {
+        const short4_t a = *(const short4_t*)(tz + (nt * 16 + r) * kTrStride + 4 * g);
+        float* gr = gw + (nt * 16 + 4 * g) * K + r;
+#pragma unroll
+        for (int kt = 0; kt < K / 16; kt++) {
+            const short4_t b = *(const short4_t*)(tx + (kt * 16 + r) * kTrStride + 4 * g);
+            const floatx4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, floatx4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) atomic_addf(gr + jj * K + kt * 16, c[jj]);
+        }
+    }
+    wave_sync();
+}
+
+// One optimisation step over the training records (x[3], w[3], target[3]) of
+// rows [f, min(count, f + max_records)) with f = first % count (so that
+// consecutive windows cycle through however many records were written):
+// relative L2 loss sum_c (q_c - t_c)^2 / (q_c^2 + 0.01) averaged over rows and
+// channels (q_c without gradient in the denominator), its gradient accumulated
+// into the kGradCopies copies at `grad` (FP32 [kGradCopies][kParams], zeroed by
+// the caller) and the loss into *loss.
+__global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __restrict__ records, const uint32_t* __restrict__ count_p,
+                                                                int64_t first, int64_t max_records, f3 bmin, f3 ibsize,
+                                                                const __bf16* __restrict__ wg, float* __restrict__ grad,
+                                                                float* __restrict__ loss) {
+    __shared__ __bf16 w_lds[kWLds];
+    __shared__ __bf16 wT_lds[kWTLds];
+    __shared__ __bf16 scratch[kTrainWaves][(kH + kIn) * kTrStride];
+    load_weights(wg, w_lds, wT_lds);
+    __syncthreads();
+    grad += (size_t)((blockIdx.x * kTrainWaves + threadIdx.x / 64) % kGradCopies) * kParams;
+    const int64_t written = (int64_t)*count_p;
+    if (written <= 0) return;
+    const int64_t f = first % written;
+    const int64_t count = min(written - f, max_records);
+    records += f * 9;
+    const float inv = 1.0f / (3.0f * (float)count);
+    __bf16* tz = scratch[threadIdx.x / 64];
+    __bf16* tx = tz + kH * kTrStride;
+    const int l = __lane_id(), r = l % 16, g = l / 16;
+    float lsum = 0.0f;
+    const int64_t nw = (int64_t)gridDim.x * kTrainWaves;
+    for (int64_t base = ((int64_t)blockIdx.x * kTrainWaves + threadIdx.x / 64) * kTileRows; base < count;
+         base += nw * kTileRows) {
+        // the weights are re-read from LDS per tile: an offset the compiler
+        // cannot see through keeps it from hoisting every weight fragment of
+        // both layouts into registers (~260 VGPRs, spilled)
+        int zero;
+        asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+        const __bf16* w = w_lds + zero;
+        const __bf16* wT = wT_lds + zero;
+        const int64_t row = base + r;
+        const bool valid = row < count;
+        const float* rec = records + (valid ? row : 0) * 9;
+        bf16x8 x[3];
+        encode_frags(unit_coords(F3(rec[0], rec[1], rec[2]), bmin, ibsize), F3(rec[3], rec[4], rec[5]), valid, x);
+        // forward, keeping each layer's input fragments
+        floatx4 c[4];
+        bf16x8 h1[2], h2[2], h3[2], h4[2];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            c[nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 3; ks++) c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag_in(w + kL0, nt, ks), x[ks], c[nt], 0, 0, 0);
+        }
+        h1[0] = relu_pack(c[0], c[1]);
+        h1[1] = relu_pack(c[2], c[3]);
+        // one layer at a time: the scheduler would otherwise issue every
+        // layer's weight reads up front and hold them in registers
+        __builtin_amdgcn_sched_barrier(0);
+        bf16x8* hs[4] = {h1, h2, h3, h4};
+        const int offs[3] = {kL1, kL2, kL3};
+#pragma unroll
+        for (int L = 0; L < 3; L++) {
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                c[nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++)
+                    c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag_hidden(w + offs[L], nt, ks), hs[L][ks], c[nt], 0, 0, 0);
+            }
+            hs[L + 1][0] = relu_pack(c[0], c[1]);
+            hs[L + 1][1] = relu_pack(c[2], c[3]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        floatx4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag_hidden(w + kL4, 0, ks), h4[ks], y, 0, 0, 0);
+        // output gradient (group 0 holds outputs 0..3 of each row; RGB = 0..2)
+        floatx4 dy = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (g == 0 && valid) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float q = y[j], d = q - rec[6 + j], den = q * q + 0.01f;
+                dy[j] = 2.0f * d / den * inv;
+                lsum += d * d / den * inv;
+            }
+        }
+        bf16x8 dz[2];
+        dz[0] = pack2(dy, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+        floatx4 dx[4];
+        // output layer (16 x 64), then the hidden layers
+        __builtin_amdgcn_sched_barrier(0);
+        weight_grad<kOut, kH, false>(dz, h4, tz, tx, grad + kOff4);
+        back_layer<kOut>(wT + kT4, dz, dx);
+        mask_pack(dx, h4, dz);
+        __builtin_amdgcn_sched_barrier(0);
+        weight_grad<kH, kH, false>(dz, h3, tz, tx, grad + kOff3);
+        back_layer<kH>(wT + kT3, dz, dx);
+        mask_pack(dx, h3, dz);
+        __builtin_amdgcn_sched_barrier(0);
+        weight_grad<kH, kH, false>(dz, h2, tz, tx, grad + kOff2);
+        back_layer<kH>(wT + kT2, dz, dx);
+        mask_pack(dx, h2, dz);
+        __builtin_amdgcn_sched_barrier(0);
+        weight_grad<kH, kH, false>(dz, h1, tz, tx, grad + kOff1);
+        back_layer<kH>(wT + kT1, dz, dx);
+        mask_pack(dx, h1, dz);
+        __builtin_amdgcn_sched_barrier(0);
+        weight_grad<kH, kIn, true>(dz, x, tz, tx, grad + kOff0);
+    }
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+    if (l == 0) atomic_addf(loss, lsum);
 }
 
 // ---------------------------------------------------------------- paths --
