@@ -11,8 +11,8 @@
 //
 // Layout.  Weights are stored per layer as Wt[n][k] (output neuron n, input k,
 // row-major), in FP32 (master, optimised by Adam) and BF16 (read by the MFMA
-// kernels).  Each wave owns 32 rows (two 16-row MFMA tiles); activations live in
-// LDS as BF16 rows padded by 8 elements (16 bytes) to spread LDS banks.
+// kernels, staged in LDS as rows padded by 8 elements).  A wave works on tiles
+// of 16 rows whose activations stay in registers (see the inference section).
 //
 // MFMA 16x16x32 BF16 lane layout (checked on the device): lane l holds
 // A[l%16][8*(l/16) + j] and B[8*(l/16) + j][l%16] for j < 8, and returns
