@@ -416,7 +416,8 @@ int vr_copy_morton_to_img(const float* in, int64_t resolution, int32_t channels,
 #define VR_NRC_PARAMS 19456
 #define VR_NRC_FEATURES 96
 /* Copies of the weight gradient a training step accumulates into. */
-#define VR_NRC_GRAD_COPIES 64
+#define VR_NRC_GRAD_COPIES 16
+#define VR_NRC_LOSS_SLOTS 64
 
 /* Input features of n points x and directions w ([n][3] each): sin/cos of
  * pi 2^f u_c (f < 12, u = x mapped from the box to [0,1]^3), w, 1, zeros. */
@@ -432,8 +433,9 @@ int vr_nrc_mlp_fw(const float* features, int64_t n, const uint16_t* weights, flo
  * through the records: relative L2 loss, its gradient accumulated in `grad`
  * (VR_NRC_GRAD_COPIES * VR_NRC_PARAMS floats, overwritten; on return the first
  * VR_NRC_PARAMS hold the step's gradient), Adam step `step` (>= 1) on
- * `params`/`m`/`v`, BF16 copy into `weights`, mean loss into *loss (device).
- * lr = 0 leaves the weights. */
+ * `params`/`m`/`v`, BF16 copy into `weights`; `loss` holds VR_NRC_LOSS_SLOTS
+ * floats (device, overwritten), the step's mean loss in loss[0].  lr = 0
+ * leaves the weights. */
 int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first, int64_t max_records, const float box_min[3],
                       const float box_max[3], float* params, uint16_t* weights, float* grad, float* m, float* v,
                       int32_t step, float lr, float beta1, float beta2, float eps, float* loss, void* stream);

@@ -269,7 +269,8 @@ def u32s(values) -> ctypes.Array:
 
 
 NRC_PARAMS = 19456
-NRC_GRAD_COPIES = 64  # weight-gradient copies of a training step (VR_NRC_GRAD_COPIES)
+NRC_GRAD_COPIES = 16  # weight-gradient copies of a training step (VR_NRC_GRAD_COPIES)
+NRC_LOSS_SLOTS = 64  # floats of a training step's loss buffer (VR_NRC_LOSS_SLOTS)
 NRC_FEATURES = 96
 
 # Symbols declared in include/vr_api.h (checked by the CPU test suite).

@@ -1800,6 +1800,9 @@ unsigned nrc_blocks(int64_t rows, int waves_per_block, unsigned cap) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
 }
 
+// Blocks of the gradient/loss zeroing launch: one element per thread.
+unsigned nrc_zero_blocks() { return (unsigned)((nrc::kGradCopies * nrc::kParams + kBlock - 1) / kBlock); }
+
 }  // namespace
 
 extern "C" {
@@ -1840,15 +1843,14 @@ int vr_nrc_train_step(const float* records, const uint32_t* count, int64_t first
     if (st) return st;
     if (((uintptr_t)weights & 15u) != 0) return set_err(VR_ERR_ARG, "vr_nrc_train_step: weights must be 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_zero_f32, dim3(512), dim3(kBlock), 0, s, grad, (int64_t)nrc::kGradCopies * nrc::kParams);
-    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(kBlock), 0, s, loss, (int64_t)1);
+    hipLaunchKernelGGL(nrc::k_nrc_zero, dim3(nrc_zero_blocks()), dim3(kBlock), 0, s, grad, loss);
     if (max_records > 0)
         hipLaunchKernelGGL(nrc::k_nrc_train, dim3(nrc_blocks(max_records, nrc::kTrainWaves, 1024)), dim3(64 * nrc::kTrainWaves),
                            0, s, records, count, first, max_records, o, ib, (const __bf16*)weights, grad, loss);
     const float bc1 = (float)(1.0 - std::pow((double)beta1, step));
     const float bc2s = (float)std::sqrt(1.0 - std::pow((double)beta2, step));
     hipLaunchKernelGGL(nrc::k_nrc_adam, dim3((nrc::kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, s, params, grad, m, v,
-                       (__bf16*)weights, lr, beta1, beta2, eps, bc1, bc2s);
+                       (__bf16*)weights, lr, beta1, beta2, eps, bc1, bc2s, loss);
     return check_hip(hipGetLastError(), "vr_nrc_train_step launch");
 }
 

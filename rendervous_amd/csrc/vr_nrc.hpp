@@ -74,21 +74,54 @@ VR_DEV f3 unit_coords(f3 x, f3 bmin, f3 ibsize) {
     return F3((x.x - bmin.x) * ibsize.x, (x.y - bmin.y) * ibsize.y, (x.z - bmin.z) * ibsize.z);
 }
 
-// Copies Wt [N][K] (global BF16, 16-byte aligned) into LDS with row stride
-// K + kPad and, if `wT` is given, its transpose [K][N] with row stride N + kPad.
-// Block-wide; 8 consecutive k of one row per 16-byte load (K is a multiple of 8).
-template <int K, int N>
-VR_DEV void load_layer(const __bf16* __restrict__ g, __bf16* __restrict__ wt, __bf16* __restrict__ wT) {
-    static_assert(K % 8 == 0, "rows are copied 8 elements at a time");
-    for (int i = threadIdx.x; i < N * K / 8; i += blockDim.x) {
-        const int e = i * 8;
-        const int n = e / K, k = e - n * K;
-        const bf16x8 v = *(const bf16x8*)(g + e);
-        *(bf16x8*)(wt + n * (K + kPad) + k) = v;
-        if (wT) {
+// Weight staging.  The global weights (BF16 [kParams], 16-byte aligned) are
+// read as 16-byte chunks (8 consecutive k of one row n of a layer Wt [N][K]):
+// thread t of a block of THREADS holds chunks t, t + THREADS, ...; the loads
+// are issued first (fetch_weights) and the chunks written to LDS later
+// (stage_weights), so a kernel can do other work while they are in flight.
+// Each layer goes to LDS with row stride K + kPad and, if `wT` is given
+// (layers 1..4), transposed [K][N] with row stride N + kPad.  Within a layer
+// the chunks run down the rows first (row n = i % N, columns 8 (i / N) ..), so
+// a wave's transposed stores cover consecutive n (one LDS row segment) and its
+// natural stores start at distinct banks.  Layers hold multiples of 64 chunks,
+// so the layer branches are uniform per wave.
+constexpr int kWChunks = kParams / 8;
+template <int THREADS>
+constexpr int kWPer = (kWChunks + THREADS - 1) / THREADS;
+
+// Layer, row and first column of chunk c.
+VR_DEV void chunk_rc(int c, int& layer, int& n, int& k) {
+    constexpr int c1 = kOff1 / 8, c2 = kOff2 / 8, c3 = kOff3 / 8, c4 = kOff4 / 8;
+    static_assert(c1 % 64 == 0 && (c2 - c1) % 64 == 0 && (c4 - c3) % 64 == 0 && (kWChunks - c4) % 64 == 0,
+                  "layers hold whole waves of chunks");
+    int i, N;
+    if (c < c1) { layer = 0; i = c; N = kH; }
+    else if (c < c4) { layer = 1 + (c - c1) / (c2 - c1); i = (c - c1) % (c2 - c1); N = kH; }
+    else { layer = 4; i = c - c4; N = kOut; }
+    n = i % N;
+    k = 8 * (i / N);
+}
+
+template <int THREADS>
+VR_DEV void fetch_weights(const __bf16* __restrict__ g, bf16x8* v) {
+    const int offs[5] = {kOff0, kOff1, kOff2, kOff3, kOff4};
 #pragma unroll
-            for (int j = 0; j < 8; j++) wT[(k + j) * (N + kPad) + n] = v[j];
+    for (int t = 0; t < kWPer<THREADS>; t++) {
+        const int c = (int)threadIdx.x + t * THREADS;
+        if (c < kWChunks) {
+            int layer, n, k;
+            chunk_rc(c, layer, n, k);
+            v[t] = *(const bf16x8*)(g + offs[layer] + n * (layer == 0 ? kIn : kH) + k);
         }
+    }
+}
+
+template <int K, int N>
+VR_DEV void stage_chunk(int n, int k, bf16x8 v, __bf16* __restrict__ wt, __bf16* __restrict__ wT) {
+    *(bf16x8*)(wt + n * (K + kPad) + k) = v;
+    if (wT) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) wT[(k + j) * (N + kPad) + n] = v[j];
     }
 }
 
@@ -107,25 +140,59 @@ constexpr int kT3 = kT2 + kH * (kH + kPad);
 constexpr int kT4 = kT3 + kH * (kH + kPad);
 constexpr int kWTLds = kT4 + kH * (kOut + kPad);
 
-VR_DEV void load_weights(const __bf16* __restrict__ g, __bf16* __restrict__ w, __bf16* __restrict__ wT) {
-    load_layer<kIn, kH>(g + kOff0, w + kL0, nullptr);
-    load_layer<kH, kH>(g + kOff1, w + kL1, wT ? wT + kT1 : nullptr);
-    load_layer<kH, kH>(g + kOff2, w + kL2, wT ? wT + kT2 : nullptr);
-    load_layer<kH, kH>(g + kOff3, w + kL3, wT ? wT + kT3 : nullptr);
-    load_layer<kH, kOut>(g + kOff4, w + kL4, wT ? wT + kT4 : nullptr);
+template <int THREADS>
+VR_DEV void stage_weights(const bf16x8* v, __bf16* __restrict__ w, __bf16* __restrict__ wT) {
+#pragma unroll
+    for (int t = 0; t < kWPer<THREADS>; t++) {
+        const int c = (int)threadIdx.x + t * THREADS;
+        if (c >= kWChunks) continue;
+        int layer, n, k;
+        chunk_rc(c, layer, n, k);
+        if (layer == 0) stage_chunk<kIn, kH>(n, k, v[t], w + kL0, nullptr);
+        else if (layer == 1) stage_chunk<kH, kH>(n, k, v[t], w + kL1, wT ? wT + kT1 : nullptr);
+        else if (layer == 2) stage_chunk<kH, kH>(n, k, v[t], w + kL2, wT ? wT + kT2 : nullptr);
+        else if (layer == 3) stage_chunk<kH, kH>(n, k, v[t], w + kL3, wT ? wT + kT3 : nullptr);
+        else stage_chunk<kH, kOut>(n, k, v[t], w + kL4, wT ? wT + kT4 : nullptr);
+    }
 }
 
-// Copies of the FP32 weight gradient the training waves add into: wave w adds
-// into copy w % kGradCopies.  Float atomics are performed memory-side, and adds
-// to one address serialise there: with every tile of a step adding into one
-// copy, a step's 1536 tiles queue 1536 deep on each weight; 64 copies cut the
-// queues to 24 (the Adam kernel sums the copies).
-constexpr int kGradCopies = 64;
+// fetch_weights + stage_weights (the caller synchronises the block afterwards).
+template <int THREADS>
+VR_DEV void load_weights(const __bf16* __restrict__ g, __bf16* __restrict__ w, __bf16* __restrict__ wT) {
+    bf16x8 v[kWPer<THREADS>];
+    fetch_weights<THREADS>(g, v);
+    stage_weights<THREADS>(v, w, wT);
+}
+
+// Copies of the FP32 weight gradient the training blocks add into: block b
+// adds into copy b % kGradCopies.  Float atomics are performed memory-side, and
+// adds to one address serialise there; with one set of adds per block (a step
+// of 24576 records runs 192 blocks) 16 copies keep the queues 12 deep (the
+// Adam kernel sums the copies).
+constexpr int kGradCopies = 16;
+
+// Slots of a training step's loss buffer: the training blocks add their
+// partial sums into slots 1.., the Adam kernel leaves their total in slot 0.
+constexpr int kLossSlots = 64;
+
+// Zeroes the gradient copies and the loss slots of a training step.
+__global__ void k_nrc_zero(float* __restrict__ g, float* __restrict__ loss) {
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = i0; i < kGradCopies * kParams; i += gridDim.x * blockDim.x) g[i] = 0.0f;
+    if (i0 < kLossSlots) loss[i0] = 0.0f;
+}
 
 // Adam on the FP32 master weights (torch.optim.Adam semantics) and the BF16
-// copy, on the sum of the gradient copies g[c][i]; the sum is left in copy 0.
+// copy, on the sum of the gradient copies g[c][i]; the sum is left in copy 0,
+// and the loss slots' total in loss[0].
 __global__ void k_nrc_adam(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
-                           __bf16* __restrict__ wg, float lr, float b1, float b2, float eps, float bc1, float bc2_sqrt) {
+                           __bf16* __restrict__ wg, float lr, float b1, float b2, float eps, float bc1, float bc2_sqrt,
+                           float* __restrict__ loss) {
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        float t = threadIdx.x < kLossSlots - 1 ? loss[1 + threadIdx.x] : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+        if (threadIdx.x == 0) loss[0] = t;
+    }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kParams; i += gridDim.x * blockDim.x) {
         float gi = g[i];
 #pragma unroll
@@ -220,11 +287,34 @@ VR_DEV floatx4 mlp_tile(const __bf16* __restrict__ w, const bf16x8* x) {
 
 // Feature fragments of this lane's row: slots 8 g + j of the 3 K-blocks.
 VR_DEV void encode_frags(f3 u, f3 wd, bool valid, bf16x8* x) {
+    // lane group g of K-block ks holds features 8 p .. 8 p + 7 with p = 4 ks + g:
+    // for p < 9 the sin/cos pairs of frequencies 4 (p % 3) .. + 3 of coordinate
+    // p / 3 (the same expressions as feature()), for p = 9 the direction and
+    // the constant 1, zeros above
     const int g = (int)__lane_id() / 16;
 #pragma unroll
-    for (int ks = 0; ks < 3; ks++)
+    for (int ks = 0; ks < 3; ks++) {
+        const int p = 4 * ks + g;
+        if (p < 9) {
+            const int c = p / 3, f0 = 4 * (p - 3 * c);
+            const float uc = c == 0 ? u.x : (c == 1 ? u.y : u.z);
 #pragma unroll
-        for (int j = 0; j < 8; j++) x[ks][j] = (__bf16)(valid ? feature(u, wd, ks * 32 + g * 8 + j) : 0.0f);
+            for (int q = 0; q < 4; q++) {
+                float sn, cs;
+                sincos_turns(fract(uc * __builtin_ldexpf(1.0f, f0 + q - 1)), sn, cs);
+                x[ks][2 * q] = (__bf16)(valid ? sn : 0.0f);
+                x[ks][2 * q + 1] = (__bf16)(valid ? cs : 0.0f);
+            }
+        } else {
+            const bool dir = p == 9 && valid;
+            x[ks][0] = (__bf16)(dir ? wd.x : 0.0f);
+            x[ks][1] = (__bf16)(dir ? wd.y : 0.0f);
+            x[ks][2] = (__bf16)(dir ? wd.z : 0.0f);
+            x[ks][3] = (__bf16)(dir ? 1.0f : 0.0f);
+#pragma unroll
+            for (int j = 4; j < 8; j++) x[ks][j] = (__bf16)0.0f;
+        }
+    }
 }
 
 constexpr int kInferWaves = 8;  // waves per block (the block shares the weights in LDS)
@@ -233,7 +323,7 @@ constexpr int kInferWaves = 8;  // waves per block (the block shares the weights
 __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_mlp_fw(const float* __restrict__ feats, int64_t n,
                                                                  const __bf16* __restrict__ wg, float* __restrict__ out) {
     __shared__ __bf16 w[kWLds];
-    load_weights(wg, w, nullptr);
+    load_weights<64 * kInferWaves>(wg, w, nullptr);
     __syncthreads();
     const int l = __lane_id(), r = l % 16, g = l / 16;
     const int64_t nw = (int64_t)gridDim.x * kInferWaves;
@@ -274,7 +364,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __r
                                                                 int64_t max_queries, f3 bmin, f3 ibsize,
                                                                 const __bf16* __restrict__ wg, float* __restrict__ out) {
     __shared__ __bf16 w[kWLds];
-    load_weights(wg, w, nullptr);
+    load_weights<64 * kInferWaves>(wg, w, nullptr);
     __syncthreads();
     const int64_t count = min((int64_t)*count_p, max_queries);
     const int l = __lane_id(), r = l % 16;
@@ -301,16 +391,14 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __r
 // dX^T = Wt^T dZ^T with the transposed weights as the A operand (read in the
 // K-slot order of dZ's tiles) and dZ packed as the B operand, so dX lands in
 // the output layout and the ReLU masks are lane-local.  The weight gradient
-// dW = dZ^T X reduces over the tile's 16 rows: the wave writes dZ and X to LDS
-// transposed (one row of 16 values per column) and takes 16 x 16 x 16 MFMAs
-// (v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+j], B[4(l/16)+j][l%16]
-// and returns C[4(l/16)+j][l%16], j < 4) whose rows are the 16 rows, adding
-// each 16 x 16 block into the gradient copy (16 consecutive floats per group).
-
-typedef __attribute__((__vector_size__(4 * sizeof(short)))) short short4_t;
+// dW = dZ^T X reduces over the rows of the block's 8 tiles: the waves write dZ
+// and X to LDS transposed and share the 16 x 16 output blocks, each one a chain
+// of 16 x 16 x 32 MFMAs whose K dimension is the block's rows, added into the
+// block's gradient copy (16 consecutive floats per lane group).
 
 constexpr int kTrainWaves = 8;  // waves per block (they share both weight layouts in LDS)
-constexpr int kTrStride = 20;   // LDS row stride (elements) of a transposed column: 16 rows + pad
+constexpr int kBlockRows = kTileRows * kTrainWaves;  // rows of a block's tiles
+constexpr int kBlockStride = kBlockRows + kPad;      // LDS row stride (elements) of a transposed column
 
 // A fragment of a layer's transposed weights wT [K][NIN + kPad] (K-tile kt,
 // dZ block kb): lane (g, i) holds wT[16 kt + i][32 kb + pi(8 g + j)], zeros
@@ -367,45 +455,67 @@ VR_DEV void back_layer(const __bf16* __restrict__ wT, const bf16x8* dz, floatx4*
     }
 }
 
-// dW[n][k] += sum over the tile's rows of dZ[r][n] X[r][k] for a layer of N
+// dW[n][k] += sum over the block's rows of dZ[r][n] X[r][k] for a layer of N
 // outputs (dZ: N/32 or one half-filled B fragment) and K inputs (X: K/32 B
-// fragments, natural slot order when NATURAL, else pi), into gw (row stride K).
-// tz, tx: the wave's LDS scratch for the transposed columns.
+// fragments, natural slot order when NATURAL, else pi), into gw (row stride K),
+// in two parts.  dw_post: every wave writes its tile's dZ and X to LDS
+// transposed (zt [N][kBlockStride], xt [K][kBlockStride]: one row of kBlockRows
+// values per column, the wave's 16 rows at columns 16 w ..).  dw_issue: after a
+// barrier the waves share the (N/16) x (K/16) output blocks, each reduced over
+// all the block's rows by 16 x 16 x 32 MFMAs and added into the gradient copy --
+// one set of atomics per block instead of one per wave.  Only the RGB rows of
+// the output layer (n < 3) carry gradient; its other rows are not added.  The
+// caller alternates two LDS buffers between layers, so the barrier of the next
+// layer's dw_issue also orders this layer's reads before that buffer's reuse.
 template <int N, int K, bool NATURAL>
-VR_DEV void weight_grad(const bf16x8* dz, const bf16x8* x, __bf16* __restrict__ tz, __bf16* __restrict__ tx,
-                        float* __restrict__ gw) {
+VR_DEV void dw_post(const bf16x8* dz, const bf16x8* x, __bf16* __restrict__ zt, __bf16* __restrict__ xt) {
     const int l = __lane_id(), r = l % 16, g = l / 16;
     constexpr int ZB = (N + 31) / 32, XB = K / 32;
+    const int col = 16 * (int)(threadIdx.x / 64) + r;
 #pragma unroll
     for (int kb = 0; kb < ZB; kb++)
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = kb * 32 + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
-            if (n < N) tz[n * kTrStride + r] = dz[kb][j];
+            if (n < N) zt[n * kBlockStride + col] = dz[kb][j];
         }
 #pragma unroll
     for (int kb = 0; kb < XB; kb++)
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int k = kb * 32 + (NATURAL ? 8 * g + j : (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4));
-            tx[k * kTrStride + r] = x[kb][j];
+            xt[k * kBlockStride + col] = x[kb][j];
         }
-    wave_sync();
-    // one output tile row at a time (unrolled, the addresses of all 4 x K/16
-    // blocks' atomics would be held in registers at once)
+}
+
+template <int N, int K>
+VR_DEV void dw_issue(const __bf16* __restrict__ zt, const __bf16* __restrict__ xt, float* __restrict__ gw) {
+    const int l = __lane_id(), r = l % 16, g = l / 16;
+    __syncthreads();
+    constexpr int KT = K / 16, NBLK = (N / 16) * KT;
 #pragma unroll 1
-    for (int nt = 0; nt < N / 16; nt++) {
-        const short4_t a = *(const short4_t*)(tz + (nt * 16 + r) * kTrStride + 4 * g);
-        float* gr = gw + (nt * 16 + 4 * g) * K + r;
+    for (int b = (int)(threadIdx.x / 64); b < NBLK; b += kTrainWaves) {
+        const int nt = b / KT, kt = b - nt * KT;
+        const __bf16* za = zt + (nt * 16 + r) * kBlockStride + 8 * g;
+        const __bf16* xb = xt + (kt * 16 + r) * kBlockStride + 8 * g;
+        floatx4 c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int kt = 0; kt < K / 16; kt++) {
-            const short4_t b = *(const short4_t*)(tx + (kt * 16 + r) * kTrStride + 4 * g);
-            const floatx4 c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, floatx4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+        for (int ks = 0; ks < kBlockRows / 32; ks++)
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*(const bf16x8*)(za + 32 * ks), *(const bf16x8*)(xb + 32 * ks), c, 0, 0, 0);
+        float* gr = gw + (nt * 16 + 4 * g) * K + kt * 16 + r;
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) atomic_addf(gr + jj * K + kt * 16, c[jj]);
-        }
+        for (int jj = 0; jj < 4; jj++)
+            if (N > kOut || 4 * g + jj < 3) atomic_addf(gr + jj * K, c[jj]);
     }
-    wave_sync();
+}
+
+// Feature fragments of this lane's row of the wave's tile in the block tile at
+// bbase (zeros past count).
+VR_DEV void tile_features(const float* __restrict__ records, int64_t count, int64_t bbase, f3 bmin, f3 ibsize, bf16x8* x) {
+    const int64_t row = bbase + (int64_t)(threadIdx.x / 64) * kTileRows + __lane_id() % 16;
+    const bool valid = row < count;
+    const float* rec = records + (valid ? row : 0) * 9;
+    encode_frags(unit_coords(F3(rec[0], rec[1], rec[2]), bmin, ibsize), F3(rec[3], rec[4], rec[5]), valid, x);
 }
 
 // One optimisation step over the training records (x[3], w[3], target[3]) of
@@ -421,23 +531,40 @@ __global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __r
                                                                 float* __restrict__ loss) {
     __shared__ __bf16 w_lds[kWLds];
     __shared__ __bf16 wT_lds[kWTLds];
-    __shared__ __bf16 scratch[kTrainWaves][(kH + kIn) * kTrStride];
-    load_weights(wg, w_lds, wT_lds);
-    __syncthreads();
-    grad += (size_t)((blockIdx.x * kTrainWaves + threadIdx.x / 64) % kGradCopies) * kParams;
+    // two transposition buffers alternated by layer (A: dZ and up to 96
+    // inputs, B: dZ and 64 inputs).  Their pointers carry an offset the
+    // compiler cannot see through, so it keeps the LDS accesses of consecutive
+    // layers in program order (with provably disjoint buffers it interleaves
+    // them and runs out of registers).
+    __shared__ __bf16 tbuf[(kH + kIn + 2 * kH) * kBlockStride];
+    int toff;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(toff));
+    __bf16* ztA = tbuf + toff;
+    __bf16* xtA = ztA + kH * kBlockStride;
+    __bf16* ztB = xtA + kIn * kBlockStride;
+    __bf16* xtB = ztB + kH * kBlockStride;
+    // the weight loads are issued first, staged into LDS after the first
+    // tile's features are computed
+    bf16x8 wreg[kWPer<64 * kTrainWaves>];
+    fetch_weights<64 * kTrainWaves>(wg, wreg);
+    grad += (size_t)(blockIdx.x % kGradCopies) * kParams;
     const int64_t written = (int64_t)*count_p;
     if (written <= 0) return;
     const int64_t f = first % written;
     const int64_t count = min(written - f, max_records);
     records += f * 9;
     const float inv = 1.0f / (3.0f * (float)count);
-    __bf16* tz = scratch[threadIdx.x / 64];
-    __bf16* tx = tz + kH * kTrStride;
     const int l = __lane_id(), r = l % 16, g = l / 16;
     float lsum = 0.0f;
-    const int64_t nw = (int64_t)gridDim.x * kTrainWaves;
-    for (int64_t base = ((int64_t)blockIdx.x * kTrainWaves + threadIdx.x / 64) * kTileRows; base < count;
-         base += nw * kTileRows) {
+    const int64_t first_tile = (int64_t)blockIdx.x * kBlockRows;
+    bf16x8 x[3];
+    tile_features(records, count, first_tile, bmin, ibsize, x);
+    stage_weights<64 * kTrainWaves>(wreg, w_lds, wT_lds);
+    __syncthreads();
+    // block-uniform loop (the weight gradient synchronises the block's waves)
+    for (int64_t bbase = first_tile; bbase < count; bbase += (int64_t)gridDim.x * kBlockRows) {
+        if (bbase != first_tile) tile_features(records, count, bbase, bmin, ibsize, x);
+        const int64_t base = bbase + (threadIdx.x / 64) * kTileRows;
         // the weights are re-read from LDS per tile: an offset the compiler
         // cannot see through keeps it from hoisting every weight fragment of
         // both layouts into registers (~260 VGPRs, spilled)
@@ -448,8 +575,6 @@ __global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __r
         const int64_t row = base + r;
         const bool valid = row < count;
         const float* rec = records + (valid ? row : 0) * 9;
-        bf16x8 x[3];
-        encode_frags(unit_coords(F3(rec[0], rec[1], rec[2]), bmin, ibsize), F3(rec[3], rec[4], rec[5]), valid, x);
         // forward, keeping each layer's input fragments
         floatx4 c[4];
         bf16x8 h1[2], h2[2], h3[2], h4[2];
@@ -495,28 +620,46 @@ __global__ void __launch_bounds__(64 * kTrainWaves) k_nrc_train(const float* __r
         bf16x8 dz[2];
         dz[0] = pack2(dy, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
         floatx4 dx[4];
-        // output layer (16 x 64), then the hidden layers
+        // output layer (16 x 64), then the hidden layers; each layer's dX is
+        // formed between posting its transposed tiles and the barrier
         __builtin_amdgcn_sched_barrier(0);
-        weight_grad<kOut, kH, false>(dz, h4, tz, tx, grad + kOff4);
+        dw_post<kOut, kH, false>(dz, h4, ztA, xtA);
         back_layer<kOut>(wT + kT4, dz, dx);
+        dw_issue<kOut, kH>(ztA, xtA, grad + kOff4);
         mask_pack(dx, h4, dz);
         __builtin_amdgcn_sched_barrier(0);
-        weight_grad<kH, kH, false>(dz, h3, tz, tx, grad + kOff3);
+        dw_post<kH, kH, false>(dz, h3, ztB, xtB);
         back_layer<kH>(wT + kT3, dz, dx);
+        dw_issue<kH, kH>(ztB, xtB, grad + kOff3);
         mask_pack(dx, h3, dz);
         __builtin_amdgcn_sched_barrier(0);
-        weight_grad<kH, kH, false>(dz, h2, tz, tx, grad + kOff2);
+        dw_post<kH, kH, false>(dz, h2, ztA, xtA);
         back_layer<kH>(wT + kT2, dz, dx);
+        dw_issue<kH, kH>(ztA, xtA, grad + kOff2);
         mask_pack(dx, h2, dz);
         __builtin_amdgcn_sched_barrier(0);
-        weight_grad<kH, kH, false>(dz, h1, tz, tx, grad + kOff1);
+        dw_post<kH, kH, false>(dz, h1, ztB, xtB);
         back_layer<kH>(wT + kT1, dz, dx);
+        dw_issue<kH, kH>(ztB, xtB, grad + kOff1);
         mask_pack(dx, h1, dz);
         __builtin_amdgcn_sched_barrier(0);
-        weight_grad<kH, kIn, true>(dz, x, tz, tx, grad + kOff0);
+        dw_post<kH, kIn, true>(dz, x, ztA, xtA);
+        dw_issue<kH, kIn>(ztA, xtA, grad + kOff0);
+        // buffer A is written again by the next tile's output layer
+        __syncthreads();
     }
+    // the loss: one partial per block into slot 1 + blockIdx % (kLossSlots - 1)
+    // (float atomics on one address are serialised memory-side)
     for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-    if (l == 0) atomic_addf(loss, lsum);
+    __shared__ float wsum[kTrainWaves];
+    if (l == 0) wsum[threadIdx.x / 64] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kTrainWaves; i++) t += wsum[i];
+        atomic_addf(loss + 1 + blockIdx.x % (kLossSlots - 1), t);
+    }
 }
 
 // ---------------------------------------------------------------- paths --

@@ -55,7 +55,9 @@ class NeuralRadianceCache:
         self.grad = self.grad_copies[:L.NRC_PARAMS]
         self.m = torch.zeros_like(self.params)
         self.v = torch.zeros_like(self.params)
-        self.loss = torch.zeros(1, device=device)
+        # a training step's loss buffer; its mean loss lands in the first slot
+        self._loss_slots = torch.zeros(L.NRC_LOSS_SLOTS, device=device)
+        self.loss = self._loss_slots[:1]
         self.lr, self.betas, self.eps = lr, betas, eps
         self.terminate_after = int(terminate_after)
         self.steps = 0

@@ -174,6 +174,47 @@ def test_zero_cache_render_is_the_truncated_nee_path(gpu, tkind):
     assert match_fraction(out.cpu().numpy(), ref, rtol=1e-3, atol=1e-4) >= 0.98
 
 
+def test_fused_cache_inference_equals_encoding_then_mlp(gpu):
+    """The render's fused inference (features formed per lane in registers, the
+    MLP tile, the weighted add per pixel) equals the explicit encoder followed
+    by the MLP kernel on the same queries: the cached render minus the Q = 0
+    render of the same samples is sum W relu(Q(x, w)) per pixel (same BF16
+    inputs and MLP; only the order of the float adds differs)."""
+    K = 2
+    sc = Scene(sigma=fbm_grid(24, seed=9, scale=20.0), albedo_c=(0.9, 0.8, 0.7), phase_kind=L.VR_PHASE_HG,
+               phase_g=0.6, env_c=(0.5, 0.6, 0.7), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(3.0, 3.0, 3.0), sun_prob=0.5,
+               majorant="grid", max_bounces=8)
+    b = Built(sc, "cuda")
+    cam = Camera(device="cuda", poses=pose((0.0, 0.3, 2.5))[None], width=40, height=40, jittered=True)
+    wz = torch.zeros(L.NRC_PARAMS, dtype=torch.bfloat16, device="cuda")
+    wr = _random_weights(4).weights
+    res = []
+    for w in (wz, wr):
+        out = torch.zeros(1600, 3, device="cuda")
+        q = torch.zeros(3200, 10, device="cuda")
+        qc = torch.zeros(1, dtype=torch.int32, device="cuda")
+        L.check(L.load().vr_nrc_render(b.medium, cam.c, L.launch((7, 1, 3, 5), samples=2), K, w.data_ptr(),
+                                       out.data_ptr(), q.data_ptr(), 3200, qc.data_ptr(), None), "vr_nrc_render")
+        torch.cuda.synchronize()
+        res.append((out, q[:int(qc.item())].clone()))
+    (o0, _), (o1, q) = res
+    n = q.shape[0]
+    assert n > 200
+    x = q[:, 0:3].contiguous()
+    w = q[:, 3:6].contiguous()
+    feats = torch.empty(n, 96, device="cuda")
+    L.check(L.load().vr_nrc_encode(x.data_ptr(), w.data_ptr(), n, *_box_ptrs(), feats.data_ptr(), None), "encode")
+    y = torch.empty(n, 3, device="cuda")
+    L.check(L.load().vr_nrc_mlp_fw(feats.data_ptr(), n, wr.data_ptr(), y.data_ptr(), None), "mlp_fw")
+    torch.cuda.synchronize()
+    pix = q[:, 9].cpu().numpy().view(np.uint32).astype(np.int64)
+    add = (q[:, 6:9] * torch.relu(y)).cpu().double().numpy()
+    expect = o0.cpu().double().numpy().copy()
+    np.add.at(expect, pix, add)
+    np.testing.assert_allclose(o1.cpu().double().numpy(), expect, rtol=1e-5, atol=1e-7)
+    assert np.abs(add).max() > 1e-4  # the cache term is far above the tolerance
+
+
 @pytest.mark.parametrize("tkind,cells", [(L.VR_T_DT, True), (L.VR_T_DT, False), (L.VR_T_RT, False)])
 def test_persistent_cached_render_equals_the_per_thread_one(gpu, tkind, cells):
     """The cached render as the persistent lane state machine (the default for
