@@ -171,7 +171,8 @@ typedef struct vr_medium {
     int32_t env_quadtree_levels;   /* 1 .. 15 */
     int32_t env_reserved;
     const float* sigma_cells;      /* optional cell table of sigma (vr_grid_pack_cells); NULL = read
-                                      the grid.  Lookups give bit-identical values either way. */
+                                      the grid.  Lookups give bit-identical values either way.
+                                      Requires (nx - 1) * (ny - 1) < 2^24 (VR_ERR_ARG otherwise). */
     int32_t phase_g_kind;          /* vr_field_kind: VR_FIELD_CONST = phase_g everywhere;
                                       VR_FIELD_GRID = phase_g_grid read at each scattering
                                       position (HGPhase / VHGPhaseSampler evaluate their phase_g

@@ -200,6 +200,8 @@ int make_medium_view(const vr_medium* m, int kind, int bw_mode, MediumView& v) {
     if (m->sigma_cells && kind != VR_MAP_RAYBOX) {
         if (reinterpret_cast<uintptr_t>(m->sigma_cells) % 16 != 0)
             return set_err(VR_ERR_ARG, "sigma cell table must be 16-byte aligned");
+        if ((int64_t)(v.sigma.nx - 1) * (v.sigma.ny - 1) >= (int64_t(1) << 24))
+            return set_err(VR_ERR_ARG, "sigma cell table: an xy plane of the grid holds 2^24 cells or more");
         v.cells = reinterpret_cast<const float4*>(m->sigma_cells);
     }
     if (bw_mode != VR_BW_REFERENCE && bw_mode != VR_BW_CONSISTENT) return set_err(VR_ERR_ARG, "unknown bw mode");

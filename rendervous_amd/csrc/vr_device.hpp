@@ -248,7 +248,11 @@ VR_DEV float cells_lookup1(const GridView& g, const float4* __restrict__ cells, 
     float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
     float ax = gx - fx, ay = gy - fy, az = gz - fz;
     int px = min((int)fx, g.nx - 2), py = min((int)fy, g.ny - 2), pz = min((int)fz, g.nz - 2);
-    const float4* q = cells + 2 * ((pz * (g.ny - 1) + py) * (g.nx - 1) + px);
+    // cell index by 24-bit multiplies (full rate; the host admits tables whose
+    // xy planes hold fewer than 2^24 cells)
+    const uint32_t cell = __umul24((uint32_t)pz, (uint32_t)((g.ny - 1) * (g.nx - 1))) +
+                          __umul24((uint32_t)py, (uint32_t)(g.nx - 1)) + (uint32_t)px;
+    const float4* q = cells + 2 * (size_t)cell;
     const float4 lo = q[0], hi = q[1];
     float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
     float r = lo.x * (bx * by * bz);
@@ -593,15 +597,25 @@ struct Walk<VR_MAJ_CONST> {
     VR_DEV void cross(const MediumView&, f3) {}
 };
 
+// The coarse majorant grid staged in LDS: an internal majorant kind for
+// kernels that copy a grid of at most kMajLdsCells cells into maj_lds at their
+// start (the walk then reads LDS instead of global memory).
+constexpr int kMajGridLds = 2;
+constexpr int kMajLdsCells = 4096;
+__shared__ float maj_lds[kMajLdsCells];
+
 // Coarse max-grid DDA: value = bound of the current coarse cell, distance = to
-// that cell's exit along the ray.  Per-lane state is the packed cell index and
-// the ray parameters of the next x/y/z faces and of the tracked point; the
-// per-axis increments are recomputed from the ray direction at each crossing.
-template <>
-struct Walk<VR_MAJ_GRID> {
+// that cell's exit along the ray.  Per-lane state is the packed cell index, the
+// cell's bound (read when the walk enters the cell, so a query is register
+// arithmetic only) and the ray parameters of the next x/y/z faces and of the
+// tracked point; the per-axis increments are recomputed from the ray direction
+// at each crossing.  LDS: the grid is read from maj_lds.
+template <bool LDS>
+struct WalkGrid {
     static constexpr bool grid = true;
     int cell;                // cx | cy << 10 | cz << 20; -1 once the ray has left the grid
     float tnx, tny, tnz, t;  // ray parameter of the next x/y/z face and of the tracked point
+    float mval;              // bound of the current cell
 
     static VR_DEV int axis_cell(float xa, float bmin, float ics, int res) {
         int c = (int)floorf((xa - bmin) * ics);
@@ -612,6 +626,12 @@ struct Walk<VR_MAJ_GRID> {
         if (wa < 0.0f) return (bmin + (float)c * cs - xa) * rcp(wa);
         return kInf;
     }
+    // bound of the packed cell c (>= 0)
+    static VR_DEV float fetch(const MediumView& M, int c) {
+        const int cx = c & 1023, cy = (c >> 10) & 1023, cz = c >> 20;
+        const int i = (cz * M.mry + cy) * M.mrx + cx;
+        return LDS ? maj_lds[i] : M.maj_grid[i];
+    }
     VR_DEV void begin(const MediumView& M, f3 x, f3 w) {
         int cx = axis_cell(x.x, M.maj_bmin.x, M.maj_icsize.x, M.mrx);
         int cy = axis_cell(x.y, M.maj_bmin.y, M.maj_icsize.y, M.mry);
@@ -621,12 +641,12 @@ struct Walk<VR_MAJ_GRID> {
         tnz = axis_face(x.z, w.z, M.maj_bmin.z, M.maj_csize.z, cz);
         t = 0.0f;
         cell = cx | (cy << 10) | (cz << 20);
+        mval = fetch(M, cell);
     }
-    VR_DEV float query(const MediumView& M, float& md) const {
+    VR_DEV float query(const MediumView&, float& md) const {
         if (cell < 0) { md = kInf; return 0.0f; }
         md = fmaxf(0.0f, fminf(tnx, fminf(tny, tnz)) - t);
-        int cx = cell & 1023, cy = (cell >> 10) & 1023, cz = cell >> 20;
-        return M.maj_grid[(cz * M.mry + cy) * M.mrx + cx];
+        return mval;
     }
     VR_DEV void move(float dt) { t += dt; }
     VR_DEV void cross(const MediumView& M, f3 w) {
@@ -649,26 +669,15 @@ struct Walk<VR_MAJ_GRID> {
             if (cz < 0 || cz >= M.mrz) { cell = -1; return; }
         }
         cell = cx | (cy << 10) | (cz << 20);
+        mval = fetch(M, cell);
     }
 };
-
-// The coarse majorant grid staged in LDS: an internal majorant kind for
-// kernels that copy a grid of at most kMajLdsCells cells into maj_lds at their
-// start (majorant queries then read LDS instead of global memory); the walk is
-// that of VR_MAJ_GRID.
-constexpr int kMajGridLds = 2;
-constexpr int kMajLdsCells = 4096;
-__shared__ float maj_lds[kMajLdsCells];
 
 template <>
-struct Walk<kMajGridLds> : Walk<VR_MAJ_GRID> {
-    VR_DEV float query(const MediumView& M, float& md) const {
-        if (cell < 0) { md = kInf; return 0.0f; }
-        md = fmaxf(0.0f, fminf(tnx, fminf(tny, tnz)) - t);
-        int cx = cell & 1023, cy = (cell >> 10) & 1023, cz = cell >> 20;
-        return maj_lds[(cz * M.mry + cy) * M.mrx + cx];
-    }
-};
+struct Walk<VR_MAJ_GRID> : WalkGrid<false> {};
+
+template <>
+struct Walk<kMajGridLds> : WalkGrid<true> {};
 
 // Copies the medium's majorant grid into maj_lds (block-wide; the caller has
 // checked that it fits).

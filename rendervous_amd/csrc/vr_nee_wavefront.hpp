@@ -83,13 +83,17 @@ struct NeeLane {
     // distance already in rd when bnd == 0).
     VR_DEV void start_ray(const MediumView& M) {
         event = kEvNone;
+        // set for every ray (only read after kEvTrackEnd, which each ray end
+        // sets with its own value): a store in the miss branch alone would be
+        // merged with the walk's stores into one through a pointer select,
+        // which keeps both fields in scratch memory
+        track_T = 1.0f;
         if (bnd) {
             float tmin, tmax;
             bool hit = boundary(M, rx, rw, tmin, tmax);
             if (mode == kModeTrack) {
                 if (!hit) {
                     event = kEvTrackEnd;
-                    track_T = 1.0f;
                     return;
                 }
                 rx = rx + rw * tmin;
@@ -362,11 +366,13 @@ VR_DEV bool nee_item(const NeeSchedule& S, const CameraView& cam, int q, uint32_
     return true;
 }
 
-// Resident waves per SIMD the persistent kernel is compiled for (caps its
-// registers: 3 waves leave 168 VGPRs per lane).
-#ifndef VR_NEE_WAVES_PER_EU
-#define VR_NEE_WAVES_PER_EU 3
-#endif
+// Resident waves per SIMD the persistent kernel is compiled for, which caps its
+// registers: 4 waves (128 VGPRs) for the constant-environment forms with DT
+// shadow rays, 3 (168 VGPRs) for the larger forms.
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN>
+constexpr int nee_waves_per_eu() {
+    return (!ENVMAP && TKIND == VR_T_DT && MAJ == VR_MAJ_GRID) ? 4 : 3;
+}
 
 // Item indices a wave takes from a queue counter per atomic.
 constexpr int kPool = 64;
@@ -412,7 +418,7 @@ VR_DEV const NeeArgs& nee_args() { return *(const NeeArgs*)reread(first_kernel_a
 // collision, which becomes a query of the cache (the medium's max_bounces must
 // be -1: the cached estimator has no other cap), as nrc::nee_cached.
 template <int MAJ, int TKIND, bool ENVMAP, bool LEAN, bool CACHE = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VR_NEE_WAVES_PER_EU)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(nee_waves_per_eu<MAJ, TKIND, ENVMAP, LEAN>())))
 k_nee_persistent(NeeArgs args) {
     if constexpr (MAJ == kMajGridLds) stage_majorant(args.M);
     const NeeSchedule& S = args.S;

@@ -468,8 +468,9 @@ class _Lowered:
         ct = _internal.settings.cell_tables
         g = sigma.grid
         n_cells = (g.shape[0] - 1) * (g.shape[1] - 1) * (g.shape[2] - 1)
+        plane = (g.shape[1] - 1) * (g.shape[2] - 1)
         auto = not g.requires_grad or multiple_scattering
-        if g.is_cuda and n_cells < (1 << 30) and (ct is True or (ct == "auto" and auto)):
+        if g.is_cuda and n_cells < (1 << 30) and plane < (1 << 24) and (ct is True or (ct == "auto" and auto)):
             table = sigma.cells_tensor()
             self.keep.append(table)
             self.medium.sigma_cells = table.data_ptr()

@@ -155,6 +155,22 @@ def test_bad_medium_majorant_is_rejected():
     assert st == L.VR_ERR_ARG and b"medium majorant" in lib.vr_last_error()
 
 
+def test_cell_table_plane_limit_is_enforced():
+    """Cell-table lookups index with 24-bit multiplies: a table is refused for a
+    grid whose xy planes hold 2^24 cells or more (the grid is only described,
+    never read)."""
+    lib = L.load()
+    b, cam, out = _nee_args()
+    table = np.zeros(64, np.float32)
+    b.medium.sigma_cells = O.nptr(table) if O.nptr(table) % 16 == 0 else O.nptr(table[2:])
+    t = b.medium.sigma.t
+    for i, (n, st_) in enumerate(zip((2, 4097, 4097, 1), (4097 * 4097, 4097, 1, 1))):
+        t.shape[i] = n
+        t.stride[i] = st_
+    st = lib.vr_capture_fw(b.medium, L.VR_MAP_NEE_DT, cam.c, L.launch((1, 2, 3, 4), samples=1), O.nptr(out), None)
+    assert st == L.VR_ERR_ARG and b"cell table" in lib.vr_last_error()
+
+
 def test_sample_indices_beyond_32_bits_are_rejected():
     """Sample s of sensor i draws the stream of flat index s * sensors + i (32
     bits): 64 x 2^26 = 2^32 samples is the largest capture, one more sensor's
