@@ -59,6 +59,11 @@ struct NeeLane {
         rng = seed_capture_fw(seeds, seed_index);
         f3 x, w;
         camera_ray(cam, pose, row, col, rng, x, w);
+        begin_path(M, x, w);
+    }
+
+    // A path along the ray (x, w), the generator already seeded.
+    VR_DEV void begin_path(const MediumView& M, f3 x, f3 w) {
         float tmin, tmax;
         boundary(M, x, w, tmin, tmax);
         px = x + w * tmin;

@@ -15,7 +15,7 @@ import torch
 
 from helpers import (L, O, Scene, blob_grid, fbm_grid, gpu_capture_bw, gpu_capture_fw, gpu_map_bw, gpu_map_fw,
                      match_fraction, np_grid, oracle_capture_bw, oracle_capture_fw, oracle_map_bw, oracle_map_fw,
-                     pose, random_rays, rel_l1)
+                     pose, random_rays, rel_l1, SEED_MATCHED_L1)
 
 pytestmark = pytest.mark.gpu
 
@@ -212,7 +212,7 @@ def test_map_backward_seed_matched(gpu, kind, bw_mode):
     og = np.random.default_rng(18).uniform(0.5, 1.5, size=(2048, od)).astype(np.float32)
     ref, _ = oracle_map_bw(sc, kind, rays, og, SEEDS, bw_mode=bw_mode)
     out, _ = gpu_map_bw(sc, kind, rays, og, SEEDS, bw_mode=bw_mode)
-    assert rel_l1(out, ref) <= 0.03
+    assert rel_l1(out, ref) <= SEED_MATCHED_L1
 
 
 @pytest.mark.parametrize("tkind", [L.VR_T_DT, L.VR_T_RT])
@@ -226,7 +226,10 @@ def test_path_integrators_backward_grid_majorant(gpu, kind, tkind):
     og = np.random.default_rng(28).uniform(0.5, 1.5, size=(2048, 3)).astype(np.float32)
     ref, _ = oracle_map_bw(sc, kind, rays, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT)
     out, _ = gpu_map_bw(sc, kind, rays, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT)
-    assert rel_l1(out, ref) <= 0.03
+    # DRT with ratio-tracked shadow rays: the roulette test u < 1 - T flips with
+    # the rounding of T on a few samples (measured 5e-4)
+    tol = 5e-3 if (kind, tkind) == (L.VR_MAP_NEE_DRT, L.VR_T_RT) else SEED_MATCHED_L1
+    assert rel_l1(out, ref) <= tol
 
 
 def test_medium_dt_backward_seed_matched(gpu):
@@ -239,8 +242,8 @@ def test_medium_dt_backward_seed_matched(gpu):
     og = np.random.default_rng(22).uniform(0.5, 1.5, size=(2048, 12)).astype(np.float32)
     ref_s, ref_a = oracle_map_bw(sc, L.VR_MAP_MEDIUM_DT, seg, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT, grad_albedo=True)
     out_s, out_a = gpu_map_bw(sc, L.VR_MAP_MEDIUM_DT, seg, og, SEEDS, bw_mode=L.VR_BW_CONSISTENT, grad_albedo=True)
-    assert rel_l1(out_s, ref_s) <= 0.03
-    assert rel_l1(out_a, ref_a) <= 0.03
+    assert rel_l1(out_s, ref_s) <= SEED_MATCHED_L1
+    assert rel_l1(out_a, ref_a) <= SEED_MATCHED_L1
 
 
 @pytest.mark.parametrize("kind", [L.VR_MAP_MC_COLLISION, L.VR_MAP_NEE_DT, L.VR_MAP_T_DDA, L.VR_MAP_PATH_DT,
@@ -255,9 +258,9 @@ def test_capture_backward_seed_matched(gpu, kind):
     og = np.random.default_rng(24).uniform(0.5, 1.5, size=(24 * 24, od)).astype(np.float32)
     ref_s, ref_a = oracle_capture_bw(sc, kind, cam, SEEDS, 2, og, grad_albedo=True)
     out_s, out_a = gpu_capture_bw(sc, kind, cam, SEEDS, 2, og, grad_albedo=True)
-    assert rel_l1(out_s, ref_s) <= 0.03
+    assert rel_l1(out_s, ref_s) <= SEED_MATCHED_L1
     if kind not in (L.VR_MAP_T_DDA, L.VR_MAP_PATH_DT, L.VR_MAP_NEE_DRT):  # no albedo gradient in those estimators
-        assert rel_l1(out_a, ref_a) <= 0.03
+        assert rel_l1(out_a, ref_a) <= SEED_MATCHED_L1
 
 
 # ---------------------------------------------------------------- Adam --
