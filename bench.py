@@ -668,8 +668,8 @@ def time_to_rmse_leg(dev, radiance, camera, ref_spp=4096, rel_eps=0.05):
             "ladder": rows}
 
 
-def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32, traces=8, steps_per_trace=8,
-            batch=24576, lr=(1e-2, 1e-3), n_paths=65536, full_ladder=False):
+def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32, traces=2, steps_per_trace=32,
+            batch=24576, lr=(1e-2, 1e-3), n_paths=65536, full_ladder=False, terminate_after=2):
     """Neural radiance cache vs plain NEE on the 512^3 cloud (fBm x60, albedo
     0.99, HG g=0.8, sun + env 0.1), 1024x1024: epsilon = the RMSE of the plain
     renderer at 256 spp against a 4096-spp plain render; the cache is trained
@@ -720,7 +720,7 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32,
         rv.seed(7)
         img, plain_ms = timed(lambda: camera.capture(field, fw_samples=target_spp))
         eps = rmse(img)
-        cache = NeuralRadianceCache(dev, seed=7, lr=lr[0])
+        cache = NeuralRadianceCache(dev, seed=7, lr=lr[0], terminate_after=terminate_after)
         n_steps = traces * steps_per_trace
         rates = [lr[0] * (lr[1] / lr[0]) ** (k / max(1, n_steps - 1)) for k in range(n_steps)]
         cache.render(field, camera, spp=32)  # untimed: sizes the query buffer once
@@ -761,7 +761,7 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32,
         "workload": f"time to RMSE <= plain {target_spp} spp RMSE, {res}^3 cloud fBm x60, albedo 0.99, g 0.8, "
                     f"{hw}x{hw}, max {max_bounces} bounces, vs a {ref_spp}-spp plain render; cache trained on "
                     f"{traces} traces of {n_paths} NEE training paths (<= 32 records each), {steps_per_trace} Adam "
-                    f"steps of {batch} records per trace; cached render ends paths after 2 bounces",
+                    f"steps of {batch} records per trace; cached render ends paths after {terminate_after} bounces",
         "epsilon": eps, "plain_ms": plain_ms, "plain_spp": target_spp,
         "nrc_train_ms": train_ms, "nrc_render": hit, "nrc_ms": nrc_ms,
         "speedup": plain_ms / nrc_ms if nrc_ms else None, "ladder": ladder,
