@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # counter reads for a 256 MiB device copy (FETCH_SIZE reads 128 MiB for it,
 # WRITE_SIZE 256 MiB: factors 2 and 1).  The raw figures and the commands are in
 # profiles/nee_persistent_pmc_256cube_512x512x64spp.txt.
-NEE_PROFILED = {"workload": (256, 512, 512, 64), "fetch_kib": 5.405484e6, "write_kib": 7.218503e5,
+NEE_PROFILED = {"workload": (256, 512, 512, 64), "fetch_kib": 5.4066e6, "write_kib": 7.2189e5,
                 "fetch_scale": 2.0, "write_scale": 1.0}
 
 
