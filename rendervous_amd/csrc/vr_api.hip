@@ -838,7 +838,7 @@ struct NeeCache {
     int k = 0;
 };
 
-template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false, bool CACHE = false>
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN = false, bool CACHE = false, bool CMP = false>
 int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, int spp, int64_t total, int64_t start,
                           int64_t cnt, float* out, int* dbg, unsigned long long* counter, hipStream_t st,
                           const NeeCache& cache = NeeCache()) {
@@ -846,7 +846,7 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
     if (!counter) return set_err(VR_ERR_ARG, "persistent NEE capture without work counters");
     int rc = VR_OK;
     int blocks = 0;
-    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN, CACHE>, &blocks))) return rc;
+    if ((rc = persistent_blocks(k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN, CACHE, CMP>, &blocks))) return rc;
     const int64_t lanes = (int64_t)blocks * kBlock;
     const int64_t hw = (int64_t)C.width * C.height;
     // launches of at most 2^30 sensors (whole images when possible) keep item indices in 32 bits
@@ -888,7 +888,7 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         hipLaunchKernelGGL(k_nee_prepare, dim3(prep > 0 ? prep : 1), dim3(kBlock), 0, st, out, b0, n, dbg, counter);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw prepare"))) return rc;
         const NeeArgs args{M, C, sd, S, counter, out, dbg, cache.queries, cache.max_queries, cache.qcount, cache.k};
-        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN, CACHE>), dim3(blocks), dim3(kBlock), 0, st, args);
+        hipLaunchKernelGGL((k_nee_persistent<MAJ, TKIND, ENVMAP, LEAN, CACHE, CMP>), dim3(blocks), dim3(kBlock), 0, st, args);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_fw launch"))) return rc;
     }
     return VR_OK;
@@ -988,6 +988,13 @@ int launch_capture_fw_maj(int kind, const MediumView& M, const CameraView& C, Se
                 return launch_nee_persistent<kMajGridLds, VR_T_RT, false>(VR_NEE_ARGS);
             }
         }
+        // VR_NEE_COMPACT=1: light and phase samples computed block-wide (see
+        // k_nee_persistent's CMP form).  Off by default: the headline frame
+        // takes 4.49-4.50 ms with it against 4.44-4.47 ms without (the handlers'
+        // saved VALU issues are spent on the exchange and the block's waves
+        // waiting for each other at its barriers).
+        if (M.t_kind == VR_T_DT && lean && !envmap && env_int("VR_NEE_COMPACT", 0) != 0)
+            return launch_nee_persistent<MAJ, VR_T_DT, false, true, false, true>(VR_NEE_ARGS);
         if (M.t_kind == VR_T_DT && lean)
             return envmap ? launch_nee_persistent<MAJ, VR_T_DT, true, true>(VR_NEE_ARGS)
                           : launch_nee_persistent<MAJ, VR_T_DT, false, true>(VR_NEE_ARGS);

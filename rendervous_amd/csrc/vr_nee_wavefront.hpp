@@ -244,6 +244,17 @@ struct NeeLane {
     // Real collision of the path (vr_path_integrator_NEE.h:39-52): emission,
     // albedo, light sample; the shadow ray starts at the collision.
     VR_DEV void on_collision(const MediumView& M) {
+        collision_terms(M);
+        f3 wnee, env;
+        float ph;
+        light_sample(M, rng, LEAN ? M.g : phase_g_at(M, px), pw, wnee, env, ph);
+        collision_light(wnee, env, ph);
+    }
+
+    // The parts of on_collision: the collision's own terms, the light sample
+    // (direction, environment and phase value; the generator's only draws) and
+    // the shadow ray it starts.
+    VR_DEV void collision_terms(const MediumView& M) {
         f3 sa;
         if (LEAN || !M.albedo_grid) sa = M.albedo_c;
         else {
@@ -253,11 +264,15 @@ struct NeeLane {
         }
         R = R + (W * (F3(1.0f, 1.0f, 1.0f) - sa)) * M.emission_c;
         W = W * sa;
-        f3 wnee;
-        f3 env = ENVMAP ? env_sample(M, rng, wnee) : env_sample_const(M, rng, wnee);
         px = rx;
         pw = rw;
-        contrib = (W * env) * phase_eval_g(M, LEAN ? M.g : phase_g_at(M, px), rw, wnee);
+    }
+    static VR_DEV void light_sample(const MediumView& M, Rng& g, float pg, f3 wi, f3& wnee, f3& env, float& ph) {
+        env = ENVMAP ? env_sample(M, g, wnee) : env_sample_const(M, g, wnee);
+        ph = phase_eval_g(M, pg, wi, wnee);
+    }
+    VR_DEV void collision_light(f3 wnee, f3 env, float ph) {
+        contrib = (W * env) * ph;
         rw = wnee;
         mode = kModeTrack;
         bnd = 1;
@@ -267,6 +282,16 @@ struct NeeLane {
     // A transmittance ray ended: add its term, then start the path (after the
     // primary term) or continue it after the NEE estimate (vr_path_integrator_NEE.h:53-62).
     VR_DEV void on_track_end(const MediumView& M) {
+        if (!track_end_terms(M)) return;
+        f3 wo;
+        const float rho = phase_sample_g(M, rng, LEAN ? M.g : phase_g_at(M, px), pw, wo);
+        track_end_bounce(wo, rho);
+    }
+
+    // The parts of on_track_end: the ray's term and the bounce bookkeeping
+    // (true: the path continues and needs a phase-function sample at px along
+    // pw), then the scattered direction and weight.
+    VR_DEV bool track_end_terms(const MediumView& M) {
         R = R + contrib * track_T;
         rx = px;
         mode = kModePath;
@@ -276,14 +301,17 @@ struct NeeLane {
             rw = pw;
             rd = pd;
             bnd = 0;
-            return;
+            return false;
         }
         bounces++;
-        if (M.max_bounces >= 0 && bounces >= M.max_bounces) { event = kEvPathEnd; return; }
-        f3 wo;
-        float rho = phase_sample_g(M, rng, LEAN ? M.g : phase_g_at(M, px), pw, wo);
+        if (M.max_bounces >= 0 && bounces >= M.max_bounces) { event = kEvPathEnd; return false; }
+        return true;
+    }
+    VR_DEV void track_end_bounce(f3 wo, float rho) {
         W = W * rho;
-        if (W.x < EPS && W.y < EPS && W.z < EPS) { event = kEvPathEnd; return; }
+        if (W.x < EPS && W.y < EPS && W.z < EPS) event = kEvPathEnd;
+        // stored on both branches (unused once the path ends): stores on one
+        // branch only would be merged through a pointer select into scratch
         rw = wo;
         bnd = 1;
     }
@@ -419,12 +447,104 @@ struct NeeArgs {
 // loads of the fields each handler reads stay in that handler.
 VR_DEV const NeeArgs& nee_args() { return *(const NeeArgs*)reread(first_kernel_arg<NeeArgs>()); }
 
+// Collision and end-of-transmittance handlers with their samples computed
+// block-wide: each lane that needs a light sample (collision) or a phase
+// sample (continuing path) posts its generator state and incoming direction to
+// LDS at a slot ranked over the block (light samples first), the block's first
+// total / 64 waves compute one slot per lane, and the posting lanes read back
+// the generator state and the sample.  Every lane draws exactly what it would
+// draw in the handler itself.  Called by all threads of the block.
+constexpr int kCmpWaves = 4;  // waves per block of the persistent kernel (256 threads)
+template <typename Lane>
+VR_DEV void nee_compacted_samples(Lane& lane, bool have) {
+    __shared__ float xb[11][64 * kCmpWaves];
+    __shared__ int cnt[2][kCmpWaves];
+    const MediumView& M = nee_args().M;
+    const bool rq0 = have && lane.event == kEvCollision;
+    if (rq0) lane.collision_terms(M);
+    bool rq1 = false;
+    if (have && lane.event == kEvTrackEnd) rq1 = lane.track_end_terms(M);
+    const unsigned long long m0 = __ballot(rq0), m1 = __ballot(rq1);
+    const int wv = (int)(threadIdx.x / 64);
+    if (__lane_id() == 0) {
+        cnt[0][wv] = __popcll(m0);
+        cnt[1][wv] = __popcll(m1);
+    }
+    __syncthreads();
+    int n0 = 0, n1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+    for (int v = 0; v < kCmpWaves; v++) {
+        const int c0 = cnt[0][v], c1 = cnt[1][v];
+        if (v < wv) {
+            b0 += c0;
+            b1 += c1;
+        }
+        n0 += c0;
+        n1 += c1;
+    }
+    const int total = n0 + n1;
+    if (total == 0) return;  // the same for the whole block
+    const unsigned long long lt = lane_mask_lt();
+    const int slot = rq0 ? b0 + __popcll(m0 & lt) : n0 + b1 + __popcll(m1 & lt);
+    if (rq0 || rq1) {
+        xb[0][slot] = __uint_as_float(lane.rng.x);
+        xb[1][slot] = __uint_as_float(lane.rng.y);
+        xb[2][slot] = __uint_as_float(lane.rng.z);
+        xb[3][slot] = __uint_as_float(lane.rng.w);
+        xb[4][slot] = lane.pw.x;
+        xb[5][slot] = lane.pw.y;
+        xb[6][slot] = lane.pw.z;
+    }
+    __syncthreads();
+    const int s = (int)threadIdx.x;
+    if (s < total) {
+        Rng g;
+        g.x = __float_as_uint(xb[0][s]);
+        g.y = __float_as_uint(xb[1][s]);
+        g.z = __float_as_uint(xb[2][s]);
+        g.w = __float_as_uint(xb[3][s]);
+        const f3 wi = F3(xb[4][s], xb[5][s], xb[6][s]);
+        f3 d, e = F3(0.0f, 0.0f, 0.0f);
+        float v;
+        if (s < n0) Lane::light_sample(M, g, M.g, wi, d, e, v);
+        else v = phase_sample_g(M, g, M.g, wi, d);
+        xb[0][s] = __uint_as_float(g.x);
+        xb[1][s] = __uint_as_float(g.y);
+        xb[2][s] = __uint_as_float(g.z);
+        xb[3][s] = __uint_as_float(g.w);
+        xb[4][s] = d.x;
+        xb[5][s] = d.y;
+        xb[6][s] = d.z;
+        xb[7][s] = v;
+        xb[8][s] = e.x;
+        xb[9][s] = e.y;
+        xb[10][s] = e.z;
+    }
+    __syncthreads();
+    if (rq0 || rq1) {
+        lane.rng.x = __float_as_uint(xb[0][slot]);
+        lane.rng.y = __float_as_uint(xb[1][slot]);
+        lane.rng.z = __float_as_uint(xb[2][slot]);
+        lane.rng.w = __float_as_uint(xb[3][slot]);
+        const f3 d = F3(xb[4][slot], xb[5][slot], xb[6][slot]);
+        const float v = xb[7][slot];
+        if (rq0) lane.collision_light(d, F3(xb[8][slot], xb[9][slot], xb[10][slot]), v);
+        else lane.track_end_bounce(d, v);
+    }
+}
+
 // CACHE: the radiance-cache render -- paths end at their (cache_k + 1)-th real
 // collision, which becomes a query of the cache (the medium's max_bounces must
 // be -1: the cached estimator has no other cap), as nrc::nee_cached.
-template <int MAJ, int TKIND, bool ENVMAP, bool LEAN, bool CACHE = false>
+// CMP (lean constant-environment forms): the light samples of the collision
+// handler and the phase samples of the end-of-transmittance handler are
+// gathered block-wide through LDS and computed by the block's first waves
+// (one lane per request), then handed back; the 4 waves of a block run the
+// event loop in step (three barriers per iteration, one more for its end).
+template <int MAJ, int TKIND, bool ENVMAP, bool LEAN, bool CACHE = false, bool CMP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(nee_waves_per_eu<MAJ, TKIND, ENVMAP, LEAN>())))
 k_nee_persistent(NeeArgs args) {
+    static_assert(!CMP || (LEAN && !ENVMAP), "compacted samples: lean constant-environment forms only");
     if constexpr (MAJ == kMajGridLds) stage_majorant(args.M);
     const NeeSchedule& S = args.S;
     unsigned long long* __restrict__ counters = args.counters;
@@ -498,9 +618,14 @@ k_nee_persistent(NeeArgs args) {
             const uint32_t taken = (uint32_t)__popcll(mask);
             pool_next += taken < avail ? taken : avail;
         }
-        if (__ballot(have) == 0ull) {
-            if (__ballot(!done) == 0ull) break;  // every lane of the wave has drained the queues
-            continue;
+        if constexpr (CMP) {
+            // the block iterates while any of its lanes has or awaits work
+            if (!__syncthreads_or((int)(have || !done))) break;
+        } else {
+            if (__ballot(have) == 0ull) {
+                if (__ballot(!done) == 0ull) break;  // every lane of the wave has drained the queues
+                continue;
+            }
         }
         // Starting a sample (seeding, camera ray, box entry) is the costliest
         // handler; lanes waiting for one idle while the others track until
@@ -544,8 +669,12 @@ k_nee_persistent(NeeArgs args) {
                 }
             }
         }
-        if (have && lane.event == kEvCollision) lane.on_collision(nee_args().M);
-        if (have && lane.event == kEvTrackEnd) lane.on_track_end(nee_args().M);
+        if constexpr (CMP) {
+            nee_compacted_samples(lane, have);
+        } else {
+            if (have && lane.event == kEvCollision) lane.on_collision(nee_args().M);
+            if (have && lane.event == kEvTrackEnd) lane.on_track_end(nee_args().M);
+        }
     }
 }
 

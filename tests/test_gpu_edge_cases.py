@@ -199,6 +199,45 @@ def test_lds_majorant_equals_global(gpu, tkind, cells):
     np.testing.assert_array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("spp", [1, 8])
+def test_compacted_samples_equal_the_per_wave_handlers(gpu, spp):
+    """The persistent NEE kernel with its light and phase samples computed
+    block-wide (VR_NEE_COMPACT=1, lean constant-environment form) draws the
+    same numbers per sample as the per-wave handlers: the same frame to float
+    rounding (the moved code contracts a few products into FMAs differently:
+    ~1 % of the pixels differ by an ulp at one sample per pixel; at 8 the
+    per-pixel sums also add in the order of the atomics)."""
+    import os
+    print(f"[case] compacted samples spp={spp}", flush=True)
+    sc = Scene(sigma=fbm_grid(24, seed=4, scale=25.0), albedo_c=(0.8, 0.8, 0.8), phase_g=0.5, env_c=(0.2, 0.3, 0.4),
+               sun_dir=(0.3, 0.9, 0.3), sun_radiance=(2.0, 2.0, 2.0), sun_prob=0.5, majorant="grid",
+               majorant_res=(16, 16, 16), max_bounces=8, transmittance_kind=L.VR_T_DT)
+    b = Built(sc, "cuda")
+    table = torch.empty(23 ** 3 * 8, device="cuda")
+    L.check(L.load().vr_grid_pack_cells(b.medium.sigma, table.data_ptr(), torch.cuda.current_stream().cuda_stream),
+            "pack")
+    b.medium.sigma_cells = table.data_ptr()
+    cam = Camera(np.stack([pose((0.2, 0.4, 2.2)), pose((-1.9, 0.1, 1.0))]), 32, 32, jittered=True, device="cuda")
+    outs = []
+    old = os.environ.get("VR_NEE_COMPACT")
+    try:
+        for flag in ("0", "1"):
+            os.environ["VR_NEE_COMPACT"] = flag
+            out = torch.zeros(cam.threads, 3, device="cuda")
+            lau = L.launch((5, 6, 7, 8), samples=spp, bw_mode=L.VR_BW_CONSISTENT)
+            L.check(L.load().vr_capture_fw(b.medium, L.VR_MAP_NEE_DT, cam.c, lau, out.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream), "capture")
+            torch.cuda.synchronize()
+            outs.append(out.cpu().numpy())
+    finally:
+        if old is None:
+            os.environ.pop("VR_NEE_COMPACT", None)
+        else:
+            os.environ["VR_NEE_COMPACT"] = old
+    assert outs[0].sum() > 0
+    np.testing.assert_allclose(outs[1], outs[0], rtol=1e-5, atol=1e-6)
+
+
 @pytest.mark.parametrize("bw_mode,majorant,seed_mode,env", [
     (L.VR_BW_CONSISTENT, "grid", L.VR_SEED_CAPTURE_FW, "sun"), (L.VR_BW_REFERENCE, "grid", L.VR_SEED_CAPTURE_FW, "sun"),
     (L.VR_BW_CONSISTENT, "const", L.VR_SEED_CAPTURE_FW, "sun"), (L.VR_BW_REFERENCE, "const", L.VR_SEED_CAPTURE_FW, "sun"),
