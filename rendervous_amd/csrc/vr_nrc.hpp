@@ -44,15 +44,6 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 VR_DEV float fract(float t) { return t - floorf(t); }
 
-// Orders a wave's LDS writes before its later LDS reads of other lanes' data
-// (a wave's LDS operations execute in order; this keeps the compiler from
-// moving them across).
-VR_DEV void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Features of a point u in the unit cube (the medium box mapped to [0,1]^3) and
 // a direction w: sin and cos of pi * 2^f * u_c for f < 12, then w, then 1.
 VR_DEV float feature(f3 u, f3 w, int i) {
