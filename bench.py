@@ -756,6 +756,11 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32,
                                                                     outq.data_ptr(), L.stream_handle(dev)), "mlp_fw"))
     flops_q = 2.0 * (76 * 64 + 3 * 64 * 64 + 64 * 3)  # algorithmic FLOPs per query (no padding)
     tflops = flops_q * queries / (infer_ms * 1e-3) / 1e12
+    # vr_nrc_mlp_fw reads each query's 96 FP32 features and writes its RGB: the
+    # feature stream, not the MFMAs, bounds it (the render's fused inference
+    # forms the features in registers instead)
+    bytes_q = 96 * 4 + 3 * 4
+    gbs = bytes_q * queries / (infer_ms * 1e-3) / 1e9
     nrc_ms = train_ms + hit["ms"] if hit else None
     return {
         "workload": f"time to RMSE <= plain {target_spp} spp RMSE, {res}^3 cloud fBm x60, albedo 0.99, g 0.8, "
@@ -766,8 +771,10 @@ def nrc_leg(dev, res=512, hw=1024, ref_spp=4096, target_spp=256, max_bounces=32,
         "nrc_train_ms": train_ms, "nrc_render": hit, "nrc_ms": nrc_ms,
         "speedup": plain_ms / nrc_ms if nrc_ms else None, "ladder": ladder,
         "mlp_inference": {"queries": queries, "ms": infer_ms,
-                          "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
-                                       "frac": tflops / 2500.0}},
+                          "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": gbs / HBM_PEAK_GBS, "bytes_per_query": bytes_q},
+                          "mfma": {"achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0,
+                                   "flops_per_query": flops_q}},
     }
 
 
