@@ -359,11 +359,22 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __r
     __syncthreads();
     const int64_t count = min((int64_t)*count_p, max_queries);
     const int l = __lane_id(), r = l % 16;
-    const int64_t nw = (int64_t)gridDim.x * kInferWaves;
-    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * 16; base < count; base += nw * 16) {
-        const int64_t row = base + r;
-        const bool valid = row < count;
-        const float* q = queries + (valid ? row : 0) * 10;
+    const int64_t stride = (int64_t)gridDim.x * kInferWaves * 16;
+    // each lane's query row of the wave's tile, read one tile ahead so that the
+    // load overlaps the current tile's network
+    auto fetch = [&](int64_t b, float* d) {
+        const int64_t row = b + r;
+        const float* q = queries + (row < count ? row : 0) * 10;
+#pragma unroll
+        for (int i = 0; i < 10; i++) d[i] = q[i];
+    };
+    int64_t base = ((int64_t)blockIdx.x * kInferWaves + threadIdx.x / 64) * 16;
+    float q[10];
+    if (base < count) fetch(base, q);
+    for (; base < count; base += stride) {
+        const bool valid = base + r < count;
+        float qn[10];
+        if (base + stride < count) fetch(base + stride, qn);
         bf16x8 x[3];
         encode_frags(unit_coords(F3(q[0], q[1], q[2]), bmin, ibsize), F3(q[3], q[4], q[5]), valid, x);
         const floatx4 y = mlp_tile(w, x);
@@ -373,6 +384,8 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_nrc_apply(const float* __r
             atomic_addf(out + 3 * (size_t)pix + 1, q[7] * fmaxf(y[1], 0.0f));
             atomic_addf(out + 3 * (size_t)pix + 2, q[8] * fmaxf(y[2], 0.0f));
         }
+#pragma unroll
+        for (int i = 0; i < 10; i++) q[i] = qn[i];
     }
 }
 
