@@ -622,14 +622,15 @@ struct WalkGrid {
         return max(0, min(res - 1, c));
     }
     static VR_DEV float axis_face(float xa, float wa, float bmin, float cs, int c) {
-        if (wa > 0.0f) return (bmin + (float)(c + 1) * cs - xa) * rcp(wa);
-        if (wa < 0.0f) return (bmin + (float)c * cs - xa) * rcp(wa);
-        return kInf;
+        // one reciprocal for both signs (a divergent branch each would issue both)
+        const float face = bmin + (float)(wa > 0.0f ? c + 1 : c) * cs - xa;
+        return (wa > 0.0f || wa < 0.0f) ? face * rcp(wa) : kInf;
     }
-    // bound of the packed cell c (>= 0)
+    // bound of the packed cell c (>= 0); the grid's axes hold at most 1024
+    // cells (the packing), so the index takes 24-bit multiplies
     static VR_DEV float fetch(const MediumView& M, int c) {
-        const int cx = c & 1023, cy = (c >> 10) & 1023, cz = c >> 20;
-        const int i = (cz * M.mry + cy) * M.mrx + cx;
+        const uint32_t cx = c & 1023, cy = (c >> 10) & 1023, cz = (uint32_t)c >> 20;
+        const uint32_t i = __umul24(__umul24(cz, (uint32_t)M.mry) + cy, (uint32_t)M.mrx) + cx;
         return LDS ? maj_lds[i] : M.maj_grid[i];
     }
     VR_DEV void begin(const MediumView& M, f3 x, f3 w) {
