@@ -1083,8 +1083,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args.res, args.width, args.height, args.spp),
                      "measured_bound": "VALU issue: the persistent kernel's SIMDs issue VALU work ~100 % of "
-                                       "the frame at 37 % active lanes (2.7e9 wave instructions, 163 per camera "
-                                       "sample); HBM traffic 11.8 GB per frame is 33 % of its time at 8 TB/s "
+                                       "the frame at 38 % active lanes (2.7e9 wave instructions, 161 per camera "
+                                       "sample); HBM traffic 11.8 GB per frame is 34 % of its time at 8 TB/s "
                                        "(profiles/nee_persistent_pmc_256cube_512x512x64spp.txt)"},
     }
     if not args.no_secondary:
