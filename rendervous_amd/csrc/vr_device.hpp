@@ -271,9 +271,12 @@ VR_DEV float sigma_lookup(const MediumView& M, f3 x) {
     return M.cells ? cells_lookup1(M.sigma, M.cells, x) : grid_lookup1(M.sigma, x);
 }
 
-// Multi-channel lookup (C <= 4), grid3d.h:6-37.
+// Multi-channel lookup (C <= 4, out[4]), grid3d.h:6-37.  The channel loops run
+// over 4 constant indices guarded by C, so a caller's out[] stays in registers
+// (a loop bounded by C indexes it dynamically, which puts it in scratch).
 VR_DEV void grid_lookupC(const GridView& g, f3 c, float* out) {
-    for (int k = 0; k < g.C; k++) out[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = 0.0f;
     float gx = fmaf(c.x, g.scale.x, g.offset.x);
     float gy = fmaf(c.y, g.scale.y, g.offset.y);
     float gz = fmaf(c.z, g.scale.z, g.offset.z);
@@ -288,7 +291,9 @@ VR_DEV void grid_lookupC(const GridView& g, f3 c, float* out) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const float* q = p + ((k & 1) ? g.sx : 0) + ((k & 2) ? g.sy : 0) + ((k & 4) ? g.sz : 0);
-        for (int ch = 0; ch < g.C; ch++) out[ch] += q[ch * g.sc] * wgt[k];
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++)
+            if (ch < g.C) out[ch] += q[ch * g.sc] * wgt[k];
     }
 }
 
@@ -296,7 +301,8 @@ VR_DEV void atomic_addf(float* p, float v) {
     if (v != 0.0f) unsafeAtomicAdd(p, v);
 }
 
-// include/maps/grid3d.h:39-75 (out_grad per channel, og[C]).
+// include/maps/grid3d.h:39-75 (out_grad per channel, og[C]; channel loops as in
+// grid_lookupC).
 VR_DEV void grid_scatterC(const GridView& g, f3 c, const float* og) {
     if (g.grad == nullptr) return;
     float gx = fmaf(c.x, g.scale.x, g.offset.x);
@@ -313,11 +319,30 @@ VR_DEV void grid_scatterC(const GridView& g, f3 c, const float* og) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         float* q = p + ((k & 1) ? g.sx : 0) + ((k & 2) ? g.sy : 0) + ((k & 4) ? g.sz : 0);
-        for (int ch = 0; ch < g.C; ch++) atomic_addf(q + ch * g.sc, og[ch] * wgt[k]);
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++)
+            if (ch < g.C) atomic_addf(q + ch * g.sc, og[ch] * wgt[k]);
     }
 }
 
-VR_DEV void grid_scatter1(const GridView& g, f3 c, float og) { grid_scatterC(g, c, &og); }
+// Single channel (the density): the same corners, weights and adds.
+VR_DEV void grid_scatter1(const GridView& g, f3 c, float og) {
+    if (g.grad == nullptr) return;
+    float gx = fmaf(c.x, g.scale.x, g.offset.x);
+    float gy = fmaf(c.y, g.scale.y, g.offset.y);
+    float gz = fmaf(c.z, g.scale.z, g.offset.z);
+    if (!(gx >= 0.0f && gy >= 0.0f && gz >= 0.0f && gx < g.dimf.x && gy < g.dimf.y && gz < g.dimf.z)) return;
+    float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+    float ax = gx - fx, ay = gy - fy, az = gz - fz;
+    int px = min((int)fx, g.nx - 2), py = min((int)fy, g.ny - 2), pz = min((int)fz, g.nz - 2);
+    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    float wgt[8] = {bx * by * bz, ax * by * bz, bx * ay * bz, ax * ay * bz,
+                    bx * by * az, ax * by * az, bx * ay * az, ax * ay * az};
+    float* p = g.grad + (pz * g.sz + py * g.sy + px * g.sx);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        atomic_addf(p + ((k & 1) ? g.sx : 0) + ((k & 2) ? g.sy : 0) + ((k & 4) ? g.sz : 0), og * wgt[k]);
+}
 
 // Wave-cooperative form of grid_scatterC for a grid of C = 1 or 3 channels.
 // scatter_post (called by every lane of the wave; lanes with `active` false
