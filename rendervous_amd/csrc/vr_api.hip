@@ -24,6 +24,7 @@
 #include "vr_functions.hpp"
 #include "vr_scene.hpp"
 #include "vr_nee_bw_flat.hpp"
+#include "vr_ss_bw_flat.hpp"
 
 using namespace vr;
 
@@ -717,6 +718,101 @@ k_capture_bw_nee_pool(NeeBwArgs args) {
     L.flush_sigma(srec, nrec);
 }
 
+// The collision-integrator backward with single-scatter exitance and RT
+// shadow rays (k_capture_bw of VR_MAP_MC_COLLISION, consistent form, per-sample
+// streams) as the lane state machine of vr_ss_bw_flat.hpp, the lanes of a
+// wave pulling one-sample work items from a launch-wide counter as
+// k_capture_bw_nee_pool does.  Each sample's draws and gradients are those of
+// k_capture_bw.
+#ifndef VR_SS_BW_POOL_WAVES_PER_EU
+#define VR_SS_BW_POOL_WAVES_PER_EU 3
+#endif
+template <int MAJ>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VR_SS_BW_POOL_WAVES_PER_EU)))
+k_capture_bw_ss_pool(NeeBwArgs args) {
+    constexpr int kSRec = SsBwLane<MAJ>::kScatterRecords * 8;
+    __shared__ float scat_records[kBlock / 64 * kSRec];
+    float* srec = scat_records + (threadIdx.x / 64) * kSRec;
+    int nrec = 0;  // wave-uniform count of buffered sigma records
+    SsBwLane<MAJ> L(first_kernel_arg<MediumView>());
+    L.eps = args.M.ds_eps;
+    L.state = kSsDone;  // a lane without work sits in kSsDone with s == s1
+    bool done = false, drained = false;
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform pool of item indices
+    uint32_t sensor = 0;
+    int s = 0, s1 = 0, pose = 0, row = 0, col = 0, count0 = 0;
+    // every sample ends (its walks end); the cap only bounds a wave's loop
+    for (uint32_t it = 0; it < (1u << 28); it++) {
+        L.tr.reread();
+        bool fresh = false, need = false;
+        if (L.state == kSsDone) {
+            if (s < s1) {
+                if (++s < s1) {
+                    fresh = true;
+                } else {
+                    const NeeBwArgs& A = nee_bw_args();
+                    if (A.dbg) atomicAdd(A.dbg + sensor, (A.debug_lookups ? L.tr.lookups : L.tr.scatters) - count0);
+                }
+            }
+            need = !fresh && !done;
+        }
+        const unsigned long long mask = __ballot(need);
+        if (mask) {
+            const NeeBwArgs& A = nee_bw_args();
+            const uint32_t items = A.S.cnt * (uint32_t)A.S.chunks;
+            while (pool_next >= pool_end && !drained) {
+                unsigned long long base = 0;
+                if (__lane_id() == 0) base = atomicAdd(A.counter, (unsigned long long)kPool);
+                base = __shfl(base, 0);
+                if (base < items) {
+                    pool_next = (uint32_t)base;
+                    pool_end = items - pool_next > (uint32_t)kPool ? pool_next + (uint32_t)kPool : items;
+                } else {
+                    drained = true;
+                }
+            }
+            const uint32_t avail = pool_end - pool_next;
+            const uint32_t rank = (uint32_t)__popcll(mask & lane_mask_lt());
+            if (need) {
+                if (rank < avail) {
+                    int s0;
+                    s = s1 = 0;
+                    if (nee_item(A.S, A.cam, 0, pool_next + rank, sensor, s0, s1, pose, row, col)) {
+                        const float inv = (float)A.S.spp;
+                        const float* o = A.og + 3 * (size_t)sensor;
+                        const f3 g = F3(o[0] / inv, o[1] / inv, o[2] / inv);
+                        if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
+                            L.dL_dR = g;
+                            s = s0;
+                            count0 = A.debug_lookups ? L.tr.lookups : L.tr.scatters;
+                            fresh = true;
+                        }
+                    }
+                    if (!fresh) s = s1;
+                } else if (drained) {
+                    done = true;
+                }
+            }
+            const uint32_t taken = (uint32_t)__popcll(mask);
+            pool_next += taken < avail ? taken : avail;
+        }
+        if (fresh) {
+            const NeeBwArgs& A = nee_bw_args();
+            L.tr.rng = seed_capture_fw(A.seeds, (uint32_t)s * A.S.total + sensor);
+            camera_ray(A.cam, pose, row, col, L.tr.rng, L.x0, L.w0);
+            L.begin_sample();
+        }
+        if (__ballot(L.state != kSsDone || !done) == 0ull) break;  // the wave drained the counter
+        L.events();
+        for (int k = 0; k < 3; k++) {
+            if (L.tracking()) L.step();
+            nrec = L.post_step(srec, nrec);
+            if (__ballot(L.tracking()) == 0ull) break;
+        }
+    }
+    L.flush_sigma(srec, nrec);
+}
+
 // Zeroes the pooled backward's work counter and the debug rows [start, start+cnt).
 __global__ void k_nee_bw_prepare(unsigned long long* __restrict__ counter, int64_t start, int64_t cnt, int* __restrict__ dbg) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1030,13 +1126,29 @@ bool capture_bw_is_pooled(int kind, const MediumView& M, int seed_mode) {
            env_int("VR_NEE_BW_FLAT", 1) != 0 && env_int("VR_NEE_BW_POOL", 1) != 0;
 }
 
-template <int MAJ, bool ENVMAP>
-int launch_nee_bw_pool(const MediumView& M, const CameraView& C, Seeds sd, int spp, int debug_kind, int64_t total,
-                       int64_t start, int64_t cnt, const float* og, int* dbg, unsigned long long* counter, hipStream_t st) {
+// With VR_SS_BW_POOL=1 the collision-integrator backward (single-scatter
+// exitance) with RT shadow rays, the consistent estimators, per-sample streams
+// and no albedo gradient runs as the pooled lane state machine
+// k_capture_bw_ss_pool.  Off by default: its walks are a few steps long at the
+// densities of the 256^3 inverse render, where the per-event work of the state
+// machine costs more than the nested kernel loses to divergence (256^3, 8 views
+// of 256x256 at 16 spp, nested -> pooled: 1.43 -> 1.95-2.03 ms at a constant
+// density of 5, 1.95 -> 2.31 ms for an fBm density scaled to 40 (2.0 ms with 2
+// samples per work item), 3.74 -> 2.97 ms scaled to 120).
+bool capture_bw_is_ss_pooled(int kind, const MediumView& M, int seed_mode) {
+    return kind == VR_MAP_MC_COLLISION && M.t_kind == VR_T_RT && M.bw_mode == VR_BW_CONSISTENT &&
+           seed_mode != VR_SEED_CAPTURE_BW && !(M.albedo_grid && M.albedo.grad != nullptr) &&
+           env_int("VR_SS_BW_POOL", 0) != 0;
+}
+
+// Launches a pooled backward kernel (k_capture_bw_nee_pool or k_capture_bw_ss_pool).
+int launch_bw_pool(void (*kern)(NeeBwArgs), const MediumView& M, const CameraView& C, Seeds sd, int spp, int debug_kind,
+                   int64_t total, int64_t start, int64_t cnt, const float* og, int* dbg, unsigned long long* counter,
+                   hipStream_t st) {
     if (total > (int64_t)UINT32_MAX) return set_err(VR_ERR_UNSUPPORTED, "capture has more than 2^32 sensors");
     int rc = VR_OK;
     int blocks = 0;
-    if ((rc = persistent_blocks(k_capture_bw_nee_pool<MAJ, ENVMAP>, &blocks))) return rc;
+    if ((rc = persistent_blocks(kern, &blocks))) return rc;
     const int64_t hw = (int64_t)C.width * C.height;
     CaptureSplit P = capture_split(spp, false);
     capture_tiling(P, C, start, cnt);
@@ -1073,7 +1185,7 @@ int launch_nee_bw_pool(const MediumView& M, const CameraView& C, Seeds sd, int s
         A.debug_lookups = debug_kind == 1;
         const unsigned prep = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 4096));
         hipLaunchKernelGGL(k_nee_bw_prepare, dim3(prep), dim3(kBlock), 0, st, counter, b0, n, dbg);
-        hipLaunchKernelGGL((k_capture_bw_nee_pool<MAJ, ENVMAP>), dim3(blocks), dim3(kBlock), 0, st, A);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, st, A);
         if ((rc = check_hip(hipGetLastError(), "vr_capture_bw launch"))) return rc;
     }
     return VR_OK;
@@ -1085,9 +1197,13 @@ int launch_capture_bw_maj(int kind, const MediumView& M, const CameraView& C, Se
                           int64_t start, int64_t cnt, const float* og, int* dbg, unsigned long long* counter, hipStream_t st) {
     if (counter && capture_bw_is_pooled(kind, M, seed_mode)) {
         if (M.env_kind != VR_ENV_CONST || M.env_sampler != VR_ENVS_UNIFORM)
-            return launch_nee_bw_pool<MAJ, true>(M, C, sd, spp, debug_kind, total, start, cnt, og, dbg, counter, st);
-        return launch_nee_bw_pool<MAJ, false>(M, C, sd, spp, debug_kind, total, start, cnt, og, dbg, counter, st);
+            return launch_bw_pool(k_capture_bw_nee_pool<MAJ, true>, M, C, sd, spp, debug_kind, total, start, cnt, og, dbg,
+                                  counter, st);
+        return launch_bw_pool(k_capture_bw_nee_pool<MAJ, false>, M, C, sd, spp, debug_kind, total, start, cnt, og, dbg,
+                              counter, st);
     }
+    if (counter && capture_bw_is_ss_pooled(kind, M, seed_mode))
+        return launch_bw_pool(k_capture_bw_ss_pool<MAJ>, M, C, sd, spp, debug_kind, total, start, cnt, og, dbg, counter, st);
     CaptureSplit P = capture_split(spp, seed_mode == VR_SEED_CAPTURE_BW);
     capture_tiling(P, C, start, cnt);
     P.debug_lookups = debug_kind == 1;
@@ -1577,7 +1693,9 @@ int vr_capture_bw(const vr_medium* medium, int32_t kind, const vr_camera* camera
     Seeds sd = make_seeds(L);
     hipStream_t s = (hipStream_t)stream;
     Counters ctr;
-    if (capture_bw_is_pooled(kind, M, L->seed_mode) && (st = ctr.acquire(L, s))) return st;
+    if ((capture_bw_is_pooled(kind, M, L->seed_mode) || capture_bw_is_ss_pooled(kind, M, L->seed_mode)) &&
+        (st = ctr.acquire(L, s)))
+        return st;
     if (medium->majorant_kind == VR_MAJ_GRID)
         st = launch_capture_bw_maj<VR_MAJ_GRID>(kind, M, C, sd, spp, L->seed_mode, L->debug_kind, total, start, cnt, og,
                                                 L->debug_out, ctr.ptr, s);

@@ -295,3 +295,64 @@ def test_flat_nee_backward_equals_the_nested_one(gpu, bw_mode, majorant, seed_mo
     for (gs1, ga1), d1 in res[1:]:
         np.testing.assert_array_equal(d0, d1)
         assert rel_l1(gs1, gs0) <= 1e-5 and rel_l1(ga1, ga0) <= 1e-5, (rel_l1(gs1, gs0), rel_l1(ga1, ga0))
+
+
+@pytest.mark.parametrize("majorant,seed_mode,env,albedo_grid", [
+    ("grid", L.VR_SEED_CAPTURE_FW, "sun", False), ("const", L.VR_SEED_CAPTURE_FW, "sun", True),
+    ("grid", L.VR_SEED_CAPTURE_FW, "image", True), ("grid", L.VR_SEED_CAPTURE_BW, "sun", False)])
+def test_pooled_single_scatter_backward_equals_the_nested_one(gpu, majorant, seed_mode, env, albedo_grid):
+    """The collision-integrator backward with single-scatter exitance and RT
+    shadow rays as a pooled lane state machine (VR_SS_BW_POOL=1; it applies to
+    the consistent estimators, per-sample streams and no albedo gradient)
+    scatters the same density gradients, sample by sample, as the nested
+    estimator (VR_SS_BW_POOL=0): same scatter and lookup counts per sensor, gradients
+    equal up to the order of the float atomics.  Sensors with a zero output
+    gradient trace nothing.  With the reference's backward seeds (one stream
+    over a sensor's samples) the pooled form does not apply and both runs are
+    the nested kernel."""
+    import os
+    import torch
+    from helpers import env_density, np_quadtree, rel_l1, sky_image
+    print(f"[case] pooled ss bw majorant={majorant} seeds={seed_mode} env={env} albedo_grid={albedo_grid}", flush=True)
+    g = fbm_grid(20, seed=7, scale=30.0)
+    light = dict(env_c=(0.3, 0.4, 0.5), sun_dir=(0.3, 0.9, 0.3), sun_radiance=(2.0, 2.0, 2.0), sun_prob=0.4)
+    if env == "image":
+        img = sky_image(16, seed=2)
+        light = dict(env_image=img, env_quadtree=np_quadtree(env_density(img)), env_quadtree_levels=4)
+    alb = None
+    if albedo_grid:
+        alb = np.repeat((0.5 + 0.45 * fbm_grid(8, seed=6, scale=1.0)).astype(np.float32), 3, -1)
+    sc = Scene(sigma=g, albedo=alb, phase_g=0.4, majorant=majorant, majorant_res=(4, 4, 4),
+               transmittance_kind=L.VR_T_RT, **light)
+    cam = dict(poses=np.stack([pose((0.2, 0.4, 2.2)), pose((-1.9, 0.1, 1.0))]), width=20, height=20, jittered=True)
+    og = (np.random.default_rng(4).uniform(size=(800, 3)) + 0.5).astype(np.float32)
+    og[::7] = 0.0
+    res = []
+    old = os.environ.get("VR_SS_BW_POOL")
+    try:
+        for pool in ("0", "1"):
+            os.environ["VR_SS_BW_POOL"] = pool
+            counts = []
+            for kind in (0, 1):
+                b = Built(sc, "cuda", grad_sigma=True)
+                c = Camera(device="cuda", **cam)
+                dbg = torch.zeros(c.threads, dtype=torch.int32, device="cuda")
+                o = torch.from_numpy(og).cuda()
+                lau = L.launch((9, 8, 7, 6), samples=8, bw_mode=L.VR_BW_CONSISTENT, seed_mode=seed_mode, debug_out=dbg,
+                               debug_kind=kind)
+                L.check(L.load().vr_capture_bw(b.medium, L.VR_MAP_MC_COLLISION, c.c, lau, o.data_ptr(),
+                                               torch.cuda.current_stream().cuda_stream), "capture_bw")
+                torch.cuda.synchronize()
+                counts.append(dbg.cpu().numpy())
+            res.append((b.grads()[0], counts))
+    finally:
+        if old is None:
+            os.environ.pop("VR_SS_BW_POOL", None)
+        else:
+            os.environ["VR_SS_BW_POOL"] = old
+    (gs0, (s0, l0)), (gs1, (s1, l1)) = res
+    assert np.abs(gs0).sum() > 0
+    assert (s0[::7] == 0).all() and (s0 > 0).mean() > 0.2 and (l0 > 0).mean() > 0.2
+    np.testing.assert_array_equal(s1, s0)
+    np.testing.assert_array_equal(l1, l0)
+    assert rel_l1(gs1, gs0) <= 1e-5, rel_l1(gs1, gs0)
