@@ -638,8 +638,8 @@ k_capture_bw_nee_pool(NeeBwArgs args) {
     // A lane without work sits in kBwDone with s == s1 (a per-lane "has work"
     // flag guarding the handlers cost ~150 spilled VGPRs).
     L.state = kBwDone;
-    bool done = false, drained = false;
-    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform pool of item indices
+    bool done = false;
+    ItemPool pool;
     uint32_t sensor = 0;
     int s = 0, s1 = 0, pose = 0, row = 0, col = 0, count0 = 0;
     // every sample ends (its paths end); the cap only bounds a wave's loop
@@ -663,42 +663,23 @@ k_capture_bw_nee_pool(NeeBwArgs args) {
         const unsigned long long mask = __ballot(need);
         if (mask) {
             const NeeBwArgs& A = nee_bw_args();
-            const uint32_t items = A.S.cnt * (uint32_t)A.S.chunks;
-            while (pool_next >= pool_end && !drained) {
-                unsigned long long base = 0;
-                if (__lane_id() == 0) base = atomicAdd(A.counter, (unsigned long long)kPool);
-                base = __shfl(base, 0);
-                if (base < items) {
-                    pool_next = (uint32_t)base;
-                    pool_end = items - pool_next > (uint32_t)kPool ? pool_next + (uint32_t)kPool : items;
-                } else {
-                    drained = true;
-                }
-            }
-            const uint32_t avail = pool_end - pool_next;
-            const uint32_t rank = (uint32_t)__popcll(mask & lane_mask_lt());
-            if (need) {
-                if (rank < avail) {
-                    int s0;
-                    s = s1 = 0;  // no samples unless the item has a gradient
-                    if (nee_item(A.S, A.cam, 0, pool_next + rank, sensor, s0, s1, pose, row, col)) {
-                        const float inv = (float)A.S.spp;
-                        const float* o = A.og + 3 * (size_t)sensor;
-                        const f3 g = F3(o[0] / inv, o[1] / inv, o[2] / inv);
-                        if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
-                            L.dL_dR = g;
-                            s = s0;
-                            count0 = A.debug_lookups ? L.tr.lookups : L.tr.scatters;
-                            fresh = true;
-                        }
+            uint32_t item = 0;
+            if (pool.take(mask, need, A.counter, A.S.cnt * (uint32_t)A.S.chunks, item, done)) {
+                int s0;
+                s = s1 = 0;  // no samples unless the item has a gradient
+                if (nee_item(A.S, A.cam, 0, item, sensor, s0, s1, pose, row, col)) {
+                    const float inv = (float)A.S.spp;
+                    const float* o = A.og + 3 * (size_t)sensor;
+                    const f3 g = F3(o[0] / inv, o[1] / inv, o[2] / inv);
+                    if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
+                        L.dL_dR = g;
+                        s = s0;
+                        count0 = A.debug_lookups ? L.tr.lookups : L.tr.scatters;
+                        fresh = true;
                     }
-                    if (!fresh) s = s1;
-                } else if (drained) {
-                    done = true;
                 }
+                if (!fresh) s = s1;
             }
-            const uint32_t taken = (uint32_t)__popcll(mask);
-            pool_next += taken < avail ? taken : avail;
         }
         if (fresh) {
             const NeeBwArgs& A = nee_bw_args();
@@ -737,8 +718,8 @@ k_capture_bw_ss_pool(NeeBwArgs args) {
     SsBwLane<MAJ> L(first_kernel_arg<MediumView>());
     L.eps = args.M.ds_eps;
     L.state = kSsDone;  // a lane without work sits in kSsDone with s == s1
-    bool done = false, drained = false;
-    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform pool of item indices
+    bool done = false;
+    ItemPool pool;
     uint32_t sensor = 0;
     int s = 0, s1 = 0, pose = 0, row = 0, col = 0, count0 = 0;
     // every sample ends (its walks end); the cap only bounds a wave's loop
@@ -759,42 +740,23 @@ k_capture_bw_ss_pool(NeeBwArgs args) {
         const unsigned long long mask = __ballot(need);
         if (mask) {
             const NeeBwArgs& A = nee_bw_args();
-            const uint32_t items = A.S.cnt * (uint32_t)A.S.chunks;
-            while (pool_next >= pool_end && !drained) {
-                unsigned long long base = 0;
-                if (__lane_id() == 0) base = atomicAdd(A.counter, (unsigned long long)kPool);
-                base = __shfl(base, 0);
-                if (base < items) {
-                    pool_next = (uint32_t)base;
-                    pool_end = items - pool_next > (uint32_t)kPool ? pool_next + (uint32_t)kPool : items;
-                } else {
-                    drained = true;
-                }
-            }
-            const uint32_t avail = pool_end - pool_next;
-            const uint32_t rank = (uint32_t)__popcll(mask & lane_mask_lt());
-            if (need) {
-                if (rank < avail) {
-                    int s0;
-                    s = s1 = 0;
-                    if (nee_item(A.S, A.cam, 0, pool_next + rank, sensor, s0, s1, pose, row, col)) {
-                        const float inv = (float)A.S.spp;
-                        const float* o = A.og + 3 * (size_t)sensor;
-                        const f3 g = F3(o[0] / inv, o[1] / inv, o[2] / inv);
-                        if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
-                            L.dL_dR = g;
-                            s = s0;
-                            count0 = A.debug_lookups ? L.tr.lookups : L.tr.scatters;
-                            fresh = true;
-                        }
+            uint32_t item = 0;
+            if (pool.take(mask, need, A.counter, A.S.cnt * (uint32_t)A.S.chunks, item, done)) {
+                int s0;
+                s = s1 = 0;
+                if (nee_item(A.S, A.cam, 0, item, sensor, s0, s1, pose, row, col)) {
+                    const float inv = (float)A.S.spp;
+                    const float* o = A.og + 3 * (size_t)sensor;
+                    const f3 g = F3(o[0] / inv, o[1] / inv, o[2] / inv);
+                    if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) {
+                        L.dL_dR = g;
+                        s = s0;
+                        count0 = A.debug_lookups ? L.tr.lookups : L.tr.scatters;
+                        fresh = true;
                     }
-                    if (!fresh) s = s1;
-                } else if (drained) {
-                    done = true;
                 }
+                if (!fresh) s = s1;
             }
-            const uint32_t taken = (uint32_t)__popcll(mask);
-            pool_next += taken < avail ? taken : avail;
         }
         if (fresh) {
             const NeeBwArgs& A = nee_bw_args();

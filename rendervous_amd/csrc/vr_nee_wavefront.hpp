@@ -419,6 +419,46 @@ VR_DEV unsigned long long lane_mask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// A wave's share of one launch-wide work counter (the pooled backward
+// kernels): the lanes that need an item take consecutive indices from the
+// wave's pool, which one atomic refills with kPool indices when it is empty.
+// Every lane of the wave calls take() with the wave's ballot of `need`; a lane
+// that needs an item gets `item` (true) or, once the counter is drained and the
+// pool empty, is marked `done`.
+struct ItemPool {
+    uint32_t next = 0, end = 0;  // wave-uniform
+    bool drained = false;
+
+    VR_DEV bool take(unsigned long long mask, bool need, unsigned long long* counter, uint32_t items, uint32_t& item,
+                     bool& done) {
+        while (next >= end && !drained) {
+            unsigned long long base = 0;
+            if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)kPool);
+            base = __shfl(base, 0);
+            if (base < items) {
+                next = (uint32_t)base;
+                end = items - next > (uint32_t)kPool ? next + (uint32_t)kPool : items;
+            } else {
+                drained = true;
+            }
+        }
+        const uint32_t avail = end - next;
+        const uint32_t rank = (uint32_t)__popcll(mask & lane_mask_lt());
+        bool got = false;
+        if (need) {
+            if (rank < avail) {
+                item = next + rank;
+                got = true;
+            } else if (drained) {
+                done = true;
+            }
+        }
+        const uint32_t taken = (uint32_t)__popcll(mask);
+        next += taken < avail ? taken : avail;
+        return got;
+    }
+};
+
 // Persistent kernel: every lane pulls work items until the queues drain.  A
 // wave starts on the queue of its XCD slot (blockIdx.x % nq: workgroups are
 // dispatched to the XCDs round-robin), so each XCD's L2 serves one contiguous
