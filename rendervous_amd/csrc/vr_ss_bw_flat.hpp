@@ -60,7 +60,7 @@ struct SsBwLane {
     float rd, rt;       // DS walks: the interval length and the distance walked; RT walks: the remaining distance
     float T;            // RT estimate of the current shadow ray
     float C, Wpre, s_t; // the primal pass's collision, null-collision weights and collision ratio
-    f3 xt, wo;          // collision point; light direction of the current shadow ray
+    f3 xt;              // collision point
     f3 env, dL_dR;
     f3 sa;
     float ph;
