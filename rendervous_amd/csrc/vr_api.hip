@@ -381,6 +381,18 @@ constexpr int capture_waves() {
     return (KIND == VR_MAP_T_DDA || KIND == VR_MAP_RAYBOX || VR_CAPTURE_WAVES_PER_EU < 1) ? 1 : VR_CAPTURE_WAVES_PER_EU;
 }
 #define VR_CAPTURE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(capture_waves<KIND>())))
+// The capture forward of the collision integrator (single-scatter exitance)
+// runs fastest at 6 waves per SIMD (k_capture_fw<MC_COLLISION, GRID> of the
+// 256^3 inverse render, kernel trace: 708 us at 5 waves, 661 us at 6); the
+// other kinds keep the capture cap.
+#ifndef VR_CAPTURE_MC_FW_WAVES_PER_EU
+#define VR_CAPTURE_MC_FW_WAVES_PER_EU 6
+#endif
+template <int KIND>
+constexpr int capture_fw_waves() {
+    return KIND == VR_MAP_MC_COLLISION && VR_CAPTURE_MC_FW_WAVES_PER_EU >= 1 ? VR_CAPTURE_MC_FW_WAVES_PER_EU
+                                                                            : capture_waves<KIND>();
+}
 
 template <int KIND, int MAJ>
 __global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_map_fw(MediumView M, const float* __restrict__ in, int64_t start, int64_t cnt,
@@ -451,7 +463,7 @@ VR_DEV void capture_sensor(const CameraView& cam, const CaptureSplit& P, int64_t
 
 
 template <int KIND, int MAJ>
-__global__ void __launch_bounds__(kBlock) VR_CAPTURE_OCCUPANCY k_capture_fw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int64_t total,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(capture_fw_waves<KIND>()))) k_capture_fw(MediumView M, CameraView cam, Seeds seeds, CaptureSplit P, int64_t total,
                                                        int64_t start, int64_t cnt, float* __restrict__ out, int* __restrict__ dbg) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt * P.chunks) return;
