@@ -953,10 +953,11 @@ int launch_nee_persistent(const MediumView& M, const CameraView& C, Seeds sd, in
         S.chunks = (spp + S.chunk - 1) / S.chunk;
         S.tiled = !C.sensors && b0 % hw == 0 && n % hw == 0;
         // tracking steps per pass of the event loop: 3 for the lean form (headline
-        // frame 4.42 ms, 4.55 with 4, 4.86 with 6), 6 for the general forms without
-        // the radiance cache (512^3 + 128^3-albedo frame at 4 x 512^2 x 32 spp:
-        // 17.2 ms with 3, 16.4 with 4, 16.1 with 6)
-        S.ksteps = std::max(1, env_int("VR_NEE_STEPS", LEAN || CACHE ? 3 : 6));
+        // frame 4.42 ms, 4.55 with 4, 4.86 with 6; the 512^3 / 1024^2 x 256 spp
+        // render 87.5 ms, 109 ms with 6), 6 for the general forms (512^3 +
+        // 128^3-albedo frame at 4 x 512^2 x 32 spp: 17.2 ms with 3, 16.4 with 4,
+        // 16.1 with 6) and the radiance-cache form (its 1024^2 render 9.8 -> 7.3 ms)
+        S.ksteps = std::max(1, env_int("VR_NEE_STEPS", LEAN && !CACHE ? 3 : 6));
         // more steps while at least VR_NEE_TRACK_MIN lanes still track (default: never)
         S.kmax = std::min(256, std::max(S.ksteps, env_int("VR_NEE_STEPS_MAX", S.ksteps)));
         S.track_min = std::max(0, std::min(65, env_int("VR_NEE_TRACK_MIN", 65)));
